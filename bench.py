@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the EverythingGraph data-parallel path on one MI355X.
+
+Workload (one step): a full PageRank, 20 iterations, FP32 with compensated
+sums, r0 = 1/n, d = 0.85, over an undirected R-MAT scale-24 graph (edge factor
+16, a,b,c = 0.57,0.19,0.19, seed 1; 2^24 vertices, 536,870,912 stored
+entries), pull over the CSR of the symmetric graph, inputs resident in HBM.
+`value` is algorithmic GB/s = iterations * B_PR / step time with
+B_PR = 4m + 4(n+1) + 16n bytes per iteration.
+
+Beside it, on the same resident graph: direction-optimising BFS GTEPS
+(harmonic mean over 64 seeded roots), the batched forms (256 BFS sources as
+multi-source sweeps; 64 personalized PageRank vectors as one SpMM, 30
+iterations) against the plain per-query path, the live roofline of the
+dominant kernel, and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PR/SpMV GB/s + BFS GTEPS @RMAT-24; batched queries/s & p99 vs plain path"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--iterations", type=int, default=20)
+    ap.add_argument("--bfs-roots", type=int, default=64)
+    ap.add_argument("--batch-bfs", type=int, default=256)
+    ap.add_argument("--batch-ppr", type=int, default=64)
+    ap.add_argument("--cpu-iterations", type=int, default=2)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import everythinggraph_amd as eg
+
+    dev = eg.device_info()
+    log(f"device: {dev.name.decode()} CUs={dev.compute_units} gfx{dev.gcn_arch} "
+        f"HBM={dev.hbm_bytes / 2**30:.0f} GiB")
+    scale, ef = args.scale, 16
+    n = 1 << scale
+    t0 = time.perf_counter()
+    g = eg.Graph(n, eg.MODE_ADJ_SORT, undirected=True, symmetric=True)
+    g.generate(eg.rmat_params(scale, ef, seed=1))
+    info = g.info
+    m = info.m
+    log(f"graph: R-MAT {scale} EF{ef} -u: n={n} m_in={info.m_in} m={m} "
+        f"build {info.build_ms:.1f} ms device (wall {time.perf_counter() - t0:.1f} s)")
+    csr_bytes = 2 * 8 * info.m_in + 4 * m + 4 * (n + 1) + 8 * n
+
+    # ---- PageRank: warmup + timed steps ----
+    iters = args.iterations
+    b_pr = 4 * m + 4 * (n + 1) + 16 * n
+    for _ in range(args.warmup):
+        g.pagerank(0.85, iters, download=False)
+    g.sync()
+    g.prof_reset()
+    g.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, dh, lh = g.pagerank(0.85, iters, download=False)
+    g.sync()
+    t_steps = time.perf_counter() - t0
+    prof = g.prof_read()
+    g.prof_enable(False)
+    ms_per_step = 1e3 * t_steps / args.steps
+    value = iters * b_pr / (ms_per_step * 1e-3) / 1e9
+    launches, tot = prof.get("pr_pull", (0, 0.0))
+    k_ms = tot / max(launches, 1)
+    achieved = b_pr / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    fin_l, fin_tot = prof.get("pr_finish", (0, 0.0))
+    log(f"pagerank: {ms_per_step:.3f} ms/step ({iters} it), {value:.1f} GB/s; pr_pull "
+        f"{k_ms * 1e3:.1f} us/launch = {achieved:.1f} GB/s; pr_finish "
+        f"{fin_tot / max(fin_l, 1) * 1e3:.1f} us; final L1 delta {lh[-1]:.3e}")
+
+    # ---- BFS over 64 seeded roots ----
+    deg, _ = g.degrees()
+    roots = eg.sample_roots(deg, args.bfs_roots, seed=2)
+    g.bfs(int(roots[0]), download=False)
+    teps, walls, levels = [], [], []
+    for rt in roots:
+        w0 = time.perf_counter()
+        _, _, st = g.bfs(int(rt), download=False, parents=False)
+        walls.append(time.perf_counter() - w0)
+        teps.append(st.edges_traversed / (st.device_ms * 1e-3))
+        levels.append(st.levels)
+    gteps = len(teps) / sum(1.0 / t for t in teps) / 1e9
+    plain_bfs_qps = len(walls) / sum(walls)
+    log(f"bfs: harmonic-mean {gteps:.2f} GTEPS, plain {plain_bfs_qps:.1f} queries/s, "
+        f"levels {min(levels)}..{max(levels)}")
+
+    extra = {}
+    if not args.skip_batch:
+        broots = eg.sample_roots(deg, args.batch_bfs, seed=2)
+        g.bfs_batch(broots[:64], download=False)
+        w0 = time.perf_counter()
+        _, ms = g.bfs_batch(broots, download=False)
+        wb = time.perf_counter() - w0
+        batched_bfs_qps = len(broots) / wb
+        rng = np.random.default_rng(3)
+        seeds = [eg.sample_roots(deg, int(rng.integers(1, 9)), seed=1000 + q)
+                 for q in range(args.batch_ppr)]
+        w0 = time.perf_counter()
+        _, pms = g.ppr_batch(seeds, 0.15, 30, download=False)
+        wp = time.perf_counter() - w0
+        batched_ppr_qps = len(seeds) / wp
+        # plain per-query PPR: 30 iterations of the single-vector pull kernel
+        plain_ppr_qps = 1.0 / (30 * ms_per_step / iters * 1e-3)
+        extra.update({
+            "batched_bfs_qps": round(batched_bfs_qps, 2),
+            "plain_bfs_qps": round(plain_bfs_qps, 2),
+            "batched_bfs_speedup": round(batched_bfs_qps / plain_bfs_qps, 2),
+            "batched_ppr_qps": round(batched_ppr_qps, 2),
+            "plain_ppr_qps": round(plain_ppr_qps, 2),
+            "batched_ppr_speedup": round(batched_ppr_qps / plain_ppr_qps, 2),
+        })
+        log(f"batched: MS-BFS {batched_bfs_qps:.1f} q/s (plain {plain_bfs_qps:.1f}); "
+            f"PPR SpMM {batched_ppr_qps:.1f} q/s (plain {plain_ppr_qps:.1f})")
+
+    cpu = None
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import eg_oracle_py as O
+
+        ptr, col, _ = g.csr()
+        odeg = np.diff(ptr).astype(np.uint32)
+        threads = O.L().ego_num_threads()
+        c0 = time.perf_counter()
+        O.pagerank_pull(n, ptr, col, odeg, 0.85, args.cpu_iterations)
+        ct = time.perf_counter() - c0
+        cpu_gbs = args.cpu_iterations * b_pr / ct / 1e9
+        cpu = {"value": round(cpu_gbs, 3), "unit": "GB/s", "cores": int(threads), "kind": "port",
+               "sample": f"{args.cpu_iterations} FP64 pull-PageRank iterations of the oracle "
+                         f"(OpenMP) over the same R-MAT {scale} graph, {ct:.1f} s"}
+        log(f"cpu oracle: {cpu_gbs:.2f} GB/s on {threads} threads ({ct:.1f} s)")
+        del ptr, col
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pagerank_pull_rmat{scale}_ef{ef}_undirected_{iters}it",
+            "scale": scale, "edge_factor": ef, "n": n, "m": m, "iterations": iters,
+            "layout": "csr (load mode 2, -u -s)", "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "kernel": "k_pull<true,false> (pr_pull)", "bytes_per_launch": b_pr,
+            "avg_launch_us": round(k_ms * 1e3, 2),
+        },
+        "cpu_baseline": cpu,
+        "bfs_gteps": round(gteps, 3),
+        "bfs_roots": len(roots),
+        "csr_build_ms": round(info.build_ms, 3),
+        "csr_build_gbs": round(csr_bytes / (info.build_ms * 1e-3) / 1e9, 1),
+    }
+    out.update(extra)
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

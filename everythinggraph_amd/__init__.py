@@ -1,0 +1,353 @@
+"""ctypes view of libeg, the MI355X-native EverythingGraph data-parallel path.
+
+The product is the C ABI declared in include/eg.h and implemented by
+everythinggraph_amd/lib/libeg.so (HIP, gfx950) plus the C drop-in drivers in
+everythinggraph_amd/bin.  This module is the thin Python handle the tests and
+the benchmark use to call that ABI; it holds no compute of its own and raises
+when the shared library is missing or the device is unavailable.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libeg.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+EG_OK = 0
+EG_EINVAL = -1
+EG_ENOMEM = -2
+EG_EIO = -3
+EG_EDEVICE = -4
+EG_ERANGE = -5
+EG_ENOTSUP = -6
+EG_ESTATE = -7
+EG_EAGAIN = -8
+EG_ETIMEDOUT = -9
+
+UNDIRECTED = 1
+SYMMETRIC = 2
+WEIGHTED = 4
+
+MODE_GRID_SORT = 0
+MODE_GRID_NOSORT = 1
+MODE_ADJ_SORT = 2
+MODE_ADJ_SORT_FULL = 3
+MODE_ADJ_COUNT = 4
+MODE_ADJ_DYNAMIC = 5
+MODE_GRID_SORT_SRC = 6
+MODE_EDGES_SRC = 7
+MODE_EDGES_SRC_DST = 8
+MODE_EDGES_RAW = 9
+
+LAYOUT = {"auto": 0, "adj": 1, "grid": 2, "edges": 3}
+DANGLING = {"drop": 0, "uniform": 1}
+
+# Exported symbols of libeg (the declarations of include/eg.h).
+ABI_SYMBOLS = (
+    "eg_last_error", "eg_device_query", "eg_version", "eg_graph_create",
+    "eg_graph_load_file", "eg_graph_load_records", "eg_graph_load_edges",
+    "eg_graph_generate", "eg_generate_edges", "eg_graph_free", "eg_graph_get_info",
+    "eg_graph_get_degrees", "eg_graph_get_csr", "eg_graph_get_csc", "eg_graph_get_grid",
+    "eg_graph_get_edgelist", "eg_pagerank", "eg_spmv", "eg_bfs", "eg_wcc",
+    "eg_bfs_batch", "eg_ppr_batch", "eg_prof_enable", "eg_prof_reset", "eg_prof_read",
+    "eg_sync",
+)
+
+
+class EgError(RuntimeError):
+    def __init__(self, code: int, what: str, msg: str):
+        super().__init__(f"{what} failed ({code}): {msg}")
+        self.code = code
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("compute_units", ctypes.c_int32),
+                ("gcn_arch", ctypes.c_int32), ("hbm_bytes", ctypes.c_uint64),
+                ("free_bytes", ctypes.c_uint64), ("peak_hbm_gbs", ctypes.c_double)]
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("P", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("load_mode", ctypes.c_int32), ("m_in", ctypes.c_uint64), ("m", ctypes.c_uint64),
+                ("has_csr", ctypes.c_int32), ("has_csc", ctypes.c_int32),
+                ("has_grid", ctypes.c_int32), ("has_edgelist", ctypes.c_int32),
+                ("build_ms", ctypes.c_double), ("device_bytes", ctypes.c_uint64)]
+
+
+class GenParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("scale", ctypes.c_int32), ("m_in", ctypes.c_uint64),
+                ("seed", ctypes.c_uint64), ("a", ctypes.c_double), ("b", ctypes.c_double),
+                ("c", ctypes.c_double), ("weight_seed", ctypes.c_uint64)]
+
+
+class PrOpts(ctypes.Structure):
+    _fields_ = [("damping", ctypes.c_double), ("iterations", ctypes.c_int32),
+                ("dangling", ctypes.c_int32), ("init", ctypes.c_double), ("layout", ctypes.c_int32)]
+
+
+class BfsStats(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int32), ("pull_levels", ctypes.c_int32),
+                ("reached", ctypes.c_uint64), ("edges_traversed", ctypes.c_uint64),
+                ("device_ms", ctypes.c_double)]
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64),
+                ("total_ms", ctypes.c_double)]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libeg.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        L.eg_last_error.restype = ctypes.c_char_p
+        L.eg_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc: int, what: str):
+    if rc != EG_OK:
+        raise EgError(rc, what, lib().eg_last_error().decode(errors="replace"))
+
+
+def device_info() -> DeviceInfo:
+    d = DeviceInfo()
+    _check(lib().eg_device_query(ctypes.byref(d)), "eg_device_query")
+    return d
+
+
+def rmat_params(scale: int, edge_factor: int = 16, seed: int = 1, a=0.57, b=0.19, c=0.19,
+                weight_seed: int = 1) -> GenParams:
+    return GenParams(0, scale, (1 << scale) * edge_factor, seed, a, b, c, weight_seed)
+
+
+def er_params(m_in: int, seed: int = 1, weight_seed: int = 1) -> GenParams:
+    return GenParams(1, 0, m_in, seed, 0.0, 0.0, 0.0, weight_seed)
+
+
+def generate_edges(p: GenParams, n: int, weighted: bool = False):
+    """Device generator, records copied to host arrays."""
+    src = np.empty(p.m_in, np.uint32)
+    dst = np.empty(p.m_in, np.uint32)
+    w = np.empty(p.m_in, np.float32) if weighted else None
+    _check(lib().eg_generate_edges(ctypes.byref(p), ctypes.c_uint32(n), _ptr(src), _ptr(dst),
+                                   _ptr(w)), "eg_generate_edges")
+    return src, dst, w
+
+
+class Graph:
+    """A device-resident graph in one of the reference's load modes."""
+
+    def __init__(self, n: int, load_mode: int = MODE_ADJ_SORT, undirected: bool = False,
+                 symmetric: bool = False, weighted: bool = False, P: int = 0):
+        self.n = int(n)
+        self.flags = (UNDIRECTED if undirected else 0) | (SYMMETRIC if symmetric else 0) | (
+            WEIGHTED if weighted else 0)
+        self.weighted = weighted
+        h = ctypes.c_void_p()
+        _check(lib().eg_graph_create(ctypes.c_uint32(self.n), ctypes.c_uint32(self.flags),
+                                     ctypes.c_int32(load_mode), ctypes.c_uint32(P),
+                                     ctypes.byref(h)), "eg_graph_create")
+        self._h = h
+
+    # -- lifecycle --
+    def close(self):
+        if self._h:
+            lib().eg_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def load_file(self, path: str):
+        _check(lib().eg_graph_load_file(self._h, os.fsencode(path)), "eg_graph_load_file")
+        return self
+
+    def load_records(self, data: bytes):
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _check(lib().eg_graph_load_records(self._h, buf, ctypes.c_uint64(len(data))),
+               "eg_graph_load_records")
+        return self
+
+    def load_edges(self, src: np.ndarray, dst: np.ndarray, w: Optional[np.ndarray] = None):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        w = None if w is None else np.ascontiguousarray(w, np.float32)
+        _check(lib().eg_graph_load_edges(self._h, ctypes.c_uint64(src.size), _ptr(src), _ptr(dst),
+                                         _ptr(w)), "eg_graph_load_edges")
+        return self
+
+    def generate(self, p: GenParams):
+        _check(lib().eg_graph_generate(self._h, ctypes.byref(p)), "eg_graph_generate")
+        return self
+
+    @property
+    def info(self) -> GraphInfo:
+        i = GraphInfo()
+        _check(lib().eg_graph_get_info(self._h, ctypes.byref(i)), "eg_graph_get_info")
+        return i
+
+    # -- layouts --
+    def degrees(self):
+        o = np.empty(self.n, np.uint32)
+        i = np.empty(self.n, np.uint32)
+        _check(lib().eg_graph_get_degrees(self._h, _ptr(o), _ptr(i)), "eg_graph_get_degrees")
+        return o, i
+
+    def csr(self):
+        m = self.info.m
+        ptr = np.empty(self.n + 1, np.uint64)
+        col = np.empty(m, np.uint32)
+        w = np.empty(m, np.float32) if self.weighted else None
+        _check(lib().eg_graph_get_csr(self._h, _ptr(ptr), _ptr(col), _ptr(w)), "eg_graph_get_csr")
+        return ptr, col, w
+
+    def csc(self):
+        m = self.info.m
+        ptr = np.empty(self.n + 1, np.uint64)
+        row = np.empty(m, np.uint32)
+        w = np.empty(m, np.float32) if self.weighted else None
+        _check(lib().eg_graph_get_csc(self._h, _ptr(ptr), _ptr(row), _ptr(w)), "eg_graph_get_csc")
+        return ptr, row, w
+
+    def grid(self):
+        inf = self.info
+        cp = np.empty(inf.P * inf.P + 1, np.uint64)
+        s = np.empty(inf.m, np.uint32)
+        d = np.empty(inf.m, np.uint32)
+        w = np.empty(inf.m, np.float32) if self.weighted else None
+        _check(lib().eg_graph_get_grid(self._h, _ptr(cp), _ptr(s), _ptr(d), _ptr(w)),
+               "eg_graph_get_grid")
+        return cp, s, d, w
+
+    def edgelist(self):
+        m = self.info.m
+        s = np.empty(m, np.uint32)
+        d = np.empty(m, np.uint32)
+        w = np.empty(m, np.float32) if self.weighted else None
+        _check(lib().eg_graph_get_edgelist(self._h, _ptr(s), _ptr(d), _ptr(w)),
+               "eg_graph_get_edgelist")
+        return s, d, w
+
+    # -- algorithms --
+    def pagerank(self, damping=0.85, iterations=10, dangling="drop", init=0.0, layout="auto",
+                 download=True):
+        o = PrOpts(damping, iterations, DANGLING[dangling], init, LAYOUT[layout])
+        r = np.empty(self.n, np.float32) if download else None
+        dh = np.zeros(max(iterations, 1), np.float64)
+        lh = np.zeros(max(iterations, 1), np.float64)
+        _check(lib().eg_pagerank(self._h, ctypes.byref(o), _ptr(r), _ptr(dh), _ptr(lh)),
+               "eg_pagerank")
+        return r, dh[:iterations], lh[:iterations]
+
+    def spmv(self, x: np.ndarray, layout="auto"):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty(self.n, np.float32)
+        _check(lib().eg_spmv(self._h, ctypes.c_int32(LAYOUT[layout]), _ptr(x), _ptr(y)), "eg_spmv")
+        return y
+
+    def bfs(self, root: int, start_pull=False, switch=True, parents=True, download=True):
+        dist = np.empty(self.n, np.uint32) if download else None
+        par = np.empty(self.n, np.uint32) if (parents and download) else None
+        st = BfsStats()
+        _check(lib().eg_bfs(self._h, ctypes.c_uint32(root), ctypes.c_int32(int(start_pull)),
+                            ctypes.c_int32(int(switch)), _ptr(dist), _ptr(par), ctypes.byref(st)),
+               "eg_bfs")
+        return dist, par, st
+
+    def wcc(self):
+        lab = np.empty(self.n, np.int64)
+        rounds = ctypes.c_int32()
+        _check(lib().eg_wcc(self._h, _ptr(lab), ctypes.byref(rounds)), "eg_wcc")
+        return lab, rounds.value
+
+    def bfs_batch(self, roots: Sequence[int], download=True):
+        r = np.ascontiguousarray(roots, np.uint32)
+        out = np.empty((r.size, self.n), np.uint32) if download else None
+        ms = ctypes.c_double()
+        _check(lib().eg_bfs_batch(self._h, ctypes.c_uint32(r.size), _ptr(r), _ptr(out),
+                                  ctypes.byref(ms)), "eg_bfs_batch")
+        return out, ms.value
+
+    def ppr_batch(self, seeds: Sequence[Sequence[int]], alpha=0.15, iterations=30, download=True):
+        ptr = np.zeros(len(seeds) + 1, np.uint32)
+        ptr[1:] = np.cumsum([len(s) for s in seeds])
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(s, np.uint32) for s in seeds])
+                                    if seeds else np.zeros(0, np.uint32), np.uint32)
+        out = np.empty((len(seeds), self.n), np.float32) if download else None
+        ms = ctypes.c_double()
+        _check(lib().eg_ppr_batch(self._h, ctypes.c_uint32(len(seeds)), _ptr(ptr), _ptr(flat),
+                                  ctypes.c_double(alpha), ctypes.c_int32(iterations), _ptr(out),
+                                  ctypes.byref(ms)), "eg_ppr_batch")
+        return out, ms.value
+
+    # -- timing --
+    def prof_enable(self, on=True):
+        _check(lib().eg_prof_enable(self._h, ctypes.c_int32(int(on))), "eg_prof_enable")
+
+    def prof_reset(self):
+        _check(lib().eg_prof_reset(self._h), "eg_prof_reset")
+
+    def prof_read(self) -> dict:
+        arr = (KernelTime * 64)()
+        cnt = ctypes.c_int32()
+        _check(lib().eg_prof_read(self._h, arr, ctypes.c_int32(64), ctypes.byref(cnt)),
+               "eg_prof_read")
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms)
+                for i in range(min(cnt.value, 64))}
+
+    def sync(self):
+        _check(lib().eg_sync(self._h), "eg_sync")
+
+
+def sample_roots(deg: np.ndarray, k: int, seed: int = 2) -> np.ndarray:
+    """k distinct vertices with degree > 0, drawn from the counter-based stream
+    mix64(mix64(seed) ^ j) mapped to [0, n) by multiply-shift (the same mixing
+    function as the generator)."""
+    n = deg.size
+    M = (1 << 64) - 1
+
+    def mix(z):
+        z = (z + 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+
+    key = mix(seed)
+    out, seen = [], set()
+    j = 0
+    if int((deg > 0).sum()) < k:
+        raise ValueError("not enough vertices with edges")
+    while len(out) < k:
+        h = mix(key ^ j)
+        v = ((h >> 32) * n) >> 32
+        j += 1
+        if deg[v] > 0 and v not in seen:
+            seen.add(v)
+            out.append(v)
+    return np.asarray(out, np.uint32)

@@ -1,0 +1,272 @@
+// eg_internal.h -- shared declarations of libeg (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/eg.h"
+
+// ---------------------------------------------------------------------------
+// Errors: a thread-local message plus a negative status code.
+// ---------------------------------------------------------------------------
+int eg_fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define EG_HIP(call)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return eg_fail(EG_EDEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #call,    \
+                     hipGetErrorString(e_));                                   \
+  } while (0)
+
+#define EG_TRY(call)                                                           \
+  do {                                                                         \
+    int rc_ = (call);                                                          \
+    if (rc_ != EG_OK) return rc_;                                              \
+  } while (0)
+
+// Launch check: reports configuration errors of the launch just issued.
+#define EG_LAUNCHED() EG_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Device memory helpers.
+// ---------------------------------------------------------------------------
+int eg_dalloc(void **p, size_t bytes, const char *what);
+void eg_dfree(void **p);
+template <typename T>
+static inline int eg_dalloc_t(T **p, size_t count, const char *what) {
+  return eg_dalloc(reinterpret_cast<void **>(p), count * sizeof(T), what);
+}
+template <typename T>
+static inline void eg_dfree_t(T **p) {
+  eg_dfree(reinterpret_cast<void **>(p));
+}
+
+// ---------------------------------------------------------------------------
+// Kernel timing: hipEvents recorded around named launches on the graph's
+// stream, resolved when read.
+// ---------------------------------------------------------------------------
+struct EgProfSlot {
+  std::string name;
+  std::vector<hipEvent_t> pending;  // start, stop pairs
+  uint64_t launches = 0;
+  double total_ms = 0.0;
+};
+
+// Work decomposition for a pull kernel over one CSR-like orientation
+// (ptr[n+1], idx[m]).  Short rows are grouped into chunks of the merged
+// (row, entry) sequence; rows with more than kLongRow entries are split into
+// segments of kSegLen entries whose compensated partial sums are combined in
+// segment order.
+struct PullPlan {
+  bool built = false;
+  uint32_t nchunks = 0;     // stream chunks
+  uint32_t nlong = 0;       // long rows
+  uint32_t nseg = 0;        // segments over all long rows
+  uint32_t *chunk_row = nullptr;  // [nchunks + 1]
+  uint32_t *long_rows = nullptr;  // [nlong]
+  uint32_t *seg_base = nullptr;   // [nlong + 1]
+  float2 *seg_part = nullptr;     // [nseg * kMaxCols] (sum, compensation)
+  double2 *chunk_part = nullptr;  // [nchunks] FP64 (dangling, l1)
+  double2 *fin_part = nullptr;    // [kFinBlocks]
+  uint32_t *fin_counter = nullptr;
+};
+
+struct GridCompute;
+struct BatchState;
+
+struct eg_graph {
+  uint32_t n = 0, P = 0, flags = 0;
+  int32_t load_mode = 2;
+  uint64_t m_in = 0, m = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool loaded = false;
+  double build_ms = 0.0;
+
+  // Records in input order (memblock), reversed copies appended with -u.
+  uint32_t *e_src = nullptr, *e_dst = nullptr;
+  float *e_w = nullptr;
+  // CSR (out-edges) and the sorted src of every entry.
+  uint32_t *out_ptr = nullptr, *out_col = nullptr, *out_src = nullptr;
+  float *out_w = nullptr;
+  // CSC (in-edges).
+  uint32_t *in_ptr = nullptr, *in_idx = nullptr;
+  float *in_w = nullptr;
+  uint32_t *out_deg = nullptr, *in_deg = nullptr;
+  // 2-D grid.
+  uint32_t *cell_ptr = nullptr, *g_src = nullptr, *g_dst = nullptr;
+  float *g_w = nullptr;
+  // Edge array of modes 7/8/9.
+  uint32_t *el_src = nullptr, *el_dst = nullptr;
+  float *el_w = nullptr;
+
+  PullPlan plan_in;   // pull over in-edges
+  PullPlan plan_out;  // pull over out-edges (symmetric graphs, BFS)
+
+  // Algorithm state (allocated on first use, kept for reuse).
+  float *rank_a = nullptr, *rank_b = nullptr, *contrib_a = nullptr,
+        *contrib_b = nullptr;
+  double *pr_stats = nullptr;  // [2 * max_iters]
+  int pr_stats_cap = 0;
+  uint32_t *bfs_dist = nullptr, *bfs_parent = nullptr;
+  uint64_t *bfs_visited = nullptr, *bfs_front = nullptr, *bfs_next = nullptr;
+  uint32_t *bfs_q = nullptr, *bfs_nq = nullptr;
+  uint32_t *bfs_ctr = nullptr;      // device counters
+  uint32_t *h_ctr = nullptr;        // pinned mirror
+  int64_t *wcc_label = nullptr;
+  uint32_t *wcc_lab32 = nullptr;
+
+  // Scratch buffers shared by the builders (grown on demand).
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+
+  GridCompute *gridc = nullptr;  // compute copy of the grid (eg_grid.hip)
+  BatchState *batch = nullptr;   // multi-source state (eg_batch.hip)
+
+  int prof_on = 0;
+  std::vector<EgProfSlot> prof;
+};
+
+void eg_grid_free(eg_graph *g);
+void eg_batch_free(eg_graph *g);
+
+bool eg_has_in_edges(const eg_graph *g);
+int eg_scratch(eg_graph *g, size_t bytes, void **out);
+int eg_require_loaded(const eg_graph *g);
+
+// Profiling scope around one launch.
+struct EgProf {
+  eg_graph *g;
+  EgProfSlot *slot = nullptr;
+  hipEvent_t ev[2];
+  EgProf(eg_graph *g_, const char *name);
+  ~EgProf();
+};
+
+// ---------------------------------------------------------------------------
+// Primitives (eg_prims.hip).
+// ---------------------------------------------------------------------------
+// out[0..count) = exclusive scan of in, out[count] = total.  in and out may
+// alias.  tmp must hold eg_scan_tmp_bytes(count).
+size_t eg_scan_tmp_bytes(uint64_t count);
+int eg_scan_excl(hipStream_t s, const uint32_t *in, uint32_t *out,
+                 uint64_t count, void *tmp);
+// deg[k] += number of keys equal to k (deg zeroed by the caller).
+int eg_histogram(hipStream_t s, const uint32_t *keys, uint64_t count,
+                 uint32_t *deg, uint32_t nbins);
+// Stable LSD radix sort of count keys < 2^key_bits with up to three 32-bit
+// payload columns.  kout may be NULL when the sorted keys are not needed.
+// Inputs are not modified.
+struct EgSortCols {
+  const uint32_t *in[3] = {nullptr, nullptr, nullptr};
+  uint32_t *out[3] = {nullptr, nullptr, nullptr};
+  int np = 0;
+};
+size_t eg_radix_tmp_bytes(uint64_t count, int np);
+int eg_radix_sort(hipStream_t s, uint64_t count, int key_bits,
+                  const uint32_t *kin, uint32_t *kout, const EgSortCols &cols,
+                  void *tmp);
+int eg_bits_for(uint64_t nkeys);
+
+// Device generator (eg_prims.hip).
+int eg_gen_device(hipStream_t s, const eg_gen_params *p, uint32_t n,
+                  uint32_t *src, uint32_t *dst, float *w);
+// Unpack raw AoS records into SoA (+ reversed copies when undirected).
+int eg_unpack_records(hipStream_t s, const void *rec, uint64_t m_in,
+                      int weighted, uint32_t *src, uint32_t *dst, float *w);
+int eg_append_reverse(hipStream_t s, uint64_t m_in, uint32_t *src,
+                      uint32_t *dst, float *w);
+// Returns 1 in *bad if any id >= n.
+int eg_check_ids(hipStream_t s, const uint32_t *a, const uint32_t *b,
+                 uint64_t count, uint32_t n, uint32_t *d_flag, int *bad);
+int eg_cell_keys(hipStream_t s, const uint32_t *src, const uint32_t *dst,
+                 uint64_t count, uint32_t n, uint32_t P, uint32_t *keys);
+
+// Pull planning (eg_pull.hip).
+int eg_plan_build(eg_graph *g, PullPlan *pl, const uint32_t *ptr);
+void eg_plan_free(PullPlan *pl);
+
+// ---------------------------------------------------------------------------
+// Device helpers.
+// ---------------------------------------------------------------------------
+#define EG_WAVE 64
+
+__device__ __forceinline__ uint32_t eg_partition_id(uint32_t v, uint32_t n,
+                                                     uint32_t P) {
+  // get_partition_id, init_all.c:54-60.
+  const uint32_t mod_val = n % P;
+  const uint32_t psize = mod_val ? n / P + 1 : n / P;
+  if (mod_val == 0) return v / psize;
+  const uint64_t split = (uint64_t)mod_val * psize;
+  return v < split ? v / psize : (uint32_t)((v - split) / (psize - 1)) + mod_val;
+}
+
+// Error-free transformation a + b = s + e (Knuth TwoSum).
+__device__ __forceinline__ void eg_two_sum(float a, float b, float &s,
+                                           float &e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated accumulator (Neumaier's variant of Kahan summation).
+struct EgKahan {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    float t, e;
+    eg_two_sum(s, x, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ void merge(float s2, float c2) {
+    float t, e;
+    eg_two_sum(s, s2, t, e);
+    s = t;
+    c += e + c2;
+  }
+  __device__ __forceinline__ float value() const { return s + c; }
+};
+
+// Butterfly merge of compensated pairs across the 64 lanes of a wave.  Every
+// lane ends with the same (s, c); the combination order is fixed.
+__device__ __forceinline__ void eg_wave_kahan_allreduce(EgKahan &k) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const float s2 = __shfl_xor(k.s, o);
+    const float c2 = __shfl_xor(k.c, o);
+    // Combine in lane order so that both partners compute the same pair.
+    const int lane = __lane_id();
+    EgKahan lo, hi;
+    if (lane & o) {
+      lo.s = s2; lo.c = c2; hi.s = k.s; hi.c = k.c;
+    } else {
+      lo.s = k.s; lo.c = k.c; hi.s = s2; hi.c = c2;
+    }
+    lo.merge(hi.s, hi.c);
+    k = lo;
+  }
+}
+
+__device__ __forceinline__ double eg_wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double w = __shfl_xor(v, o);
+    const int lane = __lane_id();
+    v = (lane & o) ? (w + v) : (v + w);
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t eg_wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ uint64_t eg_lanemask_lt() {
+  return (1ull << __lane_id()) - 1ull;
+}

@@ -1,0 +1,572 @@
+// eg_prims.hip -- device primitives of the layout builders: exclusive scan,
+// vertex histogram, stable LSD radix sort, record unpacking and the synthetic
+// generator.
+//
+// The reference builds every layout with Ligra's stable radix sort
+// (intSort::iSort, radixSort_ligra.h:355) and sequential/blocked scans
+// (utils_ligra.h:185-214).  Here the same contracts (stable by key; exclusive
+// prefix sums) are met with wave64 kernels: ballot-based stable ranking inside
+// a tile, LDS staging so that every tile writes contiguous runs, and a
+// decoupled count matrix scanned on the device.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "eg_internal.h"
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;  // per thread; a wave scans 1024 items
+constexpr uint32_t kScanTile = kScanBlock * kScanItems;
+
+// Inclusive scan across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Per-tile sums.
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(
+    const uint32_t *__restrict__ in, uint64_t count, uint32_t *__restrict__ part) {
+  __shared__ uint32_t s_w[kScanBlock / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint32_t acc = 0;
+  for (uint32_t j = threadIdx.x; j < kScanTile; j += kScanBlock) {
+    const uint64_t i = base + j;
+    if (i < count) acc += in[i];
+  }
+  acc = eg_wave_sum_u32(acc);
+  if (__lane_id() == 0) s_w[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kScanBlock / 64; w++) t += s_w[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of one tile plus the scanned tile offset.  Wave w scans the
+// contiguous items [w*1024, (w+1)*1024) of the tile in 16 steps of 64.
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(
+    const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t count,
+    const uint32_t *__restrict__ part_scanned) {
+  __shared__ uint32_t s_w[kScanBlock / 64];
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)wave * 64 * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t carry = 0;
+#pragma unroll
+  for (int s = 0; s < kScanItems; s++) {
+    const uint64_t i = base + (uint64_t)s * 64 + lane;
+    const uint32_t x = i < count ? in[i] : 0u;
+    const uint32_t inc = wave_incl_scan(x);
+    v[s] = carry + inc - x;
+    carry += __shfl(inc, 63);
+  }
+  if (lane == 0) s_w[wave] = carry;
+  __syncthreads();
+  uint32_t off = part_scanned ? part_scanned[blockIdx.x] : 0u;
+  for (int w = 0; w < wave; w++) off += s_w[w];
+#pragma unroll
+  for (int s = 0; s < kScanItems; s++) {
+    const uint64_t i = base + (uint64_t)s * 64 + lane;
+    if (i < count) out[i] = off + v[s];
+  }
+}
+
+__global__ void k_store_total(const uint32_t *__restrict__ last_in,
+                              const uint32_t *__restrict__ last_out,
+                              uint32_t *__restrict__ total) {
+  // total = exclusive[count-1] + in[count-1]; written by one thread after the
+  // apply kernel (in may alias out, so the input value was saved first).
+  *total = *last_out + *last_in;
+}
+
+__global__ void k_copy_u32(const uint32_t *__restrict__ a, uint32_t *__restrict__ b) {
+  *b = *a;
+}
+
+// ---------------------------------------------------------------------------
+// Vertex histogram with LDS privatisation of the low ids, where R-MAT hubs
+// live; higher ids go straight to global atomics.
+// ---------------------------------------------------------------------------
+constexpr int kHistBlock = 256;
+constexpr int kHistItems = 32;
+constexpr uint32_t kHistLds = 4096;
+
+__global__ __launch_bounds__(kHistBlock) void k_histogram(
+    const uint32_t *__restrict__ keys, uint64_t count, uint32_t *__restrict__ deg,
+    uint32_t nbins) {
+  __shared__ uint32_t s_h[kHistLds];
+  const uint32_t lds_bins = nbins < kHistLds ? nbins : kHistLds;
+  for (uint32_t j = threadIdx.x; j < lds_bins; j += kHistBlock) s_h[j] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kHistBlock * kHistItems;
+#pragma unroll 4
+  for (int s = 0; s < kHistItems; s++) {
+    const uint64_t i = base + (uint64_t)s * kHistBlock + threadIdx.x;
+    if (i < count) {
+      const uint32_t k = keys[i];
+      if (k < lds_bins)
+        atomicAdd(&s_h[k], 1u);
+      else
+        atomicAdd(&deg[k], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < lds_bins; j += kHistBlock)
+    if (s_h[j]) atomicAdd(&deg[j], s_h[j]);
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort.  One pass = digit histogram per tile (count matrix
+// laid out digit-major), exclusive scan of the matrix, stable scatter.
+// The scatter ranks a tile stably: wave w owns the contiguous sub-tile
+// [w*1024, (w+1)*1024) and walks it in 16 steps of 64 lanes; lanes with equal
+// digits find each other with one ballot per digit bit, and a per-wave LDS
+// counter row carries the running count.  The tile is then reordered through
+// LDS so that each digit's run is written contiguously.
+// ---------------------------------------------------------------------------
+constexpr int kRadixBlock = 256;
+constexpr int kRadixItems = 16;
+constexpr uint32_t kRadixTile = kRadixBlock * kRadixItems;  // 4096
+constexpr int kRadixWaves = kRadixBlock / 64;
+
+__global__ __launch_bounds__(kRadixBlock) void k_radix_hist(
+    const uint32_t *__restrict__ keys, uint64_t count, int shift, uint32_t mask,
+    uint32_t ntiles, uint32_t *__restrict__ counts) {
+  __shared__ uint32_t s_h[256];
+  s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kRadixTile;
+#pragma unroll 4
+  for (int s = 0; s < kRadixItems; s++) {
+    const uint64_t i = base + (uint64_t)s * kRadixBlock + threadIdx.x;
+    if (i < count) atomicAdd(&s_h[(keys[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x <= mask) counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+}
+
+template <int NP, bool WRITE_KEYS>
+__global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(
+    const uint32_t *__restrict__ kin, uint32_t *__restrict__ kout,
+    const uint32_t *__restrict__ p0i, uint32_t *__restrict__ p0o,
+    const uint32_t *__restrict__ p1i, uint32_t *__restrict__ p1o,
+    const uint32_t *__restrict__ p2i, uint32_t *__restrict__ p2o, uint64_t count,
+    int shift, int nbits, uint32_t ntiles, const uint32_t *__restrict__ offs) {
+  __shared__ uint32_t s_wh[kRadixWaves][256];
+  __shared__ uint32_t s_tstart[256];
+  __shared__ uint32_t s_gofs[256];
+  __shared__ uint32_t s_wsum[kRadixWaves];
+  __shared__ uint32_t s_key[kRadixTile];
+  __shared__ uint32_t s_pay[NP > 0 ? NP : 1][kRadixTile];
+
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  const uint32_t mask = (1u << nbits) - 1u;
+  const uint64_t tbase = (uint64_t)blockIdx.x * kRadixTile;
+  const uint32_t* pin[3] = {p0i, p1i, p2i};
+  uint32_t* pout[3] = {p0o, p1o, p2o};
+
+#pragma unroll
+  for (int w = 0; w < kRadixWaves; w++) s_wh[w][tid] = 0;
+  s_gofs[tid] = tid <= (int)mask ? offs[(uint64_t)tid * ntiles + blockIdx.x] : 0u;
+  __syncthreads();
+
+  uint32_t key[kRadixItems];
+  uint32_t rank[kRadixItems];
+  uint32_t pay[NP > 0 ? NP : 1][kRadixItems];
+  const uint64_t wbase = tbase + (uint64_t)wave * 64 * kRadixItems;
+  const uint64_t lt = eg_lanemask_lt();
+#pragma unroll
+  for (int s = 0; s < kRadixItems; s++) {
+    const uint64_t i = wbase + (uint64_t)s * 64 + lane;
+    const bool valid = i < count;
+    const uint32_t k = valid ? kin[i] : 0u;
+    key[s] = k;
+#pragma unroll
+    for (int p = 0; p < NP; p++) pay[p][s] = valid ? pin[p][i] : 0u;
+    const uint32_t d = (k >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < nbits; b++) {
+      const int bit = (d >> b) & 1;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t r = __popcll(peers & lt);
+    rank[s] = 0;
+    if (valid) {
+      const uint32_t base = s_wh[wave][d];
+      rank[s] = base + r;
+      if (r == 0) s_wh[wave][d] = base + (uint32_t)__popcll(peers);
+    }
+  }
+  __syncthreads();
+
+  // Per digit (one thread each): exclusive prefix over waves and the tile's
+  // exclusive scan over digits.
+  {
+    const int d = tid;
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kRadixWaves; w++) {
+      const uint32_t c = s_wh[w][d];
+      s_wh[w][d] = run;
+      run += c;
+    }
+    const uint32_t inc = wave_incl_scan(run);
+    if (lane == 63) s_wsum[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int w = 0; w < wave; w++) off += s_wsum[w];
+    s_tstart[d] = off + inc - run;
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int s = 0; s < kRadixItems; s++) {
+    const uint64_t i = wbase + (uint64_t)s * 64 + lane;
+    if (i < count) {
+      const uint32_t d = (key[s] >> shift) & mask;
+      const uint32_t pos = s_tstart[d] + s_wh[wave][d] + rank[s];
+      s_key[pos] = key[s];
+#pragma unroll
+      for (int p = 0; p < NP; p++) s_pay[p][pos] = pay[p][s];
+    }
+  }
+  __syncthreads();
+
+  const uint64_t rem = count - tbase;
+  const uint32_t nvalid = rem < kRadixTile ? (uint32_t)rem : kRadixTile;
+  for (uint32_t j = tid; j < nvalid; j += kRadixBlock) {
+    const uint32_t k = s_key[j];
+    const uint32_t d = (k >> shift) & mask;
+    const uint32_t dst = s_gofs[d] + (j - s_tstart[d]);
+    if (WRITE_KEYS) kout[dst] = k;
+#pragma unroll
+    for (int p = 0; p < NP; p++) pout[p][dst] = s_pay[p][j];
+  }
+}
+
+template <int NP>
+int launch_scatter(hipStream_t s, bool wk, uint32_t ntiles, const uint32_t *kin,
+                   uint32_t *kout, const uint32_t *const *pi, uint32_t *const *po,
+                   uint64_t count, int shift, int nbits, const uint32_t *offs) {
+  if (wk)
+    hipLaunchKernelGGL((k_radix_scatter<NP, true>), dim3(ntiles), dim3(kRadixBlock), 0, s,
+                       kin, kout, pi[0], po[0], pi[1], po[1], pi[2], po[2], count, shift,
+                       nbits, ntiles, offs);
+  else
+    hipLaunchKernelGGL((k_radix_scatter<NP, false>), dim3(ntiles), dim3(kRadixBlock), 0, s,
+                       kin, kout, pi[0], po[0], pi[1], po[1], pi[2], po[2], count, shift,
+                       nbits, ntiles, offs);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Generator (same streams as the oracle's ego_gen_rmat / ego_gen_er /
+// ego_gen_weights).
+// ---------------------------------------------------------------------------
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += kGolden;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline uint64_t mix64_host(uint64_t z) { return mix64(z); }
+
+__global__ void k_gen_rmat(uint64_t m_in, int scale, uint64_t skey, uint32_t ta,
+                           uint32_t tab, uint32_t tabc, uint32_t *__restrict__ src,
+                           uint32_t *__restrict__ dst) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m_in) return;
+  const uint64_t ekey = mix64(skey ^ e);
+  uint32_t s = 0, d = 0;
+  uint64_t h = 0;
+  for (int l = 0; l < scale; l++) {
+    uint32_t r;
+    if ((l & 1) == 0) {
+      h = mix64(ekey + (uint64_t)(l / 2 + 1) * kGolden);
+      r = (uint32_t)(h >> 32);
+    } else {
+      r = (uint32_t)h;
+    }
+    const uint32_t qs = r >= tab ? 1u : 0u;
+    const uint32_t qd = (r >= ta && r < tab) || r >= tabc ? 1u : 0u;
+    s = (s << 1) | qs;
+    d = (d << 1) | qd;
+  }
+  src[e] = s;
+  dst[e] = d;
+}
+
+__global__ void k_gen_er(uint64_t m_in, uint32_t n, uint64_t skey,
+                         uint32_t *__restrict__ src, uint32_t *__restrict__ dst) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m_in) return;
+  const uint64_t h = mix64(mix64(skey ^ e) + kGolden);
+  src[e] = (uint32_t)(((h >> 32) * (uint64_t)n) >> 32);
+  dst[e] = (uint32_t)(((h & 0xFFFFFFFFull) * (uint64_t)n) >> 32);
+}
+
+__global__ void k_gen_weights(uint64_t m_in, uint64_t wkey, float *__restrict__ w) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m_in) return;
+  const uint64_t h = mix64(wkey ^ e);
+  w[e] = (float)((h >> 40) + 1) * 0x1p-24f;
+}
+
+__global__ void k_unpack(const uint32_t *__restrict__ rec, uint64_t m_in, int words,
+                         uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                         float *__restrict__ w) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m_in) return;
+  const uint32_t *r = rec + e * (uint64_t)words;
+  src[e] = r[0];
+  dst[e] = r[1];
+  if (words == 3 && w) w[e] = __uint_as_float(r[2]);
+}
+
+// preload_graph's reversed copies (init_all.c:102-106) at slot e + m_in.
+__global__ void k_append_reverse(uint64_t m_in, uint32_t *__restrict__ src,
+                                 uint32_t *__restrict__ dst, float *__restrict__ w) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m_in) return;
+  src[e + m_in] = dst[e];
+  dst[e + m_in] = src[e];
+  if (w) w[e + m_in] = w[e];
+}
+
+__global__ void k_check_ids(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                            uint64_t count, uint32_t n, uint32_t *__restrict__ flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool bad = i < count && (a[i] >= n || b[i] >= n);
+  if (__ballot(bad) && __lane_id() == 0) atomicOr(flag, 1u);
+}
+
+__global__ void k_cell_keys(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst,
+                            uint64_t count, uint32_t n, uint32_t P,
+                            uint32_t *__restrict__ keys) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  keys[i] = eg_partition_id(src[i], n, P) * P + eg_partition_id(dst[i], n, P);
+}
+
+inline uint32_t blocks_for(uint64_t count, uint32_t per) {
+  return (uint32_t)((count + per - 1) / per);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host entry points.
+// ---------------------------------------------------------------------------
+size_t eg_scan_tmp_bytes(uint64_t count) {
+  size_t bytes = 256;  // saved last input value
+  uint64_t c = count;
+  while (c > kScanTile) {
+    c = (c + kScanTile - 1) / kScanTile;
+    bytes += ((c + 1) * sizeof(uint32_t) + 255) / 256 * 256;
+  }
+  return bytes;
+}
+
+static int scan_level(hipStream_t s, const uint32_t *in, uint32_t *out,
+                      uint64_t count, uint8_t *tmp) {
+  if (count == 0) return EG_OK;
+  const uint32_t tiles = blocks_for(count, kScanTile);
+  if (tiles == 1) {
+    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kScanBlock), 0, s, in, out, count,
+                       (const uint32_t *)nullptr);
+    EG_LAUNCHED();
+    return EG_OK;
+  }
+  uint32_t *part = reinterpret_cast<uint32_t *>(tmp);
+  const size_t part_bytes = ((tiles + 1) * sizeof(uint32_t) + 255) / 256 * 256;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanBlock), 0, s, in, count, part);
+  EG_LAUNCHED();
+  EG_TRY(scan_level(s, part, part, tiles, tmp + part_bytes));
+  hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanBlock), 0, s, in, out, count,
+                     (const uint32_t *)part);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_scan_excl(hipStream_t s, const uint32_t *in, uint32_t *out, uint64_t count,
+                 void *tmp) {
+  uint8_t *t = reinterpret_cast<uint8_t *>(tmp);
+  uint32_t *saved = reinterpret_cast<uint32_t *>(t);
+  if (count == 0) {
+    EG_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s));
+    return EG_OK;
+  }
+  hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(1), 0, s, in + count - 1, saved);
+  EG_LAUNCHED();
+  EG_TRY(scan_level(s, in, out, count, t + 256));
+  hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, s, saved, out + count - 1,
+                     out + count);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_histogram(hipStream_t s, const uint32_t *keys, uint64_t count, uint32_t *deg,
+                 uint32_t nbins) {
+  if (count == 0) return EG_OK;
+  const uint32_t blocks = blocks_for(count, (uint64_t)kHistBlock * kHistItems);
+  hipLaunchKernelGGL(k_histogram, dim3(blocks), dim3(kHistBlock), 0, s, keys, count, deg,
+                     nbins);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_bits_for(uint64_t nkeys) {
+  int b = 0;
+  while (b < 32 && ((uint64_t)1 << b) < nkeys) b++;
+  return b;
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t eg_radix_tmp_bytes(uint64_t count, int np) {
+  const uint64_t ntiles = (count + kRadixTile - 1) / kRadixTile;
+  const uint64_t cells = 256 * (ntiles ? ntiles : 1);
+  size_t b = 0;
+  b += 2 * align256(count * 4) * (size_t)(1 + np);  // ping-pong keys + payloads
+  b += align256((cells + 1) * 4);                   // count matrix (scanned in place)
+  b += align256(eg_scan_tmp_bytes(cells));
+  return b;
+}
+
+int eg_radix_sort(hipStream_t s, uint64_t count, int key_bits, const uint32_t *kin,
+                  uint32_t *kout, const EgSortCols &cols, void *tmp) {
+  if (count == 0) return EG_OK;
+  if (count >= (1ull << 32))
+    return eg_fail(EG_ENOTSUP, "radix sort: %llu items exceed 2^32",
+                   (unsigned long long)count);
+  const int np = cols.np;
+  uint8_t *t = reinterpret_cast<uint8_t *>(tmp);
+  uint32_t *kb[2];
+  uint32_t *pb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  for (int h = 0; h < 2; h++) {
+    kb[h] = reinterpret_cast<uint32_t *>(t);
+    t += align256(count * 4);
+    for (int p = 0; p < np; p++) {
+      pb[h][p] = reinterpret_cast<uint32_t *>(t);
+      t += align256(count * 4);
+    }
+  }
+  const uint32_t ntiles = (uint32_t)((count + kRadixTile - 1) / kRadixTile);
+  uint32_t *counts = reinterpret_cast<uint32_t *>(t);
+  t += align256(((uint64_t)256 * ntiles + 1) * 4);
+  void *scan_tmp = t;
+
+  int bits = key_bits < 1 ? 1 : key_bits;
+  const int passes = (bits + 7) / 8;
+  const int bpp = (bits + passes - 1) / passes;
+  const uint32_t *cur_k = kin;
+  const uint32_t *cur_p[3] = {cols.in[0], cols.in[1], cols.in[2]};
+  for (int pass = 0; pass < passes; pass++) {
+    const int shift = pass * bpp;
+    const int nb = (bits - shift) < bpp ? (bits - shift) : bpp;
+    const uint32_t mask = (1u << nb) - 1u;
+    const bool last = pass == passes - 1;
+    uint32_t *nk = last ? kout : kb[pass & 1];
+    uint32_t *np_out[3] = {nullptr, nullptr, nullptr};
+    for (int p = 0; p < np; p++) np_out[p] = last ? cols.out[p] : pb[pass & 1][p];
+
+    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kRadixBlock), 0, s, cur_k, count,
+                       shift, mask, ntiles, counts);
+    EG_LAUNCHED();
+    const uint64_t cells = (uint64_t)(mask + 1) * ntiles;
+    EG_TRY(eg_scan_excl(s, counts, counts, cells, scan_tmp));
+    const bool wk = nk != nullptr;
+    switch (np) {
+      case 0: EG_TRY(launch_scatter<0>(s, wk, ntiles, cur_k, nk, cur_p, np_out, count, shift, nb, counts)); break;
+      case 1: EG_TRY(launch_scatter<1>(s, wk, ntiles, cur_k, nk, cur_p, np_out, count, shift, nb, counts)); break;
+      case 2: EG_TRY(launch_scatter<2>(s, wk, ntiles, cur_k, nk, cur_p, np_out, count, shift, nb, counts)); break;
+      case 3: EG_TRY(launch_scatter<3>(s, wk, ntiles, cur_k, nk, cur_p, np_out, count, shift, nb, counts)); break;
+      default: return eg_fail(EG_EINVAL, "radix sort: %d payload columns", np);
+    }
+    if (!last && nk == nullptr) return eg_fail(EG_EINVAL, "radix sort: key buffer missing");
+    cur_k = nk;
+    for (int p = 0; p < np; p++) cur_p[p] = np_out[p];
+  }
+  return EG_OK;
+}
+
+int eg_gen_device(hipStream_t s, const eg_gen_params *p, uint32_t n, uint32_t *src,
+                  uint32_t *dst, float *w) {
+  if (p->m_in == 0) return EG_OK;
+  const uint32_t blocks = blocks_for(p->m_in, 256);
+  const uint64_t skey = mix64_host(p->seed);
+  if (p->kind == 0) {
+    const double two32 = 4294967296.0;
+    const uint32_t ta = (uint32_t)(p->a * two32);
+    const uint32_t tab = (uint32_t)((p->a + p->b) * two32);
+    const uint32_t tabc = (uint32_t)((p->a + p->b + p->c) * two32);
+    hipLaunchKernelGGL(k_gen_rmat, dim3(blocks), dim3(256), 0, s, p->m_in, p->scale, skey,
+                       ta, tab, tabc, src, dst);
+  } else {
+    hipLaunchKernelGGL(k_gen_er, dim3(blocks), dim3(256), 0, s, p->m_in, n, skey, src, dst);
+  }
+  EG_LAUNCHED();
+  if (w) {
+    const uint64_t wkey = mix64_host(p->weight_seed ^ 0x5745494748545321ull);
+    hipLaunchKernelGGL(k_gen_weights, dim3(blocks), dim3(256), 0, s, p->m_in, wkey, w);
+    EG_LAUNCHED();
+  }
+  return EG_OK;
+}
+
+int eg_unpack_records(hipStream_t s, const void *rec, uint64_t m_in, int weighted,
+                      uint32_t *src, uint32_t *dst, float *w) {
+  if (m_in == 0) return EG_OK;
+  hipLaunchKernelGGL(k_unpack, dim3(blocks_for(m_in, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const uint32_t *>(rec), m_in, weighted ? 3 : 2, src,
+                     dst, w);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_append_reverse(hipStream_t s, uint64_t m_in, uint32_t *src, uint32_t *dst, float *w) {
+  if (m_in == 0) return EG_OK;
+  hipLaunchKernelGGL(k_append_reverse, dim3(blocks_for(m_in, 256)), dim3(256), 0, s, m_in,
+                     src, dst, w);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_check_ids(hipStream_t s, const uint32_t *a, const uint32_t *b, uint64_t count,
+                 uint32_t n, uint32_t *d_flag, int *bad) {
+  *bad = 0;
+  if (count == 0) return EG_OK;
+  EG_HIP(hipMemsetAsync(d_flag, 0, sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_check_ids, dim3(blocks_for(count, 256)), dim3(256), 0, s, a, b, count,
+                     n, d_flag);
+  EG_LAUNCHED();
+  uint32_t h = 0;
+  EG_HIP(hipMemcpyAsync(&h, d_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  *bad = h != 0;
+  return EG_OK;
+}
+
+int eg_cell_keys(hipStream_t s, const uint32_t *src, const uint32_t *dst, uint64_t count,
+                 uint32_t n, uint32_t P, uint32_t *keys) {
+  if (count == 0) return EG_OK;
+  hipLaunchKernelGGL(k_cell_keys, dim3(blocks_for(count, 256)), dim3(256), 0, s, src, dst,
+                     count, n, P, keys);
+  EG_LAUNCHED();
+  return EG_OK;
+}

@@ -1,0 +1,594 @@
+// eg_pull.hip -- pull PageRank and SpMV over an adjacency orientation.
+//
+// Reference: pr_algo_pull (pagerank_simple.c:72-84), the PR iterator
+// (:110-133) and spmv_algo (spmv.c:98-106).  One iteration computes, for every
+// row v, a sum over its entries; PageRank sums contrib[u] = r[u]/outdeg(u),
+// SpMV sums w * x[u].
+//
+// Work decomposition (built once per orientation, eg_plan_build):
+//  * Rows are grouped by the merged coordinate key(r) = ptr[r] + r into chunks
+//    of kChunk units, so a chunk holds at most kChunk rows and its short rows
+//    at most kChunk + kLongRow entries.  One workgroup streams the chunk's
+//    entries (coalesced index loads, gathered values) into LDS, then rows of
+//    up to kSerialRow entries are summed by one lane each and longer ones by a
+//    whole wave.
+//  * Rows with more than kLongRow entries (R-MAT hubs) are cut into segments
+//    of kSegLen entries, one workgroup each; a finalize kernel merges the
+//    segments of a row in segment order.
+// All sums are FP32 with Neumaier compensation in a fixed order, so results
+// are deterministic.  PageRank's dangling mass and L1 delta are reduced in
+// FP64 on the device (per-chunk partials, then a last-block reduction).
+#include <math.h>
+
+#include "eg_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr uint32_t kChunk = 2048;
+constexpr uint32_t kLongRow = 4096;
+constexpr uint32_t kSegLen = 4096;
+constexpr uint32_t kSerialRow = 32;
+constexpr uint32_t kLdsVals = kChunk + kLongRow;
+constexpr int kFinBlocks = 128;
+
+struct PlanDev {
+  const uint32_t *chunk_row;
+  const uint32_t *long_rows;
+  const uint32_t *seg_base;
+  float2 *seg_part;
+  double2 *chunk_part;
+  double2 *fin_part;
+  uint32_t *fin_counter;
+  uint32_t nchunks, nlong, nseg;
+};
+
+PlanDev plan_dev(const PullPlan &p) {
+  PlanDev d;
+  d.chunk_row = p.chunk_row;
+  d.long_rows = p.long_rows;
+  d.seg_base = p.seg_base;
+  d.seg_part = p.seg_part;
+  d.chunk_part = p.chunk_part;
+  d.fin_part = p.fin_part;
+  d.fin_counter = p.fin_counter;
+  d.nchunks = p.nchunks;
+  d.nlong = p.nlong;
+  d.nseg = p.nseg;
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// Planning kernels.
+// ---------------------------------------------------------------------------
+__global__ void k_chunk_rows(const uint32_t *__restrict__ ptr, uint32_t n, uint32_t nchunks,
+                             uint32_t *__restrict__ chunk_row) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nchunks) return;
+  if (b == nchunks) {
+    chunk_row[b] = n;
+    return;
+  }
+  const uint64_t target = (uint64_t)b * kChunk;
+  uint32_t lo = 0, hi = n;  // first r in [0, n] with ptr[r] + r >= target
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if ((uint64_t)ptr[mid] + mid >= target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  chunk_row[b] = lo;
+}
+
+__global__ void k_long_flags(const uint32_t *__restrict__ ptr, uint32_t n,
+                             uint32_t *__restrict__ flags) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  flags[r] = (ptr[r + 1] - ptr[r]) > kLongRow ? 1u : 0u;
+}
+
+__global__ void k_long_compact(const uint32_t *__restrict__ ptr, uint32_t n,
+                               const uint32_t *__restrict__ pos, uint32_t *__restrict__ long_rows,
+                               uint32_t *__restrict__ segcnt) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t d = ptr[r + 1] - ptr[r];
+  if (d > kLongRow) {
+    long_rows[pos[r]] = r;
+    segcnt[pos[r]] = (d + kSegLen - 1) / kSegLen;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Iteration kernels.
+// ---------------------------------------------------------------------------
+struct PrArgs {
+  const float *rank_old;  // for the L1 delta
+  float *contrib_new;
+  const uint32_t *out_deg;
+  const double *dmass_in;  // dangling mass of the input iterate
+  double *stats_out;       // [dangling of output, l1]
+  float base;              // (1 - d) / n
+  float damping;
+  double damping_over_n;   // d / n, for the uniform dangling term
+  int uniform;
+};
+
+template <bool PR, bool WEIGHTED>
+__device__ __forceinline__ float entry_value(const uint32_t *__restrict__ idx,
+                                             const float *__restrict__ w,
+                                             const float *__restrict__ x, uint32_t k) {
+  const uint32_t u = __builtin_nontemporal_load(idx + k);
+  float v = x[u];
+  if (WEIGHTED) v *= __builtin_nontemporal_load(w + k);
+  return v;
+}
+
+__device__ __forceinline__ float pr_base(const PrArgs &a) {
+  double b = (double)a.base;
+  if (a.uniform) b += a.damping_over_n * (*a.dmass_in);
+  return (float)b;
+}
+
+// Writes the row result; for PageRank also the new contribution and the FP64
+// partials (dangling mass of the new iterate, |r' - r|).
+template <bool PR>
+__device__ __forceinline__ void finish_row(uint32_t r, float sum, float *__restrict__ y,
+                                           const PrArgs &a, float basep, double &dang,
+                                           double &l1) {
+  if (PR) {
+    const float v = basep + a.damping * sum;
+    y[r] = v;
+    const uint32_t d = a.out_deg[r];
+    a.contrib_new[r] = d ? v / (float)d : 0.f;
+    if (d == 0) dang += (double)v;
+    l1 += fabs((double)v - (double)a.rank_old[r]);
+  } else {
+    y[r] = sum;
+  }
+}
+
+template <bool PR, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ ptr,
+                                                 const uint32_t *__restrict__ idx,
+                                                 const float *__restrict__ w,
+                                                 const float *__restrict__ x,
+                                                 float *__restrict__ y, PlanDev pl, PrArgs a) {
+  __shared__ float s_val[kLdsVals];
+  __shared__ uint32_t s_list[kChunk];
+  __shared__ uint32_t s_nlist;
+  __shared__ float s_ks[kWaves], s_kc[kWaves];
+  __shared__ double s_d[kWaves], s_l[kWaves];
+  __shared__ uint32_t s_seg_row;
+
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+
+  if (blockIdx.x < pl.nseg) {
+    // ---- one segment of a long row ----
+    const uint32_t s = blockIdx.x;
+    if (tid == 0) {
+      uint32_t lo = 0, hi = pl.nlong;  // last j with seg_base[j] <= s
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pl.seg_base[mid] <= s)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      s_seg_row = lo;
+    }
+    __syncthreads();
+    const uint32_t j = s_seg_row;
+    const uint32_t r = pl.long_rows[j];
+    const uint32_t k0 = ptr[r] + (s - pl.seg_base[j]) * kSegLen;
+    const uint32_t kend = ptr[r + 1];
+    const uint32_t k1 = k0 + kSegLen < kend ? k0 + kSegLen : kend;
+    EgKahan acc;
+    for (uint32_t k = k0 + tid; k < k1; k += kBlock) acc.add(entry_value<PR, WEIGHTED>(idx, w, x, k));
+    eg_wave_kahan_allreduce(acc);
+    if (lane == 0) {
+      s_ks[wave] = acc.s;
+      s_kc[wave] = acc.c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      EgKahan t;
+      t.s = s_ks[0];
+      t.c = s_kc[0];
+      for (int q = 1; q < kWaves; q++) t.merge(s_ks[q], s_kc[q]);
+      pl.seg_part[s] = make_float2(t.s, t.c);
+    }
+    return;
+  }
+
+  // ---- a chunk of short rows ----
+  const uint32_t c = blockIdx.x - pl.nseg;
+  const uint32_t r0 = pl.chunk_row[c];
+  uint32_t r1 = pl.chunk_row[c + 1];
+  if (r1 > r0 && ptr[r1] - ptr[r1 - 1] > kLongRow) r1--;  // trailing long row
+  const uint32_t k0 = ptr[r0];
+  const uint32_t k1 = ptr[r1];
+  if (tid == 0) s_nlist = 0;
+  for (uint32_t k = k0 + tid; k < k1; k += kBlock)
+    s_val[k - k0] = entry_value<PR, WEIGHTED>(idx, w, x, k);
+  __syncthreads();
+
+  const float basep = PR ? pr_base(a) : 0.f;
+  double dang = 0.0, l1 = 0.0;
+  const uint32_t nrows = r1 - r0;
+  for (uint32_t i = tid; i < nrows; i += kBlock) {
+    const uint32_t r = r0 + i;
+    const uint32_t ra = ptr[r] - k0, rb = ptr[r + 1] - k0;
+    if (rb - ra <= kSerialRow) {
+      EgKahan acc;
+      for (uint32_t q = ra; q < rb; q++) acc.add(s_val[q]);
+      finish_row<PR>(r, acc.value(), y, a, basep, dang, l1);
+    } else {
+      s_list[atomicAdd(&s_nlist, 1u)] = r;
+    }
+  }
+  __syncthreads();
+  const uint32_t nlist = s_nlist;
+  for (uint32_t li = wave; li < nlist; li += kWaves) {
+    const uint32_t r = s_list[li];
+    const uint32_t ra = ptr[r] - k0, rb = ptr[r + 1] - k0;
+    EgKahan acc;
+    for (uint32_t q = ra + lane; q < rb; q += 64) acc.add(s_val[q]);
+    eg_wave_kahan_allreduce(acc);
+    if (lane == 0) finish_row<PR>(r, acc.value(), y, a, basep, dang, l1);
+  }
+  if (PR) {
+    dang = eg_wave_sum_f64(dang);
+    l1 = eg_wave_sum_f64(l1);
+    if (lane == 0) {
+      s_d[wave] = dang;
+      s_l[wave] = l1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double td = 0.0, tl = 0.0;
+      for (int q = 0; q < kWaves; q++) {
+        td += s_d[q];
+        tl += s_l[q];
+      }
+      pl.chunk_part[c] = make_double2(td, tl);
+    }
+  }
+}
+
+// Long rows: merge segment partials in segment order.  PageRank: also reduce
+// the FP64 partials of all chunks and long rows; the last block to finish
+// writes the iteration's statistics and re-arms the counter.
+template <bool PR>
+__global__ __launch_bounds__(kBlock) void k_pull_finish(const uint32_t *__restrict__ ptr,
+                                                        float *__restrict__ y, PlanDev pl,
+                                                        PrArgs a) {
+  __shared__ double s_d[kWaves], s_l[kWaves];
+  __shared__ uint32_t s_last;
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  const uint32_t gtid = blockIdx.x * kBlock + tid;
+  const uint32_t gthreads = gridDim.x * kBlock;
+  const float basep = PR ? pr_base(a) : 0.f;
+  double dang = 0.0, l1 = 0.0;
+  for (uint32_t j = gtid; j < pl.nlong; j += gthreads) {
+    const uint32_t r = pl.long_rows[j];
+    EgKahan acc;
+    for (uint32_t s = pl.seg_base[j]; s < pl.seg_base[j + 1]; s++) {
+      const float2 p = pl.seg_part[s];
+      acc.merge(p.x, p.y);
+    }
+    finish_row<PR>(r, acc.value(), y, a, basep, dang, l1);
+  }
+  if (!PR) return;
+  for (uint32_t c = gtid; c < pl.nchunks; c += gthreads) {
+    const double2 p = pl.chunk_part[c];
+    dang += p.x;
+    l1 += p.y;
+  }
+  dang = eg_wave_sum_f64(dang);
+  l1 = eg_wave_sum_f64(l1);
+  if (lane == 0) {
+    s_d[wave] = dang;
+    s_l[wave] = l1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double td = 0.0, tl = 0.0;
+    for (int q = 0; q < kWaves; q++) {
+      td += s_d[q];
+      tl += s_l[q];
+    }
+    pl.fin_part[blockIdx.x] = make_double2(td, tl);
+    __threadfence();
+    const uint32_t prev = atomicAdd(pl.fin_counter, 1u);
+    s_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last && tid == 0) {
+    __threadfence();
+    double td = 0.0, tl = 0.0;
+    for (uint32_t b = 0; b < gridDim.x; b++) {
+      const double *pp = reinterpret_cast<const double *>(&pl.fin_part[b]);
+      const double2 p = make_double2(__builtin_nontemporal_load(pp), __builtin_nontemporal_load(pp + 1));
+      td += p.x;
+      tl += p.y;
+    }
+    a.stats_out[0] = td;
+    a.stats_out[1] = tl;
+    *pl.fin_counter = 0u;
+  }
+}
+
+__global__ void k_pr_init(uint32_t n, float init, const uint32_t *__restrict__ out_deg,
+                          float *__restrict__ rank, float *__restrict__ contrib,
+                          uint32_t *__restrict__ ndangling) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  bool dangling = false;
+  if (v < n) {
+    const uint32_t d = out_deg[v];
+    rank[v] = init;
+    contrib[v] = d ? init / (float)d : 0.f;
+    dangling = d == 0;
+  }
+  const uint64_t b = __ballot(dangling);
+  if (__lane_id() == 0 && b) atomicAdd(ndangling, (uint32_t)__popcll(b));
+}
+
+inline uint32_t blocks_for(uint64_t count, uint32_t per) {
+  return (uint32_t)((count + per - 1) / per);
+}
+
+}  // namespace
+
+void eg_plan_free(PullPlan *pl) {
+  eg_dfree_t(&pl->chunk_row);
+  eg_dfree_t(&pl->long_rows);
+  eg_dfree_t(&pl->seg_base);
+  eg_dfree_t(&pl->seg_part);
+  eg_dfree_t(&pl->chunk_part);
+  eg_dfree_t(&pl->fin_part);
+  eg_dfree_t(&pl->fin_counter);
+  *pl = PullPlan();
+}
+
+int eg_plan_build(eg_graph *g, PullPlan *pl, const uint32_t *ptr) {
+  if (pl->built) return EG_OK;
+  eg_plan_free(pl);
+  const uint32_t n = g->n;
+  const uint64_t m = g->m;
+  hipStream_t s = g->stream;
+  const uint64_t nchunks = (m + n) / kChunk + 1;
+  if (nchunks >= (1ull << 31)) return eg_fail(EG_ENOTSUP, "graph too large for the pull plan");
+  pl->nchunks = (uint32_t)nchunks;
+  EG_TRY(eg_dalloc_t(&pl->chunk_row, nchunks + 1, "chunk rows"));
+  hipLaunchKernelGGL(k_chunk_rows, dim3(blocks_for(nchunks + 1, 256)), dim3(256), 0, s, ptr, n,
+                     pl->nchunks, pl->chunk_row);
+  EG_LAUNCHED();
+
+  uint32_t *flags = nullptr, *segcnt = nullptr;
+  EG_TRY(eg_dalloc_t(&flags, (uint64_t)n + 1, "long flags"));
+  hipLaunchKernelGGL(k_long_flags, dim3(blocks_for(n, 256)), dim3(256), 0, s, ptr, n, flags);
+  EG_LAUNCHED();
+  void *tmp = nullptr;
+  EG_TRY(eg_scratch(g, eg_scan_tmp_bytes(n), &tmp));
+  EG_TRY(eg_scan_excl(s, flags, flags, n, tmp));
+  uint32_t nlong = 0;
+  EG_HIP(hipMemcpyAsync(&nlong, flags + n, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  pl->nlong = nlong;
+  EG_TRY(eg_dalloc_t(&pl->long_rows, nlong ? nlong : 1, "long rows"));
+  EG_TRY(eg_dalloc_t(&pl->seg_base, (uint64_t)nlong + 1, "segment base"));
+  EG_TRY(eg_dalloc_t(&segcnt, (uint64_t)nlong + 1, "segment counts"));
+  if (nlong) {
+    hipLaunchKernelGGL(k_long_compact, dim3(blocks_for(n, 256)), dim3(256), 0, s, ptr, n, flags,
+                       pl->long_rows, segcnt);
+    EG_LAUNCHED();
+  }
+  EG_TRY(eg_scan_excl(s, segcnt, pl->seg_base, nlong, tmp));
+  uint32_t nseg = 0;
+  EG_HIP(hipMemcpyAsync(&nseg, pl->seg_base + nlong, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  eg_dfree_t(&flags);
+  eg_dfree_t(&segcnt);
+  pl->nseg = nseg;
+  EG_TRY(eg_dalloc_t(&pl->seg_part, nseg ? nseg : 1, "segment partials"));
+  EG_TRY(eg_dalloc_t(&pl->chunk_part, nchunks, "chunk partials"));
+  EG_TRY(eg_dalloc_t(&pl->fin_part, kFinBlocks, "finish partials"));
+  EG_TRY(eg_dalloc_t(&pl->fin_counter, 1, "finish counter"));
+  EG_HIP(hipMemsetAsync(pl->fin_counter, 0, 4, s));
+  EG_HIP(hipStreamSynchronize(s));
+  pl->built = true;
+  return EG_OK;
+}
+
+// Orientation used by the pull kernels: in-edges when the CSC exists, else
+// the CSR of a symmetric graph (-s).
+static int pull_orientation(eg_graph *g, const uint32_t **ptr, const uint32_t **idx,
+                            const float **w, PullPlan **pl) {
+  if (g->in_ptr) {
+    *ptr = g->in_ptr;
+    *idx = g->in_idx;
+    *w = g->in_w;
+    *pl = &g->plan_in;
+  } else if (g->out_ptr) {
+    *ptr = g->out_ptr;
+    *idx = g->out_col;
+    *w = g->out_w;
+    *pl = &g->plan_out;
+  } else {
+    return eg_fail(EG_ENOTSUP, "no adjacency layout");
+  }
+  return eg_plan_build(g, *pl, *ptr);
+}
+
+static int launch_pull(eg_graph *g, bool pr, const uint32_t *ptr, const uint32_t *idx,
+                       const float *w, const float *x, float *y, const PullPlan &pl,
+                       const PrArgs &a) {
+  const PlanDev d = plan_dev(pl);
+  const uint32_t grid = pl.nseg + pl.nchunks;
+  {
+    EgProf prof(g, pr ? "pr_pull" : "spmv_pull");
+    if (pr)
+      hipLaunchKernelGGL((k_pull<true, false>), dim3(grid), dim3(kBlock), 0, g->stream, ptr, idx,
+                         w, x, y, d, a);
+    else if (w)
+      hipLaunchKernelGGL((k_pull<false, true>), dim3(grid), dim3(kBlock), 0, g->stream, ptr, idx,
+                         w, x, y, d, a);
+    else
+      hipLaunchKernelGGL((k_pull<false, false>), dim3(grid), dim3(kBlock), 0, g->stream, ptr, idx,
+                         w, x, y, d, a);
+    EG_LAUNCHED();
+  }
+  if (pr || pl.nlong) {
+    EgProf prof(g, pr ? "pr_finish" : "spmv_finish");
+    if (pr)
+      hipLaunchKernelGGL((k_pull_finish<true>), dim3(kFinBlocks), dim3(kBlock), 0, g->stream, ptr,
+                         y, d, a);
+    else
+      hipLaunchKernelGGL((k_pull_finish<false>), dim3(kFinBlocks), dim3(kBlock), 0, g->stream,
+                         ptr, y, d, a);
+    EG_LAUNCHED();
+  }
+  return EG_OK;
+}
+
+int eg_pagerank_grid(eg_graph *g, const eg_pr_opts *o, float *rank_out, double *dh, double *lh);
+int eg_pagerank_edges(eg_graph *g, const eg_pr_opts *o, float *rank_out, double *dh, double *lh);
+int eg_spmv_grid(eg_graph *g, const float *x, float *y);
+int eg_spmv_edges(eg_graph *g, const float *x, float *y);
+
+static int ensure_pr_state(eg_graph *g, int iters) {
+  const uint32_t n = g->n;
+  if (!g->rank_a) {
+    EG_TRY(eg_dalloc_t(&g->rank_a, n, "rank"));
+    EG_TRY(eg_dalloc_t(&g->rank_b, n, "rank"));
+    EG_TRY(eg_dalloc_t(&g->contrib_a, n, "contrib"));
+    EG_TRY(eg_dalloc_t(&g->contrib_b, n, "contrib"));
+  }
+  if (g->pr_stats_cap < iters + 1) {
+    eg_dfree_t(&g->pr_stats);
+    EG_TRY(eg_dalloc_t(&g->pr_stats, 2 * ((uint64_t)iters + 1), "pr stats"));
+    g->pr_stats_cap = iters + 1;
+  }
+  if (!g->bfs_ctr) EG_TRY(eg_dalloc_t(&g->bfs_ctr, 64, "counters"));
+  return EG_OK;
+}
+
+// Shared by the adjacency, grid and edge-array PageRank drivers: resolves the
+// options into the recurrence's constants.
+void eg_pr_resolve(const eg_graph *g, const eg_pr_opts *o, double *damping, int *iters,
+                   double *init, int *uniform) {
+  *damping = o ? o->damping : 0.85;
+  *iters = o ? o->iterations : 10;
+  *init = (o && o->init > 0.0) ? o->init : 1.0 / (double)g->n;
+  *uniform = o ? o->dangling == EG_PR_DANGLING_UNIFORM : 0;
+}
+
+extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, double *dh,
+                           double *lh) {
+  EG_TRY(eg_require_loaded(g));
+  double damping, init;
+  int iters, uniform;
+  eg_pr_resolve(g, o, &damping, &iters, &init, &uniform);
+  if (iters < 0 || !(damping >= 0.0 && damping <= 1.0))
+    return eg_fail(EG_EINVAL, "bad PageRank options");
+  const int layout = o ? o->layout : EG_LAYOUT_AUTO;
+  const bool adj = g->out_ptr != nullptr;
+  if (layout == EG_LAYOUT_GRID || (layout == EG_LAYOUT_AUTO && g->cell_ptr))
+    return eg_pagerank_grid(g, o, rank_out, dh, lh);
+  if (layout == EG_LAYOUT_EDGES || (layout == EG_LAYOUT_AUTO && g->el_src))
+    return eg_pagerank_edges(g, o, rank_out, dh, lh);
+  if (!adj) return eg_fail(EG_ENOTSUP, "load mode %d has no adjacency lists", g->load_mode);
+
+  const uint32_t *ptr, *idx;
+  const float *w;
+  PullPlan *pl;
+  EG_TRY(pull_orientation(g, &ptr, &idx, &w, &pl));
+  EG_TRY(ensure_pr_state(g, iters));
+  const uint32_t n = g->n;
+  hipStream_t s = g->stream;
+  float *rank = g->rank_a, *rank_next = g->rank_b;
+  float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
+  EG_HIP(hipMemsetAsync(g->bfs_ctr, 0, 4, s));
+  hipLaunchKernelGGL(k_pr_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (float)init,
+                     g->out_deg, rank, contrib, g->bfs_ctr);
+  EG_LAUNCHED();
+  EG_HIP(hipMemcpyAsync(g->h_ctr, g->bfs_ctr, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  // Dangling mass of the initial iterate: every sink holds `init`.
+  const double d0 = (double)g->h_ctr[0] * (double)(float)init;
+  EG_HIP(hipMemcpyAsync(g->pr_stats, &d0, sizeof(double), hipMemcpyHostToDevice, s));
+  EG_HIP(hipStreamSynchronize(s));
+
+  PrArgs a;
+  a.out_deg = g->out_deg;
+  a.base = (float)((1.0 - damping) / (double)n);
+  a.damping = (float)damping;
+  a.damping_over_n = damping / (double)n;
+  a.uniform = uniform;
+  for (int it = 0; it < iters; it++) {
+    a.rank_old = rank;
+    a.contrib_new = contrib_next;
+    a.dmass_in = g->pr_stats + 2 * it;
+    a.stats_out = g->pr_stats + 2 * (it + 1);
+    EG_TRY(launch_pull(g, true, ptr, idx, nullptr, contrib, rank_next, *pl, a));
+    std::swap(rank, rank_next);
+    std::swap(contrib, contrib_next);
+  }
+  if (rank_out) EG_HIP(hipMemcpyAsync(rank_out, rank, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  std::vector<double> st(2 * ((size_t)iters + 1));
+  if ((dh || lh) && iters > 0)
+    EG_HIP(hipMemcpyAsync(st.data(), g->pr_stats, st.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  for (int it = 0; it < iters; it++) {
+    if (dh) dh[it] = st[2 * it];
+    if (lh) lh[it] = st[2 * (it + 1) + 1];
+  }
+  // Keep the final iterate where later device-side users find it.
+  if (rank != g->rank_a) {
+    std::swap(g->rank_a, g->rank_b);
+    std::swap(g->contrib_a, g->contrib_b);
+  }
+  return EG_OK;
+}
+
+extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
+  EG_TRY(eg_require_loaded(g));
+  if (!x) return eg_fail(EG_EINVAL, "null x");
+  if (layout == EG_LAYOUT_GRID || (layout == EG_LAYOUT_AUTO && g->cell_ptr))
+    return eg_spmv_grid(g, x, y);
+  if (layout == EG_LAYOUT_EDGES || (layout == EG_LAYOUT_AUTO && g->el_src))
+    return eg_spmv_edges(g, x, y);
+  if (!g->out_ptr) return eg_fail(EG_ENOTSUP, "load mode %d has no adjacency lists", g->load_mode);
+  const uint32_t *ptr, *idx;
+  const float *w;
+  PullPlan *pl;
+  EG_TRY(pull_orientation(g, &ptr, &idx, &w, &pl));
+  EG_TRY(ensure_pr_state(g, 0));
+  const uint32_t n = g->n;
+  hipStream_t s = g->stream;
+  EG_HIP(hipMemcpyAsync(g->contrib_a, x, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  PrArgs a = {};
+  EG_TRY(launch_pull(g, false, ptr, idx, w, g->contrib_a, g->rank_a, *pl, a));
+  if (y) EG_HIP(hipMemcpyAsync(y, g->rank_a, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+// Device-resident entry used by the batched form and the benchmark: one SpMV
+// pass over x already in HBM.
+int eg_spmv_device(eg_graph *g, const float *x, float *y) {
+  const uint32_t *ptr, *idx;
+  const float *w;
+  PullPlan *pl;
+  EG_TRY(pull_orientation(g, &ptr, &idx, &w, &pl));
+  PrArgs a = {};
+  return launch_pull(g, false, ptr, idx, w, x, y, *pl, a);
+}

@@ -1,0 +1,208 @@
+/*
+ * eg.h -- C ABI of libeg, the MI355X-native implementation of EverythingGraph's
+ * data-parallel path (layout builders, pull PageRank / SpMV, direction-
+ * optimising BFS, min-label WCC) plus its batched and serving forms.
+ *
+ * The reference is a link-time plugin system: each algorithm binary defines
+ * `struct algo_func current_algo = {reset, main, construct, destruct}`
+ * (random.h:64-69) and the driver (random.c:124-143) calls it after `init()`
+ * (init_all.c:791) has built the graph selected by `-m load_mode` into process
+ * globals.  libeg replaces `init()` and the algorithm bodies behind plain C
+ * calls: a graph handle owns the device-resident layout and a HIP stream, and
+ * results come back in caller-owned host buffers in the reference encodings.
+ *
+ * Conventions: every call returns EG_OK (0) or a negative EG_E* code and
+ * eg_last_error() describes the last failure of the calling thread.  A handle
+ * is not thread-safe.  Vertex ids are uint32 and must be < n (EG_ERANGE
+ * otherwise); stored adjacency entries must number < 2^32.
+ */
+#ifndef EG_H
+#define EG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference exits or asserts: utils.h:98-102) ---- */
+#define EG_OK 0
+#define EG_EINVAL (-1)   /* bad argument */
+#define EG_ENOMEM (-2)   /* device or host allocation failed */
+#define EG_EIO (-3)      /* edge file unreadable or misaligned */
+#define EG_EDEVICE (-4)  /* HIP runtime error or no GPU */
+#define EG_ERANGE (-5)   /* a vertex id >= n */
+#define EG_ENOTSUP (-6)  /* layout / algorithm combination not supported */
+#define EG_ESTATE (-7)   /* graph not loaded yet */
+#define EG_EAGAIN (-8)   /* serving: no capacity now */
+#define EG_ETIMEDOUT (-9) /* serving: wait timed out */
+
+/* ---- graph flags: the -u, -s and -w switches of random.c:57-98 ---- */
+#define EG_UNDIRECTED 1u /* -u: append reversed records (init_all.c:102-106) */
+#define EG_SYMMETRIC 2u  /* -s: do not build in-edges (init_all.c:714) */
+#define EG_WEIGHTED 4u   /* -w: 12-byte [src][dst][f32 w] records (init_all.h:8-14) */
+
+/* ---- load modes: the -m numbering of init_all.c:855-888 ---- */
+#define EG_MODE_GRID_SORT 0     /* init_grid_sort(0,0) */
+#define EG_MODE_GRID_NOSORT 1   /* init_grid_nosort */
+#define EG_MODE_ADJ_SORT 2      /* init_adj_sort: CSR (+CSC) */
+#define EG_MODE_ADJ_SORT_FULL 3 /* same as 2 (init_all.c:864-869) */
+#define EG_MODE_ADJ_COUNT 4     /* init_adj_count: count-sort CSR (+CSC) */
+#define EG_MODE_ADJ_DYNAMIC 5   /* init_adj_dynamic: frees what it builds; EG_ENOTSUP */
+#define EG_MODE_GRID_SORT_SRC 6 /* init_grid_sort(full,1) */
+#define EG_MODE_EDGES_SRC 7     /* init_edgelist(.,0,1): sorted by src */
+#define EG_MODE_EDGES_SRC_DST 8 /* init_edgelist(.,1,1): by src then dst */
+#define EG_MODE_EDGES_RAW 9     /* init_edgelist(.,0,0): input order */
+
+typedef struct eg_graph eg_graph;
+
+typedef struct {
+  char name[64];
+  int32_t compute_units;
+  int32_t gcn_arch; /* e.g. 950 */
+  uint64_t hbm_bytes;
+  uint64_t free_bytes;
+  double peak_hbm_gbs; /* published peak used for roofline fractions */
+} eg_device_info;
+
+typedef struct {
+  uint32_t n;        /* vertices (NB_NODES, -n) */
+  uint32_t P;        /* grid partitions per side */
+  uint32_t flags;    /* EG_UNDIRECTED | EG_SYMMETRIC | EG_WEIGHTED */
+  int32_t load_mode; /* -m */
+  uint64_t m_in;     /* input records (nb_edges after init_all.c:821) */
+  uint64_t m;        /* stored entries: 2*m_in with EG_UNDIRECTED */
+  int32_t has_csr, has_csc, has_grid, has_edgelist;
+  double build_ms;   /* device time of the last layout build */
+  uint64_t device_bytes;
+} eg_graph_info;
+
+/* Synthetic input generated on the device (no reference counterpart; the
+ * reference only reads files).  kind 0 = R-MAT(scale, a, b, c), 1 = uniform
+ * Erdos-Renyi pairs over [0, n). Identical to the oracle's generator. */
+typedef struct {
+  int32_t kind;
+  int32_t scale;      /* R-MAT: n = 2^scale */
+  uint64_t m_in;      /* records to generate */
+  uint64_t seed;      /* graph seed */
+  double a, b, c;     /* R-MAT quadrant probabilities */
+  uint64_t weight_seed; /* used when the graph is EG_WEIGHTED */
+} eg_gen_params;
+
+/* PageRank options.  The reference hard-codes d = 0.85 (pagerank_simple.c:4),
+ * 10 iterations (:115) and prev = 0.15 (:95). */
+#define EG_PR_DANGLING_DROP 0    /* reference recurrence: mass of sinks dropped */
+#define EG_PR_DANGLING_UNIFORM 1 /* add d * dangling / n to every vertex */
+#define EG_LAYOUT_AUTO 0
+#define EG_LAYOUT_ADJ 1   /* pull over in-edges (CSC, or CSR when symmetric) */
+#define EG_LAYOUT_GRID 2  /* 2-D grid cells */
+#define EG_LAYOUT_EDGES 3 /* edge array */
+typedef struct {
+  double damping;     /* 0.85 */
+  int32_t iterations; /* 10 in the reference */
+  int32_t dangling;   /* EG_PR_DANGLING_* */
+  double init;        /* initial rank of every vertex (0 -> 1/n) */
+  int32_t layout;     /* EG_LAYOUT_* */
+} eg_pr_opts;
+
+typedef struct {
+  int32_t levels;          /* number of BFS levels (frontiers) */
+  int32_t pull_levels;     /* levels run bottom-up */
+  uint64_t reached;        /* vertices with dist != 0 */
+  uint64_t edges_traversed; /* input records inside the traversed component */
+  double device_ms;        /* device time of the traversal */
+} eg_bfs_stats;
+
+/* ---- library and device ---- */
+const char *eg_last_error(void);
+int eg_device_query(eg_device_info *out);
+const char *eg_version(void);
+
+/* ---- graph lifecycle (replaces init(), init_all.c:791-893) ---- */
+/* P = 0 selects the reference formula (random.c:69-71). */
+int eg_graph_create(uint32_t n, uint32_t flags, int32_t load_mode, uint32_t P,
+                    eg_graph **out);
+/* init_all.c:812-850: read the binary edge file (8- or 12-byte records; a
+ * trailing partial record is dropped as st_size/item_size does) and build the
+ * layout of the handle's load mode on the device. */
+int eg_graph_load_file(eg_graph *g, const char *path);
+/* Same from raw record bytes already in host memory. */
+int eg_graph_load_records(eg_graph *g, const void *records, uint64_t nbytes);
+/* Same from separate host arrays (w may be NULL). */
+int eg_graph_load_edges(eg_graph *g, uint64_t m_in, const uint32_t *src,
+                        const uint32_t *dst, const float *w);
+/* Same from synthetic records generated in device memory. */
+int eg_graph_generate(eg_graph *g, const eg_gen_params *p);
+/* Generates p->m_in records on the device and copies them to host arrays
+ * (w may be NULL): the input of an edge file, for tools that write one. */
+int eg_generate_edges(const eg_gen_params *p, uint32_t n, uint32_t *src, uint32_t *dst,
+                      float *w);
+int eg_graph_free(eg_graph *g);
+int eg_graph_get_info(const eg_graph *g, eg_graph_info *out);
+
+/* ---- layout downloads (parity checks, drop-in globals) ---- */
+/* Out-degree (nb_out_edges) and in-degree (nb_in_edges) of random.h:87-93. */
+int eg_graph_get_degrees(eg_graph *g, uint32_t *out_deg, uint32_t *in_deg);
+/* CSR: row_ptr[n+1] (outgoing_edges offsets), col[m] (edge_array_out[].dst),
+ * w[m] or NULL.  init_all.c:629-697. */
+int eg_graph_get_csr(eg_graph *g, uint64_t *row_ptr, uint32_t *col, float *w);
+/* CSC: col_ptr[n+1], row[m] (edge_array_in[].dst holds the src), w or NULL.
+ * init_all.c:714-767. */
+int eg_graph_get_csc(eg_graph *g, uint64_t *col_ptr, uint32_t *row, float *w);
+/* Grid: cell_ptr[P*P+1] (row-major cells (part(src), part(dst))), and the
+ * cell-ordered records.  init_all.c:252-424. */
+int eg_graph_get_grid(eg_graph *g, uint64_t *cell_ptr, uint32_t *src,
+                      uint32_t *dst, float *w);
+/* Edge array of modes 7/8/9, init_all.c:573-628. */
+int eg_graph_get_edgelist(eg_graph *g, uint32_t *src, uint32_t *dst, float *w);
+
+/* ---- algorithms (replace current_algo.main of each plugin) ---- */
+/* pagerank_simple.c:72-133 / prgrid_cilk.c:62-141.  FP32 storage, compensated
+ * sums, FP64 dangling-mass and L1-delta reductions on the device.  rank may be
+ * NULL (result stays on the device).  dangling_hist / l1_hist (length
+ * iterations, may be NULL) receive per iteration the dangling mass of the
+ * input iterate and sum |r' - r|. */
+int eg_pagerank(eg_graph *g, const eg_pr_opts *opts, float *rank,
+                double *dangling_hist, double *l1_hist);
+/* spmv.c:98-136 with FP32 weights: y[v] = sum_{u->v} w(u,v) x[u] (pull).
+ * Unweighted graphs use w = 1.  layout as eg_pr_opts.layout. */
+int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y);
+/* bfs_simple.c:136-281 (CSR modes), bfsgrid_cilk.c:121-204 (grid modes),
+ * bfs_edgecentric (edge modes).  start_pull = -a 1, switch_mode = !-r.
+ * dist: root = 1, level L -> L+1, unreached 0.  parent (may be NULL):
+ * parent[root] = root, unreached = n (bfs_numa.c:79-83).  dist may be NULL. */
+int eg_bfs(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
+           uint32_t *dist, uint32_t *parent, eg_bfs_stats *stats);
+/* wcc.c:186-286: labels[v] = fixpoint of min-label propagation (int64, as
+ * `long components[]`).  rounds may be NULL. */
+int eg_wcc(eg_graph *g, int64_t *labels, int32_t *rounds);
+
+/* ---- batched form (no reference counterpart) ---- */
+/* k BFS traversals from roots[k] in one multi-source sweep.  dist (may be
+ * NULL) is k x n, source-major, in the eg_bfs encoding. */
+int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots,
+                 uint32_t *dist, double *device_ms);
+/* k personalized PageRank vectors as one FP32 SpMM over the in-edges.
+ * Query q's seeds are seeds[seed_ptr[q] .. seed_ptr[q+1]); p = uniform over
+ * its distinct seeds, x' = alpha p + (1-alpha) A^T D^-1 x.  out (may be NULL)
+ * is k x n, query-major. */
+int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
+                 const uint32_t *seeds, double alpha, int32_t iterations,
+                 float *out, double *device_ms);
+
+/* ---- device timing of named kernels on the graph's stream ---- */
+typedef struct {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+} eg_kernel_time;
+int eg_prof_enable(eg_graph *g, int32_t on);
+int eg_prof_reset(eg_graph *g);
+int eg_prof_read(eg_graph *g, eg_kernel_time *out, int32_t max, int32_t *count);
+int eg_sync(eg_graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

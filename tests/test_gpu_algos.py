@@ -1,0 +1,311 @@
+"""Algorithms on the device vs the oracle.
+
+Integer results (BFS levels, WCC labels) are bit-exact; BFS parents are
+checked as a legal BFS tree.  PageRank, SpMV and PPR run in FP32 with
+compensated sums and are compared with the FP64 oracle:
+  * PageRank: L1 error <= 1e-6 and max error <= 1e-6 relative to max rank,
+    for ranks initialised to 1/n (sum ~ 1);
+  * SpMV: |y - y64| <= 1e-5 * (|A| |x|) elementwise;
+  * PPR: L1 error <= 1e-6 per query vector (each sums to <= 1).
+"""
+import numpy as np
+import pytest
+
+import eg_oracle_py as O
+import everythinggraph_amd as eg
+
+pytestmark = pytest.mark.gpu
+
+PR_L1_TOL = 1e-6
+
+
+def rmat(scale, ef=16, seed=1):
+    return O.gen_rmat(scale, ef << scale, seed=seed)
+
+
+def oracle_adj(n, s, d, w=None, undirected=False):
+    if undirected:
+        s, d, w = O.symmetrize(s, d, w)
+    ptr, col, ow = O.build_csr(n, s, d, w)
+    cp, row, iw = O.build_csc(n, ptr, col, ow)
+    return s, d, w, ptr, col, ow, cp, row, iw
+
+
+# ---------------------------------------------------------------------------
+# PageRank
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("scale,undirected", [(10, False), (12, True), (16, False)])
+@pytest.mark.parametrize("dangling", ["drop", "uniform"])
+def test_pagerank_adjacency(scale, undirected, dangling):
+    n = 1 << scale
+    s0, d0 = rmat(scale)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=undirected)
+    deg = np.diff(ptr).astype(np.uint32)
+    ref, rdh, rlh = O.pagerank_pull(n, cp, row, deg, 0.85, 20, dangling_uniform=dangling == "uniform")
+    with eg.Graph(n, 2, undirected=undirected) as g:
+        g.load_edges(s0, d0)
+        r, dh, lh = g.pagerank(0.85, 20, dangling=dangling)
+        r2, _, _ = g.pagerank(0.85, 20, dangling=dangling)
+    err = np.abs(r.astype(np.float64) - ref)
+    assert err.sum() <= PR_L1_TOL, err.sum()
+    assert err.max() <= 1e-6 * ref.max()
+    assert np.array_equal(r, r2)  # deterministic
+    np.testing.assert_allclose(dh, rdh, rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(lh, rlh, rtol=1e-3, atol=1e-7)
+
+
+def test_pagerank_config_c1():
+    """pagerank_simple on RMAT-16 EF16 directed, FP64 oracle, r0 = 1/n, 20 it."""
+    scale = 16
+    n = 1 << scale
+    s0, d0 = rmat(scale)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0)
+    ref, _, _ = O.pagerank_pull(n, cp, row, np.diff(ptr).astype(np.uint32), 0.85, 20)
+    with eg.Graph(n, 2) as g:
+        g.generate(eg.rmat_params(scale, 16, seed=1))
+        r, _, _ = g.pagerank(0.85, 20)
+    err = np.abs(r.astype(np.float64) - ref)
+    assert err.sum() <= PR_L1_TOL and err.max() <= 1e-6 * ref.max()
+
+
+def test_pagerank_reference_defaults():
+    """The drop-in defaults: prev = 0.15, 10 iterations (pagerank_simple.c:95,115)."""
+    n = 1 << 10
+    s0, d0 = rmat(10)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0)
+    ref, _, _ = O.pagerank_pull(n, cp, row, np.diff(ptr).astype(np.uint32), 0.85, 10, init=0.15)
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s0, d0)
+        r, _, _ = g.pagerank(0.85, 10, init=0.15)
+    np.testing.assert_allclose(r, ref, rtol=2e-6, atol=1e-9)
+
+
+def test_pagerank_symmetric_uses_csr():
+    n = 1 << 11
+    s0, d0 = rmat(11)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    ref, _, _ = O.pagerank_pull(n, cp, row, np.diff(ptr).astype(np.uint32), 0.85, 20)
+    with eg.Graph(n, 2, undirected=True, symmetric=True) as g:
+        g.load_edges(s0, d0)
+        r, _, _ = g.pagerank(0.85, 20)
+    assert np.abs(r - ref).sum() <= PR_L1_TOL
+
+
+@pytest.mark.parametrize("mode,Pp", [(0, 16), (1, 4), (6, 25)])
+def test_pagerank_grid(mode, Pp):
+    n = 1 << 12
+    s0, d0 = rmat(12)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0)
+    ref, _, rlh = O.pagerank_pull(n, cp, row, np.diff(ptr).astype(np.uint32), 0.85, 20)
+    with eg.Graph(n, mode, P=Pp) as g:
+        g.load_edges(s0, d0)
+        r, _, lh = g.pagerank(0.85, 20)
+    assert np.abs(r - ref).sum() <= PR_L1_TOL
+    np.testing.assert_allclose(lh, rlh, rtol=1e-3, atol=1e-7)
+
+
+@pytest.mark.parametrize("mode", [7, 8, 9])
+def test_pagerank_edges(mode):
+    n = 1 << 12
+    s0, d0 = rmat(12)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0)
+    ref, _, _ = O.pagerank_pull(n, cp, row, np.diff(ptr).astype(np.uint32), 0.85, 20)
+    with eg.Graph(n, mode) as g:
+        g.load_edges(s0, d0)
+        r, _, _ = g.pagerank(0.85, 20)
+    assert np.abs(r - ref).sum() <= 2 * PR_L1_TOL
+
+
+def test_pagerank_closed_form_cycle():
+    n = 1000
+    s = np.arange(n, dtype=np.uint32)
+    d = ((s + 1) % n).astype(np.uint32)
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s, d)
+        r, _, lh = g.pagerank(0.85, 10)
+    np.testing.assert_allclose(r, 1.0 / n, rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# SpMV
+# ---------------------------------------------------------------------------
+def spmv_check(y, ref, absref):
+    assert np.all(np.abs(y.astype(np.float64) - ref) <= 1e-5 * absref + 1e-30)
+
+
+@pytest.mark.parametrize("mode,Pp", [(2, 0), (0, 16), (6, 7), (1, 3), (8, 0), (9, 0)])
+def test_spmv_weighted(mode, Pp):
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=4)
+    w = O.gen_weights(s0.size, seed=2)
+    _, _, _, ptr, col, ow, cp, row, iw = oracle_adj(n, s0, d0, w)
+    x = np.random.default_rng(1).uniform(-1, 1, n).astype(np.float32)
+    ref = O.spmv_pull(n, cp, row, iw, x.astype(np.float64))
+    absref = O.spmv_pull(n, cp, row, iw, np.abs(x).astype(np.float64))
+    with eg.Graph(n, mode, weighted=True, P=Pp) as g:
+        g.load_edges(s0, d0, w)
+        y = g.spmv(x)
+    spmv_check(y, ref, absref)
+
+
+def test_spmv_unweighted_and_hub():
+    n = 3000
+    rng = np.random.default_rng(5)
+    s = rng.integers(0, n, 200000).astype(np.uint32)
+    d = rng.integers(0, n, 200000).astype(np.uint32)
+    d[:90000] = 7  # a long in-row split across segments
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s, d)
+    x = rng.uniform(0, 1, n).astype(np.float32)
+    ref = O.spmv_pull(n, cp, row, None, x.astype(np.float64))
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s, d)
+        y = g.spmv(x)
+    spmv_check(y, ref, ref)
+
+
+# ---------------------------------------------------------------------------
+# BFS
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("scale,undirected", [(10, False), (12, True), (14, True), (14, False)])
+def test_bfs_adjacency(scale, undirected):
+    n = 1 << scale
+    s0, d0 = rmat(scale, seed=2)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=undirected)
+    deg = np.diff(ptr)
+    roots = eg.sample_roots(deg, 6, seed=2)
+    with eg.Graph(n, 2, undirected=undirected) as g:
+        g.load_edges(s0, d0)
+        for root in roots:
+            ref, lv, _ = O.bfs(n, s.size, ptr, col, cp, row, int(root))
+            for start_pull in (False, True):
+                for switch in (False, True):
+                    dist, par, st = g.bfs(int(root), start_pull, switch)
+                    assert np.array_equal(dist, ref)
+                    assert st.levels == lv
+                    assert O.check_bfs_parents(n, ptr, col, int(root), dist, par) == 0
+            reached = ref != 0
+            assert st.reached == reached.sum()
+            esum = int(deg[reached].sum())
+            assert st.edges_traversed == (esum // 2 if undirected else esum)
+
+
+def test_bfs_pull_path_is_taken():
+    n = 1 << 14
+    s0, d0 = rmat(14, seed=3)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    root = int(eg.sample_roots(np.diff(ptr), 1, seed=9)[0])
+    ref, _, _ = O.bfs(n, s.size, ptr, col, cp, row, root)
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        dist, par, st = g.bfs(root, False, True)
+    assert st.pull_levels > 0 and np.array_equal(dist, ref)
+    assert O.check_bfs_parents(n, ptr, col, root, dist, par) == 0
+
+
+def test_bfs_symmetric_and_zero_degree():
+    # directed graph whose frontier holds sinks; also -s (no CSC) on -u input
+    n = 2000
+    s = np.array([0, 0, 0, 1, 2, 5], np.uint32)
+    d = np.array([1, 2, 3, 4, 5, 6], np.uint32)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s, d)
+    ref, lv, _ = O.bfs(n, s.size, ptr, col, cp, row, 0)
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s, d)
+        for sp_ in (False, True):
+            dist, par, st = g.bfs(0, sp_, True)
+            assert np.array_equal(dist, ref) and st.levels == lv
+    s0, d0 = rmat(12, seed=5)
+    us, ud, _, ptr, col, _, cp, row, _ = oracle_adj(1 << 12, s0, d0, undirected=True)
+    with eg.Graph(1 << 12, 2, undirected=True, symmetric=True) as g:
+        g.load_edges(s0, d0)
+        root = int(us[0])
+        ref, _, _ = O.bfs(1 << 12, us.size, ptr, col, ptr, col, root)
+        dist, par, _ = g.bfs(root, True, True)
+        assert np.array_equal(dist, ref)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 6, 7, 8, 9])
+def test_bfs_grid_and_edges(mode):
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=6)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    roots = eg.sample_roots(np.diff(ptr), 3, seed=1)
+    with eg.Graph(n, mode, undirected=True, P=16) as g:
+        g.load_edges(s0, d0)
+        for root in roots:
+            ref, lv, _ = O.bfs(n, s.size, ptr, col, cp, row, int(root))
+            dist, par, st = g.bfs(int(root))
+            assert np.array_equal(dist, ref) and st.levels == lv
+            assert O.check_bfs_parents(n, ptr, col, int(root), dist, par) == 0
+
+
+# ---------------------------------------------------------------------------
+# WCC
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("undirected", [False, True])
+@pytest.mark.parametrize("scale", [10, 14])
+def test_wcc_adjacency(undirected, scale):
+    n = 1 << scale
+    s0, d0 = rmat(scale, ef=2, seed=7)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=undirected)
+    ref, _ = O.wcc_push(n, ptr, col)
+    with eg.Graph(n, 4, undirected=undirected) as g:
+        g.load_edges(s0, d0)
+        lab, rounds = g.wcc()
+    assert np.array_equal(lab, ref)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 6, 7, 8, 9])
+def test_wcc_grid_and_edges(mode):
+    n = 1 << 12
+    s0, d0 = rmat(12, ef=1, seed=8)
+    ref, _ = O.wcc_edges(n, s0, d0)
+    with eg.Graph(n, mode, P=9) as g:
+        g.load_edges(s0, d0)
+        lab, _ = g.wcc()
+    assert np.array_equal(lab, ref)
+
+
+# ---------------------------------------------------------------------------
+# batched form
+# ---------------------------------------------------------------------------
+def test_bfs_batch_equals_single():
+    n = 1 << 13
+    s0, d0 = rmat(13, seed=11)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    roots = eg.sample_roots(np.diff(ptr), 70, seed=2)  # two groups (64 + 6)
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        out, ms = g.bfs_batch(roots)
+        for i, r in enumerate(roots):
+            ref, _, _ = O.bfs(n, s.size, ptr, col, cp, row, int(r))
+            assert np.array_equal(out[i], ref), i
+
+
+def test_bfs_batch_directed_duplicates():
+    n = 1 << 11
+    s0, d0 = rmat(11, seed=12)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0)
+    roots = np.array([0, 0, 5, int(s0[3])], np.uint32)
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s0, d0)
+        out, _ = g.bfs_batch(roots)
+    for i, r in enumerate(roots):
+        ref, _, _ = O.bfs(n, s0.size, ptr, col, cp, row, int(r))
+        assert np.array_equal(out[i], ref)
+
+
+def test_ppr_batch_vs_oracle():
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=13)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    deg = np.diff(ptr).astype(np.uint32)
+    rng = np.random.default_rng(3)
+    seeds = [rng.choice(n, size=rng.integers(1, 9), replace=True).astype(np.uint32)
+             for _ in range(66)]
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        out, ms = g.ppr_batch(seeds, 0.15, 30)
+    for q, sd in enumerate(seeds):
+        ref = O.ppr(n, cp, row, deg, sd, 0.15, 30)
+        assert np.abs(out[q] - ref).sum() <= PR_L1_TOL, q

@@ -1,0 +1,379 @@
+"""Pins the CPU oracle (oracle/eg_oracle.c) against independent routines and
+closed forms, so that the oracle can serve as the checker of the device path.
+
+Each oracle function is checked by something that does not share its code:
+numpy stable argsort / lexsort for the layout builders, scipy.sparse products
+for PageRank / SpMV / PPR, scipy.sparse.csgraph shortest paths and connected
+components for BFS / WCC, brute-force transitive closure for the directed
+min-label fixpoint, and closed forms (cycles, complete graphs, stars, paths).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+
+import eg_oracle_py as O
+
+
+# ---------------------------------------------------------------------------
+# small graph zoo
+# ---------------------------------------------------------------------------
+def rand_graph(n, m, seed, hub=None, loops=True):
+    rng = np.random.default_rng(seed)
+    s = rng.integers(0, n, m, dtype=np.uint32)
+    d = rng.integers(0, n, m, dtype=np.uint32)
+    if hub is not None:
+        k = m // 3
+        s[:k] = hub
+    if not loops:
+        keep = s != d
+        s, d = s[keep], d[keep]
+    return s.astype(np.uint32), d.astype(np.uint32)
+
+
+def zoo():
+    out = []
+    out.append(("empty", 5, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
+    out.append(("path", 6, np.arange(5, dtype=np.uint32), np.arange(1, 6, dtype=np.uint32)))
+    out.append(("self_dup", 4, np.array([0, 0, 1, 1, 3, 3], np.uint32),
+                np.array([0, 1, 2, 2, 3, 0], np.uint32)))
+    out.append(("isolated_tail", 50, np.array([3, 7, 3, 9], np.uint32),
+                np.array([7, 3, 9, 3], np.uint32)))
+    s, d = rand_graph(300, 5000, 1, hub=17)
+    out.append(("hub", 300, s, d))
+    s, d = O.gen_rmat(10, 16 * 1024, seed=1)
+    out.append(("rmat10", 1024, s, d))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# generator (no reference counterpart): distribution and determinism
+# ---------------------------------------------------------------------------
+def test_rmat_quadrant_frequencies():
+    scale, m = 12, 200000
+    s, d = O.gen_rmat(scale, m, seed=1)
+    assert s.max() < (1 << scale) and d.max() < (1 << scale)
+    # every level is an independent quadrant draw: top bits follow (a,b,c,d)
+    top_s = (s >> (scale - 1)) & 1
+    top_d = (d >> (scale - 1)) & 1
+    freq = np.bincount(top_s * 2 + top_d, minlength=4) / m
+    np.testing.assert_allclose(freq, [0.57, 0.19, 0.19, 0.05], atol=0.005)
+    low_s, low_d = s & 1, d & 1
+    freq = np.bincount(low_s * 2 + low_d, minlength=4) / m
+    np.testing.assert_allclose(freq, [0.57, 0.19, 0.19, 0.05], atol=0.005)
+
+
+def test_generator_determinism_and_seeds():
+    a = O.gen_rmat(10, 5000, seed=7)
+    b = O.gen_rmat(10, 5000, seed=7)
+    c = O.gen_rmat(10, 5000, seed=8)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert not np.array_equal(a[0], c[0])
+    # a prefix of a longer stream is the shorter stream (counter-based)
+    p = O.gen_rmat(10, 1000, seed=7)
+    assert np.array_equal(p[0], a[0][:1000]) and np.array_equal(p[1], a[1][:1000])
+
+
+def test_er_uniform_and_weights():
+    n, m = 1000, 400000
+    s, d = O.gen_er(n, m, seed=3)
+    assert s.max() < n and d.max() < n
+    hist = np.bincount(s, minlength=n)
+    assert abs(hist.mean() - m / n) < 1e-9 and hist.std() < 3 * np.sqrt(m / n)
+    w = O.gen_weights(m, seed=3)
+    assert w.min() > 0 and w.max() <= 1.0
+    k = w.astype(np.float64) * 2.0**24
+    assert np.array_equal(k, np.round(k))  # exact multiples of 2^-24
+    assert abs(w.mean() - 0.5) < 0.01
+
+
+def test_symmetrize_order():
+    s = np.array([0, 1, 2], np.uint32)
+    d = np.array([1, 2, 0], np.uint32)
+    w = np.array([0.5, 0.25, 1.0], np.float32)
+    os_, od, ow = O.symmetrize(s, d, w)
+    assert list(os_) == [0, 1, 2, 1, 2, 0]
+    assert list(od) == [1, 2, 0, 0, 1, 2]
+    assert list(ow) == [0.5, 0.25, 1.0, 0.5, 0.25, 1.0]
+
+
+# ---------------------------------------------------------------------------
+# partition geometry (init_all.c:29-60)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,Pp", [(10, 3), (16, 4), (7, 7), (5, 8), (1000, 16), (65536, 25),
+                                  (1 << 22, 1638)])
+def test_partition_geometry(n, Pp):
+    vs = np.unique(np.linspace(0, n - 1, min(n, 20000)).astype(np.int64))
+    ids = np.array([O.partition_id(n, Pp, int(v)) for v in vs])
+    assert np.all(np.diff(ids) >= 0) and ids[0] == 0 and ids[-1] <= Pp - 1
+    if n <= 20000:
+        sizes = np.bincount(ids, minlength=Pp)
+        if n >= Pp:
+            assert sizes.max() - sizes.min() <= 1 and sizes.sum() == n
+            # the first n % P partitions hold the larger size
+            big = n % Pp
+            assert np.all(sizes[:big] == n // Pp + 1) and np.all(sizes[big:] == n // Pp)
+
+
+def test_default_P():
+    # random.c:69-71
+    assert O.default_P(1 << 16) == 25
+    assert O.default_P(1 << 22) == 1638
+    assert O.default_P(1 << 24) == 256
+    assert O.default_P(100) == 4
+    assert O.default_P(1 << 30) == 256  # widened arithmetic (R13)
+
+
+# ---------------------------------------------------------------------------
+# builders vs numpy stable sorts
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name,n,s,d", zoo(), ids=[z[0] for z in zoo()])
+def test_csr_csc_vs_numpy(name, n, s, d):
+    w = (np.arange(s.size, dtype=np.float32) + 1) / 4096
+    ptr, col, ow = O.build_csr(n, s, d, w)
+    perm = np.argsort(s, kind="stable")
+    assert np.array_equal(col, d[perm]) and np.array_equal(ow, w[perm])
+    ref_ptr = np.concatenate([[0], np.cumsum(np.bincount(s, minlength=n))])
+    assert np.array_equal(ptr, ref_ptr)
+    cp, row, iw = O.build_csc(n, ptr, col, ow)
+    rowof = np.repeat(np.arange(n, dtype=np.uint32), np.diff(ptr).astype(np.int64))
+    perm2 = np.argsort(col, kind="stable")
+    assert np.array_equal(row, rowof[perm2]) and np.array_equal(iw, ow[perm2])
+    assert np.array_equal(cp, np.concatenate([[0], np.cumsum(np.bincount(d, minlength=n))]))
+
+
+@pytest.mark.parametrize("name,n,s,d", zoo(), ids=[z[0] for z in zoo()])
+@pytest.mark.parametrize("Pp", [1, 3, 16])
+def test_grid_vs_numpy(name, n, s, d, Pp):
+    w = (np.arange(s.size, dtype=np.float32) + 1) / 4096
+    part = np.array([O.partition_id(n, Pp, v) for v in range(n)], np.int64)
+    for presort in (True, False):
+        cp, gs, gd, gw = O.build_grid(n, Pp, s, d, w, presort=presort)
+        order = np.argsort(s, kind="stable") if presort else np.arange(s.size)
+        cell = part[s[order]] * Pp + part[d[order]]
+        perm = order[np.argsort(cell, kind="stable")]
+        assert np.array_equal(gs, s[perm]) and np.array_equal(gd, d[perm])
+        assert np.array_equal(gw, w[perm])
+        cnt = np.bincount(part[s] * Pp + part[d], minlength=Pp * Pp)
+        assert np.array_equal(cp, np.concatenate([[0], np.cumsum(cnt)]))
+
+
+@pytest.mark.parametrize("name,n,s,d", zoo(), ids=[z[0] for z in zoo()])
+def test_edgelist_vs_numpy(name, n, s, d):
+    w = (np.arange(s.size, dtype=np.float32) + 1) / 4096
+    for mode in (7, 8, 9):
+        os_, od, ow = O.build_edgelist(n, s, d, w, mode)
+        if mode == 9:
+            perm = np.arange(s.size)
+        elif mode == 7:
+            perm = np.argsort(s, kind="stable")
+        else:
+            perm = np.lexsort((np.arange(s.size), d, s))
+        assert np.array_equal(os_, s[perm]) and np.array_equal(od, d[perm])
+        assert np.array_equal(ow, w[perm])
+
+
+# ---------------------------------------------------------------------------
+# PageRank / SpMV / PPR vs scipy
+# ---------------------------------------------------------------------------
+def scipy_pr(n, s, d, damping, iters, init, uniform=False):
+    A = sp.csr_matrix((np.ones(s.size), (s.astype(np.int64), d.astype(np.int64))), shape=(n, n))
+    deg = np.asarray(A.sum(axis=1)).ravel()
+    r = np.full(n, init)
+    for _ in range(iters):
+        contrib = np.where(deg > 0, r / np.maximum(deg, 1), 0.0)
+        dang = r[deg == 0].sum()
+        r = (1 - damping) / n + damping * (A.T @ contrib) + (damping * dang / n if uniform else 0)
+    return r
+
+
+@pytest.mark.parametrize("name,n,s,d", zoo()[1:], ids=[z[0] for z in zoo()[1:]])
+@pytest.mark.parametrize("uniform", [False, True])
+def test_pagerank_vs_scipy(name, n, s, d, uniform):
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    deg = O.out_degree(n, s)
+    r, dh, lh = O.pagerank_pull(n, cp, row, deg, 0.85, 20, dangling_uniform=uniform)
+    ref = scipy_pr(n, s, d, 0.85, 20, 1.0 / n, uniform)
+    np.testing.assert_allclose(r, ref, rtol=1e-12, atol=1e-15)
+    assert lh[-1] < lh[0] or lh[0] == 0
+    # dangling mass of the input iterate of iteration 0 is init * #sinks
+    assert dh[0] == pytest.approx((deg == 0).sum() / n, rel=1e-12)
+    if uniform and name != "empty":
+        assert r.sum() == pytest.approx(1.0, rel=1e-12)  # mass is conserved
+    Pp = 3
+    gcp, gs, gd, _ = O.build_grid(n, Pp, s, d)
+    rg = O.pagerank_grid(n, Pp, gcp, gs, gd, deg, 0.85, 20)
+    if not uniform:
+        np.testing.assert_allclose(rg, r, rtol=1e-12, atol=1e-15)
+
+
+def test_pagerank_closed_forms():
+    # directed cycle: every vertex keeps 1/n
+    n = 7
+    s = np.arange(n, dtype=np.uint32)
+    d = ((s + 1) % n).astype(np.uint32)
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    r, _, lh = O.pagerank_pull(n, cp, row, O.out_degree(n, s), 0.85, 10)
+    np.testing.assert_allclose(r, 1.0 / n, rtol=1e-14)
+    assert np.all(lh < 1e-15)
+    # complete digraph without loops: stays uniform
+    n = 5
+    s, d = np.meshgrid(np.arange(n), np.arange(n))
+    keep = s != d
+    s, d = s[keep].astype(np.uint32), d[keep].astype(np.uint32)
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    r, _, _ = O.pagerank_pull(n, cp, row, O.out_degree(n, s), 0.85, 10)
+    np.testing.assert_allclose(r, 1.0 / n, rtol=1e-14)
+    # star into a sink (leaves -> 0): one iteration from r0 = 1/n
+    n = 6
+    s = np.arange(1, n, dtype=np.uint32)
+    d = np.zeros(n - 1, np.uint32)
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    r, dh, _ = O.pagerank_pull(n, cp, row, O.out_degree(n, s), 0.85, 1)
+    assert r[0] == pytest.approx(0.15 / n + 0.85 * (n - 1) / n, rel=1e-14)
+    np.testing.assert_allclose(r[1:], 0.15 / n, rtol=1e-14)
+    assert dh[0] == pytest.approx(1.0 / n, rel=1e-14)
+
+
+@pytest.mark.parametrize("name,n,s,d", zoo()[1:], ids=[z[0] for z in zoo()[1:]])
+def test_spmv_vs_scipy(name, n, s, d):
+    w = O.gen_weights(s.size, seed=5)
+    ptr, col, ow = O.build_csr(n, s, d, w)
+    cp, row, iw = O.build_csc(n, ptr, col, ow)
+    x = np.linspace(-1, 1, n)
+    y = O.spmv_pull(n, cp, row, iw, x)
+    A = sp.csr_matrix((w.astype(np.float64), (s.astype(np.int64), d.astype(np.int64))), shape=(n, n))
+    np.testing.assert_allclose(y, A.T @ x, rtol=1e-12, atol=1e-12)
+    gcp, gs, gd, gw = O.build_grid(n, 4, s, d, w)
+    np.testing.assert_allclose(O.spmv_grid(n, 4, gcp, gs, gd, gw, x), A.T @ x, rtol=1e-12,
+                               atol=1e-12)
+    # identity matrix: y = x
+    I = np.arange(n, dtype=np.uint32)
+    ip, ic, iw2 = O.build_csr(n, I, I, np.ones(n, np.float32))
+    np.testing.assert_array_equal(O.spmv_pull(n, ip, ic, iw2, x), x)
+
+
+@pytest.mark.parametrize("name,n,s,d", zoo()[3:], ids=[z[0] for z in zoo()[3:]])
+def test_ppr_vs_scipy(name, n, s, d):
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    deg = O.out_degree(n, s)
+    seeds = np.unique(s)[:3]
+    out = O.ppr(n, cp, row, deg, seeds, 0.15, 30)
+    A = sp.csr_matrix((np.ones(s.size), (s.astype(np.int64), d.astype(np.int64))), shape=(n, n))
+    p = np.zeros(n)
+    p[seeds] = 1.0 / seeds.size
+    x = p.copy()
+    dg = np.asarray(A.sum(axis=1)).ravel()
+    for _ in range(30):
+        x = 0.15 * p + 0.85 * (A.T @ np.where(dg > 0, x / np.maximum(dg, 1), 0.0))
+    np.testing.assert_allclose(out, x, rtol=1e-12, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# BFS vs scipy shortest paths
+# ---------------------------------------------------------------------------
+def scipy_levels(n, s, d, root):
+    A = sp.csr_matrix((np.ones(s.size), (s.astype(np.int64), d.astype(np.int64))), shape=(n, n))
+    dist = csg.shortest_path(A, unweighted=True, indices=[root], directed=True)[0]
+    out = np.where(np.isinf(dist), 0, dist + 1)
+    return out.astype(np.uint32)
+
+
+@pytest.mark.parametrize("name,n,s,d", zoo()[1:], ids=[z[0] for z in zoo()[1:]])
+@pytest.mark.parametrize("undirected", [False, True])
+def test_bfs_vs_scipy(name, n, s, d, undirected):
+    if undirected:
+        s, d, _ = O.symmetrize(s, d)
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    gcp, gs, gd, _ = O.build_grid(n, 4, s, d)
+    roots = sorted(set([0, int(s[0]) if s.size else 0, n - 1]))
+    for root in roots:
+        ref = scipy_levels(n, s, d, root)
+        for start_pull in (False, True):
+            for switch in (False, True):
+                dist, lv, ls = O.bfs(n, s.size, ptr, col, cp, row, root, start_pull, switch)
+                assert np.array_equal(dist, ref)
+                assert lv == ref.max()
+                assert np.array_equal(ls, np.bincount(ref, minlength=lv + 1)[1:lv + 1])
+        assert np.array_equal(O.bfs_grid(n, 4, gcp, gs, gd, root)[0], ref)
+        assert np.array_equal(O.bfs_edgelist(n, s, d, root)[0], ref)
+
+
+def test_bfs_closed_forms():
+    n = 9
+    s = np.arange(n - 1, dtype=np.uint32)
+    d = s + 1
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    dist, lv, _ = O.bfs(n, s.size, ptr, col, cp, row, 0)
+    assert list(dist) == list(range(1, n + 1)) and lv == n
+    dist, _, _ = O.bfs(n, s.size, ptr, col, cp, row, 4)
+    assert list(dist) == [0] * 4 + [1, 2, 3, 4, 5]
+
+
+def test_parent_checker_catches_planted_errors():
+    n, (s, d) = 1024, O.gen_rmat(10, 8192, seed=4)
+    s, d, _ = O.symmetrize(s, d)
+    ptr, col, _ = O.build_csr(n, s, d)
+    root = int(s[0])
+    dist = scipy_levels(n, s, d, root)
+    # a legal parent: the first out-neighbour (symmetric) one level up
+    parent = np.full(n, n, np.uint32)
+    parent[root] = root
+    for v in range(n):
+        if dist[v] > 1:
+            nb = col[ptr[v]:ptr[v + 1]]
+            parent[v] = nb[dist[nb] == dist[v] - 1][0]
+    assert O.check_bfs_parents(n, ptr, col, root, dist, parent) == 0
+    reached = np.nonzero(dist > 1)[0]
+    bad = parent.copy()
+    bad[reached[0]] = reached[0]  # self parent
+    assert O.check_bfs_parents(n, ptr, col, root, dist, bad) > 0
+    bad = parent.copy()
+    bad[root] = n  # root not its own parent
+    assert O.check_bfs_parents(n, ptr, col, root, dist, bad) > 0
+    bad_dist = dist.copy()
+    bad_dist[reached[0]] += 1  # level off by one
+    assert O.check_bfs_parents(n, ptr, col, root, bad_dist, parent) > 0
+
+
+# ---------------------------------------------------------------------------
+# WCC vs scipy connected components / brute force
+# ---------------------------------------------------------------------------
+def min_label(n, s, d):
+    A = sp.csr_matrix((np.ones(s.size), (s.astype(np.int64), d.astype(np.int64))), shape=(n, n))
+    _, comp = csg.connected_components(A, directed=True, connection="weak")
+    mins = np.full(comp.max() + 1, n)
+    np.minimum.at(mins, comp, np.arange(n))
+    return mins[comp].astype(np.int64)
+
+
+@pytest.mark.parametrize("name,n,s,d", zoo(), ids=[z[0] for z in zoo()])
+def test_wcc_vs_scipy(name, n, s, d):
+    ref = min_label(n, s, d)
+    lab, _ = O.wcc_edges(n, s, d)
+    assert np.array_equal(lab, ref)
+    us, ud, _ = O.symmetrize(s, d)
+    ptr, col, _ = O.build_csr(n, us, ud)
+    lab2, _ = O.wcc_push(n, ptr, col)
+    assert np.array_equal(lab2, ref)
+
+
+def test_wcc_push_directed_is_min_ancestor():
+    n, m = 60, 90
+    s, d = rand_graph(n, m, 11, loops=False)
+    ptr, col, _ = O.build_csr(n, s, d)
+    lab, _ = O.wcc_push(n, ptr, col)
+    # brute force: reach[u][v] by repeated squaring of the adjacency
+    R = np.eye(n, dtype=bool)
+    R[s, d] = True
+    for _ in range(7):
+        R = R | ((R.astype(np.int32) @ R.astype(np.int32)) > 0)
+    ref = np.array([np.nonzero(R[:, v])[0].min() for v in range(n)], np.int64)
+    assert np.array_equal(lab, ref)
