@@ -33,6 +33,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_calibration(nbytes: int) -> float:
+    """One device-to-device copy of nbytes through the HIP runtime (a plain
+    copy kernel of known size, the reference point for the FETCH_SIZE /
+    WRITE_SIZE counters).  Returns its wall time in ms."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    t0 = time.perf_counter()
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    ms = 1e3 * (time.perf_counter() - t0)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return ms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +67,8 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -58,6 +81,9 @@ def main():
     dev = eg.device_info()
     log(f"device: {dev.name.decode()} CUs={dev.compute_units} gfx{dev.gcn_arch} "
         f"HBM={dev.hbm_bytes / 2**30:.0f} GiB")
+    if args.copy_calib:
+        ms = copy_calibration(args.copy_calib << 20)
+        log(f"copy calibration: {args.copy_calib} MiB in {ms:.3f} ms")
     scale, ef = args.scale, 16
     n = 1 << scale
     t0 = time.perf_counter()

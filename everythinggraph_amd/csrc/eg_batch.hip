@@ -63,8 +63,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(uint32_t n, const uint32_t *
     if (s != all) {
       uint64_t acc = 0;
       const uint32_t k1 = ptr[v + 1];
-      for (uint32_t k = ptr[v]; k < k1; k++) {
-        acc |= visit[idx[k]];
+      for (uint32_t k = ptr[v]; k < k1; k += 4) {
+        // four neighbours per step, then the early-exit test
+        uint32_t u[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) u[t] = k + t < k1 ? idx[k + t] : idx[k];
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc |= visit[u[t]];
         if ((acc | s) == all) break;
       }
       nw = acc & ~s;
@@ -80,7 +85,9 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(uint32_t n, const uint32_t *
     }
     nvisit[v] = nw;
   }
-  if (__ballot(nw != 0) && __lane_id() == 0) atomicAdd(cnt, 1u);
+  // "some vertex was reached": only the first waves to see the flag clear write it
+  if (__ballot(nw != 0) && __lane_id() == 0 && __atomic_load_n(cnt, __ATOMIC_RELAXED) == 0u)
+    atomicOr(cnt, 1u);
 }
 
 __global__ void k_ms_extract(uint32_t n, uint32_t k, const uint16_t *__restrict__ lvl,
@@ -114,32 +121,96 @@ __global__ void k_ppr_init(uint32_t n, const uint64_t *__restrict__ mask,
   C[t] = d ? x / (float)d : 0.f;
 }
 
-__global__ __launch_bounds__(kBlock) void k_ppr_spmm(
-    uint32_t n, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
-    const float *__restrict__ C, const uint64_t *__restrict__ mask,
-    const float *__restrict__ alpha_inv_seeds, const uint32_t *__restrict__ deg, float one_minus_a,
-    float *__restrict__ Y, float *__restrict__ Cn) {
-  const int lane = __lane_id();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  const float seed_term = alpha_inv_seeds[lane];
-  for (uint32_t v = wave; v < n; v += nwaves) {
-    const uint32_t k0 = ptr[v], k1 = ptr[v + 1];
-    EgKahan acc;
-    for (uint32_t base = k0; base < k1; base += 64) {
-      const uint32_t cnt = k1 - base < 64 ? k1 - base : 64;
-      const uint32_t my = lane < (int)cnt ? __builtin_nontemporal_load(idx + base + lane) : 0u;
-      for (uint32_t j = 0; j < cnt; j++) {
-        const uint32_t u = __shfl(my, j);
-        acc.add(C[(uint64_t)u * 64 + lane]);
+// Adds the contribution rows of entries [k0, k1) to this lane's column.
+__device__ __forceinline__ void spmm_range(const uint32_t *__restrict__ idx,
+                                           const float *__restrict__ C, uint32_t k0, uint32_t k1,
+                                           int lane, EgKahan &acc) {
+  for (uint32_t base = k0; base < k1; base += 64) {
+    const uint32_t cnt = k1 - base < 64 ? k1 - base : 64;
+    const uint32_t my = lane < (int)cnt ? __builtin_nontemporal_load(idx + base + lane) : 0u;
+    for (uint32_t j = 0; j < cnt; j += 8) {
+      // eight independent 256-byte row gathers in flight per wave
+      float v[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const uint32_t u = __shfl(my, (int)(j + t) & 63);
+        v[t] = C[(uint64_t)u * 64 + lane];
       }
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+        if (j + t < cnt) acc.add(v[t]);
     }
-    float y = one_minus_a * acc.value();
-    if ((mask[v] >> lane) & 1ull) y += seed_term;
-    Y[(uint64_t)v * 64 + lane] = y;
-    const uint32_t d = deg[v];
-    Cn[(uint64_t)v * 64 + lane] = d ? y / (float)d : 0.f;
   }
+}
+
+__device__ __forceinline__ void spmm_finish(uint32_t v, float sum, int lane,
+                                            const uint64_t *__restrict__ mask, float seed_term,
+                                            const uint32_t *__restrict__ deg, float one_minus_a,
+                                            float *__restrict__ Y, float *__restrict__ Cn) {
+  float y = one_minus_a * sum;
+  if ((mask[v] >> lane) & 1ull) y += seed_term;
+  Y[(uint64_t)v * 64 + lane] = y;
+  const uint32_t d = deg[v];
+  Cn[(uint64_t)v * 64 + lane] = d ? y / (float)d : 0.f;
+}
+
+// Work items of the pull plan: segments of long rows (the four waves split
+// the segment, partials merged in wave order) and chunks of short rows (a
+// wave per row).  Lane q owns query column q.
+__global__ __launch_bounds__(kBlock) void k_ppr_spmm(
+    const uint32_t *__restrict__ idx, const uint4 *__restrict__ chunk_info,
+    const uint4 *__restrict__ seg_info, uint32_t nseg, float2 *__restrict__ seg_part,
+    const uint32_t *__restrict__ ptr, const float *__restrict__ C,
+    const uint64_t *__restrict__ mask, const float *__restrict__ alpha_inv_seeds,
+    const uint32_t *__restrict__ deg, float one_minus_a, float *__restrict__ Y,
+    float *__restrict__ Cn) {
+  __shared__ float2 s_p[kBlock / 64][64];
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  if (blockIdx.x < nseg) {
+    const uint4 si = seg_info[blockIdx.x];
+    const uint32_t len = si.y - si.x;
+    const uint32_t part = (len + 4 * 64 - 1) / (4 * 64) * 64;  // multiple of 64 per wave
+    const uint32_t a = si.x + wave * part < si.y ? si.x + wave * part : si.y;
+    const uint32_t b = a + part < si.y ? a + part : si.y;
+    EgKahan acc;
+    spmm_range(idx, C, a, b, lane, acc);
+    s_p[wave][lane] = make_float2(acc.s, acc.c);
+    __syncthreads();
+    if (wave == 0) {
+      EgKahan t;
+      t.s = s_p[0][lane].x;
+      t.c = s_p[0][lane].y;
+      for (int q = 1; q < kBlock / 64; q++) t.merge(s_p[q][lane].x, s_p[q][lane].y);
+      seg_part[(uint64_t)blockIdx.x * 64 + lane] = make_float2(t.s, t.c);
+    }
+    return;
+  }
+  const uint4 ci = chunk_info[blockIdx.x - nseg];
+  const float seed_term = alpha_inv_seeds[lane];
+  for (uint32_t v = ci.x + wave; v < ci.y; v += kBlock / 64) {
+    EgKahan acc;
+    spmm_range(idx, C, ptr[v], ptr[v + 1], lane, acc);
+    spmm_finish(v, acc.value(), lane, mask, seed_term, deg, one_minus_a, Y, Cn);
+  }
+}
+
+// Long rows: a wave per row merges its segment partials in segment order.
+__global__ __launch_bounds__(kBlock) void k_ppr_finish(
+    const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
+    const float2 *__restrict__ seg_part, const uint64_t *__restrict__ mask,
+    const float *__restrict__ alpha_inv_seeds, const uint32_t *__restrict__ deg,
+    float one_minus_a, float *__restrict__ Y, float *__restrict__ Cn) {
+  const int lane = __lane_id();
+  const uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (j >= nlong) return;
+  EgKahan acc;
+  for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++) {
+    const float2 p = seg_part[(uint64_t)s * 64 + lane];
+    acc.merge(p.x, p.y);
+  }
+  spmm_finish(long_rows[j], acc.value(), lane, mask, alpha_inv_seeds[lane], deg, one_minus_a, Y,
+              Cn);
 }
 
 __global__ void k_ppr_extract(uint32_t n, uint32_t k, const float *__restrict__ X,
@@ -164,6 +235,8 @@ struct BatchState {
   uint32_t *seed_ptr = nullptr, *seeds = nullptr;
   uint32_t seeds_cap = 0;
   uint32_t *cnt = nullptr;
+  float2 *seg_part = nullptr;  // [nseg * 64] long-row segment partials
+  uint64_t seg_part_cap = 0;
 };
 
 void eg_batch_free(eg_graph *g) {
@@ -184,6 +257,7 @@ void eg_batch_free(eg_graph *g) {
   eg_dfree_t(&b->seed_ptr);
   eg_dfree_t(&b->seeds);
   eg_dfree_t(&b->cnt);
+  eg_dfree_t(&b->seg_part);
   delete b;
   g->batch = nullptr;
 }
@@ -340,13 +414,29 @@ int eg_ppr_group(eg_graph *g, uint32_t k, const uint32_t *seed_ptr, const uint32
                      b->mask, b->inv, g->out_deg, b->X, b->C);
   EG_LAUNCHED();
   const uint32_t *ptr, *idx;
-  in_orientation(g, &ptr, &idx);
-  const uint32_t grid = blocks_for((uint64_t)n * 64, kBlock) < 65536 ? blocks_for((uint64_t)n * 64, kBlock) : 65536;
+  const float *w;
+  PullPlan *pl;
+  EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  if (b->seg_part_cap < (uint64_t)pl->nseg * 64) {
+    eg_dfree_t(&b->seg_part);
+    EG_TRY(eg_dalloc_t(&b->seg_part, (uint64_t)(pl->nseg ? pl->nseg : 1) * 64, "ppr partials"));
+    b->seg_part_cap = (uint64_t)(pl->nseg ? pl->nseg : 1) * 64;
+  }
+  const float oma = (float)(1.0 - alpha);
   for (int it = 0; it < iterations; it++) {
-    EgProf prof(g, "ppr_spmm");
-    hipLaunchKernelGGL(k_ppr_spmm, dim3(grid), dim3(kBlock), 0, s, n, ptr, idx, b->C, b->mask,
-                       b->inv + 64, g->out_deg, (float)(1.0 - alpha), b->Y, b->Cn);
-    EG_LAUNCHED();
+    {
+      EgProf prof(g, "ppr_spmm");
+      hipLaunchKernelGGL(k_ppr_spmm, dim3(pl->nseg + pl->nchunks), dim3(kBlock), 0, s, idx,
+                         pl->chunk_info, pl->seg_info, pl->nseg, b->seg_part, ptr, b->C, b->mask,
+                         b->inv + 64, g->out_deg, oma, b->Y, b->Cn);
+      EG_LAUNCHED();
+    }
+    if (pl->nlong) {
+      hipLaunchKernelGGL(k_ppr_finish, dim3(blocks_for((uint64_t)pl->nlong * 64, kBlock)),
+                         dim3(kBlock), 0, s, pl->long_rows, pl->seg_base, pl->nlong, b->seg_part,
+                         b->mask, b->inv + 64, g->out_deg, oma, b->Y, b->Cn);
+      EG_LAUNCHED();
+    }
     std::swap(b->X, b->Y);
     std::swap(b->C, b->Cn);
   }

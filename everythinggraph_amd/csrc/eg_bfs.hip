@@ -16,53 +16,47 @@
 //    expands an equal slice of the frontier's edges, finding each edge's
 //    owner by binary search in LDS (load balanced regardless of hubs).  A
 //    vertex is claimed with one atomicOr on its visited word; the winner
-//    writes dist / parent and appends to the next queue with one atomic per
-//    wave (ballot + mbcnt).
-//  * pull: one wave per bitmap word; each unvisited lane scans its in-edges
-//    until it meets a frontier vertex.  The wave writes the next-frontier and
-//    visited words whole (no atomics).
-//  * The host reads two counters per level (frontier size, its out-degree
-//    sum) and applies the reference's switch rule.
+//    writes dist / parent into an LDS queue that the workgroup appends to the
+//    next queue with one atomic.
+//  * pull: one wave per bitmap word; each unvisited lane scans its in-edges,
+//    four at a time, until it meets a frontier vertex.  The wave writes the
+//    next-frontier and visited words whole (no atomics).  Frontier counts are
+//    per-workgroup partials reduced by one small kernel.
+//  * The queue holds only vertices with out-edges: a vertex without out-edges
+//    is never an in-neighbour, so it cannot discover anything either way.
+//  * The host reads the level's counters (frontier size, its out-degree sum,
+//    queue length) and applies the reference's switch rule.
+#include <string.h>
+
 #include "eg_internal.h"
 
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kEdgesPerBlock = 2048;
+constexpr uint32_t kMaxParts = 4096;  // workgroups of the grid-stride kernels
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
 }
+inline uint32_t stride_blocks(uint64_t threads) {
+  const uint32_t b = blocks_for(threads, kBlock);
+  return b < kMaxParts ? (b ? b : 1) : kMaxParts;
+}
 
-// Counters (u32 words of g->bfs_ctr): [0] next frontier size, [1] next queue
-// length, [2..3] next frontier out-degree sum (u64).  The queue holds only
-// vertices with out-edges: a vertex without out-edges is never an
-// in-neighbour, so it cannot discover anything in either direction.
-struct Ctr {
-  uint32_t *count;
-  uint32_t *qlen;
-  unsigned long long *degsum;
-};
-
-// Counts the wave's discoveries and appends those with out-edges to nq (one
-// atomic per wave).  Must be called by all 64 lanes.
-__device__ __forceinline__ void wave_emit(Ctr c, bool found, uint32_t v, uint32_t deg,
-                                          uint32_t *__restrict__ nq) {
-  const uint64_t b = __ballot(found);
-  if (!b) return;
-  const bool enq = found && deg > 0;
-  const uint64_t qb = __ballot(enq);
-  unsigned long long ds = found ? deg : 0u;
+// Sum of a per-lane 64-bit value over the workgroup (result valid in thread 0).
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v,
+                                                            unsigned long long *s_tmp) {
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) ds += __shfl_xor(ds, o);
-  uint32_t base = 0;
-  if (__lane_id() == 0) {
-    atomicAdd(c.count, (uint32_t)__popcll(b));
-    atomicAdd(c.degsum, ds);
-    if (qb) base = atomicAdd(c.qlen, (uint32_t)__popcll(qb));
-  }
-  base = __shfl(base, 0);
-  if (enq && nq) nq[base + __popcll(qb & eg_lanemask_lt())] = v;
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  if (__lane_id() == 0) s_tmp[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kWaves; w++) t += s_tmp[w];
+  __syncthreads();
+  return t;
 }
 
 __global__ void k_bfs_init(uint32_t n, uint32_t root, uint32_t *__restrict__ dist,
@@ -88,18 +82,18 @@ __global__ void k_frontier_degrees(const uint32_t *__restrict__ q, uint32_t qn,
   deg[i] = ptr[u + 1] - ptr[u];
 }
 
-// Push step over the queue: block b expands frontier edges
-// [b*kEdgesPerBlock, (b+1)*kEdgesPerBlock) of the scanned degree sequence.
-// Queue entries all have out-edges, so at most kEdgesPerBlock + 1 of them
-// overlap a block's slice.
+// Counters (u64 words of ctr): [0] next frontier size, [1] its out-degree
+// sum, [2] next queue length.
 __global__ __launch_bounds__(kBlock) void k_bfs_push(
     const uint32_t *__restrict__ q, uint32_t qn, const uint32_t *__restrict__ qoff,
     const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ col,
     uint64_t *__restrict__ visited, uint32_t *__restrict__ dist, uint32_t *__restrict__ parent,
-    uint32_t newdist, uint32_t *__restrict__ nq, Ctr ctr) {
+    uint32_t newdist, uint32_t *__restrict__ nq, unsigned long long *__restrict__ ctr) {
   __shared__ uint32_t s_off[kEdgesPerBlock + 1];
   __shared__ uint32_t s_u[kEdgesPerBlock + 1];
-  __shared__ uint32_t s_first, s_cnt;
+  __shared__ uint32_t s_wq[kEdgesPerBlock];
+  __shared__ uint32_t s_first, s_cnt, s_nq, s_base;
+  __shared__ unsigned long long s_tmp[kWaves];
   const uint32_t e0 = blockIdx.x * kEdgesPerBlock;
   const uint32_t total = qoff[qn];
   const uint32_t e1 = e0 + kEdgesPerBlock < total ? e0 + kEdgesPerBlock : total;
@@ -116,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push(
     }
     s_first = lo;
     s_cnt = a - lo;
+    s_nq = 0;
   }
   __syncthreads();
   const uint32_t first = s_first;
@@ -125,10 +120,9 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push(
     s_u[i] = q[first + i];
   }
   __syncthreads();
+  unsigned long long found = 0, degsum = 0;
   for (uint32_t base = e0; base < e1; base += kBlock) {
     const uint32_t e = base + threadIdx.x;
-    bool won = false;
-    uint32_t v = 0, deg = 0;
     if (e < e1) {
       uint32_t lo = 0, hi = cnt;  // last j with s_off[j] <= e
       while (hi - lo > 1) {
@@ -136,21 +130,33 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push(
         if (s_off[mid] <= e) lo = mid; else hi = mid;
       }
       const uint32_t u = s_u[lo];
-      v = col[ptr[u] + (e - s_off[lo])];
+      const uint32_t v = col[ptr[u] + (e - s_off[lo])];
       const uint64_t bit = 1ull << (v & 63);
       if (!(__atomic_load_n(&visited[v >> 6], __ATOMIC_RELAXED) & bit)) {
         const uint64_t old = atomicOr(reinterpret_cast<unsigned long long *>(&visited[v >> 6]),
                                       (unsigned long long)bit);
         if (!(old & bit)) {
-          won = true;
           dist[v] = newdist;
           if (parent) parent[v] = u;
-          deg = ptr[v + 1] - ptr[v];
+          const uint32_t d = ptr[v + 1] - ptr[v];
+          found++;
+          degsum += d;
+          if (d) s_wq[atomicAdd(&s_nq, 1u)] = v;
         }
       }
     }
-    wave_emit(ctr, won, v, deg, nq);
   }
+  found = block_sum_u64(found, s_tmp);
+  degsum = block_sum_u64(degsum, s_tmp);
+  if (threadIdx.x == 0) {
+    if (s_nq) s_base = (uint32_t)atomicAdd(&ctr[2], (unsigned long long)s_nq);
+    if (found) {
+      atomicAdd(&ctr[0], found);
+      atomicAdd(&ctr[1], degsum);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < s_nq; i += kBlock) nq[s_base + i] = s_wq[i];
 }
 
 // Frontier queue -> frontier bitmap (before a pull level).
@@ -162,86 +168,177 @@ __global__ void k_queue_to_bitmap(const uint32_t *__restrict__ q, uint32_t qn,
   atomicOr(reinterpret_cast<unsigned long long *>(&bm[v >> 6]), 1ull << (v & 63));
 }
 
-// Pull step: wave per visited word.
+// Pull step: a wave per visited word, grid-stride; per-workgroup partials of
+// (found, out-degree sum) in part[2*block].
 __global__ __launch_bounds__(kBlock) void k_bfs_pull(
     uint32_t n, uint32_t nwords, const uint32_t *__restrict__ ptr,
     const uint32_t *__restrict__ idx, const uint32_t *__restrict__ out_ptr,
     const uint64_t *__restrict__ front, uint64_t *__restrict__ next,
     uint64_t *__restrict__ visited, uint32_t *__restrict__ dist, uint32_t *__restrict__ parent,
-    uint32_t newdist, uint32_t *__restrict__ nq, Ctr ctr) {
-  const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint32_t newdist, unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long s_tmp[kWaves];
   const int lane = __lane_id();
-  if (w >= nwords) return;
-  const uint64_t vis = visited[w];
-  const uint32_t v = w * 64 + lane;
-  bool found = false;
-  uint32_t deg = 0;
-  if (v < n && !((vis >> lane) & 1ull)) {
-    const uint32_t k1 = ptr[v + 1];
-    for (uint32_t k = ptr[v]; k < k1; k++) {
-      const uint32_t u = idx[k];
-      if ((front[u >> 6] >> (u & 63)) & 1ull) {
-        found = true;
-        dist[v] = newdist;
-        if (parent) parent[v] = u;
-        deg = out_ptr[v + 1] - out_ptr[v];
-        break;
+  const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  unsigned long long found = 0, degsum = 0;
+  for (uint32_t w = wave0; w < nwords; w += nwaves) {
+    const uint64_t vis = visited[w];
+    const uint32_t v = w * 64 + lane;
+    bool hit = false;
+    if (v < n && !((vis >> lane) & 1ull)) {
+      const uint32_t k1 = ptr[v + 1];
+      for (uint32_t k = ptr[v]; k < k1 && !hit; k += 4) {
+        uint32_t u[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) u[t] = k + t < k1 ? idx[k + t] : idx[k];
+        uint64_t fw[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) fw[t] = front[u[t] >> 6];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          if (!hit && ((fw[t] >> (u[t] & 63)) & 1ull)) {
+            hit = true;
+            dist[v] = newdist;
+            if (parent) parent[v] = u[t];
+          }
+        }
+      }
+      if (hit) {
+        found++;
+        degsum += out_ptr[v + 1] - out_ptr[v];
       }
     }
+    const uint64_t nw = __ballot(hit);
+    if (lane == 0) {
+      next[w] = nw;
+      if (nw) visited[w] = vis | nw;
+    }
   }
-  const uint64_t nw = __ballot(found);
-  if (lane == 0) {
-    next[w] = nw;
-    if (nw) visited[w] = vis | nw;
+  found = block_sum_u64(found, s_tmp);
+  degsum = block_sum_u64(degsum, s_tmp);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = found;
+    part[2 * blockIdx.x + 1] = degsum;
   }
-  wave_emit(ctr, found, v, deg, nq);
 }
 
-// Edge sweep for grid (row-active filter) and edge-array layouts.
-__global__ void k_bfs_edges(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst,
-                            uint64_t m, const uint64_t *__restrict__ front,
-                            uint64_t *__restrict__ visited, uint64_t *__restrict__ next,
-                            uint32_t *__restrict__ dist, uint32_t *__restrict__ parent,
-                            uint32_t newdist, const uint32_t *__restrict__ deg, Ctr ctr) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool won = false;
-  uint32_t d = 0;
-  if (k < m) {
-    const uint32_t s = src[k];
+// out[c] = sum over p of part[p * ncols + c] (one workgroup, fixed order).
+__global__ __launch_bounds__(kBlock) void k_sum_parts(const unsigned long long *__restrict__ part,
+                                                      uint32_t nparts, uint32_t ncols,
+                                                      unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long s_tmp[kWaves];
+  for (uint32_t c = 0; c < ncols; c++) {
+    unsigned long long acc = 0;
+    for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) acc += part[(uint64_t)p * ncols + c];
+    acc = block_sum_u64(acc, s_tmp);
+    if (threadIdx.x == 0) out[c] = acc;
+  }
+}
+
+// Bitmap -> queue of set vertices with out-edges: per-workgroup counts over a
+// fixed word range, then an emit pass at the scanned offsets.
+constexpr uint32_t kWordsPerBlock = kBlock;
+
+__device__ __forceinline__ uint64_t kept_bits(const uint64_t *__restrict__ bm, uint32_t w,
+                                              uint32_t nwords, uint32_t n,
+                                              const uint32_t *__restrict__ out_ptr) {
+  uint64_t word = w < nwords ? bm[w] : 0ull;
+  uint64_t keep = 0;
+  while (word) {
+    const int b = __ffsll((unsigned long long)word) - 1;
+    const uint32_t v = w * 64 + b;
+    if (v < n && out_ptr[v + 1] > out_ptr[v]) keep |= 1ull << b;
+    word &= word - 1;
+  }
+  return keep;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bm_count(const uint64_t *__restrict__ bm,
+                                                     uint32_t nwords, uint32_t n,
+                                                     const uint32_t *__restrict__ out_ptr,
+                                                     uint32_t *__restrict__ cnt) {
+  __shared__ unsigned long long s_tmp[kWaves];
+  const uint32_t w = blockIdx.x * kWordsPerBlock + threadIdx.x;
+  unsigned long long c = (unsigned long long)__popcll(kept_bits(bm, w, nwords, n, out_ptr));
+  c = block_sum_u64(c, s_tmp);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = (uint32_t)c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bm_emit(const uint64_t *__restrict__ bm,
+                                                    uint32_t nwords, uint32_t n,
+                                                    const uint32_t *__restrict__ out_ptr,
+                                                    const uint32_t *__restrict__ off,
+                                                    uint32_t *__restrict__ q) {
+  __shared__ uint32_t s_c[kBlock];
+  const uint32_t w = blockIdx.x * kWordsPerBlock + threadIdx.x;
+  uint64_t keep = kept_bits(bm, w, nwords, n, out_ptr);
+  const uint32_t mine = (uint32_t)__popcll(keep);
+  // inclusive scan of the per-word counts over the workgroup (Hillis-Steele)
+  s_c[threadIdx.x] = mine;
+  __syncthreads();
+  for (uint32_t o = 1; o < kBlock; o <<= 1) {
+    const uint32_t add = threadIdx.x >= o ? s_c[threadIdx.x - o] : 0u;
+    __syncthreads();
+    s_c[threadIdx.x] += add;
+    __syncthreads();
+  }
+  uint32_t pos = off[blockIdx.x] + s_c[threadIdx.x] - mine;
+  while (keep) {
+    const int b = __ffsll((unsigned long long)keep) - 1;
+    q[pos++] = w * 64 + b;
+    keep &= keep - 1;
+  }
+}
+
+// Edge sweep for grid and edge-array layouts (grid-stride, partial counts).
+__global__ __launch_bounds__(kBlock) void k_bfs_edges(
+    const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst, uint64_t m,
+    const uint64_t *__restrict__ front, uint64_t *__restrict__ visited,
+    uint64_t *__restrict__ next, uint32_t *__restrict__ dist, uint32_t *__restrict__ parent,
+    uint32_t newdist, unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long s_tmp[kWaves];
+  unsigned long long found = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += stride) {
+    const uint32_t s = __builtin_nontemporal_load(src + k);
     if ((front[s >> 6] >> (s & 63)) & 1ull) {
-      const uint32_t v = dst[k];
+      const uint32_t v = __builtin_nontemporal_load(dst + k);
       const uint64_t bit = 1ull << (v & 63);
       if (!(__atomic_load_n(&visited[v >> 6], __ATOMIC_RELAXED) & bit)) {
         const uint64_t old = atomicOr(reinterpret_cast<unsigned long long *>(&visited[v >> 6]),
                                       (unsigned long long)bit);
         if (!(old & bit)) {
-          won = true;
+          found++;
           dist[v] = newdist;
           if (parent) parent[v] = s;
           atomicOr(reinterpret_cast<unsigned long long *>(&next[v >> 6]),
                    (unsigned long long)bit);
-          d = deg[v];
         }
       }
     }
   }
-  wave_emit(ctr, won, 0, d, nullptr);
+  found = block_sum_u64(found, s_tmp);
+  if (threadIdx.x == 0) part[blockIdx.x] = found;
 }
 
-// Sum of out-degrees over reached vertices (GTEPS numerator).
-__global__ void k_reached_edges(uint32_t n, const uint32_t *__restrict__ dist,
-                                const uint32_t *__restrict__ deg,
-                                unsigned long long *__restrict__ acc,
-                                uint32_t *__restrict__ reached) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool r = v < n && dist[v] != 0;
-  unsigned long long e = r ? deg[v] : 0u;
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) e += __shfl_xor(e, o);
-  const uint64_t b = __ballot(r);
-  if (__lane_id() == 0 && b) {
-    atomicAdd(acc, e);
-    atomicAdd(reached, (uint32_t)__popcll(b));
+// Reached vertices and the sum of their out-degrees (GTEPS numerator).
+__global__ __launch_bounds__(kBlock) void k_reached_edges(uint32_t n,
+                                                          const uint32_t *__restrict__ dist,
+                                                          const uint32_t *__restrict__ out_deg,
+                                                          unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long s_tmp[kWaves];
+  unsigned long long e = 0, r = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    if (dist[v]) {
+      r++;
+      e += out_deg[v];
+    }
+  }
+  e = block_sum_u64(e, s_tmp);
+  r = block_sum_u64(r, s_tmp);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = e;
+    part[2 * blockIdx.x + 1] = r;
   }
 }
 
@@ -258,22 +355,36 @@ static int bfs_state(eg_graph *g) {
     EG_TRY(eg_dalloc_t(&g->bfs_next, nwords, "next frontier"));
     EG_TRY(eg_dalloc_t(&g->bfs_q, (uint64_t)n + 1, "queue"));
     EG_TRY(eg_dalloc_t(&g->bfs_nq, (uint64_t)n + 1, "queue"));
+    EG_TRY(eg_dalloc_t(&g->bfs_part, 2 * (uint64_t)kMaxParts, "bfs partials"));
+    EG_TRY(eg_dalloc_t(&g->bfs_bmcnt, (uint64_t)blocks_for(nwords, kWordsPerBlock) + 1,
+                       "bitmap counts"));
   }
   if (!g->bfs_ctr) EG_TRY(eg_dalloc_t(&g->bfs_ctr, 64, "counters"));
   return EG_OK;
 }
 
-struct BfsCounts {
-  uint32_t count, qlen;
-  uint64_t degsum;
-};
-
-static int read_counts(eg_graph *g, BfsCounts *c) {
-  EG_HIP(hipMemcpyAsync(g->h_ctr, g->bfs_ctr, 16, hipMemcpyDeviceToHost, g->stream));
+static int read_u64(eg_graph *g, const void *d, int count, uint64_t *h) {
+  EG_HIP(hipMemcpyAsync(g->h_ctr, d, (size_t)count * 8, hipMemcpyDeviceToHost, g->stream));
   EG_HIP(hipStreamSynchronize(g->stream));
-  c->count = g->h_ctr[0];
-  c->qlen = g->h_ctr[1];
-  c->degsum = (uint64_t)g->h_ctr[2] | ((uint64_t)g->h_ctr[3] << 32);
+  memcpy(h, g->h_ctr, (size_t)count * 8);
+  return EG_OK;
+}
+
+// Queue of the vertices with out-edges in bitmap bm; returns its length.
+static int bitmap_to_queue(eg_graph *g, const uint64_t *bm, uint32_t *q, uint32_t *qlen,
+                           void *scan_tmp) {
+  const uint32_t n = g->n, nwords = (n + 63) / 64;
+  hipStream_t s = g->stream;
+  const uint32_t nb = blocks_for(nwords, kWordsPerBlock);
+  uint32_t *cnt = g->bfs_bmcnt;
+  hipLaunchKernelGGL(k_bm_count, dim3(nb), dim3(kBlock), 0, s, bm, nwords, n, g->out_ptr, cnt);
+  EG_LAUNCHED();
+  EG_TRY(eg_scan_excl(s, cnt, cnt, nb, scan_tmp));
+  hipLaunchKernelGGL(k_bm_emit, dim3(nb), dim3(kBlock), 0, s, bm, nwords, n, g->out_ptr, cnt, q);
+  EG_LAUNCHED();
+  EG_HIP(hipMemcpyAsync(g->h_ctr, cnt + nb, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  *qlen = g->h_ctr[0];
   return EG_OK;
 }
 
@@ -288,7 +399,7 @@ static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_
   const uint32_t *iptr = g->in_ptr ? g->in_ptr : g->out_ptr;
   const uint32_t *iidx = g->in_ptr ? g->in_idx : g->out_col;
   uint32_t *parent = want_parent ? g->bfs_parent : nullptr;
-  Ctr ctr{g->bfs_ctr, g->bfs_ctr + 1, reinterpret_cast<unsigned long long *>(g->bfs_ctr + 2)};
+  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(g->bfs_ctr);
   void *tmp = nullptr;
   const size_t qoff_bytes = (((uint64_t)n + 1) * 4 + 255) / 256 * 256;
   EG_TRY(eg_scratch(g, eg_scan_tmp_bytes(n) + qoff_bytes, &tmp));
@@ -310,16 +421,19 @@ static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_
   if (qdeg == 0) qn = 0;
   bool pull = start_pull != 0;
   bool bm_valid = true;  // bfs_front holds the current frontier
+  bool q_valid = true;   // bfs_q holds the current frontier's vertices with out-edges
   const double thr = (double)g->m / 20.0;
+  const uint32_t pull_blocks = stride_blocks((uint64_t)nwords * 64);
   int levels = 0, pull_levels = 0;
   while (count) {
     levels++;
     if ((uint64_t)levels > (uint64_t)n + 1)
       return eg_fail(EG_EDEVICE, "BFS did not terminate within n levels");
-    if (qn == 0) break;  // no frontier vertex has out-edges: this is the last level
     const uint32_t newdist = (uint32_t)levels + 1;
-    EG_HIP(hipMemsetAsync(g->bfs_ctr, 0, 16, s));
     if (!pull) {
+      if (!q_valid) EG_TRY(bitmap_to_queue(g, g->bfs_front, g->bfs_q, &qn, scan_tmp));
+      if (qn == 0) break;  // no frontier vertex has out-edges: this is the last level
+      EG_HIP(hipMemsetAsync(ctr, 0, 24, s));
       hipLaunchKernelGGL(k_frontier_degrees, dim3(blocks_for(qn, 256)), dim3(256), 0, s,
                          g->bfs_q, qn, optr, qoff);
       EG_LAUNCHED();
@@ -331,32 +445,42 @@ static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_
                            newdist, g->bfs_nq, ctr);
         EG_LAUNCHED();
       }
+      std::swap(g->bfs_q, g->bfs_nq);
+      uint64_t h[3];
+      EG_TRY(read_u64(g, ctr, 3, h));
+      count = h[0];
+      qdeg = h[1];
+      qn = (uint32_t)h[2];
+      q_valid = true;
       bm_valid = false;
     } else {
       pull_levels++;
       if (!bm_valid) {
         EG_HIP(hipMemsetAsync(g->bfs_front, 0, (size_t)nwords * 8, s));
-        hipLaunchKernelGGL(k_queue_to_bitmap, dim3(blocks_for(qn, 256)), dim3(256), 0, s,
-                           g->bfs_q, qn, g->bfs_front);
-        EG_LAUNCHED();
+        if (qn) {
+          hipLaunchKernelGGL(k_queue_to_bitmap, dim3(blocks_for(qn, 256)), dim3(256), 0, s,
+                             g->bfs_q, qn, g->bfs_front);
+          EG_LAUNCHED();
+        }
       }
       {
         EgProf prof(g, "bfs_pull");
-        hipLaunchKernelGGL(k_bfs_pull, dim3(blocks_for((uint64_t)nwords * 64, kBlock)),
-                           dim3(kBlock), 0, s, n, nwords, iptr, iidx, optr, g->bfs_front,
-                           g->bfs_next, g->bfs_visited, g->bfs_dist, parent, newdist, g->bfs_nq,
-                           ctr);
+        hipLaunchKernelGGL(k_bfs_pull, dim3(pull_blocks), dim3(kBlock), 0, s, n, nwords, iptr,
+                           iidx, optr, g->bfs_front, g->bfs_next, g->bfs_visited, g->bfs_dist,
+                           parent, newdist, g->bfs_part);
         EG_LAUNCHED();
       }
+      hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, g->bfs_part, pull_blocks, 2u,
+                         ctr);
+      EG_LAUNCHED();
       std::swap(g->bfs_front, g->bfs_next);
+      uint64_t h[2];
+      EG_TRY(read_u64(g, ctr, 2, h));
+      count = h[0];
+      qdeg = h[1];
       bm_valid = true;
+      q_valid = false;
     }
-    std::swap(g->bfs_q, g->bfs_nq);
-    BfsCounts c;
-    EG_TRY(read_counts(g, &c));
-    count = c.count;
-    qn = c.qlen;
-    qdeg = c.degsum;
     // bfs_simple.c:191-197.
     const double load = (double)count + (double)qdeg;
     pull = (switch_mode && !pull && load > thr) || (pull && load > thr);
@@ -375,29 +499,29 @@ static int bfs_edges(eg_graph *g, uint32_t root, bool want_parent, eg_bfs_stats 
   const uint32_t *src = g->cell_ptr ? g->g_src : g->el_src;
   const uint32_t *dst = g->cell_ptr ? g->g_dst : g->el_dst;
   uint32_t *parent = want_parent ? g->bfs_parent : nullptr;
-  Ctr ctr{g->bfs_ctr, g->bfs_ctr + 1, reinterpret_cast<unsigned long long *>(g->bfs_ctr + 2)};
+  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(g->bfs_ctr);
   hipLaunchKernelGGL(k_bfs_init, dim3(blocks_for(n > nwords ? n : nwords, 256)), dim3(256), 0,
                      s, n, root, g->bfs_dist, parent, g->bfs_visited, g->bfs_front, nwords);
   EG_LAUNCHED();
-  uint32_t qn = 1;
+  const uint32_t nb = stride_blocks(g->m);
+  uint64_t count = 1;
   int levels = 0;
-  while (qn) {
+  while (count) {
     levels++;
     if ((uint64_t)levels > (uint64_t)n + 1)
       return eg_fail(EG_EDEVICE, "BFS did not terminate within n levels");
-    EG_HIP(hipMemsetAsync(g->bfs_ctr, 0, 16, s));
     EG_HIP(hipMemsetAsync(g->bfs_next, 0, (size_t)nwords * 8, s));
     {
       EgProf prof(g, g->cell_ptr ? "bfs_grid" : "bfs_edges");
-      hipLaunchKernelGGL(k_bfs_edges, dim3(blocks_for(g->m, 256)), dim3(256), 0, s, src, dst,
-                         g->m, g->bfs_front, g->bfs_visited, g->bfs_next, g->bfs_dist, parent,
-                         (uint32_t)levels + 1, g->out_deg, ctr);
+      hipLaunchKernelGGL(k_bfs_edges, dim3(nb), dim3(kBlock), 0, s, src, dst, g->m, g->bfs_front,
+                         g->bfs_visited, g->bfs_next, g->bfs_dist, parent, (uint32_t)levels + 1,
+                         g->bfs_part);
       EG_LAUNCHED();
     }
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, g->bfs_part, nb, 1u, ctr);
+    EG_LAUNCHED();
     std::swap(g->bfs_front, g->bfs_next);
-    BfsCounts c;
-    EG_TRY(read_counts(g, &c));
-    qn = c.count;
+    EG_TRY(read_u64(g, ctr, 1, &count));
   }
   st->levels = levels;
   st->pull_levels = 0;
@@ -435,19 +559,20 @@ extern "C" int eg_bfs(eg_graph *g, uint32_t root, int32_t start_pull, int32_t sw
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   // Reached vertices and traversed input records (Graph500 convention).
-  EG_HIP(hipMemsetAsync(g->bfs_ctr + 8, 0, 16, s));
-  hipLaunchKernelGGL(k_reached_edges, dim3(blocks_for(g->n, 256)), dim3(256), 0, s, g->n,
-                     g->bfs_dist, g->out_deg,
-                     reinterpret_cast<unsigned long long *>(g->bfs_ctr + 8), g->bfs_ctr + 10);
+  const uint32_t nb = stride_blocks(g->n);
+  hipLaunchKernelGGL(k_reached_edges, dim3(nb), dim3(kBlock), 0, s, g->n, g->bfs_dist, g->out_deg,
+                     g->bfs_part);
   EG_LAUNCHED();
-  EG_HIP(hipMemcpyAsync(g->h_ctr + 8, g->bfs_ctr + 8, 12, hipMemcpyDeviceToHost, s));
+  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(g->bfs_ctr);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, g->bfs_part, nb, 2u, ctr);
+  EG_LAUNCHED();
   if (dist) EG_HIP(hipMemcpyAsync(dist, g->bfs_dist, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
   if (parent)
     EG_HIP(hipMemcpyAsync(parent, g->bfs_parent, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  const uint64_t esum = (uint64_t)g->h_ctr[8] | ((uint64_t)g->h_ctr[9] << 32);
-  st.reached = g->h_ctr[10];
-  st.edges_traversed = (g->flags & EG_UNDIRECTED) ? esum / 2 : esum;
+  uint64_t h[2];
+  EG_TRY(read_u64(g, ctr, 2, h));
+  st.reached = h[1];
+  st.edges_traversed = (g->flags & EG_UNDIRECTED) ? h[0] / 2 : h[0];
   if (stats) *stats = st;
   return EG_OK;
 }

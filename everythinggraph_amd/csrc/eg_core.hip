@@ -261,6 +261,7 @@ static void free_layout(eg_graph *g) {
   eg_grid_free(g);
   eg_batch_free(g);
   g->loaded = false;
+  g->ndangling = 0xFFFFFFFFu;
 }
 
 static void free_algo_state(eg_graph *g) {
@@ -277,6 +278,8 @@ static void free_algo_state(eg_graph *g) {
   eg_dfree_t(&g->bfs_next);
   eg_dfree_t(&g->bfs_q);
   eg_dfree_t(&g->bfs_nq);
+  eg_dfree_t(&g->bfs_part);
+  eg_dfree_t(&g->bfs_bmcnt);
   eg_dfree_t(&g->bfs_ctr);
   eg_dfree_t(&g->wcc_label);
   eg_dfree_t(&g->wcc_lab32);
@@ -320,241 +323,7 @@ extern "C" int eg_graph_get_info(const eg_graph *g, eg_graph_info *out) {
   return EG_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Layout builders.
-// ---------------------------------------------------------------------------
-static int sort_by(eg_graph *g, uint64_t m, uint32_t nkeys, const uint32_t *kin, uint32_t *kout,
-                   EgSortCols cols) {
-  void *tmp = nullptr;
-  EG_TRY(eg_scratch(g, eg_radix_tmp_bytes(m, cols.np), &tmp));
-  return eg_radix_sort(g->stream, m, eg_bits_for(nkeys), kin, kout, cols, tmp);
-}
-
-static int scan_into(eg_graph *g, const uint32_t *in, uint32_t *out, uint64_t count) {
-  void *tmp = nullptr;
-  EG_TRY(eg_scratch(g, eg_scan_tmp_bytes(count), &tmp));
-  return eg_scan_excl(g->stream, in, out, count, tmp);
-}
-
-// init_adj_sort (init_all.c:629-790) and init_adj_count (:444-496): stable sort
-// by src with the dst (and weight) as payload; row_ptr = exclusive scan of the
-// out-degree histogram; CSC = stable sort of the CSR entries by dst.
-static int build_adjacency(eg_graph *g) {
-  const uint64_t m = g->m;
-  const uint32_t n = g->n;
-  const bool w = (g->flags & EG_WEIGHTED) != 0;
-  EG_TRY(eg_dalloc_t(&g->out_ptr, n + 1, "row_ptr"));
-  EG_TRY(eg_dalloc_t(&g->out_col, m, "col"));
-  EG_TRY(eg_dalloc_t(&g->out_src, m, "row ids"));
-  if (w) EG_TRY(eg_dalloc_t(&g->out_w, m, "weights"));
-  EG_TRY(scan_into(g, g->out_deg, g->out_ptr, n));
-  {
-    EgSortCols c;
-    c.np = w ? 2 : 1;
-    c.in[0] = g->e_dst;
-    c.out[0] = g->out_col;
-    if (w) {
-      c.in[1] = reinterpret_cast<const uint32_t *>(g->e_w);
-      c.out[1] = reinterpret_cast<uint32_t *>(g->out_w);
-    }
-    EG_TRY(sort_by(g, m, n, g->e_src, g->out_src, c));
-  }
-  if (!(g->flags & EG_SYMMETRIC)) {
-    EG_TRY(eg_dalloc_t(&g->in_ptr, n + 1, "col_ptr"));
-    EG_TRY(eg_dalloc_t(&g->in_idx, m, "in idx"));
-    if (w) EG_TRY(eg_dalloc_t(&g->in_w, m, "in weights"));
-    EG_TRY(scan_into(g, g->in_deg, g->in_ptr, n));
-    EgSortCols c;
-    c.np = w ? 2 : 1;
-    c.in[0] = g->out_src;
-    c.out[0] = g->in_idx;
-    if (w) {
-      c.in[1] = reinterpret_cast<const uint32_t *>(g->out_w);
-      c.out[1] = reinterpret_cast<uint32_t *>(g->in_w);
-    }
-    EG_TRY(sort_by(g, m, n, g->out_col, nullptr, c));
-  }
-  return EG_OK;
-}
-
-// init_grid_sort (init_all.c:252-361, modes 0/6) and init_grid_nosort
-// (:363-424, mode 1): records ordered by cell (part(src), part(dst)), with
-// (src, input order) inside a cell for the sorted modes and input order for
-// mode 1.  cell_ptr is the exact exclusive scan of the cell histogram.
-static int build_grid(eg_graph *g) {
-  const uint64_t m = g->m;
-  const uint32_t n = g->n, P = g->P;
-  const bool w = (g->flags & EG_WEIGHTED) != 0;
-  const bool presort = g->load_mode != EG_MODE_GRID_NOSORT;
-  const uint64_t ncells = (uint64_t)P * P;
-  uint32_t *ts = nullptr, *td = nullptr, *tk = nullptr, *cnt = nullptr;
-  float *tw = nullptr;
-  int rc = EG_OK;
-  const uint32_t *s_src = g->e_src, *s_dst = g->e_dst;
-  const float *s_w = g->e_w;
-  do {
-    if (presort) {
-      if ((rc = eg_dalloc_t(&ts, m, "grid tmp src")) || (rc = eg_dalloc_t(&td, m, "grid tmp dst")))
-        break;
-      if (w && (rc = eg_dalloc_t(&tw, m, "grid tmp w"))) break;
-      EgSortCols c;
-      c.np = w ? 2 : 1;
-      c.in[0] = g->e_dst;
-      c.out[0] = td;
-      if (w) {
-        c.in[1] = reinterpret_cast<const uint32_t *>(g->e_w);
-        c.out[1] = reinterpret_cast<uint32_t *>(tw);
-      }
-      if ((rc = sort_by(g, m, n, g->e_src, ts, c))) break;
-      s_src = ts;
-      s_dst = td;
-      s_w = tw;
-    }
-    if ((rc = eg_dalloc_t(&tk, m, "cell keys")) || (rc = eg_dalloc_t(&cnt, ncells, "cell counts")))
-      break;
-    if ((rc = eg_cell_keys(g->stream, s_src, s_dst, m, n, P, tk))) break;
-    if (hipMemsetAsync(cnt, 0, ncells * 4, g->stream) != hipSuccess) {
-      rc = eg_fail(EG_EDEVICE, "memset");
-      break;
-    }
-    if ((rc = eg_histogram(g->stream, tk, m, cnt, (uint32_t)ncells))) break;
-    if ((rc = eg_dalloc_t(&g->cell_ptr, ncells + 1, "cell_ptr"))) break;
-    if ((rc = scan_into(g, cnt, g->cell_ptr, ncells))) break;
-    if ((rc = eg_dalloc_t(&g->g_src, m, "grid src")) || (rc = eg_dalloc_t(&g->g_dst, m, "grid dst")))
-      break;
-    if (w && (rc = eg_dalloc_t(&g->g_w, m, "grid w"))) break;
-    EgSortCols c;
-    c.np = w ? 3 : 2;
-    c.in[0] = s_src;
-    c.out[0] = g->g_src;
-    c.in[1] = s_dst;
-    c.out[1] = g->g_dst;
-    if (w) {
-      c.in[2] = reinterpret_cast<const uint32_t *>(s_w);
-      c.out[2] = reinterpret_cast<uint32_t *>(g->g_w);
-    }
-    rc = sort_by(g, m, (uint32_t)ncells, tk, nullptr, c);
-  } while (0);
-  (void)hipStreamSynchronize(g->stream);
-  eg_dfree_t(&ts);
-  eg_dfree_t(&td);
-  eg_dfree_t(&tw);
-  eg_dfree_t(&tk);
-  eg_dfree_t(&cnt);
-  return rc;
-}
-
-// init_edgelist (init_all.c:573-628): mode 7 = stable by src, mode 8 = by src
-// then dst (LSD: stable by dst, then stable by src), mode 9 = input order.
-static int build_edgelist(eg_graph *g) {
-  const uint64_t m = g->m;
-  const uint32_t n = g->n;
-  const bool w = (g->flags & EG_WEIGHTED) != 0;
-  EG_TRY(eg_dalloc_t(&g->el_src, m, "edge src"));
-  EG_TRY(eg_dalloc_t(&g->el_dst, m, "edge dst"));
-  if (w) EG_TRY(eg_dalloc_t(&g->el_w, m, "edge w"));
-  if (g->load_mode == EG_MODE_EDGES_RAW) {
-    EG_HIP(hipMemcpyAsync(g->el_src, g->e_src, m * 4, hipMemcpyDeviceToDevice, g->stream));
-    EG_HIP(hipMemcpyAsync(g->el_dst, g->e_dst, m * 4, hipMemcpyDeviceToDevice, g->stream));
-    if (w) EG_HIP(hipMemcpyAsync(g->el_w, g->e_w, m * 4, hipMemcpyDeviceToDevice, g->stream));
-    return EG_OK;
-  }
-  if (g->load_mode == EG_MODE_EDGES_SRC) {
-    EgSortCols c;
-    c.np = w ? 2 : 1;
-    c.in[0] = g->e_dst;
-    c.out[0] = g->el_dst;
-    if (w) {
-      c.in[1] = reinterpret_cast<const uint32_t *>(g->e_w);
-      c.out[1] = reinterpret_cast<uint32_t *>(g->el_w);
-    }
-    return sort_by(g, m, n, g->e_src, g->el_src, c);
-  }
-  uint32_t *ts = nullptr, *td = nullptr;
-  float *tw = nullptr;
-  int rc = EG_OK;
-  do {
-    if ((rc = eg_dalloc_t(&ts, m, "tmp src")) || (rc = eg_dalloc_t(&td, m, "tmp dst"))) break;
-    if (w && (rc = eg_dalloc_t(&tw, m, "tmp w"))) break;
-    EgSortCols c;
-    c.np = w ? 2 : 1;
-    c.in[0] = g->e_src;
-    c.out[0] = ts;
-    if (w) {
-      c.in[1] = reinterpret_cast<const uint32_t *>(g->e_w);
-      c.out[1] = reinterpret_cast<uint32_t *>(tw);
-    }
-    if ((rc = sort_by(g, m, n, g->e_dst, td, c))) break;
-    EgSortCols c2;
-    c2.np = w ? 2 : 1;
-    c2.in[0] = td;
-    c2.out[0] = g->el_dst;
-    if (w) {
-      c2.in[1] = reinterpret_cast<const uint32_t *>(tw);
-      c2.out[1] = reinterpret_cast<uint32_t *>(g->el_w);
-    }
-    rc = sort_by(g, m, n, ts, g->el_src, c2);
-  } while (0);
-  (void)hipStreamSynchronize(g->stream);
-  eg_dfree_t(&ts);
-  eg_dfree_t(&td);
-  eg_dfree_t(&tw);
-  return rc;
-}
-
-// Common tail of every ingest path: records are in g->e_* (input order, with
-// reversed copies when undirected); validate ids, count degrees (the on-load
-// count of init_all.c:91-94, done once: R4), build the mode's layout.
-static int build_layout(eg_graph *g) {
-  const uint64_t m = g->m;
-  const uint32_t n = g->n;
-  hipEvent_t e0, e1;
-  EG_HIP(hipEventCreate(&e0));
-  EG_HIP(hipEventCreate(&e1));
-  EG_HIP(hipEventRecord(e0, g->stream));
-  int bad = 0;
-  uint32_t *flag = nullptr;
-  EG_TRY(eg_dalloc_t(&flag, 1, "id flag"));
-  int rc = eg_check_ids(g->stream, g->e_src, g->e_dst, m, n, flag, &bad);
-  eg_dfree_t(&flag);
-  if (rc) return rc;
-  if (bad) return eg_fail(EG_ERANGE, "edge file holds a vertex id >= n = %u", n);
-  EG_TRY(eg_dalloc_t(&g->out_deg, n, "out degree"));
-  EG_TRY(eg_dalloc_t(&g->in_deg, n, "in degree"));
-  EG_HIP(hipMemsetAsync(g->out_deg, 0, (size_t)n * 4, g->stream));
-  EG_HIP(hipMemsetAsync(g->in_deg, 0, (size_t)n * 4, g->stream));
-  EG_TRY(eg_histogram(g->stream, g->e_src, m, g->out_deg, n));
-  EG_TRY(eg_histogram(g->stream, g->e_dst, m, g->in_deg, n));
-  switch (g->load_mode) {
-    case EG_MODE_ADJ_SORT:
-    case EG_MODE_ADJ_SORT_FULL:
-    case EG_MODE_ADJ_COUNT:
-      EG_TRY(build_adjacency(g));
-      break;
-    case EG_MODE_GRID_SORT:
-    case EG_MODE_GRID_NOSORT:
-    case EG_MODE_GRID_SORT_SRC:
-      EG_TRY(build_grid(g));
-      break;
-    default:
-      EG_TRY(build_edgelist(g));
-      break;
-  }
-  EG_HIP(hipEventRecord(e1, g->stream));
-  EG_HIP(hipEventSynchronize(e1));
-  float ms = 0.f;
-  EG_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  g->build_ms = ms;
-  // The records (memblock) are released once the layout exists, as
-  // init_adj_sort / init_adj_count do (init_all.c:494,787).
-  eg_dfree_t(&g->e_src);
-  eg_dfree_t(&g->e_dst);
-  eg_dfree_t(&g->e_w);
-  g->loaded = true;
-  return EG_OK;
-}
+int eg_build_layout(eg_graph *g);
 
 static int alloc_records(eg_graph *g, uint64_t m_in) {
   free_layout(g);
@@ -573,7 +342,7 @@ static int alloc_records(eg_graph *g, uint64_t m_in) {
 
 static int finish_records(eg_graph *g) {
   if (g->flags & EG_UNDIRECTED) EG_TRY(eg_append_reverse(g->stream, g->m_in, g->e_src, g->e_dst, g->e_w));
-  return build_layout(g);
+  return eg_build_layout(g);
 }
 
 extern "C" int eg_graph_load_records(eg_graph *g, const void *records, uint64_t nbytes) {

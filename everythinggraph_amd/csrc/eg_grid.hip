@@ -278,21 +278,6 @@ __global__ void k_pr_apply(uint32_t n, float base, float damping, const double *
   }
 }
 
-__global__ void k_pr_init_g(uint32_t n, float init, const uint32_t *__restrict__ out_deg,
-                            float *__restrict__ rank, float *__restrict__ contrib,
-                            uint32_t *__restrict__ ndangling) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  bool dangling = false;
-  if (v < n) {
-    const uint32_t d = out_deg[v];
-    rank[v] = init;
-    contrib[v] = d ? init / (float)d : 0.f;
-    dangling = d == 0;
-  }
-  const uint64_t b = __ballot(dangling);
-  if (__lane_id() == 0 && b) atomicAdd(ndangling, (uint32_t)__popcll(b));
-}
-
 }  // namespace
 
 // Per-graph state of the grid compute copy.
@@ -397,22 +382,11 @@ static int grid_build_compute(eg_graph *g, GridCompute *c) {
   return EG_OK;
 }
 
-static int pr_state(eg_graph *g, int iters) {
-  const uint32_t n = g->n;
-  if (!g->rank_a) {
-    EG_TRY(eg_dalloc_t(&g->rank_a, n, "rank"));
-    EG_TRY(eg_dalloc_t(&g->rank_b, n, "rank"));
-    EG_TRY(eg_dalloc_t(&g->contrib_a, n, "contrib"));
-    EG_TRY(eg_dalloc_t(&g->contrib_b, n, "contrib"));
-  }
-  if (g->pr_stats_cap < iters + 1) {
-    eg_dfree_t(&g->pr_stats);
-    EG_TRY(eg_dalloc_t(&g->pr_stats, 2 * ((uint64_t)iters + 1), "pr stats"));
-    g->pr_stats_cap = iters + 1;
-  }
-  if (!g->bfs_ctr) EG_TRY(eg_dalloc_t(&g->bfs_ctr, 64, "counters"));
-  return EG_OK;
-}
+int eg_pr_state(eg_graph *g, int iters);
+int eg_pr_start(eg_graph *g, double init, float *rank, float *contrib);
+int eg_pr_finish(eg_graph *g, int iters, float *rank, float *contrib, float *rank_out, double *dh,
+                 double *lh);
+static int pr_state(eg_graph *g, int iters) { return eg_pr_state(g, iters); }
 
 // Shared PageRank driver for the grid and edge-array layouts.
 static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, double *dh,
@@ -445,15 +419,7 @@ static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, dou
   }
   float *rank = g->rank_a, *rank_next = g->rank_b;
   float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
-  EG_HIP(hipMemsetAsync(g->bfs_ctr, 0, 4, s));
-  hipLaunchKernelGGL(k_pr_init_g, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (float)init,
-                     g->out_deg, rank, contrib, g->bfs_ctr);
-  EG_LAUNCHED();
-  EG_HIP(hipMemcpyAsync(g->h_ctr, g->bfs_ctr, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  const double d0 = (double)g->h_ctr[0] * (double)(float)init;
-  EG_HIP(hipMemcpyAsync(g->pr_stats, &d0, sizeof(double), hipMemcpyHostToDevice, s));
-  EG_HIP(hipStreamSynchronize(s));
+  EG_TRY(eg_pr_start(g, init, rank, contrib));
   const float base = (float)((1.0 - damping) / (double)n);
   for (int it = 0; it < iters; it++) {
     const double *dmass_in = g->pr_stats + 2 * it;
@@ -503,21 +469,7 @@ static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, dou
     std::swap(rank, rank_next);
     std::swap(contrib, contrib_next);
   }
-  if (rank_out) EG_HIP(hipMemcpyAsync(rank_out, rank, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-  std::vector<double> st(2 * ((size_t)iters + 1));
-  if ((dh || lh) && iters > 0)
-    EG_HIP(hipMemcpyAsync(st.data(), g->pr_stats, st.size() * sizeof(double),
-                          hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  for (int it = 0; it < iters; it++) {
-    if (dh) dh[it] = st[2 * it];
-    if (lh) lh[it] = st[2 * (it + 1) + 1];
-  }
-  if (rank != g->rank_a) {
-    std::swap(g->rank_a, g->rank_b);
-    std::swap(g->contrib_a, g->contrib_b);
-  }
-  return EG_OK;
+  return eg_pr_finish(g, iters, rank, contrib, rank_out, dh, lh);
 }
 
 int eg_pagerank_grid(eg_graph *g, const eg_pr_opts *o, float *rank_out, double *dh, double *lh) {

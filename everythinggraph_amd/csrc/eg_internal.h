@@ -61,15 +61,20 @@ struct EgProfSlot {
 // (row, entry) sequence; rows with more than kLongRow entries are split into
 // segments of kSegLen entries whose compensated partial sums are combined in
 // segment order.
+constexpr uint32_t kPlanChunk = 2048;    // merged (row + entry) units per chunk
+constexpr uint32_t kPlanLongRow = 2048;  // rows longer than this are segmented
+constexpr uint32_t kPlanSegLen = 4096;   // entries per segment of a long row
+
 struct PullPlan {
   bool built = false;
   uint32_t nchunks = 0;     // stream chunks
   uint32_t nlong = 0;       // long rows
   uint32_t nseg = 0;        // segments over all long rows
-  uint32_t *chunk_row = nullptr;  // [nchunks + 1]
+  uint4 *chunk_info = nullptr;    // [nchunks] short rows [x, y), entries [z, w)
+  uint4 *seg_info = nullptr;      // [nseg] entries [x, y) of long row index z
   uint32_t *long_rows = nullptr;  // [nlong]
   uint32_t *seg_base = nullptr;   // [nlong + 1]
-  float2 *seg_part = nullptr;     // [nseg * kMaxCols] (sum, compensation)
+  float2 *seg_part = nullptr;     // [nseg] (sum, compensation)
   double2 *chunk_part = nullptr;  // [nchunks] FP64 (dangling, l1)
   double2 *fin_part = nullptr;    // [kFinBlocks]
   uint32_t *fin_counter = nullptr;
@@ -115,6 +120,9 @@ struct eg_graph {
   uint32_t *bfs_dist = nullptr, *bfs_parent = nullptr;
   uint64_t *bfs_visited = nullptr, *bfs_front = nullptr, *bfs_next = nullptr;
   uint32_t *bfs_q = nullptr, *bfs_nq = nullptr;
+  unsigned long long *bfs_part = nullptr;  // per-workgroup partial counts
+  uint32_t *bfs_bmcnt = nullptr;           // bitmap -> queue block counts
+  uint32_t ndangling = 0xFFFFFFFFu;        // vertices without out-edges (cached)
   uint32_t *bfs_ctr = nullptr;      // device counters
   uint32_t *h_ctr = nullptr;        // pinned mirror
   int64_t *wcc_label = nullptr;
@@ -158,6 +166,13 @@ int eg_scan_excl(hipStream_t s, const uint32_t *in, uint32_t *out,
 // deg[k] += number of keys equal to k (deg zeroed by the caller).
 int eg_histogram(hipStream_t s, const uint32_t *keys, uint64_t count,
                  uint32_t *deg, uint32_t nbins);
+// ptr[0..n] = offsets of the runs of ascending keys; deg (may be NULL) their
+// lengths.
+int eg_ptr_from_sorted(hipStream_t s, const uint32_t *keys, uint64_t m,
+                       uint32_t n, uint32_t *ptr, uint32_t *deg);
+// *result (host) = number of zero entries of a[0..n); d_tmp holds 1025 words.
+int eg_count_zero(hipStream_t s, const uint32_t *a, uint32_t n,
+                  uint32_t *d_tmp, uint32_t *result);
 // Stable LSD radix sort of count keys < 2^key_bits with up to three 32-bit
 // payload columns.  kout may be NULL when the sorted keys are not needed.
 // Inputs are not modified.
@@ -189,6 +204,10 @@ int eg_cell_keys(hipStream_t s, const uint32_t *src, const uint32_t *dst,
 // Pull planning (eg_pull.hip).
 int eg_plan_build(eg_graph *g, PullPlan *pl, const uint32_t *ptr);
 void eg_plan_free(PullPlan *pl);
+// In-edge orientation of the pull kernels (CSC, or the CSR of a symmetric
+// graph) and its plan, built on first use.
+int eg_pull_orientation(eg_graph *g, const uint32_t **ptr, const uint32_t **idx,
+                        const float **w, PullPlan **pl);
 
 // ---------------------------------------------------------------------------
 // Device helpers.

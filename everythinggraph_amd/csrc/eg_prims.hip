@@ -367,7 +367,77 @@ inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
 }
 
+// ptr[v] = first k with keys[k] >= v over ascending keys, v in [0, n]: the
+// boundary detection + scanIBack(min) of init_all.c:675-691 as one binary
+// search per vertex.  deg[v] = ptr[v+1] - ptr[v] when deg is given.
+__global__ void k_ptr_from_sorted(const uint32_t *__restrict__ keys, uint64_t m, uint32_t n,
+                                  uint32_t *__restrict__ ptr) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  uint64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (keys[mid] >= v) hi = mid; else lo = mid + 1;
+  }
+  ptr[v] = (uint32_t)lo;
+}
+
+__global__ void k_deg_from_ptr(const uint32_t *__restrict__ ptr, uint32_t n,
+                               uint32_t *__restrict__ deg) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) deg[v] = ptr[v + 1] - ptr[v];
+}
+
+// Number of zero entries: per-workgroup counts, then one workgroup sums them.
+constexpr uint32_t kCountBlocks = 1024;
+__global__ __launch_bounds__(256) void k_count_zero(const uint32_t *__restrict__ a, uint32_t n,
+                                                    uint32_t *__restrict__ part) {
+  __shared__ uint32_t s_w[4];
+  uint32_t c = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    c += a[i] == 0;
+  c = eg_wave_sum_u32(c);
+  if (__lane_id() == 0) s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ __launch_bounds__(256) void k_sum_u32(const uint32_t *__restrict__ part, uint32_t np,
+                                                 uint32_t *__restrict__ out) {
+  __shared__ uint32_t s_w[4];
+  uint32_t c = 0;
+  for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) c += part[i];
+  c = eg_wave_sum_u32(c);
+  if (__lane_id() == 0) s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
 }  // namespace
+
+int eg_ptr_from_sorted(hipStream_t s, const uint32_t *keys, uint64_t m, uint32_t n, uint32_t *ptr,
+                       uint32_t *deg) {
+  hipLaunchKernelGGL(k_ptr_from_sorted, dim3(blocks_for((uint64_t)n + 1, 256)), dim3(256), 0, s,
+                     keys, m, n, ptr);
+  EG_LAUNCHED();
+  if (deg) {
+    hipLaunchKernelGGL(k_deg_from_ptr, dim3(blocks_for(n, 256)), dim3(256), 0, s, ptr, n, deg);
+    EG_LAUNCHED();
+  }
+  return EG_OK;
+}
+
+int eg_count_zero(hipStream_t s, const uint32_t *a, uint32_t n, uint32_t *d_tmp,
+                  uint32_t *result) {
+  const uint32_t nb = blocks_for(n, 256) < kCountBlocks ? blocks_for(n, 256) : kCountBlocks;
+  hipLaunchKernelGGL(k_count_zero, dim3(nb ? nb : 1), dim3(256), 0, s, a, n, d_tmp);
+  EG_LAUNCHED();
+  hipLaunchKernelGGL(k_sum_u32, dim3(1), dim3(256), 0, s, d_tmp, nb ? nb : 1, d_tmp + kCountBlocks);
+  EG_LAUNCHED();
+  EG_HIP(hipMemcpyAsync(result, d_tmp + kCountBlocks, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Host entry points.

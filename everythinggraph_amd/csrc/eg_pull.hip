@@ -7,14 +7,16 @@
 //
 // Work decomposition (built once per orientation, eg_plan_build):
 //  * Rows are grouped by the merged coordinate key(r) = ptr[r] + r into chunks
-//    of kChunk units, so a chunk holds at most kChunk rows and its short rows
-//    at most kChunk + kLongRow entries.  One workgroup streams the chunk's
-//    entries (coalesced index loads, gathered values) into LDS, then rows of
-//    up to kSerialRow entries are summed by one lane each and longer ones by a
-//    whole wave.
-//  * Rows with more than kLongRow entries (R-MAT hubs) are cut into segments
-//    of kSegLen entries, one workgroup each; a finalize kernel merges the
-//    segments of a row in segment order.
+//    of kPlanChunk units, so a chunk holds at most kPlanChunk rows and its
+//    short rows at most kPlanChunk + kPlanLongRow entries.  Each chunk's row
+//    and entry ranges are precomputed, so a workgroup learns its whole task
+//    from one 16-byte load, then issues its row-pointer, index and prefetch
+//    loads together, gathers the values into LDS (eight independent requests
+//    in flight per lane), and reduces rows from LDS: rows of up to kSerialRow
+//    entries by one lane each, longer ones by a whole wave.
+//  * Rows with more than kPlanLongRow entries (R-MAT hubs) are cut into
+//    segments of kPlanSegLen entries, one workgroup each; a finish kernel
+//    merges the segments of a row in segment order.
 // All sums are FP32 with Neumaier compensation in a fixed order, so results
 // are deterministic.  PageRank's dangling mass and L1 delta are reduced in
 // FP64 on the device (per-chunk partials, then a last-block reduction).
@@ -26,15 +28,21 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr uint32_t kChunk = 2048;
-constexpr uint32_t kLongRow = 4096;
-constexpr uint32_t kSegLen = 4096;
+constexpr uint32_t kChunk = kPlanChunk;
+constexpr uint32_t kLongRow = kPlanLongRow;
+constexpr uint32_t kSegLen = kPlanSegLen;
 constexpr uint32_t kSerialRow = 32;
 constexpr uint32_t kLdsVals = kChunk + kLongRow;
 constexpr int kFinBlocks = 128;
+constexpr int kBatch = 8;
+
+inline uint32_t blocks_for(uint64_t count, uint32_t per) {
+  return (uint32_t)((count + per - 1) / per);
+}
 
 struct PlanDev {
-  const uint32_t *chunk_row;
+  const uint4 *chunk_info;
+  const uint4 *seg_info;
   const uint32_t *long_rows;
   const uint32_t *seg_base;
   float2 *seg_part;
@@ -46,7 +54,8 @@ struct PlanDev {
 
 PlanDev plan_dev(const PullPlan &p) {
   PlanDev d;
-  d.chunk_row = p.chunk_row;
+  d.chunk_info = p.chunk_info;
+  d.seg_info = p.seg_info;
   d.long_rows = p.long_rows;
   d.seg_base = p.seg_base;
   d.seg_part = p.seg_part;
@@ -62,24 +71,24 @@ PlanDev plan_dev(const PullPlan &p) {
 // ---------------------------------------------------------------------------
 // Planning kernels.
 // ---------------------------------------------------------------------------
-__global__ void k_chunk_rows(const uint32_t *__restrict__ ptr, uint32_t n, uint32_t nchunks,
-                             uint32_t *__restrict__ chunk_row) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b > nchunks) return;
-  if (b == nchunks) {
-    chunk_row[b] = n;
-    return;
-  }
-  const uint64_t target = (uint64_t)b * kChunk;
+__device__ __forceinline__ uint32_t first_row_at(const uint32_t *__restrict__ ptr, uint32_t n,
+                                                 uint64_t target) {
   uint32_t lo = 0, hi = n;  // first r in [0, n] with ptr[r] + r >= target
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
-    if ((uint64_t)ptr[mid] + mid >= target)
-      hi = mid;
-    else
-      lo = mid + 1;
+    if ((uint64_t)ptr[mid] + mid >= target) hi = mid; else lo = mid + 1;
   }
-  chunk_row[b] = lo;
+  return lo;
+}
+
+__global__ void k_chunk_info(const uint32_t *__restrict__ ptr, uint32_t n, uint32_t nchunks,
+                             uint4 *__restrict__ info) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunks) return;
+  const uint32_t r0 = first_row_at(ptr, n, (uint64_t)c * kChunk);
+  uint32_t r1 = c + 1 == nchunks ? n : first_row_at(ptr, n, (uint64_t)(c + 1) * kChunk);
+  if (r1 > r0 && ptr[r1] - ptr[r1 - 1] > kLongRow) r1--;  // a trailing long row is segmented
+  info[c] = make_uint4(r0, r1, ptr[r0], ptr[r1]);
 }
 
 __global__ void k_long_flags(const uint32_t *__restrict__ ptr, uint32_t n,
@@ -101,6 +110,18 @@ __global__ void k_long_compact(const uint32_t *__restrict__ ptr, uint32_t n,
   }
 }
 
+__global__ void k_seg_info(const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ long_rows,
+                           const uint32_t *__restrict__ seg_base, uint32_t nlong,
+                           uint4 *__restrict__ seg_info) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlong) return;
+  const uint32_t r = long_rows[j];
+  const uint32_t k0 = ptr[r], k1 = ptr[r + 1];
+  uint32_t k = k0;
+  for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++, k += kSegLen)
+    seg_info[s] = make_uint4(k, k + kSegLen < k1 ? k + kSegLen : k1, j, 0u);
+}
+
 // ---------------------------------------------------------------------------
 // Iteration kernels.
 // ---------------------------------------------------------------------------
@@ -116,14 +137,27 @@ struct PrArgs {
   int uniform;
 };
 
-template <bool PR, bool WEIGHTED>
-__device__ __forceinline__ float entry_value(const uint32_t *__restrict__ idx,
+// Values of entries k, k + kBlock, ..., k + (kBatch-1)*kBlock (0 past kend).
+// All index loads are issued before all gathers so that a lane keeps kBatch
+// independent memory requests in flight.
+template <bool WEIGHTED>
+__device__ __forceinline__ void gather_batch(const uint32_t *__restrict__ idx,
                                              const float *__restrict__ w,
-                                             const float *__restrict__ x, uint32_t k) {
-  const uint32_t u = __builtin_nontemporal_load(idx + k);
-  float v = x[u];
-  if (WEIGHTED) v *= __builtin_nontemporal_load(w + k);
-  return v;
+                                             const float *__restrict__ x, uint32_t k,
+                                             uint32_t kend, float (&v)[kBatch]) {
+  uint32_t u[kBatch];
+  float wv[kBatch];
+#pragma unroll
+  for (int q = 0; q < kBatch; q++) {
+    const uint32_t kk = k + q * kBlock;
+    u[q] = kk < kend ? __builtin_nontemporal_load(idx + kk) : 0u;
+    if (WEIGHTED) wv[q] = kk < kend ? __builtin_nontemporal_load(w + kk) : 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < kBatch; q++) {
+    const float xv = x[u[q]];
+    v[q] = (k + q * kBlock < kend) ? (WEIGHTED ? xv * wv[q] : xv) : 0.f;
+  }
 }
 
 __device__ __forceinline__ float pr_base(const PrArgs &a) {
@@ -136,15 +170,14 @@ __device__ __forceinline__ float pr_base(const PrArgs &a) {
 // partials (dangling mass of the new iterate, |r' - r|).
 template <bool PR>
 __device__ __forceinline__ void finish_row(uint32_t r, float sum, float *__restrict__ y,
-                                           const PrArgs &a, float basep, double &dang,
-                                           double &l1) {
+                                           const PrArgs &a, float basep, uint32_t deg,
+                                           float rold, double &dang, double &l1) {
   if (PR) {
     const float v = basep + a.damping * sum;
     y[r] = v;
-    const uint32_t d = a.out_deg[r];
-    a.contrib_new[r] = d ? v / (float)d : 0.f;
-    if (d == 0) dang += (double)v;
-    l1 += fabs((double)v - (double)a.rank_old[r]);
+    a.contrib_new[r] = deg ? v / (float)deg : 0.f;
+    if (deg == 0) dang += (double)v;
+    l1 += fabs((double)v - (double)rold);
   } else {
     y[r] = sum;
   }
@@ -157,38 +190,26 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
                                                  const float *__restrict__ x,
                                                  float *__restrict__ y, PlanDev pl, PrArgs a) {
   __shared__ float s_val[kLdsVals];
+  __shared__ uint32_t s_ptr[kChunk + 1];
   __shared__ uint32_t s_list[kChunk];
   __shared__ uint32_t s_nlist;
   __shared__ float s_ks[kWaves], s_kc[kWaves];
   __shared__ double s_d[kWaves], s_l[kWaves];
-  __shared__ uint32_t s_seg_row;
 
-  const int tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x;
   const int lane = __lane_id();
   const int wave = tid >> 6;
 
   if (blockIdx.x < pl.nseg) {
     // ---- one segment of a long row ----
-    const uint32_t s = blockIdx.x;
-    if (tid == 0) {
-      uint32_t lo = 0, hi = pl.nlong;  // last j with seg_base[j] <= s
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (pl.seg_base[mid] <= s)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      s_seg_row = lo;
-    }
-    __syncthreads();
-    const uint32_t j = s_seg_row;
-    const uint32_t r = pl.long_rows[j];
-    const uint32_t k0 = ptr[r] + (s - pl.seg_base[j]) * kSegLen;
-    const uint32_t kend = ptr[r + 1];
-    const uint32_t k1 = k0 + kSegLen < kend ? k0 + kSegLen : kend;
+    const uint4 si = pl.seg_info[blockIdx.x];
     EgKahan acc;
-    for (uint32_t k = k0 + tid; k < k1; k += kBlock) acc.add(entry_value<PR, WEIGHTED>(idx, w, x, k));
+    for (uint32_t base = si.x; base < si.y; base += kBlock * kBatch) {
+      float v[kBatch];
+      gather_batch<WEIGHTED>(idx, w, x, base + tid, si.y, v);
+#pragma unroll
+      for (int q = 0; q < kBatch; q++) acc.add(v[q]);
+    }
     eg_wave_kahan_allreduce(acc);
     if (lane == 0) {
       s_ks[wave] = acc.s;
@@ -200,46 +221,70 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
       t.s = s_ks[0];
       t.c = s_kc[0];
       for (int q = 1; q < kWaves; q++) t.merge(s_ks[q], s_kc[q]);
-      pl.seg_part[s] = make_float2(t.s, t.c);
+      pl.seg_part[blockIdx.x] = make_float2(t.s, t.c);
     }
     return;
   }
 
   // ---- a chunk of short rows ----
   const uint32_t c = blockIdx.x - pl.nseg;
-  const uint32_t r0 = pl.chunk_row[c];
-  uint32_t r1 = pl.chunk_row[c + 1];
-  if (r1 > r0 && ptr[r1] - ptr[r1 - 1] > kLongRow) r1--;  // trailing long row
-  const uint32_t k0 = ptr[r0];
-  const uint32_t k1 = ptr[r1];
+  const uint4 ci = pl.chunk_info[c];
+  const uint32_t r0 = ci.x, r1 = ci.y, k0 = ci.z, k1 = ci.w;
+  const uint32_t nrows = r1 - r0;
   if (tid == 0) s_nlist = 0;
-  for (uint32_t k = k0 + tid; k < k1; k += kBlock)
-    s_val[k - k0] = entry_value<PR, WEIGHTED>(idx, w, x, k);
+  // row pointers, relative to k0
+  for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = ptr[r0 + i] - k0;
+  // prefetch the per-row inputs of this lane's first row
+  uint32_t deg0 = 0;
+  float rold0 = 0.f;
+  if (PR && tid < nrows) {
+    deg0 = a.out_deg[r0 + tid];
+    rold0 = a.rank_old[r0 + tid];
+  }
+  for (uint32_t base = k0; base < k1; base += kBlock * kBatch) {
+    float v[kBatch];
+    gather_batch<WEIGHTED>(idx, w, x, base + tid, k1, v);
+#pragma unroll
+    for (int q = 0; q < kBatch; q++) {
+      const uint32_t k = base + tid + q * kBlock;
+      if (k < k1) s_val[k - k0] = v[q];
+    }
+  }
   __syncthreads();
 
   const float basep = PR ? pr_base(a) : 0.f;
   double dang = 0.0, l1 = 0.0;
-  const uint32_t nrows = r1 - r0;
   for (uint32_t i = tid; i < nrows; i += kBlock) {
     const uint32_t r = r0 + i;
-    const uint32_t ra = ptr[r] - k0, rb = ptr[r + 1] - k0;
+    const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
     if (rb - ra <= kSerialRow) {
       EgKahan acc;
       for (uint32_t q = ra; q < rb; q++) acc.add(s_val[q]);
-      finish_row<PR>(r, acc.value(), y, a, basep, dang, l1);
+      uint32_t deg = deg0;
+      float rold = rold0;
+      if (PR && i != tid) {
+        deg = a.out_deg[r];
+        rold = a.rank_old[r];
+      }
+      finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
     } else {
-      s_list[atomicAdd(&s_nlist, 1u)] = r;
+      s_list[atomicAdd(&s_nlist, 1u)] = i;
     }
   }
   __syncthreads();
   const uint32_t nlist = s_nlist;
   for (uint32_t li = wave; li < nlist; li += kWaves) {
-    const uint32_t r = s_list[li];
-    const uint32_t ra = ptr[r] - k0, rb = ptr[r + 1] - k0;
+    const uint32_t i = s_list[li];
+    const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
     EgKahan acc;
     for (uint32_t q = ra + lane; q < rb; q += 64) acc.add(s_val[q]);
     eg_wave_kahan_allreduce(acc);
-    if (lane == 0) finish_row<PR>(r, acc.value(), y, a, basep, dang, l1);
+    if (lane == 0) {
+      const uint32_t r = r0 + i;
+      const uint32_t deg = PR ? a.out_deg[r] : 0u;
+      const float rold = PR ? a.rank_old[r] : 0.f;
+      finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
+    }
   }
   if (PR) {
     dang = eg_wave_sum_f64(dang);
@@ -264,8 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
 // the FP64 partials of all chunks and long rows; the last block to finish
 // writes the iteration's statistics and re-arms the counter.
 template <bool PR>
-__global__ __launch_bounds__(kBlock) void k_pull_finish(const uint32_t *__restrict__ ptr,
-                                                        float *__restrict__ y, PlanDev pl,
+__global__ __launch_bounds__(kBlock) void k_pull_finish(float *__restrict__ y, PlanDev pl,
                                                         PrArgs a) {
   __shared__ double s_d[kWaves], s_l[kWaves];
   __shared__ uint32_t s_last;
@@ -283,7 +327,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_finish(const uint32_t *__restri
       const float2 p = pl.seg_part[s];
       acc.merge(p.x, p.y);
     }
-    finish_row<PR>(r, acc.value(), y, a, basep, dang, l1);
+    const uint32_t deg = PR ? a.out_deg[r] : 0u;
+    const float rold = PR ? a.rank_old[r] : 0.f;
+    finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
   }
   if (!PR) return;
   for (uint32_t c = gtid; c < pl.nchunks; c += gthreads) {
@@ -315,9 +361,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_finish(const uint32_t *__restri
     double td = 0.0, tl = 0.0;
     for (uint32_t b = 0; b < gridDim.x; b++) {
       const double *pp = reinterpret_cast<const double *>(&pl.fin_part[b]);
-      const double2 p = make_double2(__builtin_nontemporal_load(pp), __builtin_nontemporal_load(pp + 1));
-      td += p.x;
-      tl += p.y;
+      td += __builtin_nontemporal_load(pp);
+      tl += __builtin_nontemporal_load(pp + 1);
     }
     a.stats_out[0] = td;
     a.stats_out[1] = tl;
@@ -326,28 +371,19 @@ __global__ __launch_bounds__(kBlock) void k_pull_finish(const uint32_t *__restri
 }
 
 __global__ void k_pr_init(uint32_t n, float init, const uint32_t *__restrict__ out_deg,
-                          float *__restrict__ rank, float *__restrict__ contrib,
-                          uint32_t *__restrict__ ndangling) {
+                          float *__restrict__ rank, float *__restrict__ contrib) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  bool dangling = false;
-  if (v < n) {
-    const uint32_t d = out_deg[v];
-    rank[v] = init;
-    contrib[v] = d ? init / (float)d : 0.f;
-    dangling = d == 0;
-  }
-  const uint64_t b = __ballot(dangling);
-  if (__lane_id() == 0 && b) atomicAdd(ndangling, (uint32_t)__popcll(b));
-}
-
-inline uint32_t blocks_for(uint64_t count, uint32_t per) {
-  return (uint32_t)((count + per - 1) / per);
+  if (v >= n) return;
+  const uint32_t d = out_deg[v];
+  rank[v] = init;
+  contrib[v] = d ? init / (float)d : 0.f;
 }
 
 }  // namespace
 
 void eg_plan_free(PullPlan *pl) {
-  eg_dfree_t(&pl->chunk_row);
+  eg_dfree_t(&pl->chunk_info);
+  eg_dfree_t(&pl->seg_info);
   eg_dfree_t(&pl->long_rows);
   eg_dfree_t(&pl->seg_base);
   eg_dfree_t(&pl->seg_part);
@@ -366,9 +402,9 @@ int eg_plan_build(eg_graph *g, PullPlan *pl, const uint32_t *ptr) {
   const uint64_t nchunks = (m + n) / kChunk + 1;
   if (nchunks >= (1ull << 31)) return eg_fail(EG_ENOTSUP, "graph too large for the pull plan");
   pl->nchunks = (uint32_t)nchunks;
-  EG_TRY(eg_dalloc_t(&pl->chunk_row, nchunks + 1, "chunk rows"));
-  hipLaunchKernelGGL(k_chunk_rows, dim3(blocks_for(nchunks + 1, 256)), dim3(256), 0, s, ptr, n,
-                     pl->nchunks, pl->chunk_row);
+  EG_TRY(eg_dalloc_t(&pl->chunk_info, nchunks, "chunk info"));
+  hipLaunchKernelGGL(k_chunk_info, dim3(blocks_for(nchunks, 256)), dim3(256), 0, s, ptr, n,
+                     pl->nchunks, pl->chunk_info);
   EG_LAUNCHED();
 
   uint32_t *flags = nullptr, *segcnt = nullptr;
@@ -397,6 +433,12 @@ int eg_plan_build(eg_graph *g, PullPlan *pl, const uint32_t *ptr) {
   eg_dfree_t(&flags);
   eg_dfree_t(&segcnt);
   pl->nseg = nseg;
+  EG_TRY(eg_dalloc_t(&pl->seg_info, nseg ? nseg : 1, "segment info"));
+  if (nlong) {
+    hipLaunchKernelGGL(k_seg_info, dim3(blocks_for(nlong, 256)), dim3(256), 0, s, ptr,
+                       pl->long_rows, pl->seg_base, nlong, pl->seg_info);
+    EG_LAUNCHED();
+  }
   EG_TRY(eg_dalloc_t(&pl->seg_part, nseg ? nseg : 1, "segment partials"));
   EG_TRY(eg_dalloc_t(&pl->chunk_part, nchunks, "chunk partials"));
   EG_TRY(eg_dalloc_t(&pl->fin_part, kFinBlocks, "finish partials"));
@@ -409,8 +451,8 @@ int eg_plan_build(eg_graph *g, PullPlan *pl, const uint32_t *ptr) {
 
 // Orientation used by the pull kernels: in-edges when the CSC exists, else
 // the CSR of a symmetric graph (-s).
-static int pull_orientation(eg_graph *g, const uint32_t **ptr, const uint32_t **idx,
-                            const float **w, PullPlan **pl) {
+int eg_pull_orientation(eg_graph *g, const uint32_t **ptr, const uint32_t **idx, const float **w,
+                        PullPlan **pl) {
   if (g->in_ptr) {
     *ptr = g->in_ptr;
     *idx = g->in_idx;
@@ -448,11 +490,11 @@ static int launch_pull(eg_graph *g, bool pr, const uint32_t *ptr, const uint32_t
   if (pr || pl.nlong) {
     EgProf prof(g, pr ? "pr_finish" : "spmv_finish");
     if (pr)
-      hipLaunchKernelGGL((k_pull_finish<true>), dim3(kFinBlocks), dim3(kBlock), 0, g->stream, ptr,
-                         y, d, a);
+      hipLaunchKernelGGL((k_pull_finish<true>), dim3(kFinBlocks), dim3(kBlock), 0, g->stream, y, d,
+                         a);
     else
-      hipLaunchKernelGGL((k_pull_finish<false>), dim3(kFinBlocks), dim3(kBlock), 0, g->stream,
-                         ptr, y, d, a);
+      hipLaunchKernelGGL((k_pull_finish<false>), dim3(kFinBlocks), dim3(kBlock), 0, g->stream, y,
+                         d, a);
     EG_LAUNCHED();
   }
   return EG_OK;
@@ -463,7 +505,7 @@ int eg_pagerank_edges(eg_graph *g, const eg_pr_opts *o, float *rank_out, double 
 int eg_spmv_grid(eg_graph *g, const float *x, float *y);
 int eg_spmv_edges(eg_graph *g, const float *x, float *y);
 
-static int ensure_pr_state(eg_graph *g, int iters) {
+int eg_pr_state(eg_graph *g, int iters) {
   const uint32_t n = g->n;
   if (!g->rank_a) {
     EG_TRY(eg_dalloc_t(&g->rank_a, n, "rank"));
@@ -490,6 +532,42 @@ void eg_pr_resolve(const eg_graph *g, const eg_pr_opts *o, double *damping, int 
   *uniform = o ? o->dangling == EG_PR_DANGLING_UNIFORM : 0;
 }
 
+// First iterate: rank = init, contrib = init/outdeg, and the FP64 dangling
+// mass of that iterate (init times the number of sinks) in pr_stats[0].
+int eg_pr_start(eg_graph *g, double init, float *rank, float *contrib) {
+  const uint32_t n = g->n;
+  hipLaunchKernelGGL(k_pr_init, dim3(blocks_for(n, 256)), dim3(256), 0, g->stream, n, (float)init,
+                     g->out_deg, rank, contrib);
+  EG_LAUNCHED();
+  const double d0 = (double)g->ndangling * (double)(float)init;
+  EG_HIP(hipMemcpyAsync(g->pr_stats, &d0, sizeof(double), hipMemcpyHostToDevice, g->stream));
+  EG_HIP(hipStreamSynchronize(g->stream));
+  return EG_OK;
+}
+
+// Downloads results and statistics after the iterations; keeps the final
+// iterate in rank_a / contrib_a.
+int eg_pr_finish(eg_graph *g, int iters, float *rank, float *contrib, float *rank_out, double *dh,
+                 double *lh) {
+  hipStream_t s = g->stream;
+  if (rank_out) EG_HIP(hipMemcpyAsync(rank_out, rank, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
+  std::vector<double> st(2 * ((size_t)iters + 1));
+  if ((dh || lh) && iters > 0)
+    EG_HIP(hipMemcpyAsync(st.data(), g->pr_stats, st.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  for (int it = 0; it < iters; it++) {
+    if (dh) dh[it] = st[2 * it];
+    if (lh) lh[it] = st[2 * (it + 1) + 1];
+  }
+  if (rank != g->rank_a) {
+    std::swap(g->rank_a, g->rank_b);
+    std::swap(g->contrib_a, g->contrib_b);
+  }
+  (void)contrib;
+  return EG_OK;
+}
+
 extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, double *dh,
                            double *lh) {
   EG_TRY(eg_require_loaded(g));
@@ -509,23 +587,12 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
   const uint32_t *ptr, *idx;
   const float *w;
   PullPlan *pl;
-  EG_TRY(pull_orientation(g, &ptr, &idx, &w, &pl));
-  EG_TRY(ensure_pr_state(g, iters));
+  EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  EG_TRY(eg_pr_state(g, iters));
   const uint32_t n = g->n;
-  hipStream_t s = g->stream;
   float *rank = g->rank_a, *rank_next = g->rank_b;
   float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
-  EG_HIP(hipMemsetAsync(g->bfs_ctr, 0, 4, s));
-  hipLaunchKernelGGL(k_pr_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (float)init,
-                     g->out_deg, rank, contrib, g->bfs_ctr);
-  EG_LAUNCHED();
-  EG_HIP(hipMemcpyAsync(g->h_ctr, g->bfs_ctr, 4, hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  // Dangling mass of the initial iterate: every sink holds `init`.
-  const double d0 = (double)g->h_ctr[0] * (double)(float)init;
-  EG_HIP(hipMemcpyAsync(g->pr_stats, &d0, sizeof(double), hipMemcpyHostToDevice, s));
-  EG_HIP(hipStreamSynchronize(s));
-
+  EG_TRY(eg_pr_start(g, init, rank, contrib));
   PrArgs a;
   a.out_deg = g->out_deg;
   a.base = (float)((1.0 - damping) / (double)n);
@@ -541,22 +608,7 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
     std::swap(rank, rank_next);
     std::swap(contrib, contrib_next);
   }
-  if (rank_out) EG_HIP(hipMemcpyAsync(rank_out, rank, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-  std::vector<double> st(2 * ((size_t)iters + 1));
-  if ((dh || lh) && iters > 0)
-    EG_HIP(hipMemcpyAsync(st.data(), g->pr_stats, st.size() * sizeof(double),
-                          hipMemcpyDeviceToHost, s));
-  EG_HIP(hipStreamSynchronize(s));
-  for (int it = 0; it < iters; it++) {
-    if (dh) dh[it] = st[2 * it];
-    if (lh) lh[it] = st[2 * (it + 1) + 1];
-  }
-  // Keep the final iterate where later device-side users find it.
-  if (rank != g->rank_a) {
-    std::swap(g->rank_a, g->rank_b);
-    std::swap(g->contrib_a, g->contrib_b);
-  }
-  return EG_OK;
+  return eg_pr_finish(g, iters, rank, contrib, rank_out, dh, lh);
 }
 
 extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
@@ -570,8 +622,8 @@ extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
   const uint32_t *ptr, *idx;
   const float *w;
   PullPlan *pl;
-  EG_TRY(pull_orientation(g, &ptr, &idx, &w, &pl));
-  EG_TRY(ensure_pr_state(g, 0));
+  EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  EG_TRY(eg_pr_state(g, 0));
   const uint32_t n = g->n;
   hipStream_t s = g->stream;
   EG_HIP(hipMemcpyAsync(g->contrib_a, x, (size_t)n * 4, hipMemcpyHostToDevice, s));
@@ -580,15 +632,4 @@ extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
   if (y) EG_HIP(hipMemcpyAsync(y, g->rank_a, (size_t)n * 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
   return EG_OK;
-}
-
-// Device-resident entry used by the batched form and the benchmark: one SpMV
-// pass over x already in HBM.
-int eg_spmv_device(eg_graph *g, const float *x, float *y) {
-  const uint32_t *ptr, *idx;
-  const float *w;
-  PullPlan *pl;
-  EG_TRY(pull_orientation(g, &ptr, &idx, &w, &pl));
-  PrArgs a = {};
-  return launch_pull(g, false, ptr, idx, w, x, y, *pl, a);
 }

@@ -46,7 +46,9 @@ __global__ void k_wcc_relax(const uint32_t *__restrict__ src, const uint32_t *__
       ch |= atomicMin(&L[lu], lv) > lv;
     }
   }
-  if (__ballot(ch) && __lane_id() == 0) atomicOr(changed, 1u);
+  // one flag per round: only the first waves to see it clear write it
+  if (__ballot(ch) && __lane_id() == 0 && __atomic_load_n(changed, __ATOMIC_RELAXED) == 0u)
+    atomicOr(changed, 1u);
 }
 
 __global__ void k_wcc_shortcut(uint32_t n, uint32_t *__restrict__ L,
@@ -63,7 +65,9 @@ __global__ void k_wcc_shortcut(uint32_t n, uint32_t *__restrict__ L,
     }
     ch = atomicMin(&L[v], l) > l;
   }
-  if (__ballot(ch) && __lane_id() == 0) atomicOr(changed, 1u);
+  // one flag per round: only the first waves to see it clear write it
+  if (__ballot(ch) && __lane_id() == 0 && __atomic_load_n(changed, __ATOMIC_RELAXED) == 0u)
+    atomicOr(changed, 1u);
 }
 
 __global__ void k_widen(uint32_t n, const uint32_t *__restrict__ L, int64_t *__restrict__ out) {
