@@ -1,0 +1,80 @@
+// eg_batch.h -- multi-source state and step functions shared by the batched
+// API (eg_bfs_batch, eg_ppr_batch) and the serving engine.
+#pragma once
+
+#include "eg_internal.h"
+
+// Pull orientation of the batched kernels: the degree-ordered copy when
+// enabled (perm = new id of every vertex, deg = out-degree in new ids), else
+// the in-edges as built.  sym: the orientation also lists the out-edges
+// (symmetric graph), which allows push levels.
+struct BatchView {
+  const uint32_t *ptr, *idx, *deg, *perm;
+  PullPlan *pl;
+  bool sym;
+};
+int eg_batch_view(eg_graph *g, BatchView *v);
+
+// Up to 64 BFS lanes over one traversal: lane i of vertex v is bit i of the
+// 64-bit masks.  Levels are u16, vertex-major (lvl[v * 64 + lane]), root = 1.
+struct MsState {
+  hipStream_t stream = nullptr;  // every step is enqueued here
+  void *scan_tmp = nullptr;      // frontier degree scan workspace
+  uint64_t *seen = nullptr, *visit = nullptr, *nvisit = nullptr, *pend = nullptr;
+  uint16_t *lvl = nullptr;
+  uint64_t *seg_or = nullptr;  // [nseg] long-row partial ORs
+  uint32_t seg_cap = 0;
+  uint32_t *queue = nullptr, *qoff = nullptr;  // frontier vertices with edges, degree scan
+  unsigned long long *ctr = nullptr;           // [0] progress mask, [1] qlen, [2] degsum
+  uint16_t *lvl_tab = nullptr;                 // [64] level written per lane this step
+  uint32_t *roots = nullptr;                   // [64]
+  unsigned long long *h = nullptr;             // pinned mirror of ctr
+  uint32_t qlen = 0;                           // frontier queue length (host copy)
+  uint64_t qdeg = 0;                           // its out-degree sum
+};
+int eg_ms_alloc(eg_graph *g, MsState *st);
+void eg_ms_free(MsState *st);
+// Clears every lane.
+int eg_ms_reset(eg_graph *g, MsState *st);
+// Starts lane[i] at roots[i] (vertex ids of the layout), i < k.
+int eg_ms_seed(eg_graph *g, const BatchView &bv, MsState *st, uint32_t k, const uint32_t *lanes,
+               const uint32_t *roots);
+// One level for the lanes in `active`; tab[i] is the level value written for
+// lane i.  *progress = lanes that reached a new vertex.
+int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
+                const uint16_t *tab, uint64_t *progress);
+// out[v] (device, n words) = level of lane for vertex v of the layout.
+int eg_ms_extract_lane(eg_graph *g, const BatchView &bv, MsState *st, uint32_t lane,
+                       uint32_t *out);
+int eg_ms_clear_lane(eg_graph *g, MsState *st, uint32_t lane);
+
+// Up to 64 personalized PageRank columns (FP32, row-major, 64 per vertex).
+struct PprState {
+  hipStream_t stream = nullptr;
+  float *X = nullptr, *Y = nullptr, *C = nullptr, *Cn = nullptr;
+  uint64_t *mask = nullptr;  // seed columns of every vertex
+  float *w = nullptr;        // [128]: 1/|S_q| and alpha/|S_q| per column
+  float2 *seg_part = nullptr;
+  uint64_t seg_cap = 0;
+  uint32_t *seeds = nullptr;  // staging for column initialisation
+  uint32_t seeds_cap = 0;
+  float hw[128];              // host copy of w
+  double *dpart = nullptr;    // per-workgroup L1 delta partials [blocks][64]
+  uint64_t dpart_cap = 0;
+  double *delta = nullptr;    // [64] L1 change of every column in the last step
+  double *h_delta = nullptr;  // pinned mirror
+};
+int eg_ppr_alloc(eg_graph *g, PprState *st);
+void eg_ppr_free(PprState *st);
+int eg_ppr_reset(eg_graph *g, PprState *st);
+// Column q starts personalized at seeds[0..ns) (vertex ids of the layout).
+int eg_ppr_col_init(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q,
+                    const uint32_t *seeds, uint32_t ns, double alpha);
+// One iteration x' = alpha p + (1-alpha) A^T D^-1 x for all 64 columns,
+// enqueued on st->stream.  With want_delta, the L1 change of every column is
+// reduced in a fixed order and copied to st->h_delta (valid once the stream
+// reaches the end of the step).
+int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha,
+                bool want_delta = false);
+int eg_ppr_extract_col(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q, float *out);
+int eg_ppr_clear_col(eg_graph *g, PprState *st, uint32_t q);

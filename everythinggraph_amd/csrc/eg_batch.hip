@@ -4,121 +4,419 @@
 // pull PageRank recurrence (pagerank_simple.c:72-84) with a personalization
 // vector.
 //
-// MS-BFS: up to 64 sources share one traversal.  Every vertex carries three
-// 64-bit lane masks (seen, frontier, next frontier); a level is one pull
-// sweep in which vertex v ORs the frontier masks of its in-neighbours and
-// stops as soon as every lane it has not seen is covered.  Levels of newly
-// reached (vertex, lane) pairs are written vertex-major as u16 (root = 1).
+// MS-BFS: up to 64 sources share one traversal.  Every vertex carries 64-bit
+// lane masks (seen, frontier, next frontier).  A level is either
+//  * a pull sweep over the plan of the pull orientation (the chunk / segment
+//    decomposition of eg_pull.hip): a row ORs the frontier masks of its
+//    in-neighbours, gathered eight at a time into LDS, or
+//  * on symmetric graphs with a small frontier, a push over the frontier's
+//    edges (one workgroup per 2048-edge slice, owners found by binary search
+//    in LDS) into a pending mask, followed by an apply pass.
+// The direction follows the reference BFS rule (bfs_simple.c:191-197) on the
+// frontier of all lanes.  Levels of newly reached (vertex, lane) pairs are
+// written vertex-major as u16 from a per-lane table, so lanes that started at
+// different steps (the serving engine) keep their own level numbering.
 //
 // PPR: k <= 64 vectors stored row-major (64 FP32 columns per vertex).  One
 // wave owns a row: it loads 64 in-edge ids with one coalesced read, then for
 // each broadcasts the id and every lane adds its column of the neighbour's
 // contribution row (a coalesced 256-byte gather) with Neumaier compensation.
-// The seed term is added from a per-vertex 64-bit seed mask.
+// Long rows are split into segments as in eg_pull.hip.  The seed term is
+// added from a per-vertex 64-bit seed mask.
 #include <math.h>
 #include <string.h>
 
-#include "eg_internal.h"
+#include "eg_batch.h"
 
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr uint32_t kChunk = kPlanChunk;
+constexpr uint32_t kLds = kPlanChunk + kPlanLongRow;
+constexpr uint32_t kSerialRow = 32;
+constexpr uint32_t kEdgesPerBlock = 2048;
+constexpr uint32_t kStatBlocks = 1024;
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
 }
 
-__global__ void k_ms_init(uint32_t n, uint64_t *__restrict__ seen, uint64_t *__restrict__ visit,
-                          uint16_t *__restrict__ lvl) {
+__device__ __forceinline__ uint64_t wave_or(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+// Adds this workgroup's new lanes to ctr[0] (only bits not yet recorded).
+__device__ __forceinline__ void publish_progress(uint64_t nw, uint64_t *s_or,
+                                                 unsigned long long *ctr) {
+  nw = wave_or(nw);
+  if (__lane_id() == 0) s_or[threadIdx.x >> 6] = nw;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t b = 0;
+    for (int w = 0; w < kWaves; w++) b |= s_or[w];
+    const uint64_t known = __atomic_load_n(&ctr[0], __ATOMIC_RELAXED);
+    if (b & ~known) atomicOr(&ctr[0], (unsigned long long)b);
+  }
+}
+
+__device__ __forceinline__ uint64_t finish_ms_row(uint32_t r, uint64_t acc, uint64_t active,
+                                                  uint64_t *__restrict__ seen,
+                                                  uint64_t *__restrict__ nvisit,
+                                                  uint16_t *__restrict__ lvl, const uint16_t *tab) {
+  const uint64_t s = seen[r];
+  const uint64_t nw = acc & ~s & active;
+  nvisit[r] = nw;
+  if (nw) {
+    seen[r] = s | nw;
+    uint64_t b = nw;
+    while (b) {
+      const int i = __ffsll((unsigned long long)b) - 1;
+      lvl[(uint64_t)r * 64 + i] = tab[i];
+      b &= b - 1;
+    }
+  }
+  return nw;
+}
+
+__global__ void k_ms_reset(uint32_t n, uint64_t *__restrict__ seen, uint64_t *__restrict__ visit,
+                           uint64_t *__restrict__ nvisit, uint64_t *__restrict__ pend,
+                           uint16_t *__restrict__ lvl) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     seen[i] = 0;
     visit[i] = 0;
+    nvisit[i] = 0;
+    pend[i] = 0;
   }
   if (i < (uint64_t)n * 16) reinterpret_cast<uint64_t *>(lvl)[i] = 0;  // 64 u16 = 16 u64
 }
 
-__global__ void k_ms_seed(const uint32_t *__restrict__ roots, uint32_t k,
+__global__ void k_ms_seed(uint32_t k, const uint32_t *__restrict__ lanes,
+                          const uint32_t *__restrict__ roots, const uint32_t *__restrict__ perm,
                           uint64_t *__restrict__ seen, uint64_t *__restrict__ visit,
                           uint16_t *__restrict__ lvl) {
   const uint32_t i = threadIdx.x;
   if (i >= k) return;
-  const uint32_t r = roots[i];
-  atomicOr(reinterpret_cast<unsigned long long *>(&seen[r]), 1ull << i);
-  atomicOr(reinterpret_cast<unsigned long long *>(&visit[r]), 1ull << i);
-  lvl[(uint64_t)r * 64 + i] = 1;
+  const uint32_t r = perm ? perm[roots[i]] : roots[i];
+  const uint32_t lane = lanes[i];
+  atomicOr(reinterpret_cast<unsigned long long *>(&seen[r]), 1ull << lane);
+  atomicOr(reinterpret_cast<unsigned long long *>(&visit[r]), 1ull << lane);
+  lvl[(uint64_t)r * 64 + lane] = 1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_ms_pull(uint32_t n, const uint32_t *__restrict__ ptr,
-                                                    const uint32_t *__restrict__ idx,
-                                                    uint64_t *__restrict__ seen,
-                                                    const uint64_t *__restrict__ visit,
-                                                    uint64_t *__restrict__ nvisit,
-                                                    uint16_t *__restrict__ lvl, uint64_t all,
-                                                    uint16_t newlvl, uint32_t *__restrict__ cnt) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t nw = 0;
-  if (v < n) {
-    const uint64_t s = seen[v];
-    if (s != all) {
-      uint64_t acc = 0;
-      const uint32_t k1 = ptr[v + 1];
-      for (uint32_t k = ptr[v]; k < k1; k += 4) {
-        // four neighbours per step, then the early-exit test
-        uint32_t u[4];
+// Pull level over the plan: segments of long rows OR into seg_or; chunks of
+// short rows finish their rows directly.
+__global__ __launch_bounds__(kBlock) void k_ms_pull(
+    const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
+    const uint4 *__restrict__ chunk_info, const uint4 *__restrict__ seg_info, uint32_t nseg,
+    uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
+    uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
+    uint64_t active, unsigned long long *__restrict__ ctr) {
+  __shared__ uint64_t s_val[kLds];
+  __shared__ uint32_t s_ptr[kChunk + 1];
+  __shared__ uint32_t s_list[kChunk];
+  __shared__ uint32_t s_nlist;
+  __shared__ uint16_t s_tab[64];
+  __shared__ uint64_t s_or[kWaves];
+  const uint32_t tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  if (blockIdx.x < nseg) {
+    const uint4 si = seg_info[blockIdx.x];
+    uint64_t acc = 0;
+    for (uint32_t base = si.x; base < si.y; base += kBlock * 8) {
+      uint32_t u[8];
 #pragma unroll
-        for (int t = 0; t < 4; t++) u[t] = k + t < k1 ? idx[k + t] : idx[k];
+      for (int q = 0; q < 8; q++) {
+        const uint32_t k = base + tid + q * kBlock;
+        u[q] = k < si.y ? __builtin_nontemporal_load(idx + k) : idx[si.x];
+      }
 #pragma unroll
-        for (int t = 0; t < 4; t++) acc |= visit[u[t]];
-        if ((acc | s) == all) break;
-      }
-      nw = acc & ~s;
-      if (nw) {
-        seen[v] = s | nw;
-        uint64_t b = nw;
-        while (b) {
-          const int i = __ffsll((unsigned long long)b) - 1;
-          lvl[(uint64_t)v * 64 + i] = newlvl;
-          b &= b - 1;
-        }
-      }
+      for (int q = 0; q < 8; q++) acc |= visit[u[q]];
     }
-    nvisit[v] = nw;
+    acc = wave_or(acc);
+    if (lane == 0) s_or[wave] = acc;
+    __syncthreads();
+    if (tid == 0) seg_or[blockIdx.x] = s_or[0] | s_or[1] | s_or[2] | s_or[3];
+    return;
   }
-  // "some vertex was reached": only the first waves to see the flag clear write it
-  if (__ballot(nw != 0) && __lane_id() == 0 && __atomic_load_n(cnt, __ATOMIC_RELAXED) == 0u)
-    atomicOr(cnt, 1u);
+  const uint4 ci = chunk_info[blockIdx.x - nseg];
+  const uint32_t r0 = ci.x, r1 = ci.y, k0 = ci.z, k1 = ci.w;
+  const uint32_t nrows = r1 - r0;
+  if (tid == 0) s_nlist = 0;
+  if (tid < 64) s_tab[tid] = tab[tid];
+  for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = ptr[r0 + i] - k0;
+  for (uint32_t base = k0; base < k1; base += kBlock * 8) {
+    uint32_t u[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint32_t k = base + tid + q * kBlock;
+      u[q] = k < k1 ? __builtin_nontemporal_load(idx + k) : 0u;
+    }
+    uint64_t v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = visit[u[q]];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint32_t k = base + tid + q * kBlock;
+      if (k < k1) s_val[k - k0] = v[q];
+    }
+  }
+  __syncthreads();
+  uint64_t nwall = 0;
+  for (uint32_t i = tid; i < nrows; i += kBlock) {
+    const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
+    if (rb - ra <= kSerialRow) {
+      uint64_t acc = 0;
+      for (uint32_t q = ra; q < rb; q++) acc |= s_val[q];
+      nwall |= finish_ms_row(r0 + i, acc, active, seen, nvisit, lvl, s_tab);
+    } else {
+      s_list[atomicAdd(&s_nlist, 1u)] = i;
+    }
+  }
+  __syncthreads();
+  const uint32_t nlist = s_nlist;
+  for (uint32_t li = wave; li < nlist; li += kWaves) {
+    const uint32_t i = s_list[li];
+    uint64_t acc = 0;
+    for (uint32_t q = s_ptr[i] + lane; q < s_ptr[i + 1]; q += 64) acc |= s_val[q];
+    acc = wave_or(acc);
+    if (lane == 0) nwall |= finish_ms_row(r0 + i, acc, active, seen, nvisit, lvl, s_tab);
+  }
+  publish_progress(nwall, s_or, ctr);
 }
 
-__global__ void k_ms_extract(uint32_t n, uint32_t k, const uint16_t *__restrict__ lvl,
+__global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
+    const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
+    const uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen,
+    uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
+    uint64_t active, unsigned long long *__restrict__ ctr) {
+  __shared__ uint64_t s_or[kWaves];
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t nw = 0;
+  if (j < nlong) {
+    uint64_t acc = 0;
+    for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++) acc |= seg_or[s];
+    nw = finish_ms_row(long_rows[j], acc, active, seen, nvisit, lvl, tab);
+  }
+  publish_progress(nw, s_or, ctr);
+}
+
+// Frontier statistics of visit: [1] vertices with out-edges, [2] their degree
+// sum (per-workgroup partials, summed by k_ms_stats_sum).
+__global__ __launch_bounds__(kBlock) void k_ms_stats(uint32_t n, const uint64_t *__restrict__ visit,
+                                                     const uint32_t *__restrict__ ptr,
+                                                     unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long s_c[kWaves], s_d[kWaves];
+  unsigned long long c = 0, d = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    if (visit[v]) {
+      const uint32_t dv = ptr[v + 1] - ptr[v];
+      c += dv > 0;
+      d += dv;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    c += __shfl_xor(c, o);
+    d += __shfl_xor(d, o);
+  }
+  if (__lane_id() == 0) {
+    s_c[threadIdx.x >> 6] = c;
+    s_d[threadIdx.x >> 6] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    part[2 * blockIdx.x + 1] = s_d[0] + s_d[1] + s_d[2] + s_d[3];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ms_stats_sum(const unsigned long long *__restrict__ part,
+                                                         uint32_t np,
+                                                         unsigned long long *__restrict__ ctr) {
+  __shared__ unsigned long long s_c[kWaves], s_d[kWaves];
+  unsigned long long c = 0, d = 0;
+  for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+    c += part[2 * i];
+    d += part[2 * i + 1];
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    c += __shfl_xor(c, o);
+    d += __shfl_xor(d, o);
+  }
+  if (__lane_id() == 0) {
+    s_c[threadIdx.x >> 6] = c;
+    s_d[threadIdx.x >> 6] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ctr[1] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    ctr[2] = s_d[0] + s_d[1] + s_d[2] + s_d[3];
+  }
+}
+
+// Frontier vertices with out-edges -> queue (run only for small frontiers:
+// one atomic per workgroup that holds some).
+__global__ __launch_bounds__(kBlock) void k_ms_compact(uint32_t n, const uint64_t *__restrict__ visit,
+                                                       const uint32_t *__restrict__ ptr,
+                                                       uint32_t *__restrict__ q,
+                                                       unsigned long long *__restrict__ qlen) {
+  __shared__ uint32_t s_q[kBlock];
+  __shared__ uint32_t s_n, s_base;
+  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const uint32_t v = b0 + threadIdx.x;
+    if (v < n && visit[v] && ptr[v + 1] > ptr[v]) s_q[atomicAdd(&s_n, 1u)] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) s_base = (uint32_t)atomicAdd(qlen, (unsigned long long)s_n);
+    __syncthreads();
+    if (threadIdx.x < s_n) q[s_base + threadIdx.x] = s_q[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+__global__ void k_ms_qdeg(const uint32_t *__restrict__ q, uint32_t qn,
+                          const uint32_t *__restrict__ ptr, uint32_t *__restrict__ deg) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < qn) deg[i] = ptr[q[i] + 1] - ptr[q[i]];
+}
+
+// Push level: the frontier's edges [b*2048, (b+1)*2048) of the degree scan.
+__global__ __launch_bounds__(kBlock) void k_ms_push(
+    const uint32_t *__restrict__ q, uint32_t qn, const uint32_t *__restrict__ qoff,
+    const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
+    const uint64_t *__restrict__ visit, const uint64_t *__restrict__ seen,
+    uint64_t *__restrict__ pend, uint64_t active) {
+  __shared__ uint32_t s_off[kEdgesPerBlock + 1];
+  __shared__ uint32_t s_u[kEdgesPerBlock + 1];
+  __shared__ uint64_t s_vis[kEdgesPerBlock + 1];
+  __shared__ uint32_t s_first, s_cnt;
+  const uint32_t e0 = blockIdx.x * kEdgesPerBlock;
+  const uint32_t total = qoff[qn];
+  const uint32_t e1 = e0 + kEdgesPerBlock < total ? e0 + kEdgesPerBlock : total;
+  if (threadIdx.x == 0) {
+    uint32_t lo = 0, hi = qn;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (qoff[mid] <= e0) lo = mid; else hi = mid;
+    }
+    uint32_t a = lo, b = qn;
+    while (a < b) {
+      const uint32_t mid = a + (b - a) / 2;
+      if (qoff[mid] < e1) a = mid + 1; else b = mid;
+    }
+    s_first = lo;
+    s_cnt = a - lo;
+  }
+  __syncthreads();
+  const uint32_t cnt = s_cnt <= kEdgesPerBlock + 1 ? s_cnt : kEdgesPerBlock + 1;
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+    s_off[i] = qoff[s_first + i];
+    const uint32_t u = q[s_first + i];
+    s_u[i] = u;
+    s_vis[i] = visit[u] & active;
+  }
+  __syncthreads();
+  for (uint32_t base = e0; base < e1; base += kBlock) {
+    const uint32_t e = base + threadIdx.x;
+    if (e < e1) {
+      uint32_t lo = 0, hi = cnt;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_off[mid] <= e) lo = mid; else hi = mid;
+      }
+      const uint32_t w = idx[ptr[s_u[lo]] + (e - s_off[lo])];
+      const uint64_t bits = s_vis[lo] & ~__atomic_load_n(&seen[w], __ATOMIC_RELAXED);
+      if (bits && (__atomic_load_n(&pend[w], __ATOMIC_RELAXED) & bits) != bits)
+        atomicOr(reinterpret_cast<unsigned long long *>(&pend[w]), (unsigned long long)bits);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, uint64_t *__restrict__ pend,
+                                                     uint64_t *__restrict__ seen,
+                                                     uint64_t *__restrict__ nvisit,
+                                                     uint16_t *__restrict__ lvl,
+                                                     const uint16_t *__restrict__ tab,
+                                                     uint64_t active,
+                                                     unsigned long long *__restrict__ ctr) {
+  __shared__ uint64_t s_or[kWaves];
+  __shared__ uint16_t s_tab[64];
+  if (threadIdx.x < 64) s_tab[threadIdx.x] = tab[threadIdx.x];
+  __syncthreads();
+  uint64_t all = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint64_t p = pend[v];
+    uint64_t nw = 0;
+    if (p) {
+      pend[v] = 0;
+      nw = finish_ms_row(v, p, active, seen, nvisit, lvl, s_tab);
+    } else {
+      nvisit[v] = 0;
+    }
+    all |= nw;
+  }
+  publish_progress(all, s_or, ctr);
+}
+
+__global__ void k_ms_extract(uint32_t n, uint32_t k, uint32_t lane0,
+                             const uint16_t *__restrict__ lvl, const uint32_t *__restrict__ perm,
                              uint32_t *__restrict__ out) {
-  // out[i * n + v] = lvl[v * 64 + i]
+  // out[i * n + v] = lvl[perm(v) * 64 + lane0 + i]
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (uint64_t)n * k) return;
   const uint32_t i = (uint32_t)(t / n), v = (uint32_t)(t % n);
-  out[t] = lvl[(uint64_t)v * 64 + i];
+  const uint32_t vv = perm ? perm[v] : v;
+  out[t] = lvl[(uint64_t)vv * 64 + lane0 + i];
+}
+
+__global__ void k_ms_clear_lane(uint32_t n, uint32_t lane, uint64_t *__restrict__ seen,
+                                uint64_t *__restrict__ visit, uint64_t *__restrict__ nvisit,
+                                uint16_t *__restrict__ lvl) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint64_t keep = ~(1ull << lane);
+  seen[v] &= keep;
+  visit[v] &= keep;
+  nvisit[v] &= keep;
+  lvl[(uint64_t)v * 64 + lane] = 0;
 }
 
 // ---- PPR ----
-__global__ void k_ppr_seedmask(uint32_t k, const uint32_t *__restrict__ seed_ptr,
-                               const uint32_t *__restrict__ seeds, uint64_t *__restrict__ mask) {
-  const uint32_t q = threadIdx.x;
-  if (q >= k) return;
-  for (uint32_t j = seed_ptr[q]; j < seed_ptr[q + 1]; j++)
-    atomicOr(reinterpret_cast<unsigned long long *>(&mask[seeds[j]]), 1ull << q);
+__global__ void k_ppr_zero(uint64_t count, float *__restrict__ X, float *__restrict__ C) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) {
+    X[t] = 0.f;
+    C[t] = 0.f;
+  }
 }
 
-// x0 = p: X[v][q] = 1/|S_q| for seeds, contributions X/outdeg.
-__global__ void k_ppr_init(uint32_t n, const uint64_t *__restrict__ mask,
-                           const float *__restrict__ inv_seeds, const uint32_t *__restrict__ deg,
-                           float *__restrict__ X, float *__restrict__ C) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (uint64_t)n * 64) return;
-  const uint32_t v = (uint32_t)(t >> 6), q = (uint32_t)(t & 63);
-  const float x = ((mask[v] >> q) & 1ull) ? inv_seeds[q] : 0.f;
-  X[t] = x;
+__global__ void k_ppr_col_seed(uint32_t q, const uint32_t *__restrict__ seeds, uint32_t ns,
+                               const uint32_t *__restrict__ perm, const float *__restrict__ w,
+                               const uint32_t *__restrict__ deg, uint64_t *__restrict__ mask,
+                               float *__restrict__ X, float *__restrict__ C) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const uint32_t v = perm ? perm[seeds[i]] : seeds[i];
+  atomicOr(reinterpret_cast<unsigned long long *>(&mask[v]), 1ull << q);
+  X[(uint64_t)v * 64 + q] = w[q];
   const uint32_t d = deg[v];
-  C[t] = d ? x / (float)d : 0.f;
+  C[(uint64_t)v * 64 + q] = d ? w[q] / (float)d : 0.f;
+}
+
+__global__ void k_ppr_col_clear(uint32_t n, uint32_t q, uint64_t *__restrict__ mask,
+                                float *__restrict__ X, float *__restrict__ C) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  mask[v] &= ~(1ull << q);
+  X[(uint64_t)v * 64 + q] = 0.f;
+  C[(uint64_t)v * 64 + q] = 0.f;
 }
 
 // Adds the contribution rows of entries [k0, k1) to this lane's column.
@@ -143,34 +441,56 @@ __device__ __forceinline__ void spmm_range(const uint32_t *__restrict__ idx,
   }
 }
 
-__device__ __forceinline__ void spmm_finish(uint32_t v, float sum, int lane,
-                                            const uint64_t *__restrict__ mask, float seed_term,
-                                            const uint32_t *__restrict__ deg, float one_minus_a,
-                                            float *__restrict__ Y, float *__restrict__ Cn) {
+// Writes row v of Y and Cn; returns |y - x| (x = the row of X) when DELTA.
+template <bool DELTA>
+__device__ __forceinline__ double spmm_finish(uint32_t v, float sum, int lane,
+                                              const uint64_t *__restrict__ mask, float seed_term,
+                                              const uint32_t *__restrict__ deg, float one_minus_a,
+                                              const float *__restrict__ X, float *__restrict__ Y,
+                                              float *__restrict__ Cn) {
   float y = one_minus_a * sum;
   if ((mask[v] >> lane) & 1ull) y += seed_term;
   Y[(uint64_t)v * 64 + lane] = y;
   const uint32_t d = deg[v];
   Cn[(uint64_t)v * 64 + lane] = d ? y / (float)d : 0.f;
+  if (!DELTA) return 0.0;
+  return fabs((double)y - (double)X[(uint64_t)v * 64 + lane]);
+}
+
+// Column sums of this workgroup's per-wave deltas, merged in wave order.
+template <bool DELTA>
+__device__ __forceinline__ void store_delta(double d, double *__restrict__ dpart, uint32_t slot) {
+  if (!DELTA) return;
+  __shared__ double s_d[kWaves][64];
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  s_d[wave][lane] = d;
+  __syncthreads();
+  if (wave == 0) {
+    double t = s_d[0][lane];
+    for (int q = 1; q < kWaves; q++) t += s_d[q][lane];
+    dpart[(uint64_t)slot * 64 + lane] = t;
+  }
 }
 
 // Work items of the pull plan: segments of long rows (the four waves split
 // the segment, partials merged in wave order) and chunks of short rows (a
 // wave per row).  Lane q owns query column q.
+template <bool DELTA>
 __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
     const uint32_t *__restrict__ idx, const uint4 *__restrict__ chunk_info,
     const uint4 *__restrict__ seg_info, uint32_t nseg, float2 *__restrict__ seg_part,
     const uint32_t *__restrict__ ptr, const float *__restrict__ C,
     const uint64_t *__restrict__ mask, const float *__restrict__ alpha_inv_seeds,
-    const uint32_t *__restrict__ deg, float one_minus_a, float *__restrict__ Y,
-    float *__restrict__ Cn) {
-  __shared__ float2 s_p[kBlock / 64][64];
+    const uint32_t *__restrict__ deg, float one_minus_a, const float *__restrict__ X,
+    float *__restrict__ Y, float *__restrict__ Cn, double *__restrict__ dpart) {
+  __shared__ float2 s_p[kWaves][64];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   if (blockIdx.x < nseg) {
     const uint4 si = seg_info[blockIdx.x];
     const uint32_t len = si.y - si.x;
-    const uint32_t part = (len + 4 * 64 - 1) / (4 * 64) * 64;  // multiple of 64 per wave
+    const uint32_t part = (len + kWaves * 64 - 1) / (kWaves * 64) * 64;  // per wave, x64
     const uint32_t a = si.x + wave * part < si.y ? si.x + wave * part : si.y;
     const uint32_t b = a + part < si.y ? a + part : si.y;
     EgKahan acc;
@@ -181,83 +501,427 @@ __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
       EgKahan t;
       t.s = s_p[0][lane].x;
       t.c = s_p[0][lane].y;
-      for (int q = 1; q < kBlock / 64; q++) t.merge(s_p[q][lane].x, s_p[q][lane].y);
+      for (int q = 1; q < kWaves; q++) t.merge(s_p[q][lane].x, s_p[q][lane].y);
       seg_part[(uint64_t)blockIdx.x * 64 + lane] = make_float2(t.s, t.c);
     }
     return;
   }
   const uint4 ci = chunk_info[blockIdx.x - nseg];
   const float seed_term = alpha_inv_seeds[lane];
-  for (uint32_t v = ci.x + wave; v < ci.y; v += kBlock / 64) {
+  double d = 0.0;
+  for (uint32_t v = ci.x + wave; v < ci.y; v += kWaves) {
     EgKahan acc;
     spmm_range(idx, C, ptr[v], ptr[v + 1], lane, acc);
-    spmm_finish(v, acc.value(), lane, mask, seed_term, deg, one_minus_a, Y, Cn);
+    d += spmm_finish<DELTA>(v, acc.value(), lane, mask, seed_term, deg, one_minus_a, X, Y, Cn);
   }
+  store_delta<DELTA>(d, dpart, blockIdx.x - nseg);
 }
 
 // Long rows: a wave per row merges its segment partials in segment order.
+// Delta partials go to slots [slot0, slot0 + gridDim.x).
+template <bool DELTA>
 __global__ __launch_bounds__(kBlock) void k_ppr_finish(
     const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
     const float2 *__restrict__ seg_part, const uint64_t *__restrict__ mask,
     const float *__restrict__ alpha_inv_seeds, const uint32_t *__restrict__ deg,
-    float one_minus_a, float *__restrict__ Y, float *__restrict__ Cn) {
+    float one_minus_a, const float *__restrict__ X, float *__restrict__ Y,
+    float *__restrict__ Cn, double *__restrict__ dpart, uint32_t slot0) {
   const int lane = __lane_id();
   const uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (j >= nlong) return;
-  EgKahan acc;
-  for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++) {
-    const float2 p = seg_part[(uint64_t)s * 64 + lane];
-    acc.merge(p.x, p.y);
+  double d = 0.0;
+  if (j < nlong) {
+    EgKahan acc;
+    for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++) {
+      const float2 p = seg_part[(uint64_t)s * 64 + lane];
+      acc.merge(p.x, p.y);
+    }
+    d = spmm_finish<DELTA>(long_rows[j], acc.value(), lane, mask, alpha_inv_seeds[lane], deg,
+                           one_minus_a, X, Y, Cn);
   }
-  spmm_finish(long_rows[j], acc.value(), lane, mask, alpha_inv_seeds[lane], deg, one_minus_a, Y,
-              Cn);
+  store_delta<DELTA>(d, dpart, slot0 + blockIdx.x);
 }
 
-__global__ void k_ppr_extract(uint32_t n, uint32_t k, const float *__restrict__ X,
-                              float *__restrict__ out) {
+// Column sums of dpart[0..np) in a fixed order: workgroup b sums a
+// contiguous slice (four quarter-slices, merged in order) into part2[b];
+// k_ppr_delta_final sums part2 in order.
+constexpr uint32_t kDeltaBlocks = 512;
+__global__ __launch_bounds__(kBlock) void k_ppr_delta_partial(const double *__restrict__ dpart,
+                                                              uint32_t np,
+                                                              double *__restrict__ part2) {
+  __shared__ double s_d[kWaves][64];
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  const uint32_t per = (np + kDeltaBlocks - 1) / kDeltaBlocks;
+  const uint32_t b0 = blockIdx.x * per < np ? blockIdx.x * per : np;
+  const uint32_t b1 = b0 + per < np ? b0 + per : np;
+  const uint32_t q = (b1 - b0 + kWaves - 1) / kWaves;
+  const uint32_t a = b0 + wave * q < b1 ? b0 + wave * q : b1;
+  const uint32_t e = a + q < b1 ? a + q : b1;
+  double t = 0.0;
+  for (uint32_t i = a; i < e; i++) t += dpart[(uint64_t)i * 64 + lane];
+  s_d[wave][lane] = t;
+  __syncthreads();
+  if (wave == 0) {
+    double u = s_d[0][lane];
+    for (int w = 1; w < kWaves; w++) u += s_d[w][lane];
+    part2[(uint64_t)blockIdx.x * 64 + lane] = u;
+  }
+}
+
+__global__ void k_ppr_delta_final(const double *__restrict__ part2, double *__restrict__ delta) {
+  const int lane = threadIdx.x;
+  double t = 0.0;
+  for (uint32_t b = 0; b < kDeltaBlocks; b++) t += part2[(uint64_t)b * 64 + lane];
+  delta[lane] = t;
+}
+
+__global__ void k_ppr_extract(uint32_t n, uint32_t k, uint32_t col0, const float *__restrict__ X,
+                              const uint32_t *__restrict__ perm, float *__restrict__ out) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (uint64_t)n * k) return;
   const uint32_t q = (uint32_t)(t / n), v = (uint32_t)(t % n);
-  out[t] = X[(uint64_t)v * 64 + q];
+  const uint32_t vv = perm ? perm[v] : v;
+  out[t] = X[(uint64_t)vv * 64 + col0 + q];
 }
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Views and state.
+// ---------------------------------------------------------------------------
+int eg_batch_view(eg_graph *g, BatchView *v) {
+  v->sym = (g->flags & (EG_UNDIRECTED | EG_SYMMETRIC)) != 0 || !g->in_ptr;
+  if (eg_reorder_enabled()) {
+    EgOrdered *o;
+    EG_TRY(eg_ordered(g, &o));
+    v->ptr = o->ptr;
+    v->idx = o->idx;
+    v->deg = o->deg;
+    v->perm = o->new_of_old;
+    v->pl = &o->plan;
+    return EG_OK;
+  }
+  const float *w;
+  EG_TRY(eg_pull_orientation(g, &v->ptr, &v->idx, &w, &v->pl));
+  v->deg = g->out_deg;
+  v->perm = nullptr;
+  return EG_OK;
+}
+
+int eg_ms_alloc(eg_graph *g, MsState *st) {
+  const uint32_t n = g->n;
+  EG_TRY(eg_dalloc_t(&st->seen, n, "ms seen"));
+  EG_TRY(eg_dalloc_t(&st->visit, n, "ms visit"));
+  EG_TRY(eg_dalloc_t(&st->nvisit, n, "ms next"));
+  EG_TRY(eg_dalloc_t(&st->pend, n, "ms pending"));
+  EG_TRY(eg_dalloc_t(&st->lvl, (uint64_t)n * 64, "ms levels"));
+  EG_TRY(eg_dalloc_t(&st->queue, (uint64_t)n + 64, "ms queue"));
+  EG_TRY(eg_dalloc_t(&st->qoff, (uint64_t)n + 65, "ms queue offsets"));
+  EG_TRY(eg_dalloc_t(&st->ctr, 2 * (uint64_t)kStatBlocks + 8, "ms counters"));
+  EG_TRY(eg_dalloc_t(&st->lvl_tab, 64, "ms level table"));
+  EG_TRY(eg_dalloc_t(&st->roots, 128, "ms roots"));
+  EG_TRY(eg_dalloc_t(reinterpret_cast<uint8_t **>(&st->scan_tmp),
+                     eg_scan_tmp_bytes((uint64_t)n + 64), "ms scan workspace"));
+  if (hipHostMalloc(reinterpret_cast<void **>(&st->h), 64, hipHostMallocDefault) != hipSuccess)
+    return eg_fail(EG_ENOMEM, "pinned counters");
+  return EG_OK;
+}
+
+void eg_ms_free(MsState *st) {
+  eg_dfree_t(&st->seen);
+  eg_dfree_t(&st->visit);
+  eg_dfree_t(&st->nvisit);
+  eg_dfree_t(&st->pend);
+  eg_dfree_t(&st->lvl);
+  eg_dfree_t(&st->seg_or);
+  eg_dfree_t(&st->queue);
+  eg_dfree_t(&st->qoff);
+  eg_dfree_t(&st->ctr);
+  eg_dfree_t(&st->lvl_tab);
+  eg_dfree_t(&st->roots);
+  uint8_t *t = reinterpret_cast<uint8_t *>(st->scan_tmp);
+  eg_dfree_t(&t);
+  st->scan_tmp = nullptr;
+  if (st->h) (void)hipHostFree(st->h);
+  st->h = nullptr;
+}
+
+int eg_ms_reset(eg_graph *g, MsState *st) {
+  const uint32_t n = g->n;
+  hipLaunchKernelGGL(k_ms_reset, dim3(blocks_for((uint64_t)n * 16, 256)), dim3(256), 0, st->stream,
+                     n, st->seen, st->visit, st->nvisit, st->pend, st->lvl);
+  EG_LAUNCHED();
+  st->qlen = 0;
+  st->qdeg = 0;
+  return EG_OK;
+}
+
+// Frontier of visit: statistics, and the queue when the next level may push.
+static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *push) {
+  const uint32_t n = g->n;
+  hipStream_t s = st->stream;
+  unsigned long long *part = st->ctr + 8;
+  const uint32_t nb = blocks_for(n, kBlock) < kStatBlocks ? blocks_for(n, kBlock) : kStatBlocks;
+  hipLaunchKernelGGL(k_ms_stats, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part);
+  EG_LAUNCHED();
+  hipLaunchKernelGGL(k_ms_stats_sum, dim3(1), dim3(kBlock), 0, s, part, nb, st->ctr);
+  EG_LAUNCHED();
+  EG_HIP(hipMemcpyAsync(st->h, st->ctr, 24, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  const uint64_t cnt = st->h[1], deg = st->h[2];
+  // bfs_simple.c:191: pull once the frontier's edges exceed m/20
+  *push = bv.sym && (double)(cnt + deg) <= (double)g->m / 20.0;
+  st->qdeg = deg;
+  st->qlen = 0;
+  if (*push && cnt) {
+    EG_HIP(hipMemsetAsync(st->ctr + 3, 0, 8, s));
+    hipLaunchKernelGGL(k_ms_compact, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, st->queue,
+                       st->ctr + 3);
+    EG_LAUNCHED();
+    st->qlen = (uint32_t)cnt;
+  }
+  return EG_OK;
+}
+
+int eg_ms_seed(eg_graph *g, const BatchView &bv, MsState *st, uint32_t k, const uint32_t *lanes,
+               const uint32_t *roots) {
+  if (k == 0) return EG_OK;
+  if (k > 64) return eg_fail(EG_EINVAL, "at most 64 lanes");
+  hipStream_t s = st->stream;
+  uint32_t h[128];
+  memcpy(h, lanes, k * 4);
+  memcpy(h + 64, roots, k * 4);
+  EG_HIP(hipMemcpyAsync(st->roots, h, sizeof(h), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, k, st->roots, st->roots + 64, bv.perm,
+                     st->seen, st->visit, st->lvl);
+  EG_LAUNCHED();
+  EG_HIP(hipStreamSynchronize(s));  // h is on the stack
+  return EG_OK;
+}
+
+int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
+                const uint16_t *tab, uint64_t *progress) {
+  const uint32_t n = g->n;
+  hipStream_t s = st->stream;
+  bool push = false;
+  EG_TRY(frontier_prep(g, bv, st, &push));
+  EG_HIP(hipMemcpyAsync(st->lvl_tab, tab, 128, hipMemcpyHostToDevice, s));
+  EG_HIP(hipMemsetAsync(st->ctr, 0, 8, s));
+  const PullPlan &pl = *bv.pl;
+  if (push) {
+    if (st->qlen && st->qdeg) {
+      hipLaunchKernelGGL(k_ms_qdeg, dim3(blocks_for(st->qlen, 256)), dim3(256), 0, s, st->queue,
+                         st->qlen, bv.ptr, st->qoff);
+      EG_LAUNCHED();
+      EG_TRY(eg_scan_excl(s, st->qoff, st->qoff, st->qlen, st->scan_tmp));
+      EgProf prof(g, "msbfs_push", s);
+      hipLaunchKernelGGL(k_ms_push, dim3(blocks_for(st->qdeg, kEdgesPerBlock)), dim3(kBlock), 0, s,
+                         st->queue, st->qlen, st->qoff, bv.ptr, bv.idx, st->visit, st->seen,
+                         st->pend, active);
+      EG_LAUNCHED();
+    }
+    const uint32_t nb = blocks_for(n, kBlock) < 4096 ? blocks_for(n, kBlock) : 4096;
+    hipLaunchKernelGGL(k_ms_apply, dim3(nb), dim3(kBlock), 0, s, n, st->pend, st->seen, st->nvisit,
+                       st->lvl, st->lvl_tab, active, st->ctr);
+    EG_LAUNCHED();
+  } else {
+    if (st->seg_cap < pl.nseg) {
+      eg_dfree_t(&st->seg_or);
+      EG_TRY(eg_dalloc_t(&st->seg_or, pl.nseg, "ms segment ors"));
+      st->seg_cap = pl.nseg;
+    }
+    {
+      EgProf prof(g, "msbfs_pull", s);
+      hipLaunchKernelGGL(k_ms_pull, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, bv.ptr, bv.idx,
+                         pl.chunk_info, pl.seg_info, pl.nseg, st->seg_or, st->seen, st->visit,
+                         st->nvisit, st->lvl, st->lvl_tab, active, st->ctr);
+      EG_LAUNCHED();
+    }
+    if (pl.nlong) {
+      hipLaunchKernelGGL(k_ms_pull_finish, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock), 0, s,
+                         pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen, st->nvisit,
+                         st->lvl, st->lvl_tab, active, st->ctr);
+      EG_LAUNCHED();
+    }
+  }
+  std::swap(st->visit, st->nvisit);
+  EG_HIP(hipMemcpyAsync(st->h, st->ctr, 8, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  *progress = st->h[0];
+  return EG_OK;
+}
+
+int eg_ms_extract_lane(eg_graph *g, const BatchView &bv, MsState *st, uint32_t lane,
+                       uint32_t *out) {
+  hipLaunchKernelGGL(k_ms_extract, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream, g->n, 1u,
+                     lane, st->lvl, bv.perm, out);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_ms_clear_lane(eg_graph *g, MsState *st, uint32_t lane) {
+  hipLaunchKernelGGL(k_ms_clear_lane, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream, g->n,
+                     lane, st->seen, st->visit, st->nvisit, st->lvl);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_ppr_alloc(eg_graph *g, PprState *st) {
+  const uint64_t nn = (uint64_t)g->n * 64;
+  EG_TRY(eg_dalloc_t(&st->X, nn, "ppr X"));
+  EG_TRY(eg_dalloc_t(&st->Y, nn, "ppr Y"));
+  EG_TRY(eg_dalloc_t(&st->C, nn, "ppr C"));
+  EG_TRY(eg_dalloc_t(&st->Cn, nn, "ppr Cn"));
+  EG_TRY(eg_dalloc_t(&st->mask, g->n, "ppr seed mask"));
+  EG_TRY(eg_dalloc_t(&st->w, 128, "ppr seed weights"));
+  EG_TRY(eg_dalloc_t(&st->delta, 64, "ppr deltas"));
+  if (hipHostMalloc(reinterpret_cast<void **>(&st->h_delta), 64 * sizeof(double),
+                    hipHostMallocDefault) != hipSuccess)
+    return eg_fail(EG_ENOMEM, "pinned deltas");
+  memset(st->hw, 0, sizeof(st->hw));
+  return EG_OK;
+}
+
+void eg_ppr_free(PprState *st) {
+  eg_dfree_t(&st->X);
+  eg_dfree_t(&st->Y);
+  eg_dfree_t(&st->C);
+  eg_dfree_t(&st->Cn);
+  eg_dfree_t(&st->mask);
+  eg_dfree_t(&st->w);
+  eg_dfree_t(&st->seg_part);
+  eg_dfree_t(&st->seeds);
+  eg_dfree_t(&st->dpart);
+  eg_dfree_t(&st->delta);
+  if (st->h_delta) (void)hipHostFree(st->h_delta);
+  st->h_delta = nullptr;
+}
+
+int eg_ppr_reset(eg_graph *g, PprState *st) {
+  const uint64_t nn = (uint64_t)g->n * 64;
+  hipLaunchKernelGGL(k_ppr_zero, dim3(blocks_for(nn, 256)), dim3(256), 0, st->stream, nn, st->X,
+                     st->C);
+  EG_LAUNCHED();
+  EG_HIP(hipMemsetAsync(st->mask, 0, (size_t)g->n * 8, st->stream));
+  memset(st->hw, 0, sizeof(st->hw));
+  return EG_OK;
+}
+
+int eg_ppr_col_init(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q,
+                    const uint32_t *seeds, uint32_t ns, double alpha) {
+  if (q >= 64 || ns == 0) return eg_fail(EG_EINVAL, "bad PPR column / seeds");
+  uint32_t distinct = 0;
+  for (uint32_t j = 0; j < ns; j++) {
+    bool dup = false;
+    for (uint32_t i = 0; i < j; i++) dup |= seeds[i] == seeds[j];
+    distinct += !dup;
+  }
+  st->hw[q] = (float)(1.0 / distinct);
+  st->hw[64 + q] = (float)(alpha / distinct);
+  hipStream_t s = st->stream;
+  if (st->seeds_cap < ns) {
+    EG_HIP(hipStreamSynchronize(s));
+    eg_dfree_t(&st->seeds);
+    EG_TRY(eg_dalloc_t(&st->seeds, ns, "ppr seeds"));
+    st->seeds_cap = ns;
+  }
+  EG_HIP(hipMemcpyAsync(st->seeds, seeds, (size_t)ns * 4, hipMemcpyHostToDevice, s));
+  EG_HIP(hipMemcpyAsync(st->w, st->hw, sizeof(st->hw), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_ppr_col_seed, dim3(blocks_for(ns, 64)), dim3(64), 0, s, q, st->seeds, ns,
+                     bv.perm, st->w, bv.deg, st->mask, st->X, st->C);
+  EG_LAUNCHED();
+  EG_HIP(hipStreamSynchronize(s));  // seeds / hw may be reused by the caller
+  return EG_OK;
+}
+
+int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bool want_delta) {
+  const PullPlan &pl = *bv.pl;
+  hipStream_t s = st->stream;
+  if (st->seg_cap < (uint64_t)pl.nseg * 64) {
+    EG_HIP(hipStreamSynchronize(s));
+    eg_dfree_t(&st->seg_part);
+    EG_TRY(eg_dalloc_t(&st->seg_part, (uint64_t)(pl.nseg ? pl.nseg : 1) * 64, "ppr partials"));
+    st->seg_cap = (uint64_t)(pl.nseg ? pl.nseg : 1) * 64;
+  }
+  const uint32_t nfin = pl.nlong ? blocks_for((uint64_t)pl.nlong * 64, kBlock) : 0;
+  const uint32_t np = pl.nchunks + nfin;
+  if (want_delta && st->dpart_cap < (uint64_t)np + kDeltaBlocks) {
+    EG_HIP(hipStreamSynchronize(s));
+    eg_dfree_t(&st->dpart);
+    EG_TRY(eg_dalloc_t(&st->dpart, ((uint64_t)np + kDeltaBlocks) * 64, "ppr delta partials"));
+    st->dpart_cap = (uint64_t)np + kDeltaBlocks;
+  }
+  const float oma = (float)(1.0 - alpha);
+  {
+    EgProf prof(g, "ppr_spmm", s);
+    if (want_delta)
+      hipLaunchKernelGGL(k_ppr_spmm<true>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, bv.idx,
+                         pl.chunk_info, pl.seg_info, pl.nseg, st->seg_part, bv.ptr, st->C,
+                         st->mask, st->w + 64, bv.deg, oma, st->X, st->Y, st->Cn, st->dpart);
+    else
+      hipLaunchKernelGGL(k_ppr_spmm<false>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, bv.idx,
+                         pl.chunk_info, pl.seg_info, pl.nseg, st->seg_part, bv.ptr, st->C,
+                         st->mask, st->w + 64, bv.deg, oma, st->X, st->Y, st->Cn, nullptr);
+    EG_LAUNCHED();
+  }
+  if (pl.nlong) {
+    if (want_delta)
+      hipLaunchKernelGGL(k_ppr_finish<true>, dim3(nfin), dim3(kBlock), 0, s, pl.long_rows,
+                         pl.seg_base, pl.nlong, st->seg_part, st->mask, st->w + 64, bv.deg, oma,
+                         st->X, st->Y, st->Cn, st->dpart, pl.nchunks);
+    else
+      hipLaunchKernelGGL(k_ppr_finish<false>, dim3(nfin), dim3(kBlock), 0, s, pl.long_rows,
+                         pl.seg_base, pl.nlong, st->seg_part, st->mask, st->w + 64, bv.deg, oma,
+                         st->X, st->Y, st->Cn, nullptr, 0u);
+    EG_LAUNCHED();
+  }
+  if (want_delta) {
+    double *part2 = st->dpart + (uint64_t)np * 64;
+    hipLaunchKernelGGL(k_ppr_delta_partial, dim3(kDeltaBlocks), dim3(kBlock), 0, s, st->dpart, np,
+                       part2);
+    EG_LAUNCHED();
+    hipLaunchKernelGGL(k_ppr_delta_final, dim3(1), dim3(64), 0, s, part2, st->delta);
+    EG_LAUNCHED();
+    EG_HIP(hipMemcpyAsync(st->h_delta, st->delta, 64 * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  std::swap(st->X, st->Y);
+  std::swap(st->C, st->Cn);
+  return EG_OK;
+}
+
+int eg_ppr_extract_col(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q, float *out) {
+  hipLaunchKernelGGL(k_ppr_extract, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream, g->n, 1u,
+                     q, st->X, bv.perm, out);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_ppr_clear_col(eg_graph *g, PprState *st, uint32_t q) {
+  hipLaunchKernelGGL(k_ppr_col_clear, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream, g->n,
+                     q, st->mask, st->X, st->C);
+  EG_LAUNCHED();
+  st->hw[q] = 0.f;
+  st->hw[64 + q] = 0.f;
+  return EG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The batched API.
+// ---------------------------------------------------------------------------
 struct BatchState {
-  uint64_t *seen = nullptr, *visit = nullptr, *nvisit = nullptr;
-  uint16_t *lvl = nullptr;
-  uint32_t *roots = nullptr;
+  MsState ms;
+  PprState ppr;
+  bool ms_ok = false, ppr_ok = false;
   uint32_t *ext = nullptr;  // k x n extraction buffer
   uint64_t ext_cap = 0;
-  float *X = nullptr, *Y = nullptr, *C = nullptr, *Cn = nullptr;
-  uint64_t *mask = nullptr;
-  float *inv = nullptr;
-  uint32_t *seed_ptr = nullptr, *seeds = nullptr;
-  uint32_t seeds_cap = 0;
-  uint32_t *cnt = nullptr;
-  float2 *seg_part = nullptr;  // [nseg * 64] long-row segment partials
-  uint64_t seg_part_cap = 0;
 };
 
 void eg_batch_free(eg_graph *g) {
   BatchState *b = g->batch;
   if (!b) return;
-  eg_dfree_t(&b->seen);
-  eg_dfree_t(&b->visit);
-  eg_dfree_t(&b->nvisit);
-  eg_dfree_t(&b->lvl);
-  eg_dfree_t(&b->roots);
+  eg_ms_free(&b->ms);
+  eg_ppr_free(&b->ppr);
   eg_dfree_t(&b->ext);
-  eg_dfree_t(&b->X);
-  eg_dfree_t(&b->Y);
-  eg_dfree_t(&b->C);
-  eg_dfree_t(&b->Cn);
-  eg_dfree_t(&b->mask);
-  eg_dfree_t(&b->inv);
-  eg_dfree_t(&b->seed_ptr);
-  eg_dfree_t(&b->seeds);
-  eg_dfree_t(&b->cnt);
-  eg_dfree_t(&b->seg_part);
   delete b;
   g->batch = nullptr;
 }
@@ -267,55 +931,12 @@ static BatchState *batch_state(eg_graph *g) {
   return g->batch;
 }
 
-static void in_orientation(eg_graph *g, const uint32_t **ptr, const uint32_t **idx) {
-  if (g->in_ptr) {
-    *ptr = g->in_ptr;
-    *idx = g->in_idx;
-  } else {
-    *ptr = g->out_ptr;
-    *idx = g->out_col;
+static int ensure_ext(BatchState *b, uint64_t words) {
+  if (b->ext_cap < words) {
+    eg_dfree_t(&b->ext);
+    EG_TRY(eg_dalloc_t(&b->ext, words, "extract"));
+    b->ext_cap = words;
   }
-}
-
-// One group of up to 64 sources; levels stay in b->lvl (vertex-major).
-int eg_msbfs_group(eg_graph *g, uint32_t k, const uint32_t *d_roots, int *levels_out) {
-  BatchState *b = batch_state(g);
-  const uint32_t n = g->n;
-  hipStream_t s = g->stream;
-  if (!b->seen) {
-    EG_TRY(eg_dalloc_t(&b->seen, n, "ms seen"));
-    EG_TRY(eg_dalloc_t(&b->visit, n, "ms visit"));
-    EG_TRY(eg_dalloc_t(&b->nvisit, n, "ms next"));
-    EG_TRY(eg_dalloc_t(&b->lvl, (uint64_t)n * 64, "ms levels"));
-    EG_TRY(eg_dalloc_t(&b->cnt, 16, "ms counters"));
-  }
-  const uint32_t *ptr, *idx;
-  in_orientation(g, &ptr, &idx);
-  hipLaunchKernelGGL(k_ms_init, dim3(blocks_for((uint64_t)n * 16, 256)), dim3(256), 0, s, n,
-                     b->seen, b->visit, b->lvl);
-  EG_LAUNCHED();
-  hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, d_roots, k, b->seen, b->visit, b->lvl);
-  EG_LAUNCHED();
-  const uint64_t all = k == 64 ? ~0ull : ((1ull << k) - 1ull);
-  int levels = 1;
-  for (;;) {
-    if (levels >= 65534 || (uint64_t)levels > (uint64_t)n + 1)
-      return eg_fail(EG_ENOTSUP, "BFS depth exceeds the u16 level store");
-    EG_HIP(hipMemsetAsync(b->cnt, 0, 4, s));
-    {
-      EgProf prof(g, "msbfs_pull");
-      hipLaunchKernelGGL(k_ms_pull, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, n, ptr, idx,
-                         b->seen, b->visit, b->nvisit, b->lvl, all, (uint16_t)(levels + 1),
-                         b->cnt);
-      EG_LAUNCHED();
-    }
-    std::swap(b->visit, b->nvisit);
-    EG_HIP(hipMemcpyAsync(g->h_ctr + 32, b->cnt, 4, hipMemcpyDeviceToHost, s));
-    EG_HIP(hipStreamSynchronize(s));
-    if (g->h_ctr[32] == 0) break;
-    levels++;
-  }
-  if (levels_out) *levels_out = levels;
   return EG_OK;
 }
 
@@ -327,24 +948,39 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
   for (uint32_t i = 0; i < k; i++)
     if (roots[i] >= g->n) return eg_fail(EG_ERANGE, "root %u >= n", roots[i]);
   BatchState *b = batch_state(g);
+  if (!b->ms_ok) {
+    EG_TRY(eg_ms_alloc(g, &b->ms));
+    b->ms.stream = g->stream;
+    b->ms_ok = true;
+  }
+  BatchView bv;
+  EG_TRY(eg_batch_view(g, &bv));
   hipStream_t s = g->stream;
   const uint32_t n = g->n;
-  if (!b->roots) EG_TRY(eg_dalloc_t(&b->roots, 64, "roots"));
-  if (dist && b->ext_cap < (uint64_t)n * 64) {
-    eg_dfree_t(&b->ext);
-    EG_TRY(eg_dalloc_t(&b->ext, (uint64_t)n * 64, "extract"));
-    b->ext_cap = (uint64_t)n * 64;
-  }
+  if (dist) EG_TRY(ensure_ext(b, (uint64_t)n * 64));
   double total_ms = 0.0;
+  uint32_t lanes[64];
+  for (uint32_t i = 0; i < 64; i++) lanes[i] = i;
   for (uint32_t g0 = 0; g0 < k; g0 += 64) {
     const uint32_t kk = k - g0 < 64 ? k - g0 : 64;
-    EG_HIP(hipMemcpyAsync(b->roots, roots + g0, kk * 4, hipMemcpyHostToDevice, s));
     hipEvent_t e0, e1;
     EG_HIP(hipEventCreate(&e0));
     EG_HIP(hipEventCreate(&e1));
     EG_HIP(hipEventRecord(e0, s));
-    int lv = 0;
-    int rc = eg_msbfs_group(g, kk, b->roots, &lv);
+    int rc = eg_ms_reset(g, &b->ms);
+    if (rc == EG_OK) rc = eg_ms_seed(g, bv, &b->ms, kk, lanes, roots + g0);
+    const uint64_t active = kk == 64 ? ~0ull : ((1ull << kk) - 1ull);
+    uint16_t tab[64];
+    for (int level = 2; rc == EG_OK; level++) {
+      if (level >= 65535 || (uint64_t)level > (uint64_t)n + 1) {
+        rc = eg_fail(EG_ENOTSUP, "BFS depth exceeds the u16 level store");
+        break;
+      }
+      for (int i = 0; i < 64; i++) tab[i] = (uint16_t)level;
+      uint64_t progress = 0;
+      rc = eg_ms_level(g, bv, &b->ms, active, tab, &progress);
+      if (rc == EG_OK && progress == 0) break;
+    }
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
@@ -355,7 +991,7 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
     total_ms += ms;
     if (dist) {
       hipLaunchKernelGGL(k_ms_extract, dim3(blocks_for((uint64_t)n * kk, 256)), dim3(256), 0, s,
-                         n, kk, b->lvl, b->ext);
+                         n, kk, 0u, b->ms.lvl, bv.perm, b->ext);
       EG_LAUNCHED();
       EG_HIP(hipMemcpyAsync(dist + (uint64_t)g0 * n, b->ext, (uint64_t)n * kk * 4,
                             hipMemcpyDeviceToHost, s));
@@ -363,83 +999,6 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
     }
   }
   if (device_ms) *device_ms = total_ms;
-  return EG_OK;
-}
-
-// k <= 64 queries; results stay in b->X (row-major, 64 columns).
-int eg_ppr_group(eg_graph *g, uint32_t k, const uint32_t *seed_ptr, const uint32_t *seeds,
-                 double alpha, int iterations) {
-  BatchState *b = batch_state(g);
-  const uint32_t n = g->n;
-  hipStream_t s = g->stream;
-  if (!b->X) {
-    EG_TRY(eg_dalloc_t(&b->X, (uint64_t)n * 64, "ppr X"));
-    EG_TRY(eg_dalloc_t(&b->Y, (uint64_t)n * 64, "ppr Y"));
-    EG_TRY(eg_dalloc_t(&b->C, (uint64_t)n * 64, "ppr C"));
-    EG_TRY(eg_dalloc_t(&b->Cn, (uint64_t)n * 64, "ppr Cn"));
-    EG_TRY(eg_dalloc_t(&b->mask, n, "ppr seed mask"));
-    EG_TRY(eg_dalloc_t(&b->inv, 128, "ppr seed weights"));
-    EG_TRY(eg_dalloc_t(&b->seed_ptr, 65, "seed ptr"));
-  }
-  const uint32_t nseeds = seed_ptr[k];
-  if (b->seeds_cap < nseeds + 1) {
-    eg_dfree_t(&b->seeds);
-    EG_TRY(eg_dalloc_t(&b->seeds, (uint64_t)nseeds + 1, "seeds"));
-    b->seeds_cap = nseeds + 1;
-  }
-  // 1/|distinct seeds| per query, and alpha times it.
-  float inv[128];
-  memset(inv, 0, sizeof(inv));
-  for (uint32_t q = 0; q < k; q++) {
-    uint32_t distinct = 0;
-    for (uint32_t j = seed_ptr[q]; j < seed_ptr[q + 1]; j++) {
-      bool dup = false;
-      for (uint32_t i = seed_ptr[q]; i < j; i++) dup |= seeds[i] == seeds[j];
-      distinct += !dup;
-    }
-    if (distinct == 0) return eg_fail(EG_EINVAL, "query %u has no seeds", q);
-    inv[q] = (float)(1.0 / distinct);
-    inv[64 + q] = (float)(alpha / distinct);
-  }
-  std::vector<uint32_t> sp(seed_ptr, seed_ptr + k + 1);
-  for (auto &x : sp) x -= seed_ptr[0];
-  EG_HIP(hipMemcpyAsync(b->inv, inv, sizeof(inv), hipMemcpyHostToDevice, s));
-  EG_HIP(hipMemcpyAsync(b->seed_ptr, sp.data(), (k + 1) * 4, hipMemcpyHostToDevice, s));
-  EG_HIP(hipMemcpyAsync(b->seeds, seeds + seed_ptr[0], (uint64_t)sp[k] * 4, hipMemcpyHostToDevice,
-                        s));
-  EG_HIP(hipMemsetAsync(b->mask, 0, (size_t)n * 8, s));
-  hipLaunchKernelGGL(k_ppr_seedmask, dim3(1), dim3(64), 0, s, k, b->seed_ptr, b->seeds, b->mask);
-  EG_LAUNCHED();
-  hipLaunchKernelGGL(k_ppr_init, dim3(blocks_for((uint64_t)n * 64, 256)), dim3(256), 0, s, n,
-                     b->mask, b->inv, g->out_deg, b->X, b->C);
-  EG_LAUNCHED();
-  const uint32_t *ptr, *idx;
-  const float *w;
-  PullPlan *pl;
-  EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
-  if (b->seg_part_cap < (uint64_t)pl->nseg * 64) {
-    eg_dfree_t(&b->seg_part);
-    EG_TRY(eg_dalloc_t(&b->seg_part, (uint64_t)(pl->nseg ? pl->nseg : 1) * 64, "ppr partials"));
-    b->seg_part_cap = (uint64_t)(pl->nseg ? pl->nseg : 1) * 64;
-  }
-  const float oma = (float)(1.0 - alpha);
-  for (int it = 0; it < iterations; it++) {
-    {
-      EgProf prof(g, "ppr_spmm");
-      hipLaunchKernelGGL(k_ppr_spmm, dim3(pl->nseg + pl->nchunks), dim3(kBlock), 0, s, idx,
-                         pl->chunk_info, pl->seg_info, pl->nseg, b->seg_part, ptr, b->C, b->mask,
-                         b->inv + 64, g->out_deg, oma, b->Y, b->Cn);
-      EG_LAUNCHED();
-    }
-    if (pl->nlong) {
-      hipLaunchKernelGGL(k_ppr_finish, dim3(blocks_for((uint64_t)pl->nlong * 64, kBlock)),
-                         dim3(kBlock), 0, s, pl->long_rows, pl->seg_base, pl->nlong, b->seg_part,
-                         b->mask, b->inv + 64, g->out_deg, oma, b->Y, b->Cn);
-      EG_LAUNCHED();
-    }
-    std::swap(b->X, b->Y);
-    std::swap(b->C, b->Cn);
-  }
   return EG_OK;
 }
 
@@ -451,12 +1010,22 @@ extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
   if (k && (!seed_ptr || !seeds)) return eg_fail(EG_EINVAL, "null seeds");
   if (!(alpha >= 0.0 && alpha <= 1.0) || iterations < 0)
     return eg_fail(EG_EINVAL, "bad PPR options");
-  for (uint32_t q = 0; q < k; q++)
+  for (uint32_t q = 0; q < k; q++) {
+    if (seed_ptr[q + 1] <= seed_ptr[q]) return eg_fail(EG_EINVAL, "query %u has no seeds", q);
     for (uint32_t j = seed_ptr[q]; j < seed_ptr[q + 1]; j++)
       if (seeds[j] >= g->n) return eg_fail(EG_ERANGE, "seed %u >= n", seeds[j]);
+  }
   BatchState *b = batch_state(g);
+  if (!b->ppr_ok) {
+    EG_TRY(eg_ppr_alloc(g, &b->ppr));
+    b->ppr.stream = g->stream;
+    b->ppr_ok = true;
+  }
+  BatchView bv;
+  EG_TRY(eg_batch_view(g, &bv));
   hipStream_t s = g->stream;
   const uint32_t n = g->n;
+  if (out) EG_TRY(ensure_ext(b, (uint64_t)n * 64));
   double total_ms = 0.0;
   for (uint32_t q0 = 0; q0 < k; q0 += 64) {
     const uint32_t kk = k - q0 < 64 ? k - q0 : 64;
@@ -464,7 +1033,11 @@ extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
     EG_HIP(hipEventCreate(&e0));
     EG_HIP(hipEventCreate(&e1));
     EG_HIP(hipEventRecord(e0, s));
-    int rc = eg_ppr_group(g, kk, seed_ptr + q0, seeds, alpha, iterations);
+    int rc = eg_ppr_reset(g, &b->ppr);
+    for (uint32_t q = 0; q < kk && rc == EG_OK; q++)
+      rc = eg_ppr_col_init(g, bv, &b->ppr, q, seeds + seed_ptr[q0 + q],
+                           seed_ptr[q0 + q + 1] - seed_ptr[q0 + q], alpha);
+    for (int it = 0; it < iterations && rc == EG_OK; it++) rc = eg_ppr_step(g, bv, &b->ppr, alpha);
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
@@ -474,14 +1047,9 @@ extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
     if (rc) return rc;
     total_ms += ms;
     if (out) {
-      if (b->ext_cap < (uint64_t)n * 64) {
-        eg_dfree_t(&b->ext);
-        EG_TRY(eg_dalloc_t(&b->ext, (uint64_t)n * 64, "extract"));
-        b->ext_cap = (uint64_t)n * 64;
-      }
       float *ext = reinterpret_cast<float *>(b->ext);
       hipLaunchKernelGGL(k_ppr_extract, dim3(blocks_for((uint64_t)n * kk, 256)), dim3(256), 0, s,
-                         n, kk, b->X, ext);
+                         n, kk, 0u, b->ppr.X, bv.perm, ext);
       EG_LAUNCHED();
       EG_HIP(hipMemcpyAsync(out + (uint64_t)q0 * n, ext, (uint64_t)n * kk * 4,
                             hipMemcpyDeviceToHost, s));

@@ -79,8 +79,9 @@ int eg_require_loaded(const eg_graph *g) {
 // ---------------------------------------------------------------------------
 // Timing.
 // ---------------------------------------------------------------------------
-EgProf::EgProf(eg_graph *g_, const char *name) : g(g_) {
+EgProf::EgProf(eg_graph *g_, const char *name, hipStream_t s_) : g(g_) {
   if (!g || !g->prof_on) return;
+  stream = s_ ? s_ : g->stream;
   for (auto &s : g->prof)
     if (s.name == name) slot = &s;
   if (!slot) {
@@ -92,12 +93,12 @@ EgProf::EgProf(eg_graph *g_, const char *name) : g(g_) {
     slot = nullptr;
     return;
   }
-  (void)hipEventRecord(ev[0], g->stream);
+  (void)hipEventRecord(ev[0], stream);
 }
 
 EgProf::~EgProf() {
   if (!slot) return;
-  (void)hipEventRecord(ev[1], g->stream);
+  (void)hipEventRecord(ev[1], stream);
   slot->pending.push_back(ev[0]);
   slot->pending.push_back(ev[1]);
 }
@@ -259,6 +260,7 @@ static void free_layout(eg_graph *g) {
   eg_plan_free(&g->plan_in);
   eg_plan_free(&g->plan_out);
   eg_grid_free(g);
+  eg_order_free(g);
   eg_batch_free(g);
   g->loaded = false;
   g->ndangling = 0xFFFFFFFFu;

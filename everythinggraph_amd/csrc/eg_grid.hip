@@ -383,9 +383,9 @@ static int grid_build_compute(eg_graph *g, GridCompute *c) {
 }
 
 int eg_pr_state(eg_graph *g, int iters);
-int eg_pr_start(eg_graph *g, double init, float *rank, float *contrib);
-int eg_pr_finish(eg_graph *g, int iters, float *rank, float *contrib, float *rank_out, double *dh,
-                 double *lh);
+int eg_pr_start(eg_graph *g, double init, const uint32_t *deg, float *rank, float *contrib);
+int eg_pr_finish(eg_graph *g, int iters, float *rank, float *spare, float *rank_out, double *dh,
+                 double *lh, const uint32_t *unperm);
 static int pr_state(eg_graph *g, int iters) { return eg_pr_state(g, iters); }
 
 // Shared PageRank driver for the grid and edge-array layouts.
@@ -419,7 +419,7 @@ static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, dou
   }
   float *rank = g->rank_a, *rank_next = g->rank_b;
   float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
-  EG_TRY(eg_pr_start(g, init, rank, contrib));
+  EG_TRY(eg_pr_start(g, init, g->out_deg, rank, contrib));
   const float base = (float)((1.0 - damping) / (double)n);
   for (int it = 0; it < iters; it++) {
     const double *dmass_in = g->pr_stats + 2 * it;
@@ -469,7 +469,7 @@ static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, dou
     std::swap(rank, rank_next);
     std::swap(contrib, contrib_next);
   }
-  return eg_pr_finish(g, iters, rank, contrib, rank_out, dh, lh);
+  return eg_pr_finish(g, iters, rank, rank_next, rank_out, dh, lh, nullptr);
 }
 
 int eg_pagerank_grid(eg_graph *g, const eg_pr_opts *o, float *rank_out, double *dh, double *lh) {

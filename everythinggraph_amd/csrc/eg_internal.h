@@ -83,6 +83,17 @@ struct PullPlan {
 struct GridCompute;
 struct BatchState;
 
+// Degree-ordered copy of the pull orientation (eg_order.hip): vertex i of the
+// copy is old vertex old_of_new[i]; rows keep their entry order.
+struct EgOrdered {
+  bool built = false;
+  uint32_t *old_of_new = nullptr, *new_of_old = nullptr;
+  uint32_t *ptr = nullptr, *idx = nullptr;
+  float *w = nullptr;
+  uint32_t *deg = nullptr;  // out-degree in new ids
+  PullPlan plan;
+};
+
 struct eg_graph {
   uint32_t n = 0, P = 0, flags = 0;
   int32_t load_mode = 2;
@@ -134,6 +145,7 @@ struct eg_graph {
 
   GridCompute *gridc = nullptr;  // compute copy of the grid (eg_grid.hip)
   BatchState *batch = nullptr;   // multi-source state (eg_batch.hip)
+  EgOrdered *ord = nullptr;      // degree-ordered pull copy (eg_order.hip)
 
   int prof_on = 0;
   std::vector<EgProfSlot> prof;
@@ -141,6 +153,12 @@ struct eg_graph {
 
 void eg_grid_free(eg_graph *g);
 void eg_batch_free(eg_graph *g);
+void eg_order_free(eg_graph *g);
+bool eg_reorder_enabled();
+// Builds (on first use) and returns the degree-ordered pull copy.
+int eg_ordered(eg_graph *g, EgOrdered **out);
+// dst[i] = src[perm[i]] for 32-bit vectors of length n (on the graph stream).
+int eg_gather_vec(eg_graph *g, const void *src, const uint32_t *perm, void *dst);
 
 bool eg_has_in_edges(const eg_graph *g);
 int eg_scratch(eg_graph *g, size_t bytes, void **out);
@@ -151,7 +169,10 @@ struct EgProf {
   eg_graph *g;
   EgProfSlot *slot = nullptr;
   hipEvent_t ev[2];
-  EgProf(eg_graph *g_, const char *name);
+  hipStream_t stream;
+  // Times the work enqueued on stream s (the graph's stream when null)
+  // between construction and destruction.
+  EgProf(eg_graph *g_, const char *name, hipStream_t s = nullptr);
   ~EgProf();
 };
 

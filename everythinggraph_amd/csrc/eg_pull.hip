@@ -534,10 +534,10 @@ void eg_pr_resolve(const eg_graph *g, const eg_pr_opts *o, double *damping, int 
 
 // First iterate: rank = init, contrib = init/outdeg, and the FP64 dangling
 // mass of that iterate (init times the number of sinks) in pr_stats[0].
-int eg_pr_start(eg_graph *g, double init, float *rank, float *contrib) {
+int eg_pr_start(eg_graph *g, double init, const uint32_t *deg, float *rank, float *contrib) {
   const uint32_t n = g->n;
   hipLaunchKernelGGL(k_pr_init, dim3(blocks_for(n, 256)), dim3(256), 0, g->stream, n, (float)init,
-                     g->out_deg, rank, contrib);
+                     deg, rank, contrib);
   EG_LAUNCHED();
   const double d0 = (double)g->ndangling * (double)(float)init;
   EG_HIP(hipMemcpyAsync(g->pr_stats, &d0, sizeof(double), hipMemcpyHostToDevice, g->stream));
@@ -546,11 +546,19 @@ int eg_pr_start(eg_graph *g, double init, float *rank, float *contrib) {
 }
 
 // Downloads results and statistics after the iterations; keeps the final
-// iterate in rank_a / contrib_a.
-int eg_pr_finish(eg_graph *g, int iters, float *rank, float *contrib, float *rank_out, double *dh,
-                 double *lh) {
+// iterate in rank_a / contrib_a.  With unperm (new id of every vertex) the
+// result is gathered back to the vertex ids of the layout through spare.
+int eg_pr_finish(eg_graph *g, int iters, float *rank, float *spare, float *rank_out, double *dh,
+                 double *lh, const uint32_t *unperm) {
   hipStream_t s = g->stream;
-  if (rank_out) EG_HIP(hipMemcpyAsync(rank_out, rank, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
+  if (rank_out) {
+    const float *src = rank;
+    if (unperm) {
+      EG_TRY(eg_gather_vec(g, rank, unperm, spare));
+      src = spare;
+    }
+    EG_HIP(hipMemcpyAsync(rank_out, src, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
+  }
   std::vector<double> st(2 * ((size_t)iters + 1));
   if ((dh || lh) && iters > 0)
     EG_HIP(hipMemcpyAsync(st.data(), g->pr_stats, st.size() * sizeof(double),
@@ -564,7 +572,6 @@ int eg_pr_finish(eg_graph *g, int iters, float *rank, float *contrib, float *ran
     std::swap(g->rank_a, g->rank_b);
     std::swap(g->contrib_a, g->contrib_b);
   }
-  (void)contrib;
   return EG_OK;
 }
 
@@ -584,17 +591,28 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
     return eg_pagerank_edges(g, o, rank_out, dh, lh);
   if (!adj) return eg_fail(EG_ENOTSUP, "load mode %d has no adjacency lists", g->load_mode);
 
-  const uint32_t *ptr, *idx;
+  const uint32_t *ptr, *idx, *deg = g->out_deg, *unperm = nullptr;
   const float *w;
   PullPlan *pl;
-  EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  if (eg_reorder_enabled()) {
+    EgOrdered *o;
+    EG_TRY(eg_ordered(g, &o));
+    ptr = o->ptr;
+    idx = o->idx;
+    w = o->w;
+    pl = &o->plan;
+    deg = o->deg;
+    unperm = o->new_of_old;
+  } else {
+    EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  }
   EG_TRY(eg_pr_state(g, iters));
   const uint32_t n = g->n;
   float *rank = g->rank_a, *rank_next = g->rank_b;
   float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
-  EG_TRY(eg_pr_start(g, init, rank, contrib));
+  EG_TRY(eg_pr_start(g, init, deg, rank, contrib));
   PrArgs a;
-  a.out_deg = g->out_deg;
+  a.out_deg = deg;
   a.base = (float)((1.0 - damping) / (double)n);
   a.damping = (float)damping;
   a.damping_over_n = damping / (double)n;
@@ -608,7 +626,7 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
     std::swap(rank, rank_next);
     std::swap(contrib, contrib_next);
   }
-  return eg_pr_finish(g, iters, rank, contrib, rank_out, dh, lh);
+  return eg_pr_finish(g, iters, rank, rank_next, rank_out, dh, lh, unperm);
 }
 
 extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
@@ -622,14 +640,53 @@ extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
   const uint32_t *ptr, *idx;
   const float *w;
   PullPlan *pl;
-  EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  EgOrdered *o = nullptr;
+  if (eg_reorder_enabled()) {
+    EG_TRY(eg_ordered(g, &o));
+    ptr = o->ptr;
+    idx = o->idx;
+    w = o->w;
+    pl = &o->plan;
+  } else {
+    EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  }
   EG_TRY(eg_pr_state(g, 0));
   const uint32_t n = g->n;
   hipStream_t s = g->stream;
-  EG_HIP(hipMemcpyAsync(g->contrib_a, x, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  float *xin = g->contrib_a, *yout = g->rank_a;
+  if (o) {
+    EG_HIP(hipMemcpyAsync(g->contrib_b, x, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    EG_TRY(eg_gather_vec(g, g->contrib_b, o->old_of_new, g->contrib_a));
+  } else {
+    EG_HIP(hipMemcpyAsync(g->contrib_a, x, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  }
   PrArgs a = {};
-  EG_TRY(launch_pull(g, false, ptr, idx, w, g->contrib_a, g->rank_a, *pl, a));
-  if (y) EG_HIP(hipMemcpyAsync(y, g->rank_a, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  EG_TRY(launch_pull(g, false, ptr, idx, w, xin, yout, *pl, a));
+  if (o) {
+    EG_TRY(eg_gather_vec(g, g->rank_a, o->new_of_old, g->rank_b));
+    yout = g->rank_b;
+  }
+  if (y) EG_HIP(hipMemcpyAsync(y, yout, (size_t)n * 4, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
   return EG_OK;
+}
+
+// Device-resident SpMV pass used by the benchmark (x, y in HBM, in the ids of
+// the pull copy in use).
+int eg_spmv_device(eg_graph *g, const float *x, float *y) {
+  const uint32_t *ptr, *idx;
+  const float *w;
+  PullPlan *pl;
+  if (eg_reorder_enabled()) {
+    EgOrdered *o;
+    EG_TRY(eg_ordered(g, &o));
+    ptr = o->ptr;
+    idx = o->idx;
+    w = o->w;
+    pl = &o->plan;
+  } else {
+    EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  }
+  PrArgs a = {};
+  return launch_pull(g, false, ptr, idx, w, x, y, *pl, a);
 }

@@ -309,3 +309,46 @@ def test_ppr_batch_vs_oracle():
     for q, sd in enumerate(seeds):
         ref = O.ppr(n, cp, row, deg, sd, 0.15, 30)
         assert np.abs(out[q] - ref).sum() <= PR_L1_TOL, q
+
+
+# ---------------------------------------------------------------------------
+# degree-ordered compute copy: results identical with and without it
+# ---------------------------------------------------------------------------
+_ORDER_SCRIPT = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[1] + '/tests')
+import eg_oracle_py as O, everythinggraph_amd as eg
+n = 1 << 13
+s, d = O.gen_rmat(13, 16 << 13, seed=21)
+w = O.gen_weights(s.size, seed=3)
+x = np.linspace(-1, 1, n).astype(np.float32)
+out = {}
+with eg.Graph(n, 2, weighted=True) as g:
+    g.load_edges(s, d, w)
+    out['pr'] = g.pagerank(0.85, 20)[0]
+    out['spmv'] = g.spmv(x)
+with eg.Graph(n, 2, undirected=True) as g:
+    g.load_edges(s, d)
+    out['ppr'] = g.ppr_batch([[1, 2], [5]], 0.15, 30)[0]
+    out['ms'] = g.bfs_batch([0, 7, 99])[0]
+np.savez(sys.argv[2], **out)
+"""
+
+
+def test_degree_order_is_bit_identical(tmp_path):
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "run.py"
+    script.write_text(_ORDER_SCRIPT)
+    res = {}
+    for mode in ("none", "degree"):
+        env = dict(os.environ, EG_PULL_ORDER=mode)
+        outp = tmp_path / f"{mode}.npz"
+        subprocess.run([sys.executable, str(script), root, str(outp)], env=env, check=True,
+                       timeout=120)
+        res[mode] = np.load(outp)
+    for k in ("pr", "spmv", "ppr", "ms"):
+        assert np.array_equal(res["none"][k], res["degree"][k]), k
