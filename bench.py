@@ -67,6 +67,10 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=100.0,
+                    help="offered Poisson rate of the serving replay")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -152,7 +156,11 @@ def main():
         wp = time.perf_counter() - w0
         batched_ppr_qps = len(seeds) / wp
         # plain per-query PPR: 30 iterations of the single-vector pull kernel
-        plain_ppr_qps = 1.0 / (30 * ms_per_step / iters * 1e-3)
+        g.ppr(seeds[0], 0.15, tol=0.0, max_iter=2, download=False)
+        w0 = time.perf_counter()
+        for sd in seeds[:4]:
+            g.ppr(sd, 0.15, tol=0.0, max_iter=30, download=False)
+        plain_ppr_qps = 4 / (time.perf_counter() - w0)
         extra.update({
             "batched_bfs_qps": round(batched_bfs_qps, 2),
             "plain_bfs_qps": round(plain_bfs_qps, 2),
@@ -163,6 +171,31 @@ def main():
         })
         log(f"batched: MS-BFS {batched_bfs_qps:.1f} q/s (plain {plain_bfs_qps:.1f}); "
             f"PPR SpMM {batched_ppr_qps:.1f} q/s (plain {plain_ppr_qps:.1f})")
+
+    if not args.skip_serve:
+        # serving: the same Poisson stream (80% BFS, 20% PPR with 1-8 seeds,
+        # tolerance 1e-6 L1, seed 4) through continuous batching and through
+        # the plain one-query-at-a-time path
+        srv = {}
+        for mode in ("batched", "plain"):
+            w0 = time.perf_counter()
+            r = g.serve_replay(args.serve_qps, args.serve_requests, bfs_fraction=0.8,
+                               ppr_max_seeds=8, seed=4, plain=(mode == "plain"))
+            srv[mode] = r
+            log(f"serve {mode}: offered {args.serve_qps:.0f} q/s, {r['completed']} done "
+                f"({r['bfs']} BFS / {r['ppr']} PPR) at {r['qps']:.1f} q/s; p50 "
+                f"{r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms (BFS p50 {r['bfs_p50_ms']:.1f} "
+                f"p99 {r['bfs_p99_ms']:.1f}; PPR p50 {r['ppr_p50_ms']:.1f} p99 "
+                f"{r['ppr_p99_ms']:.1f}); levels {r['stats']['bfs_levels']} spmm steps "
+                f"{r['stats']['ppr_steps']}; wall {time.perf_counter() - w0:.1f} s")
+        extra["serving"] = {
+            "offered_qps": args.serve_qps, "requests": args.serve_requests,
+            "mix": "80% BFS / 20% PPR (1-8 seeds, tol 1e-6 L1), Poisson, seed 4",
+            **{f"{k}_{f}": round(v[f], 3) for k, v in srv.items()
+               for f in ("qps", "p50_ms", "p99_ms", "bfs_p50_ms", "bfs_p99_ms", "ppr_p50_ms",
+                         "ppr_p99_ms")},
+            "batched_qps_vs_plain": round(srv["batched"]["qps"] / max(srv["plain"]["qps"], 1e-9), 2),
+        }
 
     cpu = None
     if not args.skip_cpu:

@@ -55,7 +55,8 @@ ABI_SYMBOLS = (
     "eg_graph_get_degrees", "eg_graph_get_csr", "eg_graph_get_csc", "eg_graph_get_grid",
     "eg_graph_get_edgelist", "eg_pagerank", "eg_spmv", "eg_bfs", "eg_wcc",
     "eg_bfs_batch", "eg_ppr_batch", "eg_prof_enable", "eg_prof_reset", "eg_prof_read",
-    "eg_sync",
+    "eg_sync", "eg_ppr", "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
+    "eg_serve_wait", "eg_serve_stats_get", "eg_serve_stop", "eg_serve_replay",
 )
 
 
@@ -99,6 +100,69 @@ class BfsStats(ctypes.Structure):
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64),
                 ("total_ms", ctypes.c_double)]
+
+
+REQ_BFS, REQ_PPR = 0, 1
+
+
+class ServeOpts(ctypes.Structure):
+    _fields_ = [("bfs_lanes", ctypes.c_uint32), ("ppr_columns", ctypes.c_uint32),
+                ("hbm_reserve", ctypes.c_uint64), ("ppr_alpha", ctypes.c_double),
+                ("ppr_tol", ctypes.c_double), ("ppr_max_iter", ctypes.c_int32),
+                ("plain", ctypes.c_int32), ("max_pending", ctypes.c_uint32),
+                ("ppr_grid", ctypes.c_uint32)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("levels", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("reached", ctypes.c_uint64), ("delta", ctypes.c_double),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("pending", ctypes.c_uint32),
+                ("bfs_inflight", ctypes.c_uint32), ("ppr_inflight", ctypes.c_uint32),
+                ("bfs_lanes", ctypes.c_uint32), ("ppr_columns", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32), ("bfs_levels", ctypes.c_uint64),
+                ("ppr_steps", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64)]
+
+
+class ReplayParams(ctypes.Structure):
+    _fields_ = [("qps", ctypes.c_double), ("requests", ctypes.c_uint32),
+                ("ppr_max_seeds", ctypes.c_uint32), ("bfs_fraction", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+class ReplayReport(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint32), ("failed", ctypes.c_uint32),
+                ("bfs", ctypes.c_uint32), ("ppr", ctypes.c_uint32),
+                ("wall_s", ctypes.c_double), ("qps", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("bfs_p50_ms", ctypes.c_double), ("bfs_p99_ms", ctypes.c_double),
+                ("ppr_p50_ms", ctypes.c_double), ("ppr_p99_ms", ctypes.c_double),
+                ("stats", ServeStats)]
+
+
+def _struct_dict(st) -> dict:
+    out = {}
+    for name, _ in st._fields_:
+        if name == "pad":
+            continue
+        v = getattr(st, name)
+        out[name] = _struct_dict(v) if isinstance(v, ctypes.Structure) else v
+    return out
+
+
+def serve_opts(bfs_lanes=0, ppr_columns=0, hbm_reserve=0, ppr_alpha=0.0, ppr_tol=0.0,
+               ppr_max_iter=0, plain=False, max_pending=0, ppr_grid=0) -> ServeOpts:
+    """Serving options; zeros take the library defaults (64 lanes, 64 columns,
+    1 GiB reserve, alpha 0.15, tolerance 1e-6, 200 iterations, an SpMM grid of
+    2 workgroups per compute unit)."""
+    return ServeOpts(bfs_lanes, ppr_columns, hbm_reserve, ppr_alpha, ppr_tol, ppr_max_iter,
+                     int(plain), max_pending, ppr_grid)
 
 
 _lib: Optional[ctypes.CDLL] = None
@@ -306,6 +370,28 @@ class Graph:
                                   ctypes.byref(ms)), "eg_ppr_batch")
         return out, ms.value
 
+    def ppr(self, seeds: Sequence[int], alpha=0.15, tol=1e-6, max_iter=200, download=True):
+        """Single-vector personalized PageRank: (ranks or None, iterations, last L1 change)."""
+        sd = np.ascontiguousarray(seeds, np.uint32)
+        out = np.empty(self.n, np.float32) if download else None
+        it, dl = ctypes.c_int32(), ctypes.c_double()
+        _check(lib().eg_ppr(self._h, _ptr(sd), ctypes.c_uint32(sd.size), ctypes.c_double(alpha),
+                            ctypes.c_double(tol), ctypes.c_int32(max_iter), _ptr(out),
+                            ctypes.byref(it), ctypes.byref(dl)), "eg_ppr")
+        return out, it.value, dl.value
+
+    # -- serving --
+    def serve(self, **opts) -> "Server":
+        return Server(self, serve_opts(**opts))
+
+    def serve_replay(self, qps: float, requests: int, bfs_fraction=0.8, ppr_max_seeds=8,
+                     seed=4, **opts) -> dict:
+        r = ReplayReport()
+        p = ReplayParams(qps, requests, ppr_max_seeds, bfs_fraction, seed)
+        _check(lib().eg_serve_replay(self._h, ctypes.byref(serve_opts(**opts)), ctypes.byref(p),
+                                     ctypes.byref(r)), "eg_serve_replay")
+        return _struct_dict(r)
+
     # -- timing --
     def prof_enable(self, on=True):
         _check(lib().eg_prof_enable(self._h, ctypes.c_int32(int(on))), "eg_prof_enable")
@@ -323,6 +409,63 @@ class Graph:
 
     def sync(self):
         _check(lib().eg_sync(self._h), "eg_sync")
+
+
+class Server:
+    """Continuous-batching server over a loaded Graph (eg_serve_*).  The graph
+    cannot be used by other calls until stop()."""
+
+    def __init__(self, g: Graph, opts: ServeOpts):
+        self.g = g
+        self._h = ctypes.c_void_p()
+        self._out = {}
+        _check(lib().eg_serve_start(g._h, ctypes.byref(opts), ctypes.byref(self._h)),
+               "eg_serve_start")
+
+    def submit_bfs(self, root: int, download=True) -> int:
+        out = np.empty(self.g.n, np.uint32) if download else None
+        t = ctypes.c_uint64()
+        _check(lib().eg_serve_submit_bfs(self._h, ctypes.c_uint32(root), _ptr(out),
+                                         ctypes.byref(t)), "eg_serve_submit_bfs")
+        self._out[t.value] = out
+        return t.value
+
+    def submit_ppr(self, seeds: Sequence[int], download=True) -> int:
+        sd = np.ascontiguousarray(seeds, np.uint32)
+        out = np.empty(self.g.n, np.float32) if download else None
+        t = ctypes.c_uint64()
+        _check(lib().eg_serve_submit_ppr(self._h, _ptr(sd), ctypes.c_uint32(sd.size), _ptr(out),
+                                         ctypes.byref(t)), "eg_serve_submit_ppr")
+        self._out[t.value] = out
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0):
+        """(result dict, output array or None); raises EgError if the query
+        failed or the wait timed out."""
+        r = ServeResult()
+        _check(lib().eg_serve_wait(self._h, ctypes.c_uint64(ticket), ctypes.c_double(timeout_ms),
+                                   ctypes.byref(r)), "eg_serve_wait")
+        out = self._out.pop(ticket, None)
+        if r.status != EG_OK:
+            raise EgError(r.status, "query", lib().eg_last_error().decode(errors="replace"))
+        return _struct_dict(r), out
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        _check(lib().eg_serve_stats_get(self._h, ctypes.byref(st)), "eg_serve_stats_get")
+        return _struct_dict(st)
+
+    def stop(self):
+        if self._h:
+            h, self._h = self._h, ctypes.c_void_p()
+            _check(lib().eg_serve_stop(h), "eg_serve_stop")
+        self._out.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
 
 
 def sample_roots(deg: np.ndarray, k: int, seed: int = 2) -> np.ndarray:

@@ -16,7 +16,8 @@ struct BatchView {
 int eg_batch_view(eg_graph *g, BatchView *v);
 
 // Up to 64 BFS lanes over one traversal: lane i of vertex v is bit i of the
-// 64-bit masks.  Levels are u16, vertex-major (lvl[v * 64 + lane]), root = 1.
+// 64-bit masks.  Levels are u16, lane-major (lvl[lane * n + v]), root = 1,
+// 0 = not reached.
 struct MsState {
   hipStream_t stream = nullptr;  // every step is enqueued here
   void *scan_tmp = nullptr;      // frontier degree scan workspace
@@ -25,10 +26,10 @@ struct MsState {
   uint64_t *seg_or = nullptr;  // [nseg] long-row partial ORs
   uint32_t seg_cap = 0;
   uint32_t *queue = nullptr, *qoff = nullptr;  // frontier vertices with edges, degree scan
-  unsigned long long *ctr = nullptr;           // [0] progress mask, [1] qlen, [2] degsum
+  unsigned long long *ctr = nullptr;  // [0] progress, [1] count, [2] degsum, [3] qlen, [8..) partials
   uint16_t *lvl_tab = nullptr;                 // [64] level written per lane this step
   uint32_t *roots = nullptr;                   // [64]
-  unsigned long long *h = nullptr;             // pinned mirror of ctr
+  unsigned long long *h = nullptr;             // pinned: [0..3) ctr mirror, [8..72) per-lane sums
   uint32_t qlen = 0;                           // frontier queue length (host copy)
   uint64_t qdeg = 0;                           // its out-degree sum
 };
@@ -46,7 +47,11 @@ int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
 // out[v] (device, n words) = level of lane for vertex v of the layout.
 int eg_ms_extract_lane(eg_graph *g, const BatchView &bv, MsState *st, uint32_t lane,
                        uint32_t *out);
-int eg_ms_clear_lane(eg_graph *g, MsState *st, uint32_t lane);
+// Frees lanes (a mask) for reuse; each must have finished (no frontier).
+int eg_ms_clear_lanes(eg_graph *g, MsState *st, uint64_t lanes);
+// reached[i] = vertices with a level in lane lanes[i], i < k <= 64.
+int eg_ms_reached(eg_graph *g, MsState *st, uint32_t k, const uint32_t *lanes,
+                  uint64_t *reached);
 
 // Up to 64 personalized PageRank columns (FP32, row-major, 64 per vertex).
 struct PprState {
@@ -63,6 +68,8 @@ struct PprState {
   uint64_t dpart_cap = 0;
   double *delta = nullptr;    // [64] L1 change of every column in the last step
   double *h_delta = nullptr;  // pinned mirror
+  uint64_t active = ~0ull;    // columns in use: steps gather only up to the highest one
+  uint32_t max_grid = 0;      // workgroups of the SpMM launch (0: one per work item)
 };
 int eg_ppr_alloc(eg_graph *g, PprState *st);
 void eg_ppr_free(PprState *st);
@@ -77,4 +84,5 @@ int eg_ppr_col_init(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q,
 int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha,
                 bool want_delta = false);
 int eg_ppr_extract_col(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q, float *out);
-int eg_ppr_clear_col(eg_graph *g, PprState *st, uint32_t q);
+// Zeroes columns (a mask) for reuse.
+int eg_ppr_clear_cols(eg_graph *g, PprState *st, uint64_t cols);

@@ -62,8 +62,9 @@ __device__ __forceinline__ void publish_progress(uint64_t nw, uint64_t *s_or,
   }
 }
 
-__device__ __forceinline__ uint64_t finish_ms_row(uint32_t r, uint64_t acc, uint64_t active,
-                                                  uint64_t *__restrict__ seen,
+// Level store: lane-major u16, lvl[lane * n + v].
+__device__ __forceinline__ uint64_t finish_ms_row(uint32_t n, uint32_t r, uint64_t acc,
+                                                  uint64_t active, uint64_t *__restrict__ seen,
                                                   uint64_t *__restrict__ nvisit,
                                                   uint16_t *__restrict__ lvl, const uint16_t *tab) {
   const uint64_t s = seen[r];
@@ -74,7 +75,7 @@ __device__ __forceinline__ uint64_t finish_ms_row(uint32_t r, uint64_t acc, uint
     uint64_t b = nw;
     while (b) {
       const int i = __ffsll((unsigned long long)b) - 1;
-      lvl[(uint64_t)r * 64 + i] = tab[i];
+      lvl[(uint64_t)i * n + r] = tab[i];
       b &= b - 1;
     }
   }
@@ -94,7 +95,7 @@ __global__ void k_ms_reset(uint32_t n, uint64_t *__restrict__ seen, uint64_t *__
   if (i < (uint64_t)n * 16) reinterpret_cast<uint64_t *>(lvl)[i] = 0;  // 64 u16 = 16 u64
 }
 
-__global__ void k_ms_seed(uint32_t k, const uint32_t *__restrict__ lanes,
+__global__ void k_ms_seed(uint32_t n, uint32_t k, const uint32_t *__restrict__ lanes,
                           const uint32_t *__restrict__ roots, const uint32_t *__restrict__ perm,
                           uint64_t *__restrict__ seen, uint64_t *__restrict__ visit,
                           uint16_t *__restrict__ lvl) {
@@ -104,13 +105,13 @@ __global__ void k_ms_seed(uint32_t k, const uint32_t *__restrict__ lanes,
   const uint32_t lane = lanes[i];
   atomicOr(reinterpret_cast<unsigned long long *>(&seen[r]), 1ull << lane);
   atomicOr(reinterpret_cast<unsigned long long *>(&visit[r]), 1ull << lane);
-  lvl[(uint64_t)r * 64 + lane] = 1;
+  lvl[(uint64_t)lane * n + r] = 1;
 }
 
 // Pull level over the plan: segments of long rows OR into seg_or; chunks of
 // short rows finish their rows directly.
 __global__ __launch_bounds__(kBlock) void k_ms_pull(
-    const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
+    uint32_t n, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
     const uint4 *__restrict__ chunk_info, const uint4 *__restrict__ seg_info, uint32_t nseg,
     uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
     if (rb - ra <= kSerialRow) {
       uint64_t acc = 0;
       for (uint32_t q = ra; q < rb; q++) acc |= s_val[q];
-      nwall |= finish_ms_row(r0 + i, acc, active, seen, nvisit, lvl, s_tab);
+      nwall |= finish_ms_row(n, r0 + i, acc, active, seen, nvisit, lvl, s_tab);
     } else {
       s_list[atomicAdd(&s_nlist, 1u)] = i;
     }
@@ -184,13 +185,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
     uint64_t acc = 0;
     for (uint32_t q = s_ptr[i] + lane; q < s_ptr[i + 1]; q += 64) acc |= s_val[q];
     acc = wave_or(acc);
-    if (lane == 0) nwall |= finish_ms_row(r0 + i, acc, active, seen, nvisit, lvl, s_tab);
+    if (lane == 0) nwall |= finish_ms_row(n, r0 + i, acc, active, seen, nvisit, lvl, s_tab);
   }
   publish_progress(nwall, s_or, ctr);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
-    const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
+    uint32_t n, const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
     const uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
     uint64_t active, unsigned long long *__restrict__ ctr) {
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
   if (j < nlong) {
     uint64_t acc = 0;
     for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++) acc |= seg_or[s];
-    nw = finish_ms_row(long_rows[j], acc, active, seen, nvisit, lvl, tab);
+    nw = finish_ms_row(n, long_rows[j], acc, active, seen, nvisit, lvl, tab);
   }
   publish_progress(nw, s_or, ctr);
 }
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, uint64_t *__res
     uint64_t nw = 0;
     if (p) {
       pend[v] = 0;
-      nw = finish_ms_row(v, p, active, seen, nvisit, lvl, s_tab);
+      nw = finish_ms_row(n, v, p, active, seen, nvisit, lvl, s_tab);
     } else {
       nvisit[v] = 0;
     }
@@ -368,32 +369,66 @@ __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, uint64_t *__res
 __global__ void k_ms_extract(uint32_t n, uint32_t k, uint32_t lane0,
                              const uint16_t *__restrict__ lvl, const uint32_t *__restrict__ perm,
                              uint32_t *__restrict__ out) {
-  // out[i * n + v] = lvl[perm(v) * 64 + lane0 + i]
+  // out[i * n + v] = level of vertex v (layout id) in lane lane0 + i
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (uint64_t)n * k) return;
   const uint32_t i = (uint32_t)(t / n), v = (uint32_t)(t % n);
   const uint32_t vv = perm ? perm[v] : v;
-  out[t] = lvl[(uint64_t)vv * 64 + lane0 + i];
+  out[t] = lvl[(uint64_t)(lane0 + i) * n + vv];
 }
 
-__global__ void k_ms_clear_lane(uint32_t n, uint32_t lane, uint64_t *__restrict__ seen,
-                                uint64_t *__restrict__ visit, uint64_t *__restrict__ nvisit,
-                                uint16_t *__restrict__ lvl) {
+// Retired lanes: drop their bits from every mask and zero their levels.  A
+// lane retires after a level in which it reached nothing, so its frontier
+// bits are already clear; nvisit and pend are rewritten by every level.
+__global__ void k_ms_clear_lanes(uint32_t n, uint64_t lanes, uint64_t *__restrict__ seen,
+                                 uint64_t *__restrict__ visit, uint16_t *__restrict__ lvl) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  const uint64_t keep = ~(1ull << lane);
-  seen[v] &= keep;
-  visit[v] &= keep;
-  nvisit[v] &= keep;
-  lvl[(uint64_t)v * 64 + lane] = 0;
+  seen[v] &= ~lanes;
+  visit[v] &= ~lanes;
+  uint64_t b = lanes;
+  while (b) {
+    const int i = __ffsll((unsigned long long)b) - 1;
+    lvl[(uint64_t)i * n + v] = 0;
+    b &= b - 1;
+  }
+}
+
+// Reached vertices of lane lanes[blockIdx.y]: per-workgroup partial counts.
+__global__ __launch_bounds__(kBlock) void k_ms_reached(uint32_t n, const uint16_t *__restrict__ lvl,
+                                                       const uint32_t *__restrict__ lanes,
+                                                       unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long s_c[kWaves];
+  const uint16_t *row = lvl + (uint64_t)lanes[blockIdx.y] * n;
+  unsigned long long c = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    c += row[v] != 0;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+  if (__lane_id() == 0) s_c[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    part[(uint64_t)blockIdx.y * gridDim.x + blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
+__global__ void k_ms_reached_sum(const unsigned long long *__restrict__ part, uint32_t nb,
+                                 unsigned long long *__restrict__ out) {
+  unsigned long long c = 0;
+  for (uint32_t b = threadIdx.x; b < nb; b += 64) c += part[(uint64_t)blockIdx.x * nb + b];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+  if (threadIdx.x == 0) out[blockIdx.x] = c;
 }
 
 // ---- PPR ----
-__global__ void k_ppr_zero(uint64_t count, float *__restrict__ X, float *__restrict__ C) {
+__global__ void k_ppr_zero(uint64_t count, float *__restrict__ X, float *__restrict__ C,
+                           float *__restrict__ Y, float *__restrict__ Cn) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < count) {
     X[t] = 0.f;
     C[t] = 0.f;
+    Y[t] = 0.f;
+    Cn[t] = 0.f;
   }
 }
 
@@ -410,19 +445,34 @@ __global__ void k_ppr_col_seed(uint32_t q, const uint32_t *__restrict__ seeds, u
   C[(uint64_t)v * 64 + q] = d ? w[q] / (float)d : 0.f;
 }
 
-__global__ void k_ppr_col_clear(uint32_t n, uint32_t q, uint64_t *__restrict__ mask,
-                                float *__restrict__ X, float *__restrict__ C) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+// Zeroes columns `cols` of all four iterate buffers and drops them from the
+// seed masks: a wave per row, lane q owns column q.  Columns at or above the
+// active width are neither read nor written by k_ppr_spmm, so every buffer
+// must hold zeros there.
+__global__ void k_ppr_cols_clear(uint32_t n, uint64_t cols, uint64_t *__restrict__ mask,
+                                 float *__restrict__ X, float *__restrict__ C,
+                                 float *__restrict__ Y, float *__restrict__ Cn) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v = (uint32_t)(t >> 6);
+  const int q = (int)(t & 63);
   if (v >= n) return;
-  mask[v] &= ~(1ull << q);
-  X[(uint64_t)v * 64 + q] = 0.f;
-  C[(uint64_t)v * 64 + q] = 0.f;
+  if (q == 0) {
+    const uint64_t m = mask[v];
+    if (m & cols) mask[v] = m & ~cols;
+  }
+  if ((cols >> q) & 1ull) {
+    X[(uint64_t)v * 64 + q] = 0.f;
+    C[(uint64_t)v * 64 + q] = 0.f;
+    Y[(uint64_t)v * 64 + q] = 0.f;
+    Cn[(uint64_t)v * 64 + q] = 0.f;
+  }
 }
 
 // Adds the contribution rows of entries [k0, k1) to this lane's column.
+// Lanes at or above the active width (on = false) issue no gathers.
 __device__ __forceinline__ void spmm_range(const uint32_t *__restrict__ idx,
                                            const float *__restrict__ C, uint32_t k0, uint32_t k1,
-                                           int lane, EgKahan &acc) {
+                                           int lane, bool on, EgKahan &acc) {
   for (uint32_t base = k0; base < k1; base += 64) {
     const uint32_t cnt = k1 - base < 64 ? k1 - base : 64;
     const uint32_t my = lane < (int)cnt ? __builtin_nontemporal_load(idx + base + lane) : 0u;
@@ -432,7 +482,7 @@ __device__ __forceinline__ void spmm_range(const uint32_t *__restrict__ idx,
 #pragma unroll
       for (int t = 0; t < 8; t++) {
         const uint32_t u = __shfl(my, (int)(j + t) & 63);
-        v[t] = C[(uint64_t)u * 64 + lane];
+        v[t] = on ? C[(uint64_t)u * 64 + lane] : 0.f;
       }
 #pragma unroll
       for (int t = 0; t < 8; t++)
@@ -443,11 +493,12 @@ __device__ __forceinline__ void spmm_range(const uint32_t *__restrict__ idx,
 
 // Writes row v of Y and Cn; returns |y - x| (x = the row of X) when DELTA.
 template <bool DELTA>
-__device__ __forceinline__ double spmm_finish(uint32_t v, float sum, int lane,
+__device__ __forceinline__ double spmm_finish(uint32_t v, float sum, int lane, bool on,
                                               const uint64_t *__restrict__ mask, float seed_term,
                                               const uint32_t *__restrict__ deg, float one_minus_a,
                                               const float *__restrict__ X, float *__restrict__ Y,
                                               float *__restrict__ Cn) {
+  if (!on) return 0.0;
   float y = one_minus_a * sum;
   if ((mask[v] >> lane) & 1ull) y += seed_term;
   Y[(uint64_t)v * 64 + lane] = y;
@@ -475,46 +526,55 @@ __device__ __forceinline__ void store_delta(double d, double *__restrict__ dpart
 
 // Work items of the pull plan: segments of long rows (the four waves split
 // the segment, partials merged in wave order) and chunks of short rows (a
-// wave per row).  Lane q owns query column q.
+// wave per row).  Lane q owns query column q; lanes >= width are idle.  A
+// workgroup takes items blockIdx.x, blockIdx.x + gridDim.x, ...: a grid
+// smaller than the item count leaves wave slots to concurrent kernels, and
+// every item is computed the same way whichever workgroup takes it.
 template <bool DELTA>
 __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
     const uint32_t *__restrict__ idx, const uint4 *__restrict__ chunk_info,
-    const uint4 *__restrict__ seg_info, uint32_t nseg, float2 *__restrict__ seg_part,
-    const uint32_t *__restrict__ ptr, const float *__restrict__ C,
-    const uint64_t *__restrict__ mask, const float *__restrict__ alpha_inv_seeds,
-    const uint32_t *__restrict__ deg, float one_minus_a, const float *__restrict__ X,
-    float *__restrict__ Y, float *__restrict__ Cn, double *__restrict__ dpart) {
+    const uint4 *__restrict__ seg_info, uint32_t nseg, uint32_t nitems,
+    float2 *__restrict__ seg_part, const uint32_t *__restrict__ ptr,
+    const float *__restrict__ C, const uint64_t *__restrict__ mask,
+    const float *__restrict__ alpha_inv_seeds, const uint32_t *__restrict__ deg,
+    float one_minus_a, uint32_t width, const float *__restrict__ X, float *__restrict__ Y,
+    float *__restrict__ Cn, double *__restrict__ dpart) {
   __shared__ float2 s_p[kWaves][64];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
-  if (blockIdx.x < nseg) {
-    const uint4 si = seg_info[blockIdx.x];
-    const uint32_t len = si.y - si.x;
-    const uint32_t part = (len + kWaves * 64 - 1) / (kWaves * 64) * 64;  // per wave, x64
-    const uint32_t a = si.x + wave * part < si.y ? si.x + wave * part : si.y;
-    const uint32_t b = a + part < si.y ? a + part : si.y;
-    EgKahan acc;
-    spmm_range(idx, C, a, b, lane, acc);
-    s_p[wave][lane] = make_float2(acc.s, acc.c);
-    __syncthreads();
-    if (wave == 0) {
-      EgKahan t;
-      t.s = s_p[0][lane].x;
-      t.c = s_p[0][lane].y;
-      for (int q = 1; q < kWaves; q++) t.merge(s_p[q][lane].x, s_p[q][lane].y);
-      seg_part[(uint64_t)blockIdx.x * 64 + lane] = make_float2(t.s, t.c);
+  const bool on = (uint32_t)lane < width;
+  for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+    if (item < nseg) {
+      const uint4 si = seg_info[item];
+      const uint32_t len = si.y - si.x;
+      const uint32_t part = (len + kWaves * 64 - 1) / (kWaves * 64) * 64;  // per wave, x64
+      const uint32_t a = si.x + wave * part < si.y ? si.x + wave * part : si.y;
+      const uint32_t b = a + part < si.y ? a + part : si.y;
+      EgKahan acc;
+      spmm_range(idx, C, a, b, lane, on, acc);
+      s_p[wave][lane] = make_float2(acc.s, acc.c);
+      __syncthreads();
+      if (wave == 0) {
+        EgKahan t;
+        t.s = s_p[0][lane].x;
+        t.c = s_p[0][lane].y;
+        for (int q = 1; q < kWaves; q++) t.merge(s_p[q][lane].x, s_p[q][lane].y);
+        seg_part[(uint64_t)item * 64 + lane] = make_float2(t.s, t.c);
+      }
+    } else {
+      const uint4 ci = chunk_info[item - nseg];
+      const float seed_term = alpha_inv_seeds[lane];
+      double d = 0.0;
+      for (uint32_t v = ci.x + wave; v < ci.y; v += kWaves) {
+        EgKahan acc;
+        spmm_range(idx, C, ptr[v], ptr[v + 1], lane, on, acc);
+        d += spmm_finish<DELTA>(v, acc.value(), lane, on, mask, seed_term, deg, one_minus_a, X,
+                                Y, Cn);
+      }
+      store_delta<DELTA>(d, dpart, item - nseg);
     }
-    return;
+    __syncthreads();  // LDS reuse by the next item
   }
-  const uint4 ci = chunk_info[blockIdx.x - nseg];
-  const float seed_term = alpha_inv_seeds[lane];
-  double d = 0.0;
-  for (uint32_t v = ci.x + wave; v < ci.y; v += kWaves) {
-    EgKahan acc;
-    spmm_range(idx, C, ptr[v], ptr[v + 1], lane, acc);
-    d += spmm_finish<DELTA>(v, acc.value(), lane, mask, seed_term, deg, one_minus_a, X, Y, Cn);
-  }
-  store_delta<DELTA>(d, dpart, blockIdx.x - nseg);
 }
 
 // Long rows: a wave per row merges its segment partials in segment order.
@@ -524,9 +584,10 @@ __global__ __launch_bounds__(kBlock) void k_ppr_finish(
     const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
     const float2 *__restrict__ seg_part, const uint64_t *__restrict__ mask,
     const float *__restrict__ alpha_inv_seeds, const uint32_t *__restrict__ deg,
-    float one_minus_a, const float *__restrict__ X, float *__restrict__ Y,
+    float one_minus_a, uint32_t width, const float *__restrict__ X, float *__restrict__ Y,
     float *__restrict__ Cn, double *__restrict__ dpart, uint32_t slot0) {
   const int lane = __lane_id();
+  const bool on = (uint32_t)lane < width;
   const uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   double d = 0.0;
   if (j < nlong) {
@@ -535,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_finish(
       const float2 p = seg_part[(uint64_t)s * 64 + lane];
       acc.merge(p.x, p.y);
     }
-    d = spmm_finish<DELTA>(long_rows[j], acc.value(), lane, mask, alpha_inv_seeds[lane], deg,
+    d = spmm_finish<DELTA>(long_rows[j], acc.value(), lane, on, mask, alpha_inv_seeds[lane], deg,
                            one_minus_a, X, Y, Cn);
   }
   store_delta<DELTA>(d, dpart, slot0 + blockIdx.x);
@@ -617,12 +678,14 @@ int eg_ms_alloc(eg_graph *g, MsState *st) {
   EG_TRY(eg_dalloc_t(&st->lvl, (uint64_t)n * 64, "ms levels"));
   EG_TRY(eg_dalloc_t(&st->queue, (uint64_t)n + 64, "ms queue"));
   EG_TRY(eg_dalloc_t(&st->qoff, (uint64_t)n + 65, "ms queue offsets"));
-  EG_TRY(eg_dalloc_t(&st->ctr, 2 * (uint64_t)kStatBlocks + 8, "ms counters"));
+  // [8..) holds the frontier statistics partials (2 per workgroup) or the
+  // per-lane reached partials (64 x 64) and sums (64), never both at once.
+  EG_TRY(eg_dalloc_t(&st->ctr, 8 + 2 * (uint64_t)kStatBlocks + 64 * 64 + 64, "ms counters"));
   EG_TRY(eg_dalloc_t(&st->lvl_tab, 64, "ms level table"));
   EG_TRY(eg_dalloc_t(&st->roots, 128, "ms roots"));
   EG_TRY(eg_dalloc_t(reinterpret_cast<uint8_t **>(&st->scan_tmp),
                      eg_scan_tmp_bytes((uint64_t)n + 64), "ms scan workspace"));
-  if (hipHostMalloc(reinterpret_cast<void **>(&st->h), 64, hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc(reinterpret_cast<void **>(&st->h), 72 * 8, hipHostMallocDefault) != hipSuccess)
     return eg_fail(EG_ENOMEM, "pinned counters");
   return EG_OK;
 }
@@ -692,7 +755,8 @@ int eg_ms_seed(eg_graph *g, const BatchView &bv, MsState *st, uint32_t k, const 
   memcpy(h, lanes, k * 4);
   memcpy(h + 64, roots, k * 4);
   EG_HIP(hipMemcpyAsync(st->roots, h, sizeof(h), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, k, st->roots, st->roots + 64, bv.perm,
+  hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, g->n, k, st->roots, st->roots + 64,
+                     bv.perm,
                      st->seen, st->visit, st->lvl);
   EG_LAUNCHED();
   EG_HIP(hipStreamSynchronize(s));  // h is on the stack
@@ -732,14 +796,15 @@ int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
     }
     {
       EgProf prof(g, "msbfs_pull", s);
-      hipLaunchKernelGGL(k_ms_pull, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, bv.ptr, bv.idx,
+      hipLaunchKernelGGL(k_ms_pull, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n, bv.ptr,
+                         bv.idx,
                          pl.chunk_info, pl.seg_info, pl.nseg, st->seg_or, st->seen, st->visit,
                          st->nvisit, st->lvl, st->lvl_tab, active, st->ctr);
       EG_LAUNCHED();
     }
     if (pl.nlong) {
       hipLaunchKernelGGL(k_ms_pull_finish, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock), 0, s,
-                         pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen, st->nvisit,
+                         n, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen, st->nvisit,
                          st->lvl, st->lvl_tab, active, st->ctr);
       EG_LAUNCHED();
     }
@@ -759,10 +824,31 @@ int eg_ms_extract_lane(eg_graph *g, const BatchView &bv, MsState *st, uint32_t l
   return EG_OK;
 }
 
-int eg_ms_clear_lane(eg_graph *g, MsState *st, uint32_t lane) {
-  hipLaunchKernelGGL(k_ms_clear_lane, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream, g->n,
-                     lane, st->seen, st->visit, st->nvisit, st->lvl);
+int eg_ms_clear_lanes(eg_graph *g, MsState *st, uint64_t lanes) {
+  if (!lanes) return EG_OK;
+  hipLaunchKernelGGL(k_ms_clear_lanes, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
+                     g->n, lanes, st->seen, st->visit, st->lvl);
   EG_LAUNCHED();
+  return EG_OK;
+}
+
+int eg_ms_reached(eg_graph *g, MsState *st, uint32_t k, const uint32_t *lanes,
+                  uint64_t *reached) {
+  if (k == 0) return EG_OK;
+  if (k > 64) return eg_fail(EG_EINVAL, "at most 64 lanes");
+  hipStream_t s = st->stream;
+  const uint32_t nb = blocks_for(g->n, kBlock) < 64 ? blocks_for(g->n, kBlock) : 64;
+  uint32_t h[64];
+  memcpy(h, lanes, k * 4);
+  EG_HIP(hipMemcpyAsync(st->roots, h, k * 4, hipMemcpyHostToDevice, s));
+  unsigned long long *part = st->ctr + 8, *sums = st->ctr + 8 + 64 * 64;
+  hipLaunchKernelGGL(k_ms_reached, dim3(nb, k), dim3(kBlock), 0, s, g->n, st->lvl, st->roots, part);
+  EG_LAUNCHED();
+  hipLaunchKernelGGL(k_ms_reached_sum, dim3(k), dim3(64), 0, s, part, nb, sums);
+  EG_LAUNCHED();
+  EG_HIP(hipMemcpyAsync(st->h + 8, sums, (size_t)k * 8, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < k; i++) reached[i] = st->h[8 + i];
   return EG_OK;
 }
 
@@ -800,7 +886,7 @@ void eg_ppr_free(PprState *st) {
 int eg_ppr_reset(eg_graph *g, PprState *st) {
   const uint64_t nn = (uint64_t)g->n * 64;
   hipLaunchKernelGGL(k_ppr_zero, dim3(blocks_for(nn, 256)), dim3(256), 0, st->stream, nn, st->X,
-                     st->C);
+                     st->C, st->Y, st->Cn);
   EG_LAUNCHED();
   EG_HIP(hipMemsetAsync(st->mask, 0, (size_t)g->n * 8, st->stream));
   memset(st->hw, 0, sizeof(st->hw));
@@ -834,6 +920,13 @@ int eg_ppr_col_init(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q,
   return EG_OK;
 }
 
+// Gathered lanes: every active column, rounded up to a multiple of 16.
+static uint32_t ppr_width(uint64_t active) {
+  if (!active) return 16;
+  const uint32_t hi = 63u - (uint32_t)__builtin_clzll(active);
+  return (hi / 16 + 1) * 16;
+}
+
 int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bool want_delta) {
   const PullPlan &pl = *bv.pl;
   hipStream_t s = st->stream;
@@ -852,27 +945,30 @@ int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bo
     st->dpart_cap = (uint64_t)np + kDeltaBlocks;
   }
   const float oma = (float)(1.0 - alpha);
-  {
+  const uint32_t width = ppr_width(st->active);
+  const uint32_t nitems = pl.nseg + pl.nchunks;
+  const uint32_t grid = st->max_grid && st->max_grid < nitems ? st->max_grid : nitems;
+  if (grid) {
     EgProf prof(g, "ppr_spmm", s);
     if (want_delta)
-      hipLaunchKernelGGL(k_ppr_spmm<true>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, bv.idx,
-                         pl.chunk_info, pl.seg_info, pl.nseg, st->seg_part, bv.ptr, st->C,
-                         st->mask, st->w + 64, bv.deg, oma, st->X, st->Y, st->Cn, st->dpart);
+      hipLaunchKernelGGL(k_ppr_spmm<true>, dim3(grid), dim3(kBlock), 0, s, bv.idx, pl.chunk_info,
+                         pl.seg_info, pl.nseg, nitems, st->seg_part, bv.ptr, st->C, st->mask,
+                         st->w + 64, bv.deg, oma, width, st->X, st->Y, st->Cn, st->dpart);
     else
-      hipLaunchKernelGGL(k_ppr_spmm<false>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, bv.idx,
-                         pl.chunk_info, pl.seg_info, pl.nseg, st->seg_part, bv.ptr, st->C,
-                         st->mask, st->w + 64, bv.deg, oma, st->X, st->Y, st->Cn, nullptr);
+      hipLaunchKernelGGL(k_ppr_spmm<false>, dim3(grid), dim3(kBlock), 0, s, bv.idx, pl.chunk_info,
+                         pl.seg_info, pl.nseg, nitems, st->seg_part, bv.ptr, st->C, st->mask,
+                         st->w + 64, bv.deg, oma, width, st->X, st->Y, st->Cn, nullptr);
     EG_LAUNCHED();
   }
   if (pl.nlong) {
     if (want_delta)
       hipLaunchKernelGGL(k_ppr_finish<true>, dim3(nfin), dim3(kBlock), 0, s, pl.long_rows,
                          pl.seg_base, pl.nlong, st->seg_part, st->mask, st->w + 64, bv.deg, oma,
-                         st->X, st->Y, st->Cn, st->dpart, pl.nchunks);
+                         width, st->X, st->Y, st->Cn, st->dpart, pl.nchunks);
     else
       hipLaunchKernelGGL(k_ppr_finish<false>, dim3(nfin), dim3(kBlock), 0, s, pl.long_rows,
                          pl.seg_base, pl.nlong, st->seg_part, st->mask, st->w + 64, bv.deg, oma,
-                         st->X, st->Y, st->Cn, nullptr, 0u);
+                         width, st->X, st->Y, st->Cn, nullptr, 0u);
     EG_LAUNCHED();
   }
   if (want_delta) {
@@ -896,12 +992,13 @@ int eg_ppr_extract_col(eg_graph *g, const BatchView &bv, PprState *st, uint32_t 
   return EG_OK;
 }
 
-int eg_ppr_clear_col(eg_graph *g, PprState *st, uint32_t q) {
-  hipLaunchKernelGGL(k_ppr_col_clear, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream, g->n,
-                     q, st->mask, st->X, st->C);
+int eg_ppr_clear_cols(eg_graph *g, PprState *st, uint64_t cols) {
+  if (!cols) return EG_OK;
+  hipLaunchKernelGGL(k_ppr_cols_clear, dim3(blocks_for((uint64_t)g->n * 64, 256)), dim3(256), 0,
+                     st->stream, g->n, cols, st->mask, st->X, st->C, st->Y, st->Cn);
   EG_LAUNCHED();
-  st->hw[q] = 0.f;
-  st->hw[64 + q] = 0.f;
+  for (int q = 0; q < 64; q++)
+    if ((cols >> q) & 1ull) st->hw[q] = st->hw[64 + q] = 0.f;
   return EG_OK;
 }
 
@@ -1034,6 +1131,7 @@ extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
     EG_HIP(hipEventCreate(&e1));
     EG_HIP(hipEventRecord(e0, s));
     int rc = eg_ppr_reset(g, &b->ppr);
+    b->ppr.active = kk == 64 ? ~0ull : ((1ull << kk) - 1ull);
     for (uint32_t q = 0; q < kk && rc == EG_OK; q++)
       rc = eg_ppr_col_init(g, bv, &b->ppr, q, seeds + seed_ptr[q0 + q],
                            seed_ptr[q0 + q + 1] - seed_ptr[q0 + q], alpha);

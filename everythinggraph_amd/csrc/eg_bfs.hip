@@ -539,6 +539,11 @@ int eg_bfs_device(eg_graph *g, uint32_t root, int start_pull, int switch_mode, b
 extern "C" int eg_bfs(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
                       uint32_t *dist, uint32_t *parent, eg_bfs_stats *stats) {
   EG_TRY(eg_require_loaded(g));
+  return eg_bfs_host(g, root, start_pull, switch_mode, dist, parent, stats);
+}
+
+int eg_bfs_host(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
+                uint32_t *dist, uint32_t *parent, eg_bfs_stats *stats) {
   if (root >= g->n) return eg_fail(EG_ERANGE, "root %u >= n = %u", root, g->n);
   EG_TRY(bfs_state(g));
   eg_bfs_stats st = {};

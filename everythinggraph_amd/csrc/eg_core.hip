@@ -73,6 +73,7 @@ bool eg_has_in_edges(const eg_graph *g) { return g->in_ptr != nullptr; }
 int eg_require_loaded(const eg_graph *g) {
   if (!g) return eg_fail(EG_EINVAL, "null graph");
   if (!g->loaded) return eg_fail(EG_ESTATE, "graph has no edges loaded");
+  if (g->server) return eg_fail(EG_ESTATE, "graph is owned by a running server");
   return EG_OK;
 }
 
@@ -285,10 +286,14 @@ static void free_algo_state(eg_graph *g) {
   eg_dfree_t(&g->bfs_ctr);
   eg_dfree_t(&g->wcc_label);
   eg_dfree_t(&g->wcc_lab32);
+  eg_dfree_t(&g->ppr_base);
+  eg_dfree_t(&g->ppr_seeds);
+  g->ppr_seeds_cap = 0;
 }
 
 extern "C" int eg_graph_free(eg_graph *g) {
   if (!g) return EG_OK;
+  if (g->server) return eg_fail(EG_ESTATE, "graph is owned by a running server; stop it first");
   (void)hipStreamSynchronize(g->stream);
   prof_resolve(g);
   free_layout(g);
@@ -328,6 +333,7 @@ extern "C" int eg_graph_get_info(const eg_graph *g, eg_graph_info *out) {
 int eg_build_layout(eg_graph *g);
 
 static int alloc_records(eg_graph *g, uint64_t m_in) {
+  if (g->server) return eg_fail(EG_ESTATE, "graph is owned by a running server");
   free_layout(g);
   const bool undir = (g->flags & EG_UNDIRECTED) != 0;
   const uint64_t m = undir ? 2 * m_in : m_in;

@@ -149,6 +149,10 @@ struct eg_graph {
 
   int prof_on = 0;
   std::vector<EgProfSlot> prof;
+  eg_server *server = nullptr;  // set while a server owns the graph
+  float *ppr_base = nullptr;     // single-vector PPR: alpha p
+  uint32_t *ppr_seeds = nullptr;
+  uint32_t ppr_seeds_cap = 0;
 };
 
 void eg_grid_free(eg_graph *g);
@@ -162,7 +166,16 @@ int eg_gather_vec(eg_graph *g, const void *src, const uint32_t *perm, void *dst)
 
 bool eg_has_in_edges(const eg_graph *g);
 int eg_scratch(eg_graph *g, size_t bytes, void **out);
+// EG_OK if g has a layout and no server owns it.
 int eg_require_loaded(const eg_graph *g);
+// Single-vector personalized PageRank on the PageRank pull kernel (the plain
+// per-query path): stops once the L1 change is <= tol or after max_iter
+// iterations.  out (host, may be null) receives the n ranks.
+int eg_ppr_single(eg_graph *g, const uint32_t *seeds, uint32_t ns, double alpha, double tol,
+                  int max_iter, float *out, int *iters, double *delta);
+// eg_bfs without the ownership check (the serving engine's plain path).
+int eg_bfs_host(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
+                uint32_t *dist, uint32_t *parent, eg_bfs_stats *stats);
 
 // Profiling scope around one launch.
 struct EgProf {

@@ -21,6 +21,7 @@
 // are deterministic.  PageRank's dangling mass and L1 delta are reduced in
 // FP64 on the device (per-chunk partials, then a last-block reduction).
 #include <math.h>
+#include <string.h>
 
 #include "eg_internal.h"
 
@@ -132,6 +133,7 @@ struct PrArgs {
   const double *dmass_in;  // dangling mass of the input iterate
   double *stats_out;       // [dangling of output, l1]
   float base;              // (1 - d) / n
+  const float *base_vec;   // per-row term added to base (personalized PageRank), or null
   float damping;
   double damping_over_n;   // d / n, for the uniform dangling term
   int uniform;
@@ -173,7 +175,9 @@ __device__ __forceinline__ void finish_row(uint32_t r, float sum, float *__restr
                                            const PrArgs &a, float basep, uint32_t deg,
                                            float rold, double &dang, double &l1) {
   if (PR) {
-    const float v = basep + a.damping * sum;
+    float b = basep;
+    if (a.base_vec) b += a.base_vec[r];
+    const float v = b + a.damping * sum;
     y[r] = v;
     a.contrib_new[r] = deg ? v / (float)deg : 0.f;
     if (deg == 0) dang += (double)v;
@@ -613,6 +617,7 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
   EG_TRY(eg_pr_start(g, init, deg, rank, contrib));
   PrArgs a;
   a.out_deg = deg;
+  a.base_vec = nullptr;
   a.base = (float)((1.0 - damping) / (double)n);
   a.damping = (float)damping;
   a.damping_over_n = damping / (double)n;
@@ -627,6 +632,130 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
     std::swap(contrib, contrib_next);
   }
   return eg_pr_finish(g, iters, rank, rank_next, rank_out, dh, lh, unperm);
+}
+
+// ---------------------------------------------------------------------------
+// Single-vector personalized PageRank on the PageRank pull kernel: x0 = p,
+// x' = alpha p + (1 - alpha) A^T D^-1 x with p uniform over the distinct
+// seeds, iterated until the FP64 L1 change is <= tol or max_iter iterations.
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void k_ppr1_clear(uint32_t n, float *__restrict__ rank, float *__restrict__ contrib,
+                             float *__restrict__ base) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  rank[v] = 0.f;
+  contrib[v] = 0.f;
+  base[v] = 0.f;
+}
+
+__global__ void k_ppr1_seed(const uint32_t *__restrict__ seeds, uint32_t ns,
+                            const uint32_t *__restrict__ perm, const uint32_t *__restrict__ deg,
+                            float w, float aw, float *__restrict__ rank,
+                            float *__restrict__ contrib, float *__restrict__ base) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const uint32_t v = perm ? perm[seeds[i]] : seeds[i];
+  rank[v] = w;
+  contrib[v] = deg[v] ? w / (float)deg[v] : 0.f;
+  base[v] = aw;
+}
+}  // namespace
+
+int eg_ppr_single(eg_graph *g, const uint32_t *seeds, uint32_t ns, double alpha, double tol,
+                  int max_iter, float *out, int *iters_out, double *delta_out) {
+  if (ns == 0 || max_iter < 0) return eg_fail(EG_EINVAL, "bad PPR query");
+  const uint32_t *ptr, *idx, *deg = g->out_deg, *unperm = nullptr, *perm = nullptr;
+  const float *w;
+  PullPlan *pl;
+  if (eg_reorder_enabled()) {
+    EgOrdered *o;
+    EG_TRY(eg_ordered(g, &o));
+    ptr = o->ptr;
+    idx = o->idx;
+    pl = &o->plan;
+    deg = o->deg;
+    unperm = perm = o->new_of_old;
+  } else {
+    EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+  }
+  EG_TRY(eg_pr_state(g, max_iter));
+  const uint32_t n = g->n;
+  hipStream_t s = g->stream;
+  if (!g->ppr_base) EG_TRY(eg_dalloc_t(&g->ppr_base, n, "ppr base"));
+  if (g->ppr_seeds_cap < ns) {
+    EG_HIP(hipStreamSynchronize(s));
+    eg_dfree_t(&g->ppr_seeds);
+    EG_TRY(eg_dalloc_t(&g->ppr_seeds, ns, "ppr seeds"));
+    g->ppr_seeds_cap = ns;
+  }
+  uint32_t distinct = 0;
+  for (uint32_t j = 0; j < ns; j++) {
+    bool dup = false;
+    for (uint32_t i = 0; i < j && !dup; i++) dup = seeds[i] == seeds[j];
+    distinct += !dup;
+  }
+  float *rank = g->rank_a, *rank_next = g->rank_b;
+  float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
+  hipLaunchKernelGGL(k_ppr1_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rank, contrib,
+                     g->ppr_base);
+  EG_LAUNCHED();
+  EG_HIP(hipMemcpyAsync(g->ppr_seeds, seeds, (size_t)ns * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_ppr1_seed, dim3(blocks_for(ns, 64)), dim3(64), 0, s, g->ppr_seeds, ns, perm,
+                     deg, (float)(1.0 / distinct), (float)(alpha / distinct), rank, contrib,
+                     g->ppr_base);
+  EG_LAUNCHED();
+  EG_HIP(hipMemsetAsync(g->pr_stats, 0, 2 * sizeof(double), s));
+  PrArgs a;
+  a.out_deg = deg;
+  a.base = 0.f;
+  a.base_vec = g->ppr_base;
+  a.damping = (float)(1.0 - alpha);
+  a.damping_over_n = 0.0;
+  a.uniform = 0;
+  int it = 0;
+  double delta = 0.0;
+  while (it < max_iter) {
+    a.rank_old = rank;
+    a.contrib_new = contrib_next;
+    a.dmass_in = g->pr_stats + 2 * it;
+    a.stats_out = g->pr_stats + 2 * (it + 1);
+    EG_TRY(launch_pull(g, true, ptr, idx, nullptr, contrib, rank_next, *pl, a));
+    std::swap(rank, rank_next);
+    std::swap(contrib, contrib_next);
+    it++;
+    EG_HIP(hipMemcpyAsync(g->h_ctr, a.stats_out + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    memcpy(&delta, g->h_ctr, sizeof(double));
+    if (delta <= tol) break;
+  }
+  if (out) {
+    const float *src = rank;
+    if (unperm) {
+      EG_TRY(eg_gather_vec(g, rank, unperm, rank_next));
+      src = rank_next;
+    }
+    EG_HIP(hipMemcpyAsync(out, src, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+  }
+  if (iters_out) *iters_out = it;
+  if (delta_out) *delta_out = delta;
+  return EG_OK;
+}
+
+extern "C" int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha,
+                      double tol, int32_t max_iter, float *out, int32_t *iterations,
+                      double *delta) {
+  EG_TRY(eg_require_loaded(g));
+  if (!g->out_ptr) return eg_fail(EG_ENOTSUP, "PPR needs adjacency lists (modes 2/3/4)");
+  if (nseeds && !seeds) return eg_fail(EG_EINVAL, "null seeds");
+  if (!(alpha >= 0.0 && alpha <= 1.0) || !(tol >= 0.0)) return eg_fail(EG_EINVAL, "bad PPR options");
+  for (uint32_t i = 0; i < nseeds; i++)
+    if (seeds[i] >= g->n) return eg_fail(EG_ERANGE, "seed %u >= n = %u", seeds[i], g->n);
+  int it = 0;
+  EG_TRY(eg_ppr_single(g, seeds, nseeds, alpha, tol, max_iter, out, &it, delta));
+  if (iterations) *iterations = it;
+  return EG_OK;
 }
 
 extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {

@@ -190,6 +190,100 @@ int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots,
 int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
                  const uint32_t *seeds, double alpha, int32_t iterations,
                  float *out, double *device_ms);
+/* One personalized PageRank vector on the single-vector pull kernel (the
+ * plain per-query path): x0 = p, iterated as eg_ppr_batch until the L1 change
+ * of an iteration is <= tol or max_iter iterations ran.  out (may be NULL):
+ * n floats.  iterations / delta (may be NULL): iterations run, last L1 change. */
+int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, double tol,
+           int32_t max_iter, float *out, int32_t *iterations, double *delta);
+
+/* ---- serving form (no reference counterpart) ----
+ * A server owns a loaded graph (which must not be used by other calls until
+ * eg_serve_stop) and runs one engine thread that batches continuously: BFS
+ * queries join a multi-source traversal at the next level boundary (64 lanes
+ * per lane group, each lane with its own level count) and PPR queries join
+ * the 64-column SpMM at the next iteration boundary, each column retiring when
+ * its own L1 change falls to ppr_tol.  The two kinds run on separate streams
+ * and overlap on the device.  Lane groups and the PPR block are allocated
+ * only while free device memory stays above hbm_reserve; a query waits for a
+ * free slot, and fails with EG_ENOMEM if no slot of its kind can ever be
+ * allocated.  A query's result does not depend on what else is in flight. */
+typedef struct eg_server eg_server;
+
+#define EG_REQ_BFS 0
+#define EG_REQ_PPR 1
+
+typedef struct {
+  uint32_t bfs_lanes;   /* BFS queries in flight, rounded up to 64; 0 = 64 */
+  uint32_t ppr_columns; /* PPR queries in flight, 1..64; 0 = 64 */
+  uint64_t hbm_reserve; /* device bytes left free; 0 = 1 GiB */
+  double ppr_alpha;     /* teleport probability; 0 = 0.15 */
+  double ppr_tol;       /* L1 change that ends a PPR query; 0 = 1e-6 */
+  int32_t ppr_max_iter; /* iteration cap; 0 = 200 */
+  int32_t plain;        /* 1: one query at a time, BFS through eg_bfs */
+  uint32_t max_pending; /* queued queries before submit says EG_EAGAIN; 0 = 1<<20 */
+  uint32_t ppr_grid;    /* workgroups of an SpMM iteration, leaving the rest of the
+                           device to BFS levels; 0 = 2 per compute unit */
+} eg_serve_opts;
+
+typedef struct {
+  int32_t kind;      /* EG_REQ_BFS or EG_REQ_PPR */
+  int32_t status;    /* EG_OK or this query's error */
+  uint32_t levels;   /* BFS: largest dist value; PPR: iterations run */
+  uint32_t pad;
+  uint64_t reached;  /* BFS: vertices reached; PPR: 0 */
+  double delta;      /* PPR: L1 change of the last iteration */
+  double queue_ms;   /* submit -> first device step */
+  double latency_ms; /* submit -> result written */
+} eg_serve_result;
+
+typedef struct {
+  uint64_t submitted, completed, failed;
+  uint32_t pending, bfs_inflight, ppr_inflight;
+  uint32_t bfs_lanes, ppr_columns; /* slots allocated so far */
+  uint32_t pad;
+  uint64_t bfs_levels, ppr_steps;  /* device steps run */
+  uint64_t device_bytes;           /* device memory held by the server */
+} eg_serve_stats;
+
+int eg_serve_start(eg_graph *g, const eg_serve_opts *opts, eg_server **out);
+/* out (may be NULL): host buffer of n entries, written before the query
+ * completes: uint32 levels in the eg_bfs encoding. */
+int eg_serve_submit_bfs(eg_server *s, uint32_t root, uint32_t *out, uint64_t *ticket);
+/* Personalized PageRank over the distinct seeds (as eg_ppr_batch); out (may
+ * be NULL): n floats. */
+int eg_serve_submit_ppr(eg_server *s, const uint32_t *seeds, uint32_t nseeds,
+                        float *out, uint64_t *ticket);
+/* Waits for a ticket (timeout_ms < 0: no limit) and releases it.  Returns
+ * EG_ETIMEDOUT while it is still running, else EG_OK with the query's own
+ * status in res->status. */
+int eg_serve_wait(eg_server *s, uint64_t ticket, double timeout_ms, eg_serve_result *res);
+int eg_serve_stats_get(eg_server *s, eg_serve_stats *st);
+/* Completes every submitted query, stops the engine and frees the server. */
+int eg_serve_stop(eg_server *s);
+
+/* Poisson request stream: exponential inter-arrival times at rate qps,
+ * EG_REQ_BFS with probability bfs_fraction (root uniform over vertices with
+ * out-edges), else PPR with 1..ppr_max_seeds seeds drawn the same way; all
+ * draws from `seed`.  Results are summaries only. */
+typedef struct {
+  double qps;
+  uint32_t requests;
+  uint32_t ppr_max_seeds; /* 0 = 8 */
+  double bfs_fraction;
+  uint64_t seed;
+} eg_replay_params;
+
+typedef struct {
+  uint32_t completed, failed, bfs, ppr;
+  double wall_s, qps; /* first submit -> last completion; completed / wall_s */
+  double p50_ms, p99_ms, mean_ms, max_ms;
+  double bfs_p50_ms, bfs_p99_ms, ppr_p50_ms, ppr_p99_ms;
+  eg_serve_stats stats;
+} eg_replay_report;
+
+int eg_serve_replay(eg_graph *g, const eg_serve_opts *opts, const eg_replay_params *p,
+                    eg_replay_report *r);
 
 /* ---- device timing of named kernels on the graph's stream ---- */
 typedef struct {
