@@ -54,6 +54,21 @@ def copy_calibration(nbytes: int) -> float:
     return ms
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE figures kept in profiles/traffic_rmat24.txt
+    (KiB per dispatch and their copy-calibrated factors), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_rmat24.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 6 and p[0] == kernel and p[1] == workload:
+                return int((float(p[2]) * float(p[4]) + float(p[3]) * float(p[5])) * 1024)
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,7 +250,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": measured_traffic("pr_pull", f"rmat{scale}_ef{ef}_u_s"),
             "kernel": "k_pull<true,false> (pr_pull)", "bytes_per_launch": b_pr,
             "avg_launch_us": round(k_ms * 1e3, 2),
         },
