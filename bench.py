@@ -114,6 +114,13 @@ def main():
         f"build {info.build_ms:.1f} ms device (wall {time.perf_counter() - t0:.1f} s)")
     csr_bytes = 2 * 8 * info.m_in + 4 * m + 4 * (n + 1) + 8 * n
 
+    # ---- kernel shapes: autotuned on this graph before anything is timed ----
+    t0 = time.perf_counter()
+    tune = g.autotune()
+    tuned = {p: v for p, v, _, chosen in tune if chosen}
+    log("autotune: " + ", ".join(f"{p}={v} {ms:.3f} ms{'*' if c else ''}" for p, v, ms, c in tune)
+        + f" ({time.perf_counter() - t0:.1f} s)")
+
     # ---- PageRank: warmup + timed steps ----
     iters = args.iterations
     b_pr = 4 * m + 4 * (n + 1) + 16 * n
@@ -247,6 +254,7 @@ def main():
             "workload": f"pagerank_pull_rmat{scale}_ef{ef}_undirected_{iters}it",
             "scale": scale, "edge_factor": ef, "n": n, "m": m, "iterations": iters,
             "layout": "csr (load mode 2, -u -s)", "parallelism": "single-gpu",
+            "tuned": tuned,
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

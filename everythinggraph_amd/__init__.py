@@ -55,7 +55,7 @@ ABI_SYMBOLS = (
     "eg_graph_get_degrees", "eg_graph_get_csr", "eg_graph_get_csc", "eg_graph_get_grid",
     "eg_graph_get_edgelist", "eg_pagerank", "eg_spmv", "eg_bfs", "eg_wcc",
     "eg_bfs_batch", "eg_ppr_batch", "eg_prof_enable", "eg_prof_reset", "eg_prof_read",
-    "eg_sync", "eg_ppr", "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
+    "eg_sync", "eg_ppr", "eg_autotune", "eg_tune_set", "eg_tune_get", "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
     "eg_serve_wait", "eg_serve_stats_get", "eg_serve_stop", "eg_serve_replay",
 )
 
@@ -127,6 +127,14 @@ class ServeStats(ctypes.Structure):
                 ("bfs_lanes", ctypes.c_uint32), ("ppr_columns", ctypes.c_uint32),
                 ("pad", ctypes.c_uint32), ("bfs_levels", ctypes.c_uint64),
                 ("ppr_steps", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64)]
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("param", ctypes.c_char * 24), ("value", ctypes.c_int32),
+                ("chosen", ctypes.c_int32), ("ms", ctypes.c_double)]
+
+
+TUNE_PULL, TUNE_SPMM, TUNE_MSBFS = 1, 2, 4
 
 
 class ReplayParams(ctypes.Structure):
@@ -379,6 +387,25 @@ class Graph:
                             ctypes.c_double(tol), ctypes.c_int32(max_iter), _ptr(out),
                             ctypes.byref(it), ctypes.byref(dl)), "eg_ppr")
         return out, it.value, dl.value
+
+    # -- kernel shapes --
+    def autotune(self, kernels=TUNE_PULL | TUNE_SPMM | TUNE_MSBFS) -> list:
+        """Times every variant of the selected kernels on this graph and keeps
+        the fastest: [(param, value, ms, chosen), ...]."""
+        arr = (TuneEntry * 16)()
+        cnt = ctypes.c_int32()
+        _check(lib().eg_autotune(self._h, ctypes.c_uint32(kernels), arr, ctypes.c_int32(16),
+                                 ctypes.byref(cnt)), "eg_autotune")
+        return [(arr[i].param.decode(), arr[i].value, arr[i].ms, bool(arr[i].chosen))
+                for i in range(min(cnt.value, 16))]
+
+    def tune_set(self, param: str, value: int):
+        _check(lib().eg_tune_set(self._h, param.encode(), ctypes.c_int32(value)), "eg_tune_set")
+
+    def tune_get(self, param: str) -> int:
+        v = ctypes.c_int32()
+        _check(lib().eg_tune_get(self._h, param.encode(), ctypes.byref(v)), "eg_tune_get")
+        return v.value
 
     # -- serving --
     def serve(self, **opts) -> "Server":

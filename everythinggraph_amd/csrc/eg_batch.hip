@@ -108,17 +108,46 @@ __global__ void k_ms_seed(uint32_t n, uint32_t k, const uint32_t *__restrict__ l
   lvl[(uint64_t)lane * n + r] = 1;
 }
 
-// Pull level over the plan: segments of long rows OR into seg_or; chunks of
-// short rows finish their rows directly.
+// Rows whose new lanes nw are final: record them (only this thread writes
+// row r in this level).
+__device__ __forceinline__ uint64_t finish_ms_new(uint32_t n, uint32_t r, uint64_t nw,
+                                                  uint64_t *__restrict__ seen,
+                                                  uint64_t *__restrict__ nvisit,
+                                                  uint16_t *__restrict__ lvl, const uint16_t *tab) {
+  nvisit[r] = nw;
+  if (nw) {
+    seen[r] |= nw;
+    uint64_t b = nw;
+    while (b) {
+      const int i = __ffsll((unsigned long long)b) - 1;
+      lvl[(uint64_t)i * n + r] = tab[i];
+      b &= b - 1;
+    }
+  }
+  return nw;
+}
+
+// Pull level over the plan.  A row needs the active lanes it has not seen;
+// it ORs the frontier masks of its in-neighbours until it holds all of them
+// (OR is exact, so stopping early changes nothing).
+//  * Segments of long rows OR into seg_or (skipped when the row needs nothing).
+//  * A chunk first checks whether any of its rows needs a lane; if none does it
+//    only clears their next-frontier words.  Otherwise it stages the chunk's
+//    entries in LDS (B per lane per pass, coalesced) and each needy row gathers
+//    from them: rows of up to kSerialRow entries by one lane, four gathers in
+//    flight, longer rows by a wave, 128 entries per check.
+template <int B>
 __global__ __launch_bounds__(kBlock) void k_ms_pull(
     uint32_t n, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
-    const uint4 *__restrict__ chunk_info, const uint4 *__restrict__ seg_info, uint32_t nseg,
-    uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
+    const uint4 *__restrict__ chunk_info, const uint4 *__restrict__ seg_info,
+    const uint32_t *__restrict__ long_rows, uint32_t nseg, uint64_t *__restrict__ seg_or,
+    uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
     uint64_t active, unsigned long long *__restrict__ ctr) {
-  __shared__ uint64_t s_val[kLds];
-  __shared__ uint32_t s_ptr[kChunk + 1];
-  __shared__ uint32_t s_list[kChunk];
+  static_assert(kLds <= 65535 && kChunk <= 65535, "u16 LDS offsets");
+  __shared__ uint32_t s_idx[kLds];
+  __shared__ uint16_t s_ptr[kChunk + 1];
+  __shared__ uint16_t s_list[kChunk];
   __shared__ uint32_t s_nlist;
   __shared__ uint16_t s_tab[64];
   __shared__ uint64_t s_or[kWaves];
@@ -127,16 +156,19 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
   const int wave = tid >> 6;
   if (blockIdx.x < nseg) {
     const uint4 si = seg_info[blockIdx.x];
+    const uint64_t need = active & ~seen[long_rows[si.z]];
     uint64_t acc = 0;
-    for (uint32_t base = si.x; base < si.y; base += kBlock * 8) {
-      uint32_t u[8];
+    if (need) {
+      for (uint32_t base = si.x; base < si.y; base += kBlock * B) {
+        uint32_t u[B];
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const uint32_t k = base + tid + q * kBlock;
-        u[q] = k < si.y ? __builtin_nontemporal_load(idx + k) : idx[si.x];
+        for (int q = 0; q < B; q++) {
+          const uint32_t k = base + tid + q * kBlock;
+          u[q] = k < si.y ? __builtin_nontemporal_load(idx + k) : idx[si.x];
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) acc |= visit[u[q]];
       }
-#pragma unroll
-      for (int q = 0; q < 8; q++) acc |= visit[u[q]];
     }
     acc = wave_or(acc);
     if (lane == 0) s_or[wave] = acc;
@@ -149,21 +181,25 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
   const uint32_t nrows = r1 - r0;
   if (tid == 0) s_nlist = 0;
   if (tid < 64) s_tab[tid] = tab[tid];
-  for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = ptr[r0 + i] - k0;
-  for (uint32_t base = k0; base < k1; base += kBlock * 8) {
-    uint32_t u[8];
+  int any = 0;
+  for (uint32_t i = tid; i < nrows; i += kBlock) any |= (active & ~seen[r0 + i]) != 0;
+  for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = (uint16_t)(ptr[r0 + i] - k0);
+  any = __syncthreads_or(any);
+  if (!any) {
+    for (uint32_t i = tid; i < nrows; i += kBlock) nvisit[r0 + i] = 0;
+    return;
+  }
+  for (uint32_t base = k0; base < k1; base += kBlock * B) {
+    uint32_t u[B];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < B; q++) {
       const uint32_t k = base + tid + q * kBlock;
       u[q] = k < k1 ? __builtin_nontemporal_load(idx + k) : 0u;
     }
-    uint64_t v[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) v[q] = visit[u[q]];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < B; q++) {
       const uint32_t k = base + tid + q * kBlock;
-      if (k < k1) s_val[k - k0] = v[q];
+      if (k < k1) s_idx[k - k0] = u[q];
     }
   }
   __syncthreads();
@@ -171,21 +207,42 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
   for (uint32_t i = tid; i < nrows; i += kBlock) {
     const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
     if (rb - ra <= kSerialRow) {
+      const uint32_t r = r0 + i;
+      const uint64_t need = active & ~seen[r];
       uint64_t acc = 0;
-      for (uint32_t q = ra; q < rb; q++) acc |= s_val[q];
-      nwall |= finish_ms_row(n, r0 + i, acc, active, seen, nvisit, lvl, s_tab);
+      if (need) {
+        for (uint32_t q = ra; q < rb; q += 4) {
+          const uint64_t v0 = visit[s_idx[q]];
+          const uint64_t v1 = q + 1 < rb ? visit[s_idx[q + 1]] : 0ull;
+          const uint64_t v2 = q + 2 < rb ? visit[s_idx[q + 2]] : 0ull;
+          const uint64_t v3 = q + 3 < rb ? visit[s_idx[q + 3]] : 0ull;
+          acc |= (v0 | v1) | (v2 | v3);
+          if ((acc & need) == need) break;
+        }
+      }
+      nwall |= finish_ms_new(n, r, acc & need, seen, nvisit, lvl, s_tab);
     } else {
-      s_list[atomicAdd(&s_nlist, 1u)] = i;
+      s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
     }
   }
   __syncthreads();
   const uint32_t nlist = s_nlist;
   for (uint32_t li = wave; li < nlist; li += kWaves) {
     const uint32_t i = s_list[li];
+    const uint32_t r = r0 + i;
+    const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
+    const uint64_t need = active & ~seen[r];
     uint64_t acc = 0;
-    for (uint32_t q = s_ptr[i] + lane; q < s_ptr[i + 1]; q += 64) acc |= s_val[q];
-    acc = wave_or(acc);
-    if (lane == 0) nwall |= finish_ms_row(n, r0 + i, acc, active, seen, nvisit, lvl, s_tab);
+    if (need) {
+      for (uint32_t base = ra; base < rb; base += 128) {
+        const uint32_t qa = base + lane, qb = base + 64 + lane;
+        const uint64_t va = qa < rb ? visit[s_idx[qa]] : 0ull;
+        const uint64_t vb = qb < rb ? visit[s_idx[qb]] : 0ull;
+        acc = wave_or(acc | va | vb);
+        if ((acc & need) == need) break;
+      }
+    }
+    if (lane == 0) nwall |= finish_ms_new(n, r, acc & need, seen, nvisit, lvl, s_tab);
   }
   publish_progress(nwall, s_or, ctr);
 }
@@ -468,24 +525,25 @@ __global__ void k_ppr_cols_clear(uint32_t n, uint64_t cols, uint64_t *__restrict
   }
 }
 
-// Adds the contribution rows of entries [k0, k1) to this lane's column.
-// Lanes at or above the active width (on = false) issue no gathers.
+// Adds the contribution rows of entries [k0, k1) to this lane's column, D
+// row gathers in flight per wave; entries are added in ascending order for
+// every D.  Lanes at or above the active width (on = false) issue no gathers.
+template <int D>
 __device__ __forceinline__ void spmm_range(const uint32_t *__restrict__ idx,
                                            const float *__restrict__ C, uint32_t k0, uint32_t k1,
                                            int lane, bool on, EgKahan &acc) {
   for (uint32_t base = k0; base < k1; base += 64) {
     const uint32_t cnt = k1 - base < 64 ? k1 - base : 64;
     const uint32_t my = lane < (int)cnt ? __builtin_nontemporal_load(idx + base + lane) : 0u;
-    for (uint32_t j = 0; j < cnt; j += 8) {
-      // eight independent 256-byte row gathers in flight per wave
-      float v[8];
+    for (uint32_t j = 0; j < cnt; j += D) {
+      float v[D];
 #pragma unroll
-      for (int t = 0; t < 8; t++) {
+      for (int t = 0; t < D; t++) {
         const uint32_t u = __shfl(my, (int)(j + t) & 63);
         v[t] = on ? C[(uint64_t)u * 64 + lane] : 0.f;
       }
 #pragma unroll
-      for (int t = 0; t < 8; t++)
+      for (int t = 0; t < D; t++)
         if (j + t < cnt) acc.add(v[t]);
     }
   }
@@ -530,7 +588,7 @@ __device__ __forceinline__ void store_delta(double d, double *__restrict__ dpart
 // workgroup takes items blockIdx.x, blockIdx.x + gridDim.x, ...: a grid
 // smaller than the item count leaves wave slots to concurrent kernels, and
 // every item is computed the same way whichever workgroup takes it.
-template <bool DELTA>
+template <bool DELTA, int D>
 __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
     const uint32_t *__restrict__ idx, const uint4 *__restrict__ chunk_info,
     const uint4 *__restrict__ seg_info, uint32_t nseg, uint32_t nitems,
@@ -551,7 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
       const uint32_t a = si.x + wave * part < si.y ? si.x + wave * part : si.y;
       const uint32_t b = a + part < si.y ? a + part : si.y;
       EgKahan acc;
-      spmm_range(idx, C, a, b, lane, on, acc);
+      spmm_range<D>(idx, C, a, b, lane, on, acc);
       s_p[wave][lane] = make_float2(acc.s, acc.c);
       __syncthreads();
       if (wave == 0) {
@@ -567,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
       double d = 0.0;
       for (uint32_t v = ci.x + wave; v < ci.y; v += kWaves) {
         EgKahan acc;
-        spmm_range(idx, C, ptr[v], ptr[v + 1], lane, on, acc);
+        spmm_range<D>(idx, C, ptr[v], ptr[v + 1], lane, on, acc);
         d += spmm_finish<DELTA>(v, acc.value(), lane, on, mask, seed_term, deg, one_minus_a, X,
                                 Y, Cn);
       }
@@ -796,10 +854,16 @@ int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
     }
     {
       EgProf prof(g, "msbfs_pull", s);
-      hipLaunchKernelGGL(k_ms_pull, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n, bv.ptr,
-                         bv.idx,
-                         pl.chunk_info, pl.seg_info, pl.nseg, st->seg_or, st->seen, st->visit,
-                         st->nvisit, st->lvl, st->lvl_tab, active, st->ctr);
+      if (g->tune.ms_batch == 16)
+        hipLaunchKernelGGL(k_ms_pull<16>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
+                           bv.ptr, bv.idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
+                           st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
+                           active, st->ctr);
+      else
+        hipLaunchKernelGGL(k_ms_pull<8>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
+                           bv.ptr, bv.idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
+                           st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
+                           active, st->ctr);
       EG_LAUNCHED();
     }
     if (pl.nlong) {
@@ -950,14 +1014,20 @@ int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bo
   const uint32_t grid = st->max_grid && st->max_grid < nitems ? st->max_grid : nitems;
   if (grid) {
     EgProf prof(g, "ppr_spmm", s);
-    if (want_delta)
-      hipLaunchKernelGGL(k_ppr_spmm<true>, dim3(grid), dim3(kBlock), 0, s, bv.idx, pl.chunk_info,
-                         pl.seg_info, pl.nseg, nitems, st->seg_part, bv.ptr, st->C, st->mask,
-                         st->w + 64, bv.deg, oma, width, st->X, st->Y, st->Cn, st->dpart);
+    const bool d16 = g->tune.spmm_depth == 16;
+#define EG_SPMM_ARGS                                                                       \
+  dim3(grid), dim3(kBlock), 0, s, bv.idx, pl.chunk_info, pl.seg_info, pl.nseg, nitems,     \
+      st->seg_part, bv.ptr, st->C, st->mask, st->w + 64, bv.deg, oma, width, st->X, st->Y, \
+      st->Cn
+    if (want_delta && d16)
+      hipLaunchKernelGGL((k_ppr_spmm<true, 16>), EG_SPMM_ARGS, st->dpart);
+    else if (want_delta)
+      hipLaunchKernelGGL((k_ppr_spmm<true, 8>), EG_SPMM_ARGS, st->dpart);
+    else if (d16)
+      hipLaunchKernelGGL((k_ppr_spmm<false, 16>), EG_SPMM_ARGS, nullptr);
     else
-      hipLaunchKernelGGL(k_ppr_spmm<false>, dim3(grid), dim3(kBlock), 0, s, bv.idx, pl.chunk_info,
-                         pl.seg_info, pl.nseg, nitems, st->seg_part, bv.ptr, st->C, st->mask,
-                         st->w + 64, bv.deg, oma, width, st->X, st->Y, st->Cn, nullptr);
+      hipLaunchKernelGGL((k_ppr_spmm<false, 8>), EG_SPMM_ARGS, nullptr);
+#undef EG_SPMM_ARGS
     EG_LAUNCHED();
   }
   if (pl.nlong) {

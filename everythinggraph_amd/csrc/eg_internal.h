@@ -94,6 +94,14 @@ struct EgOrdered {
   PullPlan plan;
 };
 
+// Kernel shapes of a graph, set by eg_autotune or eg_tune_set.  Every
+// variant computes bit-identical results; only the speed differs.
+struct EgTune {
+  int pull_batch = 16;   // k_pull: entries per lane per gather pass (8 or 16)
+  int spmm_depth = 8;    // k_ppr_spmm: row gathers in flight per wave (8 or 16)
+  int ms_batch = 8;      // k_ms_pull: entries per lane per gather pass (8 or 16)
+};
+
 struct eg_graph {
   uint32_t n = 0, P = 0, flags = 0;
   int32_t load_mode = 2;
@@ -150,6 +158,7 @@ struct eg_graph {
   int prof_on = 0;
   std::vector<EgProfSlot> prof;
   eg_server *server = nullptr;  // set while a server owns the graph
+  EgTune tune;
   float *ppr_base = nullptr;     // single-vector PPR: alpha p
   uint32_t *ppr_seeds = nullptr;
   uint32_t ppr_seeds_cap = 0;

@@ -35,7 +35,6 @@ constexpr uint32_t kSegLen = kPlanSegLen;
 constexpr uint32_t kSerialRow = 32;
 constexpr uint32_t kLdsVals = kChunk + kLongRow;
 constexpr int kFinBlocks = 128;
-constexpr int kBatch = 8;
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
@@ -139,24 +138,25 @@ struct PrArgs {
   int uniform;
 };
 
-// Values of entries k, k + kBlock, ..., k + (kBatch-1)*kBlock (0 past kend).
-// All index loads are issued before all gathers so that a lane keeps kBatch
-// independent memory requests in flight.
-template <bool WEIGHTED>
+// Values of entries k, k + kBlock, ..., k + (B-1)*kBlock (0 past kend).
+// All index loads are issued before all gathers so that a lane keeps B
+// independent memory requests in flight.  A lane meets its entries in the same
+// ascending order for every B, so B changes no result.
+template <bool WEIGHTED, int B>
 __device__ __forceinline__ void gather_batch(const uint32_t *__restrict__ idx,
                                              const float *__restrict__ w,
                                              const float *__restrict__ x, uint32_t k,
-                                             uint32_t kend, float (&v)[kBatch]) {
-  uint32_t u[kBatch];
-  float wv[kBatch];
+                                             uint32_t kend, float (&v)[B]) {
+  uint32_t u[B];
+  float wv[B];
 #pragma unroll
-  for (int q = 0; q < kBatch; q++) {
+  for (int q = 0; q < B; q++) {
     const uint32_t kk = k + q * kBlock;
     u[q] = kk < kend ? __builtin_nontemporal_load(idx + kk) : 0u;
     if (WEIGHTED) wv[q] = kk < kend ? __builtin_nontemporal_load(w + kk) : 0.f;
   }
 #pragma unroll
-  for (int q = 0; q < kBatch; q++) {
+  for (int q = 0; q < B; q++) {
     const float xv = x[u[q]];
     v[q] = (k + q * kBlock < kend) ? (WEIGHTED ? xv * wv[q] : xv) : 0.f;
   }
@@ -187,15 +187,19 @@ __device__ __forceinline__ void finish_row(uint32_t r, float sum, float *__restr
   }
 }
 
-template <bool PR, bool WEIGHTED>
+// B: entries per lane per gather pass (16 covers a whole chunk in one pass).
+// Row offsets and the long-row list are u16 in LDS (a chunk has at most
+// kLdsVals entries and kChunk rows), which leaves room for more workgroups.
+template <bool PR, bool WEIGHTED, int B>
 __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ ptr,
                                                  const uint32_t *__restrict__ idx,
                                                  const float *__restrict__ w,
                                                  const float *__restrict__ x,
                                                  float *__restrict__ y, PlanDev pl, PrArgs a) {
+  static_assert(kLdsVals <= 65535 && kChunk <= 65535, "u16 LDS offsets");
   __shared__ float s_val[kLdsVals];
-  __shared__ uint32_t s_ptr[kChunk + 1];
-  __shared__ uint32_t s_list[kChunk];
+  __shared__ uint16_t s_ptr[kChunk + 1];
+  __shared__ uint16_t s_list[kChunk];
   __shared__ uint32_t s_nlist;
   __shared__ float s_ks[kWaves], s_kc[kWaves];
   __shared__ double s_d[kWaves], s_l[kWaves];
@@ -208,11 +212,11 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
     // ---- one segment of a long row ----
     const uint4 si = pl.seg_info[blockIdx.x];
     EgKahan acc;
-    for (uint32_t base = si.x; base < si.y; base += kBlock * kBatch) {
-      float v[kBatch];
-      gather_batch<WEIGHTED>(idx, w, x, base + tid, si.y, v);
+    for (uint32_t base = si.x; base < si.y; base += kBlock * B) {
+      float v[B];
+      gather_batch<WEIGHTED, B>(idx, w, x, base + tid, si.y, v);
 #pragma unroll
-      for (int q = 0; q < kBatch; q++) acc.add(v[q]);
+      for (int q = 0; q < B; q++) acc.add(v[q]);
     }
     eg_wave_kahan_allreduce(acc);
     if (lane == 0) {
@@ -237,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
   const uint32_t nrows = r1 - r0;
   if (tid == 0) s_nlist = 0;
   // row pointers, relative to k0
-  for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = ptr[r0 + i] - k0;
+  for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = (uint16_t)(ptr[r0 + i] - k0);
   // prefetch the per-row inputs of this lane's first row
   uint32_t deg0 = 0;
   float rold0 = 0.f;
@@ -245,11 +249,11 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
     deg0 = a.out_deg[r0 + tid];
     rold0 = a.rank_old[r0 + tid];
   }
-  for (uint32_t base = k0; base < k1; base += kBlock * kBatch) {
-    float v[kBatch];
-    gather_batch<WEIGHTED>(idx, w, x, base + tid, k1, v);
+  for (uint32_t base = k0; base < k1; base += kBlock * B) {
+    float v[B];
+    gather_batch<WEIGHTED, B>(idx, w, x, base + tid, k1, v);
 #pragma unroll
-    for (int q = 0; q < kBatch; q++) {
+    for (int q = 0; q < B; q++) {
       const uint32_t k = base + tid + q * kBlock;
       if (k < k1) s_val[k - k0] = v[q];
     }
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
       }
       finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
     } else {
-      s_list[atomicAdd(&s_nlist, 1u)] = i;
+      s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
     }
   }
   __syncthreads();
@@ -473,22 +477,32 @@ int eg_pull_orientation(eg_graph *g, const uint32_t **ptr, const uint32_t **idx,
   return eg_plan_build(g, *pl, *ptr);
 }
 
+template <int B>
+static void launch_k_pull(hipStream_t s, bool pr, uint32_t grid, const uint32_t *ptr,
+                          const uint32_t *idx, const float *w, const float *x, float *y,
+                          const PlanDev &d, const PrArgs &a) {
+  if (pr)
+    hipLaunchKernelGGL((k_pull<true, false, B>), dim3(grid), dim3(kBlock), 0, s, ptr, idx, w, x,
+                       y, d, a);
+  else if (w)
+    hipLaunchKernelGGL((k_pull<false, true, B>), dim3(grid), dim3(kBlock), 0, s, ptr, idx, w, x,
+                       y, d, a);
+  else
+    hipLaunchKernelGGL((k_pull<false, false, B>), dim3(grid), dim3(kBlock), 0, s, ptr, idx, w, x,
+                       y, d, a);
+}
+
 static int launch_pull(eg_graph *g, bool pr, const uint32_t *ptr, const uint32_t *idx,
                        const float *w, const float *x, float *y, const PullPlan &pl,
                        const PrArgs &a) {
   const PlanDev d = plan_dev(pl);
   const uint32_t grid = pl.nseg + pl.nchunks;
-  {
+  if (grid) {
     EgProf prof(g, pr ? "pr_pull" : "spmv_pull");
-    if (pr)
-      hipLaunchKernelGGL((k_pull<true, false>), dim3(grid), dim3(kBlock), 0, g->stream, ptr, idx,
-                         w, x, y, d, a);
-    else if (w)
-      hipLaunchKernelGGL((k_pull<false, true>), dim3(grid), dim3(kBlock), 0, g->stream, ptr, idx,
-                         w, x, y, d, a);
+    if (g->tune.pull_batch == 8)
+      launch_k_pull<8>(g->stream, pr, grid, ptr, idx, w, x, y, d, a);
     else
-      hipLaunchKernelGGL((k_pull<false, false>), dim3(grid), dim3(kBlock), 0, g->stream, ptr, idx,
-                         w, x, y, d, a);
+      launch_k_pull<16>(g->stream, pr, grid, ptr, idx, w, x, y, d, a);
     EG_LAUNCHED();
   }
   if (pr || pl.nlong) {

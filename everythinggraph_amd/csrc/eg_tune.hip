@@ -1,0 +1,152 @@
+// eg_tune.hip -- kernel-shape autotuner.  No reference counterpart.
+//
+// Each tunable kernel has variants that compute bit-identical results (the
+// shape changes how many memory requests a lane keeps in flight, never the
+// order of a row's additions), so the choice is free to depend on timing:
+//  * pull_batch  -- k_pull (PageRank / SpMV / single-vector PPR): entries per
+//                   lane per gather pass, 8 or 16;
+//  * spmm_depth  -- k_ppr_spmm: row gathers in flight per wave, 8 or 16;
+//  * ms_batch    -- k_ms_pull: entries per lane per index pass, 8 or 16.
+// eg_autotune times every variant on the graph itself through the public
+// entry points (device time from HIP events, best of a few repetitions) and
+// keeps the fastest.
+#include <string.h>
+
+#include <vector>
+
+#include "eg_internal.h"
+
+namespace {
+
+int *param_slot(eg_graph *g, const char *name) {
+  if (!strcmp(name, "pull_batch")) return &g->tune.pull_batch;
+  if (!strcmp(name, "spmm_depth")) return &g->tune.spmm_depth;
+  if (!strcmp(name, "ms_batch")) return &g->tune.ms_batch;
+  return nullptr;
+}
+
+// Device milliseconds of fn() on the graph's stream, best of reps.
+template <typename F>
+int time_best(eg_graph *g, int reps, F fn, double *best) {
+  hipEvent_t e0, e1;
+  EG_HIP(hipEventCreate(&e0));
+  EG_HIP(hipEventCreate(&e1));
+  *best = 1e300;
+  int rc = EG_OK;
+  for (int r = 0; r < reps && rc == EG_OK; r++) {
+    (void)hipEventRecord(e0, g->stream);
+    rc = fn();
+    (void)hipEventRecord(e1, g->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < *best) *best = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+struct Result {
+  const char *param;
+  int value;
+  double ms;
+};
+
+}  // namespace
+
+extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
+  if (!g || !param) return eg_fail(EG_EINVAL, "null argument");
+  int *slot = param_slot(g, param);
+  if (!slot) return eg_fail(EG_EINVAL, "unknown tuning parameter '%s'", param);
+  if (value != 8 && value != 16) return eg_fail(EG_EINVAL, "%s takes 8 or 16", param);
+  *slot = value;
+  return EG_OK;
+}
+
+extern "C" int eg_tune_get(eg_graph *g, const char *param, int32_t *value) {
+  if (!g || !param || !value) return eg_fail(EG_EINVAL, "null argument");
+  int *slot = param_slot(g, param);
+  if (!slot) return eg_fail(EG_EINVAL, "unknown tuning parameter '%s'", param);
+  *value = *slot;
+  return EG_OK;
+}
+
+extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, int32_t cap,
+                           int32_t *count) {
+  EG_TRY(eg_require_loaded(g));
+  if (cap < 0 || (cap > 0 && !out)) return eg_fail(EG_EINVAL, "bad output array");
+  const bool adj = g->out_ptr != nullptr;
+  std::vector<Result> res;
+  const int variants[2] = {8, 16};
+  if (kernels & EG_TUNE_PULL) {
+    if (!adj) return eg_fail(EG_ENOTSUP, "pull tuning needs adjacency lists");
+    eg_pr_opts o = {0.85, 5, EG_PR_DANGLING_DROP, 0.0, EG_LAYOUT_ADJ};
+    for (int v : variants) {
+      g->tune.pull_batch = v;
+      double ms;
+      EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // warm (plan, order, state)
+      EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); }, &ms));
+      res.push_back({"pull_batch", v, ms / o.iterations});
+    }
+  }
+  std::vector<uint32_t> roots;
+  if (kernels & (EG_TUNE_SPMM | EG_TUNE_MSBFS)) {
+    if (!adj) return eg_fail(EG_ENOTSUP, "batched tuning needs adjacency lists");
+    std::vector<uint32_t> deg(g->n);
+    EG_HIP(hipMemcpyAsync(deg.data(), g->out_deg, (size_t)g->n * 4, hipMemcpyDeviceToHost,
+                          g->stream));
+    EG_HIP(hipStreamSynchronize(g->stream));
+    const uint64_t stride = g->n / 64 ? g->n / 64 : 1;
+    for (uint64_t v = 0; v < g->n && roots.size() < 64; v += stride)
+      if (deg[v]) roots.push_back((uint32_t)v);
+    for (uint32_t v = 0; v < g->n && roots.size() < 64; v++)
+      if (deg[v]) roots.push_back(v);
+    if (roots.empty()) roots.push_back(0);
+  }
+  if (kernels & EG_TUNE_SPMM) {
+    std::vector<uint32_t> sp(roots.size() + 1);
+    for (size_t i = 0; i <= roots.size(); i++) sp[i] = (uint32_t)i;
+    const uint32_t k = (uint32_t)roots.size();
+    for (int v : variants) {
+      g->tune.spmm_depth = v;
+      double ms;
+      EG_TRY(eg_ppr_batch(g, k, sp.data(), roots.data(), 0.15, 1, nullptr, nullptr));
+      EG_TRY(time_best(
+          g, 2, [&] { return eg_ppr_batch(g, k, sp.data(), roots.data(), 0.15, 3, nullptr, nullptr); },
+          &ms));
+      res.push_back({"spmm_depth", v, ms / 3});
+    }
+  }
+  if (kernels & EG_TUNE_MSBFS) {
+    const uint32_t k = (uint32_t)roots.size();
+    for (int v : variants) {
+      g->tune.ms_batch = v;
+      double ms;
+      EG_TRY(eg_bfs_batch(g, k, roots.data(), nullptr, nullptr));
+      EG_TRY(time_best(g, 2, [&] { return eg_bfs_batch(g, k, roots.data(), nullptr, nullptr); },
+                       &ms));
+      res.push_back({"ms_batch", v, ms});
+    }
+  }
+  // keep the fastest variant of each parameter
+  for (size_t i = 0; i < res.size(); i++) {
+    bool best = true;
+    for (size_t j = 0; j < res.size(); j++)
+      if (!strcmp(res[j].param, res[i].param) && res[j].ms < res[i].ms) best = false;
+    if (best) *param_slot(g, res[i].param) = res[i].value;
+  }
+  int32_t nout = 0;
+  for (const Result &r : res) {
+    if (nout < cap) {
+      memset(&out[nout], 0, sizeof(out[nout]));
+      strncpy(out[nout].param, r.param, sizeof(out[nout].param) - 1);
+      out[nout].value = r.value;
+      out[nout].ms = r.ms;
+      out[nout].chosen = *param_slot(g, r.param) == r.value;
+    }
+    nout++;
+  }
+  if (count) *count = nout;
+  return EG_OK;
+}

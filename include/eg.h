@@ -197,6 +197,26 @@ int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
 int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, double tol,
            int32_t max_iter, float *out, int32_t *iterations, double *delta);
 
+/* ---- kernel-shape autotuning (no reference counterpart) ----
+ * Tunable shapes: "pull_batch" (pull PageRank / SpMV / PPR gather depth),
+ * "spmm_depth" (batched PPR row gathers in flight), "ms_batch" (multi-source
+ * BFS index pass depth); each takes 8 or 16.  Every variant computes
+ * bit-identical results.  eg_autotune times the variants of the selected
+ * kernels on the graph itself, keeps the fastest, and reports each timing. */
+#define EG_TUNE_PULL 1u
+#define EG_TUNE_SPMM 2u
+#define EG_TUNE_MSBFS 4u
+typedef struct {
+  char param[24];
+  int32_t value;
+  int32_t chosen; /* 1 for the variant kept */
+  double ms;      /* device ms per iteration (pull, SpMM) or per 64-source sweep */
+} eg_tune_entry;
+int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, int32_t cap,
+                int32_t *count);
+int eg_tune_set(eg_graph *g, const char *param, int32_t value);
+int eg_tune_get(eg_graph *g, const char *param, int32_t *value);
+
 /* ---- serving form (no reference counterpart) ----
  * A server owns a loaded graph (which must not be used by other calls until
  * eg_serve_stop) and runs one engine thread that batches continuously: BFS

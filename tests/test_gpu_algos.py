@@ -352,3 +352,51 @@ def test_degree_order_is_bit_identical(tmp_path):
         res[mode] = np.load(outp)
     for k in ("pr", "spmv", "ppr", "ms"):
         assert np.array_equal(res["none"][k], res["degree"][k]), k
+
+
+# ---------------------------------------------------------------------------
+# kernel shapes: every variant gives bit-identical results
+# ---------------------------------------------------------------------------
+def test_tuning_variants_bit_identical():
+    n = 1 << 13
+    s0, d0 = rmat(13, seed=51)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    roots = eg.sample_roots(np.diff(ptr), 40, seed=2)
+    seeds = [[1, 2], [7], [100, 3, 3, 9]]
+    outs = {}
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        for v in (8, 16):
+            for p in ("pull_batch", "spmm_depth", "ms_batch"):
+                g.tune_set(p, v)
+                assert g.tune_get(p) == v
+            outs[v] = (g.pagerank(0.85, 20)[0], g.ppr_batch(seeds, 0.15, 20)[0],
+                       g.bfs_batch(roots)[0], g.ppr([5, 6], 0.15, tol=0.0, max_iter=25)[0])
+        with pytest.raises(eg.EgError) as e:
+            g.tune_set("pull_batch", 12)
+        assert e.value.code == eg.EG_EINVAL
+        with pytest.raises(eg.EgError) as e:
+            g.tune_set("no_such_knob", 8)
+        assert e.value.code == eg.EG_EINVAL
+    for a, b in zip(outs[8], outs[16]):
+        assert np.array_equal(a, b)
+    for i, r in enumerate(roots):
+        ref, _, _ = O.bfs(n, s0.size * 2, ptr, col, cp, row, int(r))
+        assert np.array_equal(outs[16][2][i], ref)
+
+
+def test_autotune_reports_and_keeps_fastest():
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=52)
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        rep = g.autotune()
+        assert len(rep) == 6
+        for p in ("pull_batch", "spmm_depth", "ms_batch"):
+            rows = [r for r in rep if r[0] == p]
+            assert sorted(r[1] for r in rows) == [8, 16] and all(r[2] > 0 for r in rows)
+            best = min(rows, key=lambda r: r[2])
+            chosen = [r for r in rows if r[3]]
+            assert len(chosen) == 1 and chosen[0][1] == g.tune_get(p) == best[1]
+        r, _, _ = g.pagerank(0.85, 20)  # still runs after tuning
+        assert np.isfinite(r).all()
