@@ -86,6 +86,12 @@ def main():
                     help="offered Poisson rate of the serving replay")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-configs", action="store_true",
+                    help="skip the grid (ER 2^24 / 2^28, 16x16) and RMAT-22 BFS/WCC figures")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in kernel shapes (plus any --tune)")
+    ap.add_argument("--tune", action="append", default=[], metavar="PARAM=VALUE",
+                    help="force a kernel shape, e.g. pull_hot=262144")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -116,10 +122,15 @@ def main():
 
     # ---- kernel shapes: autotuned on this graph before anything is timed ----
     t0 = time.perf_counter()
-    tune = g.autotune()
-    tuned = {p: v for p, v, _, chosen in tune if chosen}
-    log("autotune: " + ", ".join(f"{p}={v} {ms:.3f} ms{'*' if c else ''}" for p, v, ms, c in tune)
-        + f" ({time.perf_counter() - t0:.1f} s)")
+    if not args.no_autotune:
+        tune = g.autotune()
+        log("autotune: " + ", ".join(f"{p}={v} {ms:.3f} ms{'*' if c else ''}"
+                                     for p, v, ms, c in tune)
+            + f" ({time.perf_counter() - t0:.1f} s)")
+    for kv in args.tune:
+        p, v = kv.split("=")
+        g.tune_set(p, int(v))
+    tuned = {p: g.tune_get(p) for p in ("pull_batch", "pull_hot", "spmm_depth", "ms_batch")}
 
     # ---- PageRank: warmup + timed steps ----
     iters = args.iterations
@@ -271,7 +282,74 @@ def main():
     }
     out.update(extra)
     g.close()
+    if not args.skip_configs:
+        out["configs"] = other_configs(eg)
     print(json.dumps(out), flush=True)
+
+
+def other_configs(eg) -> dict:
+    """Device figures of the other configurations on fresh graphs: PageRank /
+    SpMV on the 2-D grid (directed Erdos-Renyi, 2^24 vertices, 2^28 weighted
+    edges, 16 x 16 cells) and BFS + WCC on undirected R-MAT 22 (CSR)."""
+    res = {}
+    n, m = 1 << 24, 1 << 28
+    g = eg.Graph(n, eg.MODE_GRID_SORT, weighted=True, P=16)
+    g.generate(eg.er_params(m, seed=1, weight_seed=1))
+    build_ms = g.info.build_ms
+    b_pr = 4 * m + 4 * (n + 1) + 16 * n
+    b_spmv = 8 * m + 4 * (n + 1) + 8 * n
+    g.pagerank(0.85, 2, layout="grid", download=False)
+    g.prof_reset()
+    g.prof_enable(True)
+    _, _, lh = g.pagerank(0.85, 20, layout="grid", download=False)
+    x = np.ones(n, np.float32)
+    for _ in range(5):
+        g.spmv(x, layout="grid")
+    prof = g.prof_read()
+    g.prof_enable(False)
+    pl, pt = prof.get("pr_grid", (0, 0.0))
+    sl, stt = prof.get("spmv_grid", (0, 0.0))
+    pr_ms, sp_ms = pt / max(pl, 1), stt / max(sl, 1)
+    res["grid_er24"] = {
+        "n": n, "m": m, "P": 16, "build_ms": round(build_ms, 2),
+        "pr_iter_ms": round(pr_ms, 4), "pr_gbs": round(b_pr / (pr_ms * 1e-3) / 1e9, 1),
+        "spmv_ms": round(sp_ms, 4), "spmv_gbs": round(b_spmv / (sp_ms * 1e-3) / 1e9, 1),
+        "pr_final_l1_delta": float(lh[-1]),
+    }
+    log(f"grid ER 2^24/2^28 16x16: build {build_ms:.1f} ms; PR {pr_ms * 1e3:.1f} us/iter = "
+        f"{b_pr / (pr_ms * 1e-3) / 1e9:.1f} GB/s; SpMV {sp_ms * 1e3:.1f} us = "
+        f"{b_spmv / (sp_ms * 1e-3) / 1e9:.1f} GB/s")
+    g.close()
+
+    n = 1 << 22
+    g = eg.Graph(n, eg.MODE_ADJ_SORT, undirected=True)
+    g.generate(eg.rmat_params(22, 16, seed=1))
+    build_ms = g.info.build_ms
+    deg, _ = g.degrees()
+    roots = eg.sample_roots(deg, 64, seed=2)
+    g.bfs(int(roots[0]), download=False)
+    teps = []
+    for rt in roots:
+        _, _, st = g.bfs(int(rt), download=False, parents=False)
+        teps.append(st.edges_traversed / (st.device_ms * 1e-3))
+    gteps = len(teps) / sum(1.0 / t for t in teps) / 1e9
+    g.wcc()
+    g.prof_reset()
+    g.prof_enable(True)
+    w0 = time.perf_counter()
+    _, rounds = g.wcc()
+    wcc_wall = time.perf_counter() - w0
+    prof = g.prof_read()
+    g.prof_enable(False)
+    wl, wt = prof.get("wcc_relax", (0, 0.0))
+    res["rmat22_u"] = {
+        "n": n, "m": g.info.m, "csr_build_ms": round(build_ms, 2), "bfs_gteps": round(gteps, 2),
+        "wcc_rounds": rounds, "wcc_relax_ms": round(wt, 3), "wcc_wall_ms": round(wcc_wall * 1e3, 2),
+    }
+    log(f"RMAT-22 -u: CSR+CSC build {build_ms:.1f} ms; BFS {gteps:.1f} GTEPS; WCC {rounds} rounds, "
+        f"relax {wt:.2f} ms, call {wcc_wall * 1e3:.1f} ms (incl. label download)")
+    g.close()
+    return res
 
 
 if __name__ == "__main__":

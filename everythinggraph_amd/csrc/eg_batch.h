@@ -70,6 +70,7 @@ struct PprState {
   double *h_delta = nullptr;  // pinned mirror
   uint64_t active = ~0ull;    // columns in use: steps gather only up to the highest one
   uint32_t max_grid = 0;      // workgroups of the SpMM launch (0: one per work item)
+  bool capturing = false;     // a step is being recorded into a hipGraph
 };
 int eg_ppr_alloc(eg_graph *g, PprState *st);
 void eg_ppr_free(PprState *st);

@@ -991,8 +991,8 @@ static uint32_t ppr_width(uint64_t active) {
   return (hi / 16 + 1) * 16;
 }
 
-int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bool want_delta) {
-  const PullPlan &pl = *bv.pl;
+// Partial-sum buffers of a step (allocated outside any stream capture).
+static int ppr_buffers(const PullPlan &pl, PprState *st, bool want_delta) {
   hipStream_t s = st->stream;
   if (st->seg_cap < (uint64_t)pl.nseg * 64) {
     EG_HIP(hipStreamSynchronize(s));
@@ -1008,12 +1008,21 @@ int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bo
     EG_TRY(eg_dalloc_t(&st->dpart, ((uint64_t)np + kDeltaBlocks) * 64, "ppr delta partials"));
     st->dpart_cap = (uint64_t)np + kDeltaBlocks;
   }
+  return EG_OK;
+}
+
+int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bool want_delta) {
+  const PullPlan &pl = *bv.pl;
+  hipStream_t s = st->stream;
+  EG_TRY(ppr_buffers(pl, st, want_delta));
+  const uint32_t nfin = pl.nlong ? blocks_for((uint64_t)pl.nlong * 64, kBlock) : 0;
+  const uint32_t np = pl.nchunks + nfin;
   const float oma = (float)(1.0 - alpha);
   const uint32_t width = ppr_width(st->active);
   const uint32_t nitems = pl.nseg + pl.nchunks;
   const uint32_t grid = st->max_grid && st->max_grid < nitems ? st->max_grid : nitems;
   if (grid) {
-    EgProf prof(g, "ppr_spmm", s);
+    EgProf prof(st->capturing ? nullptr : g, "ppr_spmm", s);
     const bool d16 = g->tune.spmm_depth == 16;
 #define EG_SPMM_ARGS                                                                       \
   dim3(grid), dim3(kBlock), 0, s, bv.idx, pl.chunk_info, pl.seg_info, pl.nseg, nitems,     \
@@ -1169,6 +1178,34 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
   return EG_OK;
 }
 
+// Runs 2 * pairs SpMM iterations by replaying a hipGraph of two iterations
+// (one full X/Y, C/Cn swap cycle, so every replay sees the same buffers).
+// The captured steps hold only kernel launches.
+static int ppr_graph_steps(eg_graph *g, const BatchView &bv, PprState *st, double alpha,
+                           int pairs) {
+  EG_TRY(ppr_buffers(*bv.pl, st, false));
+  hipStream_t s = st->stream;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  EG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  st->capturing = true;
+  int rc = eg_ppr_step(g, bv, st, alpha);
+  if (rc == EG_OK) rc = eg_ppr_step(g, bv, st, alpha);
+  st->capturing = false;
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (rc == EG_OK && ec != hipSuccess)
+    rc = eg_fail(EG_EDEVICE, "stream capture failed: %s", hipGetErrorString(ec));
+  if (rc == EG_OK && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = eg_fail(EG_EDEVICE, "graph instantiation failed");
+  for (int p = 0; p < pairs && rc == EG_OK; p++)
+    if (hipGraphLaunch(exec, s) != hipSuccess) rc = eg_fail(EG_EDEVICE, "graph launch failed");
+  if (rc == EG_OK && hipStreamSynchronize(s) != hipSuccess)
+    rc = eg_fail(EG_EDEVICE, "graph replay failed");
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  return rc;
+}
+
 extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
                             const uint32_t *seeds, double alpha, int32_t iterations, float *out,
                             double *device_ms) {
@@ -1205,7 +1242,12 @@ extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
     for (uint32_t q = 0; q < kk && rc == EG_OK; q++)
       rc = eg_ppr_col_init(g, bv, &b->ppr, q, seeds + seed_ptr[q0 + q],
                            seed_ptr[q0 + q + 1] - seed_ptr[q0 + q], alpha);
-    for (int it = 0; it < iterations && rc == EG_OK; it++) rc = eg_ppr_step(g, bv, &b->ppr, alpha);
+    int it = 0;
+    if (rc == EG_OK && g->tune.graphs && iterations >= 4) {
+      rc = ppr_graph_steps(g, bv, &b->ppr, alpha, iterations / 2);
+      it = 2 * (iterations / 2);
+    }
+    for (; it < iterations && rc == EG_OK; it++) rc = eg_ppr_step(g, bv, &b->ppr, alpha);
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;

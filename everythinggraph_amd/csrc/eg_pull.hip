@@ -136,6 +136,7 @@ struct PrArgs {
   float damping;
   double damping_over_n;   // d / n, for the uniform dangling term
   int uniform;
+  uint32_t hot;            // gathers of x[u] with u >= hot are non-temporal loads
 };
 
 // Values of entries k, k + kBlock, ..., k + (B-1)*kBlock (0 past kend).
@@ -146,7 +147,7 @@ template <bool WEIGHTED, int B>
 __device__ __forceinline__ void gather_batch(const uint32_t *__restrict__ idx,
                                              const float *__restrict__ w,
                                              const float *__restrict__ x, uint32_t k,
-                                             uint32_t kend, float (&v)[B]) {
+                                             uint32_t kend, uint32_t hot, float (&v)[B]) {
   uint32_t u[B];
   float wv[B];
 #pragma unroll
@@ -157,7 +158,9 @@ __device__ __forceinline__ void gather_batch(const uint32_t *__restrict__ idx,
   }
 #pragma unroll
   for (int q = 0; q < B; q++) {
-    const float xv = x[u[q]];
+    // In the degree-ordered copy the sources below `hot` take most gathers;
+    // colder ones are loaded non-temporally so that they do not evict them.
+    const float xv = u[q] < hot ? x[u[q]] : __builtin_nontemporal_load(x + u[q]);
     v[q] = (k + q * kBlock < kend) ? (WEIGHTED ? xv * wv[q] : xv) : 0.f;
   }
 }
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
     EgKahan acc;
     for (uint32_t base = si.x; base < si.y; base += kBlock * B) {
       float v[B];
-      gather_batch<WEIGHTED, B>(idx, w, x, base + tid, si.y, v);
+      gather_batch<WEIGHTED, B>(idx, w, x, base + tid, si.y, a.hot, v);
 #pragma unroll
       for (int q = 0; q < B; q++) acc.add(v[q]);
     }
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
   }
   for (uint32_t base = k0; base < k1; base += kBlock * B) {
     float v[B];
-    gather_batch<WEIGHTED, B>(idx, w, x, base + tid, k1, v);
+    gather_batch<WEIGHTED, B>(idx, w, x, base + tid, k1, a.hot, v);
 #pragma unroll
     for (int q = 0; q < B; q++) {
       const uint32_t k = base + tid + q * kBlock;
@@ -494,8 +497,11 @@ static void launch_k_pull(hipStream_t s, bool pr, uint32_t grid, const uint32_t 
 
 static int launch_pull(eg_graph *g, bool pr, const uint32_t *ptr, const uint32_t *idx,
                        const float *w, const float *x, float *y, const PullPlan &pl,
-                       const PrArgs &a) {
+                       const PrArgs &a_in) {
   const PlanDev d = plan_dev(pl);
+  PrArgs a = a_in;
+  const bool ordered = g->ord && &pl == &g->ord->plan;
+  a.hot = ordered && g->tune.pull_hot > 0 ? (uint32_t)g->tune.pull_hot : 0xFFFFFFFFu;
   const uint32_t grid = pl.nseg + pl.nchunks;
   if (grid) {
     EgProf prof(g, pr ? "pr_pull" : "spmv_pull");

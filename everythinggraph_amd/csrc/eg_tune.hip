@@ -6,7 +6,11 @@
 //  * pull_batch  -- k_pull (PageRank / SpMV / single-vector PPR): entries per
 //                   lane per gather pass, 8 or 16;
 //  * spmm_depth  -- k_ppr_spmm: row gathers in flight per wave, 8 or 16;
-//  * ms_batch    -- k_ms_pull: entries per lane per index pass, 8 or 16.
+//  * ms_batch    -- k_ms_pull: entries per lane per index pass, 8 or 16;
+//  * pull_hot    -- k_pull on the degree-ordered copy: gathers of sources with
+//                   new id >= pull_hot bypass normal caching (non-temporal
+//                   loads) so that the hottest sources stay in L2; 0 = off;
+//  * graphs      -- 1: batched PPR iterations are replayed from a hipGraph.
 // eg_autotune times every variant on the graph itself through the public
 // entry points (device time from HIP events, best of a few repetitions) and
 // keeps the fastest.
@@ -22,6 +26,8 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "pull_batch")) return &g->tune.pull_batch;
   if (!strcmp(name, "spmm_depth")) return &g->tune.spmm_depth;
   if (!strcmp(name, "ms_batch")) return &g->tune.ms_batch;
+  if (!strcmp(name, "pull_hot")) return &g->tune.pull_hot;
+  if (!strcmp(name, "graphs")) return &g->tune.graphs;
   return nullptr;
 }
 
@@ -59,7 +65,13 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
   if (!g || !param) return eg_fail(EG_EINVAL, "null argument");
   int *slot = param_slot(g, param);
   if (!slot) return eg_fail(EG_EINVAL, "unknown tuning parameter '%s'", param);
-  if (value != 8 && value != 16) return eg_fail(EG_EINVAL, "%s takes 8 or 16", param);
+  if (!strcmp(param, "pull_hot")) {
+    if (value < 0) return eg_fail(EG_EINVAL, "pull_hot takes a vertex count >= 0");
+  } else if (!strcmp(param, "graphs")) {
+    if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "graphs takes 0 or 1");
+  } else if (value != 8 && value != 16) {
+    return eg_fail(EG_EINVAL, "%s takes 8 or 16", param);
+  }
   *slot = value;
   return EG_OK;
 }
@@ -88,6 +100,20 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
       EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // warm (plan, order, state)
       EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); }, &ms));
       res.push_back({"pull_batch", v, ms / o.iterations});
+    }
+    // keep the faster depth, then try cache limits for the ordered copy
+    g->tune.pull_batch = res[0].ms <= res[1].ms ? res[0].value : res[1].value;
+    if (eg_reorder_enabled()) {
+      const int hots[4] = {0, (int)(g->n / 64), (int)(g->n / 16), (int)(g->n / 4)};
+      for (int i = 0; i < 4; i++) {
+        const int h = hots[i];
+        if (i > 0 && h == hots[i - 1]) continue;
+        g->tune.pull_hot = h;
+        double ms;
+        EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); },
+                         &ms));
+        res.push_back({"pull_hot", h, ms / o.iterations});
+      }
     }
   }
   std::vector<uint32_t> roots;

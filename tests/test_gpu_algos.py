@@ -370,6 +370,7 @@ def test_tuning_variants_bit_identical():
             for p in ("pull_batch", "spmm_depth", "ms_batch"):
                 g.tune_set(p, v)
                 assert g.tune_get(p) == v
+            g.tune_set("pull_hot", 0 if v == 8 else n // 16)
             outs[v] = (g.pagerank(0.85, 20)[0], g.ppr_batch(seeds, 0.15, 20)[0],
                        g.bfs_batch(roots)[0], g.ppr([5, 6], 0.15, tol=0.0, max_iter=25)[0])
         with pytest.raises(eg.EgError) as e:
@@ -391,12 +392,41 @@ def test_autotune_reports_and_keeps_fastest():
     with eg.Graph(n, 2, undirected=True) as g:
         g.load_edges(s0, d0)
         rep = g.autotune()
-        assert len(rep) == 6
-        for p in ("pull_batch", "spmm_depth", "ms_batch"):
+        params = {r[0] for r in rep}
+        assert params == {"pull_batch", "pull_hot", "spmm_depth", "ms_batch"}
+        hot = sorted(r[1] for r in rep if r[0] == "pull_hot")
+        assert hot == [0, n // 64, n // 16, n // 4]
+        for p in ("pull_batch", "spmm_depth", "ms_batch", "pull_hot"):
             rows = [r for r in rep if r[0] == p]
-            assert sorted(r[1] for r in rows) == [8, 16] and all(r[2] > 0 for r in rows)
+            assert all(r[2] > 0 for r in rows)
+            if p != "pull_hot":
+                assert sorted(r[1] for r in rows) == [8, 16]
             best = min(rows, key=lambda r: r[2])
             chosen = [r for r in rows if r[3]]
             assert len(chosen) == 1 and chosen[0][1] == g.tune_get(p) == best[1]
         r, _, _ = g.pagerank(0.85, 20)  # still runs after tuning
         assert np.isfinite(r).all()
+
+
+def test_ppr_graph_replay_matches_eager():
+    """Batched PPR iterations replayed from a hipGraph equal eager launches,
+    on every replay, with other device work between the calls."""
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=53)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    deg = np.diff(ptr).astype(np.uint32)
+    seeds = [[1, 2], [7], [100, 3, 3, 9], [4000]]
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        g.tune_set("graphs", 0)
+        eager = {it: g.ppr_batch(seeds, 0.15, it)[0] for it in (4, 7, 30)}
+        g.tune_set("graphs", 1)
+        for rep in range(3):
+            for it in (4, 7, 30):
+                out = g.ppr_batch(seeds, 0.15, it)[0]
+                assert np.array_equal(out, eager[it]), (rep, it)
+            g.pagerank(0.85, 3)          # unrelated eager work between replays
+            g.bfs(1)
+    for q, sd in enumerate(seeds):
+        ref = O.ppr(n, cp, row, deg, sd, 0.15, 30)
+        assert np.abs(eager[30][q] - ref).sum() <= PR_L1_TOL
