@@ -82,9 +82,11 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
-    ap.add_argument("--serve-qps", type=float, default=100.0,
-                    help="offered Poisson rate of the serving replay")
+    ap.add_argument("--serve-qps", type=float, action="append", default=None,
+                    help="offered Poisson rate(s) of the serving replay (default 50 and 100)")
     ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-ppr-grid", type=int, default=0,
+                    help="SpMM workgroups while serving (0: library default)")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-configs", action="store_true",
                     help="skip the grid (ER 2^24 / 2^28, 16x16) and RMAT-22 BFS/WCC figures")
@@ -209,26 +211,34 @@ def main():
         # serving: the same Poisson stream (80% BFS, 20% PPR with 1-8 seeds,
         # tolerance 1e-6 L1, seed 4) through continuous batching and through
         # the plain one-query-at-a-time path
-        srv = {}
-        for mode in ("batched", "plain"):
-            w0 = time.perf_counter()
-            r = g.serve_replay(args.serve_qps, args.serve_requests, bfs_fraction=0.8,
-                               ppr_max_seeds=8, seed=4, plain=(mode == "plain"))
-            srv[mode] = r
-            log(f"serve {mode}: offered {args.serve_qps:.0f} q/s, {r['completed']} done "
-                f"({r['bfs']} BFS / {r['ppr']} PPR) at {r['qps']:.1f} q/s; p50 "
-                f"{r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms (BFS p50 {r['bfs_p50_ms']:.1f} "
-                f"p99 {r['bfs_p99_ms']:.1f}; PPR p50 {r['ppr_p50_ms']:.1f} p99 "
-                f"{r['ppr_p99_ms']:.1f}); levels {r['stats']['bfs_levels']} spmm steps "
-                f"{r['stats']['ppr_steps']}; wall {time.perf_counter() - w0:.1f} s")
-        extra["serving"] = {
-            "offered_qps": args.serve_qps, "requests": args.serve_requests,
-            "mix": "80% BFS / 20% PPR (1-8 seeds, tol 1e-6 L1), Poisson, seed 4",
-            **{f"{k}_{f}": round(v[f], 3) for k, v in srv.items()
-               for f in ("qps", "p50_ms", "p99_ms", "bfs_p50_ms", "bfs_p99_ms", "ppr_p50_ms",
-                         "ppr_p99_ms")},
-            "batched_qps_vs_plain": round(srv["batched"]["qps"] / max(srv["plain"]["qps"], 1e-9), 2),
-        }
+        rates = args.serve_qps or [50.0, 100.0]
+        serving = []
+        for qps in rates:
+            srv = {}
+            for mode in ("batched", "plain"):
+                w0 = time.perf_counter()
+                r = g.serve_replay(qps, args.serve_requests, bfs_fraction=0.8,
+                                   ppr_max_seeds=8, seed=4, plain=(mode == "plain"),
+                                   ppr_grid=args.serve_ppr_grid)
+                srv[mode] = r
+                log(f"serve {mode}: offered {qps:.0f} q/s, {r['completed']} done "
+                    f"({r['bfs']} BFS / {r['ppr']} PPR) at {r['qps']:.1f} q/s; p50 "
+                    f"{r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms (BFS p50 "
+                    f"{r['bfs_p50_ms']:.1f} p99 {r['bfs_p99_ms']:.1f}; PPR p50 "
+                    f"{r['ppr_p50_ms']:.1f} p99 {r['ppr_p99_ms']:.1f}); levels "
+                    f"{r['stats']['bfs_levels']} spmm steps {r['stats']['ppr_steps']}; "
+                    f"wall {time.perf_counter() - w0:.1f} s")
+            b, p = srv["batched"], srv["plain"]
+            serving.append({
+                "offered_qps": qps, "requests": args.serve_requests,
+                "mix": "80% BFS / 20% PPR (1-8 seeds, tol 1e-6 L1), Poisson, seed 4",
+                **{f"{k}_{f}": round(v[f], 3) for k, v in srv.items()
+                   for f in ("qps", "p50_ms", "p99_ms", "bfs_p50_ms", "bfs_p99_ms",
+                             "ppr_p50_ms", "ppr_p99_ms")},
+                "batched_qps_vs_plain": round(b["qps"] / max(p["qps"], 1e-9), 2),
+                "plain_p99_over_batched_p99": round(p["p99_ms"] / max(b["p99_ms"], 1e-9), 2),
+            })
+        extra["serving"] = serving
 
     cpu = None
     if not args.skip_cpu:

@@ -168,7 +168,7 @@ def serve_opts(bfs_lanes=0, ppr_columns=0, hbm_reserve=0, ppr_alpha=0.0, ppr_tol
                ppr_max_iter=0, plain=False, max_pending=0, ppr_grid=0) -> ServeOpts:
     """Serving options; zeros take the library defaults (64 lanes, 64 columns,
     1 GiB reserve, alpha 0.15, tolerance 1e-6, 200 iterations, an SpMM grid of
-    2 workgroups per compute unit)."""
+    8 workgroups per compute unit)."""
     return ServeOpts(bfs_lanes, ppr_columns, hbm_reserve, ppr_alpha, ppr_tol, ppr_max_iter,
                      int(plain), max_pending, ppr_grid)
 

@@ -577,7 +577,7 @@ extern "C" int eg_serve_start(eg_graph *g, const eg_serve_opts *opts, eg_server 
             hipSuccess ||
         cus <= 0)
       cus = 256;
-    o.ppr_grid = 2u * (uint32_t)cus;
+    o.ppr_grid = 8u * (uint32_t)cus;
   }
   if (o.ppr_columns > 64 || !(o.ppr_alpha > 0.0 && o.ppr_alpha <= 1.0) || !(o.ppr_tol > 0.0) ||
       o.ppr_max_iter < 0 || o.bfs_lanes > 64 * 64)

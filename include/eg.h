@@ -243,7 +243,7 @@ typedef struct {
   int32_t plain;        /* 1: one query at a time, BFS through eg_bfs */
   uint32_t max_pending; /* queued queries before submit says EG_EAGAIN; 0 = 1<<20 */
   uint32_t ppr_grid;    /* workgroups of an SpMM iteration, leaving the rest of the
-                           device to BFS levels; 0 = 2 per compute unit */
+                           device to BFS levels; 0 = 8 per compute unit */
 } eg_serve_opts;
 
 typedef struct {
