@@ -308,6 +308,10 @@ def other_configs(eg) -> dict:
     build_ms = g.info.build_ms
     b_pr = 4 * m + 4 * (n + 1) + 16 * n
     b_spmv = 8 * m + 4 * (n + 1) + 8 * n
+    tune = g.autotune(eg.TUNE_GRID)
+    band = g.tune_get("grid_band")
+    log("grid autotune: " + ", ".join(f"grid_band={v} {ms:.3f} ms{'*' if c else ''}"
+                                      for _, v, ms, c in tune))
     g.pagerank(0.85, 2, layout="grid", download=False)
     g.prof_reset()
     g.prof_enable(True)
@@ -321,7 +325,7 @@ def other_configs(eg) -> dict:
     sl, stt = prof.get("spmv_grid", (0, 0.0))
     pr_ms, sp_ms = pt / max(pl, 1), stt / max(sl, 1)
     res["grid_er24"] = {
-        "n": n, "m": m, "P": 16, "build_ms": round(build_ms, 2),
+        "n": n, "m": m, "P": 16, "grid_band": band, "build_ms": round(build_ms, 2),
         "pr_iter_ms": round(pr_ms, 4), "pr_gbs": round(b_pr / (pr_ms * 1e-3) / 1e9, 1),
         "spmv_ms": round(sp_ms, 4), "spmv_gbs": round(b_spmv / (sp_ms * 1e-3) / 1e9, 1),
         "pr_final_l1_delta": float(lh[-1]),

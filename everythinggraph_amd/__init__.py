@@ -110,7 +110,8 @@ class ServeOpts(ctypes.Structure):
                 ("hbm_reserve", ctypes.c_uint64), ("ppr_alpha", ctypes.c_double),
                 ("ppr_tol", ctypes.c_double), ("ppr_max_iter", ctypes.c_int32),
                 ("plain", ctypes.c_int32), ("max_pending", ctypes.c_uint32),
-                ("ppr_grid", ctypes.c_uint32)]
+                ("ppr_grid", ctypes.c_uint32), ("idle_plain", ctypes.c_int32),
+                ("pad", ctypes.c_uint32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -126,7 +127,8 @@ class ServeStats(ctypes.Structure):
                 ("bfs_inflight", ctypes.c_uint32), ("ppr_inflight", ctypes.c_uint32),
                 ("bfs_lanes", ctypes.c_uint32), ("ppr_columns", ctypes.c_uint32),
                 ("pad", ctypes.c_uint32), ("bfs_levels", ctypes.c_uint64),
-                ("ppr_steps", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64)]
+                ("ppr_steps", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("served_plain", ctypes.c_uint64)]
 
 
 class TuneEntry(ctypes.Structure):
@@ -134,7 +136,7 @@ class TuneEntry(ctypes.Structure):
                 ("chosen", ctypes.c_int32), ("ms", ctypes.c_double)]
 
 
-TUNE_PULL, TUNE_SPMM, TUNE_MSBFS = 1, 2, 4
+TUNE_PULL, TUNE_SPMM, TUNE_MSBFS, TUNE_GRID = 1, 2, 4, 8
 
 
 class ReplayParams(ctypes.Structure):
@@ -165,12 +167,14 @@ def _struct_dict(st) -> dict:
 
 
 def serve_opts(bfs_lanes=0, ppr_columns=0, hbm_reserve=0, ppr_alpha=0.0, ppr_tol=0.0,
-               ppr_max_iter=0, plain=False, max_pending=0, ppr_grid=0) -> ServeOpts:
+               ppr_max_iter=0, plain=False, max_pending=0, ppr_grid=0,
+               idle_plain=0) -> ServeOpts:
     """Serving options; zeros take the library defaults (64 lanes, 64 columns,
     1 GiB reserve, alpha 0.15, tolerance 1e-6, 200 iterations, an SpMM grid of
-    8 workgroups per compute unit)."""
+    8 workgroups per compute unit, idle engine serves up to 2 waiting queries
+    through the plain path; idle_plain=-1 always batches)."""
     return ServeOpts(bfs_lanes, ppr_columns, hbm_reserve, ppr_alpha, ppr_tol, ppr_max_iter,
-                     int(plain), max_pending, ppr_grid)
+                     int(plain), max_pending, ppr_grid, idle_plain, 0)
 
 
 _lib: Optional[ctypes.CDLL] = None

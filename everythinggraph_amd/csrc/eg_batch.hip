@@ -525,22 +525,31 @@ __global__ void k_ppr_cols_clear(uint32_t n, uint64_t cols, uint64_t *__restrict
   }
 }
 
-// Adds the contribution rows of entries [k0, k1) to this lane's column, D
-// row gathers in flight per wave; entries are added in ascending order for
-// every D.  Lanes at or above the active width (on = false) issue no gathers.
-template <int D>
-__device__ __forceinline__ void spmm_range(const uint32_t *__restrict__ idx,
-                                           const float *__restrict__ C, uint32_t k0, uint32_t k1,
-                                           int lane, bool on, EgKahan &acc) {
-  for (uint32_t base = k0; base < k1; base += 64) {
-    const uint32_t cnt = k1 - base < 64 ? k1 - base : 64;
-    const uint32_t my = lane < (int)cnt ? __builtin_nontemporal_load(idx + base + lane) : 0u;
-    for (uint32_t j = 0; j < cnt; j += D) {
+// Adds the contribution rows of entries [k0, k1) to column q, for the W-lane
+// slot `slot` of the wave (each slot walks its own row).  D row gathers are in
+// flight per slot; entries are added in ascending order whatever W and D, so
+// a column's sum never depends on how many columns are active.  Lanes with
+// on = false issue no gathers.
+template <int D, int W>
+__device__ __forceinline__ void spmm_slot(const uint32_t *__restrict__ idx,
+                                          const float *__restrict__ C, uint32_t k0, uint32_t k1,
+                                          int slot, int q, bool on, EgKahan &acc) {
+  const uint32_t len = k1 - k0;
+  uint32_t lmax = len;
+#pragma unroll
+  for (int o = W; o < 64; o <<= 1) lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, o));
+  for (uint32_t blk = 0; blk < lmax; blk += W) {
+    const uint32_t cnt = len > blk ? (len - blk < (uint32_t)W ? len - blk : (uint32_t)W) : 0u;
+    const uint32_t my = (uint32_t)q < cnt ? __builtin_nontemporal_load(idx + k0 + blk + q) : 0u;
+    uint32_t cmax = cnt;
+#pragma unroll
+    for (int o = W; o < 64; o <<= 1) cmax = max(cmax, (uint32_t)__shfl_xor((int)cmax, o));
+    for (uint32_t j = 0; j < cmax; j += D) {
       float v[D];
 #pragma unroll
       for (int t = 0; t < D; t++) {
-        const uint32_t u = __shfl(my, (int)(j + t) & 63);
-        v[t] = on ? C[(uint64_t)u * 64 + lane] : 0.f;
+        const uint32_t u = __shfl(my, slot * W + (int)((j + t) & (W - 1)));
+        v[t] = (on && j + t < cnt) ? C[(uint64_t)u * 64 + q] : 0.f;
       }
 #pragma unroll
       for (int t = 0; t < D; t++)
@@ -582,13 +591,20 @@ __device__ __forceinline__ void store_delta(double d, double *__restrict__ dpart
   }
 }
 
-// Work items of the pull plan: segments of long rows (the four waves split
-// the segment, partials merged in wave order) and chunks of short rows (a
-// wave per row).  Lane q owns query column q; lanes >= width are idle.  A
-// workgroup takes items blockIdx.x, blockIdx.x + gridDim.x, ...: a grid
-// smaller than the item count leaves wave slots to concurrent kernels, and
-// every item is computed the same way whichever workgroup takes it.
-template <bool DELTA, int D>
+// Work items of the pull plan, for W = 16, 32 or 64 gathered columns (a wave
+// works on 64 / W rows at once, one W-lane slot per row; lane slot*W + q owns
+// column q).  Everything that is summed uses an order independent of W:
+//  * a chunk row is summed by one slot over all its entries;
+//  * a long-row segment is cut into kSegParts parts, each summed by one slot,
+//    and the part sums are merged in part order;
+//  * a chunk's per-column L1 change is binned by row index mod kDeltaBins and
+//    the bins are merged in bin order.
+// A workgroup takes items blockIdx.x, blockIdx.x + gridDim.x, ...: a grid
+// smaller than the item count leaves wave slots to concurrent kernels.
+constexpr int kSegParts = 16;
+constexpr int kDeltaBins = 16;
+
+template <bool DELTA, int D, int W>
 __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
     const uint32_t *__restrict__ idx, const uint4 *__restrict__ chunk_info,
     const uint4 *__restrict__ seg_info, uint32_t nseg, uint32_t nitems,
@@ -597,39 +613,70 @@ __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
     const float *__restrict__ alpha_inv_seeds, const uint32_t *__restrict__ deg,
     float one_minus_a, uint32_t width, const float *__restrict__ X, float *__restrict__ Y,
     float *__restrict__ Cn, double *__restrict__ dpart) {
-  __shared__ float2 s_p[kWaves][64];
+  constexpr int R = 64 / W;                       // rows per wave
+  constexpr int kUnits = kWaves * R;              // row slots per workgroup
+  constexpr int kBinsPerUnit = kDeltaBins / kUnits;
+  static_assert(kSegParts % kUnits == 0 && kDeltaBins % kUnits == 0, "slot layout");
+  __shared__ float2 s_p[kSegParts][64];
+  __shared__ double s_d[kDeltaBins][64];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
-  const bool on = (uint32_t)lane < width;
+  const int slot = lane / W;
+  const int q = lane % W;
+  const int unit = wave * R + slot;
+  const bool on = (uint32_t)q < width;
   for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
     if (item < nseg) {
       const uint4 si = seg_info[item];
-      const uint32_t len = si.y - si.x;
-      const uint32_t part = (len + kWaves * 64 - 1) / (kWaves * 64) * 64;  // per wave, x64
-      const uint32_t a = si.x + wave * part < si.y ? si.x + wave * part : si.y;
-      const uint32_t b = a + part < si.y ? a + part : si.y;
-      EgKahan acc;
-      spmm_range<D>(idx, C, a, b, lane, on, acc);
-      s_p[wave][lane] = make_float2(acc.s, acc.c);
+      const uint32_t part = (si.y - si.x + kSegParts - 1) / kSegParts;
+      for (int p = unit; p < kSegParts; p += kUnits) {
+        const uint32_t a = si.x + p * part < si.y ? si.x + p * part : si.y;
+        const uint32_t b = a + part < si.y ? a + part : si.y;
+        EgKahan acc;
+        spmm_slot<D, W>(idx, C, a, b, slot, q, on, acc);
+        s_p[p][q] = make_float2(acc.s, acc.c);
+      }
       __syncthreads();
       if (wave == 0) {
-        EgKahan t;
-        t.s = s_p[0][lane].x;
-        t.c = s_p[0][lane].y;
-        for (int q = 1; q < kWaves; q++) t.merge(s_p[q][lane].x, s_p[q][lane].y);
-        seg_part[(uint64_t)item * 64 + lane] = make_float2(t.s, t.c);
+        float2 out = make_float2(0.f, 0.f);
+        if (lane < W) {
+          EgKahan t;
+          t.s = s_p[0][lane].x;
+          t.c = s_p[0][lane].y;
+          for (int p = 1; p < kSegParts; p++) t.merge(s_p[p][lane].x, s_p[p][lane].y);
+          out = make_float2(t.s, t.c);
+        }
+        seg_part[(uint64_t)item * 64 + lane] = out;
       }
     } else {
       const uint4 ci = chunk_info[item - nseg];
-      const float seed_term = alpha_inv_seeds[lane];
-      double d = 0.0;
-      for (uint32_t v = ci.x + wave; v < ci.y; v += kWaves) {
+      const float seed_term = alpha_inv_seeds[q];
+      double dbin[kBinsPerUnit];
+#pragma unroll
+      for (int b = 0; b < kBinsPerUnit; b++) dbin[b] = 0.0;
+      uint32_t j = 0;
+      for (uint32_t v0 = ci.x + wave * R; v0 < ci.y; v0 += kUnits, j++) {
+        const uint32_t v = v0 + slot;
+        const bool row = v < ci.y;
         EgKahan acc;
-        spmm_range<D>(idx, C, ptr[v], ptr[v + 1], lane, on, acc);
-        d += spmm_finish<DELTA>(v, acc.value(), lane, on, mask, seed_term, deg, one_minus_a, X,
-                                Y, Cn);
+        spmm_slot<D, W>(idx, C, row ? ptr[v] : 0u, row ? ptr[v + 1] : 0u, slot, q, on, acc);
+        if (row) {
+          const double d = spmm_finish<DELTA>(v, acc.value(), q, on, mask, seed_term, deg,
+                                              one_minus_a, X, Y, Cn);
+          if (DELTA) dbin[j % kBinsPerUnit] += d;
+        }
       }
-      store_delta<DELTA>(d, dpart, item - nseg);
+      if (DELTA) {
+#pragma unroll
+        for (int b = 0; b < kBinsPerUnit; b++) s_d[unit + b * kUnits][q] = dbin[b];
+        __syncthreads();
+        if (wave == 0) {
+          double t = 0.0;
+          if (lane < W)
+            for (int b = 0; b < kDeltaBins; b++) t += s_d[b][lane];
+          dpart[(uint64_t)(item - nseg) * 64 + lane] = t;
+        }
+      }
     }
     __syncthreads();  // LDS reuse by the next item
   }
@@ -984,6 +1031,45 @@ int eg_ppr_col_init(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q,
   return EG_OK;
 }
 
+// Launches k_ppr_spmm for the narrowest slot width that holds `width`
+// columns, with the tuned gather depth.
+template <bool DELTA, int D, int W>
+static void spmm_go(hipStream_t s, const BatchView &bv, const PprState *st, uint32_t grid,
+                    uint32_t nitems, float oma, uint32_t width) {
+  const PullPlan &pl = *bv.pl;
+  hipLaunchKernelGGL((k_ppr_spmm<DELTA, D, W>), dim3(grid), dim3(kBlock), 0, s, bv.idx,
+                     pl.chunk_info, pl.seg_info, pl.nseg, nitems, st->seg_part, bv.ptr, st->C,
+                     st->mask, st->w + 64, bv.deg, oma, width, st->X, st->Y, st->Cn,
+                     DELTA ? st->dpart : nullptr);
+}
+
+template <bool DELTA, int D>
+static void spmm_launch_w(hipStream_t s, const BatchView &bv, const PprState *st, uint32_t grid,
+                          uint32_t nitems, float oma, uint32_t width) {
+  if (width <= 16)
+    spmm_go<DELTA, D, 16>(s, bv, st, grid, nitems, oma, width);
+  else if (width <= 32)
+    spmm_go<DELTA, D, 32>(s, bv, st, grid, nitems, oma, width);
+  else
+    spmm_go<DELTA, D, 64>(s, bv, st, grid, nitems, oma, width);
+}
+
+static int launch_spmm(eg_graph *g, const BatchView &bv, const PprState *st, bool delta,
+                       uint32_t grid, uint32_t nitems, float oma, uint32_t width) {
+  const bool d16 = g->tune.spmm_depth == 16;
+  hipStream_t s = st->stream;
+  if (delta && d16)
+    spmm_launch_w<true, 16>(s, bv, st, grid, nitems, oma, width);
+  else if (delta)
+    spmm_launch_w<true, 8>(s, bv, st, grid, nitems, oma, width);
+  else if (d16)
+    spmm_launch_w<false, 16>(s, bv, st, grid, nitems, oma, width);
+  else
+    spmm_launch_w<false, 8>(s, bv, st, grid, nitems, oma, width);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
 // Gathered lanes: every active column, rounded up to a multiple of 16.
 static uint32_t ppr_width(uint64_t active) {
   if (!active) return 16;
@@ -1023,21 +1109,7 @@ int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bo
   const uint32_t grid = st->max_grid && st->max_grid < nitems ? st->max_grid : nitems;
   if (grid) {
     EgProf prof(st->capturing ? nullptr : g, "ppr_spmm", s);
-    const bool d16 = g->tune.spmm_depth == 16;
-#define EG_SPMM_ARGS                                                                       \
-  dim3(grid), dim3(kBlock), 0, s, bv.idx, pl.chunk_info, pl.seg_info, pl.nseg, nitems,     \
-      st->seg_part, bv.ptr, st->C, st->mask, st->w + 64, bv.deg, oma, width, st->X, st->Y, \
-      st->Cn
-    if (want_delta && d16)
-      hipLaunchKernelGGL((k_ppr_spmm<true, 16>), EG_SPMM_ARGS, st->dpart);
-    else if (want_delta)
-      hipLaunchKernelGGL((k_ppr_spmm<true, 8>), EG_SPMM_ARGS, st->dpart);
-    else if (d16)
-      hipLaunchKernelGGL((k_ppr_spmm<false, 16>), EG_SPMM_ARGS, nullptr);
-    else
-      hipLaunchKernelGGL((k_ppr_spmm<false, 8>), EG_SPMM_ARGS, nullptr);
-#undef EG_SPMM_ARGS
-    EG_LAUNCHED();
+    EG_TRY(launch_spmm(g, bv, st, want_delta, grid, nitems, oma, width));
   }
   if (pl.nlong) {
     if (want_delta)

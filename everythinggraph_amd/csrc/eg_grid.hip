@@ -13,7 +13,11 @@
 // (cell, dst, src) and indexed by dst tiles of kTile vertices, so one
 // workgroup owns a tile of a grid column and walks the cells of that column
 // row by row, exactly the reference's row-sequential / column-parallel
-// schedule with the column cut into tiles.  Each workgroup stages one
+// schedule with the column cut into tiles.  The grid rows (source
+// partitions) can be split into bands launched one after another: every
+// workgroup of a launch then gathers from the same band of x, which stays in
+// L2, and the compensated accumulators (s, c) of each dst vertex are carried
+// between launches exactly, so the sums are the same for every band width.  Each workgroup stages one
 // cell-tile (its entries' dst offsets and values) in LDS per step and every
 // lane sums the entries of the vertices it owns with a compensated FP32
 // accumulator, in src order (deterministic, and the same order as the CSC
@@ -92,9 +96,13 @@ struct GridArgs {
   int uniform;
 };
 
+// Cells (i, j) for i in [i0, i1): carried accumulators come from / go to
+// carry (per dst vertex) unless the band is the first / last.
 template <bool PR, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void k_grid_pull(const float *__restrict__ x,
-                                                      float *__restrict__ y, GridArgs a) {
+                                                      float *__restrict__ y, GridArgs a,
+                                                      uint32_t i0, uint32_t i1,
+                                                      float2 *__restrict__ carry) {
   __shared__ uint32_t s_dst[kStage];
   __shared__ float s_val[kStage];
   __shared__ double s_d[kWaves], s_l[kWaves];
@@ -109,7 +117,18 @@ __global__ __launch_bounds__(kBlock) void k_grid_pull(const float *__restrict__ 
   if (vb < pe) {
     const uint32_t ve = vb + kTile < pe ? vb + kTile : pe;
     EgKahan acc[kVPT];
-    for (uint32_t i = 0; i < a.P; i++) {
+    if (i0 > 0) {
+#pragma unroll
+      for (uint32_t r = 0; r < kVPT; r++) {
+        const uint32_t v = vb + r * kBlock + tid;
+        if (v < ve) {
+          const float2 c = carry[v];
+          acc[r].s = c.x;
+          acc[r].c = c.y;
+        }
+      }
+    }
+    for (uint32_t i = i0; i < i1; i++) {
       const uint64_t c = (uint64_t)i * a.P + j;
       const uint32_t k0 = a.tile_off[c * (a.ntiles + 1) + t];
       const uint32_t k1 = a.tile_off[c * (a.ntiles + 1) + t + 1];
@@ -136,6 +155,14 @@ __global__ __launch_bounds__(kBlock) void k_grid_pull(const float *__restrict__ 
           for (uint32_t q = lo; q < cnt && s_dst[q] == lv; q++) acc[r].add(s_val[q]);
         }
       }
+    }
+    if (i1 < a.P) {
+#pragma unroll
+      for (uint32_t r = 0; r < kVPT; r++) {
+        const uint32_t v = vb + r * kBlock + tid;
+        if (v < ve) carry[v] = make_float2(acc[r].s, acc[r].c);
+      }
+      return;  // not the last band: no results yet
     }
     float basep = 0.f;
     if (PR) {
@@ -289,6 +316,7 @@ struct GridCompute {
   double2 *fin_part = nullptr;
   uint32_t *fin_counter = nullptr;
   float *acc = nullptr;  // edge-array accumulator
+  float2 *carry = nullptr;  // grid: accumulators carried between source bands
   double2 *apart = nullptr;
   uint32_t apart_n = 0;
 };
@@ -312,6 +340,7 @@ void eg_grid_free(eg_graph *g) {
   eg_dfree_t(&c->fin_part);
   eg_dfree_t(&c->fin_counter);
   eg_dfree_t(&c->acc);
+  eg_dfree_t(&c->carry);
   eg_dfree_t(&c->apart);
   g->gridc = nullptr;
   delete c;
@@ -375,10 +404,28 @@ static int grid_build_compute(eg_graph *g, GridCompute *c) {
                      g->cell_ptr, c->cdst, ncells, n, P, c->ntiles, c->tile_off);
   EG_LAUNCHED();
   EG_TRY(eg_dalloc_t(&c->part, (uint64_t)P * c->ntiles, "grid partials"));
+  EG_TRY(eg_dalloc_t(&c->carry, n, "grid carried sums"));
   EG_TRY(eg_dalloc_t(&c->fin_part, 256, "grid fin partials"));
   EG_TRY(eg_dalloc_t(&c->fin_counter, 1, "grid counter"));
   EG_HIP(hipMemsetAsync(c->fin_counter, 0, 4, s));
   EG_HIP(hipStreamSynchronize(s));
+  return EG_OK;
+}
+
+// One pass over the grid in bands of g->tune.grid_band source partitions
+// (0: all partitions in one launch).
+template <bool PR, bool WEIGHTED>
+static int grid_pass(eg_graph *g, GridCompute *c, const float *x, float *y, const GridArgs &a) {
+  const uint32_t P = g->P;
+  const uint32_t band = g->tune.grid_band > 0 && (uint32_t)g->tune.grid_band < P
+                            ? (uint32_t)g->tune.grid_band
+                            : P;
+  for (uint32_t i0 = 0; i0 < P; i0 += band) {
+    const uint32_t i1 = i0 + band < P ? i0 + band : P;
+    hipLaunchKernelGGL((k_grid_pull<PR, WEIGHTED>), dim3(c->ntiles, P), dim3(kBlock), 0,
+                       g->stream, x, y, a, i0, i1, c->carry);
+    EG_LAUNCHED();
+  }
   return EG_OK;
 }
 
@@ -444,9 +491,7 @@ static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, dou
       a.uniform = uniform;
       {
         EgProf prof(g, "pr_grid");
-        hipLaunchKernelGGL((k_grid_pull<true, false>), dim3(c->ntiles, g->P), dim3(kBlock), 0, s,
-                           contrib, rank_next, a);
-        EG_LAUNCHED();
+        EG_TRY((grid_pass<true, false>(g, c, contrib, rank_next, a)));
       }
       hipLaunchKernelGGL(k_stats_reduce, dim3(64), dim3(kBlock), 0, s, c->part,
                          g->P * c->ntiles, c->fin_part, c->fin_counter, stats_out);
@@ -492,14 +537,8 @@ int eg_spmv_grid_device(eg_graph *g, const float *x, float *y) {
   a.P = g->P;
   a.ntiles = c->ntiles;
   EgProf prof(g, "spmv_grid");
-  if (c->cw)
-    hipLaunchKernelGGL((k_grid_pull<false, true>), dim3(c->ntiles, g->P), dim3(kBlock), 0,
-                       g->stream, x, y, a);
-  else
-    hipLaunchKernelGGL((k_grid_pull<false, false>), dim3(c->ntiles, g->P), dim3(kBlock), 0,
-                       g->stream, x, y, a);
-  EG_LAUNCHED();
-  return EG_OK;
+  if (c->cw) return grid_pass<false, true>(g, c, x, y, a);
+  return grid_pass<false, false>(g, c, x, y, a);
 }
 
 static int spmv_host(eg_graph *g, const float *x, float *y, bool grid) {

@@ -101,6 +101,7 @@ struct EgTune {
   int spmm_depth = 8;    // k_ppr_spmm: row gathers in flight per wave (8 or 16)
   int ms_batch = 8;      // k_ms_pull: entries per lane per gather pass (8 or 16)
   int graphs = 1;        // batched PPR iterations replayed from a hipGraph (0: eager)
+  int grid_band = 1;     // grid pull: source partitions per launch (0: all)
   int pull_hot = 0;      // k_pull on the degree-ordered copy: sources below this id
                          // are cached normally, the rest loaded non-temporally (0: all normal)
 };

@@ -382,7 +382,19 @@ bool bfs_active(eg_server *S) {
   return false;
 }
 
+int plain_round(eg_server *S);
+
 int batched_round(eg_server *S) {
+  // Light load: with nothing in flight, a lone query is fastest on the plain
+  // single-query kernels, which have the whole device to themselves.
+  if (S->o.idle_plain > 0 && !S->ppr_busy && !S->cols && !bfs_active(S)) {
+    size_t waiting;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      waiting = S->pend_bfs.size() + S->pend_ppr.size();
+    }
+    if (waiting > 0 && waiting <= (size_t)S->o.idle_plain) return plain_round(S);
+  }
   if (S->ppr_busy) {
     const hipError_t e = hipEventQuery(S->ppr_ev);
     if (e == hipSuccess) {
@@ -431,6 +443,10 @@ int plain_round(eg_server *S) {
   }
   if (!qr) return EG_OK;
   started(S, qr);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stats.served_plain++;
+  }
   if (qr->kind == EG_REQ_BFS) {
     eg_bfs_stats st{};
     const int rc = eg_bfs_host(S->g, qr->root, 0, 1, static_cast<uint32_t *>(qr->out), nullptr, &st);
@@ -571,6 +587,7 @@ extern "C" int eg_serve_start(eg_graph *g, const eg_serve_opts *opts, eg_server 
   if (o.ppr_tol == 0.0) o.ppr_tol = 1e-6;
   if (o.ppr_max_iter == 0) o.ppr_max_iter = 200;
   if (o.max_pending == 0) o.max_pending = 1u << 20;
+  if (o.idle_plain == 0) o.idle_plain = 2;
   if (o.ppr_grid == 0) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) !=

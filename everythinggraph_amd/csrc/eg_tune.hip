@@ -10,7 +10,9 @@
 //  * pull_hot    -- k_pull on the degree-ordered copy: gathers of sources with
 //                   new id >= pull_hot bypass normal caching (non-temporal
 //                   loads) so that the hottest sources stay in L2; 0 = off;
-//  * graphs      -- 1: batched PPR iterations are replayed from a hipGraph.
+//  * graphs      -- 1: batched PPR iterations are replayed from a hipGraph;
+//  * grid_band   -- grid pull: source partitions per launch (0 = all), so that
+//                   one launch gathers from a band of x that fits in L2.
 // eg_autotune times every variant on the graph itself through the public
 // entry points (device time from HIP events, best of a few repetitions) and
 // keeps the fastest.
@@ -28,6 +30,7 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "ms_batch")) return &g->tune.ms_batch;
   if (!strcmp(name, "pull_hot")) return &g->tune.pull_hot;
   if (!strcmp(name, "graphs")) return &g->tune.graphs;
+  if (!strcmp(name, "grid_band")) return &g->tune.grid_band;
   return nullptr;
 }
 
@@ -67,6 +70,8 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
   if (!slot) return eg_fail(EG_EINVAL, "unknown tuning parameter '%s'", param);
   if (!strcmp(param, "pull_hot")) {
     if (value < 0) return eg_fail(EG_EINVAL, "pull_hot takes a vertex count >= 0");
+  } else if (!strcmp(param, "grid_band")) {
+    if (value < 0) return eg_fail(EG_EINVAL, "grid_band takes a partition count >= 0");
   } else if (!strcmp(param, "graphs")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "graphs takes 0 or 1");
   } else if (value != 8 && value != 16) {
@@ -90,6 +95,20 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
   if (cap < 0 || (cap > 0 && !out)) return eg_fail(EG_EINVAL, "bad output array");
   const bool adj = g->out_ptr != nullptr;
   std::vector<Result> res;
+  if (kernels & EG_TUNE_GRID) {
+    if (!g->cell_ptr) return eg_fail(EG_ENOTSUP, "grid tuning needs a grid layout");
+    eg_pr_opts o = {0.85, 3, EG_PR_DANGLING_DROP, 0.0, EG_LAYOUT_GRID};
+    EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // builds the compute copy
+    const int bands[4] = {1, 2, 4, 0};
+    for (int b : bands) {
+      if (b >= (int)g->P) continue;
+      g->tune.grid_band = b;
+      double ms;
+      EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); },
+                       &ms));
+      res.push_back({"grid_band", b, ms / o.iterations});
+    }
+  }
   const int variants[2] = {8, 16};
   if (kernels & EG_TUNE_PULL) {
     if (!adj) return eg_fail(EG_ENOTSUP, "pull tuning needs adjacency lists");

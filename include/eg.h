@@ -200,12 +200,15 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
 /* ---- kernel-shape autotuning (no reference counterpart) ----
  * Tunable shapes: "pull_batch" (pull PageRank / SpMV / PPR gather depth),
  * "spmm_depth" (batched PPR row gathers in flight), "ms_batch" (multi-source
- * BFS index pass depth); each takes 8 or 16.  Every variant computes
- * bit-identical results.  eg_autotune times the variants of the selected
+ * BFS index pass depth), each 8 or 16; "pull_hot" (sources cached normally in
+ * the degree-ordered pull, a vertex count, 0 = all); "grid_band" (grid source
+ * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
+ * replayed from a hipGraph).  Every variant computes bit-identical results.  eg_autotune times the variants of the selected
  * kernels on the graph itself, keeps the fastest, and reports each timing. */
 #define EG_TUNE_PULL 1u
 #define EG_TUNE_SPMM 2u
 #define EG_TUNE_MSBFS 4u
+#define EG_TUNE_GRID 8u /* grid PageRank / SpMV: "grid_band" source partitions per launch */
 typedef struct {
   char param[24];
   int32_t value;
@@ -227,7 +230,11 @@ int eg_tune_get(eg_graph *g, const char *param, int32_t *value);
  * and overlap on the device.  Lane groups and the PPR block are allocated
  * only while free device memory stays above hbm_reserve; a query waits for a
  * free slot, and fails with EG_ENOMEM if no slot of its kind can ever be
- * allocated.  A query's result does not depend on what else is in flight. */
+ * allocated.  A query's result does not depend on what else is in flight:
+ * BFS levels are exact, and a PPR vector is bit-identical whichever queries
+ * share its block (with idle_plain = -1; by default a query that finds the
+ * engine idle runs on the single-vector kernel, which sums in another order
+ * and agrees to FP32 rounding). */
 typedef struct eg_server eg_server;
 
 #define EG_REQ_BFS 0
@@ -244,6 +251,11 @@ typedef struct {
   uint32_t max_pending; /* queued queries before submit says EG_EAGAIN; 0 = 1<<20 */
   uint32_t ppr_grid;    /* workgroups of an SpMM iteration, leaving the rest of the
                            device to BFS levels; 0 = 8 per compute unit */
+  int32_t idle_plain;   /* with nothing in flight and at most this many queries
+                           waiting, serve the oldest through the plain path (eg_bfs /
+                           eg_ppr: lowest latency at light load); 0 = 2, -1 = never
+                           (every PPR result then bit-identical at any load) */
+  uint32_t pad;
 } eg_serve_opts;
 
 typedef struct {
@@ -264,6 +276,7 @@ typedef struct {
   uint32_t pad;
   uint64_t bfs_levels, ppr_steps;  /* device steps run */
   uint64_t device_bytes;           /* device memory held by the server */
+  uint64_t served_plain;           /* queries served through the plain path */
 } eg_serve_stats;
 
 int eg_serve_start(eg_graph *g, const eg_serve_opts *opts, eg_server **out);

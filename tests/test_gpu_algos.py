@@ -430,3 +430,26 @@ def test_ppr_graph_replay_matches_eager():
     for q, sd in enumerate(seeds):
         ref = O.ppr(n, cp, row, deg, sd, 0.15, 30)
         assert np.abs(eager[30][q] - ref).sum() <= PR_L1_TOL
+
+
+@pytest.mark.parametrize("mode", [0, 6])
+def test_grid_bands_bit_identical(mode):
+    """Grid PageRank / SpMV give the same bits for every source-band width."""
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=54)
+    w = O.gen_weights(s0.size, seed=5)
+    x = np.linspace(-1, 1, n).astype(np.float32)
+    outs = []
+    with eg.Graph(n, mode, weighted=True, P=7) as g:
+        g.load_edges(s0, d0, w)
+        for band in (1, 2, 3, 0):
+            g.tune_set("grid_band", band)
+            outs.append((g.pagerank(0.85, 20, layout="grid")[0], g.spmv(x, layout="grid")))
+        rep = g.autotune(eg.TUNE_GRID)
+        assert sorted(r[1] for r in rep) == [0, 1, 2, 4] and sum(r[3] for r in rep) == 1
+    for pr, y in outs[1:]:
+        assert np.array_equal(pr, outs[0][0]) and np.array_equal(y, outs[0][1])
+    _, _, _, ptr, col, ow, cp, row, iw = oracle_adj(n, s0, d0, w)
+    deg = np.diff(ptr).astype(np.uint32)
+    ref, _, _ = O.pagerank_pull(n, cp, row, deg, 0.85, 20)
+    assert np.abs(outs[0][0] - ref).sum() <= PR_L1_TOL

@@ -3,10 +3,13 @@
 Every query is run three ways: alone (submitted only after the previous one
 finished), in a mix submitted at once, and in a mix whose arrivals are
 staggered so that queries join traversals and SpMM iterations already in
-flight.  BFS levels must equal the oracle's bit for bit in every run.  A PPR
-vector must be bit-identical across the runs, take the same number of
-iterations, and lie within L1 1e-6 of the FP64 oracle run for that many
-iterations (the vector sums to <= 1).
+flight.  BFS levels must equal the oracle's bit for bit in every run.  With
+the engine always batching (idle_plain = -1) a PPR vector must be
+bit-identical across the runs, take the same number of iterations, and lie
+within L1 1e-6 of the FP64 oracle run for that many iterations (the vector
+sums to <= 1).  With the default load-adaptive engine a query that finds it
+idle runs on the single-vector kernel: its PPR vector is held to the same
+oracle bound, and agrees with the batched one within 2e-6 L1.
 """
 import time
 
@@ -92,11 +95,11 @@ def test_alone_equals_mix(scale, undirected):
     qs = queries(n, np.diff(ptr), 120)
     with eg.Graph(n, 2, undirected=undirected) as g:
         g.load_edges(s0, d0)
-        alone = run_alone(g, qs)
-        mix, st = run_mix(g, qs)
-        stag, _ = run_mix(g, qs, stagger_s=0.002)
+        alone = run_alone(g, qs, idle_plain=-1)
+        mix, st = run_mix(g, qs, idle_plain=-1)
+        stag, _ = run_mix(g, qs, stagger_s=0.002, idle_plain=-1)
         # three SpMM workgroups walk every work item (grid-stride)
-        tiny, _ = run_mix(g, qs, stagger_s=0.001, ppr_grid=3)
+        tiny, _ = run_mix(g, qs, stagger_s=0.001, ppr_grid=3, idle_plain=-1)
     check_against_oracle(n, m, ptr, col, cp, row, qs, alone)
     check_same(qs, alone, mix)
     check_same(qs, alone, stag)
@@ -107,6 +110,30 @@ def test_alone_equals_mix(scale, undirected):
     # batching happened: fewer level sweeps than BFS queries x depth
     assert 0 < st["bfs_levels"] < nb * 3
     assert st["ppr_steps"] > 0 and st["device_bytes"] > 0
+    assert st["served_plain"] == 0
+
+
+def test_load_adaptive_default():
+    """An idle engine serves a lone query on the plain kernels; under a burst
+    it batches.  Levels stay exact and PPR stays within the oracle bound."""
+    n, s0, d0, m, ptr, col, cp, row = build(12, 39, True)
+    qs = queries(n, np.diff(ptr), 60)
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        with g.serve() as S:
+            alone = [S.wait(submit(S, q)) for q in qs]
+            st_alone = S.stats()
+        mix, st_mix = run_mix(g, qs)
+        batched, _ = run_mix(g, qs, idle_plain=-1)
+    assert st_alone["served_plain"] == len(qs)  # every lone query found it idle
+    assert st_mix["served_plain"] < len(qs)     # the burst was batched
+    check_against_oracle(n, m, ptr, col, cp, row, qs, alone)
+    check_against_oracle(n, m, ptr, col, cp, row, qs, mix)
+    for i, (q, (ia, oa), (ib, ob)) in enumerate(zip(qs, alone, batched)):
+        if q[0] == "bfs":
+            assert np.array_equal(oa, ob), i
+        else:
+            assert np.abs(oa.astype(np.float64) - ob).sum() <= 2e-6, i
 
 
 def test_many_lane_groups():
@@ -164,7 +191,7 @@ def test_iteration_cap_matches_batch():
     with eg.Graph(n, 2, undirected=True) as g:
         g.load_edges(s0, d0)
         ref, _ = g.ppr_batch(seeds, 0.15, 7)
-        with g.serve(ppr_tol=1e-300, ppr_max_iter=7) as S:
+        with g.serve(ppr_tol=1e-300, ppr_max_iter=7, idle_plain=-1) as S:
             ts = [S.submit_ppr(s) for s in seeds]
             res = [S.wait(t) for t in ts]
     for q, (info, out) in enumerate(res):
@@ -206,7 +233,7 @@ def test_no_device_memory_for_ppr_fails_cleanly():
     n, s0, d0, *_ = build(10, 46, True)
     with eg.Graph(n, 2, undirected=True) as g:
         g.load_edges(s0, d0)
-        with g.serve(hbm_reserve=1 << 60) as S:
+        with g.serve(hbm_reserve=1 << 60, idle_plain=-1) as S:
             tb, tp = S.submit_bfs(0), S.submit_ppr([1])
             for t in (tb, tp):
                 with pytest.raises(eg.EgError) as e:
