@@ -175,7 +175,8 @@ bool eg_reorder_enabled();
 // Builds (on first use) and returns the degree-ordered pull copy.
 int eg_ordered(eg_graph *g, EgOrdered **out);
 // dst[i] = src[perm[i]] for 32-bit vectors of length n (on the graph stream).
-int eg_gather_vec(eg_graph *g, const void *src, const uint32_t *perm, void *dst);
+int eg_gather_vec(eg_graph *g, const void *src, const uint32_t *perm, void *dst,
+                  hipStream_t stream = nullptr);
 
 bool eg_has_in_edges(const eg_graph *g);
 int eg_scratch(eg_graph *g, size_t bytes, void **out);
@@ -186,6 +187,25 @@ int eg_require_loaded(const eg_graph *g);
 // iterations.  out (host, may be null) receives the n ranks.
 int eg_ppr_single(eg_graph *g, const uint32_t *seeds, uint32_t ns, double alpha, double tol,
                   int max_iter, float *out, int *iters, double *delta);
+
+// The same query stepped on an explicit stream (the serving engine overlaps
+// it with other work).  It uses the graph's PageRank vectors, so one runs at a
+// time.  eg_ppr1_step enqueues one iteration and the copy of its L1 change to
+// *h_delta (valid once the stream has passed it); eg_ppr1_output enqueues the
+// result's download to host memory and waits for it.
+struct Ppr1 {
+  const uint32_t *ptr = nullptr, *idx = nullptr, *deg = nullptr, *unperm = nullptr;
+  PullPlan *pl = nullptr;
+  float *rank = nullptr, *rank_next = nullptr, *contrib = nullptr, *contrib_next = nullptr;
+  double alpha = 0.15;
+  int it = 0, max_iter = 0;
+  hipStream_t stream = nullptr;
+  double *h_delta = nullptr;  // pinned, owned by the caller
+};
+int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, double alpha,
+                  int max_iter, hipStream_t stream, double *h_delta);
+int eg_ppr1_step(eg_graph *g, Ppr1 *st);
+int eg_ppr1_output(eg_graph *g, Ppr1 *st, float *out);
 // eg_bfs without the ownership check (the serving engine's plain path).
 int eg_bfs_host(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
                 uint32_t *dist, uint32_t *parent, eg_bfs_stats *stats);

@@ -88,8 +88,10 @@ void eg_order_free(eg_graph *g) {
 }
 
 // dst[i] = src[perm[i]] for float / u32 vectors of length n.
-int eg_gather_vec(eg_graph *g, const void *src, const uint32_t *perm, void *dst) {
-  hipLaunchKernelGGL(k_gather_u32, dim3(blocks_for(g->n, 256)), dim3(256), 0, g->stream, g->n,
+int eg_gather_vec(eg_graph *g, const void *src, const uint32_t *perm, void *dst,
+                  hipStream_t stream) {
+  hipLaunchKernelGGL(k_gather_u32, dim3(blocks_for(g->n, 256)), dim3(256), 0,
+                     stream ? stream : g->stream, g->n,
                      reinterpret_cast<const uint32_t *>(src), perm,
                      reinterpret_cast<uint32_t *>(dst));
   EG_LAUNCHED();

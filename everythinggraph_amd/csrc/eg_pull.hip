@@ -497,28 +497,27 @@ static void launch_k_pull(hipStream_t s, bool pr, uint32_t grid, const uint32_t 
 
 static int launch_pull(eg_graph *g, bool pr, const uint32_t *ptr, const uint32_t *idx,
                        const float *w, const float *x, float *y, const PullPlan &pl,
-                       const PrArgs &a_in) {
+                       const PrArgs &a_in, hipStream_t stream = nullptr) {
+  hipStream_t s = stream ? stream : g->stream;
   const PlanDev d = plan_dev(pl);
   PrArgs a = a_in;
   const bool ordered = g->ord && &pl == &g->ord->plan;
   a.hot = ordered && g->tune.pull_hot > 0 ? (uint32_t)g->tune.pull_hot : 0xFFFFFFFFu;
   const uint32_t grid = pl.nseg + pl.nchunks;
   if (grid) {
-    EgProf prof(g, pr ? "pr_pull" : "spmv_pull");
+    EgProf prof(g, pr ? "pr_pull" : "spmv_pull", s);
     if (g->tune.pull_batch == 8)
-      launch_k_pull<8>(g->stream, pr, grid, ptr, idx, w, x, y, d, a);
+      launch_k_pull<8>(s, pr, grid, ptr, idx, w, x, y, d, a);
     else
-      launch_k_pull<16>(g->stream, pr, grid, ptr, idx, w, x, y, d, a);
+      launch_k_pull<16>(s, pr, grid, ptr, idx, w, x, y, d, a);
     EG_LAUNCHED();
   }
   if (pr || pl.nlong) {
-    EgProf prof(g, pr ? "pr_finish" : "spmv_finish");
+    EgProf prof(g, pr ? "pr_finish" : "spmv_finish", s);
     if (pr)
-      hipLaunchKernelGGL((k_pull_finish<true>), dim3(kFinBlocks), dim3(kBlock), 0, g->stream, y, d,
-                         a);
+      hipLaunchKernelGGL((k_pull_finish<true>), dim3(kFinBlocks), dim3(kBlock), 0, s, y, d, a);
     else
-      hipLaunchKernelGGL((k_pull_finish<false>), dim3(kFinBlocks), dim3(kBlock), 0, g->stream, y,
-                         d, a);
+      hipLaunchKernelGGL((k_pull_finish<false>), dim3(kFinBlocks), dim3(kBlock), 0, s, y, d, a);
     EG_LAUNCHED();
   }
   return EG_OK;
@@ -682,26 +681,32 @@ __global__ void k_ppr1_seed(const uint32_t *__restrict__ seeds, uint32_t ns,
 }
 }  // namespace
 
-int eg_ppr_single(eg_graph *g, const uint32_t *seeds, uint32_t ns, double alpha, double tol,
-                  int max_iter, float *out, int *iters_out, double *delta_out) {
+int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, double alpha,
+                  int max_iter, hipStream_t stream, double *h_delta) {
   if (ns == 0 || max_iter < 0) return eg_fail(EG_EINVAL, "bad PPR query");
-  const uint32_t *ptr, *idx, *deg = g->out_deg, *unperm = nullptr, *perm = nullptr;
-  const float *w;
-  PullPlan *pl;
+  st->deg = g->out_deg;
+  st->unperm = nullptr;
+  const uint32_t *perm = nullptr;
   if (eg_reorder_enabled()) {
     EgOrdered *o;
     EG_TRY(eg_ordered(g, &o));
-    ptr = o->ptr;
-    idx = o->idx;
-    pl = &o->plan;
-    deg = o->deg;
-    unperm = perm = o->new_of_old;
+    st->ptr = o->ptr;
+    st->idx = o->idx;
+    st->pl = &o->plan;
+    st->deg = o->deg;
+    st->unperm = perm = o->new_of_old;
   } else {
-    EG_TRY(eg_pull_orientation(g, &ptr, &idx, &w, &pl));
+    const float *w;
+    EG_TRY(eg_pull_orientation(g, &st->ptr, &st->idx, &w, &st->pl));
   }
   EG_TRY(eg_pr_state(g, max_iter));
   const uint32_t n = g->n;
-  hipStream_t s = g->stream;
+  hipStream_t s = stream;
+  st->stream = s;
+  st->h_delta = h_delta;
+  st->alpha = alpha;
+  st->it = 0;
+  st->max_iter = max_iter;
   if (!g->ppr_base) EG_TRY(eg_dalloc_t(&g->ppr_base, n, "ppr base"));
   if (g->ppr_seeds_cap < ns) {
     EG_HIP(hipStreamSynchronize(s));
@@ -715,50 +720,73 @@ int eg_ppr_single(eg_graph *g, const uint32_t *seeds, uint32_t ns, double alpha,
     for (uint32_t i = 0; i < j && !dup; i++) dup = seeds[i] == seeds[j];
     distinct += !dup;
   }
-  float *rank = g->rank_a, *rank_next = g->rank_b;
-  float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
-  hipLaunchKernelGGL(k_ppr1_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rank, contrib,
-                     g->ppr_base);
+  st->rank = g->rank_a;
+  st->rank_next = g->rank_b;
+  st->contrib = g->contrib_a;
+  st->contrib_next = g->contrib_b;
+  hipLaunchKernelGGL(k_ppr1_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, st->rank,
+                     st->contrib, g->ppr_base);
   EG_LAUNCHED();
   EG_HIP(hipMemcpyAsync(g->ppr_seeds, seeds, (size_t)ns * 4, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_ppr1_seed, dim3(blocks_for(ns, 64)), dim3(64), 0, s, g->ppr_seeds, ns, perm,
-                     deg, (float)(1.0 / distinct), (float)(alpha / distinct), rank, contrib,
-                     g->ppr_base);
+                     st->deg, (float)(1.0 / distinct), (float)(alpha / distinct), st->rank,
+                     st->contrib, g->ppr_base);
   EG_LAUNCHED();
   EG_HIP(hipMemsetAsync(g->pr_stats, 0, 2 * sizeof(double), s));
+  EG_HIP(hipStreamSynchronize(s));  // seeds come from the caller's memory
+  return EG_OK;
+}
+
+int eg_ppr1_step(eg_graph *g, Ppr1 *st) {
+  if (st->it >= st->max_iter) return eg_fail(EG_ESTATE, "PPR query already at its iteration cap");
   PrArgs a;
-  a.out_deg = deg;
+  a.out_deg = st->deg;
   a.base = 0.f;
   a.base_vec = g->ppr_base;
-  a.damping = (float)(1.0 - alpha);
+  a.damping = (float)(1.0 - st->alpha);
   a.damping_over_n = 0.0;
   a.uniform = 0;
-  int it = 0;
+  a.rank_old = st->rank;
+  a.contrib_new = st->contrib_next;
+  a.dmass_in = g->pr_stats + 2 * st->it;
+  a.stats_out = g->pr_stats + 2 * (st->it + 1);
+  EG_TRY(launch_pull(g, true, st->ptr, st->idx, nullptr, st->contrib, st->rank_next, *st->pl, a,
+                     st->stream));
+  std::swap(st->rank, st->rank_next);
+  std::swap(st->contrib, st->contrib_next);
+  st->it++;
+  EG_HIP(hipMemcpyAsync(st->h_delta, a.stats_out + 1, sizeof(double), hipMemcpyDeviceToHost,
+                        st->stream));
+  return EG_OK;
+}
+
+int eg_ppr1_output(eg_graph *g, Ppr1 *st, float *out) {
+  if (out) {
+    const float *src = st->rank;
+    if (st->unperm) {
+      EG_TRY(eg_gather_vec(g, st->rank, st->unperm, st->rank_next, st->stream));
+      src = st->rank_next;
+    }
+    EG_HIP(hipMemcpyAsync(out, src, (size_t)g->n * 4, hipMemcpyDeviceToHost, st->stream));
+  }
+  EG_HIP(hipStreamSynchronize(st->stream));
+  return EG_OK;
+}
+
+int eg_ppr_single(eg_graph *g, const uint32_t *seeds, uint32_t ns, double alpha, double tol,
+                  int max_iter, float *out, int *iters_out, double *delta_out) {
+  Ppr1 st;
+  double *h_delta = reinterpret_cast<double *>(g->h_ctr);
+  EG_TRY(eg_ppr1_begin(g, &st, seeds, ns, alpha, max_iter, g->stream, h_delta));
   double delta = 0.0;
-  while (it < max_iter) {
-    a.rank_old = rank;
-    a.contrib_new = contrib_next;
-    a.dmass_in = g->pr_stats + 2 * it;
-    a.stats_out = g->pr_stats + 2 * (it + 1);
-    EG_TRY(launch_pull(g, true, ptr, idx, nullptr, contrib, rank_next, *pl, a));
-    std::swap(rank, rank_next);
-    std::swap(contrib, contrib_next);
-    it++;
-    EG_HIP(hipMemcpyAsync(g->h_ctr, a.stats_out + 1, sizeof(double), hipMemcpyDeviceToHost, s));
-    EG_HIP(hipStreamSynchronize(s));
-    memcpy(&delta, g->h_ctr, sizeof(double));
+  while (st.it < max_iter) {
+    EG_TRY(eg_ppr1_step(g, &st));
+    EG_HIP(hipStreamSynchronize(g->stream));
+    delta = *h_delta;
     if (delta <= tol) break;
   }
-  if (out) {
-    const float *src = rank;
-    if (unperm) {
-      EG_TRY(eg_gather_vec(g, rank, unperm, rank_next));
-      src = rank_next;
-    }
-    EG_HIP(hipMemcpyAsync(out, src, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-    EG_HIP(hipStreamSynchronize(s));
-  }
-  if (iters_out) *iters_out = it;
+  EG_TRY(eg_ppr1_output(g, &st, out));
+  if (iters_out) *iters_out = st.it;
   if (delta_out) *delta_out = delta;
   return EG_OK;
 }

@@ -76,6 +76,12 @@ struct eg_server {
   bool ppr_alloc = false, ppr_nomem = false;
   Query *col[64] = {};
   uint64_t cols = 0;
+  // single-vector PPR query stepped on the PPR stream (light load)
+  Ppr1 solo;
+  Query *solo_q = nullptr;
+  bool solo_busy = false;  // an iteration is in flight
+  hipEvent_t solo_ev = nullptr;
+  double *h_solo = nullptr;  // pinned L1 change of the solo iteration
   uint32_t *stage = nullptr;    // n words, BFS stream
   float *stage_ppr = nullptr;   // n floats, PPR stream
   uint64_t device_bytes = 0;
@@ -384,40 +390,142 @@ bool bfs_active(eg_server *S) {
 
 int plain_round(eg_server *S);
 
-int batched_round(eg_server *S) {
-  // Light load: with nothing in flight, a lone query is fastest on the plain
-  // single-query kernels, which have the whole device to themselves.
-  if (S->o.idle_plain > 0 && !S->ppr_busy && !S->cols && !bfs_active(S)) {
-    size_t waiting;
-    {
-      std::lock_guard<std::mutex> lk(S->mu);
-      waiting = S->pend_bfs.size() + S->pend_ppr.size();
-    }
-    if (waiting > 0 && waiting <= (size_t)S->o.idle_plain) return plain_round(S);
+size_t waiting(eg_server *S, const std::deque<Query *> &dq) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  return dq.size();
+}
+
+// Light-load PPR: the oldest waiting query runs on the single-vector kernel,
+// one iteration per round on the PPR stream, overlapping BFS work.
+int solo_start(eg_server *S) {
+  std::vector<Query *> q = take(S, S->pend_ppr, 1);
+  if (q.empty()) return EG_OK;
+  Query *qr = q[0];
+  started(S, qr);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stats.served_plain++;
   }
+  qr->level = 0;
+  const int rc = eg_ppr1_begin(S->g, &S->solo, qr->seeds.data(), (uint32_t)qr->seeds.size(),
+                               S->o.ppr_alpha, S->o.ppr_max_iter, S->s_ppr, S->h_solo);
+  if (rc != EG_OK) {
+    if (rc == EG_EDEVICE) return rc;
+    complete(S, qr, rc, eg_last_error());
+    return EG_OK;
+  }
+  S->solo_q = qr;
+  return EG_OK;
+}
+
+int solo_launch(eg_server *S) {
+  EG_TRY(eg_ppr1_step(S->g, &S->solo));
+  EG_HIP(hipEventRecord(S->solo_ev, S->s_ppr));
+  S->solo_busy = true;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->stats.ppr_steps++;
+  return EG_OK;
+}
+
+// After a solo iteration: retire the query or launch the next iteration.
+int solo_after(eg_server *S) {
+  Query *qr = S->solo_q;
+  qr->level = (uint32_t)S->solo.it;
+  qr->res.delta = *S->h_solo;
+  if (qr->res.delta <= S->o.ppr_tol || (int32_t)qr->level >= S->o.ppr_max_iter) {
+    EG_TRY(eg_ppr1_output(S->g, &S->solo, static_cast<float *>(qr->out)));
+    qr->res.levels = qr->level;
+    qr->res.reached = 0;
+    S->solo_q = nullptr;
+    complete(S, qr, EG_OK, nullptr);
+    return EG_OK;
+  }
+  return solo_launch(S);
+}
+
+// Light-load BFS: the oldest waiting query on the direction-optimising
+// single-source path (synchronous, on the graph's stream).
+int plain_bfs(eg_server *S) {
+  std::vector<Query *> q = take(S, S->pend_bfs, 1);
+  if (q.empty()) return EG_OK;
+  Query *qr = q[0];
+  started(S, qr);
+  eg_bfs_stats st{};
+  const int rc = eg_bfs_host(S->g, qr->root, 0, 1, static_cast<uint32_t *>(qr->out), nullptr, &st);
+  if (rc == EG_EDEVICE) return rc;
+  qr->res.levels = (uint32_t)st.levels;
+  qr->res.reached = st.reached;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stats.bfs_levels += (uint64_t)st.levels;
+    S->stats.served_plain++;
+  }
+  complete(S, qr, rc, rc ? eg_last_error() : nullptr);
+  return EG_OK;
+}
+
+int poll(eg_server *S, hipEvent_t ev, bool *done) {
+  const hipError_t e = hipEventQuery(ev);
+  *done = e == hipSuccess;
+  if (e != hipSuccess && e != hipErrorNotReady) EG_HIP(e);
+  return EG_OK;
+}
+
+int batched_round(eg_server *S) {
+  // A lone single-vector PPR takes ~20 pull iterations; an SpMM block with up
+  // to 16 columns costs ~3x one of them per step, so it only pays for a
+  // backlog of more than about three queries: PPR switches at twice the BFS
+  // threshold.
+  const size_t light = S->o.idle_plain > 0 ? (size_t)S->o.idle_plain : 0;
+  const size_t light_ppr = 2 * light;
+  // ---- PPR side (PPR stream) ----
+  bool done = false;
   if (S->ppr_busy) {
-    const hipError_t e = hipEventQuery(S->ppr_ev);
-    if (e == hipSuccess) {
+    EG_TRY(poll(S, S->ppr_ev, &done));
+    if (done) {
       S->ppr_busy = false;
       EG_TRY(ppr_after(S));
-    } else if (e != hipErrorNotReady) {
-      EG_HIP(e);
+    }
+  }
+  if (S->solo_busy) {
+    EG_TRY(poll(S, S->solo_ev, &done));
+    if (done) {
+      S->solo_busy = false;
+      EG_TRY(solo_after(S));
+    }
+  }
+  if (!S->solo_q && !S->cols && light_ppr) {
+    const size_t w = waiting(S, S->pend_ppr);
+    if (w > 0 && w <= light_ppr) {
+      EG_TRY(solo_start(S));
+      if (S->solo_q) EG_TRY(solo_launch(S));
     }
   }
   if (!S->ppr_busy) {
-    EG_TRY(ppr_admit(S));
+    // A short queue waits for the single-vector path; a longer one (or one
+    // that arrives while columns are busy) joins the SpMM block.
+    if (S->cols || !light_ppr || waiting(S, S->pend_ppr) > light_ppr) EG_TRY(ppr_admit(S));
     if (S->cols) EG_TRY(ppr_launch(S));
   }
-  EG_TRY(bfs_admit(S));
+  // ---- BFS side ----
   bool ran = false;
-  for (LaneGroup *G : S->groups) {
-    if (!G->active) continue;
-    EG_TRY(bfs_level(S, G));
-    ran = true;
+  if (light && !bfs_active(S)) {
+    const size_t w = waiting(S, S->pend_bfs);
+    if (w > 0 && w <= light) {
+      EG_TRY(plain_bfs(S));
+      ran = true;
+    }
   }
-  if (!ran && S->ppr_busy) {
-    // Only an SpMM iteration is in flight: sleep until it ends or a BFS
-    // query arrives.
+  if (!ran) {
+    EG_TRY(bfs_admit(S));
+    for (LaneGroup *G : S->groups) {
+      if (!G->active) continue;
+      EG_TRY(bfs_level(S, G));
+      ran = true;
+    }
+  }
+  if (!ran && (S->ppr_busy || S->solo_busy)) {
+    // Only PPR work is in flight: sleep until it ends or a BFS query arrives.
     std::unique_lock<std::mutex> lk(S->mu);
     S->cv_work.wait_for(lk, std::chrono::microseconds(200),
                         [&] { return !S->pend_bfs.empty(); });
@@ -492,6 +600,10 @@ void fail_everything(eg_server *S) {
       S->col[q] = nullptr;
     }
   S->cols = 0;
+  if (S->solo_q) {
+    complete(S, S->solo_q, S->fatal, msg.c_str());
+    S->solo_q = nullptr;
+  }
 }
 
 void engine(eg_server *S) {
@@ -501,7 +613,7 @@ void engine(eg_server *S) {
       std::unique_lock<std::mutex> lk(S->mu);
       for (;;) {
         const bool work = !S->pend_bfs.empty() || !S->pend_ppr.empty() || S->cols ||
-                          S->ppr_busy || bfs_active(S);
+                          S->ppr_busy || S->solo_q || bfs_active(S);
         if (work) break;
         if (S->stopping) return;
         S->cv_work.wait(lk);
@@ -565,6 +677,8 @@ void destroy(eg_server *S) {
   for (auto &kv : S->all) delete kv.second;
   S->all.clear();
   if (S->ppr_ev) (void)hipEventDestroy(S->ppr_ev);
+  if (S->solo_ev) (void)hipEventDestroy(S->solo_ev);
+  if (S->h_solo) (void)hipHostFree(S->h_solo);
   if (S->s_bfs) (void)hipStreamDestroy(S->s_bfs);
   if (S->s_ppr) (void)hipStreamDestroy(S->s_ppr);
   if (S->g && S->g->server == S) S->g->server = nullptr;
@@ -616,6 +730,11 @@ extern "C" int eg_serve_start(eg_graph *g, const eg_serve_opts *opts, eg_server 
     rc = eg_fail(EG_EDEVICE, "stream creation failed");
   if (rc == EG_OK && hipEventCreateWithFlags(&S->ppr_ev, hipEventDisableTiming) != hipSuccess)
     rc = eg_fail(EG_EDEVICE, "event creation failed");
+  if (rc == EG_OK && hipEventCreateWithFlags(&S->solo_ev, hipEventDisableTiming) != hipSuccess)
+    rc = eg_fail(EG_EDEVICE, "event creation failed");
+  if (rc == EG_OK && hipHostMalloc(reinterpret_cast<void **>(&S->h_solo), sizeof(double),
+                                   hipHostMallocDefault) != hipSuccess)
+    rc = eg_fail(EG_ENOMEM, "pinned solo delta");
   if (rc == EG_OK && hipStreamSynchronize(g->stream) != hipSuccess)
     rc = eg_fail(EG_EDEVICE, "graph stream failed");
   if (rc != EG_OK) {

@@ -232,9 +232,9 @@ int eg_tune_get(eg_graph *g, const char *param, int32_t *value);
  * free slot, and fails with EG_ENOMEM if no slot of its kind can ever be
  * allocated.  A query's result does not depend on what else is in flight:
  * BFS levels are exact, and a PPR vector is bit-identical whichever queries
- * share its block (with idle_plain = -1; by default a query that finds the
- * engine idle runs on the single-vector kernel, which sums in another order
- * and agrees to FP32 rounding). */
+ * share its block (with idle_plain = -1; by default a query that arrives at
+ * light load runs on the single-vector kernel, which sums in another order and
+ * agrees to FP32 rounding). */
 typedef struct eg_server eg_server;
 
 #define EG_REQ_BFS 0
@@ -251,10 +251,13 @@ typedef struct {
   uint32_t max_pending; /* queued queries before submit says EG_EAGAIN; 0 = 1<<20 */
   uint32_t ppr_grid;    /* workgroups of an SpMM iteration, leaving the rest of the
                            device to BFS levels; 0 = 8 per compute unit */
-  int32_t idle_plain;   /* with nothing in flight and at most this many queries
-                           waiting, serve the oldest through the plain path (eg_bfs /
-                           eg_ppr: lowest latency at light load); 0 = 2, -1 = never
-                           (every PPR result then bit-identical at any load) */
+  int32_t idle_plain;   /* light load: with no BFS lane busy and at most this many
+                           BFS queries waiting, the oldest runs on the single-source
+                           path (eg_bfs); with no PPR column busy and at most twice
+                           this many PPR queries waiting, the oldest runs on the
+                           single-vector kernel (eg_ppr), stepped beside BFS work.
+                           0 = 2, -1 = always batch (every PPR result then
+                           bit-identical at any load) */
   uint32_t pad;
 } eg_serve_opts;
 
