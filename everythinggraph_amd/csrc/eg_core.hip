@@ -16,6 +16,8 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <thread>
+#include <vector>
 
 #include "eg_internal.h"
 
@@ -353,6 +355,103 @@ static int finish_records(eg_graph *g) {
   return eg_build_layout(g);
 }
 
+// ---------------------------------------------------------------------------
+// Record ingest: a two-slot pipeline.  Chunk c is copied from its source (the
+// caller's memory or the file, by several host threads) into pinned buffer
+// c % 2, sent to device staging c % 2 and unpacked into the record arrays;
+// while the device works on chunk c the host fills the other slot.  A slot is
+// refilled only after the event that follows its unpack has completed.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Source {
+  const uint8_t *mem = nullptr;  // or
+  int fd = -1;
+};
+
+// Copies [off, off + len) of the source into dst with up to `threads` threads.
+int read_range(const Source &src, uint64_t off, uint64_t len, uint8_t *dst, unsigned threads) {
+  const uint64_t per = (len + threads - 1) / threads;
+  std::vector<std::thread> pool;
+  std::vector<int> err(threads, 0);
+  for (unsigned t = 0; t < threads; t++) {
+    const uint64_t a = std::min<uint64_t>(len, t * per), b = std::min<uint64_t>(len, a + per);
+    if (a >= b) break;
+    pool.emplace_back([&, t, a, b] {
+      if (src.mem) {
+        memcpy(dst + a, src.mem + off + a, b - a);
+        return;
+      }
+      uint64_t done = a;
+      while (done < b) {
+        const ssize_t r = pread(src.fd, dst + done, b - done, (off_t)(off + done));
+        if (r <= 0) {
+          err[t] = r < 0 ? errno : EIO;
+          return;
+        }
+        done += (uint64_t)r;
+      }
+    });
+  }
+  for (auto &th : pool) th.join();
+  for (unsigned t = 0; t < threads; t++)
+    if (err[t]) return eg_fail(EG_EIO, "read failed: %s", strerror(err[t]));
+  return EG_OK;
+}
+
+}  // namespace
+
+static int ingest(eg_graph *g, const Source &src, uint64_t m_in, uint64_t isz) {
+  hipStream_t s = g->stream;
+  const uint64_t total = m_in * isz;
+  const uint64_t want = g->tune.ingest_kib > 0 ? (uint64_t)g->tune.ingest_kib << 10 : 64ull << 20;
+  const uint64_t chunk = std::max<uint64_t>(isz, want / isz * isz);
+  const uint64_t cap = std::min(chunk, total);
+  unsigned threads = std::thread::hardware_concurrency();
+  threads = std::max(1u, std::min(threads, 8u));
+  uint8_t *pin[2] = {nullptr, nullptr};
+  void *dev[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  bool used[2] = {false, false};
+  int rc = EG_OK;
+  for (int b = 0; b < 2 && rc == EG_OK; b++) {
+    if (hipHostMalloc(reinterpret_cast<void **>(&pin[b]), cap, hipHostMallocDefault) != hipSuccess)
+      rc = eg_fail(EG_ENOMEM, "pinned ingest buffer");
+    if (rc == EG_OK) rc = eg_dalloc(&dev[b], cap, "ingest staging");
+    if (rc == EG_OK && hipEventCreateWithFlags(&done[b], hipEventDisableTiming) != hipSuccess)
+      rc = eg_fail(EG_EDEVICE, "event creation failed");
+  }
+  uint64_t off = 0;
+  for (uint64_t c = 0; rc == EG_OK && off < total; c++, off += chunk) {
+    const int b = (int)(c & 1);
+    const uint64_t len = std::min(chunk, total - off);
+    if (used[b] && hipEventSynchronize(done[b]) != hipSuccess) {
+      rc = eg_fail(EG_EDEVICE, "ingest copy failed");
+      break;
+    }
+    rc = read_range(src, off, len, pin[b], threads);
+    if (rc) break;
+    const uint64_t r0 = off / isz;
+    hipError_t e = hipMemcpyAsync(dev[b], pin[b], len, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      rc = eg_fail(EG_EDEVICE, "upload: %s", hipGetErrorString(e));
+      break;
+    }
+    rc = eg_unpack_records(s, dev[b], len / isz, isz == 12, g->e_src + r0, g->e_dst + r0,
+                           g->e_w ? g->e_w + r0 : nullptr);
+    if (rc == EG_OK && hipEventRecord(done[b], s) != hipSuccess)
+      rc = eg_fail(EG_EDEVICE, "event record failed");
+    used[b] = true;
+  }
+  (void)hipStreamSynchronize(s);
+  for (int b = 0; b < 2; b++) {
+    if (done[b]) (void)hipEventDestroy(done[b]);
+    eg_dfree(&dev[b]);
+    if (pin[b]) (void)hipHostFree(pin[b]);
+  }
+  return rc;
+}
+
 extern "C" int eg_graph_load_records(eg_graph *g, const void *records, uint64_t nbytes) {
   if (!g) return eg_fail(EG_EINVAL, "null graph");
   const uint64_t isz = (g->flags & EG_WEIGHTED) ? 12 : 8;
@@ -360,15 +459,9 @@ extern "C" int eg_graph_load_records(eg_graph *g, const void *records, uint64_t 
   if (m_in && !records) return eg_fail(EG_EINVAL, "null records");
   EG_TRY(alloc_records(g, m_in));
   if (m_in) {
-    void *raw = nullptr;
-    EG_TRY(eg_dalloc(&raw, m_in * isz, "raw records"));
-    hipError_t e = hipMemcpyAsync(raw, records, m_in * isz, hipMemcpyHostToDevice, g->stream);
-    int rc = e == hipSuccess ? eg_unpack_records(g->stream, raw, m_in, isz == 12, g->e_src,
-                                                 g->e_dst, g->e_w)
-                             : eg_fail(EG_EDEVICE, "upload: %s", hipGetErrorString(e));
-    (void)hipStreamSynchronize(g->stream);
-    eg_dfree(&raw);
-    if (rc) return rc;
+    Source src;
+    src.mem = static_cast<const uint8_t *>(records);
+    EG_TRY(ingest(g, src, m_in, isz));
   }
   return finish_records(g);
 }
@@ -383,19 +476,17 @@ extern "C" int eg_graph_load_file(eg_graph *g, const char *path) {
     close(fd);
     return eg_fail(EG_EIO, "cannot stat %s", path);
   }
-  const uint64_t size = (uint64_t)sb.st_size;
-  void *map = nullptr;
-  if (size) {
-    map = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
-    if (map == MAP_FAILED) {
-      close(fd);
-      return eg_fail(EG_EIO, "cannot mmap %s", path);
-    }
+  const uint64_t isz = (g->flags & EG_WEIGHTED) ? 12 : 8;
+  const uint64_t m_in = (uint64_t)sb.st_size / isz;  // init_all.c:821
+  int rc = alloc_records(g, m_in);
+  if (rc == EG_OK && m_in) {
+    Source src;
+    src.fd = fd;
+    rc = ingest(g, src, m_in, isz);
   }
-  int rc = eg_graph_load_records(g, map, size);
-  if (map) munmap(map, size);
   close(fd);
-  return rc;
+  if (rc) return rc;
+  return finish_records(g);
 }
 
 extern "C" int eg_graph_load_edges(eg_graph *g, uint64_t m_in, const uint32_t *src,

@@ -102,6 +102,7 @@ struct EgTune {
   int ms_batch = 8;      // k_ms_pull: entries per lane per gather pass (8 or 16)
   int graphs = 1;        // batched PPR iterations replayed from a hipGraph (0: eager)
   int grid_band = 1;     // grid pull: source partitions per launch (0: all)
+  int ingest_kib = 0;    // record ingest chunk in KiB (0: 64 MiB)
   int pull_hot = 0;      // k_pull on the degree-ordered copy: sources below this id
                          // are cached normally, the rest loaded non-temporally (0: all normal)
 };

@@ -10,6 +10,8 @@
 //  * pull_hot    -- k_pull on the degree-ordered copy: gathers of sources with
 //                   new id >= pull_hot bypass normal caching (non-temporal
 //                   loads) so that the hottest sources stay in L2; 0 = off;
+//  * ingest_kib  -- record ingest chunk (KiB) of the pinned two-slot pipeline
+//                   (0 = 64 MiB); not timed by eg_autotune;
 //  * graphs      -- 1: batched PPR iterations are replayed from a hipGraph;
 //  * grid_band   -- grid pull: source partitions per launch (0 = all), so that
 //                   one launch gathers from a band of x that fits in L2.
@@ -31,6 +33,7 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "pull_hot")) return &g->tune.pull_hot;
   if (!strcmp(name, "graphs")) return &g->tune.graphs;
   if (!strcmp(name, "grid_band")) return &g->tune.grid_band;
+  if (!strcmp(name, "ingest_kib")) return &g->tune.ingest_kib;
   return nullptr;
 }
 
@@ -70,6 +73,8 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
   if (!slot) return eg_fail(EG_EINVAL, "unknown tuning parameter '%s'", param);
   if (!strcmp(param, "pull_hot")) {
     if (value < 0) return eg_fail(EG_EINVAL, "pull_hot takes a vertex count >= 0");
+  } else if (!strcmp(param, "ingest_kib")) {
+    if (value < 0) return eg_fail(EG_EINVAL, "ingest_kib takes a size >= 0");
   } else if (!strcmp(param, "grid_band")) {
     if (value < 0) return eg_fail(EG_EINVAL, "grid_band takes a partition count >= 0");
   } else if (!strcmp(param, "graphs")) {

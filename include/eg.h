@@ -203,7 +203,9 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * BFS index pass depth), each 8 or 16; "pull_hot" (sources cached normally in
  * the degree-ordered pull, a vertex count, 0 = all); "grid_band" (grid source
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
- * replayed from a hipGraph).  Every variant computes bit-identical results.  eg_autotune times the variants of the selected
+ * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
+ * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant
+ * computes bit-identical results.  eg_autotune times the variants of the selected
  * kernels on the graph itself, keeps the fastest, and reports each timing. */
 #define EG_TUNE_PULL 1u
 #define EG_TUNE_SPMM 2u
