@@ -383,10 +383,20 @@ def other_configs(eg) -> dict:
     prof = g.prof_read()
     g.prof_enable(False)
     wl, wt = prof.get("wcc_relax", (0, 0.0))
+    g.sssp(int(roots[0]), download=False)
+    sssp_ms, sssp_rounds = [], []
+    for rt in roots[:8]:
+        _, _, st = g.sssp(int(rt), download=False, parents=False)
+        sssp_ms.append(st.device_ms)
+        sssp_rounds.append(st.iterations)
     res["rmat22_u"] = {
         "n": n, "m": g.info.m, "csr_build_ms": round(build_ms, 2), "bfs_gteps": round(gteps, 2),
         "wcc_rounds": rounds, "wcc_relax_ms": round(wt, 3), "wcc_wall_ms": round(wcc_wall * 1e3, 2),
+        "sssp_ms": round(float(np.mean(sssp_ms)), 3), "sssp_rounds": float(np.mean(sssp_rounds)),
+        "sssp_weights": "(u % 10) + (v % 10)",
     }
+    log(f"RMAT-22 -u SSSP (reference weights): {np.mean(sssp_ms):.2f} ms, "
+        f"{np.mean(sssp_rounds):.1f} rounds (mean of 8 roots)")
     log(f"RMAT-22 -u: CSR+CSC build {build_ms:.1f} ms; BFS {gteps:.1f} GTEPS; WCC {rounds} rounds, "
         f"relax {wt:.2f} ms, call {wcc_wall * 1e3:.1f} ms (incl. label download)")
     g.close()

@@ -53,7 +53,7 @@ ABI_SYMBOLS = (
     "eg_graph_load_file", "eg_graph_load_records", "eg_graph_load_edges",
     "eg_graph_generate", "eg_generate_edges", "eg_graph_free", "eg_graph_get_info",
     "eg_graph_get_degrees", "eg_graph_get_csr", "eg_graph_get_csc", "eg_graph_get_grid",
-    "eg_graph_get_edgelist", "eg_pagerank", "eg_spmv", "eg_bfs", "eg_wcc",
+    "eg_graph_get_edgelist", "eg_pagerank", "eg_spmv", "eg_bfs", "eg_wcc", "eg_sssp",
     "eg_bfs_batch", "eg_ppr_batch", "eg_prof_enable", "eg_prof_reset", "eg_prof_read",
     "eg_sync", "eg_ppr", "eg_autotune", "eg_tune_set", "eg_tune_get", "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
     "eg_serve_wait", "eg_serve_stats_get", "eg_serve_stop", "eg_serve_replay",
@@ -95,6 +95,15 @@ class BfsStats(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("pull_levels", ctypes.c_int32),
                 ("reached", ctypes.c_uint64), ("edges_traversed", ctypes.c_uint64),
                 ("device_ms", ctypes.c_double)]
+
+
+class SsspStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("pull_iterations", ctypes.c_int32),
+                ("reached", ctypes.c_uint64), ("device_ms", ctypes.c_double)]
+
+
+SSSP_WEIGHTS = {"ref": 0, "unit": 1}
+SSSP_UNREACHED = 0xFFFFFFFE
 
 
 class KernelTime(ctypes.Structure):
@@ -354,6 +363,17 @@ class Graph:
         _check(lib().eg_bfs(self._h, ctypes.c_uint32(root), ctypes.c_int32(int(start_pull)),
                             ctypes.c_int32(int(switch)), _ptr(dist), _ptr(par), ctypes.byref(st)),
                "eg_bfs")
+        return dist, par, st
+
+    def sssp(self, root: int, start_pull=False, switch=True, weights="ref", parents=True,
+             download=True):
+        """(dist, parent or None, SsspStats); weights "ref" = (u%10)+(v%10), "unit" = 1."""
+        dist = np.empty(self.n, np.uint32) if download else None
+        par = np.empty(self.n, np.uint32) if (parents and download) else None
+        st = SsspStats()
+        _check(lib().eg_sssp(self._h, ctypes.c_uint32(root), ctypes.c_int32(int(start_pull)),
+                             ctypes.c_int32(int(switch)), ctypes.c_int32(SSSP_WEIGHTS[weights]),
+                             _ptr(dist), _ptr(par), ctypes.byref(st)), "eg_sssp")
         return dist, par, st
 
     def wcc(self):

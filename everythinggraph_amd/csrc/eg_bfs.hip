@@ -342,6 +342,186 @@ __global__ __launch_bounds__(kBlock) void k_reached_edges(uint32_t n,
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// SSSP (sssp_pushpull.c): frontier Bellman-Ford on the same bitmaps.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSsspInf = 0xFFFFFFFEu;  // UINT32_MAX - 1, sssp_pushpull.c:8
+
+// Edge weight: (u % 10) + (v % 10) as CREATE_WEIGHT builds it
+// (init_all.c:661-686), or 1.
+__device__ __forceinline__ uint32_t sssp_w(uint32_t u, uint32_t v, int unit) {
+  return unit ? 1u : (u % 10u) + (v % 10u);
+}
+
+__global__ void k_sssp_init(uint32_t n, uint32_t nwords, uint32_t root, uint32_t *__restrict__ dist,
+                            uint64_t *__restrict__ front, uint64_t *__restrict__ next) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dist[i] = i == root ? 0u : kSsspInf;
+  if (i < nwords) {
+    front[i] = (root >> 6) == i ? 1ull << (root & 63) : 0ull;
+    next[i] = 0ull;
+  }
+}
+
+// Push: the frontier queue's edges, kEdgesPerBlock per workgroup (owners by
+// binary search in LDS).  A shorter distance is published with atomicMin and
+// the vertex joins the next frontier.
+__global__ __launch_bounds__(kBlock) void k_sssp_push(
+    const uint32_t *__restrict__ q, uint32_t qn, const uint32_t *__restrict__ qoff,
+    const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ col,
+    uint32_t *__restrict__ dist, uint64_t *__restrict__ next, int unit) {
+  __shared__ uint32_t s_off[kEdgesPerBlock + 1];
+  __shared__ uint32_t s_u[kEdgesPerBlock + 1];
+  __shared__ uint32_t s_first, s_cnt;
+  const uint32_t e0 = blockIdx.x * kEdgesPerBlock;
+  const uint32_t total = qoff[qn];
+  const uint32_t e1 = e0 + kEdgesPerBlock < total ? e0 + kEdgesPerBlock : total;
+  if (threadIdx.x == 0) {
+    uint32_t lo = 0, hi = qn;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (qoff[mid] <= e0) lo = mid; else hi = mid;
+    }
+    uint32_t a = lo, b = qn;
+    while (a < b) {
+      const uint32_t mid = a + (b - a) / 2;
+      if (qoff[mid] < e1) a = mid + 1; else b = mid;
+    }
+    s_first = lo;
+    s_cnt = a - lo;
+  }
+  __syncthreads();
+  const uint32_t first = s_first;
+  const uint32_t cnt = s_cnt <= kEdgesPerBlock + 1 ? s_cnt : kEdgesPerBlock + 1;
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+    s_off[i] = qoff[first + i];
+    s_u[i] = q[first + i];
+  }
+  __syncthreads();
+  for (uint32_t base = e0; base < e1; base += kBlock) {
+    const uint32_t e = base + threadIdx.x;
+    if (e >= e1) continue;
+    uint32_t lo = 0, hi = cnt;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (s_off[mid] <= e) lo = mid; else hi = mid;
+    }
+    const uint32_t u = s_u[lo];
+    const uint32_t v = col[ptr[u] + (e - s_off[lo])];
+    if (v == u) continue;  // sssp_pushpull.c:56
+    const uint64_t nd = (uint64_t)__atomic_load_n(&dist[u], __ATOMIC_RELAXED) + sssp_w(u, v, unit);
+    if (nd >= __atomic_load_n(&dist[v], __ATOMIC_RELAXED)) continue;
+    const uint32_t old = atomicMin(&dist[v], (uint32_t)nd);
+    if (nd < old) {
+      const uint64_t bit = 1ull << (v & 63);
+      if (!(__atomic_load_n(&next[v >> 6], __ATOMIC_RELAXED) & bit))
+        atomicOr(reinterpret_cast<unsigned long long *>(&next[v >> 6]), (unsigned long long)bit);
+    }
+  }
+}
+
+// Pull: a wave per bitmap word; every vertex takes the minimum over its
+// in-neighbours in the frontier (sssp_pushpull.c:75-104) and the wave writes
+// its next-frontier word whole.
+__global__ __launch_bounds__(kBlock) void k_sssp_pull(
+    uint32_t n, uint32_t nwords, const uint32_t *__restrict__ ptr,
+    const uint32_t *__restrict__ idx, const uint64_t *__restrict__ front,
+    uint64_t *__restrict__ next, uint32_t *__restrict__ dist, int unit) {
+  const int lane = __lane_id();
+  const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t w = wave0; w < nwords; w += nwaves) {
+    const uint32_t v = w * 64 + lane;
+    bool changed = false;
+    if (v < n) {
+      const uint32_t cur = dist[v];
+      uint64_t best = cur;
+      const uint32_t k1 = ptr[v + 1];
+      for (uint32_t k = ptr[v]; k < k1; k += 4) {
+        uint32_t u[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) u[t] = k + t < k1 ? idx[k + t] : v;
+        uint64_t fw[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) fw[t] = front[u[t] >> 6];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          if (u[t] != v && ((fw[t] >> (u[t] & 63)) & 1ull)) {
+            const uint64_t nd =
+                (uint64_t)__atomic_load_n(&dist[u[t]], __ATOMIC_RELAXED) + sssp_w(u[t], v, unit);
+            if (nd < best) best = nd;
+          }
+        }
+      }
+      if (best < cur) {
+        dist[v] = (uint32_t)best;
+        changed = true;
+      }
+    }
+    const uint64_t nw = __ballot(changed);
+    if (lane == 0) next[w] = nw;
+  }
+}
+
+// Per-workgroup (set bits, their out-degree sum) of a frontier bitmap.
+__global__ __launch_bounds__(kBlock) void k_front_stats(const uint64_t *__restrict__ bm,
+                                                        uint32_t nwords,
+                                                        const uint32_t *__restrict__ out_ptr,
+                                                        unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long s_tmp[kWaves];
+  unsigned long long c = 0, d = 0;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+       w += gridDim.x * blockDim.x) {
+    uint64_t b = bm[w];
+    c += __popcll(b);
+    while (b) {
+      const uint32_t v = w * 64 + (uint32_t)(__ffsll((unsigned long long)b) - 1);
+      d += out_ptr[v + 1] - out_ptr[v];
+      b &= b - 1;
+    }
+  }
+  c = block_sum_u64(c, s_tmp);
+  d = block_sum_u64(d, s_tmp);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = c;
+    part[2 * blockIdx.x + 1] = d;
+  }
+}
+
+// Parents: the first in-neighbour (in list order) on a shortest path; root =
+// root, unreached = UINT32_MAX - 1 (the reference's initial value,
+// sssp_pushpull.c:203).  Also per-workgroup counts of reached vertices.
+__global__ __launch_bounds__(kBlock) void k_sssp_finish(uint32_t n, uint32_t root,
+                                                        const uint32_t *__restrict__ ptr,
+                                                        const uint32_t *__restrict__ idx,
+                                                        const uint32_t *__restrict__ dist,
+                                                        uint32_t *__restrict__ parent, int unit,
+                                                        unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long s_tmp[kWaves];
+  unsigned long long reached = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t dv = dist[v];
+    reached += dv != kSsspInf;
+    if (!parent) continue;
+    uint32_t p = kSsspInf;
+    if (v == root) {
+      p = root;
+    } else if (dv != kSsspInf) {
+      for (uint32_t k = ptr[v]; k < ptr[v + 1]; k++) {
+        const uint32_t u = idx[k];
+        if (u != v && dist[u] != kSsspInf && (uint64_t)dist[u] + sssp_w(u, v, unit) == dv) {
+          p = u;
+          break;
+        }
+      }
+    }
+    parent[v] = p;
+  }
+  reached = block_sum_u64(reached, s_tmp);
+  if (threadIdx.x == 0) part[blockIdx.x] = reached;
+}
+
 }  // namespace
 
 static int bfs_state(eg_graph *g) {
@@ -579,5 +759,113 @@ int eg_bfs_host(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_m
   st.reached = h[1];
   st.edges_traversed = (g->flags & EG_UNDIRECTED) ? h[0] / 2 : h[0];
   if (stats) *stats = st;
+  return EG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SSSP driver: the iterator of sssp_pushpull.c:139-188 with the bfs_simple
+// switch rule (:174-179), until no distance changes.
+// ---------------------------------------------------------------------------
+extern "C" int eg_sssp(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
+                       int32_t weights, uint32_t *dist, uint32_t *parent, eg_sssp_stats *stats) {
+  EG_TRY(eg_require_loaded(g));
+  if (!g->out_ptr) return eg_fail(EG_ENOTSUP, "SSSP needs adjacency lists (modes 2/3/4)");
+  if (root >= g->n) return eg_fail(EG_ERANGE, "root %u >= n = %u", root, g->n);
+  if (weights != EG_SSSP_WEIGHT_REF && weights != EG_SSSP_WEIGHT_UNIT)
+    return eg_fail(EG_EINVAL, "unknown SSSP weight kind %d", weights);
+  EG_TRY(bfs_state(g));
+  const uint32_t n = g->n, nwords = (n + 63) / 64;
+  const int unit = weights == EG_SSSP_WEIGHT_UNIT;
+  hipStream_t s = g->stream;
+  const uint32_t *optr = g->out_ptr, *ocol = g->out_col;
+  const uint32_t *iptr = g->in_ptr ? g->in_ptr : g->out_ptr;
+  const uint32_t *iidx = g->in_ptr ? g->in_idx : g->out_col;
+  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(g->bfs_ctr);
+  uint32_t *qoff = g->bfs_nq;
+  void *scan_tmp = nullptr;
+  EG_TRY(eg_scratch(g, eg_scan_tmp_bytes((uint64_t)n + 1), &scan_tmp));
+  hipEvent_t e0, e1;
+  EG_HIP(hipEventCreate(&e0));
+  EG_HIP(hipEventCreate(&e1));
+  EG_HIP(hipEventRecord(e0, s));
+  hipLaunchKernelGGL(k_sssp_init, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, nwords, root,
+                     g->bfs_dist, g->bfs_front, g->bfs_next);
+  EG_LAUNCHED();
+  uint64_t h[2];
+  uint64_t count = 1;
+  EG_HIP(hipMemcpyAsync(g->h_ctr, optr + root, 8, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  uint64_t degsum = (uint64_t)g->h_ctr[1] - g->h_ctr[0];
+  bool pull = start_pull != 0;
+  const double thr = (double)g->m / 20.0;
+  const uint32_t wblocks = stride_blocks((uint64_t)nwords * 64);
+  const uint32_t sblocks = stride_blocks(nwords);
+  int iters = 0, pull_iters = 0;
+  int rc = EG_OK;
+  while (count && rc == EG_OK) {
+    if ((uint64_t)iters > (uint64_t)n + 1) {
+      rc = eg_fail(EG_EDEVICE, "SSSP did not converge within n rounds");
+      break;
+    }
+    iters++;
+    EG_HIP(hipMemsetAsync(g->bfs_next, 0, (size_t)nwords * 8, s));
+    if (!pull) {
+      uint32_t qn = 0;
+      EG_TRY(bitmap_to_queue(g, g->bfs_front, g->bfs_q, &qn, scan_tmp));
+      if (qn && degsum) {
+        hipLaunchKernelGGL(k_frontier_degrees, dim3(blocks_for(qn, 256)), dim3(256), 0, s, g->bfs_q,
+                           qn, optr, qoff);
+        EG_LAUNCHED();
+        EG_TRY(eg_scan_excl(s, qoff, qoff, qn, scan_tmp));
+        EgProf prof(g, "sssp_push");
+        hipLaunchKernelGGL(k_sssp_push, dim3(blocks_for(degsum, kEdgesPerBlock)), dim3(kBlock), 0,
+                           s, g->bfs_q, qn, qoff, optr, ocol, g->bfs_dist, g->bfs_next, unit);
+        EG_LAUNCHED();
+      }
+    } else {
+      pull_iters++;
+      EgProf prof(g, "sssp_pull");
+      hipLaunchKernelGGL(k_sssp_pull, dim3(wblocks), dim3(kBlock), 0, s, n, nwords, iptr, iidx,
+                         g->bfs_front, g->bfs_next, g->bfs_dist, unit);
+      EG_LAUNCHED();
+    }
+    std::swap(g->bfs_front, g->bfs_next);
+    hipLaunchKernelGGL(k_front_stats, dim3(sblocks), dim3(kBlock), 0, s, g->bfs_front, nwords,
+                       optr, g->bfs_part);
+    EG_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, g->bfs_part, sblocks, 2u, ctr);
+    EG_LAUNCHED();
+    EG_TRY(read_u64(g, ctr, 2, h));
+    count = h[0];
+    degsum = h[1];
+    // bfs_simple.c:191-197 (the same rule, sssp_pushpull.c:174-179)
+    const double load = (double)count + (double)degsum;
+    pull = (switch_mode && !pull && load > thr) || (pull && load > thr);
+  }
+  if (rc == EG_OK) {
+    const uint32_t fb = stride_blocks(n);
+    hipLaunchKernelGGL(k_sssp_finish, dim3(fb), dim3(kBlock), 0, s, n, root, iptr, iidx,
+                       g->bfs_dist, parent ? g->bfs_parent : nullptr, unit, g->bfs_part);
+    EG_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, g->bfs_part, fb, 1u, ctr);
+    EG_LAUNCHED();
+  }
+  (void)hipEventRecord(e1, s);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (dist) EG_HIP(hipMemcpyAsync(dist, g->bfs_dist, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  if (parent)
+    EG_HIP(hipMemcpyAsync(parent, g->bfs_parent, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  EG_TRY(read_u64(g, ctr, 1, h));
+  if (stats) {
+    stats->iterations = iters;
+    stats->pull_iterations = pull_iters;
+    stats->reached = h[0];
+    stats->device_ms = ms;
+  }
   return EG_OK;
 }

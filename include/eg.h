@@ -178,6 +178,27 @@ int eg_bfs(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
  * `long components[]`).  rounds may be NULL. */
 int eg_wcc(eg_graph *g, int64_t *labels, int32_t *rounds);
 
+/* sssp_pushpull.c:39-242: single-source shortest paths by frontier
+ * relaxation, push over out-edges or pull over in-edges with the bfs_simple
+ * switch rule (switch_mode = !-r, start_pull = -a 1).  Edge weights: the
+ * reference's CREATE_WEIGHT weights (u % 10) + (v % 10) (init_all.c:661-686)
+ * or unit weights.  dist: root 0, unreached EG_SSSP_UNREACHED
+ * (sssp_pushpull.c:8); both directions add the edge weight (the reference's
+ * push adds 1, :54).  parent (may be NULL): parent[root] = root, unreached
+ * EG_SSSP_UNREACHED, else the first in-neighbour on a shortest path.
+ * Adjacency modes (2/3/4) only. */
+#define EG_SSSP_WEIGHT_REF 0
+#define EG_SSSP_WEIGHT_UNIT 1
+#define EG_SSSP_UNREACHED 0xFFFFFFFEu
+typedef struct {
+  int32_t iterations;      /* relaxation rounds */
+  int32_t pull_iterations; /* rounds run bottom-up */
+  uint64_t reached;        /* vertices with a finite distance */
+  double device_ms;
+} eg_sssp_stats;
+int eg_sssp(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
+            int32_t weights, uint32_t *dist, uint32_t *parent, eg_sssp_stats *stats);
+
 /* ---- batched form (no reference counterpart) ---- */
 /* k BFS traversals from roots[k] in one multi-source sweep.  dist (may be
  * NULL) is k x n, source-major, in the eg_bfs encoding. */

@@ -599,6 +599,77 @@ void ego_ppr(uint32_t n, const uint64_t *in_ptr, const uint32_t *in_idx,
   free(p);
 }
 
+/* sssp_pushpull.c relaxes frontier vertices until no distance changes
+ * (push: :45-62, pull: :75-104, loop: :139-188).  The fixpoint is the
+ * shortest-path distance, whatever order the relaxations take, so the oracle
+ * runs a FIFO worklist Bellman-Ford (each vertex queued at most once at a
+ * time).  The reference's push adds 1 per edge while its pull adds the edge
+ * weight (:54 vs :88); both directions here use the edge weight. */
+uint32_t ego_sssp_weight(uint32_t u, uint32_t v, int kind) {
+  if (kind == 1) return 1u;
+  return (u % 10u) + (v % 10u) + (kind == 2 ? 1u : 0u);
+}
+
+int ego_sssp(uint32_t n, const uint64_t *out_ptr, const uint32_t *out_col, uint32_t root,
+             int kind, uint32_t *dist) {
+  const uint32_t MAXD = UINT32_MAX - 1u;
+  for (uint32_t v = 0; v < n; v++) dist[v] = MAXD;
+  if (root >= n) return 0;
+  uint32_t *queue = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+  unsigned char *inq = (unsigned char *)calloc(n, 1);
+  uint64_t head = 0, tail = 0, pops = 0;
+  dist[root] = 0;
+  queue[tail++ % n] = root;
+  inq[root] = 1;
+  while (head < tail) {
+    const uint32_t u = queue[head++ % n];
+    inq[u] = 0;
+    pops++;
+    for (uint64_t k = out_ptr[u]; k < out_ptr[u + 1]; k++) {
+      const uint32_t v = out_col[k];
+      if (v == u) continue; /* sssp_pushpull.c:56,84 skip self loops */
+      const uint64_t nd = (uint64_t)dist[u] + ego_sssp_weight(u, v, kind);
+      if (nd < dist[v]) {
+        dist[v] = (uint32_t)nd;
+        if (!inq[v]) {
+          inq[v] = 1;
+          queue[tail++ % n] = v;
+        }
+      }
+    }
+  }
+  free(queue);
+  free(inq);
+  return (int)(pops > 0x7fffffff ? 0x7fffffff : pops);
+}
+
+int ego_check_sssp_parents(uint32_t n, const uint64_t *out_ptr, const uint32_t *out_col,
+                           uint32_t root, int kind, const uint32_t *dist,
+                           const uint32_t *parent) {
+  const uint32_t MAXD = UINT32_MAX - 1u;
+  int bad = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    if (v == root) {
+      if (parent[v] != root || dist[v] != 0) bad++;
+      continue;
+    }
+    if (dist[v] == MAXD) {
+      if (parent[v] != MAXD) bad++;
+      continue;
+    }
+    const uint32_t p = parent[v];
+    if (p >= n || dist[p] == MAXD ||
+        (uint64_t)dist[p] + ego_sssp_weight(p, v, kind) != dist[v]) {
+      bad++;
+      continue;
+    }
+    int found = 0;
+    for (uint64_t k = out_ptr[p]; k < out_ptr[p + 1] && !found; k++) found = out_col[k] == v;
+    if (!found) bad++;
+  }
+  return bad;
+}
+
 /* Legality of a BFS tree with the parent encoding of bfs_numa.c:79-83,257:
  * parent[root] = root, unreached = n, and for every other reached v the edge
  * parent[v] -> v exists and dist[parent[v]] = dist[v] - 1.  Returns the number

@@ -88,6 +88,22 @@ int ego_check_bfs_parents(uint32_t n, const uint64_t *out_ptr,
                           const uint32_t *out_col, uint32_t root,
                           const uint32_t *dist, const uint32_t *parent);
 
+/* Single-source shortest paths (sssp_pushpull.c:39-190): dist[root] = 0,
+ * unreached = UINT32_MAX - 1 (sssp_pushpull.c:8).  Edge weights by kind:
+ * 0 = (u % 10) + (v % 10) (CREATE_WEIGHT, init_all.c:661-686), 1 = unit,
+ * 2 = kind 0 plus one (strictly positive; used only to pin the oracle
+ * against library routines that read zero as "no edge").  Returns the number
+ * of relaxation rounds. */
+int ego_sssp(uint32_t n, const uint64_t *out_ptr, const uint32_t *out_col, uint32_t root,
+             int kind, uint32_t *dist);
+uint32_t ego_sssp_weight(uint32_t u, uint32_t v, int kind);
+/* Legality of an SSSP tree: parent[root] = root, unreached = UINT32_MAX - 1,
+ * else the edge parent[v] -> v exists and dist[parent] + w = dist[v].
+ * Returns the number of violations. */
+int ego_check_sssp_parents(uint32_t n, const uint64_t *out_ptr, const uint32_t *out_col,
+                           uint32_t root, int kind, const uint32_t *dist,
+                           const uint32_t *parent);
+
 int ego_num_threads(void);
 
 #ifdef __cplusplus

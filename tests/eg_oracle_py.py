@@ -196,5 +196,28 @@ def check_bfs_parents(n, out_ptr, out_col, root, dist, parent):
     return L().ego_check_bfs_parents(U32(n), P(out_ptr), P(out_col), U32(root), P(dist), P(parent))
 
 
+SSSP_UNREACHED = 0xFFFFFFFE  # UINT32_MAX - 1, sssp_pushpull.c:8
+
+
+def sssp_weight(u, v, kind=0):
+    """Edge weight of the SSSP oracle (numpy-vectorised twin of ego_sssp_weight)."""
+    u = np.asarray(u, np.uint64)
+    v = np.asarray(v, np.uint64)
+    if kind == 1:
+        return np.ones(np.broadcast(u, v).shape, np.uint64)
+    return u % 10 + v % 10 + (1 if kind == 2 else 0)
+
+
+def sssp(n, out_ptr, out_col, root, kind=0):
+    dist = np.empty(n, np.uint32)
+    rounds = L().ego_sssp(U32(n), P(out_ptr), P(out_col), U32(root), I32(kind), P(dist))
+    return dist, rounds
+
+
+def check_sssp_parents(n, out_ptr, out_col, root, kind, dist, parent):
+    return L().ego_check_sssp_parents(U32(n), P(out_ptr), P(out_col), U32(root), I32(kind),
+                                      P(dist), P(parent))
+
+
 def out_degree(n, src):
     return np.bincount(src, minlength=n).astype(np.uint32)

@@ -39,7 +39,8 @@ def test_version_string():
     assert b"gfx950" in eg.lib().eg_version()
 
 
-DRIVERS = ["pagerank_simple", "prgrid_cilk", "spmv", "bfs_simple", "bfsgrid_cilk", "wcc"]
+DRIVERS = ["pagerank_simple", "prgrid_cilk", "spmv", "bfs_simple", "bfsgrid_cilk", "wcc",
+           "sssp_pushpull"]
 
 
 @pytest.mark.parametrize("drv", DRIVERS)

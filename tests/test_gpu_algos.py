@@ -453,3 +453,54 @@ def test_grid_bands_bit_identical(mode):
     deg = np.diff(ptr).astype(np.uint32)
     ref, _, _ = O.pagerank_pull(n, cp, row, deg, 0.85, 20)
     assert np.abs(outs[0][0] - ref).sum() <= PR_L1_TOL
+
+
+# ---------------------------------------------------------------------------
+# SSSP (sssp_pushpull.c)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("scale,undirected,mode", [(10, False, 2), (12, True, 2), (11, False, 4),
+                                                   (11, True, 3)])
+@pytest.mark.parametrize("weights", ["ref", "unit"])
+def test_sssp_vs_oracle(scale, undirected, mode, weights):
+    n = 1 << scale
+    s0, d0 = rmat(scale, seed=60 + scale)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=undirected)
+    kind = eg.SSSP_WEIGHTS[weights]
+    roots = eg.sample_roots(np.diff(ptr), 4, seed=2)
+    with eg.Graph(n, mode, undirected=undirected) as g:
+        g.load_edges(s0, d0)
+        for r in roots:
+            ref, _ = O.sssp(n, ptr, col, int(r), kind)
+            for start_pull, switch in ((False, True), (True, True), (False, False)):
+                dist, par, st = g.sssp(int(r), start_pull, switch, weights)
+                assert np.array_equal(dist, ref), (int(r), start_pull, switch)
+                assert O.check_sssp_parents(n, ptr, col, int(r), kind, dist, par) == 0
+                assert st.reached == np.count_nonzero(ref != O.SSSP_UNREACHED)
+                if start_pull and switch:
+                    assert st.pull_iterations >= 1
+            if weights == "unit":  # unit weights are BFS levels minus one
+                lv, _, _ = O.bfs(n, s.size, ptr, col, cp, row, int(r))
+                assert np.array_equal(np.where(lv == 0, O.SSSP_UNREACHED, lv - 1), ref)
+
+
+def test_sssp_symmetric_flag_and_errors():
+    n = 1 << 10
+    s0, d0 = rmat(10, seed=66)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    with eg.Graph(n, 2, undirected=True, symmetric=True) as g:  # CSR only, pulls over it
+        g.load_edges(s0, d0)
+        ref, _ = O.sssp(n, ptr, col, 3, 0)
+        assert np.array_equal(g.sssp(3, start_pull=True)[0], ref)
+        with pytest.raises(eg.EgError) as e:
+            g.sssp(n)
+        assert e.value.code == eg.EG_ERANGE
+    with eg.Graph(n, 0) as g:
+        g.load_edges(s0, d0)
+        with pytest.raises(eg.EgError) as e:
+            g.sssp(0)
+        assert e.value.code == eg.EG_ENOTSUP
+    with eg.Graph(n, 2) as g:  # isolated root and an empty graph
+        g.load_edges(np.array([5], np.uint32), np.array([6], np.uint32))
+        dist, par, st = g.sssp(9)
+        assert st.reached == 1 and dist[9] == 0 and par[9] == 9
+        assert np.all(np.delete(dist, 9) == O.SSSP_UNREACHED)

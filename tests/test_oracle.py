@@ -377,3 +377,106 @@ def test_wcc_push_directed_is_min_ancestor():
         R = R | ((R.astype(np.int32) @ R.astype(np.int32)) > 0)
     ref = np.array([np.nonzero(R[:, v])[0].min() for v in range(n)], np.int64)
     assert np.array_equal(lab, ref)
+
+
+# ---------------------------------------------------------------------------
+# SSSP (sssp_pushpull.c)
+# ---------------------------------------------------------------------------
+def bellman_ford_np(n, s, d, w, root):
+    """Brute-force Bellman-Ford over the edge list, vectorised, no shared code."""
+    INF = np.uint64(O.SSSP_UNREACHED)
+    dist = np.full(n, INF, np.uint64)
+    dist[root] = 0
+    keep = s != d
+    s, d, w = s[keep].astype(np.int64), d[keep].astype(np.int64), w[keep]
+    for _ in range(n + 1):
+        cand = np.where(dist[s] < INF, dist[s] + w, INF)
+        new = dist.copy()
+        np.minimum.at(new, d, cand)
+        if np.array_equal(new, dist):
+            break
+        dist = new
+    return dist.astype(np.uint32)
+
+
+@pytest.mark.parametrize("name,n,s,d", zoo()[1:], ids=[z[0] for z in zoo()[1:]])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_sssp_vs_independent(name, n, s, d, kind):
+    ptr, col, _ = O.build_csr(n, s, d)
+    roots = sorted({0, int(s[0]) if s.size else 0, n - 1})
+    for r in roots:
+        dist, _ = O.sssp(n, ptr, col, r, kind)
+        w = O.sssp_weight(s, d, kind)
+        assert np.array_equal(dist, bellman_ford_np(n, s, d, w, r)), (name, kind, r)
+        if kind == 0:
+            continue
+        # library shortest paths (weights must be > 0 there: zero means no edge)
+        keep = s != d
+        A = sp.coo_matrix((O.sssp_weight(s[keep], d[keep], kind).astype(np.float64),
+                           (s[keep], d[keep])), shape=(n, n)).tocsr()
+        A.sum_duplicates()
+        A.data = np.minimum.reduceat(
+            O.sssp_weight(*_coo_sorted(s[keep], d[keep]), kind).astype(np.float64),
+            _run_starts(*_coo_sorted(s[keep], d[keep]))) if keep.any() else A.data
+        ref = csg.dijkstra(A, directed=True, indices=r)
+        ref = np.where(np.isinf(ref), O.SSSP_UNREACHED, ref).astype(np.uint64)
+        assert np.array_equal(dist.astype(np.uint64), ref), (name, kind, r)
+
+
+def _coo_sorted(s, d):
+    o = np.lexsort((d, s))
+    return s[o], d[o]
+
+
+def _run_starts(s, d):
+    key = s.astype(np.uint64) << np.uint64(32) | d.astype(np.uint64)
+    return np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+
+
+def test_sssp_closed_forms():
+    # path 0 -> 1 -> ... -> 9: dist[k] = sum of (i%10 + (i+1)%10) for i < k
+    n = 10
+    s = np.arange(9, dtype=np.uint32)
+    d = s + 1
+    ptr, col, _ = O.build_csr(n, s, d)
+    dist, _ = O.sssp(n, ptr, col, 0, 0)
+    expect = np.concatenate([[0], np.cumsum([i % 10 + (i + 1) % 10 for i in range(9)])])
+    assert np.array_equal(dist, expect.astype(np.uint32))
+    # unreachable vertices and an out-of-graph root
+    dist, _ = O.sssp(n, ptr, col, 5, 1)
+    assert np.all(dist[:5] == O.SSSP_UNREACHED) and np.array_equal(dist[5:], np.arange(5))
+    # zero-weight edges (10 -> 20 has weight 0) still give shortest paths
+    s = np.array([10, 20, 10], np.uint32)
+    d = np.array([20, 30, 30], np.uint32)
+    ptr, col, _ = O.build_csr(40, s, d)
+    dist, _ = O.sssp(40, ptr, col, 10, 0)
+    assert dist[20] == 0 and dist[30] == 0
+
+
+def test_sssp_parent_checker_catches_planted_errors():
+    n = 1 << 9
+    s, d = O.gen_rmat(9, 8 << 9, seed=17)
+    ptr, col, _ = O.build_csr(n, s, d)
+    root = int(s[0])
+    dist, _ = O.sssp(n, ptr, col, root, 0)
+    # a legal tree: for every reached v, the first in-neighbour on a shortest path
+    parent = np.full(n, O.SSSP_UNREACHED, np.uint32)
+    parent[root] = root
+    for u in range(n):
+        for k in range(ptr[u], ptr[u + 1]):
+            v = col[k]
+            if v != root and v != u and parent[v] == O.SSSP_UNREACHED and dist[u] != O.SSSP_UNREACHED \
+                    and int(dist[u]) + int(O.sssp_weight(u, v, 0)) == int(dist[v]):
+                parent[v] = u
+    assert O.check_sssp_parents(n, ptr, col, root, 0, dist, parent) == 0
+    reached = np.flatnonzero((dist != O.SSSP_UNREACHED) & (np.arange(n) != root))
+    bad = parent.copy()
+    bad[reached[0]] = root if parent[reached[0]] != root else reached[1]
+    bad2 = parent.copy()
+    bad2[root] = reached[0]
+    dist3 = dist.copy()
+    dist3[reached[2]] += 1
+    errs = [O.check_sssp_parents(n, ptr, col, root, 0, dist, bad),
+            O.check_sssp_parents(n, ptr, col, root, 0, dist, bad2),
+            O.check_sssp_parents(n, ptr, col, root, 0, dist3, parent)]
+    assert all(e > 0 for e in errs), errs
