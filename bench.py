@@ -132,7 +132,8 @@ def main():
     for kv in args.tune:
         p, v = kv.split("=")
         g.tune_set(p, int(v))
-    tuned = {p: g.tune_get(p) for p in ("pull_batch", "pull_hot", "spmm_depth", "ms_batch")}
+    tuned = {p: g.tune_get(p) for p in ("pull_batch", "pull_hot", "pull_persist", "spmm_depth",
+                                        "ms_batch")}
 
     # ---- PageRank: warmup + timed steps ----
     iters = args.iterations

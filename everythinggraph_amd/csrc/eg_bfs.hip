@@ -421,47 +421,143 @@ __global__ __launch_bounds__(kBlock) void k_sssp_push(
   }
 }
 
-// Pull: a wave per bitmap word; every vertex takes the minimum over its
-// in-neighbours in the frontier (sssp_pushpull.c:75-104) and the wave writes
-// its next-frontier word whole.
+// Pull (sssp_pushpull.c:75-104): every vertex takes the minimum over its
+// in-neighbours in the frontier.  SSSP has no early exit, so the work follows
+// the pull plan: chunks of short rows stage their in-lists in LDS (a lane per
+// row of up to kSerialRow entries, a wave per longer row), and hub rows are
+// cut into segments, one workgroup each, whose minima k_sssp_pull_finish
+// merges.  A vertex whose distance drops joins the next frontier.
+constexpr uint32_t kSerialRow = 32;
+constexpr uint64_t kNoCand = ~0ull;
+
+__device__ __forceinline__ uint64_t sssp_cand(uint32_t u, uint32_t v,
+                                              const uint64_t *__restrict__ front,
+                                              const uint32_t *__restrict__ dist, int unit) {
+  if (u == v || !((front[u >> 6] >> (u & 63)) & 1ull)) return kNoCand;
+  return (uint64_t)__atomic_load_n(&dist[u], __ATOMIC_RELAXED) + sssp_w(u, v, unit);
+}
+
+// Lowers dist[v] to best (the row's only writer in a pull pass).
+__device__ __forceinline__ void sssp_lower(uint32_t v, uint64_t best, uint32_t *__restrict__ dist,
+                                           uint64_t *__restrict__ next) {
+  if (best < __atomic_load_n(&dist[v], __ATOMIC_RELAXED)) {
+    __atomic_store_n(&dist[v], (uint32_t)best, __ATOMIC_RELAXED);
+    atomicOr(reinterpret_cast<unsigned long long *>(&next[v >> 6]), 1ull << (v & 63));
+  }
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint64_t y = __shfl_xor(x, o);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
 __global__ __launch_bounds__(kBlock) void k_sssp_pull(
-    uint32_t n, uint32_t nwords, const uint32_t *__restrict__ ptr,
-    const uint32_t *__restrict__ idx, const uint64_t *__restrict__ front,
-    uint64_t *__restrict__ next, uint32_t *__restrict__ dist, int unit) {
+    const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
+    const uint4 *__restrict__ chunk_info, const uint4 *__restrict__ seg_info,
+    const uint32_t *__restrict__ long_rows, uint32_t nseg, uint32_t *__restrict__ seg_min,
+    const uint64_t *__restrict__ front, uint64_t *__restrict__ next,
+    uint32_t *__restrict__ dist, int unit) {
+  constexpr uint32_t kChunk = kPlanChunk, kLds = kPlanChunk + kPlanLongRow;
+  static_assert(kLds <= 65535 && kChunk <= 65535, "u16 LDS offsets");
+  __shared__ uint32_t s_idx[kLds];
+  __shared__ uint16_t s_ptr[kChunk + 1];
+  __shared__ uint16_t s_list[kChunk];
+  __shared__ uint32_t s_nlist;
+  __shared__ uint64_t s_min[kWaves];
+  const uint32_t tid = threadIdx.x;
   const int lane = __lane_id();
-  const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t w = wave0; w < nwords; w += nwaves) {
-    const uint32_t v = w * 64 + lane;
-    bool changed = false;
-    if (v < n) {
-      const uint32_t cur = dist[v];
-      uint64_t best = cur;
-      const uint32_t k1 = ptr[v + 1];
-      for (uint32_t k = ptr[v]; k < k1; k += 4) {
-        uint32_t u[4];
+  const int wave = tid >> 6;
+  if (blockIdx.x < nseg) {
+    const uint4 si = seg_info[blockIdx.x];
+    const uint32_t v = long_rows[si.z];
+    uint64_t best = kNoCand;
+    for (uint32_t base = si.x; base < si.y; base += kBlock * 8) {
+      uint32_t u[8];
 #pragma unroll
-        for (int t = 0; t < 4; t++) u[t] = k + t < k1 ? idx[k + t] : v;
-        uint64_t fw[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) fw[t] = front[u[t] >> 6];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-          if (u[t] != v && ((fw[t] >> (u[t] & 63)) & 1ull)) {
-            const uint64_t nd =
-                (uint64_t)__atomic_load_n(&dist[u[t]], __ATOMIC_RELAXED) + sssp_w(u[t], v, unit);
-            if (nd < best) best = nd;
-          }
-        }
+      for (int q = 0; q < 8; q++) {
+        const uint32_t k = base + tid + q * kBlock;
+        u[q] = k < si.y ? __builtin_nontemporal_load(idx + k) : v;
       }
-      if (best < cur) {
-        dist[v] = (uint32_t)best;
-        changed = true;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const uint64_t c = sssp_cand(u[q], v, front, dist, unit);
+        best = c < best ? c : best;
       }
     }
-    const uint64_t nw = __ballot(changed);
-    if (lane == 0) next[w] = nw;
+    best = wave_min_u64(best);
+    if (lane == 0) s_min[wave] = best;
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t b = s_min[0];
+      for (int q = 1; q < kWaves; q++) b = s_min[q] < b ? s_min[q] : b;
+      seg_min[blockIdx.x] = b < kSsspInf ? (uint32_t)b : kSsspInf;
+    }
+    return;
   }
+  const uint4 ci = chunk_info[blockIdx.x - nseg];
+  const uint32_t r0 = ci.x, r1 = ci.y, k0 = ci.z, k1 = ci.w;
+  const uint32_t nrows = r1 - r0;
+  if (tid == 0) s_nlist = 0;
+  for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = (uint16_t)(ptr[r0 + i] - k0);
+  for (uint32_t base = k0; base < k1; base += kBlock * 8) {
+    uint32_t u[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint32_t k = base + tid + q * kBlock;
+      u[q] = k < k1 ? __builtin_nontemporal_load(idx + k) : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint32_t k = base + tid + q * kBlock;
+      if (k < k1) s_idx[k - k0] = u[q];
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < nrows; i += kBlock) {
+    const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
+    if (rb - ra <= kSerialRow) {
+      const uint32_t v = r0 + i;
+      uint64_t best = kNoCand;
+      for (uint32_t q = ra; q < rb; q++) {
+        const uint64_t c = sssp_cand(s_idx[q], v, front, dist, unit);
+        best = c < best ? c : best;
+      }
+      sssp_lower(v, best, dist, next);
+    } else {
+      s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
+    }
+  }
+  __syncthreads();
+  const uint32_t nlist = s_nlist;
+  for (uint32_t li = wave; li < nlist; li += kWaves) {
+    const uint32_t i = s_list[li];
+    const uint32_t v = r0 + i;
+    const uint32_t rb = s_ptr[i + 1];
+    uint64_t best = kNoCand;
+    for (uint32_t q = s_ptr[i] + lane; q < rb; q += 64) {
+      const uint64_t c = sssp_cand(s_idx[q], v, front, dist, unit);
+      best = c < best ? c : best;
+    }
+    best = wave_min_u64(best);
+    if (lane == 0) sssp_lower(v, best, dist, next);
+  }
+}
+
+// Merges the segment minima of each long row (in any order: min is exact).
+__global__ void k_sssp_pull_finish(const uint32_t *__restrict__ long_rows,
+                                   const uint32_t *__restrict__ seg_base, uint32_t nlong,
+                                   const uint32_t *__restrict__ seg_min,
+                                   uint32_t *__restrict__ dist, uint64_t *__restrict__ next) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlong) return;
+  uint32_t best = kSsspInf;
+  for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++)
+    best = seg_min[s] < best ? seg_min[s] : best;
+  sssp_lower(long_rows[j], best, dist, next);
 }
 
 // Per-workgroup (set bits, their out-degree sum) of a frontier bitmap.
@@ -778,8 +874,16 @@ extern "C" int eg_sssp(eg_graph *g, uint32_t root, int32_t start_pull, int32_t s
   const int unit = weights == EG_SSSP_WEIGHT_UNIT;
   hipStream_t s = g->stream;
   const uint32_t *optr = g->out_ptr, *ocol = g->out_col;
-  const uint32_t *iptr = g->in_ptr ? g->in_ptr : g->out_ptr;
-  const uint32_t *iidx = g->in_ptr ? g->in_idx : g->out_col;
+  const uint32_t *iptr, *iidx;
+  const float *iw;
+  PullPlan *pl;
+  EG_TRY(eg_pull_orientation(g, &iptr, &iidx, &iw, &pl));
+  if (g->sssp_segmin_cap < pl->nseg) {
+    eg_dfree_t(&g->sssp_segmin);
+    g->sssp_segmin_cap = 0;
+    EG_TRY(eg_dalloc_t(&g->sssp_segmin, pl->nseg, "SSSP segment minima"));
+    g->sssp_segmin_cap = pl->nseg;
+  }
   unsigned long long *ctr = reinterpret_cast<unsigned long long *>(g->bfs_ctr);
   uint32_t *qoff = g->bfs_nq;
   void *scan_tmp = nullptr;
@@ -798,7 +902,6 @@ extern "C" int eg_sssp(eg_graph *g, uint32_t root, int32_t start_pull, int32_t s
   uint64_t degsum = (uint64_t)g->h_ctr[1] - g->h_ctr[0];
   bool pull = start_pull != 0;
   const double thr = (double)g->m / 20.0;
-  const uint32_t wblocks = stride_blocks((uint64_t)nwords * 64);
   const uint32_t sblocks = stride_blocks(nwords);
   int iters = 0, pull_iters = 0;
   int rc = EG_OK;
@@ -825,9 +928,16 @@ extern "C" int eg_sssp(eg_graph *g, uint32_t root, int32_t start_pull, int32_t s
     } else {
       pull_iters++;
       EgProf prof(g, "sssp_pull");
-      hipLaunchKernelGGL(k_sssp_pull, dim3(wblocks), dim3(kBlock), 0, s, n, nwords, iptr, iidx,
-                         g->bfs_front, g->bfs_next, g->bfs_dist, unit);
+      hipLaunchKernelGGL(k_sssp_pull, dim3(pl->nseg + pl->nchunks), dim3(kBlock), 0, s, iptr,
+                         iidx, pl->chunk_info, pl->seg_info, pl->long_rows, pl->nseg,
+                         g->sssp_segmin, g->bfs_front, g->bfs_next, g->bfs_dist, unit);
       EG_LAUNCHED();
+      if (pl->nlong) {
+        hipLaunchKernelGGL(k_sssp_pull_finish, dim3(blocks_for(pl->nlong, 256)), dim3(256), 0, s,
+                           pl->long_rows, pl->seg_base, pl->nlong, g->sssp_segmin, g->bfs_dist,
+                           g->bfs_next);
+        EG_LAUNCHED();
+      }
     }
     std::swap(g->bfs_front, g->bfs_next);
     hipLaunchKernelGGL(k_front_stats, dim3(sblocks), dim3(kBlock), 0, s, g->bfs_front, nwords,

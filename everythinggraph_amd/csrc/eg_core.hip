@@ -291,6 +291,8 @@ static void free_algo_state(eg_graph *g) {
   eg_dfree_t(&g->ppr_base);
   eg_dfree_t(&g->ppr_seeds);
   g->ppr_seeds_cap = 0;
+  eg_dfree_t(&g->sssp_segmin);
+  g->sssp_segmin_cap = 0;
 }
 
 extern "C" int eg_graph_free(eg_graph *g) {

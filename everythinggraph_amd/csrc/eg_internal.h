@@ -103,6 +103,9 @@ struct EgTune {
   int graphs = 1;        // batched PPR iterations replayed from a hipGraph (0: eager)
   int grid_band = 1;     // grid pull: source partitions per launch (0: all)
   int ingest_kib = 0;    // record ingest chunk in KiB (0: 64 MiB)
+  int pull_persist = 0;  // k_pull on the degree-ordered copy: 0 = a workgroup per work
+                         // item; 1 or 2 = persistent workgroups per CU holding the
+                         // highest-degree sources' values in LDS
   int pull_hot = 0;      // k_pull on the degree-ordered copy: sources below this id
                          // are cached normally, the rest loaded non-temporally (0: all normal)
 };
@@ -164,6 +167,8 @@ struct eg_graph {
   std::vector<EgProfSlot> prof;
   eg_server *server = nullptr;  // set while a server owns the graph
   EgTune tune;
+  uint32_t *sssp_segmin = nullptr;  // SSSP pull: minimum per long-row segment
+  uint32_t sssp_segmin_cap = 0;
   float *ppr_base = nullptr;     // single-vector PPR: alpha p
   uint32_t *ppr_seeds = nullptr;
   uint32_t ppr_seeds_cap = 0;

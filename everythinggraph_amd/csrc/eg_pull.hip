@@ -23,6 +23,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "eg_internal.h"
 
 namespace {
@@ -143,11 +145,13 @@ struct PrArgs {
 // All index loads are issued before all gathers so that a lane keeps B
 // independent memory requests in flight.  A lane meets its entries in the same
 // ascending order for every B, so B changes no result.
-template <bool WEIGHTED, int B>
+template <bool WEIGHTED, int B, bool LDSHOT>
 __device__ __forceinline__ void gather_batch(const uint32_t *__restrict__ idx,
                                              const float *__restrict__ w,
                                              const float *__restrict__ x, uint32_t k,
-                                             uint32_t kend, uint32_t hot, float (&v)[B]) {
+                                             uint32_t kend, uint32_t hot,
+                                             const float *__restrict__ s_hot, uint32_t hot_n,
+                                             float (&v)[B]) {
   uint32_t u[B];
   float wv[B];
 #pragma unroll
@@ -160,7 +164,11 @@ __device__ __forceinline__ void gather_batch(const uint32_t *__restrict__ idx,
   for (int q = 0; q < B; q++) {
     // In the degree-ordered copy the sources below `hot` take most gathers;
     // colder ones are loaded non-temporally so that they do not evict them.
-    const float xv = u[q] < hot ? x[u[q]] : __builtin_nontemporal_load(x + u[q]);
+    // With LDSHOT the first hot_n entries of x (the highest-degree sources of
+    // the degree-ordered copy) are read from an LDS copy.
+    const float xv = (LDSHOT && u[q] < hot_n) ? s_hot[u[q]]
+                     : u[q] < hot             ? x[u[q]]
+                                              : __builtin_nontemporal_load(x + u[q]);
     v[q] = (k + q * kBlock < kend) ? (WEIGHTED ? xv * wv[q] : xv) : 0.f;
   }
 }
@@ -193,12 +201,17 @@ __device__ __forceinline__ void finish_row(uint32_t r, float sum, float *__restr
 // B: entries per lane per gather pass (16 covers a whole chunk in one pass).
 // Row offsets and the long-row list are u16 in LDS (a chunk has at most
 // kLdsVals entries and kChunk rows), which leaves room for more workgroups.
-template <bool PR, bool WEIGHTED, int B>
+// PERSIST: a workgroup first copies x[0, hot_n) into dynamic LDS and then takes
+// work items blockIdx.x, blockIdx.x + gridDim.x, ...; otherwise one item per
+// workgroup.  Thread-to-entry mapping and every summation order are the same
+// in both forms, so they compute the same bits.
+template <bool PR, bool WEIGHTED, int B, bool PERSIST>
 __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ ptr,
                                                  const uint32_t *__restrict__ idx,
                                                  const float *__restrict__ w,
                                                  const float *__restrict__ x,
-                                                 float *__restrict__ y, PlanDev pl, PrArgs a) {
+                                                 float *__restrict__ y, PlanDev pl, PrArgs a,
+                                                 uint32_t hot_n) {
   static_assert(kLdsVals <= 65535 && kChunk <= 65535, "u16 LDS offsets");
   __shared__ float s_val[kLdsVals];
   __shared__ uint16_t s_ptr[kChunk + 1];
@@ -206,113 +219,123 @@ __global__ __launch_bounds__(kBlock) void k_pull(const uint32_t *__restrict__ pt
   __shared__ uint32_t s_nlist;
   __shared__ float s_ks[kWaves], s_kc[kWaves];
   __shared__ double s_d[kWaves], s_l[kWaves];
+  extern __shared__ float s_hot[];
 
   const uint32_t tid = threadIdx.x;
   const int lane = __lane_id();
   const int wave = tid >> 6;
-
-  if (blockIdx.x < pl.nseg) {
-    // ---- one segment of a long row ----
-    const uint4 si = pl.seg_info[blockIdx.x];
-    EgKahan acc;
-    for (uint32_t base = si.x; base < si.y; base += kBlock * B) {
-      float v[B];
-      gather_batch<WEIGHTED, B>(idx, w, x, base + tid, si.y, a.hot, v);
-#pragma unroll
-      for (int q = 0; q < B; q++) acc.add(v[q]);
-    }
-    eg_wave_kahan_allreduce(acc);
-    if (lane == 0) {
-      s_ks[wave] = acc.s;
-      s_kc[wave] = acc.c;
-    }
+  const uint32_t nitems = pl.nseg + pl.nchunks;
+  if (PERSIST) {
+    for (uint32_t i = tid; i < hot_n; i += kBlock) s_hot[i] = x[i];
     __syncthreads();
-    if (tid == 0) {
-      EgKahan t;
-      t.s = s_ks[0];
-      t.c = s_kc[0];
-      for (int q = 1; q < kWaves; q++) t.merge(s_ks[q], s_kc[q]);
-      pl.seg_part[blockIdx.x] = make_float2(t.s, t.c);
-    }
-    return;
   }
-
-  // ---- a chunk of short rows ----
-  const uint32_t c = blockIdx.x - pl.nseg;
-  const uint4 ci = pl.chunk_info[c];
-  const uint32_t r0 = ci.x, r1 = ci.y, k0 = ci.z, k1 = ci.w;
-  const uint32_t nrows = r1 - r0;
-  if (tid == 0) s_nlist = 0;
-  // row pointers, relative to k0
-  for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = (uint16_t)(ptr[r0 + i] - k0);
-  // prefetch the per-row inputs of this lane's first row
-  uint32_t deg0 = 0;
-  float rold0 = 0.f;
-  if (PR && tid < nrows) {
-    deg0 = a.out_deg[r0 + tid];
-    rold0 = a.rank_old[r0 + tid];
-  }
-  for (uint32_t base = k0; base < k1; base += kBlock * B) {
-    float v[B];
-    gather_batch<WEIGHTED, B>(idx, w, x, base + tid, k1, a.hot, v);
-#pragma unroll
-    for (int q = 0; q < B; q++) {
-      const uint32_t k = base + tid + q * kBlock;
-      if (k < k1) s_val[k - k0] = v[q];
-    }
-  }
-  __syncthreads();
-
-  const float basep = PR ? pr_base(a) : 0.f;
-  double dang = 0.0, l1 = 0.0;
-  for (uint32_t i = tid; i < nrows; i += kBlock) {
-    const uint32_t r = r0 + i;
-    const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
-    if (rb - ra <= kSerialRow) {
+  const uint32_t stride = PERSIST ? gridDim.x : nitems;
+  for (uint32_t item = blockIdx.x; item < nitems; item += stride) {
+    if (item < pl.nseg) {
+      // ---- one segment of a long row ----
+      const uint4 si = pl.seg_info[item];
       EgKahan acc;
-      for (uint32_t q = ra; q < rb; q++) acc.add(s_val[q]);
-      uint32_t deg = deg0;
-      float rold = rold0;
-      if (PR && i != tid) {
-        deg = a.out_deg[r];
-        rold = a.rank_old[r];
+      for (uint32_t base = si.x; base < si.y; base += kBlock * B) {
+        float v[B];
+        gather_batch<WEIGHTED, B, PERSIST>(idx, w, x, base + tid, si.y, a.hot, s_hot, hot_n, v);
+#pragma unroll
+        for (int q = 0; q < B; q++) acc.add(v[q]);
       }
-      finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
-    } else {
-      s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
+      eg_wave_kahan_allreduce(acc);
+      if (lane == 0) {
+        s_ks[wave] = acc.s;
+        s_kc[wave] = acc.c;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        EgKahan t;
+        t.s = s_ks[0];
+        t.c = s_kc[0];
+        for (int q = 1; q < kWaves; q++) t.merge(s_ks[q], s_kc[q]);
+        pl.seg_part[item] = make_float2(t.s, t.c);
+      }
+      if (PERSIST) __syncthreads();
+      continue;
     }
-  }
-  __syncthreads();
-  const uint32_t nlist = s_nlist;
-  for (uint32_t li = wave; li < nlist; li += kWaves) {
-    const uint32_t i = s_list[li];
-    const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
-    EgKahan acc;
-    for (uint32_t q = ra + lane; q < rb; q += 64) acc.add(s_val[q]);
-    eg_wave_kahan_allreduce(acc);
-    if (lane == 0) {
-      const uint32_t r = r0 + i;
-      const uint32_t deg = PR ? a.out_deg[r] : 0u;
-      const float rold = PR ? a.rank_old[r] : 0.f;
-      finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
+
+    // ---- a chunk of short rows ----
+    const uint32_t c = item - pl.nseg;
+    const uint4 ci = pl.chunk_info[c];
+    const uint32_t r0 = ci.x, r1 = ci.y, k0 = ci.z, k1 = ci.w;
+    const uint32_t nrows = r1 - r0;
+    if (tid == 0) s_nlist = 0;
+    // row pointers, relative to k0
+    for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = (uint16_t)(ptr[r0 + i] - k0);
+    // prefetch the per-row inputs of this lane's first row
+    uint32_t deg0 = 0;
+    float rold0 = 0.f;
+    if (PR && tid < nrows) {
+      deg0 = a.out_deg[r0 + tid];
+      rold0 = a.rank_old[r0 + tid];
     }
-  }
-  if (PR) {
-    dang = eg_wave_sum_f64(dang);
-    l1 = eg_wave_sum_f64(l1);
-    if (lane == 0) {
-      s_d[wave] = dang;
-      s_l[wave] = l1;
+    for (uint32_t base = k0; base < k1; base += kBlock * B) {
+      float v[B];
+      gather_batch<WEIGHTED, B, PERSIST>(idx, w, x, base + tid, k1, a.hot, s_hot, hot_n, v);
+#pragma unroll
+      for (int q = 0; q < B; q++) {
+        const uint32_t k = base + tid + q * kBlock;
+        if (k < k1) s_val[k - k0] = v[q];
+      }
     }
     __syncthreads();
-    if (tid == 0) {
-      double td = 0.0, tl = 0.0;
-      for (int q = 0; q < kWaves; q++) {
-        td += s_d[q];
-        tl += s_l[q];
+
+    const float basep = PR ? pr_base(a) : 0.f;
+    double dang = 0.0, l1 = 0.0;
+    for (uint32_t i = tid; i < nrows; i += kBlock) {
+      const uint32_t r = r0 + i;
+      const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
+      if (rb - ra <= kSerialRow) {
+        EgKahan acc;
+        for (uint32_t q = ra; q < rb; q++) acc.add(s_val[q]);
+        uint32_t deg = deg0;
+        float rold = rold0;
+        if (PR && i != tid) {
+          deg = a.out_deg[r];
+          rold = a.rank_old[r];
+        }
+        finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
+      } else {
+        s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
       }
-      pl.chunk_part[c] = make_double2(td, tl);
     }
+    __syncthreads();
+    const uint32_t nlist = s_nlist;
+    for (uint32_t li = wave; li < nlist; li += kWaves) {
+      const uint32_t i = s_list[li];
+      const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
+      EgKahan acc;
+      for (uint32_t q = ra + lane; q < rb; q += 64) acc.add(s_val[q]);
+      eg_wave_kahan_allreduce(acc);
+      if (lane == 0) {
+        const uint32_t r = r0 + i;
+        const uint32_t deg = PR ? a.out_deg[r] : 0u;
+        const float rold = PR ? a.rank_old[r] : 0.f;
+        finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
+      }
+    }
+    if (PR) {
+      dang = eg_wave_sum_f64(dang);
+      l1 = eg_wave_sum_f64(l1);
+      if (lane == 0) {
+        s_d[wave] = dang;
+        s_l[wave] = l1;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double td = 0.0, tl = 0.0;
+        for (int q = 0; q < kWaves; q++) {
+          td += s_d[q];
+          tl += s_l[q];
+        }
+        pl.chunk_part[c] = make_double2(td, tl);
+      }
+    }
+    if (PERSIST) __syncthreads();  // LDS reuse by the next item
   }
 }
 
@@ -480,19 +503,52 @@ int eg_pull_orientation(eg_graph *g, const uint32_t **ptr, const uint32_t **idx,
   return eg_plan_build(g, *pl, *ptr);
 }
 
-template <int B>
-static void launch_k_pull(hipStream_t s, bool pr, uint32_t grid, const uint32_t *ptr,
+template <int B, bool PERSIST>
+static void launch_k_pull(hipStream_t s, bool pr, uint32_t grid, uint32_t lds, const uint32_t *ptr,
                           const uint32_t *idx, const float *w, const float *x, float *y,
-                          const PlanDev &d, const PrArgs &a) {
+                          const PlanDev &d, const PrArgs &a, uint32_t hot_n) {
   if (pr)
-    hipLaunchKernelGGL((k_pull<true, false, B>), dim3(grid), dim3(kBlock), 0, s, ptr, idx, w, x,
-                       y, d, a);
+    hipLaunchKernelGGL((k_pull<true, false, B, PERSIST>), dim3(grid), dim3(kBlock), lds, s, ptr,
+                       idx, w, x, y, d, a, hot_n);
   else if (w)
-    hipLaunchKernelGGL((k_pull<false, true, B>), dim3(grid), dim3(kBlock), 0, s, ptr, idx, w, x,
-                       y, d, a);
+    hipLaunchKernelGGL((k_pull<false, true, B, PERSIST>), dim3(grid), dim3(kBlock), lds, s, ptr,
+                       idx, w, x, y, d, a, hot_n);
   else
-    hipLaunchKernelGGL((k_pull<false, false, B>), dim3(grid), dim3(kBlock), 0, s, ptr, idx, w, x,
-                       y, d, a);
+    hipLaunchKernelGGL((k_pull<false, false, B, PERSIST>), dim3(grid), dim3(kBlock), lds, s, ptr,
+                       idx, w, x, y, d, a, hot_n);
+}
+
+// Persistent form: per_cu workgroups per compute unit, each holding as many
+// leading entries of x in LDS as the unit's 160 KB leave beside the static
+// arrays of its workgroups.
+template <int B>
+static int launch_k_pull_persist(eg_graph *g, hipStream_t s, bool pr, uint32_t nitems,
+                                 const uint32_t *ptr, const uint32_t *idx, const float *w,
+                                 const float *x, float *y, const PlanDev &d, const PrArgs &a) {
+  int cus = 0, lds_max = 0;
+  EG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+  EG_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device));
+  const void *fn = pr ? reinterpret_cast<const void *>(&k_pull<true, false, B, true>)
+                   : w ? reinterpret_cast<const void *>(&k_pull<false, true, B, true>)
+                       : reinterpret_cast<const void *>(&k_pull<false, false, B, true>);
+  hipFuncAttributes attr;
+  EG_HIP(hipFuncGetAttributes(&attr, fn));
+  const uint32_t per_cu = (uint32_t)g->tune.pull_persist;
+  const uint64_t unit_lds = 160u * 1024u;
+  const uint64_t per_wg = unit_lds / per_cu;
+  if (per_wg <= attr.sharedSizeBytes + 4096) return eg_fail(EG_EINVAL, "no LDS left for the hot set");
+  uint64_t dyn = std::min<uint64_t>(per_wg - attr.sharedSizeBytes - 1024,
+                                    (uint64_t)lds_max - attr.sharedSizeBytes);
+  const uint32_t hot_n = (uint32_t)std::min<uint64_t>(g->n, dyn / 4);
+  dyn = (uint64_t)hot_n * 4;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) {
+    (void)hipGetLastError();
+    return eg_fail(EG_EINVAL, "the device refuses %llu bytes of dynamic LDS",
+                   (unsigned long long)dyn);
+  }
+  const uint32_t grid = std::min<uint32_t>(nitems, (uint32_t)cus * per_cu);
+  launch_k_pull<B, true>(s, pr, grid, (uint32_t)dyn, ptr, idx, w, x, y, d, a, hot_n);
+  return EG_OK;
 }
 
 static int launch_pull(eg_graph *g, bool pr, const uint32_t *ptr, const uint32_t *idx,
@@ -506,10 +562,16 @@ static int launch_pull(eg_graph *g, bool pr, const uint32_t *ptr, const uint32_t
   const uint32_t grid = pl.nseg + pl.nchunks;
   if (grid) {
     EgProf prof(g, pr ? "pr_pull" : "spmv_pull", s);
-    if (g->tune.pull_batch == 8)
-      launch_k_pull<8>(s, pr, grid, ptr, idx, w, x, y, d, a);
-    else
-      launch_k_pull<16>(s, pr, grid, ptr, idx, w, x, y, d, a);
+    if (ordered && g->tune.pull_persist > 0) {
+      if (g->tune.pull_batch == 8)
+        EG_TRY(launch_k_pull_persist<8>(g, s, pr, grid, ptr, idx, w, x, y, d, a));
+      else
+        EG_TRY(launch_k_pull_persist<16>(g, s, pr, grid, ptr, idx, w, x, y, d, a));
+    } else if (g->tune.pull_batch == 8) {
+      launch_k_pull<8, false>(s, pr, grid, 0, ptr, idx, w, x, y, d, a, 0);
+    } else {
+      launch_k_pull<16, false>(s, pr, grid, 0, ptr, idx, w, x, y, d, a, 0);
+    }
     EG_LAUNCHED();
   }
   if (pr || pl.nlong) {

@@ -7,6 +7,9 @@
 //                   lane per gather pass, 8 or 16;
 //  * spmm_depth  -- k_ppr_spmm: row gathers in flight per wave, 8 or 16;
 //  * ms_batch    -- k_ms_pull: entries per lane per index pass, 8 or 16;
+//  * pull_persist -- k_pull on the degree-ordered copy: 1 or 2 persistent
+//                   workgroups per CU that keep the leading (highest-degree)
+//                   entries of the gathered vector in LDS; 0 = off;
 //  * pull_hot    -- k_pull on the degree-ordered copy: gathers of sources with
 //                   new id >= pull_hot bypass normal caching (non-temporal
 //                   loads) so that the hottest sources stay in L2; 0 = off;
@@ -31,6 +34,7 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "spmm_depth")) return &g->tune.spmm_depth;
   if (!strcmp(name, "ms_batch")) return &g->tune.ms_batch;
   if (!strcmp(name, "pull_hot")) return &g->tune.pull_hot;
+  if (!strcmp(name, "pull_persist")) return &g->tune.pull_persist;
   if (!strcmp(name, "graphs")) return &g->tune.graphs;
   if (!strcmp(name, "grid_band")) return &g->tune.grid_band;
   if (!strcmp(name, "ingest_kib")) return &g->tune.ingest_kib;
@@ -73,6 +77,8 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
   if (!slot) return eg_fail(EG_EINVAL, "unknown tuning parameter '%s'", param);
   if (!strcmp(param, "pull_hot")) {
     if (value < 0) return eg_fail(EG_EINVAL, "pull_hot takes a vertex count >= 0");
+  } else if (!strcmp(param, "pull_persist")) {
+    if (value < 0 || value > 2) return eg_fail(EG_EINVAL, "pull_persist takes 0, 1 or 2");
   } else if (!strcmp(param, "ingest_kib")) {
     if (value < 0) return eg_fail(EG_EINVAL, "ingest_kib takes a size >= 0");
   } else if (!strcmp(param, "grid_band")) {
@@ -137,6 +143,24 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
         EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); },
                          &ms));
         res.push_back({"pull_hot", h, ms / o.iterations});
+      }
+      // keep the best limit, then try persistent workgroups with an LDS hot set
+      double best = 1e300;
+      for (const Result &r : res)
+        if (!strcmp(r.param, "pull_hot") && r.ms < best) {
+          best = r.ms;
+          g->tune.pull_hot = r.value;
+        }
+      for (int pp = 0; pp <= 2; pp++) {
+        g->tune.pull_persist = pp;
+        double ms;
+        const int rc = time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); },
+                                 &ms);
+        if (rc != EG_OK) {
+          if (pp == 0 || rc == EG_EDEVICE) return rc;
+          continue;  // this device cannot give the variant its LDS: not a candidate
+        }
+        res.push_back({"pull_persist", pp, ms / o.iterations});
       }
     }
   }

@@ -222,7 +222,9 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * Tunable shapes: "pull_batch" (pull PageRank / SpMV / PPR gather depth),
  * "spmm_depth" (batched PPR row gathers in flight), "ms_batch" (multi-source
  * BFS index pass depth), each 8 or 16; "pull_hot" (sources cached normally in
- * the degree-ordered pull, a vertex count, 0 = all); "grid_band" (grid source
+ * the degree-ordered pull, a vertex count, 0 = all); "pull_persist" (0, or 1-2
+ * persistent pull workgroups per CU keeping the highest-degree sources' values
+ * in LDS); "grid_band" (grid source
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
  * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
  * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant

@@ -371,6 +371,7 @@ def test_tuning_variants_bit_identical():
                 g.tune_set(p, v)
                 assert g.tune_get(p) == v
             g.tune_set("pull_hot", 0 if v == 8 else n // 16)
+            g.tune_set("pull_persist", 0 if v == 8 else 2)
             outs[v] = (g.pagerank(0.85, 20)[0], g.ppr_batch(seeds, 0.15, 20)[0],
                        g.bfs_batch(roots)[0], g.ppr([5, 6], 0.15, tol=0.0, max_iter=25)[0])
         with pytest.raises(eg.EgError) as e:
@@ -381,6 +382,11 @@ def test_tuning_variants_bit_identical():
         assert e.value.code == eg.EG_EINVAL
     for a, b in zip(outs[8], outs[16]):
         assert np.array_equal(a, b)
+    with eg.Graph(n, 2, undirected=True) as g:  # one persistent workgroup per CU
+        g.load_edges(s0, d0)
+        g.tune_set("pull_persist", 1)
+        assert np.array_equal(g.pagerank(0.85, 20)[0], outs[8][0])
+        assert np.array_equal(g.ppr([5, 6], 0.15, tol=0.0, max_iter=25)[0], outs[8][3])
     for i, r in enumerate(roots):
         ref, _, _ = O.bfs(n, s0.size * 2, ptr, col, cp, row, int(r))
         assert np.array_equal(outs[16][2][i], ref)
@@ -393,13 +399,13 @@ def test_autotune_reports_and_keeps_fastest():
         g.load_edges(s0, d0)
         rep = g.autotune()
         params = {r[0] for r in rep}
-        assert params == {"pull_batch", "pull_hot", "spmm_depth", "ms_batch"}
+        assert params == {"pull_batch", "pull_hot", "pull_persist", "spmm_depth", "ms_batch"}
         hot = sorted(r[1] for r in rep if r[0] == "pull_hot")
         assert hot == [0, n // 64, n // 16, n // 4]
-        for p in ("pull_batch", "spmm_depth", "ms_batch", "pull_hot"):
+        for p in ("pull_batch", "spmm_depth", "ms_batch", "pull_hot", "pull_persist"):
             rows = [r for r in rep if r[0] == p]
             assert all(r[2] > 0 for r in rows)
-            if p != "pull_hot":
+            if p in ("pull_batch", "spmm_depth", "ms_batch"):
                 assert sorted(r[1] for r in rows) == [8, 16]
             best = min(rows, key=lambda r: r[2])
             chosen = [r for r in rows if r[3]]
@@ -481,6 +487,27 @@ def test_sssp_vs_oracle(scale, undirected, mode, weights):
             if weights == "unit":  # unit weights are BFS levels minus one
                 lv, _, _ = O.bfs(n, s.size, ptr, col, cp, row, int(r))
                 assert np.array_equal(np.where(lv == 0, O.SSSP_UNREACHED, lv - 1), ref)
+
+
+@pytest.mark.parametrize("weights", ["ref", "unit"])
+def test_sssp_hub_rows(weights):
+    """In-rows long enough to be cut into segments (9000 and 3000 entries) and
+    rows reduced by a whole wave (500 entries), on every direction schedule."""
+    n = 4096
+    rng = np.random.default_rng(61)
+    s = rng.integers(0, n, 80000).astype(np.uint32)
+    d = rng.integers(0, n, 80000).astype(np.uint32)
+    d[:9000], d[9000:12000], d[12000:12500] = 7, 100, 200
+    _, _, _, ptr, col, _, _, _, _ = oracle_adj(n, s, d)
+    kind = eg.SSSP_WEIGHTS[weights]
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s, d)
+        for r in (int(s[0]), int(s[9500]), 7):
+            ref, _ = O.sssp(n, ptr, col, r, kind)
+            for start_pull, switch in ((True, False), (True, True), (False, True)):
+                dist, par, st = g.sssp(r, start_pull, switch, weights)
+                assert np.array_equal(dist, ref), (r, start_pull, switch)
+                assert O.check_sssp_parents(n, ptr, col, r, kind, dist, par) == 0
 
 
 def test_sssp_symmetric_flag_and_errors():
