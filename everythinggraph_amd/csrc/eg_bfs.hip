@@ -82,146 +82,6 @@ __global__ void k_frontier_degrees(const uint32_t *__restrict__ q, uint32_t qn,
   deg[i] = ptr[u + 1] - ptr[u];
 }
 
-// Counters (u64 words of ctr): [0] next frontier size, [1] its out-degree
-// sum, [2] next queue length.
-__global__ __launch_bounds__(kBlock) void k_bfs_push(
-    const uint32_t *__restrict__ q, uint32_t qn, const uint32_t *__restrict__ qoff,
-    const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ col,
-    uint64_t *__restrict__ visited, uint32_t *__restrict__ dist, uint32_t *__restrict__ parent,
-    uint32_t newdist, uint32_t *__restrict__ nq, unsigned long long *__restrict__ ctr) {
-  __shared__ uint32_t s_off[kEdgesPerBlock + 1];
-  __shared__ uint32_t s_u[kEdgesPerBlock + 1];
-  __shared__ uint32_t s_wq[kEdgesPerBlock];
-  __shared__ uint32_t s_first, s_cnt, s_nq, s_base;
-  __shared__ unsigned long long s_tmp[kWaves];
-  const uint32_t e0 = blockIdx.x * kEdgesPerBlock;
-  const uint32_t total = qoff[qn];
-  const uint32_t e1 = e0 + kEdgesPerBlock < total ? e0 + kEdgesPerBlock : total;
-  if (threadIdx.x == 0) {
-    uint32_t lo = 0, hi = qn;  // last i with qoff[i] <= e0
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (qoff[mid] <= e0) lo = mid; else hi = mid;
-    }
-    uint32_t a = lo, b = qn;  // first i > lo with qoff[i] >= e1
-    while (a < b) {
-      const uint32_t mid = a + (b - a) / 2;
-      if (qoff[mid] < e1) a = mid + 1; else b = mid;
-    }
-    s_first = lo;
-    s_cnt = a - lo;
-    s_nq = 0;
-  }
-  __syncthreads();
-  const uint32_t first = s_first;
-  const uint32_t cnt = s_cnt <= kEdgesPerBlock + 1 ? s_cnt : kEdgesPerBlock + 1;
-  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
-    s_off[i] = qoff[first + i];
-    s_u[i] = q[first + i];
-  }
-  __syncthreads();
-  unsigned long long found = 0, degsum = 0;
-  for (uint32_t base = e0; base < e1; base += kBlock) {
-    const uint32_t e = base + threadIdx.x;
-    if (e < e1) {
-      uint32_t lo = 0, hi = cnt;  // last j with s_off[j] <= e
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (s_off[mid] <= e) lo = mid; else hi = mid;
-      }
-      const uint32_t u = s_u[lo];
-      const uint32_t v = col[ptr[u] + (e - s_off[lo])];
-      const uint64_t bit = 1ull << (v & 63);
-      if (!(__atomic_load_n(&visited[v >> 6], __ATOMIC_RELAXED) & bit)) {
-        const uint64_t old = atomicOr(reinterpret_cast<unsigned long long *>(&visited[v >> 6]),
-                                      (unsigned long long)bit);
-        if (!(old & bit)) {
-          dist[v] = newdist;
-          if (parent) parent[v] = u;
-          const uint32_t d = ptr[v + 1] - ptr[v];
-          found++;
-          degsum += d;
-          if (d) s_wq[atomicAdd(&s_nq, 1u)] = v;
-        }
-      }
-    }
-  }
-  found = block_sum_u64(found, s_tmp);
-  degsum = block_sum_u64(degsum, s_tmp);
-  if (threadIdx.x == 0) {
-    if (s_nq) s_base = (uint32_t)atomicAdd(&ctr[2], (unsigned long long)s_nq);
-    if (found) {
-      atomicAdd(&ctr[0], found);
-      atomicAdd(&ctr[1], degsum);
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < s_nq; i += kBlock) nq[s_base + i] = s_wq[i];
-}
-
-// Frontier queue -> frontier bitmap (before a pull level).
-__global__ void k_queue_to_bitmap(const uint32_t *__restrict__ q, uint32_t qn,
-                                  uint64_t *__restrict__ bm) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= qn) return;
-  const uint32_t v = q[i];
-  atomicOr(reinterpret_cast<unsigned long long *>(&bm[v >> 6]), 1ull << (v & 63));
-}
-
-// Pull step: a wave per visited word, grid-stride; per-workgroup partials of
-// (found, out-degree sum) in part[2*block].
-__global__ __launch_bounds__(kBlock) void k_bfs_pull(
-    uint32_t n, uint32_t nwords, const uint32_t *__restrict__ ptr,
-    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ out_ptr,
-    const uint64_t *__restrict__ front, uint64_t *__restrict__ next,
-    uint64_t *__restrict__ visited, uint32_t *__restrict__ dist, uint32_t *__restrict__ parent,
-    uint32_t newdist, unsigned long long *__restrict__ part) {
-  __shared__ unsigned long long s_tmp[kWaves];
-  const int lane = __lane_id();
-  const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  unsigned long long found = 0, degsum = 0;
-  for (uint32_t w = wave0; w < nwords; w += nwaves) {
-    const uint64_t vis = visited[w];
-    const uint32_t v = w * 64 + lane;
-    bool hit = false;
-    if (v < n && !((vis >> lane) & 1ull)) {
-      const uint32_t k1 = ptr[v + 1];
-      for (uint32_t k = ptr[v]; k < k1 && !hit; k += 4) {
-        uint32_t u[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) u[t] = k + t < k1 ? idx[k + t] : idx[k];
-        uint64_t fw[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) fw[t] = front[u[t] >> 6];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-          if (!hit && ((fw[t] >> (u[t] & 63)) & 1ull)) {
-            hit = true;
-            dist[v] = newdist;
-            if (parent) parent[v] = u[t];
-          }
-        }
-      }
-      if (hit) {
-        found++;
-        degsum += out_ptr[v + 1] - out_ptr[v];
-      }
-    }
-    const uint64_t nw = __ballot(hit);
-    if (lane == 0) {
-      next[w] = nw;
-      if (nw) visited[w] = vis | nw;
-    }
-  }
-  found = block_sum_u64(found, s_tmp);
-  degsum = block_sum_u64(degsum, s_tmp);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = found;
-    part[2 * blockIdx.x + 1] = degsum;
-  }
-}
-
 // out[c] = sum over p of part[p * ncols + c] (one workgroup, fixed order).
 __global__ __launch_bounds__(kBlock) void k_sum_parts(const unsigned long long *__restrict__ part,
                                                       uint32_t nparts, uint32_t ncols,
@@ -618,6 +478,350 @@ __global__ __launch_bounds__(kBlock) void k_sssp_finish(uint32_t n, uint32_t roo
   if (threadIdx.x == 0) part[blockIdx.x] = reached;
 }
 
+// ---------------------------------------------------------------------------
+// Device-driven adjacency BFS (bfs_simple.c:136-229): one launch per level and
+// no host round trip between levels.  Every workgroup of a level reads the
+// frontier's statistics from the control words and applies the switch rule
+// (bfs_simple.c:191-197) itself, so the direction is decided on the device.
+//
+//  * Push walks the frontier queue's edges, kLvlEdges per block (owners by a
+//    wave-wide 64-ary search of the queue's edge offsets, staged in LDS).
+//  * Pull walks tiles of kTileWords bitmap words (grid-stride over tiles), a
+//    wave per word: a lane scans the first kPullSerial entries of its
+//    vertex's in-list and the whole wave the rest of a longer list, stopping
+//    at the first in-neighbour found in the frontier.
+//  * Both emit the next frontier twice: as a bitmap (push: atomicOr into a
+//    buffer cleared one level earlier; pull: whole words) and as a queue of
+//    its vertices with out-edges plus their edge offsets.  A workgroup
+//    collects the winners of a block (push) or tile (pull) in LDS and
+//    reserves queue slots and edge offsets with ONE 64-bit atomicAdd of
+//    (count << 32 | degree sum), so the offsets increase in queue order
+//    without a scan.
+//  * Three frontier bitmaps rotate: level L reads F[L % 3], writes
+//    F[(L + 1) % 3] and clears F[(L + 2) % 3].
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLvlEdges = 2048;   // push: frontier edges per block
+constexpr uint32_t kTileWords = 32;    // pull: bitmap words per tile (2048 vertices)
+constexpr uint32_t kEmitCap = 2048;    // winners per block or tile
+constexpr uint32_t kPullSerial = 64;   // pull: entries a lane scans before its wave helps
+constexpr int kCtlWords = 12;
+// Control words (u64): [2s] = queue length << 32 | out-degree sum and
+// [2s + 1] = vertex count of the frontier of the level L with L % 3 == s;
+// [6 + s] = direction of that level (1 = pull); [9] = last active level;
+// [10] = pull levels; [11] = done.
+
+struct LevelArgs {
+  uint32_t n, nwords, ntiles;
+  uint64_t m;
+  int start_pull, switch_mode;
+  const uint32_t *optr, *ocol, *iptr, *iidx;
+  uint64_t *visited;
+  uint64_t *F[3];
+  uint32_t *dist, *parent;
+  uint32_t *q[2], *qoff[2];
+  unsigned long long *ctl;
+};
+
+__global__ void k_bfs_level_init(LevelArgs a, uint32_t root) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n) {
+    a.dist[i] = i == root ? 1u : 0u;
+    if (a.parent) a.parent[i] = i == root ? root : a.n;
+  }
+  if (i < a.nwords) {
+    const uint64_t w = (root >> 6) == i ? (1ull << (root & 63)) : 0ull;
+    a.visited[i] = w;
+    a.F[0][i] = 0ull;
+    a.F[1][i] = w;
+    a.F[2][i] = 0ull;
+  }
+  if (i == 0) {
+    const uint32_t d = a.optr[root + 1] - a.optr[root];
+    for (int k = 0; k < kCtlWords; k++) a.ctl[k] = 0ull;
+    a.q[1][0] = root;
+    a.qoff[1][0] = 0u;
+    a.ctl[2] = d ? (1ull << 32) | d : 0ull;  // level 1: slot 1
+    a.ctl[3] = 1ull;
+  }
+}
+
+// Smallest i in [lo, hi) with a[i] >= key (hi if none); a is non-decreasing.
+// Called by a whole wave; each pass narrows the range 64-fold.
+__device__ uint32_t wave_lower_bound(const uint32_t *__restrict__ a, uint32_t lo, uint32_t hi,
+                                     uint32_t key) {
+  const int lane = __lane_id();
+  while (hi > lo) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint64_t p = (uint64_t)lo + (uint64_t)lane * step;
+    const bool ge = p >= hi || a[p] >= key;
+    const uint64_t mask = __ballot(ge);
+    const uint32_t j = mask ? (uint32_t)(__ffsll((unsigned long long)mask) - 1) : 64u;
+    if (j == 0) return lo;
+    const uint64_t nhi = (uint64_t)lo + (uint64_t)j * step;
+    lo = lo + (j - 1) * step + 1;
+    hi = nhi < hi ? (uint32_t)nhi : hi;
+  }
+  return lo;
+}
+
+// Appends v for the lanes with put to the workgroup's buffer (one LDS atomic
+// per wave).
+__device__ __forceinline__ void wg_append(bool put, uint32_t v, uint32_t *s_w, uint32_t *s_n) {
+  const uint64_t mask = __ballot(put);
+  if (!mask) return;
+  uint32_t base = 0;
+  if (__lane_id() == 0) base = atomicAdd(s_n, (uint32_t)__popcll(mask));
+  base = __shfl(base, 0);
+  if (put) s_w[base + __popcll(mask & ((1ull << __lane_id()) - 1))] = v;
+}
+
+// Emits the workgroup's buffer to the next queue with one 64-bit atomicAdd.
+// Called by the whole workgroup; leaves the buffer empty.
+__device__ void wg_emit(const uint32_t *s_w, uint32_t *s_n, uint32_t *s_sum,
+                        unsigned long long *s_base, const uint32_t *__restrict__ optr,
+                        unsigned long long *slot, uint32_t *__restrict__ q_out,
+                        uint32_t *__restrict__ qoff_out) {
+  __syncthreads();
+  const uint32_t cnt = *s_n;
+  __syncthreads();  // every thread has the count before it is reset
+  if (cnt == 0) return;
+  const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  if (tid == 0) *s_n = 0;
+  const uint32_t per = (cnt + kBlock - 1) / kBlock;
+  const uint32_t b0 = tid * per < cnt ? tid * per : cnt;
+  const uint32_t b1 = b0 + per < cnt ? b0 + per : cnt;
+  uint32_t deg[kEmitCap / kBlock];
+  uint32_t mine = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kEmitCap / kBlock; j++) {
+    const uint32_t i = b0 + j;
+    deg[j] = 0;
+    if (i < b1) {
+      const uint32_t v = s_w[i];
+      deg[j] = optr[v + 1] - optr[v];
+      mine += deg[j];
+    }
+  }
+  uint32_t inc = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) s_sum[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    before += w < wave ? s_sum[w] : 0u;
+    total += s_sum[w];
+  }
+  if (tid == 0) *s_base = atomicAdd(slot, ((unsigned long long)cnt << 32) | total);
+  __syncthreads();
+  const unsigned long long base = *s_base;
+  const uint32_t qb = (uint32_t)(base >> 32);
+  uint32_t off = (uint32_t)base + before + inc - mine;
+#pragma unroll
+  for (uint32_t j = 0; j < kEmitCap / kBlock; j++) {
+    const uint32_t i = b0 + j;
+    if (i < b1) {
+      q_out[qb + i] = s_w[i];
+      qoff_out[qb + i] = off;
+      off += deg[j];
+    }
+  }
+  __syncthreads();  // the buffer is read before the next appends overwrite it
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfs_level(LevelArgs a, uint32_t L) {
+  __shared__ uint32_t s_off[kLvlEdges + 1];
+  __shared__ uint32_t s_u[kLvlEdges + 1];
+  __shared__ uint32_t s_w[kEmitCap];
+  __shared__ uint32_t s_n, s_first, s_cnt, s_sum[kWaves];
+  __shared__ unsigned long long s_base;
+  __shared__ unsigned long long s_tmp[kWaves];
+  unsigned long long *ctl = a.ctl;
+  if (ctl[11]) return;  // an earlier level found the frontier empty
+  const uint32_t s0 = L % 3, s1 = (L + 1) % 3, s2 = (L + 2) % 3;
+  const unsigned long long packed = ctl[2 * s0];
+  const unsigned long long count = ctl[2 * s0 + 1];
+  if (count == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[11] = 1ull;
+    return;
+  }
+  const uint32_t qn = (uint32_t)(packed >> 32);
+  const uint32_t total = (uint32_t)packed;
+  bool pull;
+  if (L == 1) {
+    pull = a.start_pull != 0;
+  } else {
+    const bool prev = ctl[6 + (L - 1) % 3] != 0;
+    const bool heavy = 20ull * (count + (unsigned long long)total) > a.m;  // load > m/20
+    pull = (a.switch_mode && !prev && heavy) || (prev && heavy);
+  }
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  if (blockIdx.x == 0 && tid == 0) {
+    ctl[6 + s0] = pull ? 1ull : 0ull;
+    ctl[9] = L;
+    if (pull) ctl[10] += 1ull;
+    ctl[2 * s2] = 0ull;  // statistics slot of level L + 2
+    ctl[2 * s2 + 1] = 0ull;
+  }
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  const uint32_t newdist = L + 1;
+  const uint64_t *F_in = a.F[s0];
+  uint64_t *F_out = a.F[s1];
+  uint64_t *F_clr = a.F[s2];
+  uint32_t *q_out = a.q[(L + 1) & 1], *qoff_out = a.qoff[(L + 1) & 1];
+  unsigned long long *slot = &ctl[2 * s1];
+  for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const uint32_t w = t * kTileWords + (uint32_t)tid;
+    if (tid < (int)kTileWords && w < a.nwords) F_clr[w] = 0ull;
+  }
+  unsigned long long found = 0;
+  if (!pull) {
+    const uint32_t *q = a.q[L & 1], *qoff = a.qoff[L & 1];
+    const uint32_t nblk = (total + kLvlEdges - 1) / kLvlEdges;
+    for (uint32_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+      const uint32_t e0 = blk * kLvlEdges;
+      const uint32_t e1 = e0 + kLvlEdges < total ? e0 + kLvlEdges : total;
+      __syncthreads();
+      if (wave == 0) {
+        // owners: the last entry with offset <= e0 up to the first with offset >= e1
+        const uint32_t first = wave_lower_bound(qoff, 0, qn, e0 + 1) - 1;
+        const uint32_t last = wave_lower_bound(qoff, first, qn, e1);
+        if (lane == 0) {
+          s_first = first;
+          s_cnt = last - first;
+        }
+      }
+      __syncthreads();
+      const uint32_t first = s_first;
+      const uint32_t cnt = s_cnt <= kLvlEdges + 1 ? s_cnt : kLvlEdges + 1;
+      for (uint32_t i = tid; i < cnt; i += kBlock) {
+        s_off[i] = qoff[first + i];
+        s_u[i] = q[first + i];
+      }
+      __syncthreads();
+      for (uint32_t base = e0; base < e0 + kLvlEdges; base += kBlock) {
+        const uint32_t e = base + tid;
+        bool put = false;
+        uint32_t v = 0;
+        if (e < e1) {
+          uint32_t lo = 0, hi = cnt;  // last j with s_off[j] <= e
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (s_off[mid] <= e) lo = mid; else hi = mid;
+          }
+          const uint32_t u = s_u[lo];
+          v = a.ocol[a.optr[u] + (e - s_off[lo])];
+          const uint64_t bit = 1ull << (v & 63);
+          if (!(__atomic_load_n(&a.visited[v >> 6], __ATOMIC_RELAXED) & bit)) {
+            const uint64_t old = atomicOr(
+                reinterpret_cast<unsigned long long *>(&a.visited[v >> 6]), (unsigned long long)bit);
+            if (!(old & bit)) {
+              a.dist[v] = newdist;
+              if (a.parent) a.parent[v] = u;
+              atomicOr(reinterpret_cast<unsigned long long *>(&F_out[v >> 6]),
+                       (unsigned long long)bit);
+              found++;
+              put = a.optr[v + 1] != a.optr[v];
+            }
+          }
+        }
+        wg_append(put, v, s_w, &s_n);
+      }
+      wg_emit(s_w, &s_n, s_sum, &s_base, a.optr, slot, q_out, qoff_out);
+    }
+  } else {
+    for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+      for (uint32_t j = (uint32_t)wave; j < kTileWords; j += kWaves) {
+        const uint32_t w = t * kTileWords + j;
+        if (w >= a.nwords) break;  // uniform over the wave
+        const uint64_t vis = a.visited[w];
+        const uint32_t v = w * 64 + lane;
+        bool hit = false, longrow = false;
+        uint32_t par = 0;
+        if (v < a.n && !((vis >> lane) & 1ull)) {
+          const uint32_t k0 = a.iptr[v], k1 = a.iptr[v + 1];
+          const uint32_t ke = k1 - k0 > kPullSerial ? k0 + kPullSerial : k1;
+          for (uint32_t k = k0; k < ke && !hit; k += 4) {
+            uint32_t u[4];
+#pragma unroll
+            for (int x = 0; x < 4; x++) u[x] = k + x < ke ? a.iidx[k + x] : a.iidx[k];
+            uint64_t fw[4];
+#pragma unroll
+            for (int x = 0; x < 4; x++) fw[x] = F_in[u[x] >> 6];
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+              if (!hit && ((fw[x] >> (u[x] & 63)) & 1ull)) {
+                hit = true;
+                par = u[x];
+              }
+            }
+          }
+          longrow = !hit && ke < k1;
+        }
+        // The rest of an in-list longer than kPullSerial without a hit in its
+        // first kPullSerial entries: the whole wave scans it, 256 entries per
+        // pass, and stops at the first entry (in list order) in the frontier.
+        uint64_t lm = __ballot(longrow);
+        while (lm) {
+          const int b = __ffsll((unsigned long long)lm) - 1;
+          lm &= lm - 1;
+          const uint32_t vv = w * 64 + (uint32_t)b;
+          const uint32_t k0 = a.iptr[vv] + kPullSerial, k1 = a.iptr[vv + 1];
+          bool got = false;
+          uint32_t gp = 0;
+          for (uint32_t base = k0; base < k1 && !got; base += 256) {
+            uint32_t u[4];
+            bool h[4];
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+              const uint32_t k = base + (uint32_t)x * 64 + (uint32_t)lane;
+              u[x] = k < k1 ? a.iidx[k] : 0u;
+            }
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+              const uint32_t k = base + (uint32_t)x * 64 + (uint32_t)lane;
+              h[x] = k < k1 && ((F_in[u[x] >> 6] >> (u[x] & 63)) & 1ull);
+            }
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+              const uint64_t hm = __ballot(h[x]);
+              if (!got && hm) {
+                got = true;
+                gp = __shfl(u[x], __ffsll((unsigned long long)hm) - 1);
+              }
+            }
+          }
+          if (got && lane == b) {
+            hit = true;
+            par = gp;
+          }
+        }
+        if (hit) {
+          a.dist[v] = newdist;
+          if (a.parent) a.parent[v] = par;
+          found++;
+        }
+        const uint64_t nw = __ballot(hit);
+        if (lane == 0) {
+          F_out[w] = nw;
+          if (nw) a.visited[w] = vis | nw;
+        }
+        wg_append(hit && a.optr[v + 1] != a.optr[v], v, s_w, &s_n);
+      }
+      wg_emit(s_w, &s_n, s_sum, &s_base, a.optr, slot, q_out, qoff_out);
+    }
+  }
+  found = block_sum_u64(found, s_tmp);
+  if (tid == 0 && found) atomicAdd(&ctl[2 * s1 + 1], found);
+}
+
 }  // namespace
 
 static int bfs_state(eg_graph *g) {
@@ -664,105 +868,69 @@ static int bitmap_to_queue(eg_graph *g, const uint64_t *bm, uint32_t *q, uint32_
   return EG_OK;
 }
 
-// Adjacency BFS.  The traversal stays on the device; the host reads the
-// per-level counters to apply the switch rule and size the next launch.
+// Adjacency BFS: k_bfs_level once per level.  The host only enqueues levels,
+// in batches, and reads the control words after each batch.
 static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_mode,
                          bool want_parent, eg_bfs_stats *st) {
   const uint32_t n = g->n;
   const uint32_t nwords = (n + 63) / 64;
   hipStream_t s = g->stream;
-  const uint32_t *optr = g->out_ptr, *ocol = g->out_col;
-  const uint32_t *iptr = g->in_ptr ? g->in_ptr : g->out_ptr;
-  const uint32_t *iidx = g->in_ptr ? g->in_idx : g->out_col;
-  uint32_t *parent = want_parent ? g->bfs_parent : nullptr;
-  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(g->bfs_ctr);
-  void *tmp = nullptr;
-  const size_t qoff_bytes = (((uint64_t)n + 1) * 4 + 255) / 256 * 256;
-  EG_TRY(eg_scratch(g, eg_scan_tmp_bytes(n) + qoff_bytes, &tmp));
-  uint32_t *qoff = reinterpret_cast<uint32_t *>(tmp);
-  void *scan_tmp = reinterpret_cast<uint8_t *>(tmp) + qoff_bytes;
-
-  hipLaunchKernelGGL(k_bfs_init, dim3(blocks_for(n > nwords ? n : nwords, 256)), dim3(256), 0,
-                     s, n, root, g->bfs_dist, parent, g->bfs_visited, g->bfs_front, nwords);
+  if (!g->bfs_f3) {
+    EG_TRY(eg_dalloc_t(&g->bfs_f3, nwords, "frontier bitmap"));
+    EG_TRY(eg_dalloc_t(&g->bfs_qoff, (uint64_t)n + 1, "queue offsets"));
+    EG_TRY(eg_dalloc_t(&g->bfs_nqoff, (uint64_t)n + 1, "queue offsets"));
+  }
+  LevelArgs a;
+  a.n = n;
+  a.nwords = nwords;
+  a.ntiles = blocks_for(nwords, kTileWords);
+  const uint32_t grid = a.ntiles < 2048 ? a.ntiles : 2048;
+  a.m = g->m;
+  a.start_pull = start_pull;
+  a.switch_mode = switch_mode;
+  a.optr = g->out_ptr;
+  a.ocol = g->out_col;
+  a.iptr = g->in_ptr ? g->in_ptr : g->out_ptr;
+  a.iidx = g->in_ptr ? g->in_idx : g->out_col;
+  a.visited = g->bfs_visited;
+  a.F[0] = g->bfs_front;
+  a.F[1] = g->bfs_next;
+  a.F[2] = g->bfs_f3;
+  a.dist = g->bfs_dist;
+  a.parent = want_parent ? g->bfs_parent : nullptr;
+  a.q[0] = g->bfs_q;
+  a.q[1] = g->bfs_nq;
+  a.qoff[0] = g->bfs_qoff;
+  a.qoff[1] = g->bfs_nqoff;
+  // control words after the counters that other passes use
+  a.ctl = reinterpret_cast<unsigned long long *>(g->bfs_ctr) + 16;
+  hipLaunchKernelGGL(k_bfs_level_init, dim3(blocks_for(n > nwords ? n : nwords, 256)), dim3(256),
+                     0, s, a, root);
   EG_LAUNCHED();
-  EG_HIP(hipMemcpyAsync(g->bfs_q, &root, 4, hipMemcpyHostToDevice, s));
-  uint64_t count = 1, qdeg = 0;
-  uint32_t qn = 1;
-  {
-    uint32_t h[2];
-    EG_HIP(hipMemcpyAsync(h, optr + root, 8, hipMemcpyDeviceToHost, s));
-    EG_HIP(hipStreamSynchronize(s));
-    qdeg = h[1] - h[0];
-  }
-  if (qdeg == 0) qn = 0;
-  bool pull = start_pull != 0;
-  bool bm_valid = true;  // bfs_front holds the current frontier
-  bool q_valid = true;   // bfs_q holds the current frontier's vertices with out-edges
-  const double thr = (double)g->m / 20.0;
-  const uint32_t pull_blocks = stride_blocks((uint64_t)nwords * 64);
-  int levels = 0, pull_levels = 0;
-  while (count) {
-    levels++;
-    if ((uint64_t)levels > (uint64_t)n + 1)
-      return eg_fail(EG_EDEVICE, "BFS did not terminate within n levels");
-    const uint32_t newdist = (uint32_t)levels + 1;
-    if (!pull) {
-      if (!q_valid) EG_TRY(bitmap_to_queue(g, g->bfs_front, g->bfs_q, &qn, scan_tmp));
-      if (qn == 0) break;  // no frontier vertex has out-edges: this is the last level
-      EG_HIP(hipMemsetAsync(ctr, 0, 24, s));
-      hipLaunchKernelGGL(k_frontier_degrees, dim3(blocks_for(qn, 256)), dim3(256), 0, s,
-                         g->bfs_q, qn, optr, qoff);
-      EG_LAUNCHED();
-      EG_TRY(eg_scan_excl(s, qoff, qoff, qn, scan_tmp));
-      {
-        EgProf prof(g, "bfs_push");
-        hipLaunchKernelGGL(k_bfs_push, dim3(blocks_for(qdeg, kEdgesPerBlock)), dim3(kBlock), 0, s,
-                           g->bfs_q, qn, qoff, optr, ocol, g->bfs_visited, g->bfs_dist, parent,
-                           newdist, g->bfs_nq, ctr);
+  // the first batch covers the previous traversal's depth, later ones double
+  uint64_t L = 1;
+  uint32_t batch = g->bfs_depth_hint + 1 > 4 ? g->bfs_depth_hint + 1 : 4;
+  if (batch > 64) batch = 64;
+  bool first_batch = true;
+  uint64_t h[kCtlWords];
+  for (;;) {
+    {
+      EgProf prof(g, "bfs_level");
+      for (uint32_t i = 0; i < batch; i++) {
+        hipLaunchKernelGGL(k_bfs_level, dim3(grid), dim3(kBlock), 0, s, a, (uint32_t)(L + i));
         EG_LAUNCHED();
       }
-      std::swap(g->bfs_q, g->bfs_nq);
-      uint64_t h[3];
-      EG_TRY(read_u64(g, ctr, 3, h));
-      count = h[0];
-      qdeg = h[1];
-      qn = (uint32_t)h[2];
-      q_valid = true;
-      bm_valid = false;
-    } else {
-      pull_levels++;
-      if (!bm_valid) {
-        EG_HIP(hipMemsetAsync(g->bfs_front, 0, (size_t)nwords * 8, s));
-        if (qn) {
-          hipLaunchKernelGGL(k_queue_to_bitmap, dim3(blocks_for(qn, 256)), dim3(256), 0, s,
-                             g->bfs_q, qn, g->bfs_front);
-          EG_LAUNCHED();
-        }
-      }
-      {
-        EgProf prof(g, "bfs_pull");
-        hipLaunchKernelGGL(k_bfs_pull, dim3(pull_blocks), dim3(kBlock), 0, s, n, nwords, iptr,
-                           iidx, optr, g->bfs_front, g->bfs_next, g->bfs_visited, g->bfs_dist,
-                           parent, newdist, g->bfs_part);
-        EG_LAUNCHED();
-      }
-      hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, g->bfs_part, pull_blocks, 2u,
-                         ctr);
-      EG_LAUNCHED();
-      std::swap(g->bfs_front, g->bfs_next);
-      uint64_t h[2];
-      EG_TRY(read_u64(g, ctr, 2, h));
-      count = h[0];
-      qdeg = h[1];
-      bm_valid = true;
-      q_valid = false;
     }
-    // bfs_simple.c:191-197.
-    const double load = (double)count + (double)qdeg;
-    pull = (switch_mode && !pull && load > thr) || (pull && load > thr);
+    L += batch;
+    EG_TRY(read_u64(g, a.ctl, kCtlWords, h));
+    if (h[11] || h[2 * (L % 3) + 1] == 0) break;
+    if (L > (uint64_t)n + 1) return eg_fail(EG_EDEVICE, "BFS did not terminate within n levels");
+    batch = first_batch ? 2 : (batch < 32 ? batch * 2 : 64);
+    first_batch = false;
   }
-  st->levels = levels;
-  st->pull_levels = pull_levels;
+  g->bfs_depth_hint = (uint32_t)h[9];
+  st->levels = (int32_t)h[9];
+  st->pull_levels = (int32_t)h[10];
   return EG_OK;
 }
 

@@ -283,6 +283,9 @@ static void free_algo_state(eg_graph *g) {
   eg_dfree_t(&g->bfs_next);
   eg_dfree_t(&g->bfs_q);
   eg_dfree_t(&g->bfs_nq);
+  eg_dfree_t(&g->bfs_f3);
+  eg_dfree_t(&g->bfs_qoff);
+  eg_dfree_t(&g->bfs_nqoff);
   eg_dfree_t(&g->bfs_part);
   eg_dfree_t(&g->bfs_bmcnt);
   eg_dfree_t(&g->bfs_ctr);

@@ -147,6 +147,9 @@ struct eg_graph {
   uint32_t *bfs_dist = nullptr, *bfs_parent = nullptr;
   uint64_t *bfs_visited = nullptr, *bfs_front = nullptr, *bfs_next = nullptr;
   uint32_t *bfs_q = nullptr, *bfs_nq = nullptr;
+  uint64_t *bfs_f3 = nullptr;                    // third frontier bitmap (device-driven BFS)
+  uint32_t *bfs_qoff = nullptr, *bfs_nqoff = nullptr;  // queue edge offsets
+  uint32_t bfs_depth_hint = 0;  // levels of the last traversal (first level batch)
   unsigned long long *bfs_part = nullptr;  // per-workgroup partial counts
   uint32_t *bfs_bmcnt = nullptr;           // bitmap -> queue block counts
   uint32_t ndangling = 0xFFFFFFFFu;        // vertices without out-edges (cached)

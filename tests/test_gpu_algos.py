@@ -202,6 +202,30 @@ def test_bfs_pull_path_is_taken():
     assert O.check_bfs_parents(n, ptr, col, root, dist, par) == 0
 
 
+def test_bfs_long_diameter():
+    """A 3000-vertex chain into a dense cluster: thousands of levels (many
+    batches of device-side levels), then pull levels at the far end."""
+    rng = np.random.default_rng(12)
+    chain = np.arange(3000, dtype=np.uint32)
+    cs = rng.integers(3000, 6000, 40000).astype(np.uint32)
+    cd = rng.integers(3000, 6000, 40000).astype(np.uint32)
+    s0 = np.concatenate([chain, cs])
+    d0 = np.concatenate([chain + 1, cd])
+    n = 6000
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        for root in (0, 1500, 4000):
+            ref, lv, _ = O.bfs(n, s.size, ptr, col, cp, row, root)
+            for start_pull, switch in ((False, True), (True, True), (True, False)):
+                dist, par, st = g.bfs(root, start_pull, switch)
+                assert np.array_equal(dist, ref) and st.levels == lv, (root, start_pull, switch)
+                assert O.check_bfs_parents(n, ptr, col, root, dist, par) == 0
+            assert lv > 1000 or root == 4000
+        dist, par, st = g.bfs(0, False, True)
+        assert st.pull_levels > 0  # the cluster's frontier crosses m/20
+
+
 def test_bfs_symmetric_and_zero_degree():
     # directed graph whose frontier holds sinks; also -s (no CSC) on -u input
     n = 2000
