@@ -55,7 +55,8 @@ ABI_SYMBOLS = (
     "eg_graph_get_degrees", "eg_graph_get_csr", "eg_graph_get_csc", "eg_graph_get_grid",
     "eg_graph_get_edgelist", "eg_pagerank", "eg_spmv", "eg_bfs", "eg_wcc", "eg_sssp",
     "eg_bfs_batch", "eg_ppr_batch", "eg_prof_enable", "eg_prof_reset", "eg_prof_read",
-    "eg_sync", "eg_ppr", "eg_autotune", "eg_tune_set", "eg_tune_get", "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
+    "eg_sync", "eg_ppr", "eg_autotune", "eg_tune_set", "eg_tune_get", "eg_pr_binned_geometry",
+    "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
     "eg_serve_wait", "eg_serve_stats_get", "eg_serve_stop", "eg_serve_replay",
 )
 
@@ -416,12 +417,19 @@ class Graph:
     def autotune(self, kernels=TUNE_PULL | TUNE_SPMM | TUNE_MSBFS) -> list:
         """Times every variant of the selected kernels on this graph and keeps
         the fastest: [(param, value, ms, chosen), ...]."""
-        arr = (TuneEntry * 16)()
+        arr = (TuneEntry * 32)()
         cnt = ctypes.c_int32()
-        _check(lib().eg_autotune(self._h, ctypes.c_uint32(kernels), arr, ctypes.c_int32(16),
+        _check(lib().eg_autotune(self._h, ctypes.c_uint32(kernels), arr, ctypes.c_int32(32),
                                  ctypes.byref(cnt)), "eg_autotune")
         return [(arr[i].param.decode(), arr[i].value, arr[i].ms, bool(arr[i].chosen))
-                for i in range(min(cnt.value, 16))]
+                for i in range(min(cnt.value, 32))]
+
+    def binned_geometry(self) -> dict:
+        """Geometry of the binned PageRank plan (after a binned pagerank call)."""
+        out = np.zeros(6, np.uint32)
+        _check(lib().eg_pr_binned_geometry(self._h, _ptr(out)), "eg_pr_binned_geometry")
+        return dict(zip(("bins", "chunks", "tiles", "items", "long_rows", "segments"),
+                        (int(v) for v in out)))
 
     def tune_set(self, param: str, value: int):
         _check(lib().eg_tune_set(self._h, param.encode(), ctypes.c_int32(value)), "eg_tune_set")

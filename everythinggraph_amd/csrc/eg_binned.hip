@@ -1,0 +1,810 @@
+// eg_binned.hip -- binned (two-phase) PageRank over an adjacency orientation.
+//
+// Reference: the pull iteration of pagerank_simple.c:72-84 / :110-133
+// (rank[v] = (1-d)/n + d * sum_{u in in(v)} prev[u]/deg(u)).  This computes
+// the same recurrence with another data movement.
+//
+// The pull kernel (eg_pull.hip) makes one random 4-byte gather of contrib[u]
+// per stored entry; on R-MAT graphs that is bound by the device's random
+// request rate, not by HBM bandwidth.  The binned form turns every access
+// into a stream:
+//  * destination rows are cut into bins (at most kBinRows rows, balanced by
+//    entries), sources into chunks (at most kChunkSrc sources, balanced by
+//    entries);
+//  * the entries are stored twice, as the u16 source offset within their
+//    chunk in chunk-major order (chunk, bin, row, entry), and as the u16 row
+//    offset within their bin in bin-major order (bin, chunk, row, entry); a
+//    (chunk, bin) segment is contiguous in both orders;
+//  * phase 1 (k_pb_scatter): a workgroup stages its chunk's contributions in
+//    LDS, streams the chunk's source offsets and writes every entry's value
+//    contrib[u] to its bin-major slot: coalesced writes, LDS reads;
+//  * phase 2 (k_pb_accum): a workgroup streams one bin's values and row
+//    offsets and adds them into per-row accumulators in LDS, then applies the
+//    PageRank update to the bin's rows.  Single rows with more entries than a
+//    bin takes are cut into parts summed by separate workgroups.
+// Sums are exact: every FP32 contribution is converted to a 64-bit fixed-point
+// integer with a scale 2^S chosen so that no row sum can overflow, and integer
+// addition is associative, so the result does not depend on the order of the
+// entries, on the bin/chunk cut or on scheduling.  The row sum is rounded to
+// FP32 once at the end.
+//
+// Traffic per iteration, stored entries m: 2m (source offsets) + 4m (values
+// written) + 4m (values read) + 2m (row offsets) + the per-vertex vectors,
+// about 12m + 20n bytes, all streamed.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "eg_internal.h"
+#include "eg_pr.h"
+
+namespace {
+
+constexpr uint32_t kBinRows = 8192;    // rows per bin (u64 accumulators in LDS: 64 KB)
+constexpr uint32_t kChunkSrc = 12288;  // sources per chunk (FP32 window in LDS: 48 KB)
+constexpr uint32_t kTileBins = 2048;   // bins one scatter tile may span (table in LDS: 16 KB)
+constexpr uint32_t kTileLen = 32768;   // positions per scatter tile
+constexpr int kBlock = 1024;           // two workgroups per CU (64 KB LDS each)
+constexpr int kWaves = kBlock / 64;
+constexpr int kWinPer = (kChunkSrc + kBlock - 1) / kBlock;     // window loads per thread
+constexpr int kTabPer = (kTileBins + 1 + kBlock - 1) / kBlock;  // table loads per thread
+constexpr int kRowPer = (kBinRows + kBlock - 1) / kBlock;       // rows finished per thread
+constexpr int kFinBlocks = 128;
+
+inline uint32_t blocks_for(uint64_t count, uint32_t per) {
+  return (uint32_t)((count + per - 1) / per);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed point: value * 2^S as an unsigned 64-bit integer.
+// ---------------------------------------------------------------------------
+// Round-half-up conversion of a finite c >= 0; exact whenever c * 2^S is an
+// integer (always for c >= 2^(23 - S)).
+__device__ __forceinline__ uint64_t to_fixed(float c, int S) {
+  const uint32_t b = __float_as_uint(c);
+  const int e = (int)((b >> 23) & 0xFFu);
+  if (e == 0) return 0;  // zero (and subnormals, below 1.2e-38)
+  const uint64_t mant = (uint64_t)((b & 0x7FFFFFu) | 0x800000u);
+  const int sh = e - 150 + S;  // c = mant * 2^(e - 150)
+  if (sh >= 0) return mant << sh;
+  if (sh < -24) return 0;  // below half a unit
+  return (mant + (1ull << (-sh - 1))) >> (-sh);
+}
+
+__device__ __forceinline__ float from_fixed(uint64_t v, int S) {
+  return (float)ldexp((double)v, -S);
+}
+
+struct PbTile {
+  uint32_t c;       // chunk
+  uint32_t p0, p1;  // chunk-major positions [p0, p1)
+  uint32_t b0;      // first bin (segment of p0)
+  uint32_t nb;      // bins spanned
+  uint32_t pad[3];
+};
+
+// Phase-2 work item: bin rows [row0, row0 + nrows) with bin-major entries
+// [e0, e1); nrows == 0 marks a part of a long row whose sum goes to
+// part_sum[row0].
+struct PbItem {
+  uint32_t e0, e1, row0, nrows;
+};
+
+struct PbDev {
+  const PbTile *tiles;
+  const uint2 *chunk_src;  // (first source, sources)
+  const uint2 *tab;        // [nchunks][nbins + 1]: (chunk-major start, bin-major start)
+  const uint16_t *sloc;    // chunk-major source offsets
+  const uint16_t *dloc;    // bin-major row offsets
+  float *val;              // bin-major values
+  const PbItem *items;
+  const uint4 *long_rows;  // (row, first part, parts, 0)
+  unsigned long long *part_sum;
+  double2 *item_part;
+  double2 *fin_part;
+  uint32_t *fin_counter;
+  uint32_t nbins, nitems, nlong;
+};
+
+// ---------------------------------------------------------------------------
+// Build kernels.
+// ---------------------------------------------------------------------------
+// key / payload of every entry, a wave per row (grid-stride over rows).
+// bin_major: key = bin * nchunks + chunk, payload = row - first row of bin;
+// else key = chunk * nbins + bin, payload = source - first source of chunk.
+__global__ void k_pb_keys(uint32_t n, const uint32_t *__restrict__ ptr,
+                          const uint32_t *__restrict__ idx, const uint32_t *__restrict__ bin_of,
+                          const uint32_t *__restrict__ chunk_of,
+                          const uint32_t *__restrict__ bin_row0,
+                          const uint32_t *__restrict__ chunk_src0, uint32_t nbins,
+                          uint32_t nchunks, int bin_major, uint32_t *__restrict__ key,
+                          uint32_t *__restrict__ pay) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int lane = __lane_id();
+  for (uint32_t v = wave; v < n; v += nwaves) {
+    const uint32_t k0 = ptr[v], k1 = ptr[v + 1];
+    if (k0 == k1) continue;
+    const uint32_t b = bin_of[v];
+    for (uint32_t k = k0 + lane; k < k1; k += 64) {
+      const uint32_t u = idx[k];
+      const uint32_t c = chunk_of[u];
+      if (bin_major) {
+        key[k] = b * nchunks + c;
+        pay[k] = v - bin_row0[b];
+      } else {
+        key[k] = c * nbins + b;
+        pay[k] = u - chunk_src0[c];
+      }
+    }
+  }
+}
+
+__global__ void k_pb_narrow(uint64_t m, const uint32_t *__restrict__ in,
+                            uint16_t *__restrict__ out) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) out[k] = (uint16_t)in[k];
+}
+
+// ---------------------------------------------------------------------------
+// Phase 1: values into bin-major slots.
+// ---------------------------------------------------------------------------
+// B: source offsets loaded per lane before any value is written.
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_pb_scatter(PbDev d, const float *__restrict__ contrib) {
+  __shared__ float s_win[kChunkSrc];
+  __shared__ uint2 s_tab[kTileBins + 1];
+  const PbTile t = d.tiles[blockIdx.x];
+  const uint2 cs = d.chunk_src[t.c];
+  const uint32_t tid = threadIdx.x;
+  // the window and the table: every load issued before the first LDS store
+  {
+    const uint2 *trow = d.tab + (size_t)t.c * (d.nbins + 1) + t.b0;
+    float wv[kWinPer];
+    uint2 tv[kTabPer];
+#pragma unroll
+    for (int j = 0; j < kWinPer; j++) {
+      const uint32_t i = tid + j * kBlock;
+      wv[j] = i < cs.y ? contrib[cs.x + i] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kTabPer; j++) {
+      const uint32_t i = tid + j * kBlock;
+      tv[j] = i <= t.nb ? trow[i] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int j = 0; j < kWinPer; j++) {
+      const uint32_t i = tid + j * kBlock;
+      if (i < cs.y) s_win[i] = wv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kTabPer; j++) {
+      const uint32_t i = tid + j * kBlock;
+      if (i <= t.nb) s_tab[i] = tv[j];
+    }
+  }
+  __syncthreads();
+
+  // a contiguous range of positions per wave
+  const int lane = __lane_id();
+  const uint32_t wave = tid >> 6;
+  const uint32_t len = t.p1 - t.p0;
+  const uint32_t per = ((len + kWaves - 1) / kWaves + 63) & ~63u;
+  const uint32_t w0 = t.p0 + min(len, wave * per);
+  const uint32_t w1 = t.p0 + min(len, (wave + 1) * per);
+  if (w0 >= w1) return;
+  // segment of this lane's first position: the last bin whose start is <= p
+  uint32_t bi = 0;
+  {
+    const uint32_t p = min(w0 + lane, w1 - 1);
+    uint32_t lo = 0, hi = t.nb - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (s_tab[mid].x <= p) lo = mid; else hi = mid - 1;
+    }
+    bi = lo;
+  }
+  for (uint32_t base = w0; base < w1; base += 64 * B) {
+    uint32_t s[B];
+#pragma unroll
+    for (int q = 0; q < B; q++) {
+      const uint32_t p = base + q * 64 + lane;
+      s[q] = p < w1 ? (uint32_t)__builtin_nontemporal_load(d.sloc + p) : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < B; q++) {
+      const uint32_t p = base + q * 64 + lane;
+      if (p < w1) {
+        while (p >= s_tab[bi + 1].x) bi++;
+        d.val[s_tab[bi].y + (p - s_tab[bi].x)] = s_win[s[q]];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Phase 2: per-bin accumulation and the PageRank update.
+// ---------------------------------------------------------------------------
+// A lane takes groups of 8 consecutive entries (one 16-byte load of row
+// offsets, two of values), sums runs of equal rows in registers and adds each
+// run to its LDS accumulator with one atomic.  Entries of a (chunk, bin)
+// segment are in row order, so a hub row's entries form long runs and cost
+// about one atomic per group.  Groups are aligned to 8 entries; entries of a
+// group outside [e0, e1) are masked.
+constexpr int kGroupsInFlight = 2;  // groups loaded per lane before any is summed
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void group_accumulate(unsigned long long *acc, const u32x4 dl,
+                                                 const f32x4 v0, const f32x4 v1,
+                                                 uint32_t e, uint32_t e0, uint32_t e1, int S,
+                                                 bool part, unsigned long long &psum) {
+  const uint32_t key[8] = {dl.x & 0xFFFFu, dl.x >> 16, dl.y & 0xFFFFu, dl.y >> 16,
+                           dl.z & 0xFFFFu, dl.z >> 16, dl.w & 0xFFFFu, dl.w >> 16};
+  const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+  uint32_t cur = 0xFFFFFFFFu;
+  unsigned long long run = 0ull;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const bool ok = e + j >= e0 && e + j < e1;
+    const unsigned long long f = ok ? to_fixed(v[j], S) : 0ull;
+    if (part) {
+      psum += f;
+      continue;
+    }
+    const uint32_t k = ok ? key[j] : 0xFFFFFFFFu;
+    if (k != cur) {
+      if (cur != 0xFFFFFFFFu) atomicAdd(&acc[cur], run);
+      cur = k;
+      run = 0ull;
+    }
+    run += f;
+  }
+  if (!part && cur != 0xFFFFFFFFu) atomicAdd(&acc[cur], run);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pb_accum(PbDev d, int S, float *__restrict__ y,
+                                                     PrArgs a) {
+  __shared__ unsigned long long s_acc[kBinRows];
+  __shared__ unsigned long long s_ps[kWaves];
+  __shared__ double s_d[kWaves], s_l[kWaves];
+  const PbItem it = d.items[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+  const int lane = __lane_id();
+  const uint32_t wave = tid >> 6;
+  const bool part = it.nrows == 0;
+  if (!part)
+    for (uint32_t i = tid; i < it.nrows; i += kBlock) s_acc[i] = 0ull;
+  __syncthreads();
+
+  // groups of 8 entries from the aligned start; a contiguous range per wave
+  const uint32_t g0 = it.e0 >> 3;
+  const uint32_t ng = ((it.e1 + 7) >> 3) - g0;
+  const uint32_t per = ((ng + kWaves - 1) / kWaves + 63) & ~63u;
+  const uint32_t wg0 = g0 + min(ng, wave * per);
+  const uint32_t wg1 = g0 + min(ng, (wave + 1) * per);
+  const u32x4 *dl4 = reinterpret_cast<const u32x4 *>(d.dloc);
+  const f32x4 *v4 = reinterpret_cast<const f32x4 *>(d.val);
+  unsigned long long psum = 0ull;
+  for (uint32_t base = wg0; base < wg1; base += 64 * kGroupsInFlight) {
+    u32x4 dl[kGroupsInFlight];
+    f32x4 va[kGroupsInFlight], vb[kGroupsInFlight];
+#pragma unroll
+    for (int q = 0; q < kGroupsInFlight; q++) {
+      const uint32_t gi = base + q * 64 + lane;
+      if (gi < wg1) {
+        dl[q] = __builtin_nontemporal_load(dl4 + gi);
+        va[q] = __builtin_nontemporal_load(v4 + 2 * (size_t)gi);
+        vb[q] = __builtin_nontemporal_load(v4 + 2 * (size_t)gi + 1);
+      } else {
+        dl[q] = u32x4{0u, 0u, 0u, 0u};
+        va[q] = vb[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kGroupsInFlight; q++) {
+      const uint32_t gi = base + q * 64 + lane;
+      if (gi < wg1) group_accumulate(s_acc, dl[q], va[q], vb[q], gi * 8, it.e0, it.e1, S, part, psum);
+    }
+  }
+  if (part) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) psum += __shfl_xor(psum, o);
+    if (lane == 0) s_ps[wave] = psum;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long t = 0ull;
+      for (int q = 0; q < kWaves; q++) t += s_ps[q];
+      d.part_sum[it.row0] = t;
+      d.item_part[blockIdx.x] = make_double2(0.0, 0.0);
+    }
+    return;
+  }
+  // per-row inputs of this thread's rows, loaded before the LDS barrier
+  uint32_t dg[kRowPer];
+  float ro[kRowPer];
+#pragma unroll
+  for (int j = 0; j < kRowPer; j++) {
+    const uint32_t i = tid + j * kBlock;
+    dg[j] = i < it.nrows ? a.out_deg[it.row0 + i] : 0u;
+    ro[j] = i < it.nrows ? a.rank_old[it.row0 + i] : 0.f;
+  }
+  __syncthreads();
+  const float basep = pr_base(a);
+  double dang = 0.0, l1 = 0.0;
+#pragma unroll
+  for (int j = 0; j < kRowPer; j++) {
+    const uint32_t i = tid + j * kBlock;
+    if (i < it.nrows)
+      finish_row<true>(it.row0 + i, from_fixed(s_acc[i], S), y, a, basep, dg[j], ro[j], dang, l1);
+  }
+  dang = eg_wave_sum_f64(dang);
+  l1 = eg_wave_sum_f64(l1);
+  if (lane == 0) {
+    s_d[wave] = dang;
+    s_l[wave] = l1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double td = 0.0, tl = 0.0;
+    for (int q = 0; q < kWaves; q++) {
+      td += s_d[q];
+      tl += s_l[q];
+    }
+    d.item_part[blockIdx.x] = make_double2(td, tl);
+  }
+}
+
+// Long rows (sum of their parts) and the iteration's FP64 statistics; the
+// last block to finish writes them and re-arms the counter.
+__global__ __launch_bounds__(256) void k_pb_finish(PbDev d, int S, float *__restrict__ y,
+                                                   PrArgs a) {
+  __shared__ double s_d[4], s_l[4];
+  __shared__ uint32_t s_last;
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid >> 6;
+  const uint32_t gtid = blockIdx.x * 256 + tid;
+  const uint32_t gthreads = gridDim.x * 256;
+  const float basep = pr_base(a);
+  double dang = 0.0, l1 = 0.0;
+  for (uint32_t j = gtid; j < d.nlong; j += gthreads) {
+    const uint4 lr = d.long_rows[j];
+    unsigned long long t = 0ull;
+    for (uint32_t q = 0; q < lr.z; q++) t += d.part_sum[lr.y + q];
+    finish_row<true>(lr.x, from_fixed(t, S), y, a, basep, a.out_deg[lr.x], a.rank_old[lr.x],
+                     dang, l1);
+  }
+  for (uint32_t i = gtid; i < d.nitems; i += gthreads) {
+    const double2 p = d.item_part[i];
+    dang += p.x;
+    l1 += p.y;
+  }
+  dang = eg_wave_sum_f64(dang);
+  l1 = eg_wave_sum_f64(l1);
+  if (lane == 0) {
+    s_d[wave] = dang;
+    s_l[wave] = l1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double td = 0.0, tl = 0.0;
+    for (int q = 0; q < 4; q++) {
+      td += s_d[q];
+      tl += s_l[q];
+    }
+    d.fin_part[blockIdx.x] = make_double2(td, tl);
+    __threadfence();
+    const uint32_t prev = atomicAdd(d.fin_counter, 1u);
+    s_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last && tid == 0) {
+    __threadfence();
+    double td = 0.0, tl = 0.0;
+    for (uint32_t b = 0; b < gridDim.x; b++) {
+      const double *pp = reinterpret_cast<const double *>(&d.fin_part[b]);
+      td += __builtin_nontemporal_load(pp);
+      tl += __builtin_nontemporal_load(pp + 1);
+    }
+    a.stats_out[0] = td;
+    a.stats_out[1] = tl;
+    *d.fin_counter = 0u;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+struct PbPlan {
+  const uint32_t *for_ptr = nullptr;  // orientation the plan was built for
+  uint32_t nbins = 0, nchunks = 0, ntiles = 0, nitems = 0, nlong = 0, nparts = 0;
+  uint32_t tile_len = 0;   // scatter tile length the plan was cut with
+  uint64_t nsegs = 0;      // non-empty (chunk, bin) segments
+  PbTile *tiles = nullptr;
+  uint2 *chunk_src = nullptr;
+  uint2 *tab = nullptr;
+  uint16_t *sloc = nullptr, *dloc = nullptr;
+  float *val = nullptr;
+  PbItem *items = nullptr;
+  uint4 *long_rows = nullptr;
+  unsigned long long *part_sum = nullptr;
+  double2 *item_part = nullptr, *fin_part = nullptr;
+  uint32_t *fin_counter = nullptr;
+};
+
+void eg_pb_free(eg_graph *g) {
+  PbPlan *p = g->pb;
+  if (!p) return;
+  eg_dfree_t(&p->tiles);
+  eg_dfree_t(&p->chunk_src);
+  eg_dfree_t(&p->tab);
+  eg_dfree_t(&p->sloc);
+  eg_dfree_t(&p->dloc);
+  eg_dfree_t(&p->val);
+  eg_dfree_t(&p->items);
+  eg_dfree_t(&p->long_rows);
+  eg_dfree_t(&p->part_sum);
+  eg_dfree_t(&p->item_part);
+  eg_dfree_t(&p->fin_part);
+  eg_dfree_t(&p->fin_counter);
+  delete p;
+  g->pb = nullptr;
+}
+
+namespace {
+
+template <typename T>
+int upload(eg_graph *g, T **dst, const std::vector<T> &src, const char *what) {
+  EG_TRY(eg_dalloc_t(dst, src.empty() ? 1 : src.size(), what));
+  if (!src.empty())
+    EG_HIP(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice,
+                          g->stream));
+  return EG_OK;
+}
+
+// Sorted keys and their u16 payloads of one order; seg receives the
+// exclusive scan of the segment sizes (nkeys + 1 entries) on the host.
+int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32_t *bin_of,
+             const uint32_t *chunk_of, const uint32_t *bin_row0, const uint32_t *chunk_src0,
+             uint32_t nbins, uint32_t nchunks, int bin_major, uint16_t *out,
+             std::vector<uint32_t> &seg) {
+  const uint32_t n = g->n;
+  const uint64_t m = g->m;
+  hipStream_t s = g->stream;
+  const uint64_t nkeys = (uint64_t)nbins * nchunks;
+  uint32_t *key = nullptr, *pay = nullptr, *skey = nullptr, *spay = nullptr, *segd = nullptr;
+  int rc = EG_OK;
+  do {
+    if ((rc = eg_dalloc_t(&key, m, "binned keys")) || (rc = eg_dalloc_t(&pay, m, "binned payload")) ||
+        (rc = eg_dalloc_t(&skey, m, "binned sorted keys")) ||
+        (rc = eg_dalloc_t(&spay, m, "binned sorted payload")) ||
+        (rc = eg_dalloc_t(&segd, nkeys + 1, "binned segments")))
+      break;
+    const uint32_t grid = std::min<uint64_t>(blocks_for((uint64_t)n * 64, 256), 65536);
+    hipLaunchKernelGGL(k_pb_keys, dim3(grid), dim3(256), 0, s, n, ptr, idx, bin_of, chunk_of,
+                       bin_row0, chunk_src0, nbins, nchunks, bin_major, key, pay);
+    if (hipGetLastError() != hipSuccess) {
+      rc = eg_fail(EG_EDEVICE, "binned key launch");
+      break;
+    }
+    void *tmp = nullptr;
+    EgSortCols c;
+    c.np = 1;
+    c.in[0] = pay;
+    c.out[0] = spay;
+    if ((rc = eg_scratch(g, eg_radix_tmp_bytes(m, 1), &tmp))) break;
+    if ((rc = eg_radix_sort(s, m, eg_bits_for(nkeys), key, skey, c, tmp))) break;
+    hipLaunchKernelGGL(k_pb_narrow, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, spay, out);
+    if (hipGetLastError() != hipSuccess) {
+      rc = eg_fail(EG_EDEVICE, "binned narrow launch");
+      break;
+    }
+    if ((rc = eg_ptr_from_sorted(s, skey, m, (uint32_t)nkeys, segd, nullptr))) break;
+    seg.resize(nkeys + 1);
+    if (hipMemcpyAsync(seg.data(), segd, (nkeys + 1) * 4, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = eg_fail(EG_EDEVICE, "binned segment download");
+      break;
+    }
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  eg_dfree_t(&key);
+  eg_dfree_t(&pay);
+  eg_dfree_t(&skey);
+  eg_dfree_t(&spay);
+  eg_dfree_t(&segd);
+  return rc;
+}
+
+}  // namespace
+
+// Builds (on first use for this orientation) the binned plan of the pull
+// orientation (ptr, idx): row v's entries are the sources u of its in-edges.
+int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
+  const uint32_t want_tile = g->tune.pr_tile > 0 ? (uint32_t)g->tune.pr_tile : kTileLen;
+  if (g->pb && g->pb->for_ptr == ptr && g->pb->tile_len == want_tile) return EG_OK;
+  eg_pb_free(g);
+  const uint32_t n = g->n;
+  const uint64_t m = g->m;
+  hipStream_t s = g->stream;
+  if (m == 0 || n == 0) return eg_fail(EG_ENOTSUP, "binned PageRank needs entries");
+  // 1. row lengths and source counts on the host
+  std::vector<uint32_t> hptr((size_t)n + 1), scnt(n);
+  {
+    uint32_t *cnt = nullptr;
+    EG_TRY(eg_dalloc_t(&cnt, n, "binned source counts"));
+    int rc = EG_OK;
+    if (hipMemsetAsync(cnt, 0, (size_t)n * 4, s) != hipSuccess)
+      rc = eg_fail(EG_EDEVICE, "binned count clear");
+    if (!rc) rc = eg_histogram(s, idx, m, cnt, n);
+    if (!rc && (hipMemcpyAsync(scnt.data(), cnt, (size_t)n * 4, hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipMemcpyAsync(hptr.data(), ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+      rc = eg_fail(EG_EDEVICE, "binned degree download");
+    eg_dfree_t(&cnt);
+    EG_TRY(rc);
+  }
+  // 2. bins of rows and chunks of sources, each balanced by entries.  With
+  // EG_PB_GEOMETRY=small in the environment every cap is cut down so that
+  // small graphs exercise long-row parts and tile cuts (tests).
+  uint32_t rows_cap = kBinRows, src_cap = kChunkSrc, tile_len = want_tile, tile_bins = kTileBins;
+  uint64_t target = std::max<uint64_t>(m / 2048, 4096);
+  {
+    const char *e = getenv("EG_PB_GEOMETRY");
+    if (e && !strcmp(e, "small")) {
+      rows_cap = 64;
+      src_cap = 96;
+      tile_len = 512;
+      tile_bins = 16;
+      target = 256;
+    }
+  }
+  std::vector<uint32_t> bin_row0, bin_of(n), chunk_src0, chunk_of(n);
+  {
+    uint32_t rows = 0;
+    uint64_t ents = 0;
+    for (uint32_t v = 0; v < n; v++) {
+      const uint32_t dv = hptr[v + 1] - hptr[v];
+      if (rows == 0 || rows + 1 > rows_cap || ents + dv > target) {
+        bin_row0.push_back(v);
+        rows = 0;
+        ents = 0;
+      }
+      bin_of[v] = (uint32_t)bin_row0.size() - 1;
+      rows++;
+      ents += dv;
+    }
+    bin_row0.push_back(n);
+    uint32_t srcs = 0;
+    ents = 0;
+    for (uint32_t u = 0; u < n; u++) {
+      if (srcs == 0 || srcs + 1 > src_cap || ents + scnt[u] > target) {
+        chunk_src0.push_back(u);
+        srcs = 0;
+        ents = 0;
+      }
+      chunk_of[u] = (uint32_t)chunk_src0.size() - 1;
+      srcs++;
+      ents += scnt[u];
+    }
+    chunk_src0.push_back(n);
+  }
+  const uint32_t nbins = (uint32_t)bin_row0.size() - 1;
+  const uint32_t nchunks = (uint32_t)chunk_src0.size() - 1;
+  if ((uint64_t)nbins * nchunks >= (1ull << 31))
+    return eg_fail(EG_ENOTSUP, "binned plan: %u bins x %u chunks", nbins, nchunks);
+
+  PbPlan *p = new PbPlan();
+  g->pb = p;
+  p->nbins = nbins;
+  p->nchunks = nchunks;
+  std::vector<uint32_t> seg_bm, seg_cm;
+  {
+    uint32_t *d_bin_of = nullptr, *d_chunk_of = nullptr, *d_row0 = nullptr, *d_src0 = nullptr;
+    int rc = EG_OK;
+    do {
+      if ((rc = upload(g, &d_bin_of, bin_of, "binned bin_of")) ||
+          (rc = upload(g, &d_chunk_of, chunk_of, "binned chunk_of")) ||
+          (rc = upload(g, &d_row0, bin_row0, "binned bin rows")) ||
+          (rc = upload(g, &d_src0, chunk_src0, "binned chunk sources")))
+        break;
+      if ((rc = eg_dalloc_t(&p->dloc, m + 16, "binned row offsets")) ||
+          (rc = eg_dalloc_t(&p->sloc, m, "binned source offsets")))
+        break;
+      if ((rc = pb_order(g, ptr, idx, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, 1,
+                         p->dloc, seg_bm)))
+        break;
+      rc = pb_order(g, ptr, idx, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, 0, p->sloc,
+                    seg_cm);
+    } while (0);
+    (void)hipStreamSynchronize(s);
+    eg_dfree_t(&d_bin_of);
+    eg_dfree_t(&d_chunk_of);
+    eg_dfree_t(&d_row0);
+    eg_dfree_t(&d_src0);
+    if (rc) {
+      eg_pb_free(g);
+      return rc;
+    }
+  }
+  // 3. segment table, scatter tiles, accumulation items
+  std::vector<uint2> tab((size_t)nchunks * (nbins + 1));
+  std::vector<uint2> csrc(nchunks);
+  std::vector<PbTile> tiles;
+  for (uint32_t c = 0; c < nchunks; c++) {
+    uint2 *row = &tab[(size_t)c * (nbins + 1)];
+    for (uint32_t b = 0; b < nbins; b++)
+      row[b] = make_uint2(seg_cm[(size_t)c * nbins + b], seg_bm[(size_t)b * nchunks + c]);
+    const uint32_t cend = seg_cm[(size_t)(c + 1) * nbins];
+    row[nbins] = make_uint2(cend, 0u);
+    csrc[c] = make_uint2(chunk_src0[c], chunk_src0[c + 1] - chunk_src0[c]);
+    // segment containing position q: last b with row[b].x <= q
+    auto seg_of = [&](uint32_t q) {
+      uint32_t lo = 0, hi = nbins - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (row[mid].x <= q) lo = mid; else hi = mid - 1;
+      }
+      return lo;
+    };
+    for (uint32_t q = row[0].x; q < cend;) {
+      PbTile t = {};
+      t.c = c;
+      t.p0 = q;
+      t.b0 = seg_of(q);
+      uint32_t q1 = std::min<uint64_t>(cend, (uint64_t)q + tile_len);
+      uint32_t b1 = seg_of(q1 - 1);
+      if (b1 - t.b0 + 1 > tile_bins) {
+        b1 = t.b0 + tile_bins - 1;
+        q1 = row[b1 + 1].x;
+      }
+      t.p1 = q1;
+      t.nb = b1 - t.b0 + 1;
+      tiles.push_back(t);
+      q = q1;
+    }
+  }
+  std::vector<PbItem> items;
+  std::vector<uint4> long_rows;
+  uint32_t nparts = 0;
+  for (uint32_t b = 0; b < nbins; b++) {
+    const uint32_t e0 = seg_bm[(size_t)b * nchunks], e1 = seg_bm[(size_t)(b + 1) * nchunks];
+    const uint32_t r0 = bin_row0[b], nr = bin_row0[b + 1] - r0;
+    if (nr == 1 && e1 - e0 > target) {
+      const uint32_t np = (uint32_t)((e1 - e0 + target - 1) / target);
+      long_rows.push_back(make_uint4(r0, nparts, np, 0u));
+      for (uint32_t q = 0; q < np; q++) {
+        const uint32_t a0 = e0 + (uint32_t)(q * target);
+        const uint32_t a1 = std::min<uint64_t>(e1, (uint64_t)a0 + target);
+        items.push_back({a0, a1, nparts + q, 0u});
+      }
+      nparts += np;
+    } else {
+      items.push_back({e0, e1, r0, nr});
+    }
+  }
+  // Workgroups are dealt to the 8 XCDs round-robin by launch index.  Launch
+  // index j runs tile (j % 8) * ceil(T / 8) + j / 8, so every XCD walks a
+  // contiguous eighth of the chunk-ordered tile list and the tiles of one
+  // chunk share that XCD's L2 for the chunk's window and table.
+  {
+    const size_t T = tiles.size(), per = (T + 7) / 8;
+    std::vector<PbTile> order;
+    order.reserve(T);
+    for (size_t j = 0; j < 8 * per; j++) {
+      const size_t src = (j % 8) * per + j / 8;
+      if (src < T) order.push_back(tiles[src]);
+    }
+    tiles.swap(order);
+  }
+  for (size_t k = 0; k + 1 < seg_cm.size(); k++) p->nsegs += seg_cm[k + 1] > seg_cm[k];
+  p->tile_len = tile_len;
+  p->ntiles = (uint32_t)tiles.size();
+  p->nitems = (uint32_t)items.size();
+  p->nlong = (uint32_t)long_rows.size();
+  p->nparts = nparts;
+  int rc = EG_OK;
+  do {
+    if ((rc = upload(g, &p->tab, tab, "binned table")) ||
+        (rc = upload(g, &p->chunk_src, csrc, "binned chunks")) ||
+        (rc = upload(g, &p->tiles, tiles, "binned tiles")) ||
+        (rc = upload(g, &p->items, items, "binned items")) ||
+        (rc = upload(g, &p->long_rows, long_rows, "binned long rows")))
+      break;
+    if ((rc = eg_dalloc_t(&p->val, m + 16, "binned values")) ||
+        (rc = eg_dalloc_t(&p->part_sum, nparts ? nparts : 1, "binned part sums")) ||
+        (rc = eg_dalloc_t(&p->item_part, p->nitems, "binned item partials")) ||
+        (rc = eg_dalloc_t(&p->fin_part, kFinBlocks, "binned finish partials")) ||
+        (rc = eg_dalloc_t(&p->fin_counter, 1, "binned finish counter")))
+      break;
+    if (hipMemsetAsync(p->fin_counter, 0, 4, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      rc = eg_fail(EG_EDEVICE, "binned plan upload");
+  } while (0);
+  if (rc) {
+    eg_pb_free(g);
+    return rc;
+  }
+  p->for_ptr = ptr;
+  return EG_OK;
+}
+
+bool eg_pb_selected(const eg_graph *g) {
+  return g->tune.pr_binned > 0 || (g->tune.pr_binned < 0 && g->m >= (1ull << 22));
+}
+
+// Fixed-point scale for row sums bounded by `bound`: the largest S with
+// bound * 2^S <= 2^62.
+int eg_pb_scale(double bound) {
+  if (!(bound > 0.0)) bound = 1.0;
+  return 62 - (int)ceil(log2(bound));
+}
+
+// One PageRank iteration from contrib (the input iterate's contributions) into
+// y / a.contrib_new, with the statistics of eg_pull.hip.
+int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, int S,
+                  hipStream_t stream) {
+  const PbPlan *p = g->pb;
+  if (!p) return eg_fail(EG_ESTATE, "binned plan not built");
+  hipStream_t s = stream ? stream : g->stream;
+  PbDev d;
+  d.tiles = p->tiles;
+  d.chunk_src = p->chunk_src;
+  d.tab = p->tab;
+  d.sloc = p->sloc;
+  d.dloc = p->dloc;
+  d.val = p->val;
+  d.items = p->items;
+  d.long_rows = p->long_rows;
+  d.part_sum = p->part_sum;
+  d.item_part = p->item_part;
+  d.fin_part = p->fin_part;
+  d.fin_counter = p->fin_counter;
+  d.nbins = p->nbins;
+  d.nitems = p->nitems;
+  d.nlong = p->nlong;
+  {
+    EgProf prof(g, "pr_scatter", s);
+    const int sb = g->tune.pr_scatter;
+    if (sb == 8)
+      hipLaunchKernelGGL((k_pb_scatter<8>), dim3(p->ntiles), dim3(kBlock), 0, s, d, contrib);
+    else if (sb == 32)
+      hipLaunchKernelGGL((k_pb_scatter<32>), dim3(p->ntiles), dim3(kBlock), 0, s, d, contrib);
+    else
+      hipLaunchKernelGGL((k_pb_scatter<16>), dim3(p->ntiles), dim3(kBlock), 0, s, d, contrib);
+    EG_LAUNCHED();
+  }
+  {
+    EgProf prof(g, "pr_accum", s);
+    hipLaunchKernelGGL(k_pb_accum, dim3(p->nitems), dim3(kBlock), 0, s, d, S, y, a);
+    EG_LAUNCHED();
+  }
+  {
+    EgProf prof(g, "pr_finish", s);
+    hipLaunchKernelGGL(k_pb_finish, dim3(kFinBlocks), dim3(256), 0, s, d, S, y, a);
+    EG_LAUNCHED();
+  }
+  return EG_OK;
+}
+
+extern "C" int eg_pr_binned_geometry(eg_graph *g, uint32_t *out) {
+  if (!g || !out) return eg_fail(EG_EINVAL, "null argument");
+  const PbPlan *p = g->pb;
+  if (!p) return eg_fail(EG_ESTATE, "binned plan not built");
+  out[0] = p->nbins;
+  out[1] = p->nchunks;
+  out[2] = p->ntiles;
+  out[3] = p->nitems;
+  out[4] = p->nlong;
+  out[5] = (uint32_t)std::min<uint64_t>(p->nsegs, 0xFFFFFFFFull);
+  return EG_OK;
+}

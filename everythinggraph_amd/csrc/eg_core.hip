@@ -264,6 +264,7 @@ static void free_layout(eg_graph *g) {
   eg_plan_free(&g->plan_out);
   eg_grid_free(g);
   eg_order_free(g);
+  eg_pb_free(g);
   eg_batch_free(g);
   g->loaded = false;
   g->ndangling = 0xFFFFFFFFu;

@@ -82,6 +82,7 @@ struct PullPlan {
 
 struct GridCompute;
 struct BatchState;
+struct PbPlan;
 
 // Degree-ordered copy of the pull orientation (eg_order.hip): vertex i of the
 // copy is old vertex old_of_new[i]; rows keep their entry order.
@@ -108,6 +109,11 @@ struct EgTune {
                          // highest-degree sources' values in LDS
   int pull_hot = 0;      // k_pull on the degree-ordered copy: sources below this id
                          // are cached normally, the rest loaded non-temporally (0: all normal)
+  int pr_binned = -1;    // PageRank on adjacency lists: 1 = binned two-phase kernels
+                         // (exact fixed-point row sums), 0 = pull kernel (compensated
+                         // FP32 sums), -1 = binned when the graph has >= 2^22 entries
+  int pr_tile = 0;       // binned PageRank: source offsets per scatter tile (0: 32768)
+  int pr_scatter = 16;   // binned scatter: source offsets in flight per lane (8, 16 or 32)
 };
 
 struct eg_graph {
@@ -165,6 +171,7 @@ struct eg_graph {
   GridCompute *gridc = nullptr;  // compute copy of the grid (eg_grid.hip)
   BatchState *batch = nullptr;   // multi-source state (eg_batch.hip)
   EgOrdered *ord = nullptr;      // degree-ordered pull copy (eg_order.hip)
+  PbPlan *pb = nullptr;          // binned PageRank plan (eg_binned.hip)
 
   int prof_on = 0;
   std::vector<EgProfSlot> prof;
@@ -180,6 +187,13 @@ struct eg_graph {
 void eg_grid_free(eg_graph *g);
 void eg_batch_free(eg_graph *g);
 void eg_order_free(eg_graph *g);
+void eg_pb_free(eg_graph *g);
+// Binned PageRank (eg_binned.hip): the plan of the pull orientation (ptr,
+// idx), built on first use; the fixed-point scale for row sums <= bound.
+int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx);
+int eg_pb_scale(double bound);
+// True when PageRank on adjacency lists uses the binned kernels.
+bool eg_pb_selected(const eg_graph *g);
 bool eg_reorder_enabled();
 // Builds (on first use) and returns the degree-ordered pull copy.
 int eg_ordered(eg_graph *g, EgOrdered **out);

@@ -26,6 +26,7 @@
 #include <algorithm>
 
 #include "eg_internal.h"
+#include "eg_pr.h"
 
 namespace {
 
@@ -127,20 +128,6 @@ __global__ void k_seg_info(const uint32_t *__restrict__ ptr, const uint32_t *__r
 // ---------------------------------------------------------------------------
 // Iteration kernels.
 // ---------------------------------------------------------------------------
-struct PrArgs {
-  const float *rank_old;  // for the L1 delta
-  float *contrib_new;
-  const uint32_t *out_deg;
-  const double *dmass_in;  // dangling mass of the input iterate
-  double *stats_out;       // [dangling of output, l1]
-  float base;              // (1 - d) / n
-  const float *base_vec;   // per-row term added to base (personalized PageRank), or null
-  float damping;
-  double damping_over_n;   // d / n, for the uniform dangling term
-  int uniform;
-  uint32_t hot;            // gathers of x[u] with u >= hot are non-temporal loads
-};
-
 // Values of entries k, k + kBlock, ..., k + (B-1)*kBlock (0 past kend).
 // All index loads are issued before all gathers so that a lane keeps B
 // independent memory requests in flight.  A lane meets its entries in the same
@@ -170,31 +157,6 @@ __device__ __forceinline__ void gather_batch(const uint32_t *__restrict__ idx,
                      : u[q] < hot             ? x[u[q]]
                                               : __builtin_nontemporal_load(x + u[q]);
     v[q] = (k + q * kBlock < kend) ? (WEIGHTED ? xv * wv[q] : xv) : 0.f;
-  }
-}
-
-__device__ __forceinline__ float pr_base(const PrArgs &a) {
-  double b = (double)a.base;
-  if (a.uniform) b += a.damping_over_n * (*a.dmass_in);
-  return (float)b;
-}
-
-// Writes the row result; for PageRank also the new contribution and the FP64
-// partials (dangling mass of the new iterate, |r' - r|).
-template <bool PR>
-__device__ __forceinline__ void finish_row(uint32_t r, float sum, float *__restrict__ y,
-                                           const PrArgs &a, float basep, uint32_t deg,
-                                           float rold, double &dang, double &l1) {
-  if (PR) {
-    float b = basep;
-    if (a.base_vec) b += a.base_vec[r];
-    const float v = b + a.damping * sum;
-    y[r] = v;
-    a.contrib_new[r] = deg ? v / (float)deg : 0.f;
-    if (deg == 0) dang += (double)v;
-    l1 += fabs((double)v - (double)rold);
-  } else {
-    y[r] = sum;
   }
 }
 
@@ -693,6 +655,15 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
   }
   EG_TRY(eg_pr_state(g, iters));
   const uint32_t n = g->n;
+  // Binned form: every row sum is at most the total rank mass, which never
+  // exceeds max(init * n, 1) (each iteration maps mass R to at most
+  // (1 - d) + d R); the margin covers the FP32 rounding of the iterates.
+  const bool binned = eg_pb_selected(g);
+  int S = 0;
+  if (binned) {
+    EG_TRY(eg_pb_build(g, ptr, idx));
+    S = eg_pb_scale(std::max(init * (double)n, 1.0) * 1.001);
+  }
   float *rank = g->rank_a, *rank_next = g->rank_b;
   float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
   EG_TRY(eg_pr_start(g, init, deg, rank, contrib));
@@ -708,7 +679,10 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
     a.contrib_new = contrib_next;
     a.dmass_in = g->pr_stats + 2 * it;
     a.stats_out = g->pr_stats + 2 * (it + 1);
-    EG_TRY(launch_pull(g, true, ptr, idx, nullptr, contrib, rank_next, *pl, a));
+    if (binned)
+      EG_TRY(eg_pb_iterate(g, contrib, rank_next, a, S, nullptr));
+    else
+      EG_TRY(launch_pull(g, true, ptr, idx, nullptr, contrib, rank_next, *pl, a));
     std::swap(rank, rank_next);
     std::swap(contrib, contrib_next);
   }

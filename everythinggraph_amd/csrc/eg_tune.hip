@@ -17,7 +17,15 @@
 //                   (0 = 64 MiB); not timed by eg_autotune;
 //  * graphs      -- 1: batched PPR iterations are replayed from a hipGraph;
 //  * grid_band   -- grid pull: source partitions per launch (0 = all), so that
-//                   one launch gathers from a band of x that fits in L2.
+//                   one launch gathers from a band of x that fits in L2;
+//  * pr_binned   -- PageRank by the binned two-phase kernels (1) or the pull
+//                   kernel (0), timed against the best pull shape; the two sum
+//                   in different arithmetic (exact fixed point vs compensated
+//                   FP32), so this is the one choice that changes result bits;
+//  * pr_tile     -- binned PageRank: source offsets per scatter workgroup
+//                   (0 = 32768); not timed by eg_autotune;
+//  * pr_scatter  -- binned PageRank scatter: source offsets loaded per lane
+//                   before any value is written, 8, 16 or 32.
 // eg_autotune times every variant on the graph itself through the public
 // entry points (device time from HIP events, best of a few repetitions) and
 // keeps the fastest.
@@ -38,6 +46,9 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "graphs")) return &g->tune.graphs;
   if (!strcmp(name, "grid_band")) return &g->tune.grid_band;
   if (!strcmp(name, "ingest_kib")) return &g->tune.ingest_kib;
+  if (!strcmp(name, "pr_binned")) return &g->tune.pr_binned;
+  if (!strcmp(name, "pr_tile")) return &g->tune.pr_tile;
+  if (!strcmp(name, "pr_scatter")) return &g->tune.pr_scatter;
   return nullptr;
 }
 
@@ -83,6 +94,14 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
     if (value < 0) return eg_fail(EG_EINVAL, "ingest_kib takes a size >= 0");
   } else if (!strcmp(param, "grid_band")) {
     if (value < 0) return eg_fail(EG_EINVAL, "grid_band takes a partition count >= 0");
+  } else if (!strcmp(param, "pr_tile")) {
+    if (value < 0 || (value > 0 && value < 1024))
+      return eg_fail(EG_EINVAL, "pr_tile takes 0 or a length >= 1024");
+  } else if (!strcmp(param, "pr_scatter")) {
+    if (value != 8 && value != 16 && value != 32)
+      return eg_fail(EG_EINVAL, "pr_scatter takes 8, 16 or 32");
+  } else if (!strcmp(param, "pr_binned")) {
+    if (value < -1 || value > 1) return eg_fail(EG_EINVAL, "pr_binned takes -1, 0 or 1");
   } else if (!strcmp(param, "graphs")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "graphs takes 0 or 1");
   } else if (value != 8 && value != 16) {
@@ -124,6 +143,7 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
   if (kernels & EG_TUNE_PULL) {
     if (!adj) return eg_fail(EG_ENOTSUP, "pull tuning needs adjacency lists");
     eg_pr_opts o = {0.85, 5, EG_PR_DANGLING_DROP, 0.0, EG_LAYOUT_ADJ};
+    g->tune.pr_binned = 0;  // the pull shapes are timed on the pull kernel
     for (int v : variants) {
       g->tune.pull_batch = v;
       double ms;
@@ -162,6 +182,20 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
         }
         res.push_back({"pull_persist", pp, ms / o.iterations});
       }
+      double best_pp = 1e300;
+      for (const Result &r : res)
+        if (!strcmp(r.param, "pull_persist") && r.ms < best_pp) {
+          best_pp = r.ms;
+          g->tune.pull_persist = r.value;
+        }
+    }
+    // the binned two-phase kernels against the best pull shape
+    for (int pb = 0; pb <= 1; pb++) {
+      g->tune.pr_binned = pb;
+      double ms;
+      EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // warm (binned plan)
+      EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); }, &ms));
+      res.push_back({"pr_binned", pb, ms / o.iterations});
     }
   }
   std::vector<uint32_t> roots;

@@ -224,11 +224,16 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * BFS index pass depth), each 8 or 16; "pull_hot" (sources cached normally in
  * the degree-ordered pull, a vertex count, 0 = all); "pull_persist" (0, or 1-2
  * persistent pull workgroups per CU keeping the highest-degree sources' values
- * in LDS); "grid_band" (grid source
+ * in LDS); "pr_binned" (PageRank on adjacency lists: 1 = binned two-phase
+ * kernels whose row sums are exact in 64-bit fixed point, 0 = the pull kernel
+ * with compensated FP32 sums, -1 = binned from 2^22 stored entries up; the two
+ * forms round differently, each is deterministic); "pr_tile" (source offsets
+ * per binned scatter workgroup, 0 = 32768); "pr_scatter" (binned scatter
+ * source offsets in flight per lane, 8, 16 or 32); "grid_band" (grid source
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
  * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
  * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant
- * computes bit-identical results.  eg_autotune times the variants of the selected
+ * except pr_binned computes bit-identical results.  eg_autotune times the variants of the selected
  * kernels on the graph itself, keeps the fastest, and reports each timing. */
 #define EG_TUNE_PULL 1u
 #define EG_TUNE_SPMM 2u
@@ -242,6 +247,12 @@ typedef struct {
 } eg_tune_entry;
 int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, int32_t cap,
                 int32_t *count);
+/* Geometry of the binned PageRank plan of g (built by the first eg_pagerank
+ * call that uses the binned kernels; no reference counterpart): out[0..5] =
+ * destination bins, source chunks, scatter tiles, accumulation work items,
+ * rows summed in parts, non-empty (chunk, bin) segments.  EG_ESTATE if no
+ * plan exists. */
+int eg_pr_binned_geometry(eg_graph *g, uint32_t *out);
 int eg_tune_set(eg_graph *g, const char *param, int32_t value);
 int eg_tune_get(eg_graph *g, const char *param, int32_t *value);
 

@@ -423,10 +423,12 @@ def test_autotune_reports_and_keeps_fastest():
         g.load_edges(s0, d0)
         rep = g.autotune()
         params = {r[0] for r in rep}
-        assert params == {"pull_batch", "pull_hot", "pull_persist", "spmm_depth", "ms_batch"}
+        assert params == {"pull_batch", "pull_hot", "pull_persist", "pr_binned", "spmm_depth",
+                          "ms_batch"}
+        assert sorted(r[1] for r in rep if r[0] == "pr_binned") == [0, 1]
         hot = sorted(r[1] for r in rep if r[0] == "pull_hot")
         assert hot == [0, n // 64, n // 16, n // 4]
-        for p in ("pull_batch", "spmm_depth", "ms_batch", "pull_hot", "pull_persist"):
+        for p in ("pull_batch", "spmm_depth", "ms_batch", "pull_hot", "pull_persist", "pr_binned"):
             rows = [r for r in rep if r[0] == p]
             assert all(r[2] > 0 for r in rows)
             if p in ("pull_batch", "spmm_depth", "ms_batch"):

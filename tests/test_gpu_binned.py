@@ -1,0 +1,149 @@
+"""Binned (two-phase) PageRank on the device vs the FP64 oracle.
+
+The binned kernels sum every row exactly in 64-bit fixed point, so their
+ranks must match the oracle to the PageRank tolerance (L1 <= 1e-6, max error
+<= 1e-6 of the largest rank), repeat bit for bit, and not depend on how rows
+are cut into bins, sources into chunks, or vertices ordered: those cuts only
+change the order of integer additions.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import eg_oracle_py as O
+import everythinggraph_amd as eg
+
+pytestmark = pytest.mark.gpu
+
+PR_L1_TOL = 1e-6
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def rmat(scale, ef=16, seed=1):
+    return O.gen_rmat(scale, ef << scale, seed=seed)
+
+
+def oracle_pr(n, s, d, undirected, iters, dangling="drop", init=0.0):
+    if undirected:
+        s, d, _ = O.symmetrize(s, d)
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    deg = np.diff(ptr).astype(np.uint32)
+    kw = {"dangling_uniform": dangling == "uniform"}
+    if init:
+        kw["init"] = init
+    return O.pagerank_pull(n, cp, row, deg, 0.85, iters, **kw)
+
+
+@pytest.mark.parametrize("scale,undirected", [(10, False), (12, True), (16, False), (17, True)])
+@pytest.mark.parametrize("dangling", ["drop", "uniform"])
+def test_binned_pagerank_matches_oracle(scale, undirected, dangling):
+    n = 1 << scale
+    s0, d0 = rmat(scale, seed=scale)
+    ref, rdh, rlh = oracle_pr(n, s0, d0, undirected, 20, dangling)
+    with eg.Graph(n, 2, undirected=undirected) as g:
+        g.load_edges(s0, d0)
+        g.tune_set("pr_binned", 1)
+        r, dh, lh = g.pagerank(0.85, 20, dangling=dangling)
+        r2, _, _ = g.pagerank(0.85, 20, dangling=dangling)
+        geo = g.binned_geometry()
+        g.tune_set("pr_binned", 0)
+        rp, _, _ = g.pagerank(0.85, 20, dangling=dangling)
+    err = np.abs(r.astype(np.float64) - ref)
+    assert err.sum() <= PR_L1_TOL, err.sum()
+    assert err.max() <= 1e-6 * ref.max()
+    assert np.array_equal(r, r2)  # deterministic
+    np.testing.assert_allclose(dh, rdh, rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(lh, rlh, rtol=1e-3, atol=1e-7)
+    # the pull kernel agrees to FP32 rounding; the exact sums are no further
+    # from the FP64 oracle than the compensated ones
+    assert np.abs(r.astype(np.float64) - rp).sum() <= PR_L1_TOL
+    assert err.sum() <= np.abs(rp.astype(np.float64) - ref).sum() * 1.5 + 1e-9
+    assert geo["bins"] >= 1 and geo["chunks"] >= 1 and geo["items"] >= geo["bins"]
+
+
+def test_binned_reference_defaults():
+    """prev = 0.15 per vertex and 10 iterations (pagerank_simple.c:95,115):
+    the rank mass starts at 0.15 n, and the fixed-point scale must hold it."""
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=7)
+    ref, _, _ = oracle_pr(n, s0, d0, False, 10, init=0.15)
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s0, d0)
+        g.tune_set("pr_binned", 1)
+        r, _, _ = g.pagerank(0.85, 10, init=0.15)
+    np.testing.assert_allclose(r, ref, rtol=2e-6, atol=1e-9)
+
+
+def test_binned_closed_forms():
+    """Directed cycle: every rank stays 1/n.  Star into a hub (a row with
+    n - 1 entries, summed in parts): the hub's rank is (1-d)/n + d (n-1)/n *
+    r_leaf with leaves at (1-d)/n after the first iteration."""
+    n = 4096
+    src = np.arange(n, dtype=np.uint32)
+    dst = ((src + 1) % n).astype(np.uint32)
+    with eg.Graph(n, 2) as g:
+        g.load_edges(src, dst)
+        g.tune_set("pr_binned", 1)
+        r, _, _ = g.pagerank(0.85, 20)
+    np.testing.assert_allclose(r, 1.0 / n, rtol=1e-6)
+
+    n = 1 << 15
+    leaves = np.arange(1, n, dtype=np.uint32)
+    with eg.Graph(n, 2) as g:
+        g.load_edges(leaves, np.zeros(n - 1, np.uint32))
+        g.tune_set("pr_binned", 1)
+        r, _, _ = g.pagerank(0.85, 5)
+    base = 0.15 / n
+    np.testing.assert_allclose(r[1:], base, rtol=1e-6)
+    np.testing.assert_allclose(r[0], base + 0.85 * (n - 1) * base, rtol=1e-6)
+
+
+_GEOMETRY_SCRIPT = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[1] + '/tests')
+import eg_oracle_py as O, everythinggraph_amd as eg
+n = 1 << 13
+s, d = O.gen_rmat(13, 16 << 13, seed=31)
+out = {}
+for und in (False, True):
+    with eg.Graph(n, 2, undirected=und) as g:
+        g.load_edges(s, d)
+        g.tune_set("pr_binned", 1)
+        out['pr%d' % und] = g.pagerank(0.85, 15)[0]
+        out['uni%d' % und] = g.pagerank(0.85, 15, dangling="uniform")[0]
+        geo = g.binned_geometry()
+        out['geo%d' % und] = np.array([geo[k] for k in ("bins", "chunks", "tiles", "items",
+                                                       "long_rows")])
+np.savez(sys.argv[2], **out)
+"""
+
+
+def test_binned_independent_of_cut_and_order(tmp_path):
+    """The same ranks, bit for bit, with the default bins and chunks, with
+    tiny ones (long rows in parts, tiles cut by length and by bins), and with
+    or without the degree-ordered copy."""
+    script = tmp_path / "run.py"
+    script.write_text(_GEOMETRY_SCRIPT)
+    res = {}
+    for geo in ("default", "small"):
+        for order in ("none", "degree"):
+            env = dict(os.environ, EG_PULL_ORDER=order, EG_PB_GEOMETRY=geo)
+            outp = tmp_path / f"{geo}_{order}.npz"
+            print(f"binned geometry={geo} order={order}", flush=True)
+            subprocess.run([sys.executable, str(script), ROOT, str(outp)], env=env, check=True,
+                           timeout=120)
+            res[(geo, order)] = np.load(outp)
+    first = res[("default", "none")]
+    for key, r in res.items():
+        for k in ("pr0", "pr1", "uni0", "uni1"):
+            assert np.array_equal(r[k], first[k]), (key, k)
+    for und in (0, 1):
+        small = res[("small", "degree")][f"geo{und}"]
+        bins, chunks, tiles, items, long_rows = (int(v) for v in small)
+        assert long_rows > 0 and items > bins  # hub rows summed in parts
+        assert tiles > chunks                  # chunk regions cut into several tiles
+        assert bins > 16                       # tiles also cut by the bins they span
