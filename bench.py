@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
 """Benchmark of the EverythingGraph data-parallel path on one MI355X.
 
-Workload (one step): a full PageRank, 20 iterations, FP32 with compensated
-sums, r0 = 1/n, d = 0.85, over an undirected R-MAT scale-24 graph (edge factor
-16, a,b,c = 0.57,0.19,0.19, seed 1; 2^24 vertices, 536,870,912 stored
-entries), pull over the CSR of the symmetric graph, inputs resident in HBM.
+Workload (one step): a full PageRank, 20 iterations, FP32 iterates, r0 = 1/n,
+d = 0.85, over an undirected R-MAT scale-24 graph (edge factor 16, a,b,c =
+0.57,0.19,0.19, seed 1; 2^24 vertices, 536,870,912 stored entries) on the CSR
+of the symmetric graph, inputs resident in HBM, with the kernels the
+autotuner picks (the binned two-phase kernels with exact fixed-point row
+sums, or the pull kernel with compensated FP32 sums).
 `value` is algorithmic GB/s = iterations * B_PR / step time with
 B_PR = 4m + 4(n+1) + 16n bytes per iteration.
 
@@ -79,7 +81,10 @@ def main():
     ap.add_argument("--bfs-roots", type=int, default=64)
     ap.add_argument("--batch-bfs", type=int, default=256)
     ap.add_argument("--batch-ppr", type=int, default=64)
-    ap.add_argument("--cpu-iterations", type=int, default=2)
+    ap.add_argument("--cpu-iterations", type=int, default=2,
+                    help="least number of CPU oracle iterations")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU work the oracle sample is sized to")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-qps", type=float, action="append", default=None,
@@ -132,8 +137,8 @@ def main():
     for kv in args.tune:
         p, v = kv.split("=")
         g.tune_set(p, int(v))
-    tuned = {p: g.tune_get(p) for p in ("pull_batch", "pull_hot", "pull_persist", "spmm_depth",
-                                        "ms_batch")}
+    tuned = {p: g.tune_get(p) for p in ("pr_binned", "pr_scatter", "pull_batch", "pull_hot",
+                                        "pull_persist", "spmm_depth", "ms_batch")}
 
     # ---- PageRank: warmup + timed steps ----
     iters = args.iterations
@@ -152,13 +157,29 @@ def main():
     g.prof_enable(False)
     ms_per_step = 1e3 * t_steps / args.steps
     value = iters * b_pr / (ms_per_step * 1e-3) / 1e9
-    launches, tot = prof.get("pr_pull", (0, 0.0))
-    k_ms = tot / max(launches, 1)
+
+    def avg_ms(name):
+        launches, tot = prof.get(name, (0, 0.0))
+        return tot / launches if launches else 0.0
+
+    # The dominant work of one iteration: the binned kernels (scatter +
+    # accumulate + finish, three launches) or the pull kernel (one launch).
+    binned = "pr_scatter" in prof
+    if binned:
+        parts = {k: avg_ms(k) for k in ("pr_scatter", "pr_accum", "pr_finish")}
+        kernel = "k_pb_scatter + k_pb_accum + k_pb_finish (one binned iteration)"
+        tname = ("pr_scatter", "pr_accum")
+    else:
+        parts = {"pr_pull": avg_ms("pr_pull")}
+        kernel = "k_pull<true,false> (pr_pull)"
+        tname = ("pr_pull",)
+    k_ms = sum(parts.values())
     achieved = b_pr / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
-    fin_l, fin_tot = prof.get("pr_finish", (0, 0.0))
-    log(f"pagerank: {ms_per_step:.3f} ms/step ({iters} it), {value:.1f} GB/s; pr_pull "
-        f"{k_ms * 1e3:.1f} us/launch = {achieved:.1f} GB/s; pr_finish "
-        f"{fin_tot / max(fin_l, 1) * 1e3:.1f} us; final L1 delta {lh[-1]:.3e}")
+    traffic = [measured_traffic(t, f"rmat{scale}_ef{ef}_u_s") for t in tname]
+    traffic = None if any(t is None for t in traffic) else sum(traffic)
+    log(f"pagerank: {ms_per_step:.3f} ms/step ({iters} it), {value:.1f} GB/s; iteration "
+        + " + ".join(f"{k} {v * 1e3:.1f} us" for k, v in parts.items())
+        + f" = {achieved:.1f} GB/s; final L1 delta {lh[-1]:.3e}")
 
     # ---- BFS over 64 seeded roots ----
     deg, _ = g.degrees()
@@ -249,12 +270,18 @@ def main():
         ptr, col, _ = g.csr()
         odeg = np.diff(ptr).astype(np.uint32)
         threads = O.L().ego_num_threads()
+        # a bounded sample of about --cpu-seconds of CPU work: a short probe
+        # sizes the iteration count, then that many iterations are timed
         c0 = time.perf_counter()
-        O.pagerank_pull(n, ptr, col, odeg, 0.85, args.cpu_iterations)
+        O.pagerank_pull(n, ptr, col, odeg, 0.85, 2)
+        probe = (time.perf_counter() - c0) / 2
+        cpu_iters = max(args.cpu_iterations, min(400, int(args.cpu_seconds / max(probe, 1e-3))))
+        c0 = time.perf_counter()
+        O.pagerank_pull(n, ptr, col, odeg, 0.85, cpu_iters)
         ct = time.perf_counter() - c0
-        cpu_gbs = args.cpu_iterations * b_pr / ct / 1e9
+        cpu_gbs = cpu_iters * b_pr / ct / 1e9
         cpu = {"value": round(cpu_gbs, 3), "unit": "GB/s", "cores": int(threads), "kind": "port",
-               "sample": f"{args.cpu_iterations} FP64 pull-PageRank iterations of the oracle "
+               "sample": f"{cpu_iters} FP64 pull-PageRank iterations of the oracle "
                          f"(OpenMP) over the same R-MAT {scale} graph, {ct:.1f} s"}
         log(f"cpu oracle: {cpu_gbs:.2f} GB/s on {threads} threads ({ct:.1f} s)")
         del ptr, col
@@ -281,9 +308,9 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": measured_traffic("pr_pull", f"rmat{scale}_ef{ef}_u_s"),
-            "kernel": "k_pull<true,false> (pr_pull)", "bytes_per_launch": b_pr,
+            "traffic": traffic, "kernel": kernel, "bytes_per_launch": b_pr,
             "avg_launch_us": round(k_ms * 1e3, 2),
+            "per_kernel_us": {k: round(v * 1e3, 2) for k, v in parts.items()},
         },
         "cpu_baseline": cpu,
         "bfs_gteps": round(gteps, 3),

@@ -224,6 +224,8 @@ struct Ppr1 {
   int it = 0, max_iter = 0;
   hipStream_t stream = nullptr;
   double *h_delta = nullptr;  // pinned, owned by the caller
+  bool binned = false;        // iterations on the binned kernels
+  int scale = 0;              // their fixed-point scale
 };
 int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, double alpha,
                   int max_iter, hipStream_t stream, double *h_delta);

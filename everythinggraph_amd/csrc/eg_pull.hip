@@ -737,6 +737,12 @@ int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, dou
   }
   EG_TRY(eg_pr_state(g, max_iter));
   const uint32_t n = g->n;
+  // the iterate is a probability vector: row sums stay below 1
+  st->binned = eg_pb_selected(g);
+  if (st->binned) {
+    EG_TRY(eg_pb_build(g, st->ptr, st->idx));
+    st->scale = eg_pb_scale(1.001);
+  }
   hipStream_t s = stream;
   st->stream = s;
   st->h_delta = h_delta;
@@ -786,8 +792,11 @@ int eg_ppr1_step(eg_graph *g, Ppr1 *st) {
   a.contrib_new = st->contrib_next;
   a.dmass_in = g->pr_stats + 2 * st->it;
   a.stats_out = g->pr_stats + 2 * (st->it + 1);
-  EG_TRY(launch_pull(g, true, st->ptr, st->idx, nullptr, st->contrib, st->rank_next, *st->pl, a,
-                     st->stream));
+  if (st->binned)
+    EG_TRY(eg_pb_iterate(g, st->contrib, st->rank_next, a, st->scale, st->stream));
+  else
+    EG_TRY(launch_pull(g, true, st->ptr, st->idx, nullptr, st->contrib, st->rank_next, *st->pl, a,
+                       st->stream));
   std::swap(st->rank, st->rank_next);
   std::swap(st->contrib, st->contrib_next);
   st->it++;

@@ -211,8 +211,8 @@ int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots,
 int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
                  const uint32_t *seeds, double alpha, int32_t iterations,
                  float *out, double *device_ms);
-/* One personalized PageRank vector on the single-vector pull kernel (the
- * plain per-query path): x0 = p, iterated as eg_ppr_batch until the L1 change
+/* One personalized PageRank vector on the single-vector PageRank kernels
+ * (the plain per-query path; binned or pull as "pr_binned" selects): x0 = p, iterated as eg_ppr_batch until the L1 change
  * of an iteration is <= tol or max_iter iterations ran.  out (may be NULL):
  * n floats.  iterations / delta (may be NULL): iterations run, last L1 change. */
 int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, double tol,
@@ -224,7 +224,8 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * BFS index pass depth), each 8 or 16; "pull_hot" (sources cached normally in
  * the degree-ordered pull, a vertex count, 0 = all); "pull_persist" (0, or 1-2
  * persistent pull workgroups per CU keeping the highest-degree sources' values
- * in LDS); "pr_binned" (PageRank on adjacency lists: 1 = binned two-phase
+ * in LDS); "pr_binned" (PageRank and single-vector PPR on adjacency lists:
+ * 1 = binned two-phase
  * kernels whose row sums are exact in 64-bit fixed point, 0 = the pull kernel
  * with compensated FP32 sums, -1 = binned from 2^22 stored entries up; the two
  * forms round differently, each is deterministic); "pr_tile" (source offsets
@@ -233,8 +234,9 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
  * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
  * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant
- * except pr_binned computes bit-identical results.  eg_autotune times the variants of the selected
- * kernels on the graph itself, keeps the fastest, and reports each timing. */
+ * except pr_binned computes bit-identical results.  eg_autotune times the
+ * variants of the selected kernels on the graph itself, keeps the fastest, and
+ * reports each timing. */
 #define EG_TUNE_PULL 1u
 #define EG_TUNE_SPMM 2u
 #define EG_TUNE_MSBFS 4u
@@ -248,7 +250,8 @@ typedef struct {
 int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, int32_t cap,
                 int32_t *count);
 /* Geometry of the binned PageRank plan of g (built by the first eg_pagerank
- * call that uses the binned kernels; no reference counterpart): out[0..5] =
+ * or eg_ppr call that uses the binned kernels; no reference counterpart):
+ * out[0..5] =
  * destination bins, source chunks, scatter tiles, accumulation work items,
  * rows summed in parts, non-empty (chunk, bin) segments.  EG_ESTATE if no
  * plan exists. */

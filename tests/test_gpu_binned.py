@@ -102,6 +102,30 @@ def test_binned_closed_forms():
     np.testing.assert_allclose(r[0], base + 0.85 * (n - 1) * base, rtol=1e-6)
 
 
+@pytest.mark.parametrize("undirected", [False, True])
+def test_binned_single_vector_ppr(undirected):
+    """The plain per-query PPR on the binned kernels: within 1e-6 L1 of the
+    FP64 oracle at the same iteration count, and of the pull kernel."""
+    n = 1 << 13
+    s0, d0 = rmat(13, seed=41)
+    s, d = (O.symmetrize(s0, d0)[:2]) if undirected else (s0, d0)
+    ptr, col, _ = O.build_csr(n, s, d)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    deg = np.diff(ptr).astype(np.uint32)
+    with eg.Graph(n, 2, undirected=undirected) as g:
+        g.load_edges(s0, d0)
+        out = {}
+        for pb in (1, 0):
+            g.tune_set("pr_binned", pb)
+            out[pb] = [g.ppr(sd, 0.15, tol=0.0, max_iter=30) for sd in ([3], [1, 2, 2, 900])]
+    for q, sd in enumerate(([3], [1, 2, 2, 900])):
+        ref = O.ppr(n, cp, row, deg, sd, 0.15, 30)
+        r, it, _ = out[1][q]
+        assert it == 30
+        assert np.abs(r - ref).sum() <= PR_L1_TOL
+        assert np.abs(r.astype(np.float64) - out[0][q][0]).sum() <= PR_L1_TOL
+
+
 _GEOMETRY_SCRIPT = r"""
 import sys, numpy as np
 sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[1] + '/tests')
