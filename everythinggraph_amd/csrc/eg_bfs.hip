@@ -24,8 +24,10 @@
 //    per-workgroup partials reduced by one small kernel.
 //  * The queue holds only vertices with out-edges: a vertex without out-edges
 //    is never an in-neighbour, so it cannot discover anything either way.
-//  * The host reads the level's counters (frontier size, its out-degree sum,
-//    queue length) and applies the reference's switch rule.
+//  * Adjacency layouts run one k_bfs_level launch per level that applies the
+//    reference's switch rule on the device (below); the grid, edge-array and
+//    SSSP paths read the level's counters (frontier size, its out-degree sum,
+//    queue length) on the host and apply the rule there.
 #include <string.h>
 
 #include "eg_internal.h"
