@@ -53,6 +53,8 @@ constexpr int kWinPer = (kChunkSrc + kBlock - 1) / kBlock;     // window loads p
 constexpr int kTabPer = (kTileBins + 1 + kBlock - 1) / kBlock;  // table loads per thread
 constexpr int kRowPer = (kBinRows + kBlock - 1) / kBlock;       // rows finished per thread
 constexpr int kFinBlocks = 128;
+constexpr uint32_t kHotMax = 16384;    // hot sources whose values phase 2 keeps in LDS (64 KB)
+constexpr int kHotPer = (kHotMax + kBlock - 1) / kBlock;        // hot loads per thread
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
@@ -87,10 +89,10 @@ struct PbTile {
 };
 
 // Phase-2 work item: bin rows [row0, row0 + nrows) with bin-major entries
-// [e0, e1); nrows == 0 marks a part of a long row whose sum goes to
-// part_sum[row0].
+// [e0, e1), of which [e0, eh) are hot; nrows == 0 marks a part of a long row
+// whose sum goes to part_sum[row0].
 struct PbItem {
-  uint32_t e0, e1, row0, nrows;
+  uint32_t e0, eh, e1, row0, nrows, pad[3];
 };
 
 struct PbDev {
@@ -107,21 +109,27 @@ struct PbDev {
   double2 *fin_part;
   uint32_t *fin_counter;
   uint32_t nbins, nitems, nlong;
+  uint32_t nhot;           // sources below this id are hot
+  uint32_t persistent;     // phase 2: workgroups loop over items (else one each)
 };
 
 // ---------------------------------------------------------------------------
 // Build kernels.
 // ---------------------------------------------------------------------------
-// key / payload of every entry, a wave per row (grid-stride over rows).
-// bin_major: key = bin * nchunks + chunk, payload = row - first row of bin;
-// else key = chunk * nbins + bin, payload = source - first source of chunk.
+// key / payloads of every entry, a wave per row (grid-stride over rows).
+// Sources below nhot are hot: they belong to no chunk.
+// bin_major: key = bin * (nchunks + 1) + (hot ? 0 : 1 + chunk), payloads =
+//   row - first row of the bin, and the source;
+// else key = hot ? nchunks * nbins : chunk * nbins + bin, payload = source -
+//   first source of the chunk (hot entries sort last and are not used).
 __global__ void k_pb_keys(uint32_t n, const uint32_t *__restrict__ ptr,
                           const uint32_t *__restrict__ idx, const uint32_t *__restrict__ bin_of,
                           const uint32_t *__restrict__ chunk_of,
                           const uint32_t *__restrict__ bin_row0,
                           const uint32_t *__restrict__ chunk_src0, uint32_t nbins,
-                          uint32_t nchunks, int bin_major, uint32_t *__restrict__ key,
-                          uint32_t *__restrict__ pay) {
+                          uint32_t nchunks, uint32_t nhot, int bin_major,
+                          uint32_t *__restrict__ key, uint32_t *__restrict__ pay,
+                          uint32_t *__restrict__ pay2) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   const int lane = __lane_id();
@@ -131,22 +139,30 @@ __global__ void k_pb_keys(uint32_t n, const uint32_t *__restrict__ ptr,
     const uint32_t b = bin_of[v];
     for (uint32_t k = k0 + lane; k < k1; k += 64) {
       const uint32_t u = idx[k];
-      const uint32_t c = chunk_of[u];
+      const bool hot = u < nhot;
+      const uint32_t c = hot ? 0u : chunk_of[u];
       if (bin_major) {
-        key[k] = b * nchunks + c;
+        key[k] = b * (nchunks + 1) + (hot ? 0u : 1u + c);
         pay[k] = v - bin_row0[b];
+        pay2[k] = u;
       } else {
-        key[k] = c * nbins + b;
-        pay[k] = u - chunk_src0[c];
+        key[k] = hot ? nchunks * nbins : c * nbins + b;
+        pay[k] = hot ? 0u : u - chunk_src0[c];
       }
     }
   }
 }
 
+// u16 offsets of the sorted order; for the bin-major order also the packed
+// (source | row offset << 16) value slots of the hot entries.
 __global__ void k_pb_narrow(uint64_t m, const uint32_t *__restrict__ in,
-                            uint16_t *__restrict__ out) {
+                            uint16_t *__restrict__ out, const uint32_t *__restrict__ skey,
+                            const uint32_t *__restrict__ src, uint32_t key_mod,
+                            uint32_t *__restrict__ packed) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < m) out[k] = (uint16_t)in[k];
+  if (k >= m) return;
+  out[k] = (uint16_t)in[k];
+  if (packed && skey[k] % key_mod == 0) packed[k] = src[k] | (in[k] << 16);
 }
 
 // ---------------------------------------------------------------------------
@@ -228,34 +244,46 @@ __global__ __launch_bounds__(kBlock) void k_pb_scatter(PbDev d, const float *__r
 // ---------------------------------------------------------------------------
 // Phase 2: per-bin accumulation and the PageRank update.
 // ---------------------------------------------------------------------------
-// A lane takes groups of 8 consecutive entries (one 16-byte load of row
-// offsets, two of values), sums runs of equal rows in registers and adds each
-// run to its LDS accumulator with one atomic.  Entries of a (chunk, bin)
-// segment are in row order, so a hub row's entries form long runs and cost
-// about one atomic per group.  Groups are aligned to 8 entries; entries of a
-// group outside [e0, e1) are masked.
-constexpr int kGroupsInFlight = 2;  // groups loaded per lane before any is summed
+// A lane takes groups of 8 consecutive entries, sums runs of equal rows in
+// registers and adds each run to its LDS accumulator with one atomic.  Entries
+// of a segment are in row order, so a hub row's entries form long runs and
+// cost about one atomic per group.  Groups are aligned to 8 entries; entries
+// of a group outside the range being summed are masked.
+//  * cold entries: one 16-byte load of row offsets, two of values;
+//  * hot entries (sources below the hot threshold, first in every bin): their
+//    value slot holds (source | row offset << 16) from the build, and the
+//    value is read from the LDS copy of the hot sources' contributions, so
+//    phase 1 never touches them (two 16-byte loads per group).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void group_accumulate(unsigned long long *acc, const u32x4 dl,
-                                                 const f32x4 v0, const f32x4 v1,
+// Adds the 8 entries of the group that starts at entry e (those outside
+// [e0, e1) masked): HOTRANGE, w0 / w1 hold packed (source | row offset << 16)
+// words; otherwise w0 holds 8 u16 row offsets and v0 / v1 the values.  part:
+// the whole range belongs to one row and is summed into psum.
+template <bool HOTRANGE>
+__device__ __forceinline__ void group_accumulate(unsigned long long *acc, const u32x4 w0,
+                                                 const u32x4 w1, const f32x4 v0, const f32x4 v1,
                                                  uint32_t e, uint32_t e0, uint32_t e1, int S,
-                                                 bool part, unsigned long long &psum) {
-  const uint32_t key[8] = {dl.x & 0xFFFFu, dl.x >> 16, dl.y & 0xFFFFu, dl.y >> 16,
-                           dl.z & 0xFFFFu, dl.z >> 16, dl.w & 0xFFFFu, dl.w >> 16};
-  const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                                                 bool part, const float *s_hot,
+                                                 unsigned long long &psum) {
+  const uint32_t ws[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+  const uint32_t ks[8] = {w0.x & 0xFFFFu, w0.x >> 16, w0.y & 0xFFFFu, w0.y >> 16,
+                          w0.z & 0xFFFFu, w0.z >> 16, w0.w & 0xFFFFu, w0.w >> 16};
+  const float vs[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
   uint32_t cur = 0xFFFFFFFFu;
   unsigned long long run = 0ull;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     const bool ok = e + j >= e0 && e + j < e1;
-    const unsigned long long f = ok ? to_fixed(v[j], S) : 0ull;
+    const uint32_t key = HOTRANGE ? ws[j] >> 16 : ks[j];
+    const float v = HOTRANGE ? (ok ? s_hot[ws[j] & 0xFFFFu] : 0.f) : vs[j];
+    const unsigned long long f = ok ? to_fixed(v, S) : 0ull;
     if (part) {
       psum += f;
       continue;
     }
-    const uint32_t k = ok ? key[j] : 0xFFFFFFFFu;
+    const uint32_t k = ok ? key : 0xFFFFFFFFu;
     if (k != cur) {
       if (cur != 0xFFFFFFFFu) atomicAdd(&acc[cur], run);
       cur = k;
@@ -266,96 +294,157 @@ __device__ __forceinline__ void group_accumulate(unsigned long long *acc, const 
   if (!part && cur != 0xFFFFFFFFu) atomicAdd(&acc[cur], run);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pb_accum(PbDev d, int S, float *__restrict__ y,
-                                                     PrArgs a) {
-  __shared__ unsigned long long s_acc[kBinRows];
-  __shared__ unsigned long long s_ps[kWaves];
-  __shared__ double s_d[kWaves], s_l[kWaves];
-  const PbItem it = d.items[blockIdx.x];
-  const uint32_t tid = threadIdx.x;
+// Sums the entries [e0, e1) of one item, each wave a contiguous range of
+// 8-entry groups, G groups loaded per lane before any is summed.  HOTRANGE:
+// the entries are hot (packed value slots).
+template <bool HOTRANGE, int G>
+__device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, uint32_t e1, int S,
+                                                 bool part, unsigned long long *acc,
+                                                 const float *s_hot,
+                                                 unsigned long long &psum) {
+  if (e0 >= e1) return;
   const int lane = __lane_id();
-  const uint32_t wave = tid >> 6;
-  const bool part = it.nrows == 0;
-  if (!part)
-    for (uint32_t i = tid; i < it.nrows; i += kBlock) s_acc[i] = 0ull;
-  __syncthreads();
-
-  // groups of 8 entries from the aligned start; a contiguous range per wave
-  const uint32_t g0 = it.e0 >> 3;
-  const uint32_t ng = ((it.e1 + 7) >> 3) - g0;
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t g0 = e0 >> 3;
+  const uint32_t ng = ((e1 + 7) >> 3) - g0;
   const uint32_t per = ((ng + kWaves - 1) / kWaves + 63) & ~63u;
   const uint32_t wg0 = g0 + min(ng, wave * per);
   const uint32_t wg1 = g0 + min(ng, (wave + 1) * per);
   const u32x4 *dl4 = reinterpret_cast<const u32x4 *>(d.dloc);
   const f32x4 *v4 = reinterpret_cast<const f32x4 *>(d.val);
-  unsigned long long psum = 0ull;
-  for (uint32_t base = wg0; base < wg1; base += 64 * kGroupsInFlight) {
-    u32x4 dl[kGroupsInFlight];
-    f32x4 va[kGroupsInFlight], vb[kGroupsInFlight];
+  const u32x4 *h4 = reinterpret_cast<const u32x4 *>(d.val);
+  const u32x4 z4 = u32x4{0u, 0u, 0u, 0u};
+  const f32x4 zf = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (uint32_t base = wg0; base < wg1; base += 64 * G) {
+    u32x4 wa[G], wb[G];
+    f32x4 va[G], vb[G];
 #pragma unroll
-    for (int q = 0; q < kGroupsInFlight; q++) {
+    for (int q = 0; q < G; q++) {
       const uint32_t gi = base + q * 64 + lane;
-      if (gi < wg1) {
-        dl[q] = __builtin_nontemporal_load(dl4 + gi);
-        va[q] = __builtin_nontemporal_load(v4 + 2 * (size_t)gi);
-        vb[q] = __builtin_nontemporal_load(v4 + 2 * (size_t)gi + 1);
+      const bool in = gi < wg1;
+      if (HOTRANGE) {
+        wa[q] = in ? __builtin_nontemporal_load(h4 + 2 * (size_t)gi) : z4;
+        wb[q] = in ? __builtin_nontemporal_load(h4 + 2 * (size_t)gi + 1) : z4;
+        va[q] = vb[q] = zf;
       } else {
-        dl[q] = u32x4{0u, 0u, 0u, 0u};
-        va[q] = vb[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+        wa[q] = in ? __builtin_nontemporal_load(dl4 + gi) : z4;
+        wb[q] = z4;
+        va[q] = in ? __builtin_nontemporal_load(v4 + 2 * (size_t)gi) : zf;
+        vb[q] = in ? __builtin_nontemporal_load(v4 + 2 * (size_t)gi + 1) : zf;
       }
     }
 #pragma unroll
-    for (int q = 0; q < kGroupsInFlight; q++) {
+    for (int q = 0; q < G; q++) {
       const uint32_t gi = base + q * 64 + lane;
-      if (gi < wg1) group_accumulate(s_acc, dl[q], va[q], vb[q], gi * 8, it.e0, it.e1, S, part, psum);
+      if (gi < wg1)
+        group_accumulate<HOTRANGE>(acc, wa[q], wb[q], va[q], vb[q], gi * 8, e0, e1, S, part,
+                                   s_hot, psum);
     }
   }
-  if (part) {
+}
+
+// HOT: persistent workgroups (one per CU) that first copy the hot sources'
+// contributions into LDS, then take items blockIdx.x, blockIdx.x + gridDim.x,
+// ...; otherwise one item per workgroup and no hot range.
+template <bool HOT>
+__device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restrict__ y,
+                                         const PrArgs &a, const float *__restrict__ contrib) {
+  __shared__ unsigned long long s_acc[kBinRows];
+  __shared__ float s_hot[HOT ? kHotMax : 1];
+  __shared__ unsigned long long s_ps[kWaves];
+  __shared__ double s_d[kWaves], s_l[kWaves];
+  const uint32_t tid = threadIdx.x;
+  const int lane = __lane_id();
+  const uint32_t wave = tid >> 6;
+  if (HOT) {
+    float hv[kHotPer];
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) psum += __shfl_xor(psum, o);
-    if (lane == 0) s_ps[wave] = psum;
+    for (int j = 0; j < kHotPer; j++) {
+      const uint32_t i = tid + j * kBlock;
+      hv[j] = i < d.nhot ? contrib[i] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kHotPer; j++) {
+      const uint32_t i = tid + j * kBlock;
+      if (i < d.nhot) s_hot[i] = hv[j];
+    }
+  }
+  // one item per workgroup unless the launch is persistent
+  const uint32_t item_end = d.persistent ? d.nitems : blockIdx.x + 1;
+  for (uint32_t item = blockIdx.x; item < item_end; item += gridDim.x) {
+    const PbItem it = d.items[item];
+    const bool part = it.nrows == 0;
+    if (!part)
+      for (uint32_t i = tid; i < it.nrows; i += kBlock) s_acc[i] = 0ull;
+    __syncthreads();  // accumulators cleared (and, the first time, the hot copy made)
+    unsigned long long psum = 0ull;
+    // the persistent form runs 16 waves per CU: more loads in flight per lane
+    constexpr int G = HOT ? 4 : 2;
+    if (HOT) accumulate_range<true, G>(d, it.e0, it.eh, S, part, s_acc, s_hot, psum);
+    accumulate_range<false, G>(d, HOT ? it.eh : it.e0, it.e1, S, part, s_acc, s_hot, psum);
+    if (part) {
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) psum += __shfl_xor(psum, o);
+      if (lane == 0) s_ps[wave] = psum;
+      __syncthreads();
+      if (tid == 0) {
+        unsigned long long t = 0ull;
+        for (int q = 0; q < kWaves; q++) t += s_ps[q];
+        d.part_sum[it.row0] = t;
+        d.item_part[item] = make_double2(0.0, 0.0);
+      }
+      __syncthreads();  // s_ps reuse
+      continue;
+    }
+    __syncthreads();
+    // per-row inputs of this thread's rows, every load issued before the first use
+    uint32_t dg[kRowPer];
+    float ro[kRowPer];
+#pragma unroll
+    for (int j = 0; j < kRowPer; j++) {
+      const uint32_t i = tid + j * kBlock;
+      dg[j] = i < it.nrows ? a.out_deg[it.row0 + i] : 0u;
+      ro[j] = i < it.nrows ? a.rank_old[it.row0 + i] : 0.f;
+    }
+    const float basep = pr_base(a);
+    double dang = 0.0, l1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < kRowPer; j++) {
+      const uint32_t i = tid + j * kBlock;
+      if (i < it.nrows)
+        finish_row<true>(it.row0 + i, from_fixed(s_acc[i], S), y, a, basep, dg[j], ro[j], dang,
+                         l1);
+    }
+    dang = eg_wave_sum_f64(dang);
+    l1 = eg_wave_sum_f64(l1);
+    if (lane == 0) {
+      s_d[wave] = dang;
+      s_l[wave] = l1;
+    }
     __syncthreads();
     if (tid == 0) {
-      unsigned long long t = 0ull;
-      for (int q = 0; q < kWaves; q++) t += s_ps[q];
-      d.part_sum[it.row0] = t;
-      d.item_part[blockIdx.x] = make_double2(0.0, 0.0);
+      double td = 0.0, tl = 0.0;
+      for (int q = 0; q < kWaves; q++) {
+        td += s_d[q];
+        tl += s_l[q];
+      }
+      d.item_part[item] = make_double2(td, tl);
     }
-    return;
+    __syncthreads();  // s_acc / s_d reuse by the next item
   }
-  // per-row inputs of this thread's rows, loaded before the LDS barrier
-  uint32_t dg[kRowPer];
-  float ro[kRowPer];
-#pragma unroll
-  for (int j = 0; j < kRowPer; j++) {
-    const uint32_t i = tid + j * kBlock;
-    dg[j] = i < it.nrows ? a.out_deg[it.row0 + i] : 0u;
-    ro[j] = i < it.nrows ? a.rank_old[it.row0 + i] : 0.f;
-  }
-  __syncthreads();
-  const float basep = pr_base(a);
-  double dang = 0.0, l1 = 0.0;
-#pragma unroll
-  for (int j = 0; j < kRowPer; j++) {
-    const uint32_t i = tid + j * kBlock;
-    if (i < it.nrows)
-      finish_row<true>(it.row0 + i, from_fixed(s_acc[i], S), y, a, basep, dg[j], ro[j], dang, l1);
-  }
-  dang = eg_wave_sum_f64(dang);
-  l1 = eg_wave_sum_f64(l1);
-  if (lane == 0) {
-    s_d[wave] = dang;
-    s_l[wave] = l1;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double td = 0.0, tl = 0.0;
-    for (int q = 0; q < kWaves; q++) {
-      td += s_d[q];
-      tl += s_l[q];
-    }
-    d.item_part[blockIdx.x] = make_double2(td, tl);
-  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pb_accum_hot(PbDev d, int S, float *__restrict__ y,
+                                                         PrArgs a,
+                                                         const float *__restrict__ contrib) {
+  pb_accum<true>(d, S, y, a, contrib);
+}
+
+// Two workgroups per CU: held to 64 VGPRs (8 waves per SIMD).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+k_pb_accum_cold(PbDev d, int S, float *__restrict__ y, PrArgs a,
+                const float *__restrict__ contrib) {
+  pb_accum<false>(d, S, y, a, contrib);
 }
 
 // Long rows (sum of their parts) and the iteration's FP64 statistics; the
@@ -425,6 +514,9 @@ struct PbPlan {
   const uint32_t *for_ptr = nullptr;  // orientation the plan was built for
   uint32_t nbins = 0, nchunks = 0, ntiles = 0, nitems = 0, nlong = 0, nparts = 0;
   uint32_t tile_len = 0;   // scatter tile length the plan was cut with
+  int hot_req = 0;         // hot setting the plan was built for
+  uint32_t nhot = 0;       // sources below this id are hot
+  uint32_t cus = 0;        // compute units (persistent accumulation grid)
   uint64_t nsegs = 0;      // non-empty (chunk, bin) segments
   PbTile *tiles = nullptr;
   uint2 *chunk_src = nullptr;
@@ -468,17 +560,19 @@ int upload(eg_graph *g, T **dst, const std::vector<T> &src, const char *what) {
   return EG_OK;
 }
 
-// Sorted keys and their u16 payloads of one order; seg receives the
-// exclusive scan of the segment sizes (nkeys + 1 entries) on the host.
+// Sorts the entries into one order and writes their u16 offsets (and, for
+// the bin-major order, the packed hot value slots); seg receives the start of
+// every key's segment (nkeys + 1 entries) on the host.
 int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32_t *bin_of,
              const uint32_t *chunk_of, const uint32_t *bin_row0, const uint32_t *chunk_src0,
-             uint32_t nbins, uint32_t nchunks, int bin_major, uint16_t *out,
-             std::vector<uint32_t> &seg) {
+             uint32_t nbins, uint32_t nchunks, uint32_t nhot, int bin_major, uint16_t *out,
+             uint32_t *packed, std::vector<uint32_t> &seg) {
   const uint32_t n = g->n;
   const uint64_t m = g->m;
   hipStream_t s = g->stream;
-  const uint64_t nkeys = (uint64_t)nbins * nchunks;
-  uint32_t *key = nullptr, *pay = nullptr, *skey = nullptr, *spay = nullptr, *segd = nullptr;
+  const uint64_t nkeys = bin_major ? (uint64_t)nbins * (nchunks + 1) : (uint64_t)nchunks * nbins + 1;
+  uint32_t *key = nullptr, *pay = nullptr, *pay2 = nullptr, *skey = nullptr, *spay = nullptr,
+           *spay2 = nullptr, *segd = nullptr;
   int rc = EG_OK;
   do {
     if ((rc = eg_dalloc_t(&key, m, "binned keys")) || (rc = eg_dalloc_t(&pay, m, "binned payload")) ||
@@ -486,21 +580,27 @@ int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32
         (rc = eg_dalloc_t(&spay, m, "binned sorted payload")) ||
         (rc = eg_dalloc_t(&segd, nkeys + 1, "binned segments")))
       break;
+    if (bin_major && ((rc = eg_dalloc_t(&pay2, m, "binned sources")) ||
+                      (rc = eg_dalloc_t(&spay2, m, "binned sorted sources"))))
+      break;
     const uint32_t grid = std::min<uint64_t>(blocks_for((uint64_t)n * 64, 256), 65536);
     hipLaunchKernelGGL(k_pb_keys, dim3(grid), dim3(256), 0, s, n, ptr, idx, bin_of, chunk_of,
-                       bin_row0, chunk_src0, nbins, nchunks, bin_major, key, pay);
+                       bin_row0, chunk_src0, nbins, nchunks, nhot, bin_major, key, pay, pay2);
     if (hipGetLastError() != hipSuccess) {
       rc = eg_fail(EG_EDEVICE, "binned key launch");
       break;
     }
     void *tmp = nullptr;
     EgSortCols c;
-    c.np = 1;
+    c.np = bin_major ? 2 : 1;
     c.in[0] = pay;
     c.out[0] = spay;
-    if ((rc = eg_scratch(g, eg_radix_tmp_bytes(m, 1), &tmp))) break;
+    c.in[1] = pay2;
+    c.out[1] = spay2;
+    if ((rc = eg_scratch(g, eg_radix_tmp_bytes(m, c.np), &tmp))) break;
     if ((rc = eg_radix_sort(s, m, eg_bits_for(nkeys), key, skey, c, tmp))) break;
-    hipLaunchKernelGGL(k_pb_narrow, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, spay, out);
+    hipLaunchKernelGGL(k_pb_narrow, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, spay, out, skey,
+                       spay2, nchunks + 1, bin_major && nhot ? packed : nullptr);
     if (hipGetLastError() != hipSuccess) {
       rc = eg_fail(EG_EDEVICE, "binned narrow launch");
       break;
@@ -517,8 +617,10 @@ int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32
   (void)hipStreamSynchronize(s);
   eg_dfree_t(&key);
   eg_dfree_t(&pay);
+  eg_dfree_t(&pay2);
   eg_dfree_t(&skey);
   eg_dfree_t(&spay);
+  eg_dfree_t(&spay2);
   eg_dfree_t(&segd);
   return rc;
 }
@@ -529,12 +631,16 @@ int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32
 // orientation (ptr, idx): row v's entries are the sources u of its in-edges.
 int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
   const uint32_t want_tile = g->tune.pr_tile > 0 ? (uint32_t)g->tune.pr_tile : kTileLen;
-  if (g->pb && g->pb->for_ptr == ptr && g->pb->tile_len == want_tile) return EG_OK;
+  const int want_hot = g->tune.pr_hot;
+  if (g->pb && g->pb->for_ptr == ptr && g->pb->tile_len == want_tile && g->pb->hot_req == want_hot)
+    return EG_OK;
   eg_pb_free(g);
   const uint32_t n = g->n;
   const uint64_t m = g->m;
   hipStream_t s = g->stream;
   if (m == 0 || n == 0) return eg_fail(EG_ENOTSUP, "binned PageRank needs entries");
+  int cus = 0;
+  EG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
   // 1. row lengths and source counts on the host
   std::vector<uint32_t> hptr((size_t)n + 1), scnt(n);
   {
@@ -553,10 +659,12 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
     eg_dfree_t(&cnt);
     EG_TRY(rc);
   }
-  // 2. bins of rows and chunks of sources, each balanced by entries.  With
-  // EG_PB_GEOMETRY=small in the environment every cap is cut down so that
-  // small graphs exercise long-row parts and tile cuts (tests).
+  // 2. hot sources, bins of rows and chunks of the other sources, each
+  // balanced by entries.  With EG_PB_GEOMETRY=small in the environment every
+  // cap is cut down so that small graphs exercise hot and cold entries in one
+  // bin, long-row parts and tile cuts (tests).
   uint32_t rows_cap = kBinRows, src_cap = kChunkSrc, tile_len = want_tile, tile_bins = kTileBins;
+  uint32_t hot_cap = want_hot > 0 ? (uint32_t)want_hot : 0u;
   uint64_t target = std::max<uint64_t>(m / 2048, 4096);
   {
     const char *e = getenv("EG_PB_GEOMETRY");
@@ -566,9 +674,11 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
       tile_len = 512;
       tile_bins = 16;
       target = 256;
+      hot_cap = std::min<uint32_t>(hot_cap, 40);
     }
   }
-  std::vector<uint32_t> bin_row0, bin_of(n), chunk_src0, chunk_of(n);
+  const uint32_t nhot = std::min<uint32_t>(hot_cap, n);
+  std::vector<uint32_t> bin_row0, bin_of(n), chunk_src0, chunk_of(n, 0u);
   {
     uint32_t rows = 0;
     uint64_t ents = 0;
@@ -586,7 +696,7 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
     bin_row0.push_back(n);
     uint32_t srcs = 0;
     ents = 0;
-    for (uint32_t u = 0; u < n; u++) {
+    for (uint32_t u = nhot; u < n; u++) {
       if (srcs == 0 || srcs + 1 > src_cap || ents + scnt[u] > target) {
         chunk_src0.push_back(u);
         srcs = 0;
@@ -600,13 +710,15 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
   }
   const uint32_t nbins = (uint32_t)bin_row0.size() - 1;
   const uint32_t nchunks = (uint32_t)chunk_src0.size() - 1;
-  if ((uint64_t)nbins * nchunks >= (1ull << 31))
+  if ((uint64_t)nbins * (nchunks + 1) >= (1ull << 31))
     return eg_fail(EG_ENOTSUP, "binned plan: %u bins x %u chunks", nbins, nchunks);
 
   PbPlan *p = new PbPlan();
   g->pb = p;
   p->nbins = nbins;
   p->nchunks = nchunks;
+  p->nhot = nhot;
+  p->cus = (uint32_t)cus;
   std::vector<uint32_t> seg_bm, seg_cm;
   {
     uint32_t *d_bin_of = nullptr, *d_chunk_of = nullptr, *d_row0 = nullptr, *d_src0 = nullptr;
@@ -618,13 +730,14 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
           (rc = upload(g, &d_src0, chunk_src0, "binned chunk sources")))
         break;
       if ((rc = eg_dalloc_t(&p->dloc, m + 16, "binned row offsets")) ||
-          (rc = eg_dalloc_t(&p->sloc, m, "binned source offsets")))
+          (rc = eg_dalloc_t(&p->sloc, m, "binned source offsets")) ||
+          (rc = eg_dalloc_t(&p->val, m + 16, "binned values")))
         break;
-      if ((rc = pb_order(g, ptr, idx, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, 1,
-                         p->dloc, seg_bm)))
+      if ((rc = pb_order(g, ptr, idx, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot,
+                         1, p->dloc, reinterpret_cast<uint32_t *>(p->val), seg_bm)))
         break;
-      rc = pb_order(g, ptr, idx, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, 0, p->sloc,
-                    seg_cm);
+      rc = pb_order(g, ptr, idx, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, 0,
+                    p->sloc, nullptr, seg_cm);
     } while (0);
     (void)hipStreamSynchronize(s);
     eg_dfree_t(&d_bin_of);
@@ -636,14 +749,17 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
       return rc;
     }
   }
-  // 3. segment table, scatter tiles, accumulation items
+  // 3. segment table, scatter tiles, accumulation items.  Bin-major key of
+  // (bin b, chunk c) is b * (nchunks + 1) + 1 + c (the bin's hot entries at
+  // + 0 come first); chunk-major key is c * nbins + b.
+  const uint32_t kb = nchunks + 1;
   std::vector<uint2> tab((size_t)nchunks * (nbins + 1));
   std::vector<uint2> csrc(nchunks);
   std::vector<PbTile> tiles;
   for (uint32_t c = 0; c < nchunks; c++) {
     uint2 *row = &tab[(size_t)c * (nbins + 1)];
     for (uint32_t b = 0; b < nbins; b++)
-      row[b] = make_uint2(seg_cm[(size_t)c * nbins + b], seg_bm[(size_t)b * nchunks + c]);
+      row[b] = make_uint2(seg_cm[(size_t)c * nbins + b], seg_bm[(size_t)b * kb + 1 + c]);
     const uint32_t cend = seg_cm[(size_t)(c + 1) * nbins];
     row[nbins] = make_uint2(cend, 0u);
     csrc[c] = make_uint2(chunk_src0[c], chunk_src0[c + 1] - chunk_src0[c]);
@@ -677,7 +793,8 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
   std::vector<uint4> long_rows;
   uint32_t nparts = 0;
   for (uint32_t b = 0; b < nbins; b++) {
-    const uint32_t e0 = seg_bm[(size_t)b * nchunks], e1 = seg_bm[(size_t)(b + 1) * nchunks];
+    const uint32_t e0 = seg_bm[(size_t)b * kb], eh = seg_bm[(size_t)b * kb + 1];
+    const uint32_t e1 = seg_bm[(size_t)(b + 1) * kb];
     const uint32_t r0 = bin_row0[b], nr = bin_row0[b + 1] - r0;
     if (nr == 1 && e1 - e0 > target) {
       const uint32_t np = (uint32_t)((e1 - e0 + target - 1) / target);
@@ -685,13 +802,30 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
       for (uint32_t q = 0; q < np; q++) {
         const uint32_t a0 = e0 + (uint32_t)(q * target);
         const uint32_t a1 = std::min<uint64_t>(e1, (uint64_t)a0 + target);
-        items.push_back({a0, a1, nparts + q, 0u});
+        PbItem it = {};
+        it.e0 = a0;
+        it.eh = std::min(std::max(eh, a0), a1);
+        it.e1 = a1;
+        it.row0 = nparts + q;
+        it.nrows = 0;
+        items.push_back(it);
       }
       nparts += np;
     } else {
-      items.push_back({e0, e1, r0, nr});
+      PbItem it = {};
+      it.e0 = e0;
+      it.eh = eh;
+      it.e1 = e1;
+      it.row0 = r0;
+      it.nrows = nr;
+      items.push_back(it);
     }
   }
+  // Accumulation items, largest first: workgroups are dispatched in launch
+  // order, so the longest items start first and the short ones fill the tail.
+  std::stable_sort(items.begin(), items.end(), [](const PbItem &x, const PbItem &y) {
+    return x.e1 - x.e0 + 4 * x.nrows > y.e1 - y.e0 + 4 * y.nrows;
+  });
   // Workgroups are dealt to the 8 XCDs round-robin by launch index.  Launch
   // index j runs tile (j % 8) * ceil(T / 8) + j / 8, so every XCD walks a
   // contiguous eighth of the chunk-ordered tile list and the tiles of one
@@ -708,6 +842,7 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
   }
   for (size_t k = 0; k + 1 < seg_cm.size(); k++) p->nsegs += seg_cm[k + 1] > seg_cm[k];
   p->tile_len = tile_len;
+  p->hot_req = want_hot;
   p->ntiles = (uint32_t)tiles.size();
   p->nitems = (uint32_t)items.size();
   p->nlong = (uint32_t)long_rows.size();
@@ -720,8 +855,7 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
         (rc = upload(g, &p->items, items, "binned items")) ||
         (rc = upload(g, &p->long_rows, long_rows, "binned long rows")))
       break;
-    if ((rc = eg_dalloc_t(&p->val, m + 16, "binned values")) ||
-        (rc = eg_dalloc_t(&p->part_sum, nparts ? nparts : 1, "binned part sums")) ||
+    if ((rc = eg_dalloc_t(&p->part_sum, nparts ? nparts : 1, "binned part sums")) ||
         (rc = eg_dalloc_t(&p->item_part, p->nitems, "binned item partials")) ||
         (rc = eg_dalloc_t(&p->fin_part, kFinBlocks, "binned finish partials")) ||
         (rc = eg_dalloc_t(&p->fin_counter, 1, "binned finish counter")))
@@ -772,10 +906,13 @@ int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, 
   d.nbins = p->nbins;
   d.nitems = p->nitems;
   d.nlong = p->nlong;
+  d.nhot = p->nhot;
   {
     EgProf prof(g, "pr_scatter", s);
     const int sb = g->tune.pr_scatter;
-    if (sb == 8)
+    if (p->ntiles == 0) {
+      // every source is hot: nothing to scatter
+    } else if (sb == 8)
       hipLaunchKernelGGL((k_pb_scatter<8>), dim3(p->ntiles), dim3(kBlock), 0, s, d, contrib);
     else if (sb == 32)
       hipLaunchKernelGGL((k_pb_scatter<32>), dim3(p->ntiles), dim3(kBlock), 0, s, d, contrib);
@@ -785,7 +922,16 @@ int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, 
   }
   {
     EgProf prof(g, "pr_accum", s);
-    hipLaunchKernelGGL(k_pb_accum, dim3(p->nitems), dim3(kBlock), 0, s, d, S, y, a);
+    d.persistent = g->tune.pr_persist ? 1u : 0u;
+    if (p->nhot)
+      hipLaunchKernelGGL(k_pb_accum_hot,
+                         dim3(d.persistent ? std::min(p->nitems, p->cus) : p->nitems),
+                         dim3(kBlock), 0, s, d, S, y, a, contrib);
+    else {
+      d.persistent = 0u;
+      hipLaunchKernelGGL(k_pb_accum_cold, dim3(p->nitems), dim3(kBlock), 0, s, d, S, y, a,
+                         contrib);
+    }
     EG_LAUNCHED();
   }
   {

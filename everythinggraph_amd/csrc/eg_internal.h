@@ -114,6 +114,10 @@ struct EgTune {
                          // FP32 sums), -1 = binned when the graph has >= 2^22 entries
   int pr_tile = 0;       // binned PageRank: source offsets per scatter tile (0: 32768)
   int pr_scatter = 16;   // binned scatter: source offsets in flight per lane (8, 16 or 32)
+  int pr_hot = 16384;    // binned PageRank: sources below this id are read from an LDS copy
+                         // in phase 2 instead of being scattered (0: none, at most 16384)
+  int pr_persist = 0;    // binned PageRank with hot sources: phase-2 workgroups loop over
+                         // items (one per CU) instead of one workgroup per item
 };
 
 struct eg_graph {

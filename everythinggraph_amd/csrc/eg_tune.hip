@@ -25,7 +25,9 @@
 //  * pr_tile     -- binned PageRank: source offsets per scatter workgroup
 //                   (0 = 32768); not timed by eg_autotune;
 //  * pr_scatter  -- binned PageRank scatter: source offsets loaded per lane
-//                   before any value is written, 8, 16 or 32.
+//                   before any value is written, 8, 16 or 32;
+//  * pr_hot      -- binned PageRank: sources below this id are read from an
+//                   LDS copy in phase 2 instead of being scattered (0..16384).
 // eg_autotune times every variant on the graph itself through the public
 // entry points (device time from HIP events, best of a few repetitions) and
 // keeps the fastest.
@@ -49,6 +51,8 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "pr_binned")) return &g->tune.pr_binned;
   if (!strcmp(name, "pr_tile")) return &g->tune.pr_tile;
   if (!strcmp(name, "pr_scatter")) return &g->tune.pr_scatter;
+  if (!strcmp(name, "pr_hot")) return &g->tune.pr_hot;
+  if (!strcmp(name, "pr_persist")) return &g->tune.pr_persist;
   return nullptr;
 }
 
@@ -100,6 +104,10 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
   } else if (!strcmp(param, "pr_scatter")) {
     if (value != 8 && value != 16 && value != 32)
       return eg_fail(EG_EINVAL, "pr_scatter takes 8, 16 or 32");
+  } else if (!strcmp(param, "pr_hot")) {
+    if (value < 0 || value > 16384) return eg_fail(EG_EINVAL, "pr_hot takes 0..16384");
+  } else if (!strcmp(param, "pr_persist")) {
+    if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "pr_persist takes 0 or 1");
   } else if (!strcmp(param, "pr_binned")) {
     if (value < -1 || value > 1) return eg_fail(EG_EINVAL, "pr_binned takes -1, 0 or 1");
   } else if (!strcmp(param, "graphs")) {
@@ -237,11 +245,13 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
       res.push_back({"ms_batch", v, ms});
     }
   }
-  // keep the fastest variant of each parameter
+  // keep the fastest variant of each parameter (of equal timings, the first)
   for (size_t i = 0; i < res.size(); i++) {
     bool best = true;
     for (size_t j = 0; j < res.size(); j++)
-      if (!strcmp(res[j].param, res[i].param) && res[j].ms < res[i].ms) best = false;
+      if (!strcmp(res[j].param, res[i].param) &&
+          (res[j].ms < res[i].ms || (res[j].ms == res[i].ms && j < i)))
+        best = false;
     if (best) *param_slot(g, res[i].param) = res[i].value;
   }
   int32_t nout = 0;

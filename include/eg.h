@@ -230,7 +230,10 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * with compensated FP32 sums, -1 = binned from 2^22 stored entries up; the two
  * forms round differently, each is deterministic); "pr_tile" (source offsets
  * per binned scatter workgroup, 0 = 32768); "pr_scatter" (binned scatter
- * source offsets in flight per lane, 8, 16 or 32); "grid_band" (grid source
+ * source offsets in flight per lane, 8, 16 or 32); "pr_hot" (binned PageRank:
+ * sources below this id, at most 16384, are read from an LDS copy in phase 2
+ * instead of being scattered; 0 = none, default 16384); "pr_persist" (1:
+ * phase-2 workgroups loop over work items, one per CU; default 0); "grid_band" (grid source
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
  * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
  * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant

@@ -62,7 +62,9 @@ def test_binned_pagerank_matches_oracle(scale, undirected, dangling):
     # from the FP64 oracle than the compensated ones
     assert np.abs(r.astype(np.float64) - rp).sum() <= PR_L1_TOL
     assert err.sum() <= np.abs(rp.astype(np.float64) - ref).sum() * 1.5 + 1e-9
-    assert geo["bins"] >= 1 and geo["chunks"] >= 1 and geo["items"] >= geo["bins"]
+    # sources below 16384 are hot (read from LDS in phase 2), the rest in chunks
+    assert geo["bins"] >= 1 and geo["items"] >= geo["bins"]
+    assert (geo["chunks"] >= 1) == (n > 16384)
 
 
 def test_binned_reference_defaults():
@@ -142,14 +144,19 @@ for und in (False, True):
         geo = g.binned_geometry()
         out['geo%d' % und] = np.array([geo[k] for k in ("bins", "chunks", "tiles", "items",
                                                        "long_rows")])
+        g.tune_set("pr_hot", 0)      # every source scattered
+        out['cold%d' % und] = g.pagerank(0.85, 15)[0]
+        g.tune_set("pr_hot", 4096)   # half hot, half scattered
+        out['half%d' % und] = g.pagerank(0.85, 15)[0]
 np.savez(sys.argv[2], **out)
 """
 
 
 def test_binned_independent_of_cut_and_order(tmp_path):
     """The same ranks, bit for bit, with the default bins and chunks, with
-    tiny ones (long rows in parts, tiles cut by length and by bins), and with
-    or without the degree-ordered copy."""
+    tiny ones (40 hot sources, long rows in parts, tiles cut by length and by
+    bins), with all, half or none of the sources hot, and with or without the
+    degree-ordered copy."""
     script = tmp_path / "run.py"
     script.write_text(_GEOMETRY_SCRIPT)
     res = {}
@@ -165,6 +172,9 @@ def test_binned_independent_of_cut_and_order(tmp_path):
     for key, r in res.items():
         for k in ("pr0", "pr1", "uni0", "uni1"):
             assert np.array_equal(r[k], first[k]), (key, k)
+        for und in (0, 1):  # no hot sources, or half of them: the same bits
+            assert np.array_equal(r[f"cold{und}"], first[f"pr{und}"]), (key, und)
+            assert np.array_equal(r[f"half{und}"], first[f"pr{und}"]), (key, und)
     for und in (0, 1):
         small = res[("small", "degree")][f"geo{und}"]
         bins, chunks, tiles, items, long_rows = (int(v) for v in small)
