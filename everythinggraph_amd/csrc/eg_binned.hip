@@ -64,16 +64,20 @@ inline uint32_t blocks_for(uint64_t count, uint32_t per) {
 // Fixed point: value * 2^S as an unsigned 64-bit integer.
 // ---------------------------------------------------------------------------
 // Round-half-up conversion of a finite c >= 0; exact whenever c * 2^S is an
-// integer (always for c >= 2^(23 - S)).
+// integer (always for c >= 2^(23 - S)).  c = mant * 2^(e - 150), so
+// c * 2^S = mant * 2^sh with sh = e - 150 + S.  Every value that is
+// converted is below the row-sum bound B with B * 2^S <= 2^62, so sh <= 38
+// and mant << 39 (< 2^63) shifted right by t = 39 - sh >= 1, after adding
+// half a unit, is the rounded product: the add cannot carry out for sh >= 0
+// (the low t bits are zero), and for sh < 0 it is (mant + 2^(-sh-1)) >> -sh.
+// Zero and subnormals (e == 0) give t >= 64, that is 0.
 __device__ __forceinline__ uint64_t to_fixed(float c, int S) {
   const uint32_t b = __float_as_uint(c);
-  const int e = (int)((b >> 23) & 0xFFu);
-  if (e == 0) return 0;  // zero (and subnormals, below 1.2e-38)
-  const uint64_t mant = (uint64_t)((b & 0x7FFFFFu) | 0x800000u);
-  const int sh = e - 150 + S;  // c = mant * 2^(e - 150)
-  if (sh >= 0) return mant << sh;
-  if (sh < -24) return 0;  // below half a unit
-  return (mant + (1ull << (-sh - 1))) >> (-sh);
+  const uint32_t e = b >> 23;
+  const uint64_t mant = (uint64_t)((b & 0x7FFFFFu) | (e ? 0x800000u : 0u));
+  const int t = max(189 - S - (int)e, 1);
+  if (t >= 64) return 0;
+  return ((mant << 39) + (1ull << (t - 1))) >> t;
 }
 
 __device__ __forceinline__ float from_fixed(uint64_t v, int S) {
@@ -260,38 +264,60 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // Adds the 8 entries of the group that starts at entry e (those outside
 // [e0, e1) masked): HOTRANGE, w0 / w1 hold packed (source | row offset << 16)
 // words; otherwise w0 holds 8 u16 row offsets and v0 / v1 the values.  part:
-// the whole range belongs to one row and is summed into psum.
+// the whole range belongs to one row and is summed into psum.  A run of equal
+// row offsets is added with one atomic at its last entry.
 template <bool HOTRANGE>
 __device__ __forceinline__ void group_accumulate(unsigned long long *acc, const u32x4 w0,
                                                  const u32x4 w1, const f32x4 v0, const f32x4 v1,
                                                  uint32_t e, uint32_t e0, uint32_t e1, int S,
                                                  bool part, const float *s_hot,
                                                  unsigned long long &psum) {
-  const uint32_t ws[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-  const uint32_t ks[8] = {w0.x & 0xFFFFu, w0.x >> 16, w0.y & 0xFFFFu, w0.y >> 16,
-                          w0.z & 0xFFFFu, w0.z >> 16, w0.w & 0xFFFFu, w0.w >> 16};
-  const float vs[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-  uint32_t cur = 0xFFFFFFFFu;
+  uint32_t key[8];
+  float v[8];
+  if (HOTRANGE) {
+    const uint32_t w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      key[j] = w[j] >> 16;
+      // masked slots may hold other words: keep the read inside the copy
+      v[j] = s_hot[w[j] & (kHotMax - 1)];
+    }
+  } else {
+    const uint32_t ks[8] = {w0.x & 0xFFFFu, w0.x >> 16, w0.y & 0xFFFFu, w0.y >> 16,
+                            w0.z & 0xFFFFu, w0.z >> 16, w0.w & 0xFFFFu, w0.w >> 16};
+    const float vs[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      key[j] = ks[j];
+      v[j] = vs[j];
+    }
+  }
+  if (e < e0 || e + 8 > e1) {  // an edge group of the range
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const bool ok = e + j >= e0 && e + j < e1;
+      key[j] = ok ? key[j] : 0xFFFFFFFFu;
+      v[j] = ok ? v[j] : 0.f;
+    }
+  }
+  if (part) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) psum += to_fixed(v[j], S);
+    return;
+  }
+  // Branch-free: every entry issues one atomic; the last entry of a run adds
+  // the run to its row, the others add to this lane's dummy slot behind the
+  // accumulators (never read), so no lane waits on a branch and dummies never
+  // collide.
+  const uint32_t dummy = kBinRows + (uint32_t)__lane_id();
   unsigned long long run = 0ull;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    const bool ok = e + j >= e0 && e + j < e1;
-    const uint32_t key = HOTRANGE ? ws[j] >> 16 : ks[j];
-    const float v = HOTRANGE ? (ok ? s_hot[ws[j] & 0xFFFFu] : 0.f) : vs[j];
-    const unsigned long long f = ok ? to_fixed(v, S) : 0ull;
-    if (part) {
-      psum += f;
-      continue;
-    }
-    const uint32_t k = ok ? key : 0xFFFFFFFFu;
-    if (k != cur) {
-      if (cur != 0xFFFFFFFFu) atomicAdd(&acc[cur], run);
-      cur = k;
-      run = 0ull;
-    }
-    run += f;
+    run += to_fixed(v[j], S);
+    const bool end = j == 7 || key[j] != key[j + 1];
+    atomicAdd(&acc[end && key[j] != 0xFFFFFFFFu ? key[j] : dummy], run);
+    run = end ? 0ull : run;
   }
-  if (!part && cur != 0xFFFFFFFFu) atomicAdd(&acc[cur], run);
 }
 
 // Sums the entries [e0, e1) of one item, each wave a contiguous range of
@@ -349,7 +375,7 @@ __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, ui
 template <bool HOT>
 __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restrict__ y,
                                          const PrArgs &a, const float *__restrict__ contrib) {
-  __shared__ unsigned long long s_acc[kBinRows];
+  __shared__ unsigned long long s_acc[kBinRows + 64];  // + a dummy slot per lane
   __shared__ float s_hot[HOT ? kHotMax : 1];
   __shared__ unsigned long long s_ps[kWaves];
   __shared__ double s_d[kWaves], s_l[kWaves];
