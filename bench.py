@@ -198,6 +198,22 @@ def main():
         f"levels {min(levels)}..{max(levels)}")
 
     extra = {}
+    # SpMV over the same graph (unweighted, w = 1): y = A^T x, signed x
+    x = np.random.default_rng(5).uniform(-1, 1, n).astype(np.float32)
+    g.spmv(x)
+    g.prof_reset()
+    g.prof_enable(True)
+    for _ in range(5):
+        g.spmv(x)
+    sp_prof = g.prof_read()
+    g.prof_enable(False)
+    sp_ms = sum(t for k, (_, t) in sp_prof.items() if k.startswith("spmv_")) / 5
+    b_spmv1 = 4 * m + 4 * (n + 1) + 8 * n
+    extra["spmv_rmat24"] = {"ms": round(sp_ms, 4), "gbs": round(b_spmv1 / (sp_ms * 1e-3) / 1e9, 1),
+                            "bytes": b_spmv1, "weights": "unit"}
+    log(f"spmv: {sp_ms * 1e3:.1f} us = {b_spmv1 / (sp_ms * 1e-3) / 1e9:.1f} GB/s of "
+        f"4m + 4(n+1) + 8n")
+    del x
     if not args.skip_batch:
         broots = eg.sample_roots(deg, args.batch_bfs, seed=2)
         g.bfs_batch(broots[:64], download=False)
@@ -338,29 +354,35 @@ def other_configs(eg) -> dict:
     b_spmv = 8 * m + 4 * (n + 1) + 8 * n
     tune = g.autotune(eg.TUNE_GRID)
     band = g.tune_get("grid_band")
-    log("grid autotune: " + ", ".join(f"grid_band={v} {ms:.3f} ms{'*' if c else ''}"
-                                      for _, v, ms, c in tune))
+    binned = g.tune_get("pr_binned")
+    log("grid autotune: " + ", ".join(f"{p}={v} {ms:.3f} ms{'*' if c else ''}"
+                                      for p, v, ms, c in tune))
     g.pagerank(0.85, 2, layout="grid", download=False)
+    x = np.random.default_rng(1).uniform(-1, 1, n).astype(np.float32)
+    g.spmv(x, layout="grid")
     g.prof_reset()
     g.prof_enable(True)
     _, _, lh = g.pagerank(0.85, 20, layout="grid", download=False)
-    x = np.ones(n, np.float32)
+    pr_prof = g.prof_read()
+    g.prof_reset()
     for _ in range(5):
         g.spmv(x, layout="grid")
-    prof = g.prof_read()
+    sp_prof = g.prof_read()
     g.prof_enable(False)
-    pl, pt = prof.get("pr_grid", (0, 0.0))
-    sl, stt = prof.get("spmv_grid", (0, 0.0))
-    pr_ms, sp_ms = pt / max(pl, 1), stt / max(sl, 1)
+    # device time of the PageRank iterations (grid kernel, or the binned
+    # scatter + accumulate + finish) and of the SpMV passes
+    pr_ms = sum(t for k, (_, t) in pr_prof.items() if k.startswith("pr_")) / 20
+    sp_ms = sum(t for k, (_, t) in sp_prof.items() if k.startswith("spmv_")) / 5
     res["grid_er24"] = {
-        "n": n, "m": m, "P": 16, "grid_band": band, "build_ms": round(build_ms, 2),
+        "n": n, "m": m, "P": 16, "weighted": True, "grid_band": band,
+        "kernels": "binned" if binned == 1 else "grid pull", "build_ms": round(build_ms, 2),
         "pr_iter_ms": round(pr_ms, 4), "pr_gbs": round(b_pr / (pr_ms * 1e-3) / 1e9, 1),
         "spmv_ms": round(sp_ms, 4), "spmv_gbs": round(b_spmv / (sp_ms * 1e-3) / 1e9, 1),
         "pr_final_l1_delta": float(lh[-1]),
     }
-    log(f"grid ER 2^24/2^28 16x16: build {build_ms:.1f} ms; PR {pr_ms * 1e3:.1f} us/iter = "
-        f"{b_pr / (pr_ms * 1e-3) / 1e9:.1f} GB/s; SpMV {sp_ms * 1e3:.1f} us = "
-        f"{b_spmv / (sp_ms * 1e-3) / 1e9:.1f} GB/s")
+    log(f"grid ER 2^24/2^28 16x16 ({res['grid_er24']['kernels']}): build {build_ms:.1f} ms; "
+        f"PR {pr_ms * 1e3:.1f} us/iter = {b_pr / (pr_ms * 1e-3) / 1e9:.1f} GB/s; "
+        f"SpMV {sp_ms * 1e3:.1f} us = {b_spmv / (sp_ms * 1e-3) / 1e9:.1f} GB/s")
     g.close()
 
     # file ingest: the R-MAT 24 records (2.15 GB) read back from a file in

@@ -426,9 +426,9 @@ class Graph:
 
     def binned_geometry(self) -> dict:
         """Geometry of the binned PageRank plan (after a binned pagerank call)."""
-        out = np.zeros(6, np.uint32)
+        out = np.zeros(7, np.uint32)
         _check(lib().eg_pr_binned_geometry(self._h, _ptr(out)), "eg_pr_binned_geometry")
-        return dict(zip(("bins", "chunks", "tiles", "items", "long_rows", "segments"),
+        return dict(zip(("bins", "chunks", "tiles", "items", "long_rows", "segments", "hot"),
                         (int(v) for v in out)))
 
     def tune_set(self, param: str, value: int):

@@ -22,11 +22,18 @@
 //    offsets and adds them into per-row accumulators in LDS, then applies the
 //    PageRank update to the bin's rows.  Single rows with more entries than a
 //    bin takes are cut into parts summed by separate workgroups.
-// Sums are exact: every FP32 contribution is converted to a 64-bit fixed-point
-// integer with a scale 2^S chosen so that no row sum can overflow, and integer
-// addition is associative, so the result does not depend on the order of the
-// entries, on the bin/chunk cut or on scheduling.  The row sum is rounded to
-// FP32 once at the end.
+// Sums are exact: every FP32 term (a contribution, or a product w * x for
+// SpMV) is converted to a signed 64-bit fixed-point integer with a scale 2^S
+// chosen so that no row sum can overflow, and integer addition is
+// associative, so the result does not depend on the order of the entries, on
+// the bin/chunk cut or on scheduling.  The row sum is rounded to FP32 once at
+// the end.
+//
+// The entries come from an adjacency orientation (row pointers + sources) or
+// from an edge list (destination and source of every entry: the 2-D grid of
+// load modes 0/1/6 and the edge array of modes 7/8/9), so the same kernels
+// serve pagerank_simple, prgrid (prgrid_cilk.c:62-141), pr_algo_edgecentric
+// (pagerank_simple.c:40-61) and spmv (spmv.c:98-136).
 //
 // Traffic per iteration, stored entries m: 2m (source offsets) + 4m (values
 // written) + 4m (values read) + 2m (row offsets) + the per-vertex vectors,
@@ -52,6 +59,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kWinPer = (kChunkSrc + kBlock - 1) / kBlock;     // window loads per thread
 constexpr int kTabPer = (kTileBins + 1 + kBlock - 1) / kBlock;  // table loads per thread
 constexpr int kRowPer = (kBinRows + kBlock - 1) / kBlock;       // rows finished per thread
+constexpr int kRowBatch = 4;                                    // of which loaded together
 constexpr int kFinBlocks = 128;
 constexpr uint32_t kHotMax = 16384;    // hot sources whose values phase 2 keeps in LDS (64 KB)
 constexpr int kHotPer = (kHotMax + kBlock - 1) / kBlock;        // hot loads per thread
@@ -63,25 +71,27 @@ inline uint32_t blocks_for(uint64_t count, uint32_t per) {
 // ---------------------------------------------------------------------------
 // Fixed point: value * 2^S as an unsigned 64-bit integer.
 // ---------------------------------------------------------------------------
-// Round-half-up conversion of a finite c >= 0; exact whenever c * 2^S is an
-// integer (always for c >= 2^(23 - S)).  c = mant * 2^(e - 150), so
-// c * 2^S = mant * 2^sh with sh = e - 150 + S.  Every value that is
-// converted is below the row-sum bound B with B * 2^S <= 2^62, so sh <= 38
-// and mant << 39 (< 2^63) shifted right by t = 39 - sh >= 1, after adding
-// half a unit, is the rounded product: the add cannot carry out for sh >= 0
-// (the low t bits are zero), and for sh < 0 it is (mant + 2^(-sh-1)) >> -sh.
-// Zero and subnormals (e == 0) give t >= 64, that is 0.
+// Round-half-up conversion of |c| (finite), negated for c < 0, as a two's
+// complement u64; exact whenever c * 2^S is an integer (always for |c| >=
+// 2^(23 - S)).  |c| = mant * 2^(e - 150), so |c| * 2^S = mant * 2^sh with
+// sh = e - 150 + S.  Every value that is converted is below the bound B with
+// B * 2^S <= 2^62, so sh <= 38 and mant << 39 (< 2^63) shifted right by
+// t = 39 - sh >= 1, after adding half a unit, is the rounded product: the add
+// cannot carry out for sh >= 0 (the low t bits are zero), and for sh < 0 it
+// is (mant + 2^(-sh-1)) >> -sh.  Zero and subnormals (e == 0) give t >= 64,
+// that is 0.
 __device__ __forceinline__ uint64_t to_fixed(float c, int S) {
   const uint32_t b = __float_as_uint(c);
-  const uint32_t e = b >> 23;
+  const uint32_t e = (b >> 23) & 0xFFu;
   const uint64_t mant = (uint64_t)((b & 0x7FFFFFu) | (e ? 0x800000u : 0u));
   const int t = max(189 - S - (int)e, 1);
   if (t >= 64) return 0;
-  return ((mant << 39) + (1ull << (t - 1))) >> t;
+  const uint64_t mag = ((mant << 39) + (1ull << (t - 1))) >> t;
+  return (b >> 31) ? 0ull - mag : mag;
 }
 
 __device__ __forceinline__ float from_fixed(uint64_t v, int S) {
-  return (float)ldexp((double)v, -S);
+  return (float)ldexp((double)(int64_t)v, -S);
 }
 
 struct PbTile {
@@ -115,46 +125,86 @@ struct PbDev {
   uint32_t nbins, nitems, nlong;
   uint32_t nhot;           // sources below this id are hot
   uint32_t persistent;     // phase 2: workgroups loop over items (else one each)
+  const float *w;          // bin-major weights (SpMV of a weighted graph), or null
 };
 
 // ---------------------------------------------------------------------------
 // Build kernels.
 // ---------------------------------------------------------------------------
-// key / payloads of every entry, a wave per row (grid-stride over rows).
-// Sources below nhot are hot: they belong to no chunk.
+// Where the entries come from: rows of an adjacency orientation (ptr, idx)
+// or an edge list (row[k], idx[k]); w (may be null): the weight of entry k.
+struct PbSrcDev {
+  const uint32_t *ptr, *row, *idx;
+  const float *w;
+};
+
+__device__ __forceinline__ void pb_entry_keys(uint32_t v, uint32_t k, const PbSrcDev &src,
+                                              const uint32_t *__restrict__ bin_of,
+                                              const uint32_t *__restrict__ chunk_of,
+                                              const uint32_t *__restrict__ bin_row0,
+                                              const uint32_t *__restrict__ chunk_src0,
+                                              uint32_t nbins, uint32_t nchunks, uint32_t nhot,
+                                              int bin_major, uint32_t *__restrict__ key,
+                                              uint32_t *__restrict__ pay,
+                                              uint32_t *__restrict__ pay2,
+                                              uint32_t *__restrict__ pay3) {
+  const uint32_t u = src.idx[k];
+  const uint32_t b = bin_of[v];
+  const bool hot = u < nhot;
+  const uint32_t c = hot ? 0u : chunk_of[u];
+  if (bin_major) {
+    key[k] = b * (nchunks + 1) + (hot ? 0u : 1u + c);
+    pay[k] = v - bin_row0[b];
+    pay2[k] = u;
+    if (pay3) pay3[k] = __float_as_uint(src.w[k]);
+  } else {
+    key[k] = hot ? nchunks * nbins : c * nbins + b;
+    pay[k] = hot ? 0u : u - chunk_src0[c];
+  }
+}
+
+// key / payloads of every entry.  Sources below nhot are hot: they belong to
+// no chunk.
 // bin_major: key = bin * (nchunks + 1) + (hot ? 0 : 1 + chunk), payloads =
-//   row - first row of the bin, and the source;
+//   row - first row of the bin, the source, and the weight bits (pay3);
 // else key = hot ? nchunks * nbins : chunk * nbins + bin, payload = source -
 //   first source of the chunk (hot entries sort last and are not used).
-__global__ void k_pb_keys(uint32_t n, const uint32_t *__restrict__ ptr,
-                          const uint32_t *__restrict__ idx, const uint32_t *__restrict__ bin_of,
+// Adjacency source: a wave per row, grid-stride; edge list: a thread per entry.
+__global__ void k_pb_keys(uint32_t n, uint64_t m, PbSrcDev src,
+                          const uint32_t *__restrict__ bin_of,
                           const uint32_t *__restrict__ chunk_of,
                           const uint32_t *__restrict__ bin_row0,
                           const uint32_t *__restrict__ chunk_src0, uint32_t nbins,
                           uint32_t nchunks, uint32_t nhot, int bin_major,
                           uint32_t *__restrict__ key, uint32_t *__restrict__ pay,
-                          uint32_t *__restrict__ pay2) {
+                          uint32_t *__restrict__ pay2, uint32_t *__restrict__ pay3) {
+  if (src.row) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += stride)
+      pb_entry_keys(src.row[k], (uint32_t)k, src, bin_of, chunk_of, bin_row0, chunk_src0, nbins,
+                    nchunks, nhot, bin_major, key, pay, pay2, pay3);
+    return;
+  }
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   const int lane = __lane_id();
   for (uint32_t v = wave; v < n; v += nwaves) {
-    const uint32_t k0 = ptr[v], k1 = ptr[v + 1];
-    if (k0 == k1) continue;
-    const uint32_t b = bin_of[v];
-    for (uint32_t k = k0 + lane; k < k1; k += 64) {
-      const uint32_t u = idx[k];
-      const bool hot = u < nhot;
-      const uint32_t c = hot ? 0u : chunk_of[u];
-      if (bin_major) {
-        key[k] = b * (nchunks + 1) + (hot ? 0u : 1u + c);
-        pay[k] = v - bin_row0[b];
-        pay2[k] = u;
-      } else {
-        key[k] = hot ? nchunks * nbins : c * nbins + b;
-        pay[k] = hot ? 0u : u - chunk_src0[c];
-      }
-    }
+    const uint32_t k0 = src.ptr[v], k1 = src.ptr[v + 1];
+    for (uint32_t k = k0 + lane; k < k1; k += 64)
+      pb_entry_keys(v, k, src, bin_of, chunk_of, bin_row0, chunk_src0, nbins, nchunks, nhot,
+                    bin_major, key, pay, pay2, pay3);
   }
+}
+
+// Largest |w| as float bits (non-negative floats order like their bits).
+__global__ void k_pb_absmax(uint64_t m, const float *__restrict__ w, uint32_t *__restrict__ out) {
+  uint32_t mx = 0u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += stride)
+    mx = max(mx, __float_as_uint(w[k]) & 0x7FFFFFFFu);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o));
+  if (__lane_id() == 0) atomicMax(out, mx);
 }
 
 // u16 offsets of the sorted order; for the bin-major order also the packed
@@ -266,9 +316,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // words; otherwise w0 holds 8 u16 row offsets and v0 / v1 the values.  part:
 // the whole range belongs to one row and is summed into psum.  A run of equal
 // row offsets is added with one atomic at its last entry.
-template <bool HOTRANGE>
+template <bool HOTRANGE, bool WEIGHTED>
 __device__ __forceinline__ void group_accumulate(unsigned long long *acc, const u32x4 w0,
                                                  const u32x4 w1, const f32x4 v0, const f32x4 v1,
+                                                 const f32x4 x0, const f32x4 x1,
                                                  uint32_t e, uint32_t e0, uint32_t e1, int S,
                                                  bool part, const float *s_hot,
                                                  unsigned long long &psum) {
@@ -291,6 +342,11 @@ __device__ __forceinline__ void group_accumulate(unsigned long long *acc, const 
       key[j] = ks[j];
       v[j] = vs[j];
     }
+  }
+  if (WEIGHTED) {  // SpMV of a weighted graph: the FP32 product w * x
+    const float ws8[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] *= ws8[j];
   }
   if (e < e0 || e + 8 > e1) {  // an edge group of the range
 #pragma unroll
@@ -323,7 +379,7 @@ __device__ __forceinline__ void group_accumulate(unsigned long long *acc, const 
 // Sums the entries [e0, e1) of one item, each wave a contiguous range of
 // 8-entry groups, G groups loaded per lane before any is summed.  HOTRANGE:
 // the entries are hot (packed value slots).
-template <bool HOTRANGE, int G>
+template <bool HOTRANGE, bool WEIGHTED, int G>
 __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, uint32_t e1, int S,
                                                  bool part, unsigned long long *acc,
                                                  const float *s_hot,
@@ -339,11 +395,12 @@ __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, ui
   const u32x4 *dl4 = reinterpret_cast<const u32x4 *>(d.dloc);
   const f32x4 *v4 = reinterpret_cast<const f32x4 *>(d.val);
   const u32x4 *h4 = reinterpret_cast<const u32x4 *>(d.val);
+  const f32x4 *w4 = reinterpret_cast<const f32x4 *>(d.w);
   const u32x4 z4 = u32x4{0u, 0u, 0u, 0u};
   const f32x4 zf = f32x4{0.f, 0.f, 0.f, 0.f};
   for (uint32_t base = wg0; base < wg1; base += 64 * G) {
     u32x4 wa[G], wb[G];
-    f32x4 va[G], vb[G];
+    f32x4 va[G], vb[G], xa[G], xb[G];
 #pragma unroll
     for (int q = 0; q < G; q++) {
       const uint32_t gi = base + q * 64 + lane;
@@ -358,13 +415,19 @@ __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, ui
         va[q] = in ? __builtin_nontemporal_load(v4 + 2 * (size_t)gi) : zf;
         vb[q] = in ? __builtin_nontemporal_load(v4 + 2 * (size_t)gi + 1) : zf;
       }
+      if (WEIGHTED) {
+        xa[q] = in ? __builtin_nontemporal_load(w4 + 2 * (size_t)gi) : zf;
+        xb[q] = in ? __builtin_nontemporal_load(w4 + 2 * (size_t)gi + 1) : zf;
+      } else {
+        xa[q] = xb[q] = zf;
+      }
     }
 #pragma unroll
     for (int q = 0; q < G; q++) {
       const uint32_t gi = base + q * 64 + lane;
       if (gi < wg1)
-        group_accumulate<HOTRANGE>(acc, wa[q], wb[q], va[q], vb[q], gi * 8, e0, e1, S, part,
-                                   s_hot, psum);
+        group_accumulate<HOTRANGE, WEIGHTED>(acc, wa[q], wb[q], va[q], vb[q], xa[q], xb[q],
+                                             gi * 8, e0, e1, S, part, s_hot, psum);
     }
   }
 }
@@ -372,7 +435,9 @@ __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, ui
 // HOT: persistent workgroups (one per CU) that first copy the hot sources'
 // contributions into LDS, then take items blockIdx.x, blockIdx.x + gridDim.x,
 // ...; otherwise one item per workgroup and no hot range.
-template <bool HOT>
+// SPMV: rows receive the plain sum (y[r] = sum); otherwise the PageRank
+// update.  WEIGHTED: terms are w * x with the bin-major weights.
+template <bool HOT, bool SPMV, bool WEIGHTED>
 __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restrict__ y,
                                          const PrArgs &a, const float *__restrict__ contrib) {
   __shared__ unsigned long long s_acc[kBinRows + 64];  // + a dummy slot per lane
@@ -405,9 +470,10 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
     __syncthreads();  // accumulators cleared (and, the first time, the hot copy made)
     unsigned long long psum = 0ull;
     // the persistent form runs 16 waves per CU: more loads in flight per lane
-    constexpr int G = HOT ? 4 : 2;
-    if (HOT) accumulate_range<true, G>(d, it.e0, it.eh, S, part, s_acc, s_hot, psum);
-    accumulate_range<false, G>(d, HOT ? it.eh : it.e0, it.e1, S, part, s_acc, s_hot, psum);
+    constexpr int G = HOT && !WEIGHTED ? 4 : 2;
+    if (HOT) accumulate_range<true, WEIGHTED, G>(d, it.e0, it.eh, S, part, s_acc, s_hot, psum);
+    accumulate_range<false, WEIGHTED, G>(d, HOT ? it.eh : it.e0, it.e1, S, part, s_acc, s_hot,
+                                         psum);
     if (part) {
 #pragma unroll
       for (int o = 32; o >= 1; o >>= 1) psum += __shfl_xor(psum, o);
@@ -423,23 +489,32 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
       continue;
     }
     __syncthreads();
-    // per-row inputs of this thread's rows, every load issued before the first use
-    uint32_t dg[kRowPer];
-    float ro[kRowPer];
-#pragma unroll
-    for (int j = 0; j < kRowPer; j++) {
-      const uint32_t i = tid + j * kBlock;
-      dg[j] = i < it.nrows ? a.out_deg[it.row0 + i] : 0u;
-      ro[j] = i < it.nrows ? a.rank_old[it.row0 + i] : 0.f;
+    if (SPMV) {
+      for (uint32_t i = tid; i < it.nrows; i += kBlock) y[it.row0 + i] = from_fixed(s_acc[i], S);
+      if (tid == 0) d.item_part[item] = make_double2(0.0, 0.0);
+      __syncthreads();  // s_acc reuse by the next item
+      continue;
     }
+    // per-row inputs of this thread's rows, in batches of kRowBatch rows whose
+    // loads are all issued before the first use
     const float basep = pr_base(a);
     double dang = 0.0, l1 = 0.0;
+    for (int j0 = 0; j0 < kRowPer; j0 += kRowBatch) {
+      uint32_t dg[kRowBatch];
+      float ro[kRowBatch];
 #pragma unroll
-    for (int j = 0; j < kRowPer; j++) {
-      const uint32_t i = tid + j * kBlock;
-      if (i < it.nrows)
-        finish_row<true>(it.row0 + i, from_fixed(s_acc[i], S), y, a, basep, dg[j], ro[j], dang,
-                         l1);
+      for (int j = 0; j < kRowBatch; j++) {
+        const uint32_t i = tid + (j0 + j) * kBlock;
+        dg[j] = i < it.nrows ? a.out_deg[it.row0 + i] : 0u;
+        ro[j] = i < it.nrows ? a.rank_old[it.row0 + i] : 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < kRowBatch; j++) {
+        const uint32_t i = tid + (j0 + j) * kBlock;
+        if (i < it.nrows)
+          finish_row<true>(it.row0 + i, from_fixed(s_acc[i], S), y, a, basep, dg[j], ro[j], dang,
+                           l1);
+      }
     }
     dang = eg_wave_sum_f64(dang);
     l1 = eg_wave_sum_f64(l1);
@@ -460,21 +535,24 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
   }
 }
 
+template <bool SPMV, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void k_pb_accum_hot(PbDev d, int S, float *__restrict__ y,
                                                          PrArgs a,
                                                          const float *__restrict__ contrib) {
-  pb_accum<true>(d, S, y, a, contrib);
+  pb_accum<true, SPMV, WEIGHTED>(d, S, y, a, contrib);
 }
 
 // Two workgroups per CU: held to 64 VGPRs (8 waves per SIMD).
+template <bool SPMV, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 k_pb_accum_cold(PbDev d, int S, float *__restrict__ y, PrArgs a,
                 const float *__restrict__ contrib) {
-  pb_accum<false>(d, S, y, a, contrib);
+  pb_accum<false, SPMV, WEIGHTED>(d, S, y, a, contrib);
 }
 
 // Long rows (sum of their parts) and the iteration's FP64 statistics; the
 // last block to finish writes them and re-arms the counter.
+template <bool SPMV>
 __global__ __launch_bounds__(256) void k_pb_finish(PbDev d, int S, float *__restrict__ y,
                                                    PrArgs a) {
   __shared__ double s_d[4], s_l[4];
@@ -484,6 +562,15 @@ __global__ __launch_bounds__(256) void k_pb_finish(PbDev d, int S, float *__rest
   const int wave = tid >> 6;
   const uint32_t gtid = blockIdx.x * 256 + tid;
   const uint32_t gthreads = gridDim.x * 256;
+  if (SPMV) {
+    for (uint32_t j = gtid; j < d.nlong; j += gthreads) {
+      const uint4 lr = d.long_rows[j];
+      unsigned long long t = 0ull;
+      for (uint32_t q = 0; q < lr.z; q++) t += d.part_sum[lr.y + q];
+      y[lr.x] = from_fixed(t, S);
+    }
+    return;
+  }
   const float basep = pr_base(a);
   double dang = 0.0, l1 = 0.0;
   for (uint32_t j = gtid; j < d.nlong; j += gthreads) {
@@ -537,18 +624,21 @@ __global__ __launch_bounds__(256) void k_pb_finish(PbDev d, int S, float *__rest
 // Host side.
 // ---------------------------------------------------------------------------
 struct PbPlan {
-  const uint32_t *for_ptr = nullptr;  // orientation the plan was built for
+  const void *for_key = nullptr;  // entry source the plan was built for
   uint32_t nbins = 0, nchunks = 0, ntiles = 0, nitems = 0, nlong = 0, nparts = 0;
   uint32_t tile_len = 0;   // scatter tile length the plan was cut with
   int hot_req = 0;         // hot setting the plan was built for
   uint32_t nhot = 0;       // sources below this id are hot
   uint32_t cus = 0;        // compute units (persistent accumulation grid)
+  uint32_t max_rowlen = 0; // most entries of one row
+  float wmax = 1.f;        // largest |w| (1 without weights)
   uint64_t nsegs = 0;      // non-empty (chunk, bin) segments
   PbTile *tiles = nullptr;
   uint2 *chunk_src = nullptr;
   uint2 *tab = nullptr;
   uint16_t *sloc = nullptr, *dloc = nullptr;
   float *val = nullptr;
+  float *w = nullptr;      // bin-major weights, or null
   PbItem *items = nullptr;
   uint4 *long_rows = nullptr;
   unsigned long long *part_sum = nullptr;
@@ -565,6 +655,7 @@ void eg_pb_free(eg_graph *g) {
   eg_dfree_t(&p->sloc);
   eg_dfree_t(&p->dloc);
   eg_dfree_t(&p->val);
+  eg_dfree_t(&p->w);
   eg_dfree_t(&p->items);
   eg_dfree_t(&p->long_rows);
   eg_dfree_t(&p->part_sum);
@@ -586,19 +677,20 @@ int upload(eg_graph *g, T **dst, const std::vector<T> &src, const char *what) {
   return EG_OK;
 }
 
-// Sorts the entries into one order and writes their u16 offsets (and, for
-// the bin-major order, the packed hot value slots); seg receives the start of
-// every key's segment (nkeys + 1 entries) on the host.
-int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32_t *bin_of,
-             const uint32_t *chunk_of, const uint32_t *bin_row0, const uint32_t *chunk_src0,
-             uint32_t nbins, uint32_t nchunks, uint32_t nhot, int bin_major, uint16_t *out,
-             uint32_t *packed, std::vector<uint32_t> &seg) {
+// Sorts the entries into one order and writes their u16 offsets (for the
+// bin-major order also the packed hot value slots and the weights); seg
+// receives the start of every key's segment (nkeys + 1 entries) on the host.
+int pb_order(eg_graph *g, const PbSrcDev &src, const uint32_t *bin_of, const uint32_t *chunk_of,
+             const uint32_t *bin_row0, const uint32_t *chunk_src0, uint32_t nbins,
+             uint32_t nchunks, uint32_t nhot, int bin_major, uint16_t *out, uint32_t *packed,
+             float *wout, std::vector<uint32_t> &seg) {
   const uint32_t n = g->n;
   const uint64_t m = g->m;
   hipStream_t s = g->stream;
   const uint64_t nkeys = bin_major ? (uint64_t)nbins * (nchunks + 1) : (uint64_t)nchunks * nbins + 1;
-  uint32_t *key = nullptr, *pay = nullptr, *pay2 = nullptr, *skey = nullptr, *spay = nullptr,
-           *spay2 = nullptr, *segd = nullptr;
+  const bool wt = bin_major && wout;
+  uint32_t *key = nullptr, *pay = nullptr, *pay2 = nullptr, *pay3 = nullptr, *skey = nullptr,
+           *spay = nullptr, *spay2 = nullptr, *segd = nullptr;
   int rc = EG_OK;
   do {
     if ((rc = eg_dalloc_t(&key, m, "binned keys")) || (rc = eg_dalloc_t(&pay, m, "binned payload")) ||
@@ -609,20 +701,27 @@ int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32
     if (bin_major && ((rc = eg_dalloc_t(&pay2, m, "binned sources")) ||
                       (rc = eg_dalloc_t(&spay2, m, "binned sorted sources"))))
       break;
-    const uint32_t grid = std::min<uint64_t>(blocks_for((uint64_t)n * 64, 256), 65536);
-    hipLaunchKernelGGL(k_pb_keys, dim3(grid), dim3(256), 0, s, n, ptr, idx, bin_of, chunk_of,
-                       bin_row0, chunk_src0, nbins, nchunks, nhot, bin_major, key, pay, pay2);
+    if (wt && (rc = eg_dalloc_t(&pay3, m, "binned weights")))
+      break;
+    const uint32_t grid =
+        src.row ? std::min<uint64_t>(blocks_for(m, 256), 65536)
+                : std::min<uint64_t>(blocks_for((uint64_t)n * 64, 256), 65536);
+    hipLaunchKernelGGL(k_pb_keys, dim3(grid), dim3(256), 0, s, n, m, src, bin_of, chunk_of,
+                       bin_row0, chunk_src0, nbins, nchunks, nhot, bin_major, key, pay, pay2,
+                       wt ? pay3 : nullptr);
     if (hipGetLastError() != hipSuccess) {
       rc = eg_fail(EG_EDEVICE, "binned key launch");
       break;
     }
     void *tmp = nullptr;
     EgSortCols c;
-    c.np = bin_major ? 2 : 1;
+    c.np = bin_major ? (wt ? 3 : 2) : 1;
     c.in[0] = pay;
     c.out[0] = spay;
     c.in[1] = pay2;
     c.out[1] = spay2;
+    c.in[2] = pay3;
+    c.out[2] = reinterpret_cast<uint32_t *>(wout);
     if ((rc = eg_scratch(g, eg_radix_tmp_bytes(m, c.np), &tmp))) break;
     if ((rc = eg_radix_sort(s, m, eg_bits_for(nkeys), key, skey, c, tmp))) break;
     hipLaunchKernelGGL(k_pb_narrow, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, spay, out, skey,
@@ -644,6 +743,7 @@ int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32
   eg_dfree_t(&key);
   eg_dfree_t(&pay);
   eg_dfree_t(&pay2);
+  eg_dfree_t(&pay3);
   eg_dfree_t(&skey);
   eg_dfree_t(&spay);
   eg_dfree_t(&spay2);
@@ -651,14 +751,34 @@ int pb_order(eg_graph *g, const uint32_t *ptr, const uint32_t *idx, const uint32
   return rc;
 }
 
+// Per-vertex counts of a u32 array of m ids on the host.
+int pb_counts(eg_graph *g, const uint32_t *ids, std::vector<uint32_t> &cnt) {
+  const uint32_t n = g->n;
+  hipStream_t s = g->stream;
+  cnt.assign(n, 0u);
+  uint32_t *d = nullptr;
+  EG_TRY(eg_dalloc_t(&d, n, "binned counts"));
+  int rc = EG_OK;
+  if (hipMemsetAsync(d, 0, (size_t)n * 4, s) != hipSuccess)
+    rc = eg_fail(EG_EDEVICE, "binned count clear");
+  if (!rc) rc = eg_histogram(s, ids, g->m, d, n);
+  if (!rc && (hipMemcpyAsync(cnt.data(), d, (size_t)n * 4, hipMemcpyDeviceToHost, s) !=
+                  hipSuccess ||
+              hipStreamSynchronize(s) != hipSuccess))
+    rc = eg_fail(EG_EDEVICE, "binned count download");
+  eg_dfree_t(&d);
+  return rc;
+}
+
 }  // namespace
 
-// Builds (on first use for this orientation) the binned plan of the pull
-// orientation (ptr, idx): row v's entries are the sources u of its in-edges.
-int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
+// Builds (on first use for this entry source) the binned plan of src: the
+// entries of row v are the sources u of its in-edges.
+int eg_pb_build(eg_graph *g, const PbSource &src) {
   const uint32_t want_tile = g->tune.pr_tile > 0 ? (uint32_t)g->tune.pr_tile : kTileLen;
   const int want_hot = g->tune.pr_hot;
-  if (g->pb && g->pb->for_ptr == ptr && g->pb->tile_len == want_tile && g->pb->hot_req == want_hot)
+  const void *key = src.row ? (const void *)src.row : (const void *)src.ptr;
+  if (g->pb && g->pb->for_key == key && g->pb->tile_len == want_tile && g->pb->hot_req == want_hot)
     return EG_OK;
   eg_pb_free(g);
   const uint32_t n = g->n;
@@ -667,31 +787,33 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
   if (m == 0 || n == 0) return eg_fail(EG_ENOTSUP, "binned PageRank needs entries");
   int cus = 0;
   EG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+  PbSrcDev sd;
+  sd.ptr = src.ptr;
+  sd.row = src.row;
+  sd.idx = src.idx;
+  sd.w = src.w;
   // 1. row lengths and source counts on the host
-  std::vector<uint32_t> hptr((size_t)n + 1), scnt(n);
-  {
-    uint32_t *cnt = nullptr;
-    EG_TRY(eg_dalloc_t(&cnt, n, "binned source counts"));
-    int rc = EG_OK;
-    if (hipMemsetAsync(cnt, 0, (size_t)n * 4, s) != hipSuccess)
-      rc = eg_fail(EG_EDEVICE, "binned count clear");
-    if (!rc) rc = eg_histogram(s, idx, m, cnt, n);
-    if (!rc && (hipMemcpyAsync(scnt.data(), cnt, (size_t)n * 4, hipMemcpyDeviceToHost, s) !=
-                    hipSuccess ||
-                hipMemcpyAsync(hptr.data(), ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, s) !=
-                    hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess))
-      rc = eg_fail(EG_EDEVICE, "binned degree download");
-    eg_dfree_t(&cnt);
-    EG_TRY(rc);
+  std::vector<uint32_t> hptr((size_t)n + 1), scnt;
+  if (src.row) {
+    std::vector<uint32_t> rcnt;
+    EG_TRY(pb_counts(g, src.row, rcnt));
+    hptr[0] = 0;
+    for (uint32_t v = 0; v < n; v++) hptr[v + 1] = hptr[v] + rcnt[v];
+  } else {
+    EG_HIP(hipMemcpyAsync(hptr.data(), src.ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
   }
+  EG_TRY(pb_counts(g, src.idx, scnt));
   // 2. hot sources, bins of rows and chunks of the other sources, each
-  // balanced by entries.  With EG_PB_GEOMETRY=small in the environment every
+  // balanced by entries.  The hot copy pays only where the first sources take
+  // a fair share of the entries (degree-ordered or R-MAT ids), so it is
+  // dropped below 15%.  With EG_PB_GEOMETRY=small in the environment every
   // cap is cut down so that small graphs exercise hot and cold entries in one
   // bin, long-row parts and tile cuts (tests).
   uint32_t rows_cap = kBinRows, src_cap = kChunkSrc, tile_len = want_tile, tile_bins = kTileBins;
   uint32_t hot_cap = want_hot > 0 ? (uint32_t)want_hot : 0u;
   uint64_t target = std::max<uint64_t>(m / 2048, 4096);
+  double min_hot_share = 0.15;
   {
     const char *e = getenv("EG_PB_GEOMETRY");
     if (e && !strcmp(e, "small")) {
@@ -701,15 +823,23 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
       tile_bins = 16;
       target = 256;
       hot_cap = std::min<uint32_t>(hot_cap, 40);
+      min_hot_share = 0.0;
     }
   }
-  const uint32_t nhot = std::min<uint32_t>(hot_cap, n);
+  uint32_t nhot = std::min<uint32_t>(hot_cap, n);
+  {
+    uint64_t hot_entries = 0;
+    for (uint32_t u = 0; u < nhot; u++) hot_entries += scnt[u];
+    if ((double)hot_entries < min_hot_share * (double)m) nhot = 0;
+  }
+  uint32_t max_rowlen = 0;
   std::vector<uint32_t> bin_row0, bin_of(n), chunk_src0, chunk_of(n, 0u);
   {
     uint32_t rows = 0;
     uint64_t ents = 0;
     for (uint32_t v = 0; v < n; v++) {
       const uint32_t dv = hptr[v + 1] - hptr[v];
+      max_rowlen = std::max(max_rowlen, dv);
       if (rows == 0 || rows + 1 > rows_cap || ents + dv > target) {
         bin_row0.push_back(v);
         rows = 0;
@@ -745,9 +875,11 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
   p->nchunks = nchunks;
   p->nhot = nhot;
   p->cus = (uint32_t)cus;
+  p->max_rowlen = max_rowlen;
   std::vector<uint32_t> seg_bm, seg_cm;
   {
     uint32_t *d_bin_of = nullptr, *d_chunk_of = nullptr, *d_row0 = nullptr, *d_src0 = nullptr;
+    uint32_t *d_wmax = nullptr;
     int rc = EG_OK;
     do {
       if ((rc = upload(g, &d_bin_of, bin_of, "binned bin_of")) ||
@@ -759,17 +891,37 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
           (rc = eg_dalloc_t(&p->sloc, m, "binned source offsets")) ||
           (rc = eg_dalloc_t(&p->val, m + 16, "binned values")))
         break;
-      if ((rc = pb_order(g, ptr, idx, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot,
-                         1, p->dloc, reinterpret_cast<uint32_t *>(p->val), seg_bm)))
+      if (src.w) {
+        if ((rc = eg_dalloc_t(&p->w, m + 16, "binned weights")) ||
+            (rc = eg_dalloc_t(&d_wmax, 1, "binned weight max")))
+          break;
+        uint32_t wbits = 0;
+        if (hipMemsetAsync(d_wmax, 0, 4, s) != hipSuccess) {
+          rc = eg_fail(EG_EDEVICE, "binned weight max clear");
+          break;
+        }
+        hipLaunchKernelGGL(k_pb_absmax, dim3(std::min<uint64_t>(blocks_for(m, 256), 4096)),
+                           dim3(256), 0, s, m, src.w, d_wmax);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&wbits, d_wmax, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+          rc = eg_fail(EG_EDEVICE, "binned weight max");
+          break;
+        }
+        memcpy(&p->wmax, &wbits, 4);
+      }
+      if ((rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, 1,
+                         p->dloc, reinterpret_cast<uint32_t *>(p->val), p->w, seg_bm)))
         break;
-      rc = pb_order(g, ptr, idx, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, 0,
-                    p->sloc, nullptr, seg_cm);
+      rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, 0,
+                    p->sloc, nullptr, nullptr, seg_cm);
     } while (0);
     (void)hipStreamSynchronize(s);
     eg_dfree_t(&d_bin_of);
     eg_dfree_t(&d_chunk_of);
     eg_dfree_t(&d_row0);
     eg_dfree_t(&d_src0);
+    eg_dfree_t(&d_wmax);
     if (rc) {
       eg_pb_free(g);
       return rc;
@@ -894,7 +1046,7 @@ int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx) {
     eg_pb_free(g);
     return rc;
   }
-  p->for_ptr = ptr;
+  p->for_key = key;
   return EG_OK;
 }
 
@@ -909,10 +1061,12 @@ int eg_pb_scale(double bound) {
   return 62 - (int)ceil(log2(bound));
 }
 
-// One PageRank iteration from contrib (the input iterate's contributions) into
-// y / a.contrib_new, with the statistics of eg_pull.hip.
-int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, int S,
-                  hipStream_t stream) {
+// Launches one pass of the binned kernels from x into y.  PageRank (spmv
+// false): x is the input iterate's contributions and a carries the update
+// and the statistics of eg_pull.hip.  SpMV: y[v] = sum of w * x[u] (w = 1
+// without weights); a is not used.
+static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int S,
+                     hipStream_t stream, bool spmv) {
   const PbPlan *p = g->pb;
   if (!p) return eg_fail(EG_ESTATE, "binned plan not built");
   hipStream_t s = stream ? stream : g->stream;
@@ -933,39 +1087,66 @@ int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, 
   d.nitems = p->nitems;
   d.nlong = p->nlong;
   d.nhot = p->nhot;
+  d.w = spmv ? p->w : nullptr;
+  const bool wt = d.w != nullptr;
   {
-    EgProf prof(g, "pr_scatter", s);
+    EgProf prof(g, spmv ? "spmv_scatter" : "pr_scatter", s);
     const int sb = g->tune.pr_scatter;
     if (p->ntiles == 0) {
       // every source is hot: nothing to scatter
     } else if (sb == 8)
-      hipLaunchKernelGGL((k_pb_scatter<8>), dim3(p->ntiles), dim3(kBlock), 0, s, d, contrib);
+      hipLaunchKernelGGL((k_pb_scatter<8>), dim3(p->ntiles), dim3(kBlock), 0, s, d, x);
     else if (sb == 32)
-      hipLaunchKernelGGL((k_pb_scatter<32>), dim3(p->ntiles), dim3(kBlock), 0, s, d, contrib);
+      hipLaunchKernelGGL((k_pb_scatter<32>), dim3(p->ntiles), dim3(kBlock), 0, s, d, x);
     else
-      hipLaunchKernelGGL((k_pb_scatter<16>), dim3(p->ntiles), dim3(kBlock), 0, s, d, contrib);
+      hipLaunchKernelGGL((k_pb_scatter<16>), dim3(p->ntiles), dim3(kBlock), 0, s, d, x);
     EG_LAUNCHED();
   }
   {
-    EgProf prof(g, "pr_accum", s);
-    d.persistent = g->tune.pr_persist ? 1u : 0u;
-    if (p->nhot)
-      hipLaunchKernelGGL(k_pb_accum_hot,
-                         dim3(d.persistent ? std::min(p->nitems, p->cus) : p->nitems),
-                         dim3(kBlock), 0, s, d, S, y, a, contrib);
-    else {
-      d.persistent = 0u;
-      hipLaunchKernelGGL(k_pb_accum_cold, dim3(p->nitems), dim3(kBlock), 0, s, d, S, y, a,
-                         contrib);
+    EgProf prof(g, spmv ? "spmv_accum" : "pr_accum", s);
+    d.persistent = p->nhot && g->tune.pr_persist ? 1u : 0u;
+    const dim3 grid(d.persistent ? std::min(p->nitems, p->cus) : p->nitems);
+    if (p->nhot) {
+      if (!spmv)
+        hipLaunchKernelGGL((k_pb_accum_hot<false, false>), grid, dim3(kBlock), 0, s, d, S, y, a, x);
+      else if (wt)
+        hipLaunchKernelGGL((k_pb_accum_hot<true, true>), grid, dim3(kBlock), 0, s, d, S, y, a, x);
+      else
+        hipLaunchKernelGGL((k_pb_accum_hot<true, false>), grid, dim3(kBlock), 0, s, d, S, y, a, x);
+    } else {
+      if (!spmv)
+        hipLaunchKernelGGL((k_pb_accum_cold<false, false>), grid, dim3(kBlock), 0, s, d, S, y, a, x);
+      else if (wt)
+        hipLaunchKernelGGL((k_pb_accum_cold<true, true>), grid, dim3(kBlock), 0, s, d, S, y, a, x);
+      else
+        hipLaunchKernelGGL((k_pb_accum_cold<true, false>), grid, dim3(kBlock), 0, s, d, S, y, a, x);
     }
     EG_LAUNCHED();
   }
-  {
-    EgProf prof(g, "pr_finish", s);
-    hipLaunchKernelGGL(k_pb_finish, dim3(kFinBlocks), dim3(256), 0, s, d, S, y, a);
+  if (!spmv || p->nlong) {
+    EgProf prof(g, spmv ? "spmv_finish" : "pr_finish", s);
+    if (spmv)
+      hipLaunchKernelGGL((k_pb_finish<true>), dim3(kFinBlocks), dim3(256), 0, s, d, S, y, a);
+    else
+      hipLaunchKernelGGL((k_pb_finish<false>), dim3(kFinBlocks), dim3(256), 0, s, d, S, y, a);
     EG_LAUNCHED();
   }
   return EG_OK;
+}
+
+int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, int S,
+                  hipStream_t stream) {
+  return pb_launch(g, contrib, y, a, S, stream, false);
+}
+
+// SpMV pass y = A^T x over the built plan, x and y in device memory; xmax =
+// max |x[u]| bounds the terms: a row sum is at most wmax * longest row * xmax.
+int eg_pb_spmv(eg_graph *g, const float *x, float *y, double xmax, hipStream_t stream) {
+  const PbPlan *p = g->pb;
+  if (!p) return eg_fail(EG_ESTATE, "binned plan not built");
+  const double bound = (double)p->wmax * (double)p->max_rowlen * xmax * 1.001;
+  PrArgs a = {};
+  return pb_launch(g, x, y, a, eg_pb_scale(bound), stream, true);
 }
 
 extern "C" int eg_pr_binned_geometry(eg_graph *g, uint32_t *out) {
@@ -978,5 +1159,6 @@ extern "C" int eg_pr_binned_geometry(eg_graph *g, uint32_t *out) {
   out[3] = p->nitems;
   out[4] = p->nlong;
   out[5] = (uint32_t)std::min<uint64_t>(p->nsegs, 0xFFFFFFFFull);
+  out[6] = p->nhot;
   return EG_OK;
 }

@@ -27,7 +27,11 @@
 // reference's CAS add); the sum order is not deterministic.
 #include <math.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "eg_internal.h"
+#include "eg_pr.h"
 
 namespace {
 
@@ -435,6 +439,47 @@ int eg_pr_finish(eg_graph *g, int iters, float *rank, float *spare, float *rank_
                  double *lh, const uint32_t *unperm);
 static int pr_state(eg_graph *g, int iters) { return eg_pr_state(g, iters); }
 
+// The cells' entries as an edge list for the binned kernels: (dst, src)
+// pairs of the grid or of the edge array.
+static PbSource cells_source(const eg_graph *g, bool grid) {
+  PbSource src;
+  src.row = grid ? g->g_dst : g->el_dst;
+  src.idx = grid ? g->g_src : g->el_src;
+  src.w = grid ? g->g_w : g->el_w;
+  return src;
+}
+
+// PageRank over the cells' entries on the binned kernels (exact row sums;
+// the same recurrence and statistics as the adjacency driver).
+static int pagerank_binned_cells(eg_graph *g, int iters, double damping, double init,
+                                 int uniform, float *rank_out, double *dh, double *lh, bool grid) {
+  EG_TRY(pr_state(g, iters));
+  const uint32_t n = g->n;
+  EG_TRY(eg_pb_build(g, cells_source(g, grid)));
+  const int S = eg_pb_scale(std::max(init * (double)n, 1.0) * 1.001);
+  float *rank = g->rank_a, *rank_next = g->rank_b;
+  float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
+  EG_TRY(eg_pr_start(g, init, g->out_deg, rank, contrib));
+  PrArgs a;
+  a.out_deg = g->out_deg;
+  a.base_vec = nullptr;
+  a.base = (float)((1.0 - damping) / (double)n);
+  a.damping = (float)damping;
+  a.damping_over_n = damping / (double)n;
+  a.uniform = uniform;
+  a.hot = 0xFFFFFFFFu;
+  for (int it = 0; it < iters; it++) {
+    a.rank_old = rank;
+    a.contrib_new = contrib_next;
+    a.dmass_in = g->pr_stats + 2 * it;
+    a.stats_out = g->pr_stats + 2 * (it + 1);
+    EG_TRY(eg_pb_iterate(g, contrib, rank_next, a, S, nullptr));
+    std::swap(rank, rank_next);
+    std::swap(contrib, contrib_next);
+  }
+  return eg_pr_finish(g, iters, rank, rank_next, rank_out, dh, lh, nullptr);
+}
+
 // Shared PageRank driver for the grid and edge-array layouts.
 static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, double *dh,
                           double *lh, bool grid) {
@@ -443,13 +488,13 @@ static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, dou
   eg_pr_resolve(g, o, &damping, &iters, &init, &uniform);
   if (iters < 0 || !(damping >= 0.0 && damping <= 1.0))
     return eg_fail(EG_EINVAL, "bad PageRank options");
+  if (grid && !g->cell_ptr) return eg_fail(EG_ENOTSUP, "load mode %d has no grid", g->load_mode);
+  if (!grid && !g->el_src)
+    return eg_fail(EG_ENOTSUP, "load mode %d has no edge array", g->load_mode);
+  if (eg_pb_selected(g)) return pagerank_binned_cells(g, iters, damping, init, uniform, rank_out,
+                                                      dh, lh, grid);
   GridCompute *c = grid_state(g, true);
-  if (grid) {
-    if (!g->cell_ptr) return eg_fail(EG_ENOTSUP, "load mode %d has no grid", g->load_mode);
-    EG_TRY(grid_build_compute(g, c));
-  } else {
-    if (!g->el_src) return eg_fail(EG_ENOTSUP, "load mode %d has no edge array", g->load_mode);
-  }
+  if (grid) EG_TRY(grid_build_compute(g, c));
   EG_TRY(pr_state(g, iters));
   const uint32_t n = g->n;
   hipStream_t s = g->stream;
@@ -549,7 +594,19 @@ static int spmv_host(eg_graph *g, const float *x, float *y, bool grid) {
   const uint32_t n = g->n;
   hipStream_t s = g->stream;
   EG_HIP(hipMemcpyAsync(g->contrib_a, x, (size_t)n * 4, hipMemcpyHostToDevice, s));
-  if (grid) {
+  // Binned kernels sum exactly; their scale needs a finite bound on |x|.
+  double xmax = 0.0;
+  bool finite = true;
+  if (eg_pb_selected(g))
+    for (uint32_t i = 0; i < n && finite; i++) {
+      const double ax = fabs((double)x[i]);
+      finite = std::isfinite(ax);
+      xmax = std::max(xmax, ax);
+    }
+  if (eg_pb_selected(g) && finite) {
+    EG_TRY(eg_pb_build(g, cells_source(g, grid)));
+    EG_TRY(eg_pb_spmv(g, g->contrib_a, g->rank_a, xmax));
+  } else if (grid) {
     EG_TRY(eg_spmv_grid_device(g, g->contrib_a, g->rank_a));
   } else {
     EG_HIP(hipMemsetAsync(g->rank_a, 0, (size_t)n * 4, s));

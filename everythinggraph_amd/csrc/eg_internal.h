@@ -192,10 +192,19 @@ void eg_grid_free(eg_graph *g);
 void eg_batch_free(eg_graph *g);
 void eg_order_free(eg_graph *g);
 void eg_pb_free(eg_graph *g);
-// Binned PageRank (eg_binned.hip): the plan of the pull orientation (ptr,
-// idx), built on first use; the fixed-point scale for row sums <= bound.
-int eg_pb_build(eg_graph *g, const uint32_t *ptr, const uint32_t *idx);
+// Binned PageRank / SpMV (eg_binned.hip).  The entries of row v are the
+// sources u of its in-edges, from an adjacency orientation (ptr[n+1], idx[m])
+// or an edge list (row[k], idx[k]; ptr null); w (may be null): entry weights,
+// kept for SpMV.  The plan is built on first use for that source; the
+// fixed-point scale for row sums <= bound; an SpMV pass y = A^T x over the
+// plan (x, y on the device, xmax = max |x|).
+struct PbSource {
+  const uint32_t *ptr = nullptr, *row = nullptr, *idx = nullptr;
+  const float *w = nullptr;
+};
+int eg_pb_build(eg_graph *g, const PbSource &src);
 int eg_pb_scale(double bound);
+int eg_pb_spmv(eg_graph *g, const float *x, float *y, double xmax, hipStream_t stream = nullptr);
 // True when PageRank on adjacency lists uses the binned kernels.
 bool eg_pb_selected(const eg_graph *g);
 bool eg_reorder_enabled();
@@ -222,6 +231,7 @@ int eg_ppr_single(eg_graph *g, const uint32_t *seeds, uint32_t ns, double alpha,
 // result's download to host memory and waits for it.
 struct Ppr1 {
   const uint32_t *ptr = nullptr, *idx = nullptr, *deg = nullptr, *unperm = nullptr;
+  const float *w = nullptr;  // the orientation's weights (the binned plan keeps them)
   PullPlan *pl = nullptr;
   float *rank = nullptr, *rank_next = nullptr, *contrib = nullptr, *contrib_next = nullptr;
   double alpha = 0.15;

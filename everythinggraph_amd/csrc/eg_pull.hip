@@ -24,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "eg_internal.h"
 #include "eg_pr.h"
@@ -661,7 +662,11 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
   const bool binned = eg_pb_selected(g);
   int S = 0;
   if (binned) {
-    EG_TRY(eg_pb_build(g, ptr, idx));
+    PbSource src;
+    src.ptr = ptr;
+    src.idx = idx;
+    src.w = w;  // kept for SpMV over the same plan
+    EG_TRY(eg_pb_build(g, src));
     S = eg_pb_scale(std::max(init * (double)n, 1.0) * 1.001);
   }
   float *rank = g->rank_a, *rank_next = g->rank_b;
@@ -728,19 +733,23 @@ int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, dou
     EG_TRY(eg_ordered(g, &o));
     st->ptr = o->ptr;
     st->idx = o->idx;
+    st->w = o->w;
     st->pl = &o->plan;
     st->deg = o->deg;
     st->unperm = perm = o->new_of_old;
   } else {
-    const float *w;
-    EG_TRY(eg_pull_orientation(g, &st->ptr, &st->idx, &w, &st->pl));
+    EG_TRY(eg_pull_orientation(g, &st->ptr, &st->idx, &st->w, &st->pl));
   }
   EG_TRY(eg_pr_state(g, max_iter));
   const uint32_t n = g->n;
   // the iterate is a probability vector: row sums stay below 1
   st->binned = eg_pb_selected(g);
   if (st->binned) {
-    EG_TRY(eg_pb_build(g, st->ptr, st->idx));
+    PbSource src;
+    src.ptr = st->ptr;
+    src.idx = st->idx;
+    src.w = st->w;
+    EG_TRY(eg_pb_build(g, src));
     st->scale = eg_pb_scale(1.001);
   }
   hipStream_t s = stream;
@@ -882,8 +891,26 @@ extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
   } else {
     EG_HIP(hipMemcpyAsync(g->contrib_a, x, (size_t)n * 4, hipMemcpyHostToDevice, s));
   }
-  PrArgs a = {};
-  EG_TRY(launch_pull(g, false, ptr, idx, w, xin, yout, *pl, a));
+  // Binned kernels sum exactly; their scale needs a finite bound on |x|.
+  double xmax = 0.0;
+  bool finite = true;
+  if (eg_pb_selected(g))
+    for (uint32_t i = 0; i < n && finite; i++) {
+      const double ax = fabs((double)x[i]);
+      finite = std::isfinite(ax);
+      xmax = std::max(xmax, ax);
+    }
+  if (eg_pb_selected(g) && finite) {
+    PbSource src;
+    src.ptr = ptr;
+    src.idx = idx;
+    src.w = w;
+    EG_TRY(eg_pb_build(g, src));
+    EG_TRY(eg_pb_spmv(g, xin, yout, xmax));
+  } else {
+    PrArgs a = {};
+    EG_TRY(launch_pull(g, false, ptr, idx, w, xin, yout, *pl, a));
+  }
   if (o) {
     EG_TRY(eg_gather_vec(g, g->rank_a, o->new_of_old, g->rank_b));
     yout = g->rank_b;

@@ -136,6 +136,7 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
   if (kernels & EG_TUNE_GRID) {
     if (!g->cell_ptr) return eg_fail(EG_ENOTSUP, "grid tuning needs a grid layout");
     eg_pr_opts o = {0.85, 3, EG_PR_DANGLING_DROP, 0.0, EG_LAYOUT_GRID};
+    g->tune.pr_binned = 0;  // the bands are timed on the grid kernel
     EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // builds the compute copy
     const int bands[4] = {1, 2, 4, 0};
     for (int b : bands) {
@@ -145,6 +146,20 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
       EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); },
                        &ms));
       res.push_back({"grid_band", b, ms / o.iterations});
+    }
+    double best = 1e300;
+    for (const Result &r : res)
+      if (!strcmp(r.param, "grid_band") && r.ms < best) {
+        best = r.ms;
+        g->tune.grid_band = r.value;
+      }
+    // the binned kernels over the grid's entries against the best band
+    for (int pb = 0; pb <= 1; pb++) {
+      g->tune.pr_binned = pb;
+      double ms;
+      EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // warm (binned plan)
+      EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); }, &ms));
+      res.push_back({"pr_binned", pb, ms / o.iterations});
     }
   }
   const int variants[2] = {8, 16};

@@ -252,12 +252,12 @@ typedef struct {
 } eg_tune_entry;
 int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, int32_t cap,
                 int32_t *count);
-/* Geometry of the binned PageRank plan of g (built by the first eg_pagerank
- * or eg_ppr call that uses the binned kernels; no reference counterpart):
- * out[0..5] =
+/* Geometry of the binned plan of g (built by the first eg_pagerank, eg_ppr
+ * or eg_spmv call that uses the binned kernels; no reference counterpart):
+ * out[0..6] =
  * destination bins, source chunks, scatter tiles, accumulation work items,
- * rows summed in parts, non-empty (chunk, bin) segments.  EG_ESTATE if no
- * plan exists. */
+ * rows summed in parts, non-empty (chunk, bin) segments, hot sources (read
+ * from LDS in phase 2).  EG_ESTATE if no plan exists. */
 int eg_pr_binned_geometry(eg_graph *g, uint32_t *out);
 int eg_tune_set(eg_graph *g, const char *param, int32_t value);
 int eg_tune_get(eg_graph *g, const char *param, int32_t *value);

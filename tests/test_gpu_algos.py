@@ -478,7 +478,9 @@ def test_grid_bands_bit_identical(mode):
             g.tune_set("grid_band", band)
             outs.append((g.pagerank(0.85, 20, layout="grid")[0], g.spmv(x, layout="grid")))
         rep = g.autotune(eg.TUNE_GRID)
-        assert sorted(r[1] for r in rep) == [0, 1, 2, 4] and sum(r[3] for r in rep) == 1
+        bands = [r for r in rep if r[0] == "grid_band"]
+        assert sorted(r[1] for r in bands) == [0, 1, 2, 4] and sum(r[3] for r in bands) == 1
+        assert sorted(r[1] for r in rep if r[0] == "pr_binned") == [0, 1]
     for pr, y in outs[1:]:
         assert np.array_equal(pr, outs[0][0]) and np.array_equal(y, outs[0][1])
     _, _, _, ptr, col, ow, cp, row, iw = oracle_adj(n, s0, d0, w)

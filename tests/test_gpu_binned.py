@@ -3,8 +3,10 @@
 The binned kernels sum every row exactly in 64-bit fixed point, so their
 ranks must match the oracle to the PageRank tolerance (L1 <= 1e-6, max error
 <= 1e-6 of the largest rank), repeat bit for bit, and not depend on how rows
-are cut into bins, sources into chunks, or vertices ordered: those cuts only
-change the order of integer additions.
+are cut into bins, sources into chunks, or vertices ordered, nor on the layout
+the entries come from (CSR, 2-D grid, edge array): those only change the
+order of integer additions.  SpMV sums the FP32 products w * x exactly and
+is checked against the FP64 oracle at |y - y64| <= 1e-5 (|A| |x|).
 """
 import os
 import subprocess
@@ -181,3 +183,95 @@ def test_binned_independent_of_cut_and_order(tmp_path):
         assert long_rows > 0 and items > bins  # hub rows summed in parts
         assert tiles > chunks                  # chunk regions cut into several tiles
         assert bins > 16                       # tiles also cut by the bins they span
+
+
+LAYOUTS = [(2, 0), (0, 16), (1, 4), (6, 25), (7, 0), (8, 0), (9, 0)]
+
+
+@pytest.mark.parametrize("undirected", [False, True])
+def test_binned_pagerank_every_layout(undirected):
+    """Adjacency, grid (modes 0/1/6) and edge-array (7/8/9) PageRank on the
+    binned kernels: within the tolerance of the oracle and bit-identical to
+    each other (hot and scattered sources both present at this size)."""
+    n = 1 << 15
+    s0, d0 = rmat(15, seed=61)
+    ref, rdh, rlh = oracle_pr(n, s0, d0, undirected, 20)
+    out = {}
+    for mode, Pp in LAYOUTS:
+        with eg.Graph(n, mode, undirected=undirected, P=Pp) as g:
+            g.load_edges(s0, d0)
+            g.tune_set("pr_binned", 1)
+            out[mode] = g.pagerank(0.85, 20)
+    for mode, (r, dh, lh) in out.items():
+        err = np.abs(r.astype(np.float64) - ref)
+        assert err.sum() <= PR_L1_TOL and err.max() <= 1e-6 * ref.max(), mode
+        assert np.array_equal(r, out[2][0]), mode
+        np.testing.assert_allclose(dh, rdh, rtol=1e-5, atol=1e-12)
+        np.testing.assert_allclose(lh, rlh, rtol=1e-3, atol=1e-7)
+
+
+def spmv_check(y, ref, absref):
+    assert np.all(np.abs(y.astype(np.float64) - ref) <= 1e-5 * absref + 1e-30)
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_binned_spmv_every_layout(weighted):
+    """SpMV y = A^T x with signed x on the binned kernels, every layout:
+    within 1e-5 (|A| |x|) of the FP64 oracle and bit-identical across
+    layouts (the same FP32 products, summed exactly)."""
+    n = 1 << 15
+    s0, d0 = rmat(15, seed=62)
+    w = O.gen_weights(s0.size, seed=3) if weighted else None
+    ptr, col, ow = O.build_csr(n, s0, d0, w)
+    cp, row, iw = O.build_csc(n, ptr, col, ow)
+    x = np.random.default_rng(7).uniform(-2, 2, n).astype(np.float32)
+    x[::97] = 0.0
+    ref = O.spmv_pull(n, cp, row, iw, x.astype(np.float64))
+    absref = O.spmv_pull(n, cp, row, iw, np.abs(x).astype(np.float64))
+    out = {}
+    for mode, Pp in LAYOUTS:
+        with eg.Graph(n, mode, weighted=weighted, P=Pp) as g:
+            g.load_edges(s0, d0, w)
+            g.tune_set("pr_binned", 1)
+            out[mode] = g.spmv(x)
+            if mode == 2:
+                geo = g.binned_geometry()
+    for mode, y in out.items():
+        spmv_check(y, ref, absref)
+        assert np.array_equal(y, out[2]), mode
+    assert geo["hot"] == 16384  # the first sources of an R-MAT graph take many entries
+
+
+def test_binned_spmv_nonfinite_falls_back():
+    """An x with inf / NaN has no finite fixed-point scale: the pull kernel
+    runs instead and propagates them as IEEE arithmetic does."""
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=63)
+    x = np.ones(n, np.float32)
+    x[5] = np.inf
+    x[9] = np.nan
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s0, d0)
+        g.tune_set("pr_binned", 0)
+        y0 = g.spmv(x)
+        g.tune_set("pr_binned", 1)
+        y1 = g.spmv(x)
+    assert np.array_equal(y0, y1, equal_nan=True)
+    assert not np.isfinite(y1).all()
+
+
+def test_binned_uniform_graph_has_no_hot_set():
+    """Erdos-Renyi ids carry no degree skew: the first 16384 sources take
+    under 15% of the entries, so the plan scatters every source."""
+    n = 1 << 18
+    s0, d0 = O.gen_er(n, 1 << 21, seed=64)
+    ptr, col, _ = O.build_csr(n, s0, d0)
+    cp, row, _ = O.build_csc(n, ptr, col)
+    ref, _, _ = O.pagerank_pull(n, cp, row, np.diff(ptr).astype(np.uint32), 0.85, 10)
+    with eg.Graph(n, 0, P=16) as g:
+        g.load_edges(s0, d0)
+        g.tune_set("pr_binned", 1)
+        r, _, _ = g.pagerank(0.85, 10)
+        geo = g.binned_geometry()
+    assert geo["hot"] == 0 and geo["chunks"] > 1
+    assert np.abs(r.astype(np.float64) - ref).sum() <= PR_L1_TOL
