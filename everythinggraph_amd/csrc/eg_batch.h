@@ -12,6 +12,12 @@ struct BatchView {
   const uint32_t *ptr, *idx, *deg, *perm;
   PullPlan *pl;
   bool sym;
+  // Chunks [static_chunk0, nchunks) hold only rows without entries and
+  // without out-edges: their PPR values are the seed term (or 0) and their
+  // contributions 0 from the first step on, so once two full steps have run
+  // since the last column change both iterate buffers hold them and the steps
+  // skip those chunks.
+  uint32_t static_chunk0;
 };
 int eg_batch_view(eg_graph *g, BatchView *v);
 
@@ -71,6 +77,7 @@ struct PprState {
   uint64_t active = ~0ull;    // columns in use: steps gather only up to the highest one
   uint32_t max_grid = 0;      // workgroups of the SpMM launch (0: one per work item)
   bool capturing = false;     // a step is being recorded into a hipGraph
+  uint32_t full_steps = 0;    // steps over every chunk since the last column change
 };
 int eg_ppr_alloc(eg_graph *g, PprState *st);
 void eg_ppr_free(PprState *st);

@@ -711,8 +711,11 @@ __global__ __launch_bounds__(kBlock) void k_ppr_finish(
 // contiguous slice (four quarter-slices, merged in order) into part2[b];
 // k_ppr_delta_final sums part2 in order.
 constexpr uint32_t kDeltaBlocks = 512;
+// Partials [skip_lo, skip_hi) belong to skipped static chunks, whose deltas
+// are exactly 0: they add nothing, which gives the same bits as adding them.
 __global__ __launch_bounds__(kBlock) void k_ppr_delta_partial(const double *__restrict__ dpart,
-                                                              uint32_t np,
+                                                              uint32_t np, uint32_t skip_lo,
+                                                              uint32_t skip_hi,
                                                               double *__restrict__ part2) {
   __shared__ double s_d[kWaves][64];
   const int lane = __lane_id();
@@ -724,7 +727,8 @@ __global__ __launch_bounds__(kBlock) void k_ppr_delta_partial(const double *__re
   const uint32_t a = b0 + wave * q < b1 ? b0 + wave * q : b1;
   const uint32_t e = a + q < b1 ? a + q : b1;
   double t = 0.0;
-  for (uint32_t i = a; i < e; i++) t += dpart[(uint64_t)i * 64 + lane];
+  for (uint32_t i = a; i < e; i++)
+    if (i < skip_lo || i >= skip_hi) t += dpart[(uint64_t)i * 64 + lane];
   s_d[wave][lane] = t;
   __syncthreads();
   if (wave == 0) {
@@ -750,7 +754,55 @@ __global__ void k_ppr_extract(uint32_t n, uint32_t k, uint32_t col0, const float
   out[t] = X[(uint64_t)vv * 64 + col0 + q];
 }
 
+// static_from = 1 + the last row with entries or out-edges.
+__global__ void k_static_from(uint32_t n, const uint32_t *__restrict__ ptr,
+                              const uint32_t *__restrict__ deg, uint32_t *__restrict__ out) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t last = 0;
+  if (r < n && (ptr[r + 1] > ptr[r] || deg[r] > 0)) last = r + 1;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) last = max(last, (uint32_t)__shfl_xor(last, o));
+  if (__lane_id() == 0 && last) atomicMax(out, last);
+}
+
+// out[1] = first chunk whose first row is >= out[0] (nchunks if none).
+__global__ void k_first_chunk_from(const uint4 *__restrict__ chunk_info, uint32_t nchunks,
+                                   uint32_t *__restrict__ out) {
+  const uint32_t from = out[0];
+  uint32_t lo = 0, hi = nchunks;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (chunk_info[mid].x >= from) hi = mid; else lo = mid + 1;
+  }
+  out[1] = lo;
+}
+
 }  // namespace
+
+// First chunk of the plan whose rows have neither entries nor out-edges
+// (computed once per plan).
+static int static_chunk0(eg_graph *g, const uint32_t *ptr, const uint32_t *deg, PullPlan *pl) {
+  if (pl->static_chunk0 != 0xFFFFFFFFu) return EG_OK;
+  hipStream_t s = g->stream;
+  uint32_t *d = nullptr;
+  EG_TRY(eg_dalloc_t(&d, 2, "static rows"));
+  uint32_t h[2] = {0, 0};
+  int rc = EG_OK;
+  if (hipMemsetAsync(d, 0, 8, s) != hipSuccess) rc = eg_fail(EG_EDEVICE, "static rows clear");
+  if (!rc && g->n) {
+    hipLaunchKernelGGL(k_static_from, dim3((g->n + 255) / 256), dim3(256), 0, s, g->n, ptr, deg,
+                       d);
+    hipLaunchKernelGGL(k_first_chunk_from, dim3(1), dim3(1), 0, s, pl->chunk_info, pl->nchunks, d);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      rc = eg_fail(EG_EDEVICE, "static rows");
+  }
+  eg_dfree_t(&d);
+  EG_TRY(rc);
+  pl->static_chunk0 = g->n ? h[1] : pl->nchunks;
+  return EG_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Views and state.
@@ -765,12 +817,14 @@ int eg_batch_view(eg_graph *g, BatchView *v) {
     v->deg = o->deg;
     v->perm = o->new_of_old;
     v->pl = &o->plan;
-    return EG_OK;
+  } else {
+    const float *w;
+    EG_TRY(eg_pull_orientation(g, &v->ptr, &v->idx, &w, &v->pl));
+    v->deg = g->out_deg;
+    v->perm = nullptr;
   }
-  const float *w;
-  EG_TRY(eg_pull_orientation(g, &v->ptr, &v->idx, &w, &v->pl));
-  v->deg = g->out_deg;
-  v->perm = nullptr;
+  EG_TRY(static_chunk0(g, v->ptr, v->deg, v->pl));
+  v->static_chunk0 = v->pl->static_chunk0;
   return EG_OK;
 }
 
@@ -1001,6 +1055,7 @@ int eg_ppr_reset(eg_graph *g, PprState *st) {
   EG_LAUNCHED();
   EG_HIP(hipMemsetAsync(st->mask, 0, (size_t)g->n * 8, st->stream));
   memset(st->hw, 0, sizeof(st->hw));
+  st->full_steps = 0;
   return EG_OK;
 }
 
@@ -1015,6 +1070,7 @@ int eg_ppr_col_init(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q,
   }
   st->hw[q] = (float)(1.0 / distinct);
   st->hw[64 + q] = (float)(alpha / distinct);
+  st->full_steps = 0;
   hipStream_t s = st->stream;
   if (st->seeds_cap < ns) {
     EG_HIP(hipStreamSynchronize(s));
@@ -1105,8 +1161,13 @@ int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bo
   const uint32_t np = pl.nchunks + nfin;
   const float oma = (float)(1.0 - alpha);
   const uint32_t width = ppr_width(st->active);
-  const uint32_t nitems = pl.nseg + pl.nchunks;
+  // after two full steps since the last column change the static chunks hold
+  // their final values in both iterate buffers
+  const bool skip = st->full_steps >= 2 && bv.static_chunk0 < pl.nchunks;
+  const uint32_t nchunks = skip ? bv.static_chunk0 : pl.nchunks;
+  const uint32_t nitems = pl.nseg + nchunks;
   const uint32_t grid = st->max_grid && st->max_grid < nitems ? st->max_grid : nitems;
+  if (!skip) st->full_steps++;
   if (grid) {
     EgProf prof(st->capturing ? nullptr : g, "ppr_spmm", s);
     EG_TRY(launch_spmm(g, bv, st, want_delta, grid, nitems, oma, width));
@@ -1125,7 +1186,7 @@ int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bo
   if (want_delta) {
     double *part2 = st->dpart + (uint64_t)np * 64;
     hipLaunchKernelGGL(k_ppr_delta_partial, dim3(kDeltaBlocks), dim3(kBlock), 0, s, st->dpart, np,
-                       part2);
+                       skip ? nchunks : 0u, skip ? pl.nchunks : 0u, part2);
     EG_LAUNCHED();
     hipLaunchKernelGGL(k_ppr_delta_final, dim3(1), dim3(64), 0, s, part2, st->delta);
     EG_LAUNCHED();
@@ -1150,6 +1211,7 @@ int eg_ppr_clear_cols(eg_graph *g, PprState *st, uint64_t cols) {
   EG_LAUNCHED();
   for (int q = 0; q < 64; q++)
     if ((cols >> q) & 1ull) st->hw[q] = st->hw[64 + q] = 0.f;
+  st->full_steps = 0;
   return EG_OK;
 }
 
@@ -1315,9 +1377,11 @@ extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
       rc = eg_ppr_col_init(g, bv, &b->ppr, q, seeds + seed_ptr[q0 + q],
                            seed_ptr[q0 + q + 1] - seed_ptr[q0 + q], alpha);
     int it = 0;
-    if (rc == EG_OK && g->tune.graphs && iterations >= 4) {
-      rc = ppr_graph_steps(g, bv, &b->ppr, alpha, iterations / 2);
-      it = 2 * (iterations / 2);
+    if (rc == EG_OK && g->tune.graphs && iterations >= 6) {
+      // two eager steps first, so that the recorded steps skip static chunks
+      for (; it < 2 && rc == EG_OK; it++) rc = eg_ppr_step(g, bv, &b->ppr, alpha);
+      if (rc == EG_OK) rc = ppr_graph_steps(g, bv, &b->ppr, alpha, (iterations - it) / 2);
+      it += 2 * ((iterations - it) / 2);
     }
     for (; it < iterations && rc == EG_OK; it++) rc = eg_ppr_step(g, bv, &b->ppr, alpha);
     (void)hipEventRecord(e1, s);

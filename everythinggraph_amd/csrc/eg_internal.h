@@ -78,6 +78,9 @@ struct PullPlan {
   double2 *chunk_part = nullptr;  // [nchunks] FP64 (dangling, l1)
   double2 *fin_part = nullptr;    // [kFinBlocks]
   uint32_t *fin_counter = nullptr;
+  // First chunk whose rows all have no entries and no out-edges (such rows'
+  // batched PPR values stop changing); UINT32_MAX until computed.
+  uint32_t static_chunk0 = 0xFFFFFFFFu;
 };
 
 struct GridCompute;
