@@ -870,6 +870,66 @@ static int bitmap_to_queue(eg_graph *g, const uint64_t *bm, uint32_t *q, uint32_
   return EG_OK;
 }
 
+// key[k] = 2^24 - 1 - min(deg(col[k]), 2^24 - 1): ascending keys put the
+// highest-degree neighbours first.
+__global__ void k_deg_desc_keys(uint64_t m, const uint32_t *__restrict__ col,
+                                const uint32_t *__restrict__ deg, uint32_t *__restrict__ key) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) key[k] = 0xFFFFFFu - min(deg[col[k]], 0xFFFFFFu);
+}
+
+// Symmetric graphs: a copy of the CSR columns with every row ordered by
+// descending neighbour degree (stable), built on first use.  A pull level
+// stops at the first neighbour in the frontier, and from the second level on
+// the hubs are in it, so hubs first shortens the scans; levels do not depend
+// on the order, and a parent is still an in-neighbour in the frontier.
+static int bfs_sorted_cols(eg_graph *g) {
+  if (g->bfs_col) return EG_OK;
+  const uint64_t m = g->m;
+  const uint32_t n = g->n;
+  hipStream_t s = g->stream;
+  uint32_t *key = nullptr, *tcol = nullptr, *tsrc = nullptr, *out = nullptr;
+  int rc = EG_OK;
+  do {
+    if ((rc = eg_dalloc_t(&key, m, "bfs order keys")) ||
+        (rc = eg_dalloc_t(&tcol, m, "bfs order columns")) ||
+        (rc = eg_dalloc_t(&tsrc, m, "bfs order rows")) ||
+        (rc = eg_dalloc_t(&out, m, "bfs columns")))
+      break;
+    hipLaunchKernelGGL(k_deg_desc_keys, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, g->out_col,
+                       g->out_deg, key);
+    if (hipGetLastError() != hipSuccess) {
+      rc = eg_fail(EG_EDEVICE, "bfs order keys launch");
+      break;
+    }
+    void *tmp = nullptr;
+    if ((rc = eg_scratch(g, eg_radix_tmp_bytes(m, 2), &tmp))) break;
+    EgSortCols c1;
+    c1.np = 2;
+    c1.in[0] = g->out_col;
+    c1.out[0] = tcol;
+    c1.in[1] = g->out_src;
+    c1.out[1] = tsrc;
+    if ((rc = eg_radix_sort(s, m, 24, key, nullptr, c1, tmp))) break;
+    EgSortCols c2;
+    c2.np = 1;
+    c2.in[0] = tcol;
+    c2.out[0] = out;
+    if ((rc = eg_radix_sort(s, m, eg_bits_for(n), tsrc, nullptr, c2, tmp))) break;
+    if (hipStreamSynchronize(s) != hipSuccess) rc = eg_fail(EG_EDEVICE, "bfs order sort");
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  eg_dfree_t(&key);
+  eg_dfree_t(&tcol);
+  eg_dfree_t(&tsrc);
+  if (rc) {
+    eg_dfree_t(&out);
+    return rc;
+  }
+  g->bfs_col = out;
+  return EG_OK;
+}
+
 // Adjacency BFS: k_bfs_level once per level.  The host only enqueues levels,
 // in batches, and reads the control words after each batch.
 static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_mode,
@@ -894,6 +954,12 @@ static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_
   a.ocol = g->out_col;
   a.iptr = g->in_ptr ? g->in_ptr : g->out_ptr;
   a.iidx = g->in_ptr ? g->in_idx : g->out_col;
+  if ((g->flags & (EG_UNDIRECTED | EG_SYMMETRIC)) && g->out_src && g->tune.bfs_order) {
+    // in-lists are the out-lists: one copy, hubs first, serves both directions
+    EG_TRY(bfs_sorted_cols(g));
+    a.ocol = a.iidx = g->bfs_col;
+    a.iptr = g->out_ptr;
+  }
   a.visited = g->bfs_visited;
   a.F[0] = g->bfs_front;
   a.F[1] = g->bfs_next;

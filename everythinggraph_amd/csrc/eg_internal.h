@@ -119,6 +119,7 @@ struct EgTune {
   int pr_scatter = 16;   // binned scatter: source offsets in flight per lane (8, 16 or 32)
   int pr_hot = 16384;    // binned PageRank: sources below this id are read from an LDS copy
                          // in phase 2 instead of being scattered (0: none, at most 16384)
+  int bfs_order = 1;     // symmetric graphs: BFS on CSR rows ordered hubs first (0: input order)
   int pr_persist = 0;    // binned PageRank with hot sources: phase-2 workgroups loop over
                          // items (one per CU) instead of one workgroup per item
 };
@@ -161,6 +162,7 @@ struct eg_graph {
   uint64_t *bfs_visited = nullptr, *bfs_front = nullptr, *bfs_next = nullptr;
   uint32_t *bfs_q = nullptr, *bfs_nq = nullptr;
   uint64_t *bfs_f3 = nullptr;                    // third frontier bitmap (device-driven BFS)
+  uint32_t *bfs_col = nullptr;  // symmetric graphs: CSR columns, each row by descending degree
   uint32_t *bfs_qoff = nullptr, *bfs_nqoff = nullptr;  // queue edge offsets
   uint32_t bfs_depth_hint = 0;  // levels of the last traversal (first level batch)
   unsigned long long *bfs_part = nullptr;  // per-workgroup partial counts

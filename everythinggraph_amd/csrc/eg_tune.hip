@@ -27,7 +27,9 @@
 //  * pr_scatter  -- binned PageRank scatter: source offsets loaded per lane
 //                   before any value is written, 8, 16 or 32;
 //  * pr_hot      -- binned PageRank: sources below this id are read from an
-//                   LDS copy in phase 2 instead of being scattered (0..16384).
+//                   LDS copy in phase 2 instead of being scattered (0..16384);
+//  * bfs_order   -- symmetric graphs: BFS scans rows ordered hubs first (1) or
+//                   in input order (0); levels are the same, parents may differ.
 // eg_autotune times every variant on the graph itself through the public
 // entry points (device time from HIP events, best of a few repetitions) and
 // keeps the fastest.
@@ -53,6 +55,7 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "pr_scatter")) return &g->tune.pr_scatter;
   if (!strcmp(name, "pr_hot")) return &g->tune.pr_hot;
   if (!strcmp(name, "pr_persist")) return &g->tune.pr_persist;
+  if (!strcmp(name, "bfs_order")) return &g->tune.bfs_order;
   return nullptr;
 }
 
@@ -106,6 +109,8 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
       return eg_fail(EG_EINVAL, "pr_scatter takes 8, 16 or 32");
   } else if (!strcmp(param, "pr_hot")) {
     if (value < 0 || value > 16384) return eg_fail(EG_EINVAL, "pr_hot takes 0..16384");
+  } else if (!strcmp(param, "bfs_order")) {
+    if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "bfs_order takes 0 or 1");
   } else if (!strcmp(param, "pr_persist")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "pr_persist takes 0 or 1");
   } else if (!strcmp(param, "pr_binned")) {

@@ -233,7 +233,10 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * source offsets in flight per lane, 8, 16 or 32); "pr_hot" (binned PageRank:
  * sources below this id, at most 16384, are read from an LDS copy in phase 2
  * instead of being scattered; 0 = none, default 16384); "pr_persist" (1:
- * phase-2 workgroups loop over work items, one per CU; default 0); "grid_band" (grid source
+ * phase-2 workgroups loop over work items, one per CU; default 0); "bfs_order"
+ * (symmetric graphs: 1 = BFS scans rows ordered by descending neighbour degree,
+ * a copy built on first use; 0 = input order; levels are the same, parents may
+ * differ); "grid_band" (grid source
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
  * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
  * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant

@@ -559,3 +559,26 @@ def test_sssp_symmetric_flag_and_errors():
         dist, par, st = g.sssp(9)
         assert st.reached == 1 and dist[9] == 0 and par[9] == 9
         assert np.all(np.delete(dist, 9) == O.SSSP_UNREACHED)
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_bfs_order_same_levels(symmetric):
+    """Symmetric graphs: BFS over rows ordered hubs first (bfs_order 1) and in
+    input order (0) give the same levels, and legal parent trees."""
+    n = 1 << 13
+    s0, d0 = rmat(13, seed=66)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    roots = eg.sample_roots(np.diff(ptr), 5, seed=3)
+    with eg.Graph(n, 2, undirected=True, symmetric=symmetric) as g:
+        g.load_edges(s0, d0)
+        for root in roots:
+            out = {}
+            for order in (0, 1):
+                g.tune_set("bfs_order", order)
+                out[order] = g.bfs(int(root))
+            assert np.array_equal(out[0][0], out[1][0])
+            ref, _, _ = O.bfs(n, s.size, ptr, col, cp, row, int(root))
+            assert np.array_equal(out[1][0], ref)
+            for order in (0, 1):
+                assert O.check_bfs_parents(n, ptr, col, int(root), out[order][0],
+                                           out[order][1]) == 0
