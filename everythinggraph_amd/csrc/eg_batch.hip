@@ -41,6 +41,12 @@ constexpr uint32_t kStatBlocks = 1024;
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
 }
+// Grid of a grid-stride kernel of 256-thread blocks over count elements:
+// at most 2^20 blocks (2^28 threads), far below the 2^32 threads of a launch.
+inline uint32_t stride_grid(uint64_t count) {
+  const uint64_t b = (count + 255) / 256;
+  return (uint32_t)(b < (1u << 20) ? (b ? b : 1) : (1u << 20));
+}
 
 __device__ __forceinline__ uint64_t wave_or(uint64_t v) {
 #pragma unroll
@@ -82,17 +88,22 @@ __device__ __forceinline__ uint64_t finish_ms_row(uint32_t n, uint32_t r, uint64
   return nw;
 }
 
+// The kernels over n x 64 arrays are grid-stride loops: at n = 2^26 one
+// thread per element would exceed the 2^32 threads of a launch.
 __global__ void k_ms_reset(uint32_t n, uint64_t *__restrict__ seen, uint64_t *__restrict__ visit,
                            uint64_t *__restrict__ nvisit, uint64_t *__restrict__ pend,
                            uint16_t *__restrict__ lvl) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    seen[i] = 0;
-    visit[i] = 0;
-    nvisit[i] = 0;
-    pend[i] = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * 16;
+       i += stride) {
+    if (i < n) {
+      seen[i] = 0;
+      visit[i] = 0;
+      nvisit[i] = 0;
+      pend[i] = 0;
+    }
+    reinterpret_cast<uint64_t *>(lvl)[i] = 0;  // 64 u16 = 16 u64
   }
-  if (i < (uint64_t)n * 16) reinterpret_cast<uint64_t *>(lvl)[i] = 0;  // 64 u16 = 16 u64
 }
 
 __global__ void k_ms_seed(uint32_t n, uint32_t k, const uint32_t *__restrict__ lanes,
@@ -427,11 +438,13 @@ __global__ void k_ms_extract(uint32_t n, uint32_t k, uint32_t lane0,
                              const uint16_t *__restrict__ lvl, const uint32_t *__restrict__ perm,
                              uint32_t *__restrict__ out) {
   // out[i * n + v] = level of vertex v (layout id) in lane lane0 + i
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (uint64_t)n * k) return;
-  const uint32_t i = (uint32_t)(t / n), v = (uint32_t)(t % n);
-  const uint32_t vv = perm ? perm[v] : v;
-  out[t] = lvl[(uint64_t)(lane0 + i) * n + vv];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * k;
+       t += stride) {
+    const uint32_t i = (uint32_t)(t / n), v = (uint32_t)(t % n);
+    const uint32_t vv = perm ? perm[v] : v;
+    out[t] = lvl[(uint64_t)(lane0 + i) * n + vv];
+  }
 }
 
 // Retired lanes: drop their bits from every mask and zero their levels.  A
@@ -480,8 +493,8 @@ __global__ void k_ms_reached_sum(const unsigned long long *__restrict__ part, ui
 // ---- PPR ----
 __global__ void k_ppr_zero(uint64_t count, float *__restrict__ X, float *__restrict__ C,
                            float *__restrict__ Y, float *__restrict__ Cn) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < count) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += stride) {
     X[t] = 0.f;
     C[t] = 0.f;
     Y[t] = 0.f;
@@ -509,19 +522,21 @@ __global__ void k_ppr_col_seed(uint32_t q, const uint32_t *__restrict__ seeds, u
 __global__ void k_ppr_cols_clear(uint32_t n, uint64_t cols, uint64_t *__restrict__ mask,
                                  float *__restrict__ X, float *__restrict__ C,
                                  float *__restrict__ Y, float *__restrict__ Cn) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t v = (uint32_t)(t >> 6);
-  const int q = (int)(t & 63);
-  if (v >= n) return;
-  if (q == 0) {
-    const uint64_t m = mask[v];
-    if (m & cols) mask[v] = m & ~cols;
-  }
-  if ((cols >> q) & 1ull) {
-    X[(uint64_t)v * 64 + q] = 0.f;
-    C[(uint64_t)v * 64 + q] = 0.f;
-    Y[(uint64_t)v * 64 + q] = 0.f;
-    Cn[(uint64_t)v * 64 + q] = 0.f;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * 64;
+       t += stride) {
+    const uint32_t v = (uint32_t)(t >> 6);
+    const int q = (int)(t & 63);
+    if (q == 0) {
+      const uint64_t m = mask[v];
+      if (m & cols) mask[v] = m & ~cols;
+    }
+    if ((cols >> q) & 1ull) {
+      X[(uint64_t)v * 64 + q] = 0.f;
+      C[(uint64_t)v * 64 + q] = 0.f;
+      Y[(uint64_t)v * 64 + q] = 0.f;
+      Cn[(uint64_t)v * 64 + q] = 0.f;
+    }
   }
 }
 
@@ -747,11 +762,13 @@ __global__ void k_ppr_delta_final(const double *__restrict__ part2, double *__re
 
 __global__ void k_ppr_extract(uint32_t n, uint32_t k, uint32_t col0, const float *__restrict__ X,
                               const uint32_t *__restrict__ perm, float *__restrict__ out) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (uint64_t)n * k) return;
-  const uint32_t q = (uint32_t)(t / n), v = (uint32_t)(t % n);
-  const uint32_t vv = perm ? perm[v] : v;
-  out[t] = X[(uint64_t)vv * 64 + col0 + q];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * k;
+       t += stride) {
+    const uint32_t q = (uint32_t)(t / n), v = (uint32_t)(t % n);
+    const uint32_t vv = perm ? perm[v] : v;
+    out[t] = X[(uint64_t)vv * 64 + col0 + q];
+  }
 }
 
 // static_from = 1 + the last row with entries or out-edges.
@@ -870,7 +887,7 @@ void eg_ms_free(MsState *st) {
 
 int eg_ms_reset(eg_graph *g, MsState *st) {
   const uint32_t n = g->n;
-  hipLaunchKernelGGL(k_ms_reset, dim3(blocks_for((uint64_t)n * 16, 256)), dim3(256), 0, st->stream,
+  hipLaunchKernelGGL(k_ms_reset, dim3(stride_grid((uint64_t)n * 16)), dim3(256), 0, st->stream,
                      n, st->seen, st->visit, st->nvisit, st->pend, st->lvl);
   EG_LAUNCHED();
   st->qlen = 0;
@@ -1050,7 +1067,7 @@ void eg_ppr_free(PprState *st) {
 
 int eg_ppr_reset(eg_graph *g, PprState *st) {
   const uint64_t nn = (uint64_t)g->n * 64;
-  hipLaunchKernelGGL(k_ppr_zero, dim3(blocks_for(nn, 256)), dim3(256), 0, st->stream, nn, st->X,
+  hipLaunchKernelGGL(k_ppr_zero, dim3(stride_grid(nn)), dim3(256), 0, st->stream, nn, st->X,
                      st->C, st->Y, st->Cn);
   EG_LAUNCHED();
   EG_HIP(hipMemsetAsync(st->mask, 0, (size_t)g->n * 8, st->stream));
@@ -1206,7 +1223,7 @@ int eg_ppr_extract_col(eg_graph *g, const BatchView &bv, PprState *st, uint32_t 
 
 int eg_ppr_clear_cols(eg_graph *g, PprState *st, uint64_t cols) {
   if (!cols) return EG_OK;
-  hipLaunchKernelGGL(k_ppr_cols_clear, dim3(blocks_for((uint64_t)g->n * 64, 256)), dim3(256), 0,
+  hipLaunchKernelGGL(k_ppr_cols_clear, dim3(stride_grid((uint64_t)g->n * 64)), dim3(256), 0,
                      st->stream, g->n, cols, st->mask, st->X, st->C, st->Y, st->Cn);
   EG_LAUNCHED();
   for (int q = 0; q < 64; q++)
@@ -1300,7 +1317,7 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
     if (rc) return rc;
     total_ms += ms;
     if (dist) {
-      hipLaunchKernelGGL(k_ms_extract, dim3(blocks_for((uint64_t)n * kk, 256)), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_ms_extract, dim3(stride_grid((uint64_t)n * kk)), dim3(256), 0, s,
                          n, kk, 0u, b->ms.lvl, bv.perm, b->ext);
       EG_LAUNCHED();
       EG_HIP(hipMemcpyAsync(dist + (uint64_t)g0 * n, b->ext, (uint64_t)n * kk * 4,
@@ -1394,7 +1411,7 @@ extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
     total_ms += ms;
     if (out) {
       float *ext = reinterpret_cast<float *>(b->ext);
-      hipLaunchKernelGGL(k_ppr_extract, dim3(blocks_for((uint64_t)n * kk, 256)), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_ppr_extract, dim3(stride_grid((uint64_t)n * kk)), dim3(256), 0, s,
                          n, kk, 0u, b->ppr.X, bv.perm, ext);
       EG_LAUNCHED();
       EG_HIP(hipMemcpyAsync(out + (uint64_t)q0 * n, ext, (uint64_t)n * kk * 4,

@@ -932,6 +932,13 @@ static int bfs_sorted_cols(eg_graph *g) {
 
 // Adjacency BFS: k_bfs_level once per level.  The host only enqueues levels,
 // in batches, and reads the control words after each batch.
+int eg_bfs_prepare(eg_graph *g) {
+  if (g->out_ptr && (g->flags & (EG_UNDIRECTED | EG_SYMMETRIC)) && g->out_src &&
+      g->tune.bfs_order)
+    return bfs_sorted_cols(g);
+  return EG_OK;
+}
+
 static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_mode,
                          bool want_parent, eg_bfs_stats *st) {
   const uint32_t n = g->n;

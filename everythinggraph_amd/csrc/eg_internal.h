@@ -249,6 +249,11 @@ struct Ppr1 {
 int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, double alpha,
                   int max_iter, hipStream_t stream, double *h_delta);
 int eg_ppr1_step(eg_graph *g, Ppr1 *st);
+// Builds what eg_ppr1_begin would build on first use (orientation, vectors,
+// binned plan) without starting a query.
+int eg_ppr1_prepare(eg_graph *g, int max_iter);
+// Builds the adjacency copies a BFS would build on first use.
+int eg_bfs_prepare(eg_graph *g);
 int eg_ppr1_output(eg_graph *g, Ppr1 *st, float *out);
 // eg_bfs without the ownership check (the serving engine's plain path).
 int eg_bfs_host(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,

@@ -722,12 +722,11 @@ __global__ void k_ppr1_seed(const uint32_t *__restrict__ seeds, uint32_t ns,
 }
 }  // namespace
 
-int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, double alpha,
-                  int max_iter, hipStream_t stream, double *h_delta) {
-  if (ns == 0 || max_iter < 0) return eg_fail(EG_EINVAL, "bad PPR query");
+// The orientation, vectors and (binned) plan of a single-vector query: built
+// on first use, so eg_ppr1_prepare lets a caller pay for them up front.
+static int ppr1_view(eg_graph *g, Ppr1 *st, int max_iter) {
   st->deg = g->out_deg;
   st->unperm = nullptr;
-  const uint32_t *perm = nullptr;
   if (eg_reorder_enabled()) {
     EgOrdered *o;
     EG_TRY(eg_ordered(g, &o));
@@ -736,12 +735,11 @@ int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, dou
     st->w = o->w;
     st->pl = &o->plan;
     st->deg = o->deg;
-    st->unperm = perm = o->new_of_old;
+    st->unperm = o->new_of_old;
   } else {
     EG_TRY(eg_pull_orientation(g, &st->ptr, &st->idx, &st->w, &st->pl));
   }
   EG_TRY(eg_pr_state(g, max_iter));
-  const uint32_t n = g->n;
   // the iterate is a probability vector: row sums stay below 1
   st->binned = eg_pb_selected(g);
   if (st->binned) {
@@ -752,6 +750,20 @@ int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, dou
     EG_TRY(eg_pb_build(g, src));
     st->scale = eg_pb_scale(1.001);
   }
+  return EG_OK;
+}
+
+int eg_ppr1_prepare(eg_graph *g, int max_iter) {
+  Ppr1 st;
+  return ppr1_view(g, &st, max_iter);
+}
+
+int eg_ppr1_begin(eg_graph *g, Ppr1 *st, const uint32_t *seeds, uint32_t ns, double alpha,
+                  int max_iter, hipStream_t stream, double *h_delta) {
+  if (ns == 0 || max_iter < 0) return eg_fail(EG_EINVAL, "bad PPR query");
+  EG_TRY(ppr1_view(g, st, max_iter));
+  const uint32_t n = g->n;
+  const uint32_t *perm = st->unperm;
   hipStream_t s = stream;
   st->stream = s;
   st->h_delta = h_delta;

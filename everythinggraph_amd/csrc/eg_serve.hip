@@ -67,8 +67,13 @@ struct eg_server {
   eg_serve_opts o{};
   BatchView bv{};
   hipStream_t s_bfs = nullptr, s_ppr = nullptr;
-  hipEvent_t ppr_ev = nullptr;
+  hipEvent_t ppr_ev = nullptr, ppr_t0 = nullptr;  // end / start of the SpMM step in flight
   bool ppr_busy = false;
+  int ppr_wclass = 0;        // slot width of that step: 0, 1, 2 = 16, 32, 64 columns
+  // Measured device ms of an SpMM step per width class and of a
+  // single-vector step (moving averages; 0 = not measured yet).
+  double spmm_ms[3] = {0.0, 0.0, 0.0};
+  double solo_ms = 0.0;
   std::vector<LaneGroup *> groups;
   uint32_t max_groups = 1;
   bool bfs_nomem = false;  // no lane group could be allocated
@@ -80,7 +85,7 @@ struct eg_server {
   Ppr1 solo;
   Query *solo_q = nullptr;
   bool solo_busy = false;  // an iteration is in flight
-  hipEvent_t solo_ev = nullptr;
+  hipEvent_t solo_ev = nullptr, solo_t0 = nullptr;
   double *h_solo = nullptr;  // pinned L1 change of the solo iteration
   uint32_t *stage = nullptr;    // n words, BFS stream
   float *stage_ppr = nullptr;   // n floats, PPR stream
@@ -254,8 +259,19 @@ int ppr_after(eg_server *S) {
   return EG_OK;
 }
 
+// Slot-width class of an SpMM step over the columns `cols` (as ppr_width).
+int width_class(uint64_t cols) {
+  if (!cols) return 0;
+  const int hi = 63 - __builtin_clzll(cols);
+  return hi < 16 ? 0 : hi < 32 ? 1 : 2;
+}
+
+void ewma(double &avg, double ms) { avg = avg > 0.0 ? 0.75 * avg + 0.25 * ms : ms; }
+
 int ppr_launch(eg_server *S) {
   S->ppr.active = S->cols;
+  S->ppr_wclass = width_class(S->cols);
+  EG_HIP(hipEventRecord(S->ppr_t0, S->s_ppr));
   EG_TRY(eg_ppr_step(S->g, S->bv, &S->ppr, S->o.ppr_alpha, true));
   EG_HIP(hipEventRecord(S->ppr_ev, S->s_ppr));
   S->ppr_busy = true;
@@ -419,6 +435,7 @@ int solo_start(eg_server *S) {
 }
 
 int solo_launch(eg_server *S) {
+  EG_HIP(hipEventRecord(S->solo_t0, S->s_ppr));
   EG_TRY(eg_ppr1_step(S->g, &S->solo));
   EG_HIP(hipEventRecord(S->solo_ev, S->s_ppr));
   S->solo_busy = true;
@@ -471,19 +488,35 @@ int poll(eg_server *S, hipEvent_t ev, bool *done) {
   return EG_OK;
 }
 
+// Largest PPR backlog served one query at a time on the single-vector path.
+// An SpMM step over w columns costs spmm_ms of the width class that holds w;
+// it replaces w single-vector steps, so the block pays once w exceeds the
+// measured ratio of the two.  Until both are measured: twice the BFS
+// threshold.
+size_t ppr_light_limit(const eg_server *S, size_t light, size_t waiting_now) {
+  const size_t base = 2 * light;
+  if (!light) return base;
+  const size_t w = waiting_now < 64 ? waiting_now : 64;
+  const double blk = S->spmm_ms[width_class(w ? (1ull << (w - 1)) : 1ull)];
+  // Until both steps are timed, a block of w columns is taken to cost about
+  // 4 + w/8 single-vector steps (the binned kernels on RMAT-24).
+  const double ratio =
+      blk > 0.0 && S->solo_ms > 0.0 ? blk / S->solo_ms : 4.0 + (double)w / 8.0;
+  const size_t lim = (size_t)ceil(ratio);
+  return lim < base ? base : (lim > 63 ? 63 : lim);
+}
+
 int batched_round(eg_server *S) {
-  // A lone single-vector PPR takes ~20 pull iterations; an SpMM block with up
-  // to 16 columns costs ~3x one of them per step, so it only pays for a
-  // backlog of more than about three queries: PPR switches at twice the BFS
-  // threshold.
   const size_t light = S->o.idle_plain > 0 ? (size_t)S->o.idle_plain : 0;
-  const size_t light_ppr = 2 * light;
   // ---- PPR side (PPR stream) ----
   bool done = false;
   if (S->ppr_busy) {
     EG_TRY(poll(S, S->ppr_ev, &done));
     if (done) {
       S->ppr_busy = false;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, S->ppr_t0, S->ppr_ev) == hipSuccess)
+        ewma(S->spmm_ms[S->ppr_wclass], ms);
       EG_TRY(ppr_after(S));
     }
   }
@@ -491,9 +524,12 @@ int batched_round(eg_server *S) {
     EG_TRY(poll(S, S->solo_ev, &done));
     if (done) {
       S->solo_busy = false;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, S->solo_t0, S->solo_ev) == hipSuccess) ewma(S->solo_ms, ms);
       EG_TRY(solo_after(S));
     }
   }
+  const size_t light_ppr = ppr_light_limit(S, light, waiting(S, S->pend_ppr));
   if (!S->solo_q && !S->cols && light_ppr) {
     const size_t w = waiting(S, S->pend_ppr);
     if (w > 0 && w <= light_ppr) {
@@ -677,7 +713,9 @@ void destroy(eg_server *S) {
   for (auto &kv : S->all) delete kv.second;
   S->all.clear();
   if (S->ppr_ev) (void)hipEventDestroy(S->ppr_ev);
+  if (S->ppr_t0) (void)hipEventDestroy(S->ppr_t0);
   if (S->solo_ev) (void)hipEventDestroy(S->solo_ev);
+  if (S->solo_t0) (void)hipEventDestroy(S->solo_t0);
   if (S->h_solo) (void)hipHostFree(S->h_solo);
   if (S->s_bfs) (void)hipStreamDestroy(S->s_bfs);
   if (S->s_ppr) (void)hipStreamDestroy(S->s_ppr);
@@ -724,13 +762,19 @@ extern "C" int eg_serve_start(eg_graph *g, const eg_serve_opts *opts, eg_server 
   S->o = o;
   S->max_groups = (o.bfs_lanes + 63) / 64;
   int rc = eg_batch_view(g, &S->bv);  // builds the pull plan / ordered copy now
+  // and the single-vector PPR and BFS layouts, which would otherwise stall the
+  // engine thread on the first query that needs them
+  if (rc == EG_OK) rc = eg_ppr1_prepare(g, o.ppr_max_iter);
+  if (rc == EG_OK) rc = eg_bfs_prepare(g);
   if (rc == EG_OK && hipStreamCreateWithFlags(&S->s_bfs, hipStreamNonBlocking) != hipSuccess)
     rc = eg_fail(EG_EDEVICE, "stream creation failed");
   if (rc == EG_OK && hipStreamCreateWithFlags(&S->s_ppr, hipStreamNonBlocking) != hipSuccess)
     rc = eg_fail(EG_EDEVICE, "stream creation failed");
-  if (rc == EG_OK && hipEventCreateWithFlags(&S->ppr_ev, hipEventDisableTiming) != hipSuccess)
-    rc = eg_fail(EG_EDEVICE, "event creation failed");
-  if (rc == EG_OK && hipEventCreateWithFlags(&S->solo_ev, hipEventDisableTiming) != hipSuccess)
+  // timed events: the engine measures its SpMM and single-vector steps
+  if (rc == EG_OK && (hipEventCreate(&S->ppr_ev) != hipSuccess ||
+                      hipEventCreate(&S->ppr_t0) != hipSuccess ||
+                      hipEventCreate(&S->solo_ev) != hipSuccess ||
+                      hipEventCreate(&S->solo_t0) != hipSuccess))
     rc = eg_fail(EG_EDEVICE, "event creation failed");
   if (rc == EG_OK && hipHostMalloc(reinterpret_cast<void **>(&S->h_solo), sizeof(double),
                                    hipHostMallocDefault) != hipSuccess)

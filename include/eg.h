@@ -298,11 +298,14 @@ typedef struct {
                            device to BFS levels; 0 = 8 per compute unit */
   int32_t idle_plain;   /* light load: with no BFS lane busy and at most this many
                            BFS queries waiting, the oldest runs on the single-source
-                           path (eg_bfs); with no PPR column busy and at most twice
-                           this many PPR queries waiting, the oldest runs on the
+                           path (eg_bfs); with no PPR column busy and at most
+                           L PPR queries waiting, the oldest runs on the
                            single-vector kernel (eg_ppr), stepped beside BFS work.
-                           0 = 2, -1 = always batch (every PPR result then
-                           bit-identical at any load) */
+                           L is the larger of twice this value and the measured
+                           cost of an SpMM step over that many columns in
+                           single-vector steps (4 + columns/8 until both are
+                           timed), capped at 63.  0 = 2, -1 = always batch
+                           (every PPR result then bit-identical at any load) */
   uint32_t pad;
 } eg_serve_opts;
 
@@ -327,6 +330,9 @@ typedef struct {
   uint64_t served_plain;           /* queries served through the plain path */
 } eg_serve_stats;
 
+/* Builds every layout the engine's kernels read (pull plan or ordered copy,
+ * the binned plan of the single-vector path, the hubs-first BFS columns of a
+ * symmetric graph) before its thread starts, so no query waits for a build. */
 int eg_serve_start(eg_graph *g, const eg_serve_opts *opts, eg_server **out);
 /* out (may be NULL): host buffer of n entries, written before the query
  * completes: uint32 levels in the eg_bfs encoding. */
