@@ -71,23 +71,23 @@ inline uint32_t blocks_for(uint64_t count, uint32_t per) {
 // ---------------------------------------------------------------------------
 // Fixed point: value * 2^S as an unsigned 64-bit integer.
 // ---------------------------------------------------------------------------
-// Round-half-up conversion of |c| (finite), negated for c < 0, as a two's
-// complement u64; exact whenever c * 2^S is an integer (always for |c| >=
-// 2^(23 - S)).  |c| = mant * 2^(e - 150), so |c| * 2^S = mant * 2^sh with
-// sh = e - 150 + S.  Every value that is converted is below the bound B with
-// B * 2^S <= 2^62, so sh <= 38 and mant << 39 (< 2^63) shifted right by
-// t = 39 - sh >= 1, after adding half a unit, is the rounded product: the add
-// cannot carry out for sh >= 0 (the low t bits are zero), and for sh < 0 it
-// is (mant + 2^(-sh-1)) >> -sh.  Zero and subnormals (e == 0) give t >= 64,
-// that is 0.
+// |c| * 2^S rounded to the nearest integer, ties to even, negated for c < 0
+// as a two's complement u64 (SIGNED; PageRank terms are never negative).
+// Every value that is converted is below the bound B with B * 2^S <= 2^62.
+// With FP32 operations only: x = |c| * 2^S is exact (a power-of-two scale);
+// h = floor(x / 2^32) < 2^30 is exact; l = x - h * 2^32 (one fused
+// multiply-add) is exact, since for x >= 2^32 it keeps the low bits of a
+// 24-bit mantissa whose last bit is at 2^9 or above, and for x < 2^32 it is
+// x; rint(l) rounds it, and h:rint(l) is the rounded product.  (Checked
+// against integer rounding of mantissa * 2^(exponent + S) for every finite
+// float below 2^(62 - S) at S = 0, 1, 5, 20, 24, 31, 32, 33, 40, 50, 61, 62.)
+template <bool SIGNED>
 __device__ __forceinline__ uint64_t to_fixed(float c, int S) {
-  const uint32_t b = __float_as_uint(c);
-  const uint32_t e = (b >> 23) & 0xFFu;
-  const uint64_t mant = (uint64_t)((b & 0x7FFFFFu) | (e ? 0x800000u : 0u));
-  const int t = max(189 - S - (int)e, 1);
-  if (t >= 64) return 0;
-  const uint64_t mag = ((mant << 39) + (1ull << (t - 1))) >> t;
-  return (b >> 31) ? 0ull - mag : mag;
+  const float x = ldexpf(SIGNED ? fabsf(c) : c, S);
+  const float h = floorf(x * 0x1p-32f);
+  const float l = fmaf(-h, 0x1p32f, x);
+  const uint64_t mag = ((uint64_t)(uint32_t)h << 32) | (uint32_t)rintf(l);
+  return SIGNED && c < 0.f ? 0ull - mag : mag;
 }
 
 __device__ __forceinline__ float from_fixed(uint64_t v, int S) {
@@ -222,11 +222,59 @@ __global__ void k_pb_narrow(uint64_t m, const uint32_t *__restrict__ in,
 // ---------------------------------------------------------------------------
 // Phase 1: values into bin-major slots.
 // ---------------------------------------------------------------------------
-// B: source offsets loaded per lane before any value is written.
+typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+
+// The source offsets are loaded four per lane (8-byte loads; 2-byte loads
+// stream at well under half the rate) and passed through a per-wave LDS
+// stage, so that the value stores stay lane-contiguous.  Quad j of the step
+// at a holds positions a + 256 j + 4 lane .. + 3; quads starting at or after
+// w1 are not loaded (a quad that starts below w1 < m ends within the
+// allocation, which has 16 spare entries).
+template <int Q>
+__device__ __forceinline__ void load_quads(const PbDev &d, uint32_t a, uint32_t w1, int lane,
+                                           u16x4 (&s)[Q]) {
+  const u16x4 *q4 = reinterpret_cast<const u16x4 *>(d.sloc);
+#pragma unroll
+  for (int j = 0; j < Q; j++) {
+    const uint32_t p = a + 256 * j + 4 * lane;
+    s[j] = p < w1 ? __builtin_nontemporal_load(q4 + (p >> 2)) : u16x4{0, 0, 0, 0};
+  }
+}
+
+// Writes the values of positions a + 256 j + 64 q + lane within [w0, w1).
+template <int Q>
+__device__ __forceinline__ void write_quads(const PbDev &d, uint32_t a, uint32_t w0, uint32_t w1,
+                                            int lane, const u16x4 (&s)[Q], uint32_t &bi,
+                                            const float *s_win, const uint2 *s_tab,
+                                            u16x4 *stage) {
+  const uint16_t *st = reinterpret_cast<const uint16_t *>(stage);
+#pragma unroll
+  for (int j = 0; j < Q; j++) {
+    // LDS operations of a wave complete in order: the reads below see this
+    // store, and the next store follows them
+    stage[lane] = s[j];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t p = a + 256 * j + 64 * q + lane;
+      const uint32_t o = st[64 * q + lane];
+      if (p >= w0 && p < w1) {
+        while (p >= s_tab[bi + 1].x) bi++;
+        d.val[s_tab[bi].y + (p - s_tab[bi].x)] = s_win[o];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// B: positions per lane per step (B / 4 quads); the next step's offsets are
+// loaded before the current step's values are written.
 template <int B>
 __global__ __launch_bounds__(kBlock) void k_pb_scatter(PbDev d, const float *__restrict__ contrib) {
+  constexpr int Q = B / 4;
   __shared__ float s_win[kChunkSrc];
   __shared__ uint2 s_tab[kTileBins + 1];
+  __shared__ u16x4 s_stage[kWaves][64];
   const PbTile t = d.tiles[blockIdx.x];
   const uint2 cs = d.chunk_src[t.c];
   const uint32_t tid = threadIdx.x;
@@ -266,10 +314,11 @@ __global__ __launch_bounds__(kBlock) void k_pb_scatter(PbDev d, const float *__r
   const uint32_t w0 = t.p0 + min(len, wave * per);
   const uint32_t w1 = t.p0 + min(len, (wave + 1) * per);
   if (w0 >= w1) return;
+  const uint32_t a0 = w0 & ~3u;  // quads are 8-byte aligned
   // segment of this lane's first position: the last bin whose start is <= p
   uint32_t bi = 0;
   {
-    const uint32_t p = min(w0 + lane, w1 - 1);
+    const uint32_t p = min(max(a0 + lane, w0), w1 - 1);
     uint32_t lo = 0, hi = t.nb - 1;
     while (lo < hi) {
       const uint32_t mid = (lo + hi + 1) >> 1;
@@ -277,21 +326,19 @@ __global__ __launch_bounds__(kBlock) void k_pb_scatter(PbDev d, const float *__r
     }
     bi = lo;
   }
-  for (uint32_t base = w0; base < w1; base += 64 * B) {
-    uint32_t s[B];
-#pragma unroll
-    for (int q = 0; q < B; q++) {
-      const uint32_t p = base + q * 64 + lane;
-      s[q] = p < w1 ? (uint32_t)__builtin_nontemporal_load(d.sloc + p) : 0u;
-    }
-#pragma unroll
-    for (int q = 0; q < B; q++) {
-      const uint32_t p = base + q * 64 + lane;
-      if (p < w1) {
-        while (p >= s_tab[bi + 1].x) bi++;
-        d.val[s_tab[bi].y + (p - s_tab[bi].x)] = s_win[s[q]];
-      }
-    }
+  constexpr uint32_t kStep = 256 * Q;
+  u16x4 *stage = s_stage[wave];
+  u16x4 s0[Q], s1[Q];
+  load_quads<Q>(d, a0, w1, lane, s0);
+  for (uint32_t a = a0;; a += 2 * kStep) {
+    const bool more1 = a + kStep < w1;
+    if (more1) load_quads<Q>(d, a + kStep, w1, lane, s1);
+    write_quads<Q>(d, a, w0, w1, lane, s0, bi, s_win, s_tab, stage);
+    if (!more1) break;
+    const bool more2 = a + 2 * kStep < w1;
+    if (more2) load_quads<Q>(d, a + 2 * kStep, w1, lane, s0);
+    write_quads<Q>(d, a + kStep, w0, w1, lane, s1, bi, s_win, s_tab, stage);
+    if (!more2) break;
   }
 }
 
@@ -316,7 +363,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // words; otherwise w0 holds 8 u16 row offsets and v0 / v1 the values.  part:
 // the whole range belongs to one row and is summed into psum.  A run of equal
 // row offsets is added with one atomic at its last entry.
-template <bool HOTRANGE, bool WEIGHTED>
+template <bool HOTRANGE, bool WEIGHTED, bool SIGNED>
 __device__ __forceinline__ void group_accumulate(unsigned long long *acc, const u32x4 w0,
                                                  const u32x4 w1, const f32x4 v0, const f32x4 v1,
                                                  const f32x4 x0, const f32x4 x1,
@@ -348,38 +395,96 @@ __device__ __forceinline__ void group_accumulate(unsigned long long *acc, const 
 #pragma unroll
     for (int j = 0; j < 8; j++) v[j] *= ws8[j];
   }
+  // This lane's dummy slot behind the accumulators (never read): masked
+  // entries and the inner entries of runs add there.
+  const uint32_t dummy = kBinRows + (uint32_t)__lane_id();
   if (e < e0 || e + 8 > e1) {  // an edge group of the range
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const bool ok = e + j >= e0 && e + j < e1;
-      key[j] = ok ? key[j] : 0xFFFFFFFFu;
+      key[j] = ok ? key[j] : dummy;
       v[j] = ok ? v[j] : 0.f;
     }
   }
   if (part) {
 #pragma unroll
-    for (int j = 0; j < 8; j++) psum += to_fixed(v[j], S);
+    for (int j = 0; j < 8; j++) psum += to_fixed<SIGNED>(v[j], S);
     return;
   }
   // Branch-free: every entry issues one atomic; the last entry of a run adds
-  // the run to its row, the others add to this lane's dummy slot behind the
-  // accumulators (never read), so no lane waits on a branch and dummies never
-  // collide.
-  const uint32_t dummy = kBinRows + (uint32_t)__lane_id();
+  // the run to its row, the others add to the dummy slot, so no lane waits on
+  // a branch and dummies never collide.  A masked entry's key is the dummy,
+  // which no row offset equals, so it ends the run before it.
   unsigned long long run = 0ull;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
-    run += to_fixed(v[j], S);
+    run += to_fixed<SIGNED>(v[j], S);
     const bool end = j == 7 || key[j] != key[j + 1];
-    atomicAdd(&acc[end && key[j] != 0xFFFFFFFFu ? key[j] : dummy], run);
+    atomicAdd(&acc[end ? key[j] : dummy], run);
     run = end ? 0ull : run;
   }
 }
 
-// Sums the entries [e0, e1) of one item, each wave a contiguous range of
-// 8-entry groups, G groups loaded per lane before any is summed.  HOTRANGE:
-// the entries are hot (packed value slots).
+// G groups of 8 entries per lane, as loaded from HBM.
+template <int G>
+struct PbBatch {
+  u32x4 wa[G], wb[G];
+  f32x4 va[G], vb[G], xa[G], xb[G];
+};
+
+// Issues the loads of the groups base + q * 64 + lane (q < G) below wg1.
 template <bool HOTRANGE, bool WEIGHTED, int G>
+__device__ __forceinline__ void load_batch(const PbDev &d, uint32_t base, uint32_t wg1, int lane,
+                                           PbBatch<G> &b) {
+  const u32x4 *dl4 = reinterpret_cast<const u32x4 *>(d.dloc);
+  const f32x4 *v4 = reinterpret_cast<const f32x4 *>(d.val);
+  const u32x4 *h4 = reinterpret_cast<const u32x4 *>(d.val);
+  const f32x4 *w4 = reinterpret_cast<const f32x4 *>(d.w);
+  const u32x4 z4 = u32x4{0u, 0u, 0u, 0u};
+  const f32x4 zf = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < G; q++) {
+    const uint32_t gi = base + q * 64 + lane;
+    const bool in = gi < wg1;
+    if (HOTRANGE) {
+      b.wa[q] = in ? __builtin_nontemporal_load(h4 + 2 * (size_t)gi) : z4;
+      b.wb[q] = in ? __builtin_nontemporal_load(h4 + 2 * (size_t)gi + 1) : z4;
+      b.va[q] = b.vb[q] = zf;
+    } else {
+      b.wa[q] = in ? __builtin_nontemporal_load(dl4 + gi) : z4;
+      b.wb[q] = z4;
+      b.va[q] = in ? __builtin_nontemporal_load(v4 + 2 * (size_t)gi) : zf;
+      b.vb[q] = in ? __builtin_nontemporal_load(v4 + 2 * (size_t)gi + 1) : zf;
+    }
+    if (WEIGHTED) {
+      b.xa[q] = in ? __builtin_nontemporal_load(w4 + 2 * (size_t)gi) : zf;
+      b.xb[q] = in ? __builtin_nontemporal_load(w4 + 2 * (size_t)gi + 1) : zf;
+    } else {
+      b.xa[q] = b.xb[q] = zf;
+    }
+  }
+}
+
+template <bool HOTRANGE, bool WEIGHTED, bool SIGNED, int G>
+__device__ __forceinline__ void sum_batch(const PbBatch<G> &b, uint32_t base, uint32_t wg1,
+                                          int lane, uint32_t e0, uint32_t e1, int S, bool part,
+                                          unsigned long long *acc, const float *s_hot,
+                                          unsigned long long &psum) {
+#pragma unroll
+  for (int q = 0; q < G; q++) {
+    const uint32_t gi = base + q * 64 + lane;
+    if (gi < wg1)
+      group_accumulate<HOTRANGE, WEIGHTED, SIGNED>(acc, b.wa[q], b.wb[q], b.va[q], b.vb[q],
+                                                   b.xa[q], b.xb[q], gi * 8, e0, e1, S, part,
+                                                   s_hot, psum);
+  }
+}
+
+// Sums the entries [e0, e1) of one item in 8-entry groups, G groups loaded
+// per lane before any is summed; the workgroup's waves take consecutive
+// 64 G-group blocks, so that together they stream one region of the range.
+// HOTRANGE: the entries are hot (packed value slots).
+template <bool HOTRANGE, bool WEIGHTED, bool SIGNED, int G>
 __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, uint32_t e1, int S,
                                                  bool part, unsigned long long *acc,
                                                  const float *s_hot,
@@ -387,48 +492,12 @@ __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, ui
   if (e0 >= e1) return;
   const int lane = __lane_id();
   const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t g0 = e0 >> 3;
-  const uint32_t ng = ((e1 + 7) >> 3) - g0;
-  const uint32_t per = ((ng + kWaves - 1) / kWaves + 63) & ~63u;
-  const uint32_t wg0 = g0 + min(ng, wave * per);
-  const uint32_t wg1 = g0 + min(ng, (wave + 1) * per);
-  const u32x4 *dl4 = reinterpret_cast<const u32x4 *>(d.dloc);
-  const f32x4 *v4 = reinterpret_cast<const f32x4 *>(d.val);
-  const u32x4 *h4 = reinterpret_cast<const u32x4 *>(d.val);
-  const f32x4 *w4 = reinterpret_cast<const f32x4 *>(d.w);
-  const u32x4 z4 = u32x4{0u, 0u, 0u, 0u};
-  const f32x4 zf = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (uint32_t base = wg0; base < wg1; base += 64 * G) {
-    u32x4 wa[G], wb[G];
-    f32x4 va[G], vb[G], xa[G], xb[G];
-#pragma unroll
-    for (int q = 0; q < G; q++) {
-      const uint32_t gi = base + q * 64 + lane;
-      const bool in = gi < wg1;
-      if (HOTRANGE) {
-        wa[q] = in ? __builtin_nontemporal_load(h4 + 2 * (size_t)gi) : z4;
-        wb[q] = in ? __builtin_nontemporal_load(h4 + 2 * (size_t)gi + 1) : z4;
-        va[q] = vb[q] = zf;
-      } else {
-        wa[q] = in ? __builtin_nontemporal_load(dl4 + gi) : z4;
-        wb[q] = z4;
-        va[q] = in ? __builtin_nontemporal_load(v4 + 2 * (size_t)gi) : zf;
-        vb[q] = in ? __builtin_nontemporal_load(v4 + 2 * (size_t)gi + 1) : zf;
-      }
-      if (WEIGHTED) {
-        xa[q] = in ? __builtin_nontemporal_load(w4 + 2 * (size_t)gi) : zf;
-        xb[q] = in ? __builtin_nontemporal_load(w4 + 2 * (size_t)gi + 1) : zf;
-      } else {
-        xa[q] = xb[q] = zf;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < G; q++) {
-      const uint32_t gi = base + q * 64 + lane;
-      if (gi < wg1)
-        group_accumulate<HOTRANGE, WEIGHTED>(acc, wa[q], wb[q], va[q], vb[q], xa[q], xb[q],
-                                             gi * 8, e0, e1, S, part, s_hot, psum);
-    }
+  const uint32_t g1 = (e1 + 7) >> 3;
+  for (uint32_t base = (e0 >> 3) + wave * 64 * G; base < g1; base += kWaves * 64 * G) {
+    PbBatch<G> b;
+    load_batch<HOTRANGE, WEIGHTED, G>(d, base, g1, lane, b);
+    sum_batch<HOTRANGE, WEIGHTED, SIGNED, G>(b, base, g1, lane, e0, e1, S, part, acc, s_hot,
+                                             psum);
   }
 }
 
@@ -469,11 +538,12 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
       for (uint32_t i = tid; i < it.nrows; i += kBlock) s_acc[i] = 0ull;
     __syncthreads();  // accumulators cleared (and, the first time, the hot copy made)
     unsigned long long psum = 0ull;
-    // the persistent form runs 16 waves per CU: more loads in flight per lane
+    // the hot form runs 16 waves per CU: more loads in flight per lane
     constexpr int G = HOT && !WEIGHTED ? 4 : 2;
-    if (HOT) accumulate_range<true, WEIGHTED, G>(d, it.e0, it.eh, S, part, s_acc, s_hot, psum);
-    accumulate_range<false, WEIGHTED, G>(d, HOT ? it.eh : it.e0, it.e1, S, part, s_acc, s_hot,
-                                         psum);
+    if (HOT)
+      accumulate_range<true, WEIGHTED, SPMV, G>(d, it.e0, it.eh, S, part, s_acc, s_hot, psum);
+    accumulate_range<false, WEIGHTED, SPMV, G>(d, HOT ? it.eh : it.e0, it.e1, S, part, s_acc,
+                                               s_hot, psum);
     if (part) {
 #pragma unroll
       for (int o = 32; o >= 1; o >>= 1) psum += __shfl_xor(psum, o);
@@ -888,7 +958,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
           (rc = upload(g, &d_src0, chunk_src0, "binned chunk sources")))
         break;
       if ((rc = eg_dalloc_t(&p->dloc, m + 16, "binned row offsets")) ||
-          (rc = eg_dalloc_t(&p->sloc, m, "binned source offsets")) ||
+          (rc = eg_dalloc_t(&p->sloc, m + 16, "binned source offsets")) ||
           (rc = eg_dalloc_t(&p->val, m + 16, "binned values")))
         break;
       if (src.w) {
