@@ -95,6 +95,10 @@ def main():
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-configs", action="store_true",
                     help="skip the grid (ER 2^24 / 2^28, 16x16) and RMAT-22 BFS/WCC figures")
+    ap.add_argument("--skip-c5", action="store_true",
+                    help="skip the R-MAT 26 serving replay")
+    ap.add_argument("--c5-requests", type=int, default=150,
+                    help="requests per R-MAT 26 serving replay")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in kernel shapes (plus any --tune)")
     ap.add_argument("--tune", action="append", default=[], metavar="PARAM=VALUE",
@@ -264,7 +268,7 @@ def main():
                     f"{r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms (BFS p50 "
                     f"{r['bfs_p50_ms']:.1f} p99 {r['bfs_p99_ms']:.1f}; PPR p50 "
                     f"{r['ppr_p50_ms']:.1f} p99 {r['ppr_p99_ms']:.1f}); levels "
-                    f"{r['stats']['bfs_levels']} spmm steps {r['stats']['ppr_steps']}; "
+                    f"{r['stats']['bfs_levels']} PPR steps {r['stats']['ppr_steps']}; "
                     f"wall {time.perf_counter() - w0:.1f} s")
             b, p = srv["batched"], srv["plain"]
             serving.append({
@@ -337,14 +341,15 @@ def main():
     out.update(extra)
     g.close()
     if not args.skip_configs:
-        out["configs"] = other_configs(eg)
+        out["configs"] = other_configs(eg, args.skip_c5, args.c5_requests)
     print(json.dumps(out), flush=True)
 
 
-def other_configs(eg) -> dict:
+def other_configs(eg, skip_c5=False, c5_requests=150) -> dict:
     """Device figures of the other configurations on fresh graphs: PageRank /
     SpMV on the 2-D grid (directed Erdos-Renyi, 2^24 vertices, 2^28 weighted
-    edges, 16 x 16 cells) and BFS + WCC on undirected R-MAT 22 (CSR)."""
+    edges, 16 x 16 cells), file ingest, BFS + WCC + SSSP on undirected R-MAT
+    22 (CSR) and serving on directed R-MAT 26."""
     res = {}
     n, m = 1 << 24, 1 << 28
     g = eg.Graph(n, eg.MODE_GRID_SORT, weighted=True, P=16)
@@ -450,6 +455,32 @@ def other_configs(eg) -> dict:
     log(f"RMAT-22 -u: CSR+CSC build {build_ms:.1f} ms; BFS {gteps:.1f} GTEPS; WCC {rounds} rounds, "
         f"relax {wt:.2f} ms, call {wcc_wall * 1e3:.1f} ms (incl. label download)")
     g.close()
+
+    if not skip_c5:
+        # serving at R-MAT 26 (directed, CSR + CSC resident): the Poisson
+        # stream (80% BFS, 20% PPR with 1-8 seeds, tolerance 1e-6 L1) through
+        # continuous batching and through the plain one-query-at-a-time path
+        n = 1 << 26
+        g = eg.Graph(n, eg.MODE_ADJ_SORT)
+        g.generate(eg.rmat_params(26, 16, seed=1))
+        srv = {"n": n, "m": g.info.m, "device_gb": round(g.info.device_bytes / 1e9, 2),
+               "requests": c5_requests}
+        for qps in (10.0, 20.0):
+            row = {}
+            for mode in ("batched", "plain"):
+                r = g.serve_replay(qps, c5_requests, bfs_fraction=0.8, ppr_max_seeds=8, seed=4,
+                                   plain=(mode == "plain"))
+                row[mode] = {k: round(r[k], 2) for k in
+                             ("qps", "p50_ms", "p99_ms", "bfs_p50_ms", "bfs_p99_ms",
+                              "ppr_p50_ms", "ppr_p99_ms")}
+                row[mode]["completed"] = r["completed"]
+                log(f"RMAT-26 serve {mode}: offered {qps:.0f} q/s, {r['completed']} done at "
+                    f"{r['qps']:.1f} q/s; p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms "
+                    f"(BFS p99 {r['bfs_p99_ms']:.1f}; PPR p50 {r['ppr_p50_ms']:.1f} "
+                    f"p99 {r['ppr_p99_ms']:.1f})")
+            srv[f"qps{int(qps)}"] = row
+        res["serve_rmat26"] = srv
+        g.close()
     return res
 
 
