@@ -517,6 +517,10 @@ struct LevelArgs {
   uint64_t m;
   int start_pull, switch_mode;
   const uint32_t *optr, *ocol, *iptr, *iidx;
+  // Ordered ids: the traversal runs on relabelled vertices; id_of maps a
+  // vertex to its caller id for dist / parent, new_of_old maps the root
+  // (both null: the layout's own ids).
+  const uint32_t *id_of, *new_of_old;
   uint64_t *visited;
   uint64_t *F[3];
   uint32_t *dist, *parent;
@@ -530,17 +534,19 @@ __global__ void k_bfs_level_init(LevelArgs a, uint32_t root) {
     a.dist[i] = i == root ? 1u : 0u;
     if (a.parent) a.parent[i] = i == root ? root : a.n;
   }
+  // dist / parent above are in caller ids, everything below in traversal ids
+  const uint32_t rn = a.new_of_old ? a.new_of_old[root] : root;
   if (i < a.nwords) {
-    const uint64_t w = (root >> 6) == i ? (1ull << (root & 63)) : 0ull;
+    const uint64_t w = (rn >> 6) == i ? (1ull << (rn & 63)) : 0ull;
     a.visited[i] = w;
     a.F[0][i] = 0ull;
     a.F[1][i] = w;
     a.F[2][i] = 0ull;
   }
   if (i == 0) {
-    const uint32_t d = a.optr[root + 1] - a.optr[root];
+    const uint32_t d = a.optr[rn + 1] - a.optr[rn];
     for (int k = 0; k < kCtlWords; k++) a.ctl[k] = 0ull;
-    a.q[1][0] = root;
+    a.q[1][0] = rn;
     a.qoff[1][0] = 0u;
     a.ctl[2] = d ? (1ull << 32) | d : 0ull;  // level 1: slot 1
     a.ctl[3] = 1ull;
@@ -725,8 +731,9 @@ __global__ __launch_bounds__(kBlock) void k_bfs_level(LevelArgs a, uint32_t L) {
             const uint64_t old = atomicOr(
                 reinterpret_cast<unsigned long long *>(&a.visited[v >> 6]), (unsigned long long)bit);
             if (!(old & bit)) {
-              a.dist[v] = newdist;
-              if (a.parent) a.parent[v] = u;
+              const uint32_t vo = a.id_of ? a.id_of[v] : v;
+              a.dist[vo] = newdist;
+              if (a.parent) a.parent[vo] = a.id_of ? a.id_of[u] : u;
               atomicOr(reinterpret_cast<unsigned long long *>(&F_out[v >> 6]),
                        (unsigned long long)bit);
               found++;
@@ -806,8 +813,9 @@ __global__ __launch_bounds__(kBlock) void k_bfs_level(LevelArgs a, uint32_t L) {
           }
         }
         if (hit) {
-          a.dist[v] = newdist;
-          if (a.parent) a.parent[v] = par;
+          const uint32_t vo = a.id_of ? a.id_of[v] : v;
+          a.dist[vo] = newdist;
+          if (a.parent) a.parent[vo] = a.id_of ? a.id_of[par] : par;
           found++;
         }
         const uint64_t nw = __ballot(hit);
@@ -930,12 +938,87 @@ static int bfs_sorted_cols(eg_graph *g) {
   return EG_OK;
 }
 
+// row[k] = the row of entry k (a wave per row, grid-stride).
+__global__ void k_entry_rows(uint32_t n, const uint32_t *__restrict__ ptr,
+                             uint32_t *__restrict__ row) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int lane = __lane_id();
+  for (uint32_t v = wave; v < n; v += nwaves) {
+    const uint32_t k1 = ptr[v + 1];
+    for (uint32_t k = ptr[v] + lane; k < k1; k += 64) row[k] = v;
+  }
+}
+
+// Symmetric graphs, ordered ids: the rows of the degree-ordered copy (new
+// ids by descending degree) with every row's entries ascending, so hubs
+// first and the hubs' frontier bits packed into the first bitmap words.
+// Built on first use.
+static int bfs_ordered_cols(eg_graph *g, EgOrdered *o) {
+  if (g->bfs_ocol) return EG_OK;
+  const uint64_t m = g->m;
+  const uint32_t n = g->n;
+  hipStream_t s = g->stream;
+  uint32_t *row = nullptr, *tcol = nullptr, *trow = nullptr, *out = nullptr;
+  int rc = EG_OK;
+  do {
+    if ((rc = eg_dalloc_t(&row, m, "bfs order rows")) ||
+        (rc = eg_dalloc_t(&tcol, m, "bfs order columns")) ||
+        (rc = eg_dalloc_t(&trow, m, "bfs order rows")) ||
+        (rc = eg_dalloc_t(&out, m, "bfs ordered columns")))
+      break;
+    hipLaunchKernelGGL(k_entry_rows, dim3(2048), dim3(256), 0, s, n, o->ptr, row);
+    if (hipGetLastError() != hipSuccess) {
+      rc = eg_fail(EG_EDEVICE, "bfs order rows launch");
+      break;
+    }
+    void *tmp = nullptr;
+    if ((rc = eg_scratch(g, eg_radix_tmp_bytes(m, 2), &tmp))) break;
+    // by column (new id), then stably by row
+    EgSortCols c1;
+    c1.np = 2;
+    c1.in[0] = o->idx;
+    c1.out[0] = tcol;
+    c1.in[1] = row;
+    c1.out[1] = trow;
+    if ((rc = eg_radix_sort(s, m, eg_bits_for(n), o->idx, nullptr, c1, tmp))) break;
+    EgSortCols c2;
+    c2.np = 1;
+    c2.in[0] = tcol;
+    c2.out[0] = out;
+    if ((rc = eg_radix_sort(s, m, eg_bits_for(n), trow, nullptr, c2, tmp))) break;
+    if (hipStreamSynchronize(s) != hipSuccess) rc = eg_fail(EG_EDEVICE, "bfs order sort");
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  eg_dfree_t(&row);
+  eg_dfree_t(&tcol);
+  eg_dfree_t(&trow);
+  if (rc) {
+    eg_dfree_t(&out);
+    return rc;
+  }
+  g->bfs_ocol = out;
+  return EG_OK;
+}
+
+// Which rows a symmetric graph's BFS scans: 2 = ordered ids (needs the
+// degree-ordered copy), 1 = hubs first in the layout's ids, 0 = as built.
+static int bfs_mode(const eg_graph *g) {
+  if (!g->out_ptr || !(g->flags & (EG_UNDIRECTED | EG_SYMMETRIC)) || !g->out_src) return 0;
+  if (g->tune.bfs_order == 2 && !eg_reorder_enabled()) return 1;
+  return g->tune.bfs_order;
+}
+
 // Adjacency BFS: k_bfs_level once per level.  The host only enqueues levels,
 // in batches, and reads the control words after each batch.
 int eg_bfs_prepare(eg_graph *g) {
-  if (g->out_ptr && (g->flags & (EG_UNDIRECTED | EG_SYMMETRIC)) && g->out_src &&
-      g->tune.bfs_order)
-    return bfs_sorted_cols(g);
+  const int mode = bfs_mode(g);
+  if (mode == 1) return bfs_sorted_cols(g);
+  if (mode == 2) {
+    EgOrdered *o;
+    EG_TRY(eg_ordered(g, &o));
+    return bfs_ordered_cols(g, o);
+  }
   return EG_OK;
 }
 
@@ -961,11 +1044,21 @@ static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_
   a.ocol = g->out_col;
   a.iptr = g->in_ptr ? g->in_ptr : g->out_ptr;
   a.iidx = g->in_ptr ? g->in_idx : g->out_col;
-  if ((g->flags & (EG_UNDIRECTED | EG_SYMMETRIC)) && g->out_src && g->tune.bfs_order) {
+  a.id_of = a.new_of_old = nullptr;
+  const int mode = bfs_mode(g);
+  if (mode == 1) {
     // in-lists are the out-lists: one copy, hubs first, serves both directions
     EG_TRY(bfs_sorted_cols(g));
     a.ocol = a.iidx = g->bfs_col;
     a.iptr = g->out_ptr;
+  } else if (mode == 2) {
+    EgOrdered *o;
+    EG_TRY(eg_ordered(g, &o));
+    EG_TRY(bfs_ordered_cols(g, o));
+    a.optr = a.iptr = o->ptr;
+    a.ocol = a.iidx = g->bfs_ocol;
+    a.id_of = o->old_of_new;
+    a.new_of_old = o->new_of_old;
   }
   a.visited = g->bfs_visited;
   a.F[0] = g->bfs_front;

@@ -248,6 +248,7 @@ static void free_layout(eg_graph *g) {
   eg_dfree_t(&g->out_col);
   eg_dfree_t(&g->out_src);
   eg_dfree_t(&g->bfs_col);
+  eg_dfree_t(&g->bfs_ocol);
   eg_dfree_t(&g->out_w);
   eg_dfree_t(&g->in_ptr);
   eg_dfree_t(&g->in_idx);

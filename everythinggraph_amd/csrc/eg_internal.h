@@ -119,7 +119,8 @@ struct EgTune {
   int pr_scatter = 16;   // binned scatter: source offsets in flight per lane (8, 16 or 32)
   int pr_hot = 16384;    // binned PageRank: sources below this id are read from an LDS copy
                          // in phase 2 instead of being scattered (0: none, at most 16384)
-  int bfs_order = 1;     // symmetric graphs: BFS on CSR rows ordered hubs first (0: input order)
+  int bfs_order = 2;     // symmetric graphs: BFS on CSR rows ordered hubs first (1), on the
+                         // degree-ordered ids with rows hubs first (2), or as built (0)
   int pr_persist = 0;    // binned PageRank with hot sources: phase-2 workgroups loop over
                          // items (one per CU) instead of one workgroup per item
 };
@@ -163,6 +164,7 @@ struct eg_graph {
   uint32_t *bfs_q = nullptr, *bfs_nq = nullptr;
   uint64_t *bfs_f3 = nullptr;                    // third frontier bitmap (device-driven BFS)
   uint32_t *bfs_col = nullptr;  // symmetric graphs: CSR columns, each row by descending degree
+  uint32_t *bfs_ocol = nullptr; // symmetric graphs: degree-ordered rows, entries ascending (new ids)
   uint32_t *bfs_qoff = nullptr, *bfs_nqoff = nullptr;  // queue edge offsets
   uint32_t bfs_depth_hint = 0;  // levels of the last traversal (first level batch)
   unsigned long long *bfs_part = nullptr;  // per-workgroup partial counts

@@ -28,8 +28,11 @@
 //                   before any value is written, 8, 16 or 32;
 //  * pr_hot      -- binned PageRank: sources below this id are read from an
 //                   LDS copy in phase 2 instead of being scattered (0..16384);
-//  * bfs_order   -- symmetric graphs: BFS scans rows ordered hubs first (1) or
-//                   in input order (0); levels are the same, parents may differ.
+//  * bfs_order   -- symmetric graphs: BFS runs on the degree-ordered ids with
+//                   every row hubs first (2, the default; 1 without the
+//                   ordered copy), on the layout's ids with rows hubs first
+//                   (1), or on the rows as built (0); levels are the same,
+//                   parents may differ.
 // eg_autotune times every variant on the graph itself through the public
 // entry points (device time from HIP events, best of a few repetitions) and
 // keeps the fastest.
@@ -110,7 +113,7 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
   } else if (!strcmp(param, "pr_hot")) {
     if (value < 0 || value > 16384) return eg_fail(EG_EINVAL, "pr_hot takes 0..16384");
   } else if (!strcmp(param, "bfs_order")) {
-    if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "bfs_order takes 0 or 1");
+    if (value < 0 || value > 2) return eg_fail(EG_EINVAL, "bfs_order takes 0, 1 or 2");
   } else if (!strcmp(param, "pr_persist")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "pr_persist takes 0 or 1");
   } else if (!strcmp(param, "pr_binned")) {

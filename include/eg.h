@@ -234,9 +234,11 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * sources below this id, at most 16384, are read from an LDS copy in phase 2
  * instead of being scattered; 0 = none, default 16384); "pr_persist" (1:
  * phase-2 workgroups loop over work items, one per CU; default 0); "bfs_order"
- * (symmetric graphs: 1 = BFS scans rows ordered by descending neighbour degree,
- * a copy built on first use; 0 = input order; levels are the same, parents may
- * differ); "grid_band" (grid source
+ * (symmetric graphs: 2 = BFS runs on the degree-ordered vertex ids with every
+ * row's neighbours by descending degree, results written in the caller's ids
+ * (the default; 1 when the ordered copy is disabled); 1 = the layout's ids,
+ * rows by descending neighbour degree; 0 = rows as built; the copies are built
+ * on first use; levels are the same, parents may differ); "grid_band" (grid source
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
  * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
  * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant

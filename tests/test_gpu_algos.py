@@ -563,8 +563,9 @@ def test_sssp_symmetric_flag_and_errors():
 
 @pytest.mark.parametrize("symmetric", [False, True])
 def test_bfs_order_same_levels(symmetric):
-    """Symmetric graphs: BFS over rows ordered hubs first (bfs_order 1) and in
-    input order (0) give the same levels, and legal parent trees."""
+    """Symmetric graphs: BFS over rows ordered hubs first (bfs_order 1), over
+    the degree-ordered ids with rows hubs first (2) and in input order (0)
+    give the same levels, in the caller's ids, and legal parent trees."""
     n = 1 << 13
     s0, d0 = rmat(13, seed=66)
     s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
@@ -573,12 +574,13 @@ def test_bfs_order_same_levels(symmetric):
         g.load_edges(s0, d0)
         for root in roots:
             out = {}
-            for order in (0, 1):
+            for order in (0, 1, 2):
                 g.tune_set("bfs_order", order)
                 out[order] = g.bfs(int(root))
-            assert np.array_equal(out[0][0], out[1][0])
-            ref, _, _ = O.bfs(n, s.size, ptr, col, cp, row, int(root))
-            assert np.array_equal(out[1][0], ref)
-            for order in (0, 1):
+            ref, _, st = O.bfs(n, s.size, ptr, col, cp, row, int(root))
+            for order in (0, 1, 2):
+                assert np.array_equal(out[order][0], ref), order
+                assert out[order][2].reached == out[0][2].reached
+                assert out[order][2].edges_traversed == out[0][2].edges_traversed
                 assert O.check_bfs_parents(n, ptr, col, int(root), out[order][0],
                                            out[order][1]) == 0
