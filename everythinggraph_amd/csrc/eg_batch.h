@@ -10,6 +10,10 @@
 // (symmetric graph), which allows push levels.
 struct BatchView {
   const uint32_t *ptr, *idx, *deg, *perm;
+  // MS-BFS rows: idx with every row's entries ascending (hubs first) on
+  // symmetric graphs, where a pull row stops once it holds every needed lane;
+  // otherwise idx.  OR is order-free, so levels do not depend on it.
+  const uint32_t *ms_idx;
   PullPlan *pl;
   bool sym;
   // Chunks [static_chunk0, nchunks) hold only rows without entries and

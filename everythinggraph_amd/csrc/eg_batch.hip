@@ -834,11 +834,18 @@ int eg_batch_view(eg_graph *g, BatchView *v) {
     v->deg = o->deg;
     v->perm = o->new_of_old;
     v->pl = &o->plan;
+    v->ms_idx = o->idx;
+    if (g->flags & (EG_UNDIRECTED | EG_SYMMETRIC)) {
+      const uint32_t *rows = nullptr;
+      EG_TRY(eg_hubs_first_rows(g, &rows));
+      if (rows) v->ms_idx = rows;
+    }
   } else {
     const float *w;
     EG_TRY(eg_pull_orientation(g, &v->ptr, &v->idx, &w, &v->pl));
     v->deg = g->out_deg;
     v->perm = nullptr;
+    v->ms_idx = v->idx;
   }
   EG_TRY(static_chunk0(g, v->ptr, v->deg, v->pl));
   v->static_chunk0 = v->pl->static_chunk0;
@@ -956,7 +963,7 @@ int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
       EG_TRY(eg_scan_excl(s, st->qoff, st->qoff, st->qlen, st->scan_tmp));
       EgProf prof(g, "msbfs_push", s);
       hipLaunchKernelGGL(k_ms_push, dim3(blocks_for(st->qdeg, kEdgesPerBlock)), dim3(kBlock), 0, s,
-                         st->queue, st->qlen, st->qoff, bv.ptr, bv.idx, st->visit, st->seen,
+                         st->queue, st->qlen, st->qoff, bv.ptr, bv.ms_idx, st->visit, st->seen,
                          st->pend, active);
       EG_LAUNCHED();
     }
@@ -974,12 +981,12 @@ int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
       EgProf prof(g, "msbfs_pull", s);
       if (g->tune.ms_batch == 16)
         hipLaunchKernelGGL(k_ms_pull<16>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
-                           bv.ptr, bv.idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
+                           bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                            st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
                            active, st->ctr);
       else
         hipLaunchKernelGGL(k_ms_pull<8>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
-                           bv.ptr, bv.idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
+                           bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                            st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
                            active, st->ctr);
       EG_LAUNCHED();

@@ -1001,6 +1001,16 @@ static int bfs_ordered_cols(eg_graph *g, EgOrdered *o) {
   return EG_OK;
 }
 
+int eg_hubs_first_rows(eg_graph *g, const uint32_t **rows) {
+  *rows = nullptr;
+  if (!eg_reorder_enabled() || !(g->flags & (EG_UNDIRECTED | EG_SYMMETRIC))) return EG_OK;
+  EgOrdered *o;
+  EG_TRY(eg_ordered(g, &o));
+  EG_TRY(bfs_ordered_cols(g, o));
+  *rows = g->bfs_ocol;
+  return EG_OK;
+}
+
 // Which rows a symmetric graph's BFS scans: 2 = ordered ids (needs the
 // degree-ordered copy), 1 = hubs first in the layout's ids, 0 = as built.
 static int bfs_mode(const eg_graph *g) {

@@ -256,6 +256,10 @@ int eg_ppr1_step(eg_graph *g, Ppr1 *st);
 int eg_ppr1_prepare(eg_graph *g, int max_iter);
 // Builds the adjacency copies a BFS would build on first use.
 int eg_bfs_prepare(eg_graph *g);
+// Undirected / symmetric graphs: the rows of the degree-ordered copy with
+// every row's entries ascending (hubs first), built on first use; *rows =
+// null where the ordered copy is disabled.
+int eg_hubs_first_rows(eg_graph *g, const uint32_t **rows);
 int eg_ppr1_output(eg_graph *g, Ppr1 *st, float *out);
 // eg_bfs without the ownership check (the serving engine's plain path).
 int eg_bfs_host(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
