@@ -950,6 +950,15 @@ __global__ void k_entry_rows(uint32_t n, const uint32_t *__restrict__ ptr,
   }
 }
 
+// out[0] = the first i with deg[i] == 0 (n if none); deg is non-increasing.
+__global__ void k_first_zero(uint32_t n, const uint32_t *__restrict__ deg,
+                             uint32_t *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (deg[i] == 0 && (i == 0 || deg[i - 1] != 0)) out[0] = i;
+  if (i == n - 1 && deg[i] != 0) out[0] = n;
+}
+
 // Symmetric graphs, ordered ids: the rows of the degree-ordered copy (new
 // ids by descending degree) with every row's entries ascending, so hubs
 // first and the hubs' frontier bits packed into the first bitmap words.
@@ -987,7 +996,16 @@ static int bfs_ordered_cols(eg_graph *g, EgOrdered *o) {
     c2.in[0] = tcol;
     c2.out[0] = out;
     if ((rc = eg_radix_sort(s, m, eg_bits_for(n), trow, nullptr, c2, tmp))) break;
-    if (hipStreamSynchronize(s) != hipSuccess) rc = eg_fail(EG_EDEVICE, "bfs order sort");
+    // vertices with edges: a prefix of the ordered ids (degrees descend)
+    uint32_t nz = 0;
+    if (n) hipLaunchKernelGGL(k_first_zero, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, o->deg, row);
+    if (n && (hipGetLastError() != hipSuccess ||
+              hipMemcpyAsync(&nz, row, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+              hipStreamSynchronize(s) != hipSuccess)) {
+      rc = eg_fail(EG_EDEVICE, "bfs order sort");
+      break;
+    }
+    g->bfs_onz = nz;
   } while (0);
   (void)hipStreamSynchronize(s);
   eg_dfree_t(&row);
@@ -1045,8 +1063,8 @@ static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_
   LevelArgs a;
   a.n = n;
   a.nwords = nwords;
-  a.ntiles = blocks_for(nwords, kTileWords);
-  const uint32_t grid = a.ntiles < 2048 ? a.ntiles : 2048;
+  // bitmap words that pull levels scan (and levels clear)
+  uint32_t scan_words = nwords;
   a.m = g->m;
   a.start_pull = start_pull;
   a.switch_mode = switch_mode;
@@ -1069,7 +1087,14 @@ static int bfs_adjacency(eg_graph *g, uint32_t root, int start_pull, int switch_
     a.ocol = a.iidx = g->bfs_ocol;
     a.id_of = o->old_of_new;
     a.new_of_old = o->new_of_old;
+    // the vertices without edges are the tail of the ordered ids: no level
+    // reaches them, so no frontier bit is ever set there (a root without
+    // edges ends the traversal after its own level, and every traversal
+    // starts by clearing the whole bitmaps)
+    scan_words = (g->bfs_onz + 63) / 64;
   }
+  a.ntiles = blocks_for(scan_words ? scan_words : 1, kTileWords);
+  const uint32_t grid = a.ntiles < 2048 ? a.ntiles : 2048;
   a.visited = g->bfs_visited;
   a.F[0] = g->bfs_front;
   a.F[1] = g->bfs_next;

@@ -165,6 +165,7 @@ struct eg_graph {
   uint64_t *bfs_f3 = nullptr;                    // third frontier bitmap (device-driven BFS)
   uint32_t *bfs_col = nullptr;  // symmetric graphs: CSR columns, each row by descending degree
   uint32_t *bfs_ocol = nullptr; // symmetric graphs: degree-ordered rows, entries ascending (new ids)
+  uint32_t bfs_onz = 0;         // vertices with edges (a prefix of the ordered ids)
   uint32_t *bfs_qoff = nullptr, *bfs_nqoff = nullptr;  // queue edge offsets
   uint32_t bfs_depth_hint = 0;  // levels of the last traversal (first level batch)
   unsigned long long *bfs_part = nullptr;  // per-workgroup partial counts

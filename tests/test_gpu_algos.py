@@ -584,3 +584,30 @@ def test_bfs_order_same_levels(symmetric):
                 assert out[order][2].edges_traversed == out[0][2].edges_traversed
                 assert O.check_bfs_parents(n, ptr, col, int(root), out[order][0],
                                            out[order][1]) == 0
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_bfs_ordered_ids_isolated_roots(symmetric):
+    """Ordered ids: vertices without edges are the tail of the relabelled ids,
+    which pull levels do not scan.  Traversals from such roots (alone in
+    their component) and from ordinary roots, pulled from the first level or
+    not, in any order, give the oracle's levels."""
+    n = 1 << 13
+    s0, d0 = rmat(13, seed=67)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    deg = np.diff(ptr)
+    iso = np.nonzero(deg == 0)[0]
+    assert iso.size > 2
+    roots = eg.sample_roots(deg, 3, seed=5)
+    seq = [(int(iso[0]), False), (int(roots[0]), False), (int(iso[-1]), True),
+           (int(roots[1]), True), (int(iso[1]), False), (int(roots[2]), False)]
+    with eg.Graph(n, 2, undirected=True, symmetric=symmetric) as g:
+        g.load_edges(s0, d0)
+        g.tune_set("bfs_order", 2)
+        for r, start_pull in seq:
+            dist, par, st = g.bfs(r, start_pull=start_pull)
+            ref, _, _ = O.bfs(n, s.size, ptr, col, cp, row, r)
+            assert np.array_equal(dist, ref), (r, start_pull)
+            assert O.check_bfs_parents(n, ptr, col, r, dist, par) == 0
+            if deg[r] == 0:
+                assert st.reached == 1 and dist[r] == 1
