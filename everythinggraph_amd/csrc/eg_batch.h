@@ -25,10 +25,12 @@ struct BatchView {
 };
 int eg_batch_view(eg_graph *g, BatchView *v);
 
-// Up to 64 BFS lanes over one traversal: lane i of vertex v is bit i of the
-// 64-bit masks.  Levels are u16, lane-major (lvl[lane * n + v]), root = 1,
-// 0 = not reached.
+// Up to 64 nw BFS lanes over one traversal (nw = 1 or 2 lane words per
+// vertex): lane i of vertex v is bit i % 64 of word v * nw + i / 64 of the
+// masks.  Levels are u16, lane-major (lvl[lane * n + v]), root = 1, 0 = not
+// reached.
 struct MsState {
+  uint32_t nw = 1;
   hipStream_t stream = nullptr;  // every step is enqueued here
   void *scan_tmp = nullptr;      // frontier degree scan workspace
   uint64_t *seen = nullptr, *visit = nullptr, *nvisit = nullptr, *pend = nullptr;
@@ -37,28 +39,29 @@ struct MsState {
   uint32_t seg_cap = 0;
   uint32_t *queue = nullptr, *qoff = nullptr;  // frontier vertices with edges, degree scan
   unsigned long long *ctr = nullptr;  // [0] progress, [1] count, [2] degsum, [3] qlen, [8..) partials
-  uint16_t *lvl_tab = nullptr;                 // [64] level written per lane this step
-  uint32_t *roots = nullptr;                   // [64]
-  unsigned long long *h = nullptr;             // pinned: [0..3) ctr mirror, [8..72) per-lane sums
+  uint16_t *lvl_tab = nullptr;                 // [64 nw] level written per lane this step
+  uint32_t *roots = nullptr;                   // [256]: lanes, roots (staging)
+  unsigned long long *h = nullptr;             // pinned: [0..5) ctr mirror, [8..8+64nw) lane sums
   uint32_t qlen = 0;                           // frontier queue length (host copy)
   uint64_t qdeg = 0;                           // its out-degree sum
 };
-int eg_ms_alloc(eg_graph *g, MsState *st);
+int eg_ms_alloc(eg_graph *g, MsState *st, uint32_t nw);
 void eg_ms_free(MsState *st);
 // Clears every lane.
 int eg_ms_reset(eg_graph *g, MsState *st);
 // Starts lane[i] at roots[i] (vertex ids of the layout), i < k.
 int eg_ms_seed(eg_graph *g, const BatchView &bv, MsState *st, uint32_t k, const uint32_t *lanes,
                const uint32_t *roots);
-// One level for the lanes in `active`; tab[i] is the level value written for
-// lane i.  *progress = lanes that reached a new vertex.
-int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
+// One level for the lanes in `active` (nw words); tab[i] is the level value
+// written for lane i (64 nw entries).  progress (nw words) = lanes that
+// reached a new vertex.
+int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_t *active,
                 const uint16_t *tab, uint64_t *progress);
 // out[v] (device, n words) = level of lane for vertex v of the layout.
 int eg_ms_extract_lane(eg_graph *g, const BatchView &bv, MsState *st, uint32_t lane,
                        uint32_t *out);
 // Frees lanes (a mask) for reuse; each must have finished (no frontier).
-int eg_ms_clear_lanes(eg_graph *g, MsState *st, uint64_t lanes);
+int eg_ms_clear_lanes(eg_graph *g, MsState *st, const uint64_t *lanes);
 // reached[i] = vertices with a level in lane lanes[i], i < k <= 64.
 int eg_ms_reached(eg_graph *g, MsState *st, uint32_t k, const uint32_t *lanes,
                   uint64_t *reached);

@@ -4,8 +4,9 @@
 // pull PageRank recurrence (pagerank_simple.c:72-84) with a personalization
 // vector.
 //
-// MS-BFS: up to 64 sources share one traversal.  Every vertex carries 64-bit
-// lane masks (seen, frontier, next frontier).  A level is either
+// MS-BFS: up to 64 or 128 sources share one traversal.  Every vertex carries
+// lane masks of one or two 64-bit words (seen, frontier, next frontier); with
+// two, a pull gather loads both words with one 16-byte load.  A level is either
 //  * a pull sweep over the plan of the pull orientation (the chunk / segment
 //    decomposition of eg_pull.hip): a row ORs the frontier masks of its
 //    in-neighbours, gathered eight at a time into LDS, or
@@ -14,7 +15,7 @@
 //    in LDS) into a pending mask, followed by an apply pass.
 // The direction follows the reference BFS rule (bfs_simple.c:191-197) on the
 // frontier of all lanes.  Levels of newly reached (vertex, lane) pairs are
-// written vertex-major as u16 from a per-lane table, so lanes that started at
+// written lane-major as u16 from a per-lane table, so lanes that started at
 // different steps (the serving engine) keep their own level numbering.
 //
 // PPR: k <= 64 vectors stored row-major (64 FP32 columns per vertex).  One
@@ -54,58 +55,159 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t v) {
   return v;
 }
 
-// Adds this workgroup's new lanes to ctr[0] (only bits not yet recorded).
-__device__ __forceinline__ void publish_progress(uint64_t nw, uint64_t *s_or,
+// The lane masks of a vertex: NW 64-bit words (64 or 128 lanes), stored
+// vertex-major (words [v * NW, v * NW + NW)); lane l is bit l % 64 of word
+// l / 64.
+template <int NW>
+struct Lanes {
+  uint64_t w[NW];
+  __device__ __forceinline__ static Lanes zero() {
+    Lanes r;
+#pragma unroll
+    for (int j = 0; j < NW; j++) r.w[j] = 0ull;
+    return r;
+  }
+  __device__ __forceinline__ bool any() const {
+    uint64_t a = 0;
+#pragma unroll
+    for (int j = 0; j < NW; j++) a |= w[j];
+    return a != 0;
+  }
+  __device__ __forceinline__ Lanes operator|(const Lanes &o) const {
+    Lanes r;
+#pragma unroll
+    for (int j = 0; j < NW; j++) r.w[j] = w[j] | o.w[j];
+    return r;
+  }
+  __device__ __forceinline__ Lanes operator&(const Lanes &o) const {
+    Lanes r;
+#pragma unroll
+    for (int j = 0; j < NW; j++) r.w[j] = w[j] & o.w[j];
+    return r;
+  }
+  __device__ __forceinline__ Lanes operator~() const {
+    Lanes r;
+#pragma unroll
+    for (int j = 0; j < NW; j++) r.w[j] = ~w[j];
+    return r;
+  }
+  // every lane of need is set here
+  __device__ __forceinline__ bool covers(const Lanes &need) const {
+    uint64_t miss = 0;
+#pragma unroll
+    for (int j = 0; j < NW; j++) miss |= need.w[j] & ~w[j];
+    return miss == 0;
+  }
+};
+
+template <int NW>
+__device__ __forceinline__ Lanes<NW> ld_lanes(const uint64_t *__restrict__ a, uint64_t v) {
+  Lanes<NW> r;
+  if constexpr (NW == 2) {
+    const ulonglong2 x = reinterpret_cast<const ulonglong2 *>(a)[v];
+    r.w[0] = x.x;
+    r.w[1] = x.y;
+  } else {
+#pragma unroll
+    for (int j = 0; j < NW; j++) r.w[j] = a[v * NW + j];
+  }
+  return r;
+}
+
+template <int NW>
+__device__ __forceinline__ void st_lanes(uint64_t *__restrict__ a, uint64_t v, const Lanes<NW> &x) {
+  if constexpr (NW == 2) {
+    reinterpret_cast<ulonglong2 *>(a)[v] = make_ulonglong2(x.w[0], x.w[1]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < NW; j++) a[v * NW + j] = x.w[j];
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ Lanes<NW> wave_or(Lanes<NW> v) {
+#pragma unroll
+  for (int j = 0; j < NW; j++) v.w[j] = wave_or(v.w[j]);
+  return v;
+}
+
+// ctr word holding the progress bits of lane word j
+__host__ __device__ constexpr int progress_slot(int j) { return j == 0 ? 0 : 4; }
+
+// Adds this workgroup's new lanes to the progress words of ctr (only bits not
+// yet recorded).
+template <int NW>
+__device__ __forceinline__ void publish_progress(Lanes<NW> nw, uint64_t *s_or,
                                                  unsigned long long *ctr) {
   nw = wave_or(nw);
-  if (__lane_id() == 0) s_or[threadIdx.x >> 6] = nw;
+  if (__lane_id() == 0)
+#pragma unroll
+    for (int j = 0; j < NW; j++) s_or[(threadIdx.x >> 6) * NW + j] = nw.w[j];
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint64_t b = 0;
-    for (int w = 0; w < kWaves; w++) b |= s_or[w];
-    const uint64_t known = __atomic_load_n(&ctr[0], __ATOMIC_RELAXED);
-    if (b & ~known) atomicOr(&ctr[0], (unsigned long long)b);
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      uint64_t b = 0;
+      for (int w = 0; w < kWaves; w++) b |= s_or[w * NW + j];
+      unsigned long long *slot = &ctr[progress_slot(j)];
+      const uint64_t known = __atomic_load_n(slot, __ATOMIC_RELAXED);
+      if (b & ~known) atomicOr(slot, (unsigned long long)b);
+    }
   }
 }
 
 // Level store: lane-major u16, lvl[lane * n + v].
-__device__ __forceinline__ uint64_t finish_ms_row(uint32_t n, uint32_t r, uint64_t acc,
-                                                  uint64_t active, uint64_t *__restrict__ seen,
-                                                  uint64_t *__restrict__ nvisit,
-                                                  uint16_t *__restrict__ lvl, const uint16_t *tab) {
-  const uint64_t s = seen[r];
-  const uint64_t nw = acc & ~s & active;
-  nvisit[r] = nw;
-  if (nw) {
-    seen[r] = s | nw;
-    uint64_t b = nw;
+template <int NW>
+__device__ __forceinline__ void store_levels(uint32_t n, uint32_t r, const Lanes<NW> &nw,
+                                             uint16_t *__restrict__ lvl, const uint16_t *tab) {
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    uint64_t b = nw.w[j];
     while (b) {
-      const int i = __ffsll((unsigned long long)b) - 1;
+      const int i = 64 * j + __ffsll((unsigned long long)b) - 1;
       lvl[(uint64_t)i * n + r] = tab[i];
       b &= b - 1;
     }
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ Lanes<NW> finish_ms_row(uint32_t n, uint32_t r, const Lanes<NW> &acc,
+                                                   const Lanes<NW> &active,
+                                                   uint64_t *__restrict__ seen,
+                                                   uint64_t *__restrict__ nvisit,
+                                                   uint16_t *__restrict__ lvl,
+                                                   const uint16_t *tab) {
+  const Lanes<NW> s = ld_lanes<NW>(seen, r);
+  const Lanes<NW> nw = acc & ~s & active;
+  st_lanes<NW>(nvisit, r, nw);
+  if (nw.any()) {
+    st_lanes<NW>(seen, r, s | nw);
+    store_levels<NW>(n, r, nw, lvl, tab);
   }
   return nw;
 }
 
 // The kernels over n x 64 arrays are grid-stride loops: at n = 2^26 one
 // thread per element would exceed the 2^32 threads of a launch.
+template <int NW>
 __global__ void k_ms_reset(uint32_t n, uint64_t *__restrict__ seen, uint64_t *__restrict__ visit,
                            uint64_t *__restrict__ nvisit, uint64_t *__restrict__ pend,
                            uint16_t *__restrict__ lvl) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * 16;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * 16 * NW;
        i += stride) {
-    if (i < n) {
+    if (i < (uint64_t)n * NW) {
       seen[i] = 0;
       visit[i] = 0;
       nvisit[i] = 0;
       pend[i] = 0;
     }
-    reinterpret_cast<uint64_t *>(lvl)[i] = 0;  // 64 u16 = 16 u64
+    reinterpret_cast<uint64_t *>(lvl)[i] = 0;  // 64 NW u16 = 16 NW u64 per vertex
   }
 }
 
+template <int NW>
 __global__ void k_ms_seed(uint32_t n, uint32_t k, const uint32_t *__restrict__ lanes,
                           const uint32_t *__restrict__ roots, const uint32_t *__restrict__ perm,
                           uint64_t *__restrict__ seen, uint64_t *__restrict__ visit,
@@ -114,26 +216,24 @@ __global__ void k_ms_seed(uint32_t n, uint32_t k, const uint32_t *__restrict__ l
   if (i >= k) return;
   const uint32_t r = perm ? perm[roots[i]] : roots[i];
   const uint32_t lane = lanes[i];
-  atomicOr(reinterpret_cast<unsigned long long *>(&seen[r]), 1ull << lane);
-  atomicOr(reinterpret_cast<unsigned long long *>(&visit[r]), 1ull << lane);
+  const uint64_t word = (uint64_t)r * NW + (lane >> 6);
+  atomicOr(reinterpret_cast<unsigned long long *>(&seen[word]), 1ull << (lane & 63));
+  atomicOr(reinterpret_cast<unsigned long long *>(&visit[word]), 1ull << (lane & 63));
   lvl[(uint64_t)lane * n + r] = 1;
 }
 
 // Rows whose new lanes nw are final: record them (only this thread writes
 // row r in this level).
-__device__ __forceinline__ uint64_t finish_ms_new(uint32_t n, uint32_t r, uint64_t nw,
-                                                  uint64_t *__restrict__ seen,
-                                                  uint64_t *__restrict__ nvisit,
-                                                  uint16_t *__restrict__ lvl, const uint16_t *tab) {
-  nvisit[r] = nw;
-  if (nw) {
-    seen[r] |= nw;
-    uint64_t b = nw;
-    while (b) {
-      const int i = __ffsll((unsigned long long)b) - 1;
-      lvl[(uint64_t)i * n + r] = tab[i];
-      b &= b - 1;
-    }
+template <int NW>
+__device__ __forceinline__ Lanes<NW> finish_ms_new(uint32_t n, uint32_t r, const Lanes<NW> &nw,
+                                                   uint64_t *__restrict__ seen,
+                                                   uint64_t *__restrict__ nvisit,
+                                                   uint16_t *__restrict__ lvl,
+                                                   const uint16_t *tab) {
+  st_lanes<NW>(nvisit, r, nw);
+  if (nw.any()) {
+    st_lanes<NW>(seen, r, ld_lanes<NW>(seen, r) | nw);
+    store_levels<NW>(n, r, nw, lvl, tab);
   }
   return nw;
 }
@@ -147,29 +247,30 @@ __device__ __forceinline__ uint64_t finish_ms_new(uint32_t n, uint32_t r, uint64
 //    entries in LDS (B per lane per pass, coalesced) and each needy row gathers
 //    from them: rows of up to kSerialRow entries by one lane, four gathers in
 //    flight, longer rows by a wave, 128 entries per check.
-template <int B>
+// With NW = 2 a gather loads the 16 bytes of both lane words at once.
+template <int B, int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_pull(
     uint32_t n, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
     const uint4 *__restrict__ chunk_info, const uint4 *__restrict__ seg_info,
     const uint32_t *__restrict__ long_rows, uint32_t nseg, uint64_t *__restrict__ seg_or,
     uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
-    uint64_t active, unsigned long long *__restrict__ ctr) {
+    Lanes<NW> active, unsigned long long *__restrict__ ctr) {
   static_assert(kLds <= 65535 && kChunk <= 65535, "u16 LDS offsets");
   __shared__ uint32_t s_idx[kLds];
   __shared__ uint16_t s_ptr[kChunk + 1];
   __shared__ uint16_t s_list[kChunk];
   __shared__ uint32_t s_nlist;
-  __shared__ uint16_t s_tab[64];
-  __shared__ uint64_t s_or[kWaves];
+  __shared__ uint16_t s_tab[64 * NW];
+  __shared__ uint64_t s_or[kWaves * NW];
   const uint32_t tid = threadIdx.x;
   const int lane = __lane_id();
   const int wave = tid >> 6;
   if (blockIdx.x < nseg) {
     const uint4 si = seg_info[blockIdx.x];
-    const uint64_t need = active & ~seen[long_rows[si.z]];
-    uint64_t acc = 0;
-    if (need) {
+    const Lanes<NW> need = active & ~ld_lanes<NW>(seen, long_rows[si.z]);
+    Lanes<NW> acc = Lanes<NW>::zero();
+    if (need.any()) {
       for (uint32_t base = si.x; base < si.y; base += kBlock * B) {
         uint32_t u[B];
 #pragma unroll
@@ -178,26 +279,33 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
           u[q] = k < si.y ? __builtin_nontemporal_load(idx + k) : idx[si.x];
         }
 #pragma unroll
-        for (int q = 0; q < B; q++) acc |= visit[u[q]];
+        for (int q = 0; q < B; q++) acc = acc | ld_lanes<NW>(visit, u[q]);
       }
     }
     acc = wave_or(acc);
-    if (lane == 0) s_or[wave] = acc;
+    if (lane == 0)
+#pragma unroll
+      for (int j = 0; j < NW; j++) s_or[wave * NW + j] = acc.w[j];
     __syncthreads();
-    if (tid == 0) seg_or[blockIdx.x] = s_or[0] | s_or[1] | s_or[2] | s_or[3];
+    if (tid == 0)
+#pragma unroll
+      for (int j = 0; j < NW; j++)
+        seg_or[(uint64_t)blockIdx.x * NW + j] =
+            s_or[0 * NW + j] | s_or[1 * NW + j] | s_or[2 * NW + j] | s_or[3 * NW + j];
     return;
   }
   const uint4 ci = chunk_info[blockIdx.x - nseg];
   const uint32_t r0 = ci.x, r1 = ci.y, k0 = ci.z, k1 = ci.w;
   const uint32_t nrows = r1 - r0;
   if (tid == 0) s_nlist = 0;
-  if (tid < 64) s_tab[tid] = tab[tid];
+  for (uint32_t i = tid; i < 64 * NW; i += kBlock) s_tab[i] = tab[i];
   int any = 0;
-  for (uint32_t i = tid; i < nrows; i += kBlock) any |= (active & ~seen[r0 + i]) != 0;
+  for (uint32_t i = tid; i < nrows; i += kBlock)
+    any |= (active & ~ld_lanes<NW>(seen, r0 + i)).any();
   for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = (uint16_t)(ptr[r0 + i] - k0);
   any = __syncthreads_or(any);
   if (!any) {
-    for (uint32_t i = tid; i < nrows; i += kBlock) nvisit[r0 + i] = 0;
+    for (uint32_t i = tid; i < nrows; i += kBlock) st_lanes<NW>(nvisit, r0 + i, Lanes<NW>::zero());
     return;
   }
   for (uint32_t base = k0; base < k1; base += kBlock * B) {
@@ -214,24 +322,24 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
     }
   }
   __syncthreads();
-  uint64_t nwall = 0;
+  Lanes<NW> nwall = Lanes<NW>::zero();
   for (uint32_t i = tid; i < nrows; i += kBlock) {
     const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
     if (rb - ra <= kSerialRow) {
       const uint32_t r = r0 + i;
-      const uint64_t need = active & ~seen[r];
-      uint64_t acc = 0;
-      if (need) {
+      const Lanes<NW> need = active & ~ld_lanes<NW>(seen, r);
+      Lanes<NW> acc = Lanes<NW>::zero();
+      if (need.any()) {
         for (uint32_t q = ra; q < rb; q += 4) {
-          const uint64_t v0 = visit[s_idx[q]];
-          const uint64_t v1 = q + 1 < rb ? visit[s_idx[q + 1]] : 0ull;
-          const uint64_t v2 = q + 2 < rb ? visit[s_idx[q + 2]] : 0ull;
-          const uint64_t v3 = q + 3 < rb ? visit[s_idx[q + 3]] : 0ull;
-          acc |= (v0 | v1) | (v2 | v3);
-          if ((acc & need) == need) break;
+          const Lanes<NW> v0 = ld_lanes<NW>(visit, s_idx[q]);
+          const Lanes<NW> v1 = q + 1 < rb ? ld_lanes<NW>(visit, s_idx[q + 1]) : Lanes<NW>::zero();
+          const Lanes<NW> v2 = q + 2 < rb ? ld_lanes<NW>(visit, s_idx[q + 2]) : Lanes<NW>::zero();
+          const Lanes<NW> v3 = q + 3 < rb ? ld_lanes<NW>(visit, s_idx[q + 3]) : Lanes<NW>::zero();
+          acc = acc | ((v0 | v1) | (v2 | v3));
+          if (acc.covers(need)) break;
         }
       }
-      nwall |= finish_ms_new(n, r, acc & need, seen, nvisit, lvl, s_tab);
+      nwall = nwall | finish_ms_new<NW>(n, r, acc & need, seen, nvisit, lvl, s_tab);
     } else {
       s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
     }
@@ -242,47 +350,49 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
     const uint32_t i = s_list[li];
     const uint32_t r = r0 + i;
     const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
-    const uint64_t need = active & ~seen[r];
-    uint64_t acc = 0;
-    if (need) {
+    const Lanes<NW> need = active & ~ld_lanes<NW>(seen, r);
+    Lanes<NW> acc = Lanes<NW>::zero();
+    if (need.any()) {
       for (uint32_t base = ra; base < rb; base += 128) {
         const uint32_t qa = base + lane, qb = base + 64 + lane;
-        const uint64_t va = qa < rb ? visit[s_idx[qa]] : 0ull;
-        const uint64_t vb = qb < rb ? visit[s_idx[qb]] : 0ull;
+        const Lanes<NW> va = qa < rb ? ld_lanes<NW>(visit, s_idx[qa]) : Lanes<NW>::zero();
+        const Lanes<NW> vb = qb < rb ? ld_lanes<NW>(visit, s_idx[qb]) : Lanes<NW>::zero();
         acc = wave_or(acc | va | vb);
-        if ((acc & need) == need) break;
+        if (acc.covers(need)) break;
       }
     }
-    if (lane == 0) nwall |= finish_ms_new(n, r, acc & need, seen, nvisit, lvl, s_tab);
+    if (lane == 0) nwall = nwall | finish_ms_new<NW>(n, r, acc & need, seen, nvisit, lvl, s_tab);
   }
-  publish_progress(nwall, s_or, ctr);
+  publish_progress<NW>(nwall, s_or, ctr);
 }
 
+template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
     uint32_t n, const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
     const uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
-    uint64_t active, unsigned long long *__restrict__ ctr) {
-  __shared__ uint64_t s_or[kWaves];
+    Lanes<NW> active, unsigned long long *__restrict__ ctr) {
+  __shared__ uint64_t s_or[kWaves * NW];
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t nw = 0;
+  Lanes<NW> nw = Lanes<NW>::zero();
   if (j < nlong) {
-    uint64_t acc = 0;
-    for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++) acc |= seg_or[s];
-    nw = finish_ms_row(n, long_rows[j], acc, active, seen, nvisit, lvl, tab);
+    Lanes<NW> acc = Lanes<NW>::zero();
+    for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++) acc = acc | ld_lanes<NW>(seg_or, s);
+    nw = finish_ms_row<NW>(n, long_rows[j], acc, active, seen, nvisit, lvl, tab);
   }
-  publish_progress(nw, s_or, ctr);
+  publish_progress<NW>(nw, s_or, ctr);
 }
 
 // Frontier statistics of visit: [1] vertices with out-edges, [2] their degree
 // sum (per-workgroup partials, summed by k_ms_stats_sum).
+template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_stats(uint32_t n, const uint64_t *__restrict__ visit,
                                                      const uint32_t *__restrict__ ptr,
                                                      unsigned long long *__restrict__ part) {
   __shared__ unsigned long long s_c[kWaves], s_d[kWaves];
   unsigned long long c = 0, d = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    if (visit[v]) {
+    if (ld_lanes<NW>(visit, v).any()) {
       const uint32_t dv = ptr[v + 1] - ptr[v];
       c += dv > 0;
       d += dv;
@@ -331,6 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_stats_sum(const unsigned long lon
 
 // Frontier vertices with out-edges -> queue (run only for small frontiers:
 // one atomic per workgroup that holds some).
+template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_compact(uint32_t n, const uint64_t *__restrict__ visit,
                                                        const uint32_t *__restrict__ ptr,
                                                        uint32_t *__restrict__ q,
@@ -341,7 +452,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_compact(uint32_t n, const uint64_
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const uint32_t v = b0 + threadIdx.x;
-    if (v < n && visit[v] && ptr[v + 1] > ptr[v]) s_q[atomicAdd(&s_n, 1u)] = v;
+    if (v < n && ld_lanes<NW>(visit, v).any() && ptr[v + 1] > ptr[v])
+      s_q[atomicAdd(&s_n, 1u)] = v;
     __syncthreads();
     if (threadIdx.x == 0 && s_n) s_base = (uint32_t)atomicAdd(qlen, (unsigned long long)s_n);
     __syncthreads();
@@ -357,14 +469,15 @@ __global__ void k_ms_qdeg(const uint32_t *__restrict__ q, uint32_t qn,
 }
 
 // Push level: the frontier's edges [b*2048, (b+1)*2048) of the degree scan.
+template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_push(
     const uint32_t *__restrict__ q, uint32_t qn, const uint32_t *__restrict__ qoff,
     const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
     const uint64_t *__restrict__ visit, const uint64_t *__restrict__ seen,
-    uint64_t *__restrict__ pend, uint64_t active) {
+    uint64_t *__restrict__ pend, Lanes<NW> active) {
   __shared__ uint32_t s_off[kEdgesPerBlock + 1];
   __shared__ uint32_t s_u[kEdgesPerBlock + 1];
-  __shared__ uint64_t s_vis[kEdgesPerBlock + 1];
+  __shared__ uint64_t s_vis[(kEdgesPerBlock + 1) * NW];
   __shared__ uint32_t s_first, s_cnt;
   const uint32_t e0 = blockIdx.x * kEdgesPerBlock;
   const uint32_t total = qoff[qn];
@@ -389,7 +502,9 @@ __global__ __launch_bounds__(kBlock) void k_ms_push(
     s_off[i] = qoff[s_first + i];
     const uint32_t u = q[s_first + i];
     s_u[i] = u;
-    s_vis[i] = visit[u] & active;
+    const Lanes<NW> vis = ld_lanes<NW>(visit, u) & active;
+#pragma unroll
+    for (int j = 0; j < NW; j++) s_vis[i * NW + j] = vis.w[j];
   }
   __syncthreads();
   for (uint32_t base = e0; base < e1; base += kBlock) {
@@ -401,37 +516,41 @@ __global__ __launch_bounds__(kBlock) void k_ms_push(
         if (s_off[mid] <= e) lo = mid; else hi = mid;
       }
       const uint32_t w = idx[ptr[s_u[lo]] + (e - s_off[lo])];
-      const uint64_t bits = s_vis[lo] & ~__atomic_load_n(&seen[w], __ATOMIC_RELAXED);
-      if (bits && (__atomic_load_n(&pend[w], __ATOMIC_RELAXED) & bits) != bits)
-        atomicOr(reinterpret_cast<unsigned long long *>(&pend[w]), (unsigned long long)bits);
+#pragma unroll
+      for (int j = 0; j < NW; j++) {
+        const uint64_t word = (uint64_t)w * NW + j;
+        const uint64_t bits =
+            s_vis[lo * NW + j] & ~__atomic_load_n(&seen[word], __ATOMIC_RELAXED);
+        if (bits && (__atomic_load_n(&pend[word], __ATOMIC_RELAXED) & bits) != bits)
+          atomicOr(reinterpret_cast<unsigned long long *>(&pend[word]), (unsigned long long)bits);
+      }
     }
   }
 }
 
+template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, uint64_t *__restrict__ pend,
                                                      uint64_t *__restrict__ seen,
                                                      uint64_t *__restrict__ nvisit,
                                                      uint16_t *__restrict__ lvl,
                                                      const uint16_t *__restrict__ tab,
-                                                     uint64_t active,
+                                                     Lanes<NW> active,
                                                      unsigned long long *__restrict__ ctr) {
-  __shared__ uint64_t s_or[kWaves];
-  __shared__ uint16_t s_tab[64];
-  if (threadIdx.x < 64) s_tab[threadIdx.x] = tab[threadIdx.x];
+  __shared__ uint64_t s_or[kWaves * NW];
+  __shared__ uint16_t s_tab[64 * NW];
+  for (uint32_t i = threadIdx.x; i < 64 * NW; i += blockDim.x) s_tab[i] = tab[i];
   __syncthreads();
-  uint64_t all = 0;
+  Lanes<NW> all = Lanes<NW>::zero();
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const uint64_t p = pend[v];
-    uint64_t nw = 0;
-    if (p) {
-      pend[v] = 0;
-      nw = finish_ms_row(n, v, p, active, seen, nvisit, lvl, s_tab);
+    const Lanes<NW> p = ld_lanes<NW>(pend, v);
+    if (p.any()) {
+      st_lanes<NW>(pend, v, Lanes<NW>::zero());
+      all = all | finish_ms_row<NW>(n, v, p, active, seen, nvisit, lvl, s_tab);
     } else {
-      nvisit[v] = 0;
+      st_lanes<NW>(nvisit, v, Lanes<NW>::zero());
     }
-    all |= nw;
   }
-  publish_progress(all, s_or, ctr);
+  publish_progress<NW>(all, s_or, ctr);
 }
 
 __global__ void k_ms_extract(uint32_t n, uint32_t k, uint32_t lane0,
@@ -450,17 +569,21 @@ __global__ void k_ms_extract(uint32_t n, uint32_t k, uint32_t lane0,
 // Retired lanes: drop their bits from every mask and zero their levels.  A
 // lane retires after a level in which it reached nothing, so its frontier
 // bits are already clear; nvisit and pend are rewritten by every level.
-__global__ void k_ms_clear_lanes(uint32_t n, uint64_t lanes, uint64_t *__restrict__ seen,
+template <int NW>
+__global__ void k_ms_clear_lanes(uint32_t n, Lanes<NW> lanes, uint64_t *__restrict__ seen,
                                  uint64_t *__restrict__ visit, uint16_t *__restrict__ lvl) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  seen[v] &= ~lanes;
-  visit[v] &= ~lanes;
-  uint64_t b = lanes;
-  while (b) {
-    const int i = __ffsll((unsigned long long)b) - 1;
-    lvl[(uint64_t)i * n + v] = 0;
-    b &= b - 1;
+  st_lanes<NW>(seen, v, ld_lanes<NW>(seen, v) & ~lanes);
+  st_lanes<NW>(visit, v, ld_lanes<NW>(visit, v) & ~lanes);
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    uint64_t b = lanes.w[j];
+    while (b) {
+      const int i = 64 * j + __ffsll((unsigned long long)b) - 1;
+      lvl[(uint64_t)i * n + v] = 0;
+      b &= b - 1;
+    }
   }
 }
 
@@ -852,23 +975,29 @@ int eg_batch_view(eg_graph *g, BatchView *v) {
   return EG_OK;
 }
 
-int eg_ms_alloc(eg_graph *g, MsState *st) {
+int eg_ms_alloc(eg_graph *g, MsState *st, uint32_t nw) {
+  if (nw != 1 && nw != 2) return eg_fail(EG_EINVAL, "MS-BFS lane words must be 1 or 2");
   const uint32_t n = g->n;
-  EG_TRY(eg_dalloc_t(&st->seen, n, "ms seen"));
-  EG_TRY(eg_dalloc_t(&st->visit, n, "ms visit"));
-  EG_TRY(eg_dalloc_t(&st->nvisit, n, "ms next"));
-  EG_TRY(eg_dalloc_t(&st->pend, n, "ms pending"));
-  EG_TRY(eg_dalloc_t(&st->lvl, (uint64_t)n * 64, "ms levels"));
+  st->nw = nw;
+  const uint64_t nn = (uint64_t)n * nw;
+  EG_TRY(eg_dalloc_t(&st->seen, nn, "ms seen"));
+  EG_TRY(eg_dalloc_t(&st->visit, nn, "ms visit"));
+  EG_TRY(eg_dalloc_t(&st->nvisit, nn, "ms next"));
+  EG_TRY(eg_dalloc_t(&st->pend, nn, "ms pending"));
+  EG_TRY(eg_dalloc_t(&st->lvl, (uint64_t)n * 64 * nw, "ms levels"));
   EG_TRY(eg_dalloc_t(&st->queue, (uint64_t)n + 64, "ms queue"));
   EG_TRY(eg_dalloc_t(&st->qoff, (uint64_t)n + 65, "ms queue offsets"));
+  // [0] and [4] progress words, [1] count, [2] degree sum, [3] queue length;
   // [8..) holds the frontier statistics partials (2 per workgroup) or the
-  // per-lane reached partials (64 x 64) and sums (64), never both at once.
-  EG_TRY(eg_dalloc_t(&st->ctr, 8 + 2 * (uint64_t)kStatBlocks + 64 * 64 + 64, "ms counters"));
-  EG_TRY(eg_dalloc_t(&st->lvl_tab, 64, "ms level table"));
-  EG_TRY(eg_dalloc_t(&st->roots, 128, "ms roots"));
+  // per-lane reached partials (64 nw x 64) and sums (64 nw), never both at once.
+  EG_TRY(eg_dalloc_t(&st->ctr, 8 + 2 * (uint64_t)kStatBlocks + 64 * 64 * nw + 64 * nw,
+                     "ms counters"));
+  EG_TRY(eg_dalloc_t(&st->lvl_tab, 64 * nw, "ms level table"));
+  EG_TRY(eg_dalloc_t(&st->roots, 2 * 128, "ms roots"));
   EG_TRY(eg_dalloc_t(reinterpret_cast<uint8_t **>(&st->scan_tmp),
                      eg_scan_tmp_bytes((uint64_t)n + 64), "ms scan workspace"));
-  if (hipHostMalloc(reinterpret_cast<void **>(&st->h), 72 * 8, hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc(reinterpret_cast<void **>(&st->h), (8 + 64 * nw) * 8, hipHostMallocDefault) !=
+      hipSuccess)
     return eg_fail(EG_ENOMEM, "pinned counters");
   return EG_OK;
 }
@@ -892,10 +1021,22 @@ void eg_ms_free(MsState *st) {
   st->h = nullptr;
 }
 
+template <int NW>
+static Lanes<NW> to_lanes(const uint64_t *w) {
+  Lanes<NW> r;
+  for (int j = 0; j < NW; j++) r.w[j] = w[j];
+  return r;
+}
+
 int eg_ms_reset(eg_graph *g, MsState *st) {
   const uint32_t n = g->n;
-  hipLaunchKernelGGL(k_ms_reset, dim3(stride_grid((uint64_t)n * 16)), dim3(256), 0, st->stream,
-                     n, st->seen, st->visit, st->nvisit, st->pend, st->lvl);
+  const uint64_t count = (uint64_t)n * 16 * st->nw;
+  if (st->nw == 2)
+    hipLaunchKernelGGL(k_ms_reset<2>, dim3(stride_grid(count)), dim3(256), 0, st->stream, n,
+                       st->seen, st->visit, st->nvisit, st->pend, st->lvl);
+  else
+    hipLaunchKernelGGL(k_ms_reset<1>, dim3(stride_grid(count)), dim3(256), 0, st->stream, n,
+                       st->seen, st->visit, st->nvisit, st->pend, st->lvl);
   EG_LAUNCHED();
   st->qlen = 0;
   st->qdeg = 0;
@@ -903,12 +1044,13 @@ int eg_ms_reset(eg_graph *g, MsState *st) {
 }
 
 // Frontier of visit: statistics, and the queue when the next level may push.
+template <int NW>
 static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *push) {
   const uint32_t n = g->n;
   hipStream_t s = st->stream;
   unsigned long long *part = st->ctr + 8;
   const uint32_t nb = blocks_for(n, kBlock) < kStatBlocks ? blocks_for(n, kBlock) : kStatBlocks;
-  hipLaunchKernelGGL(k_ms_stats, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part);
+  hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part);
   EG_LAUNCHED();
   hipLaunchKernelGGL(k_ms_stats_sum, dim3(1), dim3(kBlock), 0, s, part, nb, st->ctr);
   EG_LAUNCHED();
@@ -921,8 +1063,8 @@ static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *pu
   st->qlen = 0;
   if (*push && cnt) {
     EG_HIP(hipMemsetAsync(st->ctr + 3, 0, 8, s));
-    hipLaunchKernelGGL(k_ms_compact, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, st->queue,
-                       st->ctr + 3);
+    hipLaunchKernelGGL(k_ms_compact<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr,
+                       st->queue, st->ctr + 3);
     EG_LAUNCHED();
     st->qlen = (uint32_t)cnt;
   }
@@ -932,28 +1074,35 @@ static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *pu
 int eg_ms_seed(eg_graph *g, const BatchView &bv, MsState *st, uint32_t k, const uint32_t *lanes,
                const uint32_t *roots) {
   if (k == 0) return EG_OK;
-  if (k > 64) return eg_fail(EG_EINVAL, "at most 64 lanes");
+  if (k > 64 * st->nw) return eg_fail(EG_EINVAL, "at most %u lanes", 64 * st->nw);
+  for (uint32_t i = 0; i < k; i++)
+    if (lanes[i] >= 64 * st->nw) return eg_fail(EG_EINVAL, "lane %u out of range", lanes[i]);
   hipStream_t s = st->stream;
-  uint32_t h[128];
+  uint32_t h[256];
   memcpy(h, lanes, k * 4);
-  memcpy(h + 64, roots, k * 4);
+  memcpy(h + 128, roots, k * 4);
   EG_HIP(hipMemcpyAsync(st->roots, h, sizeof(h), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, g->n, k, st->roots, st->roots + 64,
-                     bv.perm,
-                     st->seen, st->visit, st->lvl);
+  if (st->nw == 2)
+    hipLaunchKernelGGL(k_ms_seed<2>, dim3(1), dim3(128), 0, s, g->n, k, st->roots,
+                       st->roots + 128, bv.perm, st->seen, st->visit, st->lvl);
+  else
+    hipLaunchKernelGGL(k_ms_seed<1>, dim3(1), dim3(128), 0, s, g->n, k, st->roots,
+                       st->roots + 128, bv.perm, st->seen, st->visit, st->lvl);
   EG_LAUNCHED();
   EG_HIP(hipStreamSynchronize(s));  // h is on the stack
   return EG_OK;
 }
 
-int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
-                const uint16_t *tab, uint64_t *progress) {
+template <int NW>
+static int ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_t *active_w,
+                    const uint16_t *tab, uint64_t *progress) {
   const uint32_t n = g->n;
   hipStream_t s = st->stream;
+  const Lanes<NW> active = to_lanes<NW>(active_w);
   bool push = false;
-  EG_TRY(frontier_prep(g, bv, st, &push));
-  EG_HIP(hipMemcpyAsync(st->lvl_tab, tab, 128, hipMemcpyHostToDevice, s));
-  EG_HIP(hipMemsetAsync(st->ctr, 0, 8, s));
+  EG_TRY(frontier_prep<NW>(g, bv, st, &push));
+  EG_HIP(hipMemcpyAsync(st->lvl_tab, tab, 128 * NW, hipMemcpyHostToDevice, s));
+  for (int j = 0; j < NW; j++) EG_HIP(hipMemsetAsync(st->ctr + progress_slot(j), 0, 8, s));
   const PullPlan &pl = *bv.pl;
   if (push) {
     if (st->qlen && st->qdeg) {
@@ -962,61 +1111,74 @@ int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, uint64_t active,
       EG_LAUNCHED();
       EG_TRY(eg_scan_excl(s, st->qoff, st->qoff, st->qlen, st->scan_tmp));
       EgProf prof(g, "msbfs_push", s);
-      hipLaunchKernelGGL(k_ms_push, dim3(blocks_for(st->qdeg, kEdgesPerBlock)), dim3(kBlock), 0, s,
-                         st->queue, st->qlen, st->qoff, bv.ptr, bv.ms_idx, st->visit, st->seen,
-                         st->pend, active);
+      hipLaunchKernelGGL(k_ms_push<NW>, dim3(blocks_for(st->qdeg, kEdgesPerBlock)), dim3(kBlock),
+                         0, s, st->queue, st->qlen, st->qoff, bv.ptr, bv.ms_idx, st->visit,
+                         st->seen, st->pend, active);
       EG_LAUNCHED();
     }
     const uint32_t nb = blocks_for(n, kBlock) < 4096 ? blocks_for(n, kBlock) : 4096;
-    hipLaunchKernelGGL(k_ms_apply, dim3(nb), dim3(kBlock), 0, s, n, st->pend, st->seen, st->nvisit,
-                       st->lvl, st->lvl_tab, active, st->ctr);
+    hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->pend, st->seen,
+                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr);
     EG_LAUNCHED();
   } else {
     if (st->seg_cap < pl.nseg) {
       eg_dfree_t(&st->seg_or);
-      EG_TRY(eg_dalloc_t(&st->seg_or, pl.nseg, "ms segment ors"));
+      EG_TRY(eg_dalloc_t(&st->seg_or, (uint64_t)pl.nseg * 2, "ms segment ors"));
       st->seg_cap = pl.nseg;
     }
     {
       EgProf prof(g, "msbfs_pull", s);
       if (g->tune.ms_batch == 16)
-        hipLaunchKernelGGL(k_ms_pull<16>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
-                           bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
-                           st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
-                           active, st->ctr);
+        hipLaunchKernelGGL((k_ms_pull<16, NW>), dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s,
+                           n, bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows,
+                           pl.nseg, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl,
+                           st->lvl_tab, active, st->ctr);
       else
-        hipLaunchKernelGGL(k_ms_pull<8>, dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
+        hipLaunchKernelGGL((k_ms_pull<8, NW>), dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
                            bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                            st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
                            active, st->ctr);
       EG_LAUNCHED();
     }
     if (pl.nlong) {
-      hipLaunchKernelGGL(k_ms_pull_finish, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock), 0, s,
-                         n, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen, st->nvisit,
-                         st->lvl, st->lvl_tab, active, st->ctr);
+      hipLaunchKernelGGL(k_ms_pull_finish<NW>, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock),
+                         0, s, n, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen,
+                         st->nvisit, st->lvl, st->lvl_tab, active, st->ctr);
       EG_LAUNCHED();
     }
   }
   std::swap(st->visit, st->nvisit);
-  EG_HIP(hipMemcpyAsync(st->h, st->ctr, 8, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipMemcpyAsync(st->h, st->ctr, 40, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
-  *progress = st->h[0];
+  for (int j = 0; j < NW; j++) progress[j] = st->h[progress_slot(j)];
   return EG_OK;
+}
+
+int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_t *active,
+                const uint16_t *tab, uint64_t *progress) {
+  return st->nw == 2 ? ms_level<2>(g, bv, st, active, tab, progress)
+                     : ms_level<1>(g, bv, st, active, tab, progress);
 }
 
 int eg_ms_extract_lane(eg_graph *g, const BatchView &bv, MsState *st, uint32_t lane,
                        uint32_t *out) {
+  if (lane >= 64 * st->nw) return eg_fail(EG_EINVAL, "lane %u out of range", lane);
   hipLaunchKernelGGL(k_ms_extract, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream, g->n, 1u,
                      lane, st->lvl, bv.perm, out);
   EG_LAUNCHED();
   return EG_OK;
 }
 
-int eg_ms_clear_lanes(eg_graph *g, MsState *st, uint64_t lanes) {
-  if (!lanes) return EG_OK;
-  hipLaunchKernelGGL(k_ms_clear_lanes, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
-                     g->n, lanes, st->seen, st->visit, st->lvl);
+int eg_ms_clear_lanes(eg_graph *g, MsState *st, const uint64_t *lanes) {
+  uint64_t any = 0;
+  for (uint32_t j = 0; j < st->nw; j++) any |= lanes[j];
+  if (!any) return EG_OK;
+  if (st->nw == 2)
+    hipLaunchKernelGGL(k_ms_clear_lanes<2>, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
+                       g->n, to_lanes<2>(lanes), st->seen, st->visit, st->lvl);
+  else
+    hipLaunchKernelGGL(k_ms_clear_lanes<1>, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
+                       g->n, to_lanes<1>(lanes), st->seen, st->visit, st->lvl);
   EG_LAUNCHED();
   return EG_OK;
 }
@@ -1024,13 +1186,13 @@ int eg_ms_clear_lanes(eg_graph *g, MsState *st, uint64_t lanes) {
 int eg_ms_reached(eg_graph *g, MsState *st, uint32_t k, const uint32_t *lanes,
                   uint64_t *reached) {
   if (k == 0) return EG_OK;
-  if (k > 64) return eg_fail(EG_EINVAL, "at most 64 lanes");
+  if (k > 64 * st->nw) return eg_fail(EG_EINVAL, "at most %u lanes", 64 * st->nw);
   hipStream_t s = st->stream;
   const uint32_t nb = blocks_for(g->n, kBlock) < 64 ? blocks_for(g->n, kBlock) : 64;
-  uint32_t h[64];
+  uint32_t h[128];
   memcpy(h, lanes, k * 4);
   EG_HIP(hipMemcpyAsync(st->roots, h, k * 4, hipMemcpyHostToDevice, s));
-  unsigned long long *part = st->ctr + 8, *sums = st->ctr + 8 + 64 * 64;
+  unsigned long long *part = st->ctr + 8, *sums = st->ctr + 8 + 64 * 64 * st->nw;
   hipLaunchKernelGGL(k_ms_reached, dim3(nb, k), dim3(kBlock), 0, s, g->n, st->lvl, st->roots, part);
   EG_LAUNCHED();
   hipLaunchKernelGGL(k_ms_reached_sum, dim3(k), dim3(64), 0, s, part, nb, sums);
@@ -1243,9 +1405,9 @@ int eg_ppr_clear_cols(eg_graph *g, PprState *st, uint64_t cols) {
 // The batched API.
 // ---------------------------------------------------------------------------
 struct BatchState {
-  MsState ms;
+  MsState ms, ms2;  // one lane word (up to 64 sources), two (128 per traversal)
   PprState ppr;
-  bool ms_ok = false, ppr_ok = false;
+  bool ms_ok = false, ms2_ok = false, ppr_ok = false;
   uint32_t *ext = nullptr;  // k x n extraction buffer
   uint64_t ext_cap = 0;
 };
@@ -1254,6 +1416,7 @@ void eg_batch_free(eg_graph *g) {
   BatchState *b = g->batch;
   if (!b) return;
   eg_ms_free(&b->ms);
+  eg_ms_free(&b->ms2);
   eg_ppr_free(&b->ppr);
   eg_dfree_t(&b->ext);
   delete b;
@@ -1281,39 +1444,49 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
   if (k && !roots) return eg_fail(EG_EINVAL, "null roots");
   for (uint32_t i = 0; i < k; i++)
     if (roots[i] >= g->n) return eg_fail(EG_ERANGE, "root %u >= n", roots[i]);
+  // up to 64 sources in one traversal of one lane word, more in traversals
+  // of 128 lanes (two words: a gather fetches both with one 16-byte load)
+  const uint32_t nw = k > 64 ? 2u : 1u;
   BatchState *b = batch_state(g);
-  if (!b->ms_ok) {
-    EG_TRY(eg_ms_alloc(g, &b->ms));
-    b->ms.stream = g->stream;
-    b->ms_ok = true;
+  MsState *mst = nw == 2 ? &b->ms2 : &b->ms;
+  bool &ok = nw == 2 ? b->ms2_ok : b->ms_ok;
+  if (!ok) {
+    EG_TRY(eg_ms_alloc(g, mst, nw));
+    mst->stream = g->stream;
+    ok = true;
   }
   BatchView bv;
   EG_TRY(eg_batch_view(g, &bv));
   hipStream_t s = g->stream;
   const uint32_t n = g->n;
+  const uint32_t per = 64 * nw;
   if (dist) EG_TRY(ensure_ext(b, (uint64_t)n * 64));
   double total_ms = 0.0;
-  uint32_t lanes[64];
-  for (uint32_t i = 0; i < 64; i++) lanes[i] = i;
-  for (uint32_t g0 = 0; g0 < k; g0 += 64) {
-    const uint32_t kk = k - g0 < 64 ? k - g0 : 64;
+  uint32_t lanes[128];
+  for (uint32_t i = 0; i < 128; i++) lanes[i] = i;
+  for (uint32_t g0 = 0; g0 < k; g0 += per) {
+    const uint32_t kk = k - g0 < per ? k - g0 : per;
     hipEvent_t e0, e1;
     EG_HIP(hipEventCreate(&e0));
     EG_HIP(hipEventCreate(&e1));
     EG_HIP(hipEventRecord(e0, s));
-    int rc = eg_ms_reset(g, &b->ms);
-    if (rc == EG_OK) rc = eg_ms_seed(g, bv, &b->ms, kk, lanes, roots + g0);
-    const uint64_t active = kk == 64 ? ~0ull : ((1ull << kk) - 1ull);
-    uint16_t tab[64];
+    int rc = eg_ms_reset(g, mst);
+    if (rc == EG_OK) rc = eg_ms_seed(g, bv, mst, kk, lanes, roots + g0);
+    uint64_t active[2];
+    for (uint32_t j = 0; j < 2; j++) {
+      const uint32_t c = kk > 64 * j ? kk - 64 * j : 0;
+      active[j] = c >= 64 ? ~0ull : ((1ull << c) - 1ull);
+    }
+    uint16_t tab[128];
     for (int level = 2; rc == EG_OK; level++) {
       if (level >= 65535 || (uint64_t)level > (uint64_t)n + 1) {
         rc = eg_fail(EG_ENOTSUP, "BFS depth exceeds the u16 level store");
         break;
       }
-      for (int i = 0; i < 64; i++) tab[i] = (uint16_t)level;
-      uint64_t progress = 0;
-      rc = eg_ms_level(g, bv, &b->ms, active, tab, &progress);
-      if (rc == EG_OK && progress == 0) break;
+      for (int i = 0; i < 128; i++) tab[i] = (uint16_t)level;
+      uint64_t progress[2] = {0, 0};
+      rc = eg_ms_level(g, bv, mst, active, tab, progress);
+      if (rc == EG_OK && (progress[0] | (nw == 2 ? progress[1] : 0ull)) == 0) break;
     }
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
@@ -1323,11 +1496,13 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     total_ms += ms;
-    if (dist) {
-      hipLaunchKernelGGL(k_ms_extract, dim3(stride_grid((uint64_t)n * kk)), dim3(256), 0, s,
-                         n, kk, 0u, b->ms.lvl, bv.perm, b->ext);
+    // 64 lanes at a time through the n x 64 extraction buffer
+    for (uint32_t l0 = 0; dist && l0 < kk; l0 += 64) {
+      const uint32_t kl = kk - l0 < 64 ? kk - l0 : 64;
+      hipLaunchKernelGGL(k_ms_extract, dim3(stride_grid((uint64_t)n * kl)), dim3(256), 0, s, n,
+                         kl, l0, mst->lvl, bv.perm, b->ext);
       EG_LAUNCHED();
-      EG_HIP(hipMemcpyAsync(dist + (uint64_t)g0 * n, b->ext, (uint64_t)n * kk * 4,
+      EG_HIP(hipMemcpyAsync(dist + (uint64_t)(g0 + l0) * n, b->ext, (uint64_t)n * kl * 4,
                             hipMemcpyDeviceToHost, s));
       EG_HIP(hipStreamSynchronize(s));
     }

@@ -289,7 +289,7 @@ int bfs_add_group(eg_server *S, bool *ok) {
   LaneGroup *G = new LaneGroup();
   const uint32_t i = (uint32_t)S->groups.size();
   G->cap = std::min<uint32_t>(64, S->o.bfs_lanes - 64 * i);
-  int rc = eg_ms_alloc(S->g, &G->st);
+  int rc = eg_ms_alloc(S->g, &G->st, 1);
   if (rc == EG_OK) {
     G->st.stream = S->s_bfs;
     rc = eg_ms_reset(S->g, &G->st);
@@ -359,7 +359,7 @@ int bfs_level(eg_server *S, LaneGroup *G) {
   }
   uint64_t progress = 0;
   const uint64_t run = G->active & ~deep;
-  if (run) EG_TRY(eg_ms_level(S->g, S->bv, &G->st, run, tab, &progress));
+  if (run) EG_TRY(eg_ms_level(S->g, S->bv, &G->st, &run, tab, &progress));
   {
     std::lock_guard<std::mutex> lk(S->mu);
     S->stats.bfs_levels++;
@@ -385,7 +385,7 @@ int bfs_level(eg_server *S, LaneGroup *G) {
       EG_HIP(hipStreamSynchronize(S->s_bfs));
     }
   }
-  EG_TRY(eg_ms_clear_lanes(S->g, &G->st, retire));
+  EG_TRY(eg_ms_clear_lanes(S->g, &G->st, &retire));
   for (uint32_t i = 0; i < k; i++) {
     Query *qr = G->lane[lanes[i]];
     G->lane[lanes[i]] = nullptr;

@@ -306,6 +306,23 @@ def test_bfs_batch_equals_single():
             assert np.array_equal(out[i], ref), i
 
 
+@pytest.mark.parametrize("undirected,symmetric,k", [(True, True, 200), (False, False, 130)])
+def test_bfs_batch_two_lane_words(undirected, symmetric, k):
+    """More than 64 sources run in traversals of 128 lanes (two mask words per
+    vertex); push levels (symmetric graphs) and pull levels both carry the
+    second word, and a last group smaller than 64 leaves it idle."""
+    n = 1 << 13
+    s0, d0 = rmat(13, seed=14)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=undirected)
+    roots = eg.sample_roots(np.diff(ptr), k, seed=4)
+    with eg.Graph(n, 2, undirected=undirected, symmetric=symmetric) as g:
+        g.load_edges(s0, d0)
+        out, _ = g.bfs_batch(roots)
+    for i, r in enumerate(roots):
+        ref, _, _ = O.bfs(n, s.size, ptr, col, cp, row, int(r))
+        assert np.array_equal(out[i], ref), i
+
+
 def test_bfs_batch_directed_duplicates():
     n = 1 << 11
     s0, d0 = rmat(11, seed=12)
