@@ -468,10 +468,10 @@ def test_ppr_graph_replay_matches_eager():
     with eg.Graph(n, 2, undirected=True) as g:
         g.load_edges(s0, d0)
         g.tune_set("graphs", 0)
-        eager = {it: g.ppr_batch(seeds, 0.15, it)[0] for it in (4, 7, 30)}
+        eager = {it: g.ppr_batch(seeds, 0.15, it)[0] for it in (4, 6, 7, 30)}
         g.tune_set("graphs", 1)
         for rep in range(3):
-            for it in (4, 7, 30):
+            for it in (4, 6, 7, 30):
                 out = g.ppr_batch(seeds, 0.15, it)[0]
                 assert np.array_equal(out, eager[it]), (rep, it)
             g.pagerank(0.85, 3)          # unrelated eager work between replays

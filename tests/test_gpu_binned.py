@@ -295,7 +295,7 @@ def test_batched_ppr_static_tail():
         g.load_edges(s0, d0)
         for graphs in (0, 1):
             g.tune_set("graphs", graphs)
-            for it in (1, 2, 3, 7, 30):
+            for it in (1, 2, 3, 6, 7, 30):
                 out = g.ppr_batch(seeds, 0.15, it)[0]
                 for q, sd in enumerate(seeds):
                     ref = O.ppr(n, cp, row, deg, sd, 0.15, it)
