@@ -294,6 +294,7 @@ static void free_algo_state(eg_graph *g) {
   eg_dfree_t(&g->bfs_ctr);
   eg_dfree_t(&g->wcc_label);
   eg_dfree_t(&g->wcc_lab32);
+  eg_dfree_t(&g->wcc_tmp);
   eg_dfree_t(&g->ppr_base);
   eg_dfree_t(&g->ppr_seeds);
   g->ppr_seeds_cap = 0;

@@ -119,6 +119,7 @@ struct EgTune {
   int pr_scatter = 16;   // binned scatter: source offsets in flight per lane (8, 16 or 32)
   int pr_hot = 16384;    // binned PageRank: sources below this id are read from an LDS copy
                          // in phase 2 instead of being scattered (0: none, at most 16384)
+  int wcc_order = 1;     // undirected / symmetric graphs: WCC pulls on the ordered ids (0: edges)
   int bfs_order = 2;     // symmetric graphs: BFS on CSR rows ordered hubs first (1), on the
                          // degree-ordered ids with rows hubs first (2), or as built (0)
   int pr_persist = 0;    // binned PageRank with hot sources: phase-2 workgroups loop over
@@ -175,6 +176,7 @@ struct eg_graph {
   uint32_t *h_ctr = nullptr;        // pinned mirror
   int64_t *wcc_label = nullptr;
   uint32_t *wcc_lab32 = nullptr;
+  uint32_t *wcc_tmp = nullptr;  // ordered-id labels and component minima (2n)
 
   // Scratch buffers shared by the builders (grown on demand).
   void *scratch = nullptr;

@@ -32,7 +32,11 @@
 //                   every row hubs first (2, the default; 1 without the
 //                   ordered copy), on the layout's ids with rows hubs first
 //                   (1), or on the rows as built (0); levels are the same,
-//                   parents may differ.
+//                   parents may differ;
+//  * wcc_order   -- undirected / symmetric graphs: WCC pulls over the
+//                   degree-ordered ids, rows hubs first (1, the default), or
+//                   relaxes the edges in the layout's ids (0); labels are the
+//                   same.
 // eg_autotune times every variant on the graph itself through the public
 // entry points (device time from HIP events, best of a few repetitions) and
 // keeps the fastest.
@@ -59,6 +63,7 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "pr_hot")) return &g->tune.pr_hot;
   if (!strcmp(name, "pr_persist")) return &g->tune.pr_persist;
   if (!strcmp(name, "bfs_order")) return &g->tune.bfs_order;
+  if (!strcmp(name, "wcc_order")) return &g->tune.wcc_order;
   return nullptr;
 }
 
@@ -116,6 +121,8 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
     if (value < 0 || value > 2) return eg_fail(EG_EINVAL, "bfs_order takes 0, 1 or 2");
   } else if (!strcmp(param, "pr_persist")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "pr_persist takes 0 or 1");
+  } else if (!strcmp(param, "wcc_order")) {
+    if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "wcc_order takes 0 or 1");
   } else if (!strcmp(param, "pr_binned")) {
     if (value < -1 || value > 1) return eg_fail(EG_EINVAL, "pr_binned takes -1, 0 or 1");
   } else if (!strcmp(param, "graphs")) {

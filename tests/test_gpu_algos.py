@@ -279,6 +279,26 @@ def test_wcc_adjacency(undirected, scale):
     assert np.array_equal(lab, ref)
 
 
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_wcc_ordered_pull_equals_edges(symmetric):
+    """Undirected graphs: pull rounds on the degree-ordered ids (wcc_order 1)
+    and edge relaxation in the layout's ids (0) give the oracle's labels, the
+    minimum caller id of each component, with many small components beside
+    the one holding the top hub (rows that never see label 0) and hub rows
+    longer than one lane's scan."""
+    n = 1 << 14
+    s0, d0 = rmat(14, ef=2, seed=9)
+    s, d, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=True)
+    ref, _ = O.wcc_push(n, ptr, col)
+    assert np.diff(ptr).max() > 64 and np.unique(ref).size > 100
+    with eg.Graph(n, 2, undirected=True, symmetric=symmetric) as g:
+        g.load_edges(s0, d0)
+        for order in (1, 0, 1):
+            g.tune_set("wcc_order", order)
+            lab, _ = g.wcc()
+            assert np.array_equal(lab, ref), order
+
+
 @pytest.mark.parametrize("mode", [0, 1, 6, 7, 8, 9])
 def test_wcc_grid_and_edges(mode):
     n = 1 << 12
