@@ -238,7 +238,10 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * row's neighbours by descending degree, results written in the caller's ids
  * (the default; 1 when the ordered copy is disabled); 1 = the layout's ids,
  * rows by descending neighbour degree; 0 = rows as built; the copies are built
- * on first use; levels are the same, parents may differ); "grid_band" (grid source
+ * on first use; levels are the same, parents may differ); "wcc_order" (undirected
+ * / symmetric graphs: 1 = WCC rounds pull over the degree-ordered ids, rows
+ * hubs first, the default; 0 = edge relaxation in the layout's ids; labels are
+ * the same); "grid_band" (grid source
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
  * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
  * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant
