@@ -8,6 +8,8 @@ compensated sums and are compared with the FP64 oracle:
   * SpMV: |y - y64| <= 1e-5 * (|A| |x|) elementwise;
   * PPR: L1 error <= 1e-6 per query vector (each sums to <= 1).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -460,12 +462,17 @@ def test_autotune_reports_and_keeps_fastest():
         g.load_edges(s0, d0)
         rep = g.autotune()
         params = {r[0] for r in rep}
-        assert params == {"pull_batch", "pull_hot", "pull_persist", "pr_binned", "spmm_depth",
-                          "ms_batch"}
+        # the hot-set and persistent pull shapes exist only on the degree-ordered copy
+        ordered = os.environ.get("EG_PULL_ORDER") != "none"
+        expect = {"pull_batch", "pr_binned", "spmm_depth", "ms_batch"}
+        if ordered:
+            expect |= {"pull_hot", "pull_persist"}
+        assert params == expect
         assert sorted(r[1] for r in rep if r[0] == "pr_binned") == [0, 1]
-        hot = sorted(r[1] for r in rep if r[0] == "pull_hot")
-        assert hot == [0, n // 64, n // 16, n // 4]
-        for p in ("pull_batch", "spmm_depth", "ms_batch", "pull_hot", "pull_persist", "pr_binned"):
+        if ordered:
+            hot = sorted(r[1] for r in rep if r[0] == "pull_hot")
+            assert hot == [0, n // 64, n // 16, n // 4]
+        for p in sorted(expect):
             rows = [r for r in rep if r[0] == p]
             assert all(r[2] > 0 for r in rows)
             if p in ("pull_batch", "spmm_depth", "ms_batch"):
