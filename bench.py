@@ -204,7 +204,7 @@ def main():
     extra = {}
     # SpMV over the same graph (unweighted, w = 1): y = A^T x, signed x
     x = np.random.default_rng(5).uniform(-1, 1, n).astype(np.float32)
-    g.spmv(x)
+    y_gpu = g.spmv(x)
     g.prof_reset()
     g.prof_enable(True)
     for _ in range(5):
@@ -217,7 +217,6 @@ def main():
                             "bytes": b_spmv1, "weights": "unit"}
     log(f"spmv: {sp_ms * 1e3:.1f} us = {b_spmv1 / (sp_ms * 1e-3) / 1e9:.1f} GB/s of "
         f"4m + 4(n+1) + 8n")
-    del x
     if not args.skip_batch:
         broots = eg.sample_roots(deg, args.batch_bfs, seed=2)
         g.bfs_batch(broots, download=False)  # state of this batch width allocated untimed
@@ -304,7 +303,24 @@ def main():
                "sample": f"{cpu_iters} FP64 pull-PageRank iterations of the oracle "
                          f"(OpenMP) over the same R-MAT {scale} graph, {ct:.1f} s"}
         log(f"cpu oracle: {cpu_gbs:.2f} GB/s on {threads} threads ({ct:.1f} s)")
-        del ptr, col
+        # error of the device results against the FP64 oracle on this graph:
+        # the timed PageRank (same iteration count, r0 = 1/n) and the SpMV
+        r_gpu, _, _ = g.pagerank(0.85, iters)
+        r_ref, _, _ = O.pagerank_pull(n, ptr, col, odeg, 0.85, iters)
+        d = np.abs(r_gpu.astype(np.float64) - r_ref)
+        y_ref = O.spmv_pull(n, ptr, col, None, x)
+        y_abs = O.spmv_pull(n, ptr, col, None, np.abs(x))
+        e = np.abs(y_gpu.astype(np.float64) - y_ref)
+        extra["error_vs_fp64"] = {
+            "pagerank": {"l1": float(d.sum()), "max_abs": float(d.max()),
+                         "iterations": iters},
+            "spmv": {"l1": float(e.sum()), "max_abs": float(e.max()),
+                     "max_rel_to_abs_sum": float((e / np.maximum(y_abs, 1e-30)).max())},
+        }
+        log(f"error vs FP64 oracle: pagerank L1 {d.sum():.3e} max {d.max():.3e}; "
+            f"spmv L1 {e.sum():.3e} max {e.max():.3e}")
+        del ptr, col, r_gpu, r_ref, d, y_ref, y_abs, e
+    del x, y_gpu
 
     out = {
         "metric": METRIC,
