@@ -263,6 +263,10 @@ int eg_bfs_prepare(eg_graph *g);
 // every row's entries ascending (hubs first), built on first use; *rows =
 // null where the ordered copy is disabled.
 int eg_hubs_first_rows(eg_graph *g, const uint32_t **rows);
+// Takes back the last eg_ppr1_step (queued or finished): a step reads the
+// iterate in one buffer and writes the other, so the iterate before it is
+// intact; work queued after this call sees the earlier iterate.
+int eg_ppr1_undo(eg_graph *g, Ppr1 *st);
 int eg_ppr1_output(eg_graph *g, Ppr1 *st, float *out);
 // eg_bfs without the ownership check (the serving engine's plain path).
 int eg_bfs_host(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,

@@ -826,6 +826,14 @@ int eg_ppr1_step(eg_graph *g, Ppr1 *st) {
   return EG_OK;
 }
 
+int eg_ppr1_undo(eg_graph *, Ppr1 *st) {
+  if (st->it == 0) return eg_fail(EG_ESTATE, "no PPR iteration to take back");
+  std::swap(st->rank, st->rank_next);
+  std::swap(st->contrib, st->contrib_next);
+  st->it--;
+  return EG_OK;
+}
+
 int eg_ppr1_output(eg_graph *g, Ppr1 *st, float *out) {
   if (out) {
     const float *src = st->rank;

@@ -74,6 +74,7 @@ struct eg_server {
   // single-vector step (moving averages; 0 = not measured yet).
   double spmm_ms[3] = {0.0, 0.0, 0.0};
   double solo_ms = 0.0;
+  double ppr_at = 0.0;  // host clock (ms) at the launch of the SpMM step in flight
   std::vector<LaneGroup *> groups;
   uint32_t max_groups = 1;
   bool bfs_nomem = false;  // no lane group could be allocated
@@ -81,12 +82,17 @@ struct eg_server {
   bool ppr_alloc = false, ppr_nomem = false;
   Query *col[64] = {};
   uint64_t cols = 0;
-  // single-vector PPR query stepped on the PPR stream (light load)
+  // single-vector PPR query stepped on the PPR stream (light load).  Up to
+  // two iterations are in flight, indexed by the parity of the iteration
+  // they compute, so the device keeps a step queued while the engine thread
+  // waits on a BFS level.
   Ppr1 solo;
   Query *solo_q = nullptr;
-  bool solo_busy = false;  // an iteration is in flight
-  hipEvent_t solo_ev = nullptr, solo_t0 = nullptr;
-  double *h_solo = nullptr;  // pinned L1 change of the solo iteration
+  int solo_fly = 0;                        // iterations in flight (0..2)
+  hipEvent_t solo_ev[2] = {}, solo_t0[2] = {};
+  double solo_at[2] = {0.0, 0.0};          // expected host-clock start of each (ms)
+  double solo_last = -1.0;                 // last known L1 change (< 0: none yet)
+  double *h_solo = nullptr;  // pinned, L1 change per iteration parity
   uint32_t *stage = nullptr;    // n words, BFS stream
   float *stage_ppr = nullptr;   // n floats, PPR stream
   uint64_t device_bytes = 0;
@@ -275,6 +281,7 @@ int ppr_launch(eg_server *S) {
   EG_TRY(eg_ppr_step(S->g, S->bv, &S->ppr, S->o.ppr_alpha, true));
   EG_HIP(hipEventRecord(S->ppr_ev, S->s_ppr));
   S->ppr_busy = true;
+  S->ppr_at = now_ms();
   std::lock_guard<std::mutex> lk(S->mu);
   S->stats.ppr_steps++;
   return EG_OK;
@@ -405,6 +412,7 @@ bool bfs_active(eg_server *S) {
 }
 
 int plain_round(eg_server *S);
+int poll(eg_server *S, hipEvent_t ev, bool *done);
 
 size_t waiting(eg_server *S, const std::deque<Query *> &dq) {
   std::lock_guard<std::mutex> lk(S->mu);
@@ -431,33 +439,70 @@ int solo_start(eg_server *S) {
     return EG_OK;
   }
   S->solo_q = qr;
+  S->solo_fly = 0;
+  S->solo_last = -1.0;
   return EG_OK;
 }
 
+// Launches the solo query's next iteration; its L1 change lands in the
+// pinned slot of its parity.
 int solo_launch(eg_server *S) {
-  EG_HIP(hipEventRecord(S->solo_t0, S->s_ppr));
+  const int p = (S->solo.it + 1) & 1;
+  S->solo.h_delta = S->h_solo + p;
+  EG_HIP(hipEventRecord(S->solo_t0[p], S->s_ppr));
   EG_TRY(eg_ppr1_step(S->g, &S->solo));
-  EG_HIP(hipEventRecord(S->solo_ev, S->s_ppr));
-  S->solo_busy = true;
+  EG_HIP(hipEventRecord(S->solo_ev[p], S->s_ppr));
+  const double t = now_ms();
+  S->solo_at[p] = S->solo_fly ? std::max(t, S->solo_at[p ^ 1] + S->solo_ms) : t;
+  S->solo_fly++;
   std::lock_guard<std::mutex> lk(S->mu);
   S->stats.ppr_steps++;
   return EG_OK;
 }
 
-// After a solo iteration: retire the query or launch the next iteration.
-int solo_after(eg_server *S) {
-  Query *qr = S->solo_q;
-  qr->level = (uint32_t)S->solo.it;
-  qr->res.delta = *S->h_solo;
-  if (qr->res.delta <= S->o.ppr_tol || (int32_t)qr->level >= S->o.ppr_max_iter) {
-    EG_TRY(eg_ppr1_output(S->g, &S->solo, static_cast<float *>(qr->out)));
-    qr->res.levels = qr->level;
-    qr->res.reached = 0;
-    S->solo_q = nullptr;
-    complete(S, qr, EG_OK, nullptr);
-    return EG_OK;
+// Keeps the solo pipeline full: one iteration always, a second while the
+// last known L1 change is above 2 tol (the change shrinks by about 1 - alpha
+// per iteration, so the iteration in flight will not end the query).
+int solo_fill(eg_server *S) {
+  while (S->solo_q && S->solo_fly < 2 && S->solo.it < S->solo.max_iter) {
+    if (S->solo_fly && S->solo_last >= 0.0 && S->solo_last <= 2.0 * S->o.ppr_tol) break;
+    EG_TRY(solo_launch(S));
   }
-  return solo_launch(S);
+  return EG_OK;
+}
+
+// Takes the solo iterations that have ended, oldest first.  The query ends
+// at the first whose L1 change is <= tol (or at the iteration cap); an
+// iteration launched after that one is taken back, so the output is the
+// ending iteration's iterate, as on the one-step-at-a-time path.
+int solo_progress(eg_server *S) {
+  while (S->solo_fly) {
+    const int j = S->solo.it - S->solo_fly + 1, p = j & 1;  // oldest in flight
+    bool done = false;
+    EG_TRY(poll(S, S->solo_ev[p], &done));
+    if (!done) return EG_OK;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, S->solo_t0[p], S->solo_ev[p]) == hipSuccess) ewma(S->solo_ms, ms);
+    S->solo_fly--;
+    const double delta = S->h_solo[p];
+    S->solo_last = delta;
+    if (delta <= S->o.ppr_tol || j >= S->o.ppr_max_iter) {
+      while (S->solo_fly) {
+        EG_TRY(eg_ppr1_undo(S->g, &S->solo));
+        S->solo_fly--;
+      }
+      Query *qr = S->solo_q;
+      qr->level = (uint32_t)j;
+      qr->res.delta = delta;
+      EG_TRY(eg_ppr1_output(S->g, &S->solo, static_cast<float *>(qr->out)));
+      qr->res.levels = qr->level;
+      qr->res.reached = 0;
+      S->solo_q = nullptr;
+      complete(S, qr, EG_OK, nullptr);
+      return EG_OK;
+    }
+  }
+  return EG_OK;
 }
 
 // Light-load BFS: the oldest waiting query on the direction-optimising
@@ -506,6 +551,48 @@ size_t ppr_light_limit(const eg_server *S, size_t light, size_t waiting_now) {
   return lim < base ? base : (lim > 63 ? 63 : lim);
 }
 
+// Only PPR work is in flight: wait until a step ends or a BFS query arrives.
+// The engine sleeps on the queue until shortly before the step's expected end
+// (its moving-average duration from the host-side launch time), then polls
+// the step events, so a step's end is seen within microseconds rather than a
+// timer quantum (a sleep of 200 us cost about 10% of a 1.4 ms iteration).
+int wait_ppr(eg_server *S) {
+  double end = 0.0;  // earliest expected end (host ms); 0 = unknown
+  const int p0 = (S->solo.it - S->solo_fly + 1) & 1;  // oldest solo iteration in flight
+  if (S->solo_fly && S->solo_ms > 0.0) end = S->solo_at[p0] + S->solo_ms;
+  if (S->ppr_busy && S->spmm_ms[S->ppr_wclass] > 0.0) {
+    const double e = S->ppr_at + S->spmm_ms[S->ppr_wclass];
+    end = end > 0.0 ? std::min(end, e) : e;
+  }
+  const double margin = 0.15;  // ms: timer slack of a sleep plus jitter of the step
+  const double t = now_ms();
+  if (end > 0.0 && end - margin > t) {
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (S->cv_work.wait_for(lk, std::chrono::duration<double, std::milli>(end - margin - t),
+                            [&] { return !S->pend_bfs.empty(); }))
+      return EG_OK;
+  }
+  // poll for at most 20 ms before returning to the round loop
+  const double stop = now_ms() + 20.0;
+  for (;;) {
+    bool done = false;
+    if (S->solo_fly) {
+      EG_TRY(poll(S, S->solo_ev[p0], &done));
+      if (done) return EG_OK;
+    }
+    if (S->ppr_busy) {
+      EG_TRY(poll(S, S->ppr_ev, &done));
+      if (done) return EG_OK;
+    }
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (!S->pend_bfs.empty()) return EG_OK;
+    }
+    if (now_ms() > stop) return EG_OK;
+    std::this_thread::yield();
+  }
+}
+
 int batched_round(eg_server *S) {
   const size_t light = S->o.idle_plain > 0 ? (size_t)S->o.idle_plain : 0;
   // ---- PPR side (PPR stream) ----
@@ -520,23 +607,13 @@ int batched_round(eg_server *S) {
       EG_TRY(ppr_after(S));
     }
   }
-  if (S->solo_busy) {
-    EG_TRY(poll(S, S->solo_ev, &done));
-    if (done) {
-      S->solo_busy = false;
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, S->solo_t0, S->solo_ev) == hipSuccess) ewma(S->solo_ms, ms);
-      EG_TRY(solo_after(S));
-    }
-  }
+  EG_TRY(solo_progress(S));
   const size_t light_ppr = ppr_light_limit(S, light, waiting(S, S->pend_ppr));
   if (!S->solo_q && !S->cols && light_ppr) {
     const size_t w = waiting(S, S->pend_ppr);
-    if (w > 0 && w <= light_ppr) {
-      EG_TRY(solo_start(S));
-      if (S->solo_q) EG_TRY(solo_launch(S));
-    }
+    if (w > 0 && w <= light_ppr) EG_TRY(solo_start(S));
   }
+  EG_TRY(solo_fill(S));
   if (!S->ppr_busy) {
     // A short queue waits for the single-vector path; a longer one (or one
     // that arrives while columns are busy) joins the SpMM block.
@@ -560,12 +637,7 @@ int batched_round(eg_server *S) {
       ran = true;
     }
   }
-  if (!ran && (S->ppr_busy || S->solo_busy)) {
-    // Only PPR work is in flight: sleep until it ends or a BFS query arrives.
-    std::unique_lock<std::mutex> lk(S->mu);
-    S->cv_work.wait_for(lk, std::chrono::microseconds(200),
-                        [&] { return !S->pend_bfs.empty(); });
-  }
+  if (!ran && (S->ppr_busy || S->solo_fly)) EG_TRY(wait_ppr(S));
   return EG_OK;
 }
 
@@ -714,8 +786,10 @@ void destroy(eg_server *S) {
   S->all.clear();
   if (S->ppr_ev) (void)hipEventDestroy(S->ppr_ev);
   if (S->ppr_t0) (void)hipEventDestroy(S->ppr_t0);
-  if (S->solo_ev) (void)hipEventDestroy(S->solo_ev);
-  if (S->solo_t0) (void)hipEventDestroy(S->solo_t0);
+  for (int p = 0; p < 2; p++) {
+    if (S->solo_ev[p]) (void)hipEventDestroy(S->solo_ev[p]);
+    if (S->solo_t0[p]) (void)hipEventDestroy(S->solo_t0[p]);
+  }
   if (S->h_solo) (void)hipHostFree(S->h_solo);
   if (S->s_bfs) (void)hipStreamDestroy(S->s_bfs);
   if (S->s_ppr) (void)hipStreamDestroy(S->s_ppr);
@@ -773,10 +847,12 @@ extern "C" int eg_serve_start(eg_graph *g, const eg_serve_opts *opts, eg_server 
   // timed events: the engine measures its SpMM and single-vector steps
   if (rc == EG_OK && (hipEventCreate(&S->ppr_ev) != hipSuccess ||
                       hipEventCreate(&S->ppr_t0) != hipSuccess ||
-                      hipEventCreate(&S->solo_ev) != hipSuccess ||
-                      hipEventCreate(&S->solo_t0) != hipSuccess))
+                      hipEventCreate(&S->solo_ev[0]) != hipSuccess ||
+                      hipEventCreate(&S->solo_t0[0]) != hipSuccess ||
+                      hipEventCreate(&S->solo_ev[1]) != hipSuccess ||
+                      hipEventCreate(&S->solo_t0[1]) != hipSuccess))
     rc = eg_fail(EG_EDEVICE, "event creation failed");
-  if (rc == EG_OK && hipHostMalloc(reinterpret_cast<void **>(&S->h_solo), sizeof(double),
+  if (rc == EG_OK && hipHostMalloc(reinterpret_cast<void **>(&S->h_solo), 2 * sizeof(double),
                                    hipHostMallocDefault) != hipSuccess)
     rc = eg_fail(EG_ENOMEM, "pinned solo delta");
   if (rc == EG_OK && hipStreamSynchronize(g->stream) != hipSuccess)

@@ -252,3 +252,33 @@ def test_replay_report(plain):
     assert 0 < r["p50_ms"] <= r["p99_ms"] <= r["max_ms"]
     assert r["qps"] > 0 and r["wall_s"] > 0
     assert r["stats"]["completed"] == 300
+
+
+@pytest.mark.parametrize("binned", [0, 1])
+@pytest.mark.parametrize("undirected", [True, False])
+def test_solo_pipeline_equals_single_vector(binned, undirected):
+    """A lone PPR query runs on the engine's single-vector path, which keeps a
+    second iteration queued while the first is in flight and takes it back
+    when the first ends the query.  Results must equal eg_ppr (one iteration
+    at a time) bit for bit, with the same iteration count and last L1 change:
+    tol 10 ends every query at iteration 1 with iteration 2 queued, a tiny
+    tol runs to the iteration cap."""
+    n, s0, d0, m, ptr, col, cp, row = build(11, 52, undirected)
+    seeds = ([3], [9, 9, 17, 250], list(range(40, 48)))
+    with eg.Graph(n, 2, undirected=undirected) as g:
+        g.load_edges(s0, d0)
+        g.tune_set("pr_binned", binned)
+        for tol, cap in ((10.0, 200), (0.05, 200), (1e-3, 200), (1e-6, 200), (1e-300, 5)):
+            ref = [g.ppr(sd, 0.15, tol=tol, max_iter=cap) for sd in seeds]
+            with g.serve(ppr_tol=tol, ppr_max_iter=cap) as S:
+                got = [S.wait(S.submit_ppr(sd)) for sd in seeds]
+                st = S.stats()
+            assert st["served_plain"] == len(seeds) and st["ppr_columns"] == 0
+            for q, ((info, out), (rout, rit, rdelta)) in enumerate(zip(got, ref)):
+                assert info["levels"] == rit, (tol, q)
+                assert info["delta"] == rdelta, (tol, q)
+                assert np.array_equal(out, rout), (tol, q)
+            if tol == 10.0:
+                assert all(info["levels"] == 1 for info, _ in got)
+            if cap == 5:
+                assert all(info["levels"] == 5 for info, _ in got)
