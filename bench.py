@@ -141,7 +141,7 @@ def main():
     for kv in args.tune:
         p, v = kv.split("=")
         g.tune_set(p, int(v))
-    tuned = {p: g.tune_get(p) for p in ("pr_binned", "pr_scatter", "pull_batch", "pull_hot",
+    tuned = {p: g.tune_get(p) for p in ("pr_binned", "pr_xcd", "pr_scatter", "pull_batch", "pull_hot",
                                         "pull_persist", "spmm_depth", "ms_batch")}
 
     # ---- PageRank: warmup + timed steps ----

@@ -125,6 +125,7 @@ struct PbDev {
   uint32_t nbins, nitems, nlong;
   uint32_t nhot;           // sources below this id are hot
   uint32_t persistent;     // phase 2: workgroups loop over items (else one each)
+  uint32_t xcd_tiles;      // phase 1: XCD-contiguous tile order (else launch order)
   const float *w;          // bin-major weights (SpMV of a weighted graph), or null
 };
 
@@ -275,7 +276,16 @@ __global__ __launch_bounds__(kBlock) void k_pb_scatter(PbDev d, const float *__r
   __shared__ float s_win[kChunkSrc];
   __shared__ uint2 s_tab[kTileBins + 1];
   __shared__ u16x4 s_stage[kWaves][64];
-  const PbTile t = d.tiles[blockIdx.x];
+  // Workgroups are dealt to the eight XCDs round-robin (workgroup b runs on
+  // XCD b % 8).  With xcd_tiles, XCD x takes tiles [x q + min(x, r), ...) in
+  // order (T = 8 q + r tiles), so consecutive tiles of a chunk run on one XCD
+  // at about the same time and read the chunk's window from its L2.
+  uint32_t ti = blockIdx.x;
+  if (d.xcd_tiles) {
+    const uint32_t q = gridDim.x >> 3, r = gridDim.x & 7, x = ti & 7;
+    ti = x * q + min(x, r) + (ti >> 3);
+  }
+  const PbTile t = d.tiles[ti];
   const uint2 cs = d.chunk_src[t.c];
   const uint32_t tid = threadIdx.x;
   // the window and the table: every load issued before the first LDS store
@@ -1157,6 +1167,7 @@ static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int
   d.nitems = p->nitems;
   d.nlong = p->nlong;
   d.nhot = p->nhot;
+  d.xcd_tiles = g->tune.pr_xcd ? 1u : 0u;
   d.w = spmv ? p->w : nullptr;
   const bool wt = d.w != nullptr;
   {

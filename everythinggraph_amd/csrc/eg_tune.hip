@@ -28,6 +28,10 @@
 //                   before any value is written, 8, 16 or 32;
 //  * pr_hot      -- binned PageRank: sources below this id are read from an
 //                   LDS copy in phase 2 instead of being scattered (0..16384);
+//  * pr_xcd      -- binned scatter: 1 = XCD x takes the x-th eighth of the
+//                   tiles in order, so a chunk's tiles share one XCD's L2 copy
+//                   of the chunk's contributions; 0 = launch order (the
+//                   default: the XCDs then write neighbouring segments);
 //  * bfs_order   -- symmetric graphs: BFS runs on the degree-ordered ids with
 //                   every row hubs first (2, the default; 1 without the
 //                   ordered copy), on the layout's ids with rows hubs first
@@ -62,6 +66,7 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "pr_scatter")) return &g->tune.pr_scatter;
   if (!strcmp(name, "pr_hot")) return &g->tune.pr_hot;
   if (!strcmp(name, "pr_persist")) return &g->tune.pr_persist;
+  if (!strcmp(name, "pr_xcd")) return &g->tune.pr_xcd;
   if (!strcmp(name, "bfs_order")) return &g->tune.bfs_order;
   if (!strcmp(name, "wcc_order")) return &g->tune.wcc_order;
   return nullptr;
@@ -121,6 +126,8 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
     if (value < 0 || value > 2) return eg_fail(EG_EINVAL, "bfs_order takes 0, 1 or 2");
   } else if (!strcmp(param, "pr_persist")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "pr_persist takes 0 or 1");
+  } else if (!strcmp(param, "pr_xcd")) {
+    if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "pr_xcd takes 0 or 1");
   } else if (!strcmp(param, "wcc_order")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "wcc_order takes 0 or 1");
   } else if (!strcmp(param, "pr_binned")) {
@@ -227,7 +234,18 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
           g->tune.pull_persist = r.value;
         }
     }
-    // the binned two-phase kernels against the best pull shape
+    // the binned scatter's tile order, then the binned two-phase kernels
+    // against the best pull shape
+    g->tune.pr_binned = 1;
+    EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // warm (binned plan)
+    double xms[2];
+    for (int x = 0; x <= 1; x++) {
+      g->tune.pr_xcd = x;
+      EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); },
+                       &xms[x]));
+      res.push_back({"pr_xcd", x, xms[x] / o.iterations});
+    }
+    g->tune.pr_xcd = xms[1] < xms[0] ? 1 : 0;
     for (int pb = 0; pb <= 1; pb++) {
       g->tune.pr_binned = pb;
       double ms;

@@ -464,11 +464,12 @@ def test_autotune_reports_and_keeps_fastest():
         params = {r[0] for r in rep}
         # the hot-set and persistent pull shapes exist only on the degree-ordered copy
         ordered = os.environ.get("EG_PULL_ORDER") != "none"
-        expect = {"pull_batch", "pr_binned", "spmm_depth", "ms_batch"}
+        expect = {"pull_batch", "pr_binned", "pr_xcd", "spmm_depth", "ms_batch"}
         if ordered:
             expect |= {"pull_hot", "pull_persist"}
         assert params == expect
         assert sorted(r[1] for r in rep if r[0] == "pr_binned") == [0, 1]
+        assert sorted(r[1] for r in rep if r[0] == "pr_xcd") == [0, 1]
         if ordered:
             hot = sorted(r[1] for r in rep if r[0] == "pull_hot")
             assert hot == [0, n // 64, n // 16, n // 4]

@@ -150,6 +150,8 @@ for und in (False, True):
         out['cold%d' % und] = g.pagerank(0.85, 15)[0]
         g.tune_set("pr_hot", 4096)   # half hot, half scattered
         out['half%d' % und] = g.pagerank(0.85, 15)[0]
+        g.tune_set("pr_xcd", 1)      # scatter tiles dealt to XCDs in contiguous ranges
+        out['xcd%d' % und] = g.pagerank(0.85, 15)[0]
 np.savez(sys.argv[2], **out)
 """
 
@@ -158,7 +160,7 @@ def test_binned_independent_of_cut_and_order(tmp_path):
     """The same ranks, bit for bit, with the default bins and chunks, with
     tiny ones (40 hot sources, long rows in parts, tiles cut by length and by
     bins), with all, half or none of the sources hot, and with or without the
-    degree-ordered copy."""
+    degree-ordered copy, and in either scatter tile order."""
     script = tmp_path / "run.py"
     script.write_text(_GEOMETRY_SCRIPT)
     res = {}
@@ -177,6 +179,7 @@ def test_binned_independent_of_cut_and_order(tmp_path):
         for und in (0, 1):  # no hot sources, or half of them: the same bits
             assert np.array_equal(r[f"cold{und}"], first[f"pr{und}"]), (key, und)
             assert np.array_equal(r[f"half{und}"], first[f"pr{und}"]), (key, und)
+            assert np.array_equal(r[f"xcd{und}"], first[f"pr{und}"]), (key, und)
     for und in (0, 1):
         small = res[("small", "degree")][f"geo{und}"]
         bins, chunks, tiles, items, long_rows = (int(v) for v in small)
