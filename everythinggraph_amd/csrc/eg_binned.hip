@@ -125,7 +125,7 @@ struct PbDev {
   uint32_t nbins, nitems, nlong;
   uint32_t nhot;           // sources below this id are hot
   uint32_t persistent;     // phase 2: workgroups loop over items (else one each)
-  uint32_t xcd_tiles;      // phase 1: XCD-contiguous tile order (else launch order)
+  uint32_t xcd_tiles;      // phase 1: interleaved walk of the tile list (else launch order)
   const float *w;          // bin-major weights (SpMV of a weighted graph), or null
 };
 
@@ -277,9 +277,13 @@ __global__ __launch_bounds__(kBlock) void k_pb_scatter(PbDev d, const float *__r
   __shared__ uint2 s_tab[kTileBins + 1];
   __shared__ u16x4 s_stage[kWaves][64];
   // Workgroups are dealt to the eight XCDs round-robin (workgroup b runs on
-  // XCD b % 8).  With xcd_tiles, XCD x takes tiles [x q + min(x, r), ...) in
-  // order (T = 8 q + r tiles), so consecutive tiles of a chunk run on one XCD
-  // at about the same time and read the chunk's window from its L2.
+  // XCD b % 8), and the build stores the tile list so that launch order gives
+  // XCD x the x-th eighth of the chunk-ordered tiles.  With xcd_tiles,
+  // workgroup b takes list entry x q + min(x, r) + b / 8 instead (T = 8 q + r
+  // tiles): XCD x then walks one short run of tiles in every eighth of the
+  // chunk order by turns, so each XCD's work samples the whole source range
+  // (64 write regions in flight instead of 8), while the tiles of one chunk
+  // still share one XCD and its L2 copy of the chunk's window.
   uint32_t ti = blockIdx.x;
   if (d.xcd_tiles) {
     const uint32_t q = gridDim.x >> 3, r = gridDim.x & 7, x = ti & 7;

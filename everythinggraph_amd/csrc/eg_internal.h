@@ -124,8 +124,8 @@ struct EgTune {
                          // degree-ordered ids with rows hubs first (2), or as built (0)
   int pr_persist = 0;    // binned PageRank with hot sources: phase-2 workgroups loop over
                          // items (one per CU) instead of one workgroup per item
-  int pr_xcd = 0;        // binned scatter: each XCD takes a contiguous range of tiles (1)
-                         // or tiles go to XCDs round-robin in launch order (0)
+  int pr_xcd = 0;        // binned scatter: every XCD walks a contiguous eighth of the
+                         // chunk-ordered tiles (0) or runs from every eighth by turns (1)
 };
 
 struct eg_graph {

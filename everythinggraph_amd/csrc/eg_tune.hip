@@ -28,10 +28,11 @@
 //                   before any value is written, 8, 16 or 32;
 //  * pr_hot      -- binned PageRank: sources below this id are read from an
 //                   LDS copy in phase 2 instead of being scattered (0..16384);
-//  * pr_xcd      -- binned scatter: 1 = XCD x takes the x-th eighth of the
-//                   tiles in order, so a chunk's tiles share one XCD's L2 copy
-//                   of the chunk's contributions; 0 = launch order (the
-//                   default: the XCDs then write neighbouring segments);
+//  * pr_xcd      -- binned scatter tile walk: 0 = XCD x takes the x-th
+//                   eighth of the chunk-ordered tiles (the default); 1 = XCD x
+//                   takes a short run of tiles from every eighth by turns
+//                   (balances XCDs whose eighths differ in cost); the tiles
+//                   of one chunk share an XCD either way;
 //  * bfs_order   -- symmetric graphs: BFS runs on the degree-ordered ids with
 //                   every row hubs first (2, the default; 1 without the
 //                   ordered copy), on the layout's ids with rows hubs first

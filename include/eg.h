@@ -233,9 +233,10 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * source offsets in flight per lane, 8, 16 or 32); "pr_hot" (binned PageRank:
  * sources below this id, at most 16384, are read from an LDS copy in phase 2
  * instead of being scattered; 0 = none, default 16384); "pr_persist" (1:
- * phase-2 workgroups loop over work items, one per CU; default 0); "pr_xcd" (1:
- * binned scatter tiles dealt to the eight XCDs in contiguous ranges, so the
- * tiles of one source chunk share an L2; 0: launch order; default 0); "bfs_order"
+ * phase-2 workgroups loop over work items, one per CU; default 0); "pr_xcd" (binned
+ * scatter: 0 = each of the eight XCDs walks a contiguous eighth of the
+ * chunk-ordered tiles, the default; 1 = each XCD takes short runs from every
+ * eighth by turns); "bfs_order"
  * (symmetric graphs: 2 = BFS runs on the degree-ordered vertex ids with every
  * row's neighbours by descending degree, results written in the caller's ids
  * (the default; 1 when the ordered copy is disabled); 1 = the layout's ids,
