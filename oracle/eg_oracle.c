@@ -677,8 +677,24 @@ int ego_check_sssp_parents(uint32_t n, const uint64_t *out_ptr, const uint32_t *
 int ego_check_bfs_parents(uint32_t n, const uint64_t *out_ptr,
                           const uint32_t *out_col, uint32_t root,
                           const uint32_t *dist, const uint32_t *parent) {
+  /* tree edges found in one pass over the rows: entry u -> v is v's tree
+   * edge when parent[v] == u (a scan of the parent's row per vertex would
+   * cost the hub's degree for every child of a hub) */
+  unsigned char *edge = (unsigned char *)calloc(n ? n : 1, 1);
+  if (!edge) return -1;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t u = 0; u < (int64_t)n; u++)
+    for (uint64_t k = out_ptr[u]; k < out_ptr[u + 1]; k++) {
+      const uint32_t v = out_col[k];
+      if (v < n && parent[v] == (uint32_t)u) {
+#pragma omp atomic write
+        edge[v] = 1;
+      }
+    }
   int bad = 0;
-  for (uint32_t v = 0; v < n; v++) {
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const uint32_t v = (uint32_t)i;
     if (v == root) {
       if (parent[v] != root || dist[v] != 1) bad++;
       continue;
@@ -688,17 +704,8 @@ int ego_check_bfs_parents(uint32_t n, const uint64_t *out_ptr,
       continue;
     }
     const uint32_t p = parent[v];
-    if (p >= n || dist[p] + 1 != dist[v]) {
-      bad++;
-      continue;
-    }
-    int found = 0;
-    for (uint64_t k = out_ptr[p]; k < out_ptr[p + 1]; k++)
-      if (out_col[k] == v) {
-        found = 1;
-        break;
-      }
-    if (!found) bad++;
+    if (p >= n || dist[p] + 1 != dist[v] || !edge[v]) bad++;
   }
+  free(edge);
   return bad;
 }

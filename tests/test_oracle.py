@@ -341,6 +341,17 @@ def test_parent_checker_catches_planted_errors():
     bad_dist = dist.copy()
     bad_dist[reached[0]] += 1  # level off by one
     assert O.check_bfs_parents(n, ptr, col, root, bad_dist, parent) > 0
+    # a parent one level up that is not an in-neighbour
+    for v in reached:
+        nb = set(col[ptr[v]:ptr[v + 1]].tolist())
+        up = [u for u in np.nonzero(dist == dist[v] - 1)[0] if u not in nb]
+        if up:
+            bad = parent.copy()
+            bad[v] = up[0]
+            assert O.check_bfs_parents(n, ptr, col, root, dist, bad) == 1
+            break
+    else:
+        raise AssertionError("no vertex with a non-adjacent vertex one level up")
 
 
 # ---------------------------------------------------------------------------
