@@ -479,8 +479,15 @@ def other_configs(eg, skip_c5=False, c5_requests=150) -> dict:
         n = 1 << 26
         g = eg.Graph(n, eg.MODE_ADJ_SORT)
         g.generate(eg.rmat_params(26, 16, seed=1))
+        t0 = time.perf_counter()
+        tune = g.autotune()  # the shapes both modes run with
+        log("RMAT-26 autotune: " + ", ".join(f"{p}={v} {ms:.3f} ms{'*' if c else ''}"
+                                             for p, v, ms, c in tune)
+            + f" ({time.perf_counter() - t0:.1f} s)")
         srv = {"n": n, "m": g.info.m, "device_gb": round(g.info.device_bytes / 1e9, 2),
-               "requests": c5_requests}
+               "requests": c5_requests,
+               "tuned": {p: g.tune_get(p) for p in ("pr_binned", "pr_xcd", "spmm_depth",
+                                                    "ms_batch")}}
         for qps in (10.0, 20.0):
             row = {}
             for mode in ("batched", "plain"):
