@@ -357,11 +357,11 @@ def main():
     out.update(extra)
     g.close()
     if not args.skip_configs:
-        out["configs"] = other_configs(eg, args.skip_c5, args.c5_requests)
+        out["configs"] = other_configs(eg, args.skip_c5, args.c5_requests, not args.skip_cpu)
     print(json.dumps(out), flush=True)
 
 
-def other_configs(eg, skip_c5=False, c5_requests=150) -> dict:
+def other_configs(eg, skip_c5=False, c5_requests=150, fp64=True) -> dict:
     """Device figures of the other configurations on fresh graphs: PageRank /
     SpMV on the 2-D grid (directed Erdos-Renyi, 2^24 vertices, 2^28 weighted
     edges, 16 x 16 cells), file ingest, BFS + WCC + SSSP on undirected R-MAT
@@ -404,6 +404,27 @@ def other_configs(eg, skip_c5=False, c5_requests=150) -> dict:
     log(f"grid ER 2^24/2^28 16x16 ({res['grid_er24']['kernels']}): build {build_ms:.1f} ms; "
         f"PR {pr_ms * 1e3:.1f} us/iter = {b_pr / (pr_ms * 1e-3) / 1e9:.1f} GB/s; "
         f"SpMV {sp_ms * 1e3:.1f} us = {b_spmv / (sp_ms * 1e-3) / 1e9:.1f} GB/s")
+    if fp64:
+        # error against the FP64 oracle over the same grid: 20 PageRank
+        # iterations (r0 = 1/n) and the SpMV of this x
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import eg_oracle_py as O
+
+        cp, gs, gd, gw = g.grid()
+        odeg, _ = g.degrees()
+        r, _, _ = g.pagerank(0.85, 20, layout="grid")
+        d = np.abs(r.astype(np.float64) - O.pagerank_grid(n, 16, cp, gs, gd, odeg, 0.85, 20))
+        y = g.spmv(x, layout="grid")
+        e = np.abs(y.astype(np.float64) - O.spmv_grid(n, 16, cp, gs, gd, gw, x))
+        yabs = O.spmv_grid(n, 16, cp, gs, gd, gw, np.abs(x))
+        res["grid_er24"]["error_vs_fp64"] = {
+            "pagerank": {"l1": float(d.sum()), "max_abs": float(d.max()), "iterations": 20},
+            "spmv": {"l1": float(e.sum()), "max_abs": float(e.max()),
+                     "max_rel_to_abs_sum": float((e / np.maximum(yabs, 1e-30)).max())},
+        }
+        log(f"grid error vs FP64 oracle: pagerank L1 {d.sum():.3e} max {d.max():.3e}; "
+            f"spmv max {e.max():.3e} ({(e / np.maximum(yabs, 1e-30)).max():.2e} of sum |a x|)")
+        del cp, gs, gd, gw, r, d, y, e, yabs
     g.close()
 
     # file ingest: the R-MAT 24 records (2.15 GB) read back from a file in
