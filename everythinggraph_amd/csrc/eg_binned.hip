@@ -45,6 +45,9 @@
 #include <algorithm>
 #include <vector>
 
+#include <cfloat>
+#include <cmath>
+
 #include "eg_internal.h"
 #include "eg_pr.h"
 
@@ -1138,11 +1141,23 @@ bool eg_pb_selected(const eg_graph *g) {
   return g->tune.pr_binned > 0 || (g->tune.pr_binned < 0 && g->m >= (1ull << 22));
 }
 
-// Fixed-point scale for row sums bounded by `bound`: the largest S with
-// bound * 2^S <= 2^62.
+// Fixed-point scale for row sums bounded by `bound` (finite; callers check):
+// the largest S with bound * 2^S <= 2^62.
 int eg_pb_scale(double bound) {
-  if (!(bound > 0.0)) bound = 1.0;
+  if (!(bound > 0.0) || !std::isfinite(bound)) bound = 1.0;
   return 62 - (int)ceil(log2(bound));
+}
+
+// The binned SpMV sums exactly only when every FP32 product w * x is finite
+// and the row-sum bound has a scale: a weight of inf or NaN (a 12-byte record
+// may hold one), or |w| |x| at or past FLT_MAX, leaves the sum to the pull
+// kernels, which propagate inf and NaN as IEEE arithmetic does.
+bool eg_pb_spmv_exact(const eg_graph *g, double xmax) {
+  const PbPlan *p = g->pb;
+  if (!p || !std::isfinite(p->wmax) || !std::isfinite(xmax)) return false;
+  const double prod = (double)p->wmax * xmax;
+  if (!(prod < (double)FLT_MAX)) return false;
+  return std::isfinite(prod * (double)p->max_rowlen * 1.001);
 }
 
 // Launches one pass of the binned kernels from x into y.  PageRank (spmv
@@ -1229,6 +1244,8 @@ int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, 
 int eg_pb_spmv(eg_graph *g, const float *x, float *y, double xmax, hipStream_t stream) {
   const PbPlan *p = g->pb;
   if (!p) return eg_fail(EG_ESTATE, "binned plan not built");
+  if (!eg_pb_spmv_exact(g, xmax))
+    return eg_fail(EG_EINVAL, "binned SpMV needs finite weights and products");
   const double bound = (double)p->wmax * (double)p->max_rowlen * xmax * 1.001;
   PrArgs a = {};
   return pb_launch(g, x, y, a, eg_pb_scale(bound), stream, true);

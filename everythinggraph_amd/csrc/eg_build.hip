@@ -185,14 +185,22 @@ static int build_edgelist(eg_graph *g) {
 int eg_build_layout(eg_graph *g) {
   const uint64_t m = g->m;
   const uint32_t n = g->n;
-  hipEvent_t e0, e1;
-  EG_HIP(hipEventCreate(&e0));
-  EG_HIP(hipEventCreate(&e1));
-  EG_HIP(hipEventRecord(e0, g->stream));
+  // Events and the id flag are released on every exit.
+  struct Guard {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    uint32_t *tmp = nullptr;
+    ~Guard() {
+      for (hipEvent_t ev : e)
+        if (ev) (void)hipEventDestroy(ev);
+      eg_dfree_t(&tmp);
+    }
+  } G;
+  EG_HIP(hipEventCreate(&G.e[0]));
+  EG_HIP(hipEventCreate(&G.e[1]));
+  EG_HIP(hipEventRecord(G.e[0], g->stream));
   int bad = 0;
-  uint32_t *tmp = nullptr;
-  EG_TRY(eg_dalloc_t(&tmp, 1025, "id flag"));
-  int rc = eg_check_ids(g->stream, g->e_src, g->e_dst, m, n, tmp, &bad);
+  EG_TRY(eg_dalloc_t(&G.tmp, 1025, "id flag"));
+  int rc = eg_check_ids(g->stream, g->e_src, g->e_dst, m, n, G.tmp, &bad);
   if (rc == EG_OK && bad) rc = eg_fail(EG_ERANGE, "edge file holds a vertex id >= n = %u", n);
   if (rc == EG_OK) rc = eg_dalloc_t(&g->out_deg, n, "out degree");
   if (rc == EG_OK) rc = eg_dalloc_t(&g->in_deg, n, "in degree");
@@ -213,15 +221,15 @@ int eg_build_layout(eg_graph *g) {
         break;
     }
   }
-  if (rc == EG_OK) rc = eg_count_zero(g->stream, g->out_deg, n, tmp, &g->ndangling);
-  eg_dfree_t(&tmp);
-  if (rc) return rc;
-  EG_HIP(hipEventRecord(e1, g->stream));
-  EG_HIP(hipEventSynchronize(e1));
+  if (rc == EG_OK) rc = eg_count_zero(g->stream, g->out_deg, n, G.tmp, &g->ndangling);
+  if (rc) {
+    (void)hipStreamSynchronize(g->stream);  // nothing in flight reads what the guard frees
+    return rc;
+  }
+  EG_HIP(hipEventRecord(G.e[1], g->stream));
+  EG_HIP(hipEventSynchronize(G.e[1]));
   float ms = 0.f;
-  EG_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  EG_HIP(hipEventElapsedTime(&ms, G.e[0], G.e[1]));
   g->build_ms = ms;
   // The records (memblock) are released once the layout exists, as
   // init_adj_sort / init_adj_count do (init_all.c:494,787).

@@ -603,8 +603,12 @@ static int spmv_host(eg_graph *g, const float *x, float *y, bool grid) {
       finite = std::isfinite(ax);
       xmax = std::max(xmax, ax);
     }
-  if (eg_pb_selected(g) && finite) {
+  bool binned = eg_pb_selected(g) && finite;
+  if (binned) {
     EG_TRY(eg_pb_build(g, cells_source(g, grid)));
+    binned = eg_pb_spmv_exact(g, xmax);
+  }
+  if (binned) {
     EG_TRY(eg_pb_spmv(g, g->contrib_a, g->rank_a, xmax));
   } else if (grid) {
     EG_TRY(eg_spmv_grid_device(g, g->contrib_a, g->rank_a));

@@ -217,6 +217,9 @@ struct PbSource {
 int eg_pb_build(eg_graph *g, const PbSource &src);
 int eg_pb_scale(double bound);
 int eg_pb_spmv(eg_graph *g, const float *x, float *y, double xmax, hipStream_t stream = nullptr);
+// True when eg_pb_spmv can sum exactly over the built plan: finite weights
+// and |w| |x| below FLT_MAX (else the pull kernels take the product).
+bool eg_pb_spmv_exact(const eg_graph *g, double xmax);
 // True when PageRank on adjacency lists uses the binned kernels.
 bool eg_pb_selected(const eg_graph *g);
 bool eg_reorder_enabled();

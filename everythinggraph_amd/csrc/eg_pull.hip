@@ -920,12 +920,16 @@ extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
       finite = std::isfinite(ax);
       xmax = std::max(xmax, ax);
     }
-  if (eg_pb_selected(g) && finite) {
+  bool binned = eg_pb_selected(g) && finite;
+  if (binned) {
     PbSource src;
     src.ptr = ptr;
     src.idx = idx;
     src.w = w;
     EG_TRY(eg_pb_build(g, src));
+    binned = eg_pb_spmv_exact(g, xmax);
+  }
+  if (binned) {
     EG_TRY(eg_pb_spmv(g, xin, yout, xmax));
   } else {
     PrArgs a = {};

@@ -47,6 +47,7 @@ struct Query {
   double t_submit = 0.0, t_start = 0.0, t_done = 0.0;
   uint32_t level = 0;  // BFS: largest level so far (root = 1); PPR: iterations
   bool done = false;
+  bool claimed = false;  // a thread waits on this ticket (one waiter per ticket)
   eg_serve_result res{};
   std::string err;
 };
@@ -897,10 +898,16 @@ extern "C" int eg_serve_wait(eg_server *S, uint64_t ticket, double timeout_ms,
   auto it = S->all.find(ticket);
   if (it == S->all.end()) return eg_fail(EG_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
   Query *q = it->second;
+  // The waiter that sees the query done frees it: a second concurrent waiter
+  // on the same ticket would read freed memory, so it is refused.
+  if (q->claimed)
+    return eg_fail(EG_EINVAL, "ticket %llu already has a waiter", (unsigned long long)ticket);
+  q->claimed = true;
   if (timeout_ms < 0.0) {
     S->cv_done.wait(lk, [&] { return q->done; });
   } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms),
                                   [&] { return q->done; })) {
+    q->claimed = false;
     return eg_fail(EG_ETIMEDOUT, "query %llu still running", (unsigned long long)ticket);
   }
   const eg_serve_result r = q->res;

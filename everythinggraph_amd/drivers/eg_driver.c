@@ -36,6 +36,20 @@ void eg_check(int rc, const char *what) {
   }
 }
 
+void eg_dump(const void *data, size_t bytes) {
+  const char *path = getenv("EG_DUMP");
+  if (!path || !*path) return;
+  FILE *f = fopen(path, "wb");
+  if (!f || fwrite(data, 1, bytes, f) != bytes) {
+    fprintf(stderr, "cannot write %s\n", path);
+    exit(1);
+  }
+  if (fclose(f) != 0) {
+    fprintf(stderr, "cannot write %s\n", path);
+    exit(1);
+  }
+}
+
 static void usage(void) {
   /* random.c:32-48, plus the options added by this implementation. */
   printf("Usage: ./random -f <graph file> -n <nb_nodes> [-m -u -r -L -w -b -a [mode] -p [bfs_root] -s -N ]\n");

@@ -43,6 +43,10 @@ extern int opt_dangling;   /* -D: 1 = redistribute dangling mass uniformly */
 
 /* Exits with the library's message when rc != EG_OK. */
 void eg_check(int rc, const char *what);
+/* Writes the result array of the algorithm (the global the reference leaves
+ * behind: dist, components, rank, y) as raw bytes to the file named by the
+ * EG_DUMP environment variable, when it is set; a no-op otherwise. */
+void eg_dump(const void *data, size_t bytes);
 double eg_now(void);
 
 #endif

@@ -29,6 +29,7 @@ static void prgrid(struct node *nodes) {
   printf("#Algo time ( %.3f sec) \n", t1 - t0);
   for (uint32_t i = 0; i < 10 && i < NB_NODES; i++) printf("Rank[%u] = %.5f\n", i, rank_out[i]);
   free(l1);
+  eg_dump(rank_out, (size_t)NB_NODES * sizeof(float));
 }
 
 struct algo_func current_algo = {

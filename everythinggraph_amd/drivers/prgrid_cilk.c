@@ -28,6 +28,7 @@ static void prgrid(struct node *nodes) {
   printf("Iter %d time %f \n", opt_iterations, t1 - t0);
   printf("#Algo time ( %.3f sec) \n", t1 - t0);
   for (uint32_t i = 0; i < 10 && i < NB_NODES; i++) printf("Rank[%u] = %.5f\n", i, rank_out[i]);
+  eg_dump(rank_out, (size_t)NB_NODES * sizeof(float));
 }
 
 struct algo_func current_algo = {.reset = prgrid_reset,

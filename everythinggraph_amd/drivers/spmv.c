@@ -34,6 +34,7 @@ static void spmv(struct node *nodes) {
   printf("#Algo time ( %.3f sec) \n", t1 - t0);
   printf("sum(y) = %.6e\n", sum);
   for (uint32_t i = 0; i < 10 && i < NB_NODES; i++) printf("y[%u] = %.5f\n", i, y[i]);
+  eg_dump(y, (size_t)NB_NODES * sizeof(float));
 }
 
 struct algo_func current_algo = {

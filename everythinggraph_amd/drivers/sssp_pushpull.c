@@ -26,6 +26,7 @@ static void sssp_destruct(void) {
   for (uint64_t i = 0; i < NB_NODES; i++)
     if (dist[i] != EG_SSSP_UNREACHED) nodes_discovered++;
   printf("Total nodes discovered:: %lu\n", (unsigned long)nodes_discovered);
+  eg_dump(dist, (size_t)NB_NODES * sizeof(uint32_t));
   free(dist);
 }
 

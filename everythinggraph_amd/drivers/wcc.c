@@ -44,6 +44,7 @@ static void wcc(struct node *nodes) {
   printf("Iter %d\n", rounds);
   printf("Algo time %f \n", t1 - t0);
   validate();
+  eg_dump(components, (size_t)NB_NODES * sizeof(int64_t));
 }
 
 struct algo_func current_algo = {

@@ -351,7 +351,8 @@ int eg_serve_submit_ppr(eg_server *s, const uint32_t *seeds, uint32_t nseeds,
                         float *out, uint64_t *ticket);
 /* Waits for a ticket (timeout_ms < 0: no limit) and releases it.  Returns
  * EG_ETIMEDOUT while it is still running, else EG_OK with the query's own
- * status in res->status. */
+ * status in res->status.  One thread at a time may wait on a ticket: a second
+ * concurrent wait on it returns EG_EINVAL; after EG_OK the ticket is unknown. */
 int eg_serve_wait(eg_server *s, uint64_t ticket, double timeout_ms, eg_serve_result *res);
 int eg_serve_stats_get(eg_server *s, eg_serve_stats *st);
 /* Completes every submitted query, stops the engine and frees the server. */

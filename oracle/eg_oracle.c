@@ -131,8 +131,12 @@ uint32_t ego_default_P(uint32_t n) {
 /* Stable counting sort of records by key[i] < nkeys.  ptr gets nkeys+1 offsets;
  * perm[k] = index of the record placed at k.  This is the contract of the
  * reference's stable radix sort intSort::iSort (radixSort_ligra.h:355, stable
- * because radixBlock places back to front, :158-161). */
-static void stable_count_sort(uint64_t nkeys, uint64_t m, const uint32_t *key,
+ * because radixBlock places back to front, :158-161).  Small inputs take one
+ * serial counting pass; large ones (the full-size configurations) an
+ * OpenMP LSD radix sort over 11-bit digits with per-thread digit counts over
+ * contiguous blocks, which is stable digit by digit and so yields the same
+ * permutation. */
+static void serial_count_sort(uint64_t nkeys, uint64_t m, const uint32_t *key,
                               uint64_t *ptr, uint64_t *perm) {
   uint64_t *cur = (uint64_t *)calloc(nkeys + 1, sizeof(uint64_t));
   for (uint64_t i = 0; i < m; i++) cur[key[i] + 1]++;
@@ -140,6 +144,90 @@ static void stable_count_sort(uint64_t nkeys, uint64_t m, const uint32_t *key,
   if (ptr) memcpy(ptr, cur, (nkeys + 1) * sizeof(uint64_t));
   for (uint64_t i = 0; i < m; i++) perm[cur[key[i]]++] = i;
   free(cur);
+}
+
+#define EGO_DIGIT 11
+#define EGO_BINS (1u << EGO_DIGIT)
+
+static void parallel_radix_sort(uint64_t nkeys, uint64_t m, const uint32_t *key,
+                                uint64_t *ptr, uint64_t *perm) {
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < nkeys) bits++;
+  const int T = omp_get_max_threads();
+  uint32_t *kbuf[2];
+  uint64_t *pbuf[2];
+  kbuf[0] = (uint32_t *)malloc(m * sizeof(uint32_t));
+  kbuf[1] = (uint32_t *)malloc(m * sizeof(uint32_t));
+  pbuf[0] = perm;
+  pbuf[1] = (uint64_t *)malloc(m * sizeof(uint64_t));
+  uint64_t *cnt = (uint64_t *)malloc((size_t)T * EGO_BINS * sizeof(uint64_t));
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)m; i++) {
+    kbuf[0][i] = key[i];
+    pbuf[0][i] = (uint64_t)i;
+  }
+  int cur = 0;
+  for (int shift = 0; shift < bits; shift += EGO_DIGIT) {
+    const uint32_t *kin = kbuf[cur];
+    const uint64_t *pin = pbuf[cur];
+    uint32_t *kout = kbuf[cur ^ 1];
+    uint64_t *pout = pbuf[cur ^ 1];
+#pragma omp parallel num_threads(T)
+    {
+      const int t = omp_get_thread_num();
+      const uint64_t lo = m * (uint64_t)t / (uint64_t)T, hi = m * (uint64_t)(t + 1) / (uint64_t)T;
+      uint64_t *c = cnt + (size_t)t * EGO_BINS;
+      memset(c, 0, EGO_BINS * sizeof(uint64_t));
+      for (uint64_t i = lo; i < hi; i++) c[(kin[i] >> shift) & (EGO_BINS - 1)]++;
+#pragma omp barrier
+#pragma omp single
+      {
+        /* digit-major, then block order: the offsets of a stable pass */
+        uint64_t run = 0;
+        for (uint32_t b = 0; b < EGO_BINS; b++)
+          for (int u = 0; u < T; u++) {
+            const uint64_t v = cnt[(size_t)u * EGO_BINS + b];
+            cnt[(size_t)u * EGO_BINS + b] = run;
+            run += v;
+          }
+      }
+      for (uint64_t i = lo; i < hi; i++) {
+        const uint64_t at = c[(kin[i] >> shift) & (EGO_BINS - 1)]++;
+        kout[at] = kin[i];
+        pout[at] = pin[i];
+      }
+    }
+    cur ^= 1;
+  }
+  if (cur == 1) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)m; i++) perm[i] = pbuf[1][i];
+  }
+  if (ptr) {
+    /* ptr[k] = first position whose key is >= k (key sorted ascending) */
+    const uint32_t *ks = kbuf[cur];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k <= (int64_t)nkeys; k++) {
+      uint64_t a = 0, b = m;
+      while (a < b) {
+        const uint64_t mid = a + (b - a) / 2;
+        if (ks[mid] < (uint64_t)k) a = mid + 1; else b = mid;
+      }
+      ptr[k] = a;
+    }
+  }
+  free(cnt);
+  free(pbuf[1]);
+  free(kbuf[1]);
+  free(kbuf[0]);
+}
+
+static void stable_count_sort(uint64_t nkeys, uint64_t m, const uint32_t *key,
+                              uint64_t *ptr, uint64_t *perm) {
+  if (m < ((uint64_t)1 << 22) || omp_get_max_threads() == 1)
+    serial_count_sort(nkeys, m, key, ptr, perm);
+  else
+    parallel_radix_sort(nkeys, m, key, ptr, perm);
 }
 
 /* init_adj_sort, init_all.c:629-697 (modes 2/3; mode 4's count sort,
@@ -151,7 +239,8 @@ void ego_build_csr(uint32_t n, uint64_t m, const uint32_t *src,
                    uint32_t *col, float *wout) {
   uint64_t *perm = (uint64_t *)malloc((m ? m : 1) * sizeof(uint64_t));
   stable_count_sort(n, m, src, row_ptr, perm);
-  for (uint64_t k = 0; k < m; k++) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)m; k++) {
     col[k] = dst[perm[k]];
     if (w && wout) wout[k] = w[perm[k]];
   }
@@ -165,10 +254,12 @@ void ego_build_csc(uint32_t n, uint64_t m, const uint64_t *row_ptr,
                    uint32_t *row_idx, float *win) {
   uint64_t *perm = (uint64_t *)malloc((m ? m : 1) * sizeof(uint64_t));
   uint32_t *rowof = (uint32_t *)malloc((m ? m : 1) * sizeof(uint32_t));
-  for (uint32_t v = 0; v < n; v++)
-    for (uint64_t k = row_ptr[v]; k < row_ptr[v + 1]; k++) rowof[k] = v;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t v = 0; v < (int64_t)n; v++)
+    for (uint64_t k = row_ptr[v]; k < row_ptr[v + 1]; k++) rowof[k] = (uint32_t)v;
   stable_count_sort(n, m, col, col_ptr, perm);
-  for (uint64_t k = 0; k < m; k++) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)m; k++) {
     row_idx[k] = rowof[perm[k]];
     if (w && win) win[k] = w[perm[k]];
   }
@@ -194,13 +285,15 @@ void ego_build_grid(uint32_t n, uint32_t P, uint64_t m, const uint32_t *src,
     for (uint64_t i = 0; i < m; i++) perm0[i] = i;
   }
   uint32_t *cell = (uint32_t *)malloc((m ? m : 1) * sizeof(uint32_t));
-  for (uint64_t k = 0; k < m; k++) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)m; k++) {
     const uint64_t i = perm0[k];
     cell[k] = ego_partition_id(n, P, src[i]) * P + ego_partition_id(n, P, dst[i]);
   }
   uint64_t *perm1 = (uint64_t *)malloc((m ? m : 1) * sizeof(uint64_t));
   stable_count_sort((uint64_t)P * P, m, cell, cell_ptr, perm1);
-  for (uint64_t k = 0; k < m; k++) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)m; k++) {
     const uint64_t i = perm0[perm1[k]];
     gsrc[k] = src[i];
     gdst[k] = dst[i];

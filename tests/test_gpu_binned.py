@@ -263,6 +263,48 @@ def test_binned_spmv_nonfinite_falls_back():
     assert not np.isfinite(y1).all()
 
 
+@pytest.mark.parametrize("mode", [2, 0])
+@pytest.mark.parametrize("case", ["inf_weight", "nan_weight", "product_overflow"])
+def test_binned_spmv_nonfinite_weights_fall_back(mode, case):
+    """A weight of inf or NaN (12-byte records may hold them), or a finite
+    weight whose FP32 product with x overflows, has no finite fixed-point
+    scale either: the pull / grid kernel runs, on the adjacency lists and on
+    the grid, with the same IEEE results as pr_binned = 0."""
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=65)
+    w = O.gen_weights(s0.size, seed=3)
+    x = np.random.default_rng(3).uniform(-1, 1, n).astype(np.float32)
+    if case == "inf_weight":
+        w[7] = np.inf
+    elif case == "nan_weight":
+        w[11] = np.nan
+    else:
+        w[13] = 3e38
+        x[:] = 2.0  # 6e38 > FLT_MAX: the product is inf in FP32
+    with eg.Graph(n, mode, weighted=True, P=16 if mode == 0 else 0) as g:
+        g.load_edges(s0, d0, w)
+        layout = "grid" if mode == 0 else "adj"
+        g.tune_set("pr_binned", 0)
+        y0 = g.spmv(x, layout=layout)
+        g.tune_set("pr_binned", 1)
+        y1 = g.spmv(x, layout=layout)
+        # finite weights again: the binned kernels sum exactly
+        w2 = np.where(np.isfinite(w) & (w < 1.0), w, 0.5).astype(np.float32)
+        g.load_edges(s0, d0, w2)
+        x2 = np.random.default_rng(4).uniform(-1, 1, n).astype(np.float32)
+        ya = g.spmv(x2, layout=layout)
+        g.tune_set("pr_binned", 0)
+        yb = g.spmv(x2, layout=layout)
+    assert np.array_equal(y0, y1, equal_nan=True)
+    assert not np.isfinite(y1).all()
+    ptr, col, ow = O.build_csr(n, s0, d0, w2)
+    cp, row, iw = O.build_csc(n, ptr, col, ow)
+    ref = O.spmv_pull(n, cp, row, iw, x2.astype(np.float64))
+    absref = O.spmv_pull(n, cp, row, iw, np.abs(x2).astype(np.float64))
+    for y in (ya, yb):
+        assert np.all(np.abs(y.astype(np.float64) - ref) <= 1e-5 * absref + 1e-30)
+
+
 def test_binned_uniform_graph_has_no_hot_set():
     """Erdos-Renyi ids carry no degree skew: the first 16384 sources take
     under 15% of the entries, so the plan scatters every source."""

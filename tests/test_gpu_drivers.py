@@ -1,6 +1,11 @@
 """The drop-in binaries end to end: each reads an edge file with the
 reference's CLI (random.c:50-116), runs on the device and prints the lines of
-the reference binary; the printed results are checked against the oracle."""
+the reference binary; the printed results are checked against the oracle.
+
+With EG_DUMP set, each binary also writes the whole result array it leaves
+behind (dist, components, rank, y) in the reference encoding, and that array
+is compared with the oracle: bit for bit for levels, labels and distances,
+within the FP32 tolerances of the algorithm tests for ranks and products."""
 import os
 import re
 import subprocess
@@ -26,12 +31,17 @@ def graph_file(tmp_path_factory):
     return str(path), s, d
 
 
-def run(drv, *args):
+def run_dump(tmp_path, dtype, drv, *args):
+    """(stdout, the result array the binary wrote through EG_DUMP)."""
+    path = tmp_path / f"{drv}.out"
     exe = os.path.join(eg.BIN_DIR, drv)
     print("driver:", drv, *args, flush=True)
-    r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120,
+                       env={**os.environ, "EG_DUMP": str(path)})
     assert r.returncode == 0, (drv, args, r.stdout[-2000:], r.stderr[-2000:])
-    return r.stdout
+    out = np.fromfile(path, dtype)
+    assert out.size == N
+    return r.stdout, out
 
 
 def ranks_printed(out):
@@ -40,9 +50,10 @@ def ranks_printed(out):
 
 @pytest.mark.parametrize("drv,mode", [("pagerank_simple", 2), ("pagerank_simple", 8),
                                       ("prgrid_cilk", 0)])
-def test_pagerank_drivers(graph_file, drv, mode):
+def test_pagerank_drivers(graph_file, tmp_path, drv, mode):
     path, s, d = graph_file
-    out = run(drv, "-f", path, "-n", N, "-m", mode, "-i", 20, "-I", 0)
+    out, rank = run_dump(tmp_path, np.float32, drv, "-f", path, "-n", N, "-m", mode, "-i", 20,
+                         "-I", 0)
     ptr, col, _ = O.build_csr(N, s, d)
     cp, row, _ = O.build_csc(N, ptr, col)
     ref, _, _ = O.pagerank_pull(N, cp, row, np.diff(ptr).astype(np.uint32), 0.85, 20)
@@ -50,11 +61,13 @@ def test_pagerank_drivers(graph_file, drv, mode):
     assert sorted(got) == list(range(10))
     for i, v in got.items():
         assert abs(v - ref[i]) <= 6e-6, (i, v, ref[i])  # printed with 5 decimals
+    err = np.abs(rank.astype(np.float64) - ref)
+    assert err.sum() <= 1e-6 and err.max() <= 1e-6 * ref.max()
 
 
-def test_spmv_driver(graph_file):
+def test_spmv_driver(graph_file, tmp_path):
     path, s, d = graph_file
-    out = run("spmv", "-f", path, "-n", N, "-m", 2)
+    out, yall = run_dump(tmp_path, np.float32, "spmv", "-f", path, "-n", N, "-m", 2)
     y = {int(i): float(v) for i, v in re.findall(r"y\[(\d+)\] = ([0-9.eE+-]+)", out)}
     ref = np.zeros(N)
     np.add.at(ref, d, s.astype(np.float64))  # y[v] = sum over u->v of x[u], x[i] = i
@@ -62,38 +75,44 @@ def test_spmv_driver(graph_file):
         assert abs(v - ref[i]) <= 1e-5 * max(1.0, ref[i]) + 6e-6, (i, v, ref[i])
     total = float(re.search(r"sum\(y\) = ([0-9.eE+-]+)", out).group(1))
     assert abs(total - ref.sum()) <= 1e-5 * ref.sum()
+    # x >= 0 here, so |A| |x| = A x
+    assert np.all(np.abs(yall.astype(np.float64) - ref) <= 1e-5 * ref + 1e-30)
 
 
 @pytest.mark.parametrize("drv,mode", [("bfs_simple", 2), ("bfs_simple", 4),
                                       ("bfsgrid_cilk", 0)])
-def test_bfs_drivers(graph_file, drv, mode):
+def test_bfs_drivers(graph_file, tmp_path, drv, mode):
     path, s, d = graph_file
     root = int(s[0])
-    out = run(drv, "-f", path, "-n", N, "-m", mode, "-u", "-p", root)
+    out, dist = run_dump(tmp_path, np.uint32, drv, "-f", path, "-n", N, "-m", mode, "-u", "-p",
+                         root)
     su, du, _ = O.symmetrize(s, d)
     ptr, col, _ = O.build_csr(N, su, du)
     cp, row, _ = O.build_csc(N, ptr, col)
     ref, _, _ = O.bfs(N, su.size, ptr, col, cp, row, root)
     found = int(re.search(r"Total nodes discovered:: (\d+)", out).group(1))
     assert found == np.count_nonzero(ref)
+    assert np.array_equal(dist, ref)  # level + 1, 0 = unreached
 
 
 @pytest.mark.parametrize("mode", [2, 8, 0])
-def test_wcc_driver(graph_file, mode):
+def test_wcc_driver(graph_file, tmp_path, mode):
     path, s, d = graph_file
-    out = run("wcc", "-f", path, "-n", N, "-m", mode, "-u")
+    out, comp = run_dump(tmp_path, np.int64, "wcc", "-f", path, "-n", N, "-m", mode, "-u")
     su, du, _ = O.symmetrize(s, d)
     ptr, col, _ = O.build_csr(N, su, du)
     lab, _ = O.wcc_push(N, ptr, col)
     sizes = np.bincount(lab, minlength=N)
     tc, mc = map(int, re.search(r"Total comp (\d+) , max comp (\d+)", out).groups())
     assert tc == np.count_nonzero(sizes) and mc == sizes.max()
+    assert np.array_equal(comp, lab)  # long components[] of wcc.c:12
 
 
-def test_sssp_driver(graph_file):
+def test_sssp_driver(graph_file, tmp_path):
     path, s, d = graph_file
     root = int(s[0])
-    out = run("sssp_pushpull", "-f", path, "-n", N, "-m", 2, "-p", root)
+    out, dist = run_dump(tmp_path, np.uint32, "sssp_pushpull", "-f", path, "-n", N, "-m", 2,
+                         "-p", root)
     ptr, col, _ = O.build_csr(N, s, d)
     ref, _ = O.sssp(N, ptr, col, root, 0)
     fin = ref[ref != O.SSSP_UNREACHED]
@@ -103,3 +122,4 @@ def test_sssp_driver(graph_file):
     assert d1 == (int(ref[1]) if ref[1] != O.SSSP_UNREACHED else -2)
     found = int(re.search(r"Total nodes discovered:: (\d+)", out).group(1))
     assert found == fin.size
+    assert np.array_equal(dist, ref)

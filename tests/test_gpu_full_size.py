@@ -10,8 +10,11 @@ that do not depend on size.
   <= 1e-6 and max error <= 1e-6 of the largest rank against the FP64 oracle.
 * SpMV with signed x: |y - y64| <= 1e-5 (|A| |x|) elementwise.
 * BFS: levels bit-exact against the oracle, parents a legal BFS tree, the
-  Graph500 edge count; multi-source BFS (64 roots) equal to single-source.
-* WCC labels bit-exact; PPR (batched and single-vector) L1 <= 1e-6.
+  Graph500 edge count; multi-source BFS over 256 roots (the batched
+  configuration) equal to single-source, four roots equal to the oracle.
+* WCC labels bit-exact; PPR, 64 queries batched as one SpMM (the batched
+  configuration) and single-vector, L1 <= 2e-6 against each other and, for
+  four queries, <= 1e-6 against the FP64 oracle.
 """
 import numpy as np
 import pytest
@@ -93,17 +96,22 @@ def test_full_size_bfs(big):
         assert st.edges_traversed == int(deg[reached].sum()) // 2
 
 
+@pytest.mark.timeout(600)
 def test_full_size_multi_source_bfs(big):
+    """The batched configuration: 256 BFS sources in one multi-source call;
+    every lane equals the single-source traversal from its root, and four
+    lanes equal the oracle."""
     g, ptr, col, deg = big
-    roots = eg.sample_roots(deg, 64, seed=3)
-    say("multi-source bfs, 64 roots")
+    roots = eg.sample_roots(deg, 256, seed=3)
+    say("multi-source bfs, 256 roots")
     out, _ = g.bfs_batch(roots)
     for i, root in enumerate(roots):
         dist, _, _ = g.bfs(int(root), parents=False)
         assert np.array_equal(out[i], dist), i
-    for i in (0, 21, 42, 63):
+    for i in (0, 85, 170, 255):
         ref, _, _ = O.bfs(N, int(ptr[-1]), ptr, col, ptr, col, int(roots[i]))
         assert np.array_equal(out[i], ref), i
+    del out
 
 
 def test_full_size_wcc(big):
@@ -114,15 +122,23 @@ def test_full_size_wcc(big):
     assert np.array_equal(lab, ref)
 
 
+@pytest.mark.timeout(600)
 def test_full_size_ppr(big):
+    """The batched configuration: 64 personalized PageRank vectors (1-8 seeds,
+    alpha 0.15, 30 iterations) as one SpMM; four against the FP64 oracle
+    within 1e-6 L1, and all 64 against the single-vector path within 2e-6."""
     g, ptr, col, deg = big
     rng = np.random.default_rng(3)
-    seeds = [eg.sample_roots(deg, int(rng.integers(1, 9)), seed=1000 + q) for q in range(2)]
-    say("ppr, 2 queries batched and single-vector, 30 iterations")
+    seeds = [eg.sample_roots(deg, int(rng.integers(1, 9)), seed=1000 + q) for q in range(64)]
+    say("ppr, 64 queries batched, 30 iterations")
     out, _ = g.ppr_batch(seeds, 0.15, 30)
     for q, sd in enumerate(seeds):
-        ref = O.ppr(N, ptr, col, deg, sd, 0.15, 30)
-        assert np.abs(out[q].astype(np.float64) - ref).sum() <= 1e-6, q
         one, it, _ = g.ppr(sd, 0.15, tol=0.0, max_iter=30)
         assert it == 30
-        assert np.abs(one.astype(np.float64) - ref).sum() <= 1e-6, q
+        # each is within 1e-6 of the oracle, so within 2e-6 of each other
+        assert np.abs(one.astype(np.float64) - out[q]).sum() <= 2e-6, q
+        if q % 21 == 0:
+            say(f"ppr query {q} against the oracle")
+            ref = O.ppr(N, ptr, col, deg, sd, 0.15, 30)
+            assert np.abs(out[q].astype(np.float64) - ref).sum() <= 1e-6, q
+            assert np.abs(one.astype(np.float64) - ref).sum() <= 1e-6, q

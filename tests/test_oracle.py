@@ -142,6 +142,27 @@ def test_csr_csc_vs_numpy(name, n, s, d):
     assert np.array_equal(cp, np.concatenate([[0], np.cumsum(np.bincount(d, minlength=n))]))
 
 
+@pytest.mark.parametrize("n", [1 << 11, 1 << 20, (1 << 24) + 3])
+def test_parallel_sort_path_vs_numpy(n):
+    """From 2^22 records the oracle sorts with a multi-threaded LSD radix sort
+    (11-bit digits; 1, 2 and 3 passes here): same CSR / CSC as numpy's stable
+    argsort, including skewed keys and keys past 2^22."""
+    m = (1 << 22) + 17
+    s, d = O.gen_rmat(11, m, seed=9)
+    rng = np.random.default_rng(n)
+    s = (s.astype(np.uint64) * (n // 2048) + rng.integers(0, n // 2048 or 1, m)).astype(np.uint32)
+    s = np.minimum(s, n - 1).astype(np.uint32)
+    d = rng.integers(0, n, m).astype(np.uint32)
+    ptr, col, _ = O.build_csr(n, s, d)
+    perm = np.argsort(s, kind="stable")
+    assert np.array_equal(col, d[perm])
+    assert np.array_equal(ptr, np.concatenate([[0], np.cumsum(np.bincount(s, minlength=n))]))
+    cp, row, _ = O.build_csc(n, ptr, col)
+    rowof = np.repeat(np.arange(n, dtype=np.uint32), np.diff(ptr).astype(np.int64))
+    assert np.array_equal(row, rowof[np.argsort(col, kind="stable")])
+    assert np.array_equal(cp, np.concatenate([[0], np.cumsum(np.bincount(d, minlength=n))]))
+
+
 @pytest.mark.parametrize("name,n,s,d", zoo(), ids=[z[0] for z in zoo()])
 @pytest.mark.parametrize("Pp", [1, 3, 16])
 def test_grid_vs_numpy(name, n, s, d, Pp):
