@@ -45,6 +45,9 @@ MODE_EDGES_SRC_DST = 8
 MODE_EDGES_RAW = 9
 
 LAYOUT = {"auto": 0, "adj": 1, "grid": 2, "edges": 3}
+
+# Records that eg_csr_build_noalloc accepts (its fixed scratch array).
+NOALLOC_MAX_ENTRIES = 2097152
 DANGLING = {"drop": 0, "uniform": 1}
 
 # Exported symbols of libeg (the declarations of include/eg.h).
@@ -58,6 +61,7 @@ ABI_SYMBOLS = (
     "eg_sync", "eg_ppr", "eg_autotune", "eg_tune_set", "eg_tune_get", "eg_pr_binned_geometry",
     "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
     "eg_serve_wait", "eg_serve_stats_get", "eg_serve_stop", "eg_serve_replay",
+    "eg_csr_build_noalloc",
 )
 
 

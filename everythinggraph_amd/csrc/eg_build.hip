@@ -7,6 +7,8 @@
 // the sorted keys (one binary search per vertex), the contract of the
 // reference's boundary detection + scanIBack(min) (init_all.c:675-697): no
 // degree is accumulated with atomics, so R-MAT hubs cost nothing extra.
+#include <mutex>
+
 #include "eg_internal.h"
 
 static int sort_by(eg_graph *g, uint64_t m, uint64_t nkeys, const uint32_t *kin, uint32_t *kout,
@@ -238,4 +240,72 @@ int eg_build_layout(eg_graph *g) {
   eg_dfree_t(&g->e_w);
   g->loaded = true;
   return EG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CSR build that allocates nothing (eg_csr_build_noalloc): the stable sort by
+// src of init_adj_sort (init_all.c:629-697) over records already in device
+// memory, with every temporary (sorted keys, radix ping-pong buffers, digit
+// count matrix, scan partials) in one fixed device array.  Each device holds
+// its own copy of the array at its own address, found once per device with
+// hipGetSymbolAddress; the array's name is never passed to a kernel.  Calls
+// that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = 48ull << 20;
+constexpr int kNoAllocMaxDevices = 64;
+
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Sorted keys first, then the radix sort's workspace.
+size_t noalloc_need(uint64_t m) { return round256(m * 4) + eg_radix_tmp_bytes(m, 1); }
+
+// Device address of this device's scratch array (looked up on first use).
+int noalloc_scratch(void **out) {
+  int dev = 0;
+  EG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return eg_fail(EG_EDEVICE, "device %d beyond the %d supported", dev, kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    if (noalloc_need(EG_NOALLOC_MAX_ENTRIES) > kNoAllocScratchBytes)
+      return eg_fail(EG_EDEVICE, "scratch array smaller than the workspace of %u entries",
+                     EG_NOALLOC_MAX_ENTRIES);
+    void *p = nullptr;
+    EG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return eg_fail(EG_EDEVICE, "no device address for the scratch array");
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return EG_OK;
+}
+
+}  // namespace
+
+extern "C" int eg_csr_build_noalloc(uint32_t n, uint64_t m, const uint32_t *src,
+                                    const uint32_t *dst, uint32_t *row_ptr, uint32_t *col,
+                                    void *stream) {
+  if (m > EG_NOALLOC_MAX_ENTRIES)
+    return eg_fail(EG_EINVAL, "%llu entries exceed the %u that the fixed scratch holds",
+                   (unsigned long long)m, EG_NOALLOC_MAX_ENTRIES);
+  if (n == 0 || !row_ptr || (m && (!src || !dst || !col)))
+    return eg_fail(EG_EINVAL, "bad CSR build arguments");
+  void *base = nullptr;
+  EG_TRY(noalloc_scratch(&base));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint8_t *t = static_cast<uint8_t *>(base);
+  uint32_t *sorted = reinterpret_cast<uint32_t *>(t);
+  EgSortCols c;
+  c.np = 1;
+  c.in[0] = dst;
+  c.out[0] = col;
+  // every buffer of the sort is written before it is read: nothing to clear
+  EG_TRY(eg_radix_sort(s, m, eg_bits_for(n), src, sorted, c, t + round256(m * 4)));
+  return eg_ptr_from_sorted(s, sorted, m, n, row_ptr, nullptr);
 }

@@ -157,6 +157,24 @@ int eg_graph_get_grid(eg_graph *g, uint64_t *cell_ptr, uint32_t *src,
 /* Edge array of modes 7/8/9, init_all.c:573-628. */
 int eg_graph_get_edgelist(eg_graph *g, uint32_t *src, uint32_t *dst, float *w);
 
+/* ---- CSR build that allocates no device memory ----
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  It is the CSR of eg_graph_load_edges in load mode 2 -- the
+ * records stably sorted by src (init_adj_sort, init_all.c:629-697) -- over m
+ * records (src[i], dst[i]) already in device memory (append the reversed
+ * copies first for -u): row_ptr[n + 1] (u32 offsets) and col[m], both device
+ * memory owned by the caller.  Ids must be < n (not checked: that would need a
+ * host round trip).  Every temporary lives in one fixed device array of 48 MiB
+ * that each device holds until the process ends, so m is at most
+ * EG_NOALLOC_MAX_ENTRIES (EG_EINVAL beyond).  The work is enqueued on stream
+ * (a hipStream_t; NULL = the null stream) and the call returns without
+ * waiting.  Calls on one device share its array: calls whose work overlaps on
+ * one device race, so serialise them (one stream, or wait in between). */
+#define EG_NOALLOC_MAX_ENTRIES 2097152u
+int eg_csr_build_noalloc(uint32_t n, uint64_t m, const uint32_t *src, const uint32_t *dst,
+                         uint32_t *row_ptr, uint32_t *col, void *stream);
+
 /* ---- algorithms (replace current_algo.main of each plugin) ---- */
 /* pagerank_simple.c:72-133 / prgrid_cilk.c:62-141.  FP32 storage, compensated
  * sums, FP64 dangling-mass and L1-delta reductions on the device.  rank may be
