@@ -61,7 +61,7 @@ ABI_SYMBOLS = (
     "eg_sync", "eg_ppr", "eg_autotune", "eg_tune_set", "eg_tune_get", "eg_pr_binned_geometry",
     "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
     "eg_serve_wait", "eg_serve_stats_get", "eg_serve_stop", "eg_serve_replay",
-    "eg_csr_build_noalloc",
+    "eg_csr_build_noalloc", "eg_bfs_batch_info_get",
 )
 
 
@@ -99,6 +99,13 @@ class PrOpts(ctypes.Structure):
 class BfsStats(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("pull_levels", ctypes.c_int32),
                 ("reached", ctypes.c_uint64), ("edges_traversed", ctypes.c_uint64),
+                ("device_ms", ctypes.c_double)]
+
+
+class BfsBatchInfo(ctypes.Structure):
+    _fields_ = [("traversals", ctypes.c_uint32), ("lanes", ctypes.c_uint32),
+                ("levels", ctypes.c_uint32), ("push_levels", ctypes.c_uint32),
+                ("host_syncs", ctypes.c_uint32), ("fallbacks", ctypes.c_uint32),
                 ("device_ms", ctypes.c_double)]
 
 
@@ -394,6 +401,13 @@ class Graph:
         _check(lib().eg_bfs_batch(self._h, ctypes.c_uint32(r.size), _ptr(r), _ptr(out),
                                   ctypes.byref(ms)), "eg_bfs_batch")
         return out, ms.value
+
+    def bfs_batch_info(self) -> dict:
+        """Figures of the last bfs_batch call: traversals, lanes, levels,
+        push_levels, host_syncs, fallbacks, device_ms."""
+        i = BfsBatchInfo()
+        _check(lib().eg_bfs_batch_info_get(self._h, ctypes.byref(i)), "eg_bfs_batch_info_get")
+        return _struct_dict(i)
 
     def ppr_batch(self, seeds: Sequence[Sequence[int]], alpha=0.15, iterations=30, download=True):
         ptr = np.zeros(len(seeds) + 1, np.uint32)

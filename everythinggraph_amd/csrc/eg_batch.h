@@ -25,7 +25,7 @@ struct BatchView {
 };
 int eg_batch_view(eg_graph *g, BatchView *v);
 
-// Up to 64 nw BFS lanes over one traversal (nw = 1 or 2 lane words per
+// Up to 64 nw BFS lanes over one traversal (nw = 1, 2 or 4 lane words per
 // vertex): lane i of vertex v is bit i % 64 of word v * nw + i / 64 of the
 // masks.  Levels are u16, lane-major (lvl[lane * n + v]), root = 1, 0 = not
 // reached.
@@ -40,7 +40,12 @@ struct MsState {
   uint32_t *queue = nullptr, *qoff = nullptr;  // frontier vertices with edges, degree scan
   unsigned long long *ctr = nullptr;  // [0] progress, [1] count, [2] degsum, [3] qlen, [8..) partials
   uint16_t *lvl_tab = nullptr;                 // [64 nw] level written per lane this step
-  uint32_t *roots = nullptr;                   // [256]: lanes, roots (staging)
+  uint32_t *roots = nullptr;                   // [512]: lanes, roots (staging)
+  uint32_t *ctl = nullptr;                     // device-driven levels: control words
+  hipGraphExec_t exec = nullptr;               // two device-driven levels, replayed
+  uint64_t exec_key[6] = {};                   // what the recorded levels were launched with
+  uint32_t depth_hint = 0;                     // levels of the last device-driven traversal
+  bool pushed = false;                         // the last host-driven level pushed
   unsigned long long *h = nullptr;             // pinned: [0..5) ctr mirror, [8..8+64nw) lane sums
   uint32_t qlen = 0;                           // frontier queue length (host copy)
   uint64_t qdeg = 0;                           // its out-degree sum

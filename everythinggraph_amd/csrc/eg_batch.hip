@@ -27,6 +27,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "eg_batch.h"
 
 namespace {
@@ -38,6 +40,11 @@ constexpr uint32_t kLds = kPlanChunk + kPlanLongRow;
 constexpr uint32_t kSerialRow = 32;
 constexpr uint32_t kEdgesPerBlock = 2048;
 constexpr uint32_t kStatBlocks = 1024;
+constexpr uint32_t kMaxLanes = 256;  // four lane words
+// Grids of the device-driven level kernels (grid-stride): the kernels of the
+// direction a level does not take are dispatched and return at once.
+constexpr uint32_t kDevGrid = 2048;
+constexpr uint32_t kDevPullGrid = 4096;
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
@@ -103,10 +110,13 @@ struct Lanes {
 template <int NW>
 __device__ __forceinline__ Lanes<NW> ld_lanes(const uint64_t *__restrict__ a, uint64_t v) {
   Lanes<NW> r;
-  if constexpr (NW == 2) {
-    const ulonglong2 x = reinterpret_cast<const ulonglong2 *>(a)[v];
-    r.w[0] = x.x;
-    r.w[1] = x.y;
+  if constexpr (NW == 2 || NW == 4) {
+#pragma unroll
+    for (int h = 0; h < NW / 2; h++) {
+      const ulonglong2 x = reinterpret_cast<const ulonglong2 *>(a)[v * (NW / 2) + h];
+      r.w[2 * h] = x.x;
+      r.w[2 * h + 1] = x.y;
+    }
   } else {
 #pragma unroll
     for (int j = 0; j < NW; j++) r.w[j] = a[v * NW + j];
@@ -116,8 +126,10 @@ __device__ __forceinline__ Lanes<NW> ld_lanes(const uint64_t *__restrict__ a, ui
 
 template <int NW>
 __device__ __forceinline__ void st_lanes(uint64_t *__restrict__ a, uint64_t v, const Lanes<NW> &x) {
-  if constexpr (NW == 2) {
-    reinterpret_cast<ulonglong2 *>(a)[v] = make_ulonglong2(x.w[0], x.w[1]);
+  if constexpr (NW == 2 || NW == 4) {
+#pragma unroll
+    for (int h = 0; h < NW / 2; h++)
+      reinterpret_cast<ulonglong2 *>(a)[v * (NW / 2) + h] = make_ulonglong2(x.w[2 * h], x.w[2 * h + 1]);
   } else {
 #pragma unroll
     for (int j = 0; j < NW; j++) a[v * NW + j] = x.w[j];
@@ -132,7 +144,27 @@ __device__ __forceinline__ Lanes<NW> wave_or(Lanes<NW> v) {
 }
 
 // ctr word holding the progress bits of lane word j
-__host__ __device__ constexpr int progress_slot(int j) { return j == 0 ? 0 : 4; }
+__host__ __device__ constexpr int progress_slot(int j) { return j == 0 ? 0 : 3 + j; }
+
+// Device-driven levels (eg_bfs_batch): control words written by the level
+// kernels themselves, so that the host enqueues levels without reading
+// anything back between them.
+constexpr int kCtlDone = 0;        // the traversal has ended (every kernel returns at once)
+constexpr int kCtlPush = 1;        // this level pushes (1) or pulls (0)
+constexpr int kCtlLevel = 2;       // level value written for newly reached (vertex, lane) pairs
+constexpr int kCtlOverflow = 3;    // the level would pass the u16 store
+constexpr int kCtlLevels = 4;      // levels run
+constexpr int kCtlPushLevels = 5;  // of which pushed
+constexpr int kCtlWords = 8;
+constexpr uint32_t kMaxLevelValue = 65534;
+
+// True when a level kernel has nothing to do: the traversal ended, or the
+// level runs the other direction (want_push < 0: either).  ctl is null for
+// levels the host drives.
+__device__ __forceinline__ bool ms_skip(const uint32_t *__restrict__ ctl, int want_push) {
+  if (!ctl) return false;
+  return ctl[kCtlDone] != 0u || (want_push >= 0 && ctl[kCtlPush] != (uint32_t)want_push);
+}
 
 // Adds this workgroup's new lanes to the progress words of ctr (only bits not
 // yet recorded).
@@ -249,13 +281,13 @@ __device__ __forceinline__ Lanes<NW> finish_ms_new(uint32_t n, uint32_t r, const
 //    flight, longer rows by a wave, 128 entries per check.
 // With NW = 2 a gather loads the 16 bytes of both lane words at once.
 template <int B, int NW>
-__global__ __launch_bounds__(kBlock) void k_ms_pull(
-    uint32_t n, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
+__device__ __forceinline__ void ms_pull_item(
+    uint32_t item, uint32_t n, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
     const uint4 *__restrict__ chunk_info, const uint4 *__restrict__ seg_info,
     const uint32_t *__restrict__ long_rows, uint32_t nseg, uint64_t *__restrict__ seg_or,
     uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
-    Lanes<NW> active, unsigned long long *__restrict__ ctr) {
+    const Lanes<NW> &active, unsigned long long *__restrict__ ctr) {
   static_assert(kLds <= 65535 && kChunk <= 65535, "u16 LDS offsets");
   __shared__ uint32_t s_idx[kLds];
   __shared__ uint16_t s_ptr[kChunk + 1];
@@ -266,8 +298,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
   const uint32_t tid = threadIdx.x;
   const int lane = __lane_id();
   const int wave = tid >> 6;
-  if (blockIdx.x < nseg) {
-    const uint4 si = seg_info[blockIdx.x];
+  if (item < nseg) {
+    const uint4 si = seg_info[item];
     const Lanes<NW> need = active & ~ld_lanes<NW>(seen, long_rows[si.z]);
     Lanes<NW> acc = Lanes<NW>::zero();
     if (need.any()) {
@@ -290,11 +322,11 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
     if (tid == 0)
 #pragma unroll
       for (int j = 0; j < NW; j++)
-        seg_or[(uint64_t)blockIdx.x * NW + j] =
+        seg_or[(uint64_t)item * NW + j] =
             s_or[0 * NW + j] | s_or[1 * NW + j] | s_or[2 * NW + j] | s_or[3 * NW + j];
     return;
   }
-  const uint4 ci = chunk_info[blockIdx.x - nseg];
+  const uint4 ci = chunk_info[item - nseg];
   const uint32_t r0 = ci.x, r1 = ci.y, k0 = ci.z, k1 = ci.w;
   const uint32_t nrows = r1 - r0;
   if (tid == 0) s_nlist = 0;
@@ -366,13 +398,34 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
   publish_progress<NW>(nwall, s_or, ctr);
 }
 
+// The pull level's work items (segments of long rows, then chunks) for
+// item = blockIdx.x, blockIdx.x + gridDim.x, ... < nitems: one item per
+// workgroup when the host drives the levels, a fixed grid when the device
+// does (a push level then dispatches few idle workgroups).
+template <int B, int NW>
+__global__ __launch_bounds__(kBlock) void k_ms_pull(
+    uint32_t n, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
+    const uint4 *__restrict__ chunk_info, const uint4 *__restrict__ seg_info,
+    const uint32_t *__restrict__ long_rows, uint32_t nseg, uint32_t nitems,
+    uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
+    uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
+    Lanes<NW> active, unsigned long long *__restrict__ ctr, const uint32_t *__restrict__ ctl) {
+  if (ms_skip(ctl, 0)) return;
+  for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+    ms_pull_item<B, NW>(item, n, ptr, idx, chunk_info, seg_info, long_rows, nseg, seg_or, seen,
+                        visit, nvisit, lvl, tab, active, ctr);
+    __syncthreads();  // the item's LDS is reused by the next one
+  }
+}
+
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
     uint32_t n, const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
     const uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
-    Lanes<NW> active, unsigned long long *__restrict__ ctr) {
+    Lanes<NW> active, unsigned long long *__restrict__ ctr, const uint32_t *__restrict__ ctl) {
   __shared__ uint64_t s_or[kWaves * NW];
+  if (ms_skip(ctl, 0)) return;
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   Lanes<NW> nw = Lanes<NW>::zero();
   if (j < nlong) {
@@ -388,8 +441,10 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_stats(uint32_t n, const uint64_t *__restrict__ visit,
                                                      const uint32_t *__restrict__ ptr,
-                                                     unsigned long long *__restrict__ part) {
+                                                     unsigned long long *__restrict__ part,
+                                                     const uint32_t *__restrict__ ctl) {
   __shared__ unsigned long long s_c[kWaves], s_d[kWaves];
+  if (ms_skip(ctl, -1)) return;
   unsigned long long c = 0, d = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     if (ld_lanes<NW>(visit, v).any()) {
@@ -445,9 +500,11 @@ template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_compact(uint32_t n, const uint64_t *__restrict__ visit,
                                                        const uint32_t *__restrict__ ptr,
                                                        uint32_t *__restrict__ q,
-                                                       unsigned long long *__restrict__ qlen) {
+                                                       unsigned long long *__restrict__ qlen,
+                                                       const uint32_t *__restrict__ ctl) {
   __shared__ uint32_t s_q[kBlock];
   __shared__ uint32_t s_n, s_base;
+  if (ms_skip(ctl, 1)) return;
   for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
@@ -468,63 +525,72 @@ __global__ void k_ms_qdeg(const uint32_t *__restrict__ q, uint32_t qn,
   if (i < qn) deg[i] = ptr[q[i] + 1] - ptr[q[i]];
 }
 
-// Push level: the frontier's edges [b*2048, (b+1)*2048) of the degree scan.
+// Push level: the frontier's edges [b*2048, (b+1)*2048) of the degree scan,
+// blocks b = blockIdx.x, blockIdx.x + gridDim.x, ...; the queue length is
+// *qn_dev when given (device-driven levels), else qn.
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_push(
-    const uint32_t *__restrict__ q, uint32_t qn, const uint32_t *__restrict__ qoff,
-    const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
-    const uint64_t *__restrict__ visit, const uint64_t *__restrict__ seen,
-    uint64_t *__restrict__ pend, Lanes<NW> active) {
+    const uint32_t *__restrict__ q, uint32_t qn_host, const unsigned long long *__restrict__ qn_dev,
+    const uint32_t *__restrict__ qoff, const uint32_t *__restrict__ ptr,
+    const uint32_t *__restrict__ idx, const uint64_t *__restrict__ visit,
+    const uint64_t *__restrict__ seen, uint64_t *__restrict__ pend, Lanes<NW> active,
+    const uint32_t *__restrict__ ctl) {
   __shared__ uint32_t s_off[kEdgesPerBlock + 1];
   __shared__ uint32_t s_u[kEdgesPerBlock + 1];
   __shared__ uint64_t s_vis[(kEdgesPerBlock + 1) * NW];
   __shared__ uint32_t s_first, s_cnt;
-  const uint32_t e0 = blockIdx.x * kEdgesPerBlock;
+  if (ms_skip(ctl, 1)) return;
+  const uint32_t qn = qn_dev ? (uint32_t)*qn_dev : qn_host;
+  if (qn == 0) return;
   const uint32_t total = qoff[qn];
-  const uint32_t e1 = e0 + kEdgesPerBlock < total ? e0 + kEdgesPerBlock : total;
-  if (threadIdx.x == 0) {
-    uint32_t lo = 0, hi = qn;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (qoff[mid] <= e0) lo = mid; else hi = mid;
-    }
-    uint32_t a = lo, b = qn;
-    while (a < b) {
-      const uint32_t mid = a + (b - a) / 2;
-      if (qoff[mid] < e1) a = mid + 1; else b = mid;
-    }
-    s_first = lo;
-    s_cnt = a - lo;
-  }
-  __syncthreads();
-  const uint32_t cnt = s_cnt <= kEdgesPerBlock + 1 ? s_cnt : kEdgesPerBlock + 1;
-  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
-    s_off[i] = qoff[s_first + i];
-    const uint32_t u = q[s_first + i];
-    s_u[i] = u;
-    const Lanes<NW> vis = ld_lanes<NW>(visit, u) & active;
-#pragma unroll
-    for (int j = 0; j < NW; j++) s_vis[i * NW + j] = vis.w[j];
-  }
-  __syncthreads();
-  for (uint32_t base = e0; base < e1; base += kBlock) {
-    const uint32_t e = base + threadIdx.x;
-    if (e < e1) {
-      uint32_t lo = 0, hi = cnt;
+  for (uint32_t blk = blockIdx.x; (uint64_t)blk * kEdgesPerBlock < total; blk += gridDim.x) {
+    const uint32_t e0 = blk * kEdgesPerBlock;
+    const uint32_t e1 = e0 + kEdgesPerBlock < total ? e0 + kEdgesPerBlock : total;
+    if (threadIdx.x == 0) {
+      uint32_t lo = 0, hi = qn;
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
-        if (s_off[mid] <= e) lo = mid; else hi = mid;
+        if (qoff[mid] <= e0) lo = mid; else hi = mid;
       }
-      const uint32_t w = idx[ptr[s_u[lo]] + (e - s_off[lo])];
+      uint32_t a = lo, b = qn;
+      while (a < b) {
+        const uint32_t mid = a + (b - a) / 2;
+        if (qoff[mid] < e1) a = mid + 1; else b = mid;
+      }
+      s_first = lo;
+      s_cnt = a - lo;
+    }
+    __syncthreads();
+    const uint32_t cnt = s_cnt <= kEdgesPerBlock + 1 ? s_cnt : kEdgesPerBlock + 1;
+    for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+      s_off[i] = qoff[s_first + i];
+      const uint32_t u = q[s_first + i];
+      s_u[i] = u;
+      const Lanes<NW> vis = ld_lanes<NW>(visit, u) & active;
 #pragma unroll
-      for (int j = 0; j < NW; j++) {
-        const uint64_t word = (uint64_t)w * NW + j;
-        const uint64_t bits =
-            s_vis[lo * NW + j] & ~__atomic_load_n(&seen[word], __ATOMIC_RELAXED);
-        if (bits && (__atomic_load_n(&pend[word], __ATOMIC_RELAXED) & bits) != bits)
-          atomicOr(reinterpret_cast<unsigned long long *>(&pend[word]), (unsigned long long)bits);
+      for (int j = 0; j < NW; j++) s_vis[i * NW + j] = vis.w[j];
+    }
+    __syncthreads();
+    for (uint32_t base = e0; base < e1; base += kBlock) {
+      const uint32_t e = base + threadIdx.x;
+      if (e < e1) {
+        uint32_t lo = 0, hi = cnt;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) / 2;
+          if (s_off[mid] <= e) lo = mid; else hi = mid;
+        }
+        const uint32_t w = idx[ptr[s_u[lo]] + (e - s_off[lo])];
+#pragma unroll
+        for (int j = 0; j < NW; j++) {
+          const uint64_t word = (uint64_t)w * NW + j;
+          const uint64_t bits =
+              s_vis[lo * NW + j] & ~__atomic_load_n(&seen[word], __ATOMIC_RELAXED);
+          if (bits && (__atomic_load_n(&pend[word], __ATOMIC_RELAXED) & bits) != bits)
+            atomicOr(reinterpret_cast<unsigned long long *>(&pend[word]), (unsigned long long)bits);
+        }
       }
     }
+    __syncthreads();  // the LDS tables are rebuilt for the next block of edges
   }
 }
 
@@ -535,9 +601,11 @@ __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, uint64_t *__res
                                                      uint16_t *__restrict__ lvl,
                                                      const uint16_t *__restrict__ tab,
                                                      Lanes<NW> active,
-                                                     unsigned long long *__restrict__ ctr) {
+                                                     unsigned long long *__restrict__ ctr,
+                                                     const uint32_t *__restrict__ ctl) {
   __shared__ uint64_t s_or[kWaves * NW];
   __shared__ uint16_t s_tab[64 * NW];
+  if (ms_skip(ctl, 1)) return;
   for (uint32_t i = threadIdx.x; i < 64 * NW; i += blockDim.x) s_tab[i] = tab[i];
   __syncthreads();
   Lanes<NW> all = Lanes<NW>::zero();
@@ -551,6 +619,174 @@ __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, uint64_t *__res
     }
   }
   publish_progress<NW>(all, s_or, ctr);
+}
+
+// ---- device-driven levels (eg_bfs_batch) ----
+// Every level enqueues the same kernels with the same grids: statistics of
+// the frontier, the direction decision, the push kernels (queue, degrees,
+// their scan, the edge push, the apply pass) and the pull kernels; the ones
+// of the other direction return at once.  The decision applies the
+// reference's rule (bfs_simple.c:191-197) to the frontier of all lanes, as the
+// host-driven levels do.
+__global__ void k_msd_init(uint32_t *__restrict__ ctl) {
+  if (threadIdx.x < kCtlWords) ctl[threadIdx.x] = threadIdx.x == kCtlLevel ? 2u : 0u;
+}
+
+// Sums the statistics partials of k_ms_stats, picks the direction, arms the
+// level: level table, zeroed progress words and queue length.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_msd_decide(const unsigned long long *__restrict__ part,
+                                                       uint32_t np, double thr, int sym,
+                                                       uint32_t *__restrict__ ctl,
+                                                       unsigned long long *__restrict__ ctr,
+                                                       uint16_t *__restrict__ tab) {
+  __shared__ unsigned long long s_c[kWaves], s_d[kWaves];
+  if (ctl[kCtlDone]) return;
+  const uint32_t level = ctl[kCtlLevel];
+  unsigned long long c = 0, d = 0;
+  for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+    c += part[2 * i];
+    d += part[2 * i + 1];
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    c += __shfl_xor(c, o);
+    d += __shfl_xor(d, o);
+  }
+  if (__lane_id() == 0) {
+    s_c[threadIdx.x >> 6] = c;
+    s_d[threadIdx.x >> 6] = d;
+  }
+  __syncthreads();  // also orders every thread's read of the level before its update
+  for (uint32_t i = threadIdx.x; i < 64 * NW; i += blockDim.x) tab[i] = (uint16_t)level;
+  if (threadIdx.x == 0) {
+    const unsigned long long cnt = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    const unsigned long long deg = s_d[0] + s_d[1] + s_d[2] + s_d[3];
+    ctr[1] = cnt;
+    ctr[2] = deg;
+    ctr[3] = 0;
+#pragma unroll
+    for (int j = 0; j < NW; j++) ctr[progress_slot(j)] = 0;
+    if (level > kMaxLevelValue) {
+      ctl[kCtlOverflow] = 1u;
+      ctl[kCtlDone] = 1u;
+    } else {
+      const uint32_t push = sym && (double)(cnt + deg) <= thr ? 1u : 0u;
+      ctl[kCtlPush] = push;
+      ctl[kCtlLevel] = level + 1;
+      ctl[kCtlLevels] += 1u;
+      ctl[kCtlPushLevels] += push;
+    }
+  }
+}
+
+// Ends the traversal after a level that reached nothing.
+template <int NW>
+__global__ void k_msd_end(uint32_t *__restrict__ ctl, const unsigned long long *__restrict__ ctr) {
+  if (ctl[kCtlDone]) return;
+  unsigned long long p = 0;
+#pragma unroll
+  for (int j = 0; j < NW; j++) p |= ctr[progress_slot(j)];
+  if (!p) ctl[kCtlDone] = 1u;
+}
+
+// Out-degrees of the queue (length *qn), grid-stride.
+__global__ void k_msd_qdeg(const uint32_t *__restrict__ q, const unsigned long long *__restrict__ qn,
+                           const uint32_t *__restrict__ ptr, uint32_t *__restrict__ deg,
+                           const uint32_t *__restrict__ ctl) {
+  if (ms_skip(ctl, 1)) return;
+  const uint32_t n = (uint32_t)*qn;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    deg[i] = ptr[q[i] + 1] - ptr[q[i]];
+}
+
+// Exclusive scan of a[0..*qn) in place, a[*qn] = total, with a fixed grid:
+// kScanParts slices of ceil(qn / kScanParts) entries; slice sums, their scan,
+// then each slice rescanned from its offset.
+constexpr uint32_t kScanParts = 256;
+
+__device__ __forceinline__ void scan_slice(uint32_t qn, uint32_t &lo, uint32_t &hi) {
+  const uint32_t per = (uint32_t)(((uint64_t)qn + kScanParts - 1) / kScanParts);
+  lo = (uint32_t)min((uint64_t)qn, (uint64_t)blockIdx.x * per);
+  hi = (uint32_t)min((uint64_t)qn, (uint64_t)lo + per);
+}
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t x, uint32_t *s_w) {
+  x = eg_wave_sum_u32(x);
+  if (__lane_id() == 0) s_w[threadIdx.x >> 6] = x;
+  __syncthreads();
+  const uint32_t t = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_msd_scan_a(const uint32_t *__restrict__ a,
+                                                       const unsigned long long *__restrict__ qn,
+                                                       uint32_t *__restrict__ part,
+                                                       const uint32_t *__restrict__ ctl) {
+  __shared__ uint32_t s_w[kWaves];
+  if (ms_skip(ctl, 1)) return;
+  uint32_t lo, hi;
+  scan_slice((uint32_t)*qn, lo, hi);
+  uint32_t x = 0;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) x += a[i];
+  x = block_sum_u32(x, s_w);
+  if (threadIdx.x == 0) part[blockIdx.x] = x;
+}
+
+__global__ __launch_bounds__(kScanParts) void k_msd_scan_b(uint32_t *__restrict__ a,
+                                                           const unsigned long long *__restrict__ qn,
+                                                           uint32_t *__restrict__ part,
+                                                           const uint32_t *__restrict__ ctl) {
+  __shared__ uint32_t s_w[kScanParts / 64];
+  if (ms_skip(ctl, 1)) return;
+  const uint32_t t = threadIdx.x;
+  const uint32_t x = part[t];
+  uint32_t inc = x;
+  const int lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) s_w[t >> 6] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t w = 0; w < (t >> 6); w++) off += s_w[w];
+  __syncthreads();
+  part[t] = off + inc - x;
+  if (t == kScanParts - 1) a[(uint32_t)*qn] = off + inc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_msd_scan_c(uint32_t *__restrict__ a,
+                                                       const unsigned long long *__restrict__ qn,
+                                                       const uint32_t *__restrict__ part,
+                                                       const uint32_t *__restrict__ ctl) {
+  __shared__ uint32_t s_w[kWaves];
+  if (ms_skip(ctl, 1)) return;
+  uint32_t lo, hi;
+  scan_slice((uint32_t)*qn, lo, hi);
+  uint32_t run = part[blockIdx.x];
+  const int lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint32_t base = lo; base < hi; base += kBlock) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t x = i < hi ? a[i] : 0u;
+    uint32_t inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t w = 0; w < wave; w++) off += s_w[w];
+    const uint32_t total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    __syncthreads();
+    if (i < hi) a[i] = run + off + inc - x;
+    run += total;
+  }
 }
 
 __global__ void k_ms_extract(uint32_t n, uint32_t k, uint32_t lane0,
@@ -976,7 +1212,7 @@ int eg_batch_view(eg_graph *g, BatchView *v) {
 }
 
 int eg_ms_alloc(eg_graph *g, MsState *st, uint32_t nw) {
-  if (nw != 1 && nw != 2) return eg_fail(EG_EINVAL, "MS-BFS lane words must be 1 or 2");
+  if (nw != 1 && nw != 2 && nw != 4) return eg_fail(EG_EINVAL, "MS-BFS lane words must be 1, 2 or 4");
   const uint32_t n = g->n;
   st->nw = nw;
   const uint64_t nn = (uint64_t)n * nw;
@@ -993,9 +1229,13 @@ int eg_ms_alloc(eg_graph *g, MsState *st, uint32_t nw) {
   EG_TRY(eg_dalloc_t(&st->ctr, 8 + 2 * (uint64_t)kStatBlocks + 64 * 64 * nw + 64 * nw,
                      "ms counters"));
   EG_TRY(eg_dalloc_t(&st->lvl_tab, 64 * nw, "ms level table"));
-  EG_TRY(eg_dalloc_t(&st->roots, 2 * 128, "ms roots"));
-  EG_TRY(eg_dalloc_t(reinterpret_cast<uint8_t **>(&st->scan_tmp),
-                     eg_scan_tmp_bytes((uint64_t)n + 64), "ms scan workspace"));
+  EG_TRY(eg_dalloc_t(&st->roots, 2 * kMaxLanes, "ms roots"));
+  EG_TRY(eg_dalloc_t(&st->ctl, kCtlWords, "ms control words"));
+  // the host-driven queue scan, or the kScanParts slice sums of the
+  // device-driven one
+  const size_t scan_bytes = std::max<size_t>(eg_scan_tmp_bytes((uint64_t)n + 64), 4 * kScanParts);
+  EG_TRY(eg_dalloc_t(reinterpret_cast<uint8_t **>(&st->scan_tmp), scan_bytes,
+                     "ms scan workspace"));
   if (hipHostMalloc(reinterpret_cast<void **>(&st->h), (8 + 64 * nw) * 8, hipHostMallocDefault) !=
       hipSuccess)
     return eg_fail(EG_ENOMEM, "pinned counters");
@@ -1014,6 +1254,9 @@ void eg_ms_free(MsState *st) {
   eg_dfree_t(&st->ctr);
   eg_dfree_t(&st->lvl_tab);
   eg_dfree_t(&st->roots);
+  eg_dfree_t(&st->ctl);
+  if (st->exec) (void)hipGraphExecDestroy(st->exec);
+  st->exec = nullptr;
   uint8_t *t = reinterpret_cast<uint8_t *>(st->scan_tmp);
   eg_dfree_t(&t);
   st->scan_tmp = nullptr;
@@ -1031,7 +1274,10 @@ static Lanes<NW> to_lanes(const uint64_t *w) {
 int eg_ms_reset(eg_graph *g, MsState *st) {
   const uint32_t n = g->n;
   const uint64_t count = (uint64_t)n * 16 * st->nw;
-  if (st->nw == 2)
+  if (st->nw == 4)
+    hipLaunchKernelGGL(k_ms_reset<4>, dim3(stride_grid(count)), dim3(256), 0, st->stream, n,
+                       st->seen, st->visit, st->nvisit, st->pend, st->lvl);
+  else if (st->nw == 2)
     hipLaunchKernelGGL(k_ms_reset<2>, dim3(stride_grid(count)), dim3(256), 0, st->stream, n,
                        st->seen, st->visit, st->nvisit, st->pend, st->lvl);
   else
@@ -1043,6 +1289,17 @@ int eg_ms_reset(eg_graph *g, MsState *st) {
   return EG_OK;
 }
 
+// Long-row segment ORs of a pull level (nw words per segment), allocated
+// outside any stream capture.
+static int ms_seg_or(MsState *st, const PullPlan &pl) {
+  if (st->seg_cap < pl.nseg || !st->seg_or) {
+    eg_dfree_t(&st->seg_or);
+    EG_TRY(eg_dalloc_t(&st->seg_or, (uint64_t)(pl.nseg ? pl.nseg : 1) * st->nw, "ms segment ors"));
+    st->seg_cap = pl.nseg;
+  }
+  return EG_OK;
+}
+
 // Frontier of visit: statistics, and the queue when the next level may push.
 template <int NW>
 static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *push) {
@@ -1050,7 +1307,8 @@ static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *pu
   hipStream_t s = st->stream;
   unsigned long long *part = st->ctr + 8;
   const uint32_t nb = blocks_for(n, kBlock) < kStatBlocks ? blocks_for(n, kBlock) : kStatBlocks;
-  hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part);
+  hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part,
+                     (const uint32_t *)nullptr);
   EG_LAUNCHED();
   hipLaunchKernelGGL(k_ms_stats_sum, dim3(1), dim3(kBlock), 0, s, part, nb, st->ctr);
   EG_LAUNCHED();
@@ -1064,7 +1322,7 @@ static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *pu
   if (*push && cnt) {
     EG_HIP(hipMemsetAsync(st->ctr + 3, 0, 8, s));
     hipLaunchKernelGGL(k_ms_compact<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr,
-                       st->queue, st->ctr + 3);
+                       st->queue, st->ctr + 3, (const uint32_t *)nullptr);
     EG_LAUNCHED();
     st->qlen = (uint32_t)cnt;
   }
@@ -1078,16 +1336,19 @@ int eg_ms_seed(eg_graph *g, const BatchView &bv, MsState *st, uint32_t k, const 
   for (uint32_t i = 0; i < k; i++)
     if (lanes[i] >= 64 * st->nw) return eg_fail(EG_EINVAL, "lane %u out of range", lanes[i]);
   hipStream_t s = st->stream;
-  uint32_t h[256];
+  uint32_t h[2 * kMaxLanes];
   memcpy(h, lanes, k * 4);
-  memcpy(h + 128, roots, k * 4);
+  memcpy(h + kMaxLanes, roots, k * 4);
   EG_HIP(hipMemcpyAsync(st->roots, h, sizeof(h), hipMemcpyHostToDevice, s));
-  if (st->nw == 2)
-    hipLaunchKernelGGL(k_ms_seed<2>, dim3(1), dim3(128), 0, s, g->n, k, st->roots,
-                       st->roots + 128, bv.perm, st->seen, st->visit, st->lvl);
+  if (st->nw == 4)
+    hipLaunchKernelGGL(k_ms_seed<4>, dim3(1), dim3(kMaxLanes), 0, s, g->n, k, st->roots,
+                       st->roots + kMaxLanes, bv.perm, st->seen, st->visit, st->lvl);
+  else if (st->nw == 2)
+    hipLaunchKernelGGL(k_ms_seed<2>, dim3(1), dim3(kMaxLanes), 0, s, g->n, k, st->roots,
+                       st->roots + kMaxLanes, bv.perm, st->seen, st->visit, st->lvl);
   else
-    hipLaunchKernelGGL(k_ms_seed<1>, dim3(1), dim3(128), 0, s, g->n, k, st->roots,
-                       st->roots + 128, bv.perm, st->seen, st->visit, st->lvl);
+    hipLaunchKernelGGL(k_ms_seed<1>, dim3(1), dim3(kMaxLanes), 0, s, g->n, k, st->roots,
+                       st->roots + kMaxLanes, bv.perm, st->seen, st->visit, st->lvl);
   EG_LAUNCHED();
   EG_HIP(hipStreamSynchronize(s));  // h is on the stack
   return EG_OK;
@@ -1101,6 +1362,7 @@ static int ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_
   const Lanes<NW> active = to_lanes<NW>(active_w);
   bool push = false;
   EG_TRY(frontier_prep<NW>(g, bv, st, &push));
+  st->pushed = push;
   EG_HIP(hipMemcpyAsync(st->lvl_tab, tab, 128 * NW, hipMemcpyHostToDevice, s));
   for (int j = 0; j < NW; j++) EG_HIP(hipMemsetAsync(st->ctr + progress_slot(j), 0, 8, s));
   const PullPlan &pl = *bv.pl;
@@ -1112,50 +1374,182 @@ static int ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_
       EG_TRY(eg_scan_excl(s, st->qoff, st->qoff, st->qlen, st->scan_tmp));
       EgProf prof(g, "msbfs_push", s);
       hipLaunchKernelGGL(k_ms_push<NW>, dim3(blocks_for(st->qdeg, kEdgesPerBlock)), dim3(kBlock),
-                         0, s, st->queue, st->qlen, st->qoff, bv.ptr, bv.ms_idx, st->visit,
-                         st->seen, st->pend, active);
+                         0, s, st->queue, st->qlen, (const unsigned long long *)nullptr, st->qoff,
+                         bv.ptr, bv.ms_idx, st->visit, st->seen, st->pend, active,
+                         (const uint32_t *)nullptr);
       EG_LAUNCHED();
     }
     const uint32_t nb = blocks_for(n, kBlock) < 4096 ? blocks_for(n, kBlock) : 4096;
     hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->pend, st->seen,
-                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr);
+                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr,
+                       (const uint32_t *)nullptr);
     EG_LAUNCHED();
   } else {
-    if (st->seg_cap < pl.nseg) {
-      eg_dfree_t(&st->seg_or);
-      EG_TRY(eg_dalloc_t(&st->seg_or, (uint64_t)pl.nseg * 2, "ms segment ors"));
-      st->seg_cap = pl.nseg;
-    }
+    EG_TRY(ms_seg_or(st, pl));
     {
       EgProf prof(g, "msbfs_pull", s);
       if (g->tune.ms_batch == 16)
         hipLaunchKernelGGL((k_ms_pull<16, NW>), dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s,
                            n, bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows,
-                           pl.nseg, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl,
-                           st->lvl_tab, active, st->ctr);
+                           pl.nseg, pl.nseg + pl.nchunks, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl,
+                           st->lvl_tab, active, st->ctr, (const uint32_t *)nullptr);
       else
         hipLaunchKernelGGL((k_ms_pull<8, NW>), dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
                            bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
-                           st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
-                           active, st->ctr);
+                           pl.nseg + pl.nchunks, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
+                           active, st->ctr, (const uint32_t *)nullptr);
       EG_LAUNCHED();
     }
     if (pl.nlong) {
       hipLaunchKernelGGL(k_ms_pull_finish<NW>, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock),
                          0, s, n, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen,
-                         st->nvisit, st->lvl, st->lvl_tab, active, st->ctr);
+                         st->nvisit, st->lvl, st->lvl_tab, active, st->ctr,
+                         (const uint32_t *)nullptr);
       EG_LAUNCHED();
     }
   }
   std::swap(st->visit, st->nvisit);
-  EG_HIP(hipMemcpyAsync(st->h, st->ctr, 40, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipMemcpyAsync(st->h, st->ctr, 64, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
   for (int j = 0; j < NW; j++) progress[j] = st->h[progress_slot(j)];
   return EG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Device-driven traversal: levels are enqueued in batches with no host wait
+// between them; the kernels decide each level's direction and the end of the
+// traversal themselves (control words), and two levels (one swap of the
+// frontier buffers) are recorded once as a hipGraph of kernel nodes and
+// replayed.  The host reads the control words once per batch of levels.
+// ---------------------------------------------------------------------------
+struct MsRunInfo {
+  uint32_t levels = 0, push_levels = 0, syncs = 0;
+  bool overflow = false;
+};
+
+template <int NW>
+static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<NW> &active) {
+  const uint32_t n = g->n;
+  hipStream_t s = st->stream;
+  unsigned long long *part = st->ctr + 8;
+  const uint32_t nb = std::min(blocks_for(n, kBlock), kStatBlocks);
+  const PullPlan &pl = *bv.pl;
+  uint32_t *spart = reinterpret_cast<uint32_t *>(st->scan_tmp);
+  const unsigned long long *qn = st->ctr + 3;
+  hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part,
+                     st->ctl);
+  hipLaunchKernelGGL(k_msd_decide<NW>, dim3(1), dim3(kBlock), 0, s, part, nb,
+                     (double)g->m / 20.0, bv.sym ? 1 : 0, st->ctl, st->ctr, st->lvl_tab);
+  // push: the frontier's vertices with out-edges, their degree scan, the
+  // edge push (a push level's edges are at most m / 20) and the apply pass
+  hipLaunchKernelGGL(k_ms_compact<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr,
+                     st->queue, st->ctr + 3, st->ctl);
+  hipLaunchKernelGGL(k_msd_qdeg, dim3(256), dim3(256), 0, s, st->queue, qn, bv.ptr, st->qoff,
+                     st->ctl);
+  hipLaunchKernelGGL(k_msd_scan_a, dim3(kScanParts), dim3(kBlock), 0, s, st->qoff, qn, spart,
+                     st->ctl);
+  hipLaunchKernelGGL(k_msd_scan_b, dim3(1), dim3(kScanParts), 0, s, st->qoff, qn, spart, st->ctl);
+  hipLaunchKernelGGL(k_msd_scan_c, dim3(kScanParts), dim3(kBlock), 0, s, st->qoff, qn, spart,
+                     st->ctl);
+  const uint32_t pgrid = std::min<uint32_t>(blocks_for(g->m / 20 + 1, kEdgesPerBlock), kDevGrid);
+  hipLaunchKernelGGL(k_ms_push<NW>, dim3(pgrid), dim3(kBlock), 0, s, st->queue, 0u, qn, st->qoff,
+                     bv.ptr, bv.ms_idx, st->visit, st->seen, st->pend, active, st->ctl);
+  const uint32_t nba = std::min(blocks_for(n, kBlock), kDevGrid);
+  hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nba), dim3(kBlock), 0, s, n, st->pend, st->seen,
+                     st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl);
+  // pull over the plan
+  const uint32_t nitems = pl.nseg + pl.nchunks;
+  const uint32_t pull_grid = std::max(1u, std::min(nitems, kDevPullGrid));
+  if (g->tune.ms_batch == 16)
+    hipLaunchKernelGGL((k_ms_pull<16, NW>), dim3(pull_grid), dim3(kBlock), 0, s, n,
+                       bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
+                       nitems, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab, active,
+                       st->ctr, st->ctl);
+  else
+    hipLaunchKernelGGL((k_ms_pull<8, NW>), dim3(pull_grid), dim3(kBlock), 0, s, n,
+                       bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
+                       nitems, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
+                       active, st->ctr, st->ctl);
+  if (pl.nlong)
+    hipLaunchKernelGGL(k_ms_pull_finish<NW>, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock), 0,
+                       s, n, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen,
+                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl);
+  hipLaunchKernelGGL(k_msd_end<NW>, dim3(1), dim3(1), 0, s, st->ctl, st->ctr);
+  EG_LAUNCHED();
+  std::swap(st->visit, st->nvisit);
+  return EG_OK;
+}
+
+// Records two levels into st->exec unless the recording already there was
+// made with the same plan, buffers, lanes and shapes.
+template <int NW>
+static int msd_graph(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<NW> &active) {
+  uint64_t key[6] = {reinterpret_cast<uint64_t>(bv.pl), reinterpret_cast<uint64_t>(st->visit),
+                     reinterpret_cast<uint64_t>(bv.ms_idx), (uint64_t)g->tune.ms_batch, 0, 0};
+  for (int j = 0; j < NW; j++) key[4] ^= active.w[j] * (0x9E3779B97F4A7C15ull + 2 * j);
+  key[5] = (uint64_t)NW;
+  if (st->exec && memcmp(key, st->exec_key, sizeof(key)) == 0) return EG_OK;
+  if (st->exec) (void)hipGraphExecDestroy(st->exec);
+  st->exec = nullptr;
+  hipStream_t s = st->stream;
+  hipGraph_t graph = nullptr;
+  EG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = msd_level<NW>(g, bv, st, active);
+  if (rc == EG_OK) rc = msd_level<NW>(g, bv, st, active);
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (rc == EG_OK && ec != hipSuccess)
+    rc = eg_fail(EG_EDEVICE, "stream capture failed: %s", hipGetErrorString(ec));
+  if (rc == EG_OK && hipGraphInstantiate(&st->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    st->exec = nullptr;
+    rc = eg_fail(EG_EDEVICE, "graph instantiation failed");
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc == EG_OK) memcpy(st->exec_key, key, sizeof(key));
+  return rc;
+}
+
+// One traversal of the seeded lanes to its end (or to the u16 level store's
+// limit: info->overflow).
+template <int NW>
+static int msd_run(eg_graph *g, const BatchView &bv, MsState *st, const uint64_t *active_w,
+                   MsRunInfo *info) {
+  hipStream_t s = st->stream;
+  const Lanes<NW> active = to_lanes<NW>(active_w);
+  EG_TRY(ms_seg_or(st, *bv.pl));
+  const bool graphs = g->tune.graphs != 0;
+  if (graphs) EG_TRY(msd_graph<NW>(g, bv, st, active));
+  hipLaunchKernelGGL(k_msd_init, dim3(1), dim3(64), 0, s, st->ctl);
+  EG_LAUNCHED();
+  // levels per batch: the last traversal's depth first (rounded up to a swap
+  // cycle), then 4, 8, 16, 32 at a time (a few idle levels at the end of a
+  // deep traversal against one host wait per batch)
+  uint32_t batch = std::max<uint32_t>(4u, (st->depth_hint + 1) & ~1u);
+  const uint32_t *h = reinterpret_cast<const uint32_t *>(st->h);
+  for (;;) {
+    for (uint32_t l = 0; l < batch; l += 2) {
+      if (graphs) {
+        EG_HIP(hipGraphLaunch(st->exec, s));
+      } else {
+        EG_TRY(msd_level<NW>(g, bv, st, active));
+        EG_TRY(msd_level<NW>(g, bv, st, active));
+      }
+    }
+    EG_HIP(hipMemcpyAsync(st->h, st->ctl, kCtlWords * 4, hipMemcpyDeviceToHost, s));
+    EG_HIP(hipStreamSynchronize(s));
+    info->syncs++;
+    if (h[kCtlDone]) break;
+    batch = batch < 4 ? 4 : std::min(2 * batch, 32u);
+  }
+  info->levels += h[kCtlLevels];
+  info->push_levels += h[kCtlPushLevels];
+  info->overflow = h[kCtlOverflow] != 0;
+  st->depth_hint = h[kCtlLevels];
+  return EG_OK;
+}
+
 int eg_ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_t *active,
                 const uint16_t *tab, uint64_t *progress) {
+  if (st->nw == 4) return ms_level<4>(g, bv, st, active, tab, progress);
   return st->nw == 2 ? ms_level<2>(g, bv, st, active, tab, progress)
                      : ms_level<1>(g, bv, st, active, tab, progress);
 }
@@ -1173,7 +1567,10 @@ int eg_ms_clear_lanes(eg_graph *g, MsState *st, const uint64_t *lanes) {
   uint64_t any = 0;
   for (uint32_t j = 0; j < st->nw; j++) any |= lanes[j];
   if (!any) return EG_OK;
-  if (st->nw == 2)
+  if (st->nw == 4)
+    hipLaunchKernelGGL(k_ms_clear_lanes<4>, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
+                       g->n, to_lanes<4>(lanes), st->seen, st->visit, st->lvl);
+  else if (st->nw == 2)
     hipLaunchKernelGGL(k_ms_clear_lanes<2>, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
                        g->n, to_lanes<2>(lanes), st->seen, st->visit, st->lvl);
   else
@@ -1189,7 +1586,7 @@ int eg_ms_reached(eg_graph *g, MsState *st, uint32_t k, const uint32_t *lanes,
   if (k > 64 * st->nw) return eg_fail(EG_EINVAL, "at most %u lanes", 64 * st->nw);
   hipStream_t s = st->stream;
   const uint32_t nb = blocks_for(g->n, kBlock) < 64 ? blocks_for(g->n, kBlock) : 64;
-  uint32_t h[128];
+  uint32_t h[kMaxLanes];
   memcpy(h, lanes, k * 4);
   EG_HIP(hipMemcpyAsync(st->roots, h, k * 4, hipMemcpyHostToDevice, s));
   unsigned long long *part = st->ctr + 8, *sums = st->ctr + 8 + 64 * 64 * st->nw;
@@ -1405,11 +1802,12 @@ int eg_ppr_clear_cols(eg_graph *g, PprState *st, uint64_t cols) {
 // The batched API.
 // ---------------------------------------------------------------------------
 struct BatchState {
-  MsState ms, ms2;  // one lane word (up to 64 sources), two (128 per traversal)
+  MsState ms, ms2, ms4;  // one, two or four lane words (64, 128, 256 lanes per traversal)
   PprState ppr;
-  bool ms_ok = false, ms2_ok = false, ppr_ok = false;
+  bool ms_ok = false, ms2_ok = false, ms4_ok = false, ppr_ok = false;
   uint32_t *ext = nullptr;  // k x n extraction buffer
   uint64_t ext_cap = 0;
+  eg_bfs_batch_info last{};  // of the last eg_bfs_batch call
 };
 
 void eg_batch_free(eg_graph *g) {
@@ -1417,6 +1815,7 @@ void eg_batch_free(eg_graph *g) {
   if (!b) return;
   eg_ms_free(&b->ms);
   eg_ms_free(&b->ms2);
+  eg_ms_free(&b->ms4);
   eg_ppr_free(&b->ppr);
   eg_dfree_t(&b->ext);
   delete b;
@@ -1437,6 +1836,27 @@ static int ensure_ext(BatchState *b, uint64_t words) {
   return EG_OK;
 }
 
+// The host-driven form of one traversal (the serving engine's level loop):
+// two host waits per level.
+template <int NW>
+static int ms_run_host(eg_graph *g, const BatchView &bv, MsState *st, const uint64_t *active,
+                       MsRunInfo *info) {
+  uint16_t tab[kMaxLanes];
+  for (int level = 2;; level++) {
+    if ((uint32_t)level > kMaxLevelValue) {
+      info->overflow = true;
+      return EG_OK;
+    }
+    for (uint32_t i = 0; i < kMaxLanes; i++) tab[i] = (uint16_t)level;
+    uint64_t progress[4] = {0, 0, 0, 0};
+    EG_TRY(ms_level<NW>(g, bv, st, active, tab, progress));
+    info->levels++;
+    info->push_levels += st->pushed ? 1u : 0u;
+    info->syncs += 2;
+    if ((progress[0] | progress[1] | progress[2] | progress[3]) == 0) return EG_OK;
+  }
+}
+
 extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint32_t *dist,
                             double *device_ms) {
   EG_TRY(eg_require_loaded(g));
@@ -1444,12 +1864,15 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
   if (k && !roots) return eg_fail(EG_EINVAL, "null roots");
   for (uint32_t i = 0; i < k; i++)
     if (roots[i] >= g->n) return eg_fail(EG_ERANGE, "root %u >= n", roots[i]);
-  // up to 64 sources in one traversal of one lane word, more in traversals
-  // of 128 lanes (two words: a gather fetches both with one 16-byte load)
-  const uint32_t nw = k > 64 ? 2u : 1u;
+  // lanes per traversal: the fewest lane words that hold k sources, at most
+  // ms_lanes (a 16-byte pull gather fetches two words at once)
+  const uint32_t cap = (uint32_t)g->tune.ms_lanes;
+  const uint32_t want = k > 128 ? 256u : k > 64 ? 128u : 64u;
+  const uint32_t per = want < cap ? want : cap;
+  const uint32_t nw = per / 64;
   BatchState *b = batch_state(g);
-  MsState *mst = nw == 2 ? &b->ms2 : &b->ms;
-  bool &ok = nw == 2 ? b->ms2_ok : b->ms_ok;
+  MsState *mst = nw == 4 ? &b->ms4 : nw == 2 ? &b->ms2 : &b->ms;
+  bool &ok = nw == 4 ? b->ms4_ok : nw == 2 ? b->ms2_ok : b->ms_ok;
   if (!ok) {
     EG_TRY(eg_ms_alloc(g, mst, nw));
     mst->stream = g->stream;
@@ -1459,34 +1882,35 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
   EG_TRY(eg_batch_view(g, &bv));
   hipStream_t s = g->stream;
   const uint32_t n = g->n;
-  const uint32_t per = 64 * nw;
   if (dist) EG_TRY(ensure_ext(b, (uint64_t)n * 64));
+  eg_bfs_batch_info info{};
+  info.lanes = per;
   double total_ms = 0.0;
-  uint32_t lanes[128];
-  for (uint32_t i = 0; i < 128; i++) lanes[i] = i;
+  uint32_t lanes[kMaxLanes];
+  for (uint32_t i = 0; i < kMaxLanes; i++) lanes[i] = i;
   for (uint32_t g0 = 0; g0 < k; g0 += per) {
     const uint32_t kk = k - g0 < per ? k - g0 : per;
     hipEvent_t e0, e1;
     EG_HIP(hipEventCreate(&e0));
     EG_HIP(hipEventCreate(&e1));
-    EG_HIP(hipEventRecord(e0, s));
     int rc = eg_ms_reset(g, mst);
     if (rc == EG_OK) rc = eg_ms_seed(g, bv, mst, kk, lanes, roots + g0);
-    uint64_t active[2];
-    for (uint32_t j = 0; j < 2; j++) {
+    uint64_t active[4];
+    for (uint32_t j = 0; j < 4; j++) {
       const uint32_t c = kk > 64 * j ? kk - 64 * j : 0;
       active[j] = c >= 64 ? ~0ull : ((1ull << c) - 1ull);
     }
-    uint16_t tab[128];
-    for (int level = 2; rc == EG_OK; level++) {
-      if (level >= 65535 || (uint64_t)level > (uint64_t)n + 1) {
-        rc = eg_fail(EG_ENOTSUP, "BFS depth exceeds the u16 level store");
-        break;
-      }
-      for (int i = 0; i < 128; i++) tab[i] = (uint16_t)level;
-      uint64_t progress[2] = {0, 0};
-      rc = eg_ms_level(g, bv, mst, active, tab, progress);
-      if (rc == EG_OK && (progress[0] | (nw == 2 ? progress[1] : 0ull)) == 0) break;
+    MsRunInfo run;
+    if (rc == EG_OK) rc = hipEventRecord(e0, s) == hipSuccess ? EG_OK : eg_fail(EG_EDEVICE, "event");
+    if (rc == EG_OK) {
+      if (g->tune.ms_device)
+        rc = nw == 4 ? msd_run<4>(g, bv, mst, active, &run)
+             : nw == 2 ? msd_run<2>(g, bv, mst, active, &run)
+                       : msd_run<1>(g, bv, mst, active, &run);
+      else
+        rc = nw == 4 ? ms_run_host<4>(g, bv, mst, active, &run)
+             : nw == 2 ? ms_run_host<2>(g, bv, mst, active, &run)
+                       : ms_run_host<1>(g, bv, mst, active, &run);
     }
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
@@ -1496,6 +1920,22 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     total_ms += ms;
+    info.traversals++;
+    info.levels += run.levels;
+    info.push_levels += run.push_levels;
+    info.host_syncs += run.syncs;
+    if (run.overflow) {
+      // deeper than the u16 level store: these sources run one by one on the
+      // single-source path, whose levels have no such limit
+      info.fallbacks++;
+      for (uint32_t i = 0; i < kk; i++) {
+        eg_bfs_stats bst;
+        EG_TRY(eg_bfs_host(g, roots[g0 + i], 0, 1, dist ? dist + (uint64_t)(g0 + i) * n : nullptr,
+                           nullptr, &bst));
+        total_ms += bst.device_ms;
+      }
+      continue;
+    }
     // 64 lanes at a time through the n x 64 extraction buffer
     for (uint32_t l0 = 0; dist && l0 < kk; l0 += 64) {
       const uint32_t kl = kk - l0 < 64 ? kk - l0 : 64;
@@ -1507,7 +1947,16 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
       EG_HIP(hipStreamSynchronize(s));
     }
   }
+  info.device_ms = total_ms;
+  b->last = info;
   if (device_ms) *device_ms = total_ms;
+  return EG_OK;
+}
+
+extern "C" int eg_bfs_batch_info_get(eg_graph *g, eg_bfs_batch_info *out) {
+  if (!g || !out) return eg_fail(EG_EINVAL, "null argument");
+  if (!g->batch) return eg_fail(EG_ESTATE, "no batched BFS has run on this graph");
+  *out = g->batch->last;
   return EG_OK;
 }
 

@@ -126,6 +126,8 @@ struct EgTune {
                          // items (one per CU) instead of one workgroup per item
   int pr_xcd = 0;        // binned scatter: every XCD walks a contiguous eighth of the
                          // chunk-ordered tiles (0) or runs from every eighth by turns (1)
+  int ms_lanes = 256;    // eg_bfs_batch: sources per multi-source traversal (64, 128, 256)
+  int ms_device = 1;     // eg_bfs_batch: device-driven levels (1) or host-driven (0)
 };
 
 struct eg_graph {

@@ -398,10 +398,20 @@ int bfs_level(eg_server *S, LaneGroup *G) {
     Query *qr = G->lane[lanes[i]];
     G->lane[lanes[i]] = nullptr;
     G->active &= ~(1ull << lanes[i]);
-    if ((deep >> lanes[i]) & 1ull)
-      complete(S, qr, EG_ENOTSUP, "BFS depth exceeds the 16-bit level store");
-    else
+    if ((deep >> lanes[i]) & 1ull) {
+      // deeper than the 16-bit level store: the query runs again on the
+      // single-source path, whose levels have no such limit, so its result
+      // does not depend on whether it was batched
+      eg_bfs_stats st{};
+      const int rc =
+          eg_bfs_host(S->g, qr->root, 0, 1, static_cast<uint32_t *>(qr->out), nullptr, &st);
+      if (rc == EG_EDEVICE) return rc;
+      qr->res.levels = (uint32_t)st.levels;
+      qr->res.reached = st.reached;
+      complete(S, qr, rc, rc ? eg_last_error() : nullptr);
+    } else {
       complete(S, qr, EG_OK, nullptr);
+    }
   }
   return EG_OK;
 }

@@ -68,6 +68,8 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "pr_hot")) return &g->tune.pr_hot;
   if (!strcmp(name, "pr_persist")) return &g->tune.pr_persist;
   if (!strcmp(name, "pr_xcd")) return &g->tune.pr_xcd;
+  if (!strcmp(name, "ms_lanes")) return &g->tune.ms_lanes;
+  if (!strcmp(name, "ms_device")) return &g->tune.ms_device;
   if (!strcmp(name, "bfs_order")) return &g->tune.bfs_order;
   if (!strcmp(name, "wcc_order")) return &g->tune.wcc_order;
   return nullptr;
@@ -129,6 +131,11 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "pr_persist takes 0 or 1");
   } else if (!strcmp(param, "pr_xcd")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "pr_xcd takes 0 or 1");
+  } else if (!strcmp(param, "ms_lanes")) {
+    if (value != 64 && value != 128 && value != 256)
+      return eg_fail(EG_EINVAL, "ms_lanes takes 64, 128 or 256");
+  } else if (!strcmp(param, "ms_device")) {
+    if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "ms_device takes 0 or 1");
   } else if (!strcmp(param, "wcc_order")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "wcc_order takes 0 or 1");
   } else if (!strcmp(param, "pr_binned")) {

@@ -218,10 +218,27 @@ int eg_sssp(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
             int32_t weights, uint32_t *dist, uint32_t *parent, eg_sssp_stats *stats);
 
 /* ---- batched form (no reference counterpart) ---- */
-/* k BFS traversals from roots[k] in one multi-source sweep.  dist (may be
- * NULL) is k x n, source-major, in the eg_bfs encoding. */
+/* k BFS traversals from roots[k] as multi-source sweeps of up to "ms_lanes"
+ * lanes (64, 128 or 256 sources per traversal).  Levels run device-driven:
+ * the kernels apply the bfs_simple.c:191-197 direction rule to the frontier
+ * of all lanes and end the traversal themselves, levels are enqueued in
+ * batches (replayed from a hipGraph) and the host waits once per batch, not
+ * per level.  dist (may be NULL) is k x n, source-major, in the eg_bfs
+ * encoding.  A traversal deeper than 65534 levels (the u16 level store) is
+ * rerun source by source on the single-source path, so no depth is refused. */
 int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots,
                  uint32_t *dist, double *device_ms);
+typedef struct {
+  uint32_t traversals;  /* multi-source traversals (groups of sources) */
+  uint32_t lanes;       /* sources per traversal */
+  uint32_t levels;      /* levels run, summed over the traversals */
+  uint32_t push_levels; /* of which pushed */
+  uint32_t host_syncs;  /* host waits on the device while levels ran */
+  uint32_t fallbacks;   /* traversals rerun source by source (depth past 65534) */
+  double device_ms;
+} eg_bfs_batch_info;
+/* Figures of the last eg_bfs_batch call on g. */
+int eg_bfs_batch_info_get(eg_graph *g, eg_bfs_batch_info *out);
 /* k personalized PageRank vectors as one FP32 SpMM over the in-edges.
  * Query q's seeds are seeds[seed_ptr[q] .. seed_ptr[q+1]); p = uniform over
  * its distinct seeds, x' = alpha p + (1-alpha) A^T D^-1 x.  out (may be NULL)
@@ -239,8 +256,11 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
 /* ---- kernel-shape autotuning (no reference counterpart) ----
  * Tunable shapes: "pull_batch" (pull PageRank / SpMV / PPR gather depth),
  * "spmm_depth" (batched PPR row gathers in flight), "ms_batch" (multi-source
- * BFS index pass depth), each 8 or 16; "pull_hot" (sources cached normally in
- * the degree-ordered pull, a vertex count, 0 = all); "pull_persist" (0, or 1-2
+ * BFS index pass depth), each 8 or 16; "ms_lanes" (sources per multi-source
+ * traversal: 64, 128 or 256, default 256); "ms_device" (1: device-driven
+ * batched levels, the default; 0: the host decides and waits every level);
+ * "pull_hot" (sources cached normally in the degree-ordered pull, a vertex
+ * count, 0 = all); "pull_persist" (0, or 1-2
  * persistent pull workgroups per CU keeping the highest-degree sources' values
  * in LDS); "pr_binned" (PageRank and single-vector PPR on adjacency lists:
  * 1 = binned two-phase
