@@ -16,7 +16,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libeg.so")
-BIN_DIR = os.path.join(_HERE, "bin")
+# EG_BIN_DIR selects another build of the drop-in drivers (the sanitizer build)
+BIN_DIR = os.environ.get("EG_BIN_DIR") or os.path.join(_HERE, "bin")
 
 EG_OK = 0
 EG_EINVAL = -1

@@ -9,7 +9,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# EG_ORACLE_SO selects another build of the same source (the sanitizer build)
+ORACLE_SO = os.environ.get("EG_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 _L = None
 
