@@ -224,6 +224,7 @@ def main():
         _, ms = g.bfs_batch(broots, download=False)
         wb = time.perf_counter() - w0
         batched_bfs_qps = len(broots) / wb
+        bb_info = g.bfs_batch_info()
         rng = np.random.default_rng(3)
         seeds = [eg.sample_roots(deg, int(rng.integers(1, 9)), seed=1000 + q)
                  for q in range(args.batch_ppr)]
@@ -241,11 +242,15 @@ def main():
             "batched_bfs_qps": round(batched_bfs_qps, 2),
             "plain_bfs_qps": round(plain_bfs_qps, 2),
             "batched_bfs_speedup": round(batched_bfs_qps / plain_bfs_qps, 2),
+            "batched_bfs_info": {k: (round(v, 3) if isinstance(v, float) else v)
+                                 for k, v in bb_info.items()},
             "batched_ppr_qps": round(batched_ppr_qps, 2),
             "plain_ppr_qps": round(plain_ppr_qps, 2),
             "batched_ppr_speedup": round(batched_ppr_qps / plain_ppr_qps, 2),
         })
-        log(f"batched: MS-BFS {batched_bfs_qps:.1f} q/s (plain {plain_bfs_qps:.1f}); "
+        log(f"batched: MS-BFS {batched_bfs_qps:.1f} q/s (plain {plain_bfs_qps:.1f}; "
+            f"{bb_info['traversals']} traversal(s) of {bb_info['lanes']} lanes, "
+            f"{bb_info['levels']} levels, {bb_info['host_syncs']} host waits); "
             f"PPR SpMM {batched_ppr_qps:.1f} q/s (plain {plain_ppr_qps:.1f})")
 
     if not args.skip_serve:
@@ -355,6 +360,13 @@ def main():
         "csr_build_gbs": round(csr_bytes / (info.build_ms * 1e-3) / 1e9, 1),
     }
     out.update(extra)
+    # one-time preparations of this graph (built on first use): device build
+    # of the layout, the degree-ordered copy, the binned plan, the BFS rows
+    prep = g.prep
+    out["prep"] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in prep.items()}
+    log("prep: layout {layout_ms:.1f} ms / {layout_bytes} B, ordered copy {order_ms:.1f} ms / "
+        "{order_bytes} B, binned plan {binned_plan_ms:.1f} ms / {binned_plan_bytes} B, BFS rows "
+        "{bfs_rows_ms:.1f} ms / {bfs_rows_bytes} B".format(**prep))
     g.close()
     if not args.skip_configs:
         out["configs"] = other_configs(eg, args.skip_c5, args.c5_requests, not args.skip_cpu)

@@ -62,7 +62,7 @@ ABI_SYMBOLS = (
     "eg_sync", "eg_ppr", "eg_autotune", "eg_tune_set", "eg_tune_get", "eg_pr_binned_geometry",
     "eg_serve_start", "eg_serve_submit_bfs", "eg_serve_submit_ppr",
     "eg_serve_wait", "eg_serve_stats_get", "eg_serve_stop", "eg_serve_replay",
-    "eg_csr_build_noalloc", "eg_bfs_batch_info_get",
+    "eg_csr_build_noalloc", "eg_bfs_batch_info_get", "eg_graph_prep_info",
 )
 
 
@@ -84,6 +84,13 @@ class GraphInfo(ctypes.Structure):
                 ("has_csr", ctypes.c_int32), ("has_csc", ctypes.c_int32),
                 ("has_grid", ctypes.c_int32), ("has_edgelist", ctypes.c_int32),
                 ("build_ms", ctypes.c_double), ("device_bytes", ctypes.c_uint64)]
+
+
+class PrepInfo(ctypes.Structure):
+    _fields_ = [("layout_ms", ctypes.c_double), ("order_ms", ctypes.c_double),
+                ("binned_plan_ms", ctypes.c_double), ("bfs_rows_ms", ctypes.c_double),
+                ("layout_bytes", ctypes.c_uint64), ("order_bytes", ctypes.c_uint64),
+                ("binned_plan_bytes", ctypes.c_uint64), ("bfs_rows_bytes", ctypes.c_uint64)]
 
 
 class GenParams(ctypes.Structure):
@@ -309,6 +316,14 @@ class Graph:
         i = GraphInfo()
         _check(lib().eg_graph_get_info(self._h, ctypes.byref(i)), "eg_graph_get_info")
         return i
+
+    @property
+    def prep(self) -> dict:
+        """One-time preparations built so far: ms and device bytes of the
+        layout, the degree-ordered copy, the binned plan, the BFS rows."""
+        i = PrepInfo()
+        _check(lib().eg_graph_prep_info(self._h, ctypes.byref(i)), "eg_graph_prep_info")
+        return _struct_dict(i)
 
     # -- layouts --
     def degrees(self):

@@ -868,6 +868,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   if (g->pb && g->pb->for_key == key && g->pb->tile_len == want_tile && g->pb->hot_req == want_hot)
     return EG_OK;
   eg_pb_free(g);
+  const double t0 = eg_wall_ms();
   const uint32_t n = g->n;
   const uint64_t m = g->m;
   hipStream_t s = g->stream;
@@ -1134,11 +1135,19 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
     return rc;
   }
   p->for_key = key;
+  g->prep_plan_ms = eg_wall_ms() - t0;
+  // offsets in both orders, value slots (+ bin-major weights), segment table,
+  // tiles, items, long rows and partials
+  g->prep_plan_bytes = 8ull * (m + 16) + (src.w ? 4ull * (m + 16) : 0ull) +
+                       8ull * p->nchunks * ((uint64_t)p->nbins + 1) + sizeof(PbTile) * p->ntiles +
+                       (sizeof(PbItem) + 16) * p->nitems + 16ull * p->nlong + 8ull * p->nparts;
   return EG_OK;
 }
 
-bool eg_pb_selected(const eg_graph *g) {
-  return g->tune.pr_binned > 0 || (g->tune.pr_binned < 0 && g->m >= (1ull << 22));
+bool eg_pb_selected(const eg_graph *g, int iterations, const void *key) {
+  if (g->tune.pr_binned >= 0) return g->tune.pr_binned > 0;
+  if (g->m < (1ull << 22)) return false;
+  return (g->pb && g->pb->for_key == key) || iterations >= kAutoBinnedIters;
 }
 
 // Fixed-point scale for row sums bounded by `bound` (finite; callers check):

@@ -16,6 +16,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <chrono>
 #include <thread>
 #include <vector>
 
@@ -71,6 +72,12 @@ int eg_scratch(eg_graph *g, size_t bytes, void **out) {
 }
 
 bool eg_has_in_edges(const eg_graph *g) { return g->in_ptr != nullptr; }
+
+double eg_wall_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
 
 int eg_require_loaded(const eg_graph *g) {
   if (!g) return eg_fail(EG_EINVAL, "null graph");
@@ -270,6 +277,8 @@ static void free_layout(eg_graph *g) {
   eg_batch_free(g);
   g->loaded = false;
   g->ndangling = 0xFFFFFFFFu;
+  g->prep_order_ms = g->prep_plan_ms = g->prep_bfs_ms = 0.0;
+  g->prep_order_bytes = g->prep_plan_bytes = g->prep_bfs_bytes = 0;
 }
 
 static void free_algo_state(eg_graph *g) {
@@ -338,6 +347,22 @@ extern "C" int eg_graph_get_info(const eg_graph *g, eg_graph_info *out) {
   if (g->el_src) b += 8 * m + (g->el_w ? 4 * m : 0);
   b += 8 * n;
   out->device_bytes = b;
+  return EG_OK;
+}
+
+extern "C" int eg_graph_prep_info(const eg_graph *g, eg_prep_info *out) {
+  if (!g || !out) return eg_fail(EG_EINVAL, "null argument");
+  eg_graph_info gi;
+  EG_TRY(eg_graph_get_info(g, &gi));
+  memset(out, 0, sizeof(*out));
+  out->layout_ms = g->build_ms;
+  out->order_ms = g->prep_order_ms;
+  out->binned_plan_ms = g->prep_plan_ms;
+  out->bfs_rows_ms = g->prep_bfs_ms;
+  out->layout_bytes = gi.device_bytes;
+  out->order_bytes = g->prep_order_bytes;
+  out->binned_plan_bytes = g->prep_plan_bytes;
+  out->bfs_rows_bytes = g->prep_bfs_bytes;
   return EG_OK;
 }
 

@@ -491,8 +491,8 @@ static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, dou
   if (grid && !g->cell_ptr) return eg_fail(EG_ENOTSUP, "load mode %d has no grid", g->load_mode);
   if (!grid && !g->el_src)
     return eg_fail(EG_ENOTSUP, "load mode %d has no edge array", g->load_mode);
-  if (eg_pb_selected(g)) return pagerank_binned_cells(g, iters, damping, init, uniform, rank_out,
-                                                      dh, lh, grid);
+  if (eg_pb_selected(g, iters, cells_source(g, grid).row))
+    return pagerank_binned_cells(g, iters, damping, init, uniform, rank_out, dh, lh, grid);
   GridCompute *c = grid_state(g, true);
   if (grid) EG_TRY(grid_build_compute(g, c));
   EG_TRY(pr_state(g, iters));
@@ -597,13 +597,14 @@ static int spmv_host(eg_graph *g, const float *x, float *y, bool grid) {
   // Binned kernels sum exactly; their scale needs a finite bound on |x|.
   double xmax = 0.0;
   bool finite = true;
-  if (eg_pb_selected(g))
+  const bool sel = eg_pb_selected(g, 1, cells_source(g, grid).row);
+  if (sel)
     for (uint32_t i = 0; i < n && finite; i++) {
       const double ax = fabs((double)x[i]);
       finite = std::isfinite(ax);
       xmax = std::max(xmax, ax);
     }
-  bool binned = eg_pb_selected(g) && finite;
+  bool binned = sel && finite;
   if (binned) {
     EG_TRY(eg_pb_build(g, cells_source(g, grid)));
     binned = eg_pb_spmv_exact(g, xmax);

@@ -197,6 +197,11 @@ struct eg_graph {
   EgTune tune;
   uint32_t *sssp_segmin = nullptr;  // SSSP pull: minimum per long-row segment
   uint32_t sssp_segmin_cap = 0;
+  // One-time preparations built on first use (eg_graph_prep_info): wall ms
+  // and device bytes of the degree-ordered copy, the binned plan and the
+  // hubs-first BFS rows.
+  double prep_order_ms = 0.0, prep_plan_ms = 0.0, prep_bfs_ms = 0.0;
+  uint64_t prep_order_bytes = 0, prep_plan_bytes = 0, prep_bfs_bytes = 0;
   float *ppr_base = nullptr;     // single-vector PPR: alpha p
   uint32_t *ppr_seeds = nullptr;
   uint32_t ppr_seeds_cap = 0;
@@ -222,9 +227,22 @@ int eg_pb_spmv(eg_graph *g, const float *x, float *y, double xmax, hipStream_t s
 // True when eg_pb_spmv can sum exactly over the built plan: finite weights
 // and |w| |x| below FLT_MAX (else the pull kernels take the product).
 bool eg_pb_spmv_exact(const eg_graph *g, double xmax);
-// True when PageRank on adjacency lists uses the binned kernels.
-bool eg_pb_selected(const eg_graph *g);
+// Automatic choices weigh the one-time preparations against the work asked
+// for (measured at R-MAT 24 -u: the degree-ordered copy costs about 8
+// pull iterations and saves about a quarter of each; the binned plan costs
+// about 80 and saves about 60% of each): an orientation's degree-ordered copy
+// is built for at least kAutoOrderIters iterations, a binned plan (pr_binned
+// = -1) for at least kAutoBinnedIters; once built, either is used.
+constexpr int kAutoOrderIters = 32;
+constexpr int kAutoBinnedIters = 128;
+// True when PageRank / SpMV / PPR over the entry source `key` (an
+// orientation's row pointers, or an edge list's rows) uses the binned kernels
+// for a run of `iterations` iterations.
+bool eg_pb_selected(const eg_graph *g, int iterations, const void *key);
 bool eg_reorder_enabled();
+// True when a run of `iterations` pull / binned iterations uses the
+// degree-ordered copy (built already, or worth building).
+bool eg_use_order(const eg_graph *g, int iterations);
 // Builds (on first use) and returns the degree-ordered pull copy.
 int eg_ordered(eg_graph *g, EgOrdered **out);
 // dst[i] = src[perm[i]] for 32-bit vectors of length n (on the graph stream).
@@ -278,6 +296,9 @@ int eg_ppr1_output(eg_graph *g, Ppr1 *st, float *out);
 // eg_bfs without the ownership check (the serving engine's plain path).
 int eg_bfs_host(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
                 uint32_t *dist, uint32_t *parent, eg_bfs_stats *stats);
+
+// Wall-clock milliseconds since an earlier eg_wall_ms() (a steady clock).
+double eg_wall_ms();
 
 // Profiling scope around one launch.
 struct EgProf {

@@ -73,6 +73,11 @@ bool eg_reorder_enabled() {
   return cached == 1;
 }
 
+bool eg_use_order(const eg_graph *g, int iterations) {
+  if (!eg_reorder_enabled()) return false;
+  return (g->ord && g->ord->built) || iterations >= kAutoOrderIters;
+}
+
 void eg_order_free(eg_graph *g) {
   EgOrdered *o = g->ord;
   if (!o) return;
@@ -120,6 +125,7 @@ int eg_ordered(eg_graph *g, EgOrdered **out) {
   }
   (void)unused;
   eg_order_free(g);
+  const double t0 = eg_wall_ms();
   EgOrdered *o = new EgOrdered();
   g->ord = o;
   const uint32_t n = g->n;
@@ -189,6 +195,10 @@ int eg_ordered(eg_graph *g, EgOrdered **out) {
   }
   EG_TRY(eg_plan_build(g, &o->plan, o->ptr));
   o->built = true;
+  g->prep_order_ms = eg_wall_ms() - t0;
+  // permutations, row pointers, entries (+ weights), degrees, plan chunks
+  g->prep_order_bytes = 4ull * 3 * n + 4ull * (n + 1) + 4ull * m * (w ? 2 : 1) +
+                        16ull * o->plan.nchunks + 24ull * o->plan.nseg;
   *out = o;
   return EG_OK;
 }

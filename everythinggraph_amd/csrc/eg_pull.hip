@@ -642,7 +642,7 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
   const uint32_t *ptr, *idx, *deg = g->out_deg, *unperm = nullptr;
   const float *w;
   PullPlan *pl;
-  if (eg_reorder_enabled()) {
+  if (eg_use_order(g, iters)) {
     EgOrdered *o;
     EG_TRY(eg_ordered(g, &o));
     ptr = o->ptr;
@@ -659,7 +659,7 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
   // Binned form: every row sum is at most the total rank mass, which never
   // exceeds max(init * n, 1) (each iteration maps mass R to at most
   // (1 - d) + d R); the margin covers the FP32 rounding of the iterates.
-  const bool binned = eg_pb_selected(g);
+  const bool binned = eg_pb_selected(g, iters, ptr);
   int S = 0;
   if (binned) {
     PbSource src;
@@ -727,7 +727,7 @@ __global__ void k_ppr1_seed(const uint32_t *__restrict__ seeds, uint32_t ns,
 static int ppr1_view(eg_graph *g, Ppr1 *st, int max_iter) {
   st->deg = g->out_deg;
   st->unperm = nullptr;
-  if (eg_reorder_enabled()) {
+  if (eg_use_order(g, max_iter)) {
     EgOrdered *o;
     EG_TRY(eg_ordered(g, &o));
     st->ptr = o->ptr;
@@ -741,7 +741,7 @@ static int ppr1_view(eg_graph *g, Ppr1 *st, int max_iter) {
   }
   EG_TRY(eg_pr_state(g, max_iter));
   // the iterate is a probability vector: row sums stay below 1
-  st->binned = eg_pb_selected(g);
+  st->binned = eg_pb_selected(g, max_iter, st->ptr);
   if (st->binned) {
     PbSource src;
     src.ptr = st->ptr;
@@ -892,7 +892,7 @@ extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
   const float *w;
   PullPlan *pl;
   EgOrdered *o = nullptr;
-  if (eg_reorder_enabled()) {
+  if (eg_use_order(g, 1)) {
     EG_TRY(eg_ordered(g, &o));
     ptr = o->ptr;
     idx = o->idx;
@@ -914,13 +914,14 @@ extern "C" int eg_spmv(eg_graph *g, int32_t layout, const float *x, float *y) {
   // Binned kernels sum exactly; their scale needs a finite bound on |x|.
   double xmax = 0.0;
   bool finite = true;
-  if (eg_pb_selected(g))
+  const bool sel = eg_pb_selected(g, 1, ptr);
+  if (sel)
     for (uint32_t i = 0; i < n && finite; i++) {
       const double ax = fabs((double)x[i]);
       finite = std::isfinite(ax);
       xmax = std::max(xmax, ax);
     }
-  bool binned = eg_pb_selected(g) && finite;
+  bool binned = sel && finite;
   if (binned) {
     PbSource src;
     src.ptr = ptr;

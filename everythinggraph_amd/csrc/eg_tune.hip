@@ -197,6 +197,11 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
     if (!adj) return eg_fail(EG_ENOTSUP, "pull tuning needs adjacency lists");
     eg_pr_opts o = {0.85, 5, EG_PR_DANGLING_DROP, 0.0, EG_LAYOUT_ADJ};
     g->tune.pr_binned = 0;  // the pull shapes are timed on the pull kernel
+    // tuning serves long runs, which use the degree-ordered copy: time that
+    if (eg_reorder_enabled()) {
+      EgOrdered *ord;
+      EG_TRY(eg_ordered(g, &ord));
+    }
     for (int v : variants) {
       g->tune.pull_batch = v;
       double ms;

@@ -141,6 +141,19 @@ int eg_generate_edges(const eg_gen_params *p, uint32_t n, uint32_t *src, uint32_
 int eg_graph_free(eg_graph *g);
 int eg_graph_get_info(const eg_graph *g, eg_graph_info *out);
 
+/* One-time preparations of a loaded graph and their device memory (no
+ * reference counterpart): the layout build of the load mode, and what the
+ * kernels build on first use -- the degree-ordered copy of the pull
+ * orientation, the binned PageRank / SpMV plan and the hubs-first BFS rows --
+ * with the wall milliseconds each took (0 until built). */
+typedef struct {
+  double layout_ms;       /* device time of the layout build (eg_graph_info.build_ms) */
+  double order_ms, binned_plan_ms, bfs_rows_ms;
+  uint64_t layout_bytes;  /* eg_graph_info.device_bytes */
+  uint64_t order_bytes, binned_plan_bytes, bfs_rows_bytes;
+} eg_prep_info;
+int eg_graph_prep_info(const eg_graph *g, eg_prep_info *out);
+
 /* ---- layout downloads (parity checks, drop-in globals) ---- */
 /* Out-degree (nb_out_edges) and in-degree (nb_in_edges) of random.h:87-93. */
 int eg_graph_get_degrees(eg_graph *g, uint32_t *out_deg, uint32_t *in_deg);

@@ -388,7 +388,7 @@ x = np.linspace(-1, 1, n).astype(np.float32)
 out = {}
 with eg.Graph(n, 2, weighted=True) as g:
     g.load_edges(s, d, w)
-    out['pr'] = g.pagerank(0.85, 20)[0]
+    out['pr'] = g.pagerank(0.85, 40)[0]  # long enough to build the degree-ordered copy
     out['spmv'] = g.spmv(x)
 with eg.Graph(n, 2, undirected=True) as g:
     g.load_edges(s, d)

@@ -141,17 +141,17 @@ for und in (False, True):
     with eg.Graph(n, 2, undirected=und) as g:
         g.load_edges(s, d)
         g.tune_set("pr_binned", 1)
-        out['pr%d' % und] = g.pagerank(0.85, 15)[0]
-        out['uni%d' % und] = g.pagerank(0.85, 15, dangling="uniform")[0]
+        out['pr%d' % und] = g.pagerank(0.85, 40)[0]
+        out['uni%d' % und] = g.pagerank(0.85, 40, dangling="uniform")[0]
         geo = g.binned_geometry()
         out['geo%d' % und] = np.array([geo[k] for k in ("bins", "chunks", "tiles", "items",
                                                        "long_rows")])
         g.tune_set("pr_hot", 0)      # every source scattered
-        out['cold%d' % und] = g.pagerank(0.85, 15)[0]
+        out['cold%d' % und] = g.pagerank(0.85, 40)[0]
         g.tune_set("pr_hot", 4096)   # half hot, half scattered
-        out['half%d' % und] = g.pagerank(0.85, 15)[0]
+        out['half%d' % und] = g.pagerank(0.85, 40)[0]
         g.tune_set("pr_xcd", 1)      # scatter tiles dealt to XCDs in contiguous ranges
-        out['xcd%d' % und] = g.pagerank(0.85, 15)[0]
+        out['xcd%d' % und] = g.pagerank(0.85, 40)[0]
 np.savez(sys.argv[2], **out)
 """
 
@@ -160,7 +160,8 @@ def test_binned_independent_of_cut_and_order(tmp_path):
     """The same ranks, bit for bit, with the default bins and chunks, with
     tiny ones (40 hot sources, long rows in parts, tiles cut by length and by
     bins), with all, half or none of the sources hot, and with or without the
-    degree-ordered copy, and in either scatter tile order."""
+    degree-ordered copy (built for runs of 32 iterations or more), and in
+    either scatter tile order."""
     script = tmp_path / "run.py"
     script.write_text(_GEOMETRY_SCRIPT)
     res = {}
@@ -345,3 +346,55 @@ def test_batched_ppr_static_tail():
                 for q, sd in enumerate(seeds):
                     ref = O.ppr(n, cp, row, deg, sd, 0.15, it)
                     assert np.abs(out[q] - ref).sum() <= PR_L1_TOL, (graphs, it, q)
+
+
+def test_auto_choice_weighs_preparation():
+    """pr_binned = -1 builds the binned plan (and the degree-ordered copy)
+    only for runs long enough to repay it: the reference's 10 iterations
+    (pagerank_simple.c:115) on a graph of 2^26 entries build neither and give
+    the pull kernel's bits; a 200-iteration run builds both, and from then on
+    the plan is used."""
+    scale = 21
+    n = 1 << scale
+    with eg.Graph(n, eg.MODE_ADJ_SORT, undirected=True, symmetric=True) as g:
+        g.generate(eg.rmat_params(scale, 16, seed=1))
+        assert g.info.m >= 1 << 22
+        r10, _, _ = g.pagerank(0.85, 10)
+        p = g.prep
+        assert p["binned_plan_ms"] == 0 and p["order_ms"] == 0
+        g.tune_set("pr_binned", 0)
+        assert np.array_equal(r10, g.pagerank(0.85, 10)[0])
+        g.tune_set("pr_binned", -1)
+        g.pagerank(0.85, 200, download=False)
+        p = g.prep
+        assert p["binned_plan_ms"] > 0 and p["order_ms"] > 0
+        assert p["binned_plan_bytes"] > 8 * g.info.m
+        r = g.pagerank(0.85, 10)[0]
+        g.tune_set("pr_binned", 1)
+        assert np.array_equal(r, g.pagerank(0.85, 10)[0])
+
+
+def test_drop_in_pagerank_takes_the_pull_path(tmp_path):
+    """The drop-in pagerank_simple (load + the reference's 10 iterations from
+    0.15) on a graph past the binned size threshold runs the pull kernel: its
+    ranks equal pr_binned = 0's bit for bit."""
+    scale = 21
+    n = 1 << scale
+    s, d = O.gen_rmat(scale, 16 << scale, seed=2)
+    rec = np.empty((s.size, 2), np.uint32)
+    rec[:, 0], rec[:, 1] = s, d
+    path = tmp_path / "rmat21.bin"
+    path.write_bytes(rec.tobytes())
+    del rec
+    out = tmp_path / "rank.out"
+    exe = os.path.join(eg.BIN_DIR, "pagerank_simple")
+    print("driver: pagerank_simple on R-MAT 21 -u", flush=True)
+    r = subprocess.run([exe, "-f", str(path), "-n", str(n), "-m", "2", "-u"], capture_output=True,
+                       text=True, timeout=300, env={**os.environ, "EG_DUMP": str(out)})
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    got = np.fromfile(out, np.float32)
+    with eg.Graph(n, eg.MODE_ADJ_SORT, undirected=True) as g:
+        g.load_edges(s, d)
+        g.tune_set("pr_binned", 0)
+        ref, _, _ = g.pagerank(0.85, 10, init=0.15)
+    assert np.array_equal(got, ref)

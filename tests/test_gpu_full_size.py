@@ -6,8 +6,9 @@ that do not depend on size.
 * Layout: the degrees and the per-row sums of neighbour ids equal those of
   the oracle's generator output (a checksum of every row), row offsets are a
   prefix sum ending at m.
-* PageRank (20 iterations, the binned kernels that the bench times): L1 error
-  <= 1e-6 and max error <= 1e-6 of the largest rank against the FP64 oracle.
+* PageRank (20 iterations, the binned kernels that the bench times, and the
+  pull kernel): L1 error <= 1e-6 and max error <= 1e-6 of the largest rank
+  against the FP64 oracle.
 * SpMV with signed x: |y - y64| <= 1e-5 (|A| |x|) elementwise.
 * BFS: levels bit-exact against the oracle, parents a legal BFS tree, the
   Graph500 edge count; multi-source BFS over 256 roots (the batched
@@ -62,10 +63,14 @@ def test_full_size_layout(big):
     assert np.array_equal(row_sum, ref_sum.astype(np.uint64))
 
 
-def test_full_size_pagerank(big):
+@pytest.mark.parametrize("binned", [1, 0])
+def test_full_size_pagerank(big, binned):
+    """The binned kernels (what the bench times) and the pull kernel."""
     g, ptr, col, deg = big
-    say("pagerank, 20 iterations")
+    say(f"pagerank, 20 iterations, pr_binned={binned}")
+    g.tune_set("pr_binned", binned)
     r, _, _ = g.pagerank(0.85, 20)
+    g.tune_set("pr_binned", -1)
     ref, _, _ = O.pagerank_pull(N, ptr, col, deg, 0.85, 20)
     err = np.abs(r.astype(np.float64) - ref)
     assert err.sum() <= 1e-6
