@@ -257,7 +257,9 @@ def main():
         # serving: the same Poisson stream (80% BFS, 20% PPR with 1-8 seeds,
         # tolerance 1e-6 L1, seed 4) through continuous batching and through
         # the plain one-query-at-a-time path
-        rates = args.serve_qps or [50.0, 100.0]
+        # offered rates below and past the plain path's saturation (about 190
+        # queries/s at R-MAT 24), then a burst: every request at once
+        rates = args.serve_qps or [50.0, 100.0, 200.0, SERVE_BURST_QPS]
         serving = []
         for qps in rates:
             srv = {}
@@ -285,6 +287,17 @@ def main():
                 "plain_p99_over_batched_p99": round(p["p99_ms"] / max(b["p99_ms"], 1e-9), 2),
             })
         extra["serving"] = serving
+        burst = [x for x in serving if x["offered_qps"] >= SERVE_BURST_QPS]
+        if burst:
+            b = burst[-1]
+            extra["serving_saturation"] = {
+                "offered": "burst (every request submitted at once)",
+                "requests": args.serve_requests,
+                "engine_qps": b["batched_qps"], "plain_qps": b["plain_qps"],
+                "engine_over_plain": round(b["batched_qps"] / max(b["plain_qps"], 1e-9), 2),
+            }
+            log(f"serving saturation: engine {b['batched_qps']:.1f} q/s, plain "
+                f"{b['plain_qps']:.1f} q/s (burst of {args.serve_requests})")
 
     cpu = None
     if not args.skip_cpu:
@@ -371,6 +384,9 @@ def main():
     if not args.skip_configs:
         out["configs"] = other_configs(eg, args.skip_c5, args.c5_requests, not args.skip_cpu)
     print(json.dumps(out), flush=True)
+
+
+SERVE_BURST_QPS = 1e5  # offered rate of the burst replay: every request at once
 
 
 def other_configs(eg, skip_c5=False, c5_requests=150, fp64=True) -> dict:
