@@ -1189,13 +1189,14 @@ static int bfs_edges(eg_graph *g, uint32_t root, bool want_parent, eg_bfs_stats 
 int eg_bfs_device(eg_graph *g, uint32_t root, int start_pull, int switch_mode, bool want_parent,
                   eg_bfs_stats *st) {
   EG_TRY(bfs_state(g));
+  if (g->wide) return eg_wide_bfs(g, root, start_pull, switch_mode, want_parent, st);
   if (g->out_ptr) return bfs_adjacency(g, root, start_pull, switch_mode, want_parent, st);
   return bfs_edges(g, root, want_parent, st);
 }
 
 extern "C" int eg_bfs(eg_graph *g, uint32_t root, int32_t start_pull, int32_t switch_mode,
                       uint32_t *dist, uint32_t *parent, eg_bfs_stats *stats) {
-  EG_TRY(eg_require_loaded(g));
+  EG_TRY(eg_require_loaded_wide_ok(g));
   return eg_bfs_host(g, root, start_pull, switch_mode, dist, parent, stats);
 }
 

@@ -7,6 +7,7 @@
 // the sorted keys (one binary search per vertex), the contract of the
 // reference's boundary detection + scanIBack(min) (init_all.c:675-697): no
 // degree is accumulated with atomics, so R-MAT hubs cost nothing extra.
+#include <functional>
 #include <mutex>
 
 #include "eg_internal.h"
@@ -84,6 +85,184 @@ static int build_adjacency(eg_graph *g) {
   // the record arrays are free now: e_src receives the sorted dst keys
   EG_TRY(sort_by(g, m, n, g->out_col, g->e_src, cols2(g->out_src, g->in_idx, g->out_w, g->in_w)));
   return eg_ptr_from_sorted(g->stream, g->e_src, m, n, g->in_ptr, g->in_deg);
+}
+
+// ---------------------------------------------------------------------------
+// Adjacency with 64-bit row offsets (graphs with 2^32 stored entries or more).
+// The records are cut, in input order, into parts of at most 2^31 entries;
+// each part is stably sorted by its key with the 32-bit radix sort and gives
+// per-row counts (run offsets).  Row offsets are the 64-bit exclusive scan of
+// the summed counts, and part p's entries of row v are placed after those of
+// the parts before it: the result is one stable sort of all records by key,
+// the order of init_adj_sort (init_all.c:629-697) at any size.
+// ---------------------------------------------------------------------------
+namespace {
+
+// ptr[v] = entries of row v over all parts, deg[v] the same in 32 bits
+// (flag set when a row holds 2^32 entries or more).
+__global__ void k_wide_counts(uint32_t n, uint32_t nparts, const uint32_t *__restrict__ pp,
+                              uint64_t *__restrict__ ptr, uint32_t *__restrict__ deg,
+                              uint32_t *__restrict__ flag) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    uint64_t d = 0;
+    for (uint32_t p = 0; p < nparts; p++) {
+      const uint32_t *q = pp + (uint64_t)p * ((uint64_t)n + 1);
+      d += q[v + 1] - q[v];
+    }
+    ptr[v] = d;
+    if (d > 0xFFFFFFFFull) {
+      atomicOr(flag, 1u);
+      d = 0xFFFFFFFFull;
+    }
+    deg[v] = (uint32_t)d;
+  }
+}
+
+// Places one sorted part: its j-th entry (key v) goes to base[v] + j - pp[v],
+// base[v] = row offset of v + entries of v in the parts before.
+__global__ void k_wide_place(uint64_t len, const uint32_t *__restrict__ sk,
+                             const uint32_t *__restrict__ sv, const float *__restrict__ sw,
+                             const uint32_t *__restrict__ pp, const uint64_t *__restrict__ base,
+                             uint32_t *__restrict__ idx, float *__restrict__ w,
+                             uint32_t *__restrict__ rows) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < len; j += stride) {
+    const uint32_t v = sk[j];
+    const uint64_t d = base[v] + (j - pp[v]);
+    idx[d] = sv[j];
+    if (w) w[d] = sw[j];
+    if (rows) rows[d] = v;
+  }
+}
+
+__global__ void k_wide_advance(uint32_t n, const uint32_t *__restrict__ pp,
+                               uint64_t *__restrict__ base) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+    base[v] += pp[v + 1] - pp[v];
+}
+
+uint32_t wide_grid(uint64_t count) {
+  const uint64_t b = (count + 255) / 256;
+  return b < 65536 ? (b ? (uint32_t)b : 1u) : 65536u;
+}
+
+}  // namespace
+
+// One orientation: rows keyed by keys[i], entries vals[i] (weights win, may be
+// null).  release() is called once the parts are sorted (the inputs are not
+// read after it).  Allocates *ptr (n + 1), *idx (m), *w (when win) and, when
+// rows is given, *rows (m): the row of every entry.
+static int wide_orientation(eg_graph *g, uint64_t part, const uint32_t *keys,
+                            const uint32_t *vals, const float *win,
+                            const std::function<void()> &release, uint64_t **ptr, uint32_t **idx,
+                            float **w, uint32_t *deg, uint32_t **rows) {
+  const uint64_t m = g->m;
+  const uint32_t n = g->n;
+  hipStream_t s = g->stream;
+  const uint64_t nparts = m ? (m + part - 1) / part : 0;
+  if (nparts > 0xFFFFFFFFull) return eg_fail(EG_EINVAL, "too many sorted parts");
+  struct Bufs {
+    uint32_t *sk = nullptr, *sv = nullptr, *pp = nullptr, *flag = nullptr;
+    float *sw = nullptr;
+    uint64_t *base = nullptr, *scan = nullptr;
+    ~Bufs() {
+      eg_dfree_t(&sk);
+      eg_dfree_t(&sv);
+      eg_dfree_t(&pp);
+      eg_dfree_t(&flag);
+      eg_dfree_t(&sw);
+      eg_dfree_t(&base);
+      eg_dfree_t(&scan);
+    }
+  } B;
+  int rc = EG_OK;
+  const uint64_t stride = (uint64_t)n + 1;
+  if (m) {
+    EG_TRY(eg_dalloc_t(&B.sk, m, "sorted part keys"));
+    EG_TRY(eg_dalloc_t(&B.sv, m, "sorted part entries"));
+    if (win) EG_TRY(eg_dalloc_t(&B.sw, m, "sorted part weights"));
+    EG_TRY(eg_dalloc_t(&B.pp, nparts * stride, "part run offsets"));
+    for (uint64_t p = 0; p < nparts && rc == EG_OK; p++) {
+      const uint64_t off = p * part, len = m - off < part ? m - off : part;
+      rc = sort_by(g, len, n, keys + off, B.sk + off,
+                   cols2(vals + off, B.sv + off, win ? win + off : nullptr,
+                         win ? B.sw + off : nullptr));
+      if (rc == EG_OK)
+        rc = eg_ptr_from_sorted(s, B.sk + off, len, n, B.pp + p * stride, nullptr);
+    }
+    EG_HIP(hipStreamSynchronize(s));  // nothing in flight reads the inputs or the sort buffers
+    if (rc) return rc;
+    // the sort's workspace (the largest buffer of the build) goes back now
+    eg_dfree(&g->scratch);
+    g->scratch_bytes = 0;
+  }
+  release();
+  EG_TRY(eg_dalloc_t(ptr, stride, "64-bit row offsets"));
+  EG_TRY(eg_dalloc_t(idx, m ? m : 1, "entries"));
+  if (win) EG_TRY(eg_dalloc_t(w, m ? m : 1, "entry weights"));
+  if (rows) EG_TRY(eg_dalloc_t(rows, m ? m : 1, "entry rows"));
+  EG_TRY(eg_dalloc_t(&B.flag, 1, "degree flag"));
+  EG_TRY(eg_dalloc(reinterpret_cast<void **>(&B.scan), eg_scan64_tmp_bytes(), "scan partials"));
+  EG_HIP(hipMemsetAsync(B.flag, 0, 4, s));
+  hipLaunchKernelGGL(k_wide_counts, dim3(wide_grid(n)), dim3(256), 0, s, n, (uint32_t)nparts,
+                     B.pp, *ptr, deg, B.flag);
+  EG_LAUNCHED();
+  EG_TRY(eg_scan_excl_u64(s, *ptr, n, B.scan));
+  uint32_t bad = 0;
+  EG_HIP(hipMemcpyAsync(&bad, B.flag, 4, hipMemcpyDeviceToHost, s));
+  EG_HIP(hipStreamSynchronize(s));
+  if (bad) return eg_fail(EG_ENOTSUP, "a vertex holds 2^32 entries or more (32-bit degrees)");
+  if (!m) return EG_OK;
+  EG_TRY(eg_dalloc_t(&B.base, n, "placement offsets"));
+  EG_HIP(hipMemcpyAsync(B.base, *ptr, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  for (uint64_t p = 0; p < nparts; p++) {
+    const uint64_t off = p * part, len = m - off < part ? m - off : part;
+    const uint32_t *pp = B.pp + p * stride;
+    hipLaunchKernelGGL(k_wide_place, dim3(wide_grid(len)), dim3(256), 0, s, len, B.sk + off,
+                       B.sv + off, win ? B.sw + off : nullptr, pp,
+                       static_cast<const uint64_t *>(B.base), *idx, win ? *w : nullptr,
+                       rows ? *rows : nullptr);
+    EG_LAUNCHED();
+    if (p + 1 < nparts) {
+      hipLaunchKernelGGL(k_wide_advance, dim3(wide_grid(n)), dim3(256), 0, s, n, pp, B.base);
+      EG_LAUNCHED();
+    }
+  }
+  EG_HIP(hipStreamSynchronize(s));  // before the buffers go
+  return EG_OK;
+}
+
+// The CSR (and, unless -s, the CSC) of an adjacency mode with 64-bit row
+// offsets; the CSC is the stable sort of the CSR entries by column with the
+// entry's row as payload (init_all.c:714-767).
+static int build_adjacency_wide(eg_graph *g, uint64_t part) {
+  const uint32_t n = g->n;
+  const bool w = (g->flags & EG_WEIGHTED) != 0;
+  g->wide = true;
+  const bool csc = !(g->flags & EG_SYMMETRIC);
+  uint32_t *rows = nullptr;
+  int rc = wide_orientation(
+      g, part, g->e_src, g->e_dst, g->e_w,
+      [g] {
+        eg_dfree_t(&g->e_src);
+        eg_dfree_t(&g->e_dst);
+        eg_dfree_t(&g->e_w);
+      },
+      &g->out_ptr64, &g->out_col, w ? &g->out_w : nullptr, g->out_deg, csc ? &rows : nullptr);
+  if (rc == EG_OK && !csc)
+    rc = hipMemcpyAsync(g->in_deg, g->out_deg, (size_t)n * 4, hipMemcpyDeviceToDevice,
+                        g->stream) == hipSuccess
+             ? EG_OK
+             : eg_fail(EG_EDEVICE, "degree copy failed");
+  if (rc == EG_OK && csc)
+    rc = wide_orientation(
+        g, part, g->out_col, rows, g->out_w, [&rows] { eg_dfree_t(&rows); }, &g->in_ptr64,
+        &g->in_idx, w ? &g->in_w : nullptr, g->in_deg, nullptr);
+  (void)hipStreamSynchronize(g->stream);
+  eg_dfree_t(&rows);
+  return rc;
 }
 
 // init_grid_sort (init_all.c:252-361, modes 0/6) and init_grid_nosort
@@ -211,7 +390,10 @@ int eg_build_layout(eg_graph *g) {
       case EG_MODE_ADJ_SORT:
       case EG_MODE_ADJ_SORT_FULL:
       case EG_MODE_ADJ_COUNT:
-        rc = build_adjacency(g);
+        if (g->tune.wide_part > 0 || m >= (1ull << 32))
+          rc = build_adjacency_wide(g, 1ull << (g->tune.wide_part > 0 ? g->tune.wide_part : 31));
+        else
+          rc = build_adjacency(g);
         break;
       case EG_MODE_GRID_SORT:
       case EG_MODE_GRID_NOSORT:
@@ -238,6 +420,9 @@ int eg_build_layout(eg_graph *g) {
   eg_dfree_t(&g->e_src);
   eg_dfree_t(&g->e_dst);
   eg_dfree_t(&g->e_w);
+  // so is the sorts' workspace (later passes take what they need on demand)
+  eg_dfree(&g->scratch);
+  g->scratch_bytes = 0;
   g->loaded = true;
   return EG_OK;
 }

@@ -79,10 +79,20 @@ double eg_wall_ms() {
       .count();
 }
 
-int eg_require_loaded(const eg_graph *g) {
+int eg_require_loaded_wide_ok(const eg_graph *g) {
   if (!g) return eg_fail(EG_EINVAL, "null graph");
   if (!g->loaded) return eg_fail(EG_ESTATE, "graph has no edges loaded");
   if (g->server) return eg_fail(EG_ESTATE, "graph is owned by a running server");
+  return EG_OK;
+}
+
+int eg_require_loaded(const eg_graph *g) {
+  EG_TRY(eg_require_loaded_wide_ok(g));
+  if (g->wide)
+    return eg_fail(EG_ENOTSUP,
+                   "the layout has 64-bit row offsets (%llu stored entries): BFS, degrees "
+                   "and the CSR / CSC downloads run on it",
+                   (unsigned long long)g->m);
   return EG_OK;
 }
 
@@ -258,6 +268,9 @@ static void free_layout(eg_graph *g) {
   eg_dfree_t(&g->bfs_ocol);
   eg_dfree_t(&g->out_w);
   eg_dfree_t(&g->in_ptr);
+  eg_dfree_t(&g->out_ptr64);
+  eg_dfree_t(&g->in_ptr64);
+  g->wide = false;
   eg_dfree_t(&g->in_idx);
   eg_dfree_t(&g->in_w);
   eg_dfree_t(&g->out_deg);
@@ -309,6 +322,8 @@ static void free_algo_state(eg_graph *g) {
   g->ppr_seeds_cap = 0;
   eg_dfree_t(&g->sssp_segmin);
   g->sssp_segmin_cap = 0;
+  eg_dfree_t(&g->wide_seg);
+  eg_dfree_t(&g->wide_ctr);
 }
 
 extern "C" int eg_graph_free(eg_graph *g) {
@@ -334,8 +349,8 @@ extern "C" int eg_graph_get_info(const eg_graph *g, eg_graph_info *out) {
   out->load_mode = g->load_mode;
   out->m_in = g->m_in;
   out->m = g->m;
-  out->has_csr = g->out_ptr != nullptr;
-  out->has_csc = g->in_ptr != nullptr;
+  out->has_csr = g->out_ptr != nullptr || g->out_ptr64 != nullptr;
+  out->has_csc = g->in_ptr != nullptr || g->in_ptr64 != nullptr;
   out->has_grid = g->cell_ptr != nullptr;
   out->has_edgelist = g->el_src != nullptr;
   out->build_ms = g->build_ms;
@@ -343,6 +358,8 @@ extern "C" int eg_graph_get_info(const eg_graph *g, eg_graph_info *out) {
   const uint64_t m = g->m, n = g->n;
   if (g->out_ptr) b += 4 * (n + 1) + 8 * m + (g->out_w ? 4 * m : 0);
   if (g->in_ptr) b += 4 * (n + 1) + 4 * m + (g->in_w ? 4 * m : 0);
+  if (g->out_ptr64) b += 8 * (n + 1) + 4 * m + (g->out_w ? 4 * m : 0);
+  if (g->in_ptr64) b += 8 * (n + 1) + 4 * m + (g->in_w ? 4 * m : 0);
   if (g->cell_ptr) b += 4 * ((uint64_t)g->P * g->P + 1) + 8 * m + (g->g_w ? 4 * m : 0);
   if (g->el_src) b += 8 * m + (g->el_w ? 4 * m : 0);
   b += 8 * n;
@@ -373,9 +390,15 @@ static int alloc_records(eg_graph *g, uint64_t m_in) {
   free_layout(g);
   const bool undir = (g->flags & EG_UNDIRECTED) != 0;
   const uint64_t m = undir ? 2 * m_in : m_in;
-  if (m >= (1ull << 32))
-    return eg_fail(EG_ENOTSUP, "%llu stored entries exceed the 32-bit offset range",
-                   (unsigned long long)m);
+  const bool adjacency = g->load_mode == EG_MODE_ADJ_SORT ||
+                         g->load_mode == EG_MODE_ADJ_SORT_FULL ||
+                         g->load_mode == EG_MODE_ADJ_COUNT;
+  // only the adjacency modes keep 64-bit row offsets
+  if (m >= (1ull << 32) && !adjacency)
+    return eg_fail(EG_ENOTSUP,
+                   "%llu stored entries exceed the 32-bit offsets of load mode %d "
+                   "(modes 2/3/4 take 64-bit row offsets)",
+                   (unsigned long long)m, g->load_mode);
   g->m_in = m_in;
   g->m = m;
   EG_TRY(eg_dalloc_t(&g->e_src, m, "records src"));
@@ -618,25 +641,33 @@ static int download(eg_graph *g, const void *d, void *h, uint64_t bytes) {
 }
 
 extern "C" int eg_graph_get_degrees(eg_graph *g, uint32_t *out_deg, uint32_t *in_deg) {
-  EG_TRY(eg_require_loaded(g));
+  EG_TRY(eg_require_loaded_wide_ok(g));
   EG_TRY(download(g, g->out_deg, out_deg, (uint64_t)g->n * 4));
   EG_TRY(download(g, g->in_deg, in_deg, (uint64_t)g->n * 4));
   return EG_OK;
 }
 
 extern "C" int eg_graph_get_csr(eg_graph *g, uint64_t *row_ptr, uint32_t *col, float *w) {
-  EG_TRY(eg_require_loaded(g));
-  if (!g->out_ptr) return eg_fail(EG_ESTATE, "load mode %d has no CSR", g->load_mode);
-  if (row_ptr) EG_TRY(download_ptr(g, g->out_ptr, (uint64_t)g->n + 1, row_ptr));
+  EG_TRY(eg_require_loaded_wide_ok(g));
+  if (g->out_ptr64) {
+    EG_TRY(download(g, g->out_ptr64, row_ptr, ((uint64_t)g->n + 1) * 8));
+  } else {
+    if (!g->out_ptr) return eg_fail(EG_ESTATE, "load mode %d has no CSR", g->load_mode);
+    if (row_ptr) EG_TRY(download_ptr(g, g->out_ptr, (uint64_t)g->n + 1, row_ptr));
+  }
   EG_TRY(download(g, g->out_col, col, g->m * 4));
   if (w) EG_TRY(download(g, g->out_w, w, g->m * 4));
   return EG_OK;
 }
 
 extern "C" int eg_graph_get_csc(eg_graph *g, uint64_t *col_ptr, uint32_t *row, float *w) {
-  EG_TRY(eg_require_loaded(g));
-  if (!g->in_ptr) return eg_fail(EG_ESTATE, "no in-edges (load mode %d or -s)", g->load_mode);
-  if (col_ptr) EG_TRY(download_ptr(g, g->in_ptr, (uint64_t)g->n + 1, col_ptr));
+  EG_TRY(eg_require_loaded_wide_ok(g));
+  if (g->in_ptr64) {
+    EG_TRY(download(g, g->in_ptr64, col_ptr, ((uint64_t)g->n + 1) * 8));
+  } else {
+    if (!g->in_ptr) return eg_fail(EG_ESTATE, "no in-edges (load mode %d or -s)", g->load_mode);
+    if (col_ptr) EG_TRY(download_ptr(g, g->in_ptr, (uint64_t)g->n + 1, col_ptr));
+  }
   EG_TRY(download(g, g->in_idx, row, g->m * 4));
   if (w) EG_TRY(download(g, g->in_w, w, g->m * 4));
   return EG_OK;

@@ -128,6 +128,9 @@ struct EgTune {
                          // chunk-ordered tiles (0) or runs from every eighth by turns (1)
   int ms_lanes = 256;    // eg_bfs_batch: sources per multi-source traversal (64, 128, 256)
   int ms_device = 1;     // eg_bfs_batch: device-driven levels (1) or host-driven (0)
+  int wide_part = 0;     // adjacency loads: build with 64-bit row offsets from sorted parts
+                         // of 2^wide_part entries (10..31); 0: only past 2^32 entries,
+                         // in parts of 2^31
 };
 
 struct eg_graph {
@@ -147,6 +150,10 @@ struct eg_graph {
   float *out_w = nullptr;
   // CSC (in-edges).
   uint32_t *in_ptr = nullptr, *in_idx = nullptr;
+  // Adjacency with 64-bit row offsets (wide): out_ptr64 / in_ptr64 replace
+  // out_ptr / in_ptr (both null), out_src is not kept.
+  bool wide = false;
+  uint64_t *out_ptr64 = nullptr, *in_ptr64 = nullptr;
   float *in_w = nullptr;
   uint32_t *out_deg = nullptr, *in_deg = nullptr;
   // 2-D grid.
@@ -181,6 +188,8 @@ struct eg_graph {
   int64_t *wcc_label = nullptr;
   uint32_t *wcc_lab32 = nullptr;
   uint32_t *wcc_tmp = nullptr;  // ordered-id labels and component minima (2n)
+  uint32_t *wide_seg = nullptr;            // wide BFS: row segments per frontier vertex
+  unsigned long long *wide_ctr = nullptr;  // wide BFS: next-queue length, degree sum
 
   // Scratch buffers shared by the builders (grown on demand).
   void *scratch = nullptr;
@@ -251,8 +260,15 @@ int eg_gather_vec(eg_graph *g, const void *src, const uint32_t *perm, void *dst,
 
 bool eg_has_in_edges(const eg_graph *g);
 int eg_scratch(eg_graph *g, size_t bytes, void **out);
-// EG_OK if g has a layout and no server owns it.
+// EG_OK if g has a layout with 32-bit offsets and no server owns it.
 int eg_require_loaded(const eg_graph *g);
+// The same for the entry points that also run on 64-bit row offsets (BFS,
+// degrees, the CSR / CSC downloads).
+int eg_require_loaded_wide_ok(const eg_graph *g);
+// Direction-optimizing BFS over 64-bit row offsets (eg_wide.hip): dist /
+// parent of eg_bfs in g->bfs_dist / g->bfs_parent, levels in *st.
+int eg_wide_bfs(eg_graph *g, uint32_t root, int start_pull, int switch_mode, bool want_parent,
+                eg_bfs_stats *st);
 // Single-vector personalized PageRank on the PageRank pull kernel (the plain
 // per-query path): stops once the L1 change is <= tol or after max_iter
 // iterations.  out (host, may be null) receives the n ranks.
@@ -320,6 +336,10 @@ struct EgProf {
 size_t eg_scan_tmp_bytes(uint64_t count);
 int eg_scan_excl(hipStream_t s, const uint32_t *in, uint32_t *out,
                  uint64_t count, void *tmp);
+// a[0..count) = exclusive scan of a (in place), a[count] = total; tmp holds
+// eg_scan64_tmp_bytes() bytes.
+size_t eg_scan64_tmp_bytes();
+int eg_scan_excl_u64(hipStream_t s, uint64_t *a, uint64_t count, void *tmp);
 // deg[k] += number of keys equal to k (deg zeroed by the caller).
 int eg_histogram(hipStream_t s, const uint32_t *keys, uint64_t count,
                  uint32_t *deg, uint32_t nbins);

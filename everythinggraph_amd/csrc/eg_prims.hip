@@ -278,6 +278,13 @@ int launch_scatter(hipStream_t s, bool wk, uint32_t ntiles, const uint32_t *kin,
 // ---------------------------------------------------------------------------
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 
+// Record kernels stride over the grid: counts reach past 2^32 (the threads of
+// one launch do not).
+__device__ __forceinline__ uint64_t first_index() {
+  return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ uint64_t grid_threads() { return (uint64_t)gridDim.x * blockDim.x; }
+
 __host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += kGolden;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -289,82 +296,87 @@ inline uint64_t mix64_host(uint64_t z) { return mix64(z); }
 __global__ void k_gen_rmat(uint64_t m_in, int scale, uint64_t skey, uint32_t ta,
                            uint32_t tab, uint32_t tabc, uint32_t *__restrict__ src,
                            uint32_t *__restrict__ dst) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m_in) return;
-  const uint64_t ekey = mix64(skey ^ e);
-  uint32_t s = 0, d = 0;
-  uint64_t h = 0;
-  for (int l = 0; l < scale; l++) {
-    uint32_t r;
-    if ((l & 1) == 0) {
-      h = mix64(ekey + (uint64_t)(l / 2 + 1) * kGolden);
-      r = (uint32_t)(h >> 32);
-    } else {
-      r = (uint32_t)h;
+  for (uint64_t e = first_index(); e < m_in; e += grid_threads()) {
+    const uint64_t ekey = mix64(skey ^ e);
+    uint32_t s = 0, d = 0;
+    uint64_t h = 0;
+    for (int l = 0; l < scale; l++) {
+      uint32_t r;
+      if ((l & 1) == 0) {
+        h = mix64(ekey + (uint64_t)(l / 2 + 1) * kGolden);
+        r = (uint32_t)(h >> 32);
+      } else {
+        r = (uint32_t)h;
+      }
+      const uint32_t qs = r >= tab ? 1u : 0u;
+      const uint32_t qd = (r >= ta && r < tab) || r >= tabc ? 1u : 0u;
+      s = (s << 1) | qs;
+      d = (d << 1) | qd;
     }
-    const uint32_t qs = r >= tab ? 1u : 0u;
-    const uint32_t qd = (r >= ta && r < tab) || r >= tabc ? 1u : 0u;
-    s = (s << 1) | qs;
-    d = (d << 1) | qd;
+    src[e] = s;
+    dst[e] = d;
   }
-  src[e] = s;
-  dst[e] = d;
 }
 
 __global__ void k_gen_er(uint64_t m_in, uint32_t n, uint64_t skey,
                          uint32_t *__restrict__ src, uint32_t *__restrict__ dst) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m_in) return;
-  const uint64_t h = mix64(mix64(skey ^ e) + kGolden);
-  src[e] = (uint32_t)(((h >> 32) * (uint64_t)n) >> 32);
-  dst[e] = (uint32_t)(((h & 0xFFFFFFFFull) * (uint64_t)n) >> 32);
+  for (uint64_t e = first_index(); e < m_in; e += grid_threads()) {
+    const uint64_t h = mix64(mix64(skey ^ e) + kGolden);
+    src[e] = (uint32_t)(((h >> 32) * (uint64_t)n) >> 32);
+    dst[e] = (uint32_t)(((h & 0xFFFFFFFFull) * (uint64_t)n) >> 32);
+  }
 }
 
 __global__ void k_gen_weights(uint64_t m_in, uint64_t wkey, float *__restrict__ w) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m_in) return;
-  const uint64_t h = mix64(wkey ^ e);
-  w[e] = (float)((h >> 40) + 1) * 0x1p-24f;
+  for (uint64_t e = first_index(); e < m_in; e += grid_threads()) {
+    const uint64_t h = mix64(wkey ^ e);
+    w[e] = (float)((h >> 40) + 1) * 0x1p-24f;
+  }
 }
 
 __global__ void k_unpack(const uint32_t *__restrict__ rec, uint64_t m_in, int words,
                          uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
                          float *__restrict__ w) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m_in) return;
-  const uint32_t *r = rec + e * (uint64_t)words;
-  src[e] = r[0];
-  dst[e] = r[1];
-  if (words == 3 && w) w[e] = __uint_as_float(r[2]);
+  for (uint64_t e = first_index(); e < m_in; e += grid_threads()) {
+    const uint32_t *r = rec + e * (uint64_t)words;
+    src[e] = r[0];
+    dst[e] = r[1];
+    if (words == 3 && w) w[e] = __uint_as_float(r[2]);
+  }
 }
 
 // preload_graph's reversed copies (init_all.c:102-106) at slot e + m_in.
 __global__ void k_append_reverse(uint64_t m_in, uint32_t *__restrict__ src,
                                  uint32_t *__restrict__ dst, float *__restrict__ w) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m_in) return;
-  src[e + m_in] = dst[e];
-  dst[e + m_in] = src[e];
-  if (w) w[e + m_in] = w[e];
+  for (uint64_t e = first_index(); e < m_in; e += grid_threads()) {
+    src[e + m_in] = dst[e];
+    dst[e + m_in] = src[e];
+    if (w) w[e + m_in] = w[e];
+  }
 }
 
 __global__ void k_check_ids(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
                             uint64_t count, uint32_t n, uint32_t *__restrict__ flag) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool bad = i < count && (a[i] >= n || b[i] >= n);
+  bool bad = false;
+  for (uint64_t i = first_index(); i < count; i += grid_threads()) bad |= a[i] >= n || b[i] >= n;
   if (__ballot(bad) && __lane_id() == 0) atomicOr(flag, 1u);
 }
 
 __global__ void k_cell_keys(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst,
                             uint64_t count, uint32_t n, uint32_t P,
                             uint32_t *__restrict__ keys) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  keys[i] = eg_partition_id(src[i], n, P) * P + eg_partition_id(dst[i], n, P);
+  for (uint64_t i = first_index(); i < count; i += grid_threads())
+    keys[i] = eg_partition_id(src[i], n, P) * P + eg_partition_id(dst[i], n, P);
 }
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
+}
+
+// Workgroups of 256 for a grid-stride record kernel over count items.
+inline uint32_t record_grid(uint64_t count) {
+  const uint64_t b = (count + 255) / 256;
+  return b < 65536 ? (b ? (uint32_t)b : 1u) : 65536u;
 }
 
 // ptr[v] = first k with keys[k] >= v over ascending keys, v in [0, n]: the
@@ -578,7 +590,7 @@ int eg_radix_sort(hipStream_t s, uint64_t count, int key_bits, const uint32_t *k
 int eg_gen_device(hipStream_t s, const eg_gen_params *p, uint32_t n, uint32_t *src,
                   uint32_t *dst, float *w) {
   if (p->m_in == 0) return EG_OK;
-  const uint32_t blocks = blocks_for(p->m_in, 256);
+  const uint32_t blocks = record_grid(p->m_in);
   const uint64_t skey = mix64_host(p->seed);
   if (p->kind == 0) {
     const double two32 = 4294967296.0;
@@ -602,7 +614,7 @@ int eg_gen_device(hipStream_t s, const eg_gen_params *p, uint32_t n, uint32_t *s
 int eg_unpack_records(hipStream_t s, const void *rec, uint64_t m_in, int weighted,
                       uint32_t *src, uint32_t *dst, float *w) {
   if (m_in == 0) return EG_OK;
-  hipLaunchKernelGGL(k_unpack, dim3(blocks_for(m_in, 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_unpack, dim3(record_grid(m_in)), dim3(256), 0, s,
                      reinterpret_cast<const uint32_t *>(rec), m_in, weighted ? 3 : 2, src,
                      dst, w);
   EG_LAUNCHED();
@@ -611,7 +623,7 @@ int eg_unpack_records(hipStream_t s, const void *rec, uint64_t m_in, int weighte
 
 int eg_append_reverse(hipStream_t s, uint64_t m_in, uint32_t *src, uint32_t *dst, float *w) {
   if (m_in == 0) return EG_OK;
-  hipLaunchKernelGGL(k_append_reverse, dim3(blocks_for(m_in, 256)), dim3(256), 0, s, m_in,
+  hipLaunchKernelGGL(k_append_reverse, dim3(record_grid(m_in)), dim3(256), 0, s, m_in,
                      src, dst, w);
   EG_LAUNCHED();
   return EG_OK;
@@ -622,7 +634,7 @@ int eg_check_ids(hipStream_t s, const uint32_t *a, const uint32_t *b, uint64_t c
   *bad = 0;
   if (count == 0) return EG_OK;
   EG_HIP(hipMemsetAsync(d_flag, 0, sizeof(uint32_t), s));
-  hipLaunchKernelGGL(k_check_ids, dim3(blocks_for(count, 256)), dim3(256), 0, s, a, b, count,
+  hipLaunchKernelGGL(k_check_ids, dim3(record_grid(count)), dim3(256), 0, s, a, b, count,
                      n, d_flag);
   EG_LAUNCHED();
   uint32_t h = 0;
@@ -635,8 +647,102 @@ int eg_check_ids(hipStream_t s, const uint32_t *a, const uint32_t *b, uint64_t c
 int eg_cell_keys(hipStream_t s, const uint32_t *src, const uint32_t *dst, uint64_t count,
                  uint32_t n, uint32_t P, uint32_t *keys) {
   if (count == 0) return EG_OK;
-  hipLaunchKernelGGL(k_cell_keys, dim3(blocks_for(count, 256)), dim3(256), 0, s, src, dst,
+  hipLaunchKernelGGL(k_cell_keys, dim3(record_grid(count)), dim3(256), 0, s, src, dst,
                      count, n, P, keys);
+  EG_LAUNCHED();
+  return EG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of 64-bit counts (row offsets past 2^32 entries):
+// kScan64Parts contiguous ranges are reduced, their totals scanned by one
+// workgroup, and every range rescanned from its offset in tiles of 256.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint32_t kScan64Parts = 1024;
+
+__device__ __forceinline__ uint64_t block_reduce_u64(uint64_t v, uint64_t *s_w) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  if (__lane_id() == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const uint64_t t = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(256) void k_scan64_reduce(const uint64_t *__restrict__ a,
+                                                       uint64_t count, uint64_t per,
+                                                       uint64_t *__restrict__ part) {
+  __shared__ uint64_t s_w[4];
+  const uint64_t lo = (uint64_t)blockIdx.x * per;
+  const uint64_t hi = lo + per < count ? lo + per : count;
+  uint64_t v = 0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) v += a[i];
+  v = block_reduce_u64(v, s_w);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// Serial exclusive scan of at most kScan64Parts totals (one thread).
+__global__ void k_scan64_parts(uint64_t *__restrict__ part, uint32_t np) {
+  if (threadIdx.x != 0) return;
+  uint64_t run = 0;
+  for (uint32_t i = 0; i < np; i++) {
+    const uint64_t v = part[i];
+    part[i] = run;
+    run += v;
+  }
+  part[np] = run;
+}
+
+__global__ __launch_bounds__(256) void k_scan64_apply(uint64_t *__restrict__ a, uint64_t count,
+                                                      uint64_t per,
+                                                      const uint64_t *__restrict__ part) {
+  __shared__ uint64_t s_w[4];
+  const uint64_t lo = (uint64_t)blockIdx.x * per;
+  const uint64_t hi = lo + per < count ? lo + per : count;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  uint64_t run = part[blockIdx.x];
+  for (uint64_t c = lo; c < hi; c += 256) {
+    const uint64_t i = c + threadIdx.x;
+    const uint64_t x = i < hi ? a[i] : 0;
+    uint64_t inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t y = __shfl_up(inc, o);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint64_t before = 0;
+    for (int w = 0; w < wave; w++) before += s_w[w];
+    const uint64_t tile = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    if (i < hi) a[i] = run + before + inc - x;
+    run += tile;
+    __syncthreads();
+  }
+  if (hi == count && threadIdx.x == 0) a[count] = run;
+}
+
+}  // namespace
+
+size_t eg_scan64_tmp_bytes() { return (kScan64Parts + 1) * sizeof(uint64_t); }
+
+int eg_scan_excl_u64(hipStream_t s, uint64_t *a, uint64_t count, void *tmp) {
+  if (count == 0) {
+    EG_HIP(hipMemsetAsync(a, 0, sizeof(uint64_t), s));
+    return EG_OK;
+  }
+  uint64_t *part = static_cast<uint64_t *>(tmp);
+  const uint64_t per = (count + kScan64Parts - 1) / kScan64Parts;
+  const uint32_t np = (uint32_t)((count + per - 1) / per);
+  hipLaunchKernelGGL(k_scan64_reduce, dim3(np), dim3(256), 0, s, a, count, per, part);
+  EG_LAUNCHED();
+  hipLaunchKernelGGL(k_scan64_parts, dim3(1), dim3(64), 0, s, part, np);
+  EG_LAUNCHED();
+  hipLaunchKernelGGL(k_scan64_apply, dim3(np), dim3(256), 0, s, a, count, per,
+                     static_cast<const uint64_t *>(part));
   EG_LAUNCHED();
   return EG_OK;
 }

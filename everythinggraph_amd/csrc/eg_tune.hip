@@ -72,6 +72,7 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "ms_device")) return &g->tune.ms_device;
   if (!strcmp(name, "bfs_order")) return &g->tune.bfs_order;
   if (!strcmp(name, "wcc_order")) return &g->tune.wcc_order;
+  if (!strcmp(name, "wide_part")) return &g->tune.wide_part;
   return nullptr;
 }
 
@@ -136,6 +137,9 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
       return eg_fail(EG_EINVAL, "ms_lanes takes 64, 128 or 256");
   } else if (!strcmp(param, "ms_device")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "ms_device takes 0 or 1");
+  } else if (!strcmp(param, "wide_part")) {
+    if (value != 0 && (value < 10 || value > 31))
+      return eg_fail(EG_EINVAL, "wide_part takes 0 or a part size exponent 10..31");
   } else if (!strcmp(param, "wcc_order")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "wcc_order takes 0 or 1");
   } else if (!strcmp(param, "pr_binned")) {

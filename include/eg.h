@@ -14,7 +14,11 @@
  * Conventions: every call returns EG_OK (0) or a negative EG_E* code and
  * eg_last_error() describes the last failure of the calling thread.  A handle
  * is not thread-safe.  Vertex ids are uint32 and must be < n (EG_ERANGE
- * otherwise); stored adjacency entries must number < 2^32.
+ * otherwise).  Layouts with fewer than 2^32 stored entries use 32-bit
+ * offsets.  From 2^32 stored entries on, the adjacency modes (2/3/4) keep
+ * 64-bit row offsets (the reference's u64 edge offsets, random.h:87-93):
+ * eg_bfs, eg_graph_get_degrees / _csr / _csc and the info calls run on such a
+ * graph, every other call returns EG_ENOTSUP; the other modes refuse the load.
  */
 #ifndef EG_H
 #define EG_H
@@ -298,7 +302,11 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * the same); "grid_band" (grid source
  * partitions per launch, 0 = all); "graphs" (1: batched PPR iterations
  * replayed from a hipGraph); "ingest_kib" (record ingest chunk of the pinned
- * two-slot load pipeline, 0 = 64 MiB; set before loading).  Every variant
+ * two-slot load pipeline, 0 = 64 MiB; set before loading); "wide_part"
+ * (adjacency modes, set before loading: 10..31 builds the layout with 64-bit
+ * row offsets from stably sorted parts of 2^wide_part records, as every load
+ * past 2^32 stored entries is built in parts of 2^31; 0 = the default).
+ * Every variant
  * except pr_binned computes bit-identical results.  eg_autotune times the
  * variants of the selected kernels on the graph itself, keeps the fastest, and
  * reports each timing. */

@@ -802,3 +802,139 @@ int ego_check_bfs_parents(uint32_t n, const uint64_t *out_ptr,
   free(edge);
   return bad;
 }
+
+/* ---- full-size checks (graphs past 2^32 stored entries) ----
+ * These read the input records, not a built layout, so a layout is checked
+ * against what it must hold without a second full sort on the host.  With
+ * both = 1 the stored entries are the records followed by their reversed
+ * copies (-u, init_all.c:102-106). */
+
+/* Row content of every vertex from the records: cnt[v] entries, sum[v] =
+ * sum of their columns, xr[v] = xor of their columns. */
+void ego_record_row_sums(uint32_t n, uint64_t m, const uint32_t *src,
+                         const uint32_t *dst, int both, uint64_t *cnt,
+                         uint64_t *sum, uint32_t *xr) {
+  memset(cnt, 0, (size_t)n * sizeof(uint64_t));
+  memset(sum, 0, (size_t)n * sizeof(uint64_t));
+  memset(xr, 0, (size_t)n * sizeof(uint32_t));
+  for (int pass = 0; pass < (both ? 2 : 1); pass++) {
+    const uint32_t *key = pass ? dst : src, *val = pass ? src : dst;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)m; i++) {
+      const uint32_t v = key[i], c = val[i];
+#pragma omp atomic
+      cnt[v] += 1;
+#pragma omp atomic
+      sum[v] += c;
+#pragma omp atomic
+      xr[v] ^= c;
+    }
+  }
+}
+
+/* The same row content read from a CSR (ptr[n+1], col). */
+void ego_csr_row_sums(uint32_t n, const uint64_t *ptr, const uint32_t *col,
+                      uint64_t *sum, uint32_t *xr) {
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t v = 0; v < (int64_t)n; v++) {
+    uint64_t s = 0;
+    uint32_t x = 0;
+    for (uint64_t k = ptr[v]; k < ptr[v + 1]; k++) {
+      s += col[k];
+      x ^= col[k];
+    }
+    sum[v] = s;
+    xr[v] = x;
+  }
+}
+
+/* Rows of the vertices sel[0..nsel) (ascending, distinct) in stored order:
+ * row v lists dst[i] for the records with src[i] == v in input order, then
+ * (both) src[i] for the records with dst[i] == v: the rows of a stable sort
+ * of the stored entries by source (init_all.c:629-697).  out_ptr[nsel+1];
+ * out_col (may be NULL to count only) holds out_ptr[nsel] entries. */
+void ego_rows_of(uint32_t n, uint64_t m, const uint32_t *src, const uint32_t *dst,
+                 int both, uint32_t nsel, const uint32_t *sel, uint64_t *out_ptr,
+                 uint32_t *out_col) {
+  int32_t *slot = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  for (uint32_t v = 0; v < n; v++) slot[v] = -1;
+  for (uint32_t j = 0; j < nsel; j++) slot[sel[j]] = (int32_t)j;
+  const int T = omp_get_max_threads();
+  const int passes = both ? 2 : 1;
+  /* counts per (pass, thread block, selected vertex) */
+  uint64_t *c = (uint64_t *)calloc((size_t)passes * T * (nsel ? nsel : 1), sizeof(uint64_t));
+  for (int pass = 0; pass < passes; pass++) {
+    const uint32_t *key = pass ? dst : src;
+#pragma omp parallel num_threads(T)
+    {
+      const int t = omp_get_thread_num();
+      const uint64_t lo = m * (uint64_t)t / (uint64_t)T, hi = m * (uint64_t)(t + 1) / (uint64_t)T;
+      uint64_t *ct = c + ((size_t)pass * T + t) * nsel;
+      for (uint64_t i = lo; i < hi; i++) {
+        const int32_t j = slot[key[i]];
+        if (j >= 0) ct[j]++;
+      }
+    }
+  }
+  /* row j: pass 0 blocks in order, then pass 1 blocks: exclusive offsets */
+  uint64_t run = 0;
+  for (uint32_t j = 0; j < nsel; j++) {
+    out_ptr[j] = run;
+    for (int pass = 0; pass < passes; pass++)
+      for (int t = 0; t < T; t++) {
+        uint64_t *x = c + ((size_t)pass * T + t) * nsel + j;
+        const uint64_t v = *x;
+        *x = run;
+        run += v;
+      }
+  }
+  out_ptr[nsel] = run;
+  if (out_col) {
+    for (int pass = 0; pass < passes; pass++) {
+      const uint32_t *key = pass ? dst : src, *val = pass ? src : dst;
+#pragma omp parallel num_threads(T)
+      {
+        const int t = omp_get_thread_num();
+        const uint64_t lo = m * (uint64_t)t / (uint64_t)T, hi = m * (uint64_t)(t + 1) / (uint64_t)T;
+        uint64_t *ct = c + ((size_t)pass * T + t) * nsel;
+        for (uint64_t i = lo; i < hi; i++) {
+          const int32_t j = slot[key[i]];
+          if (j >= 0) out_col[ct[j]++] = val[i];
+        }
+      }
+    }
+  }
+  free(c);
+  free(slot);
+}
+
+/* Single-source BFS invariants of levels dist (root 1, unreached 0) over the
+ * edges of the records (both: each record in both directions): dist[root] =
+ * 1; for every edge u -> v with u reached, v is reached and dist[v] <=
+ * dist[u] + 1; every reached v != root has an in-neighbour u with dist[u] =
+ * dist[v] - 1.  Together these make dist - 1 the hop distance from root.
+ * Returns the number of violations. */
+int64_t ego_check_bfs_levels(uint32_t n, uint64_t m, const uint32_t *src,
+                             const uint32_t *dst, int both, uint32_t root,
+                             const uint32_t *dist) {
+  unsigned char *pred = (unsigned char *)calloc(n ? n : 1, 1);
+  int64_t bad = dist[root] != 1;
+  for (int pass = 0; pass < (both ? 2 : 1); pass++) {
+    const uint32_t *from = pass ? dst : src, *to = pass ? src : dst;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = 0; i < (int64_t)m; i++) {
+      const uint32_t u = from[i], v = to[i];
+      if (!dist[u]) continue;
+      if (!dist[v] || dist[v] > dist[u] + 1) bad++;
+      if (dist[v] == dist[u] + 1) {
+#pragma omp atomic write
+        pred[v] = 1;
+      }
+    }
+  }
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (int64_t v = 0; v < (int64_t)n; v++)
+    if ((uint32_t)v != root && dist[v] && !pred[v]) bad++;
+  free(pred);
+  return bad;
+}

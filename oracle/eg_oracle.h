@@ -106,6 +106,23 @@ int ego_check_sssp_parents(uint32_t n, const uint64_t *out_ptr, const uint32_t *
 
 int ego_num_threads(void);
 
+/* Full-size checks from the input records (both = 1: each record also
+ * stored reversed, -u).  Row content (count, column sum, column xor) of every
+ * vertex from the records and from a CSR; the rows of selected vertices
+ * (ascending, distinct) in stored order; single-source BFS level invariants
+ * (returns the number of violations). */
+void ego_record_row_sums(uint32_t n, uint64_t m, const uint32_t *src,
+                         const uint32_t *dst, int both, uint64_t *cnt,
+                         uint64_t *sum, uint32_t *xr);
+void ego_csr_row_sums(uint32_t n, const uint64_t *ptr, const uint32_t *col,
+                      uint64_t *sum, uint32_t *xr);
+void ego_rows_of(uint32_t n, uint64_t m, const uint32_t *src, const uint32_t *dst,
+                 int both, uint32_t nsel, const uint32_t *sel, uint64_t *out_ptr,
+                 uint32_t *out_col);
+int64_t ego_check_bfs_levels(uint32_t n, uint64_t m, const uint32_t *src,
+                             const uint32_t *dst, int both, uint32_t root,
+                             const uint32_t *dist);
+
 #ifdef __cplusplus
 }
 #endif

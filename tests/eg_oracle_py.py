@@ -222,3 +222,40 @@ def check_sssp_parents(n, out_ptr, out_col, root, kind, dist, parent):
 
 def out_degree(n, src):
     return np.bincount(src, minlength=n).astype(np.uint32)
+
+
+def record_row_sums(n, src, dst, both):
+    """(count, column sum, column xor) of every row, from the records."""
+    cnt = np.empty(n, np.uint64)
+    sm = np.empty(n, np.uint64)
+    xr = np.empty(n, np.uint32)
+    L().ego_record_row_sums(U32(n), U64(src.size), P(src), P(dst), I32(int(both)), P(cnt),
+                            P(sm), P(xr))
+    return cnt, sm, xr
+
+
+def csr_row_sums(n, ptr, col):
+    """(column sum, column xor) of every row of a CSR."""
+    sm = np.empty(n, np.uint64)
+    xr = np.empty(n, np.uint32)
+    L().ego_csr_row_sums(U32(n), P(ptr), P(col), P(sm), P(xr))
+    return sm, xr
+
+
+def rows_of(n, src, dst, both, sel):
+    """Rows of the ascending, distinct vertices sel in stored order: (ptr, col)."""
+    sel = np.ascontiguousarray(sel, np.uint32)
+    ptr = np.empty(sel.size + 1, np.uint64)
+    L().ego_rows_of(U32(n), U64(src.size), P(src), P(dst), I32(int(both)), U32(sel.size),
+                    P(sel), P(ptr), None)
+    col = np.empty(int(ptr[-1]), np.uint32)
+    L().ego_rows_of(U32(n), U64(src.size), P(src), P(dst), I32(int(both)), U32(sel.size),
+                    P(sel), P(ptr), P(col))
+    return ptr, col
+
+
+def check_bfs_levels(n, src, dst, both, root, dist):
+    """Violations of the single-source BFS level invariants over the records."""
+    f = L().ego_check_bfs_levels
+    f.restype = ctypes.c_int64
+    return f(U32(n), U64(src.size), P(src), P(dst), I32(int(both)), U32(root), P(dist))

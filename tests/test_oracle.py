@@ -376,6 +376,66 @@ def test_parent_checker_catches_planted_errors():
 
 
 # ---------------------------------------------------------------------------
+# full-size checkers (row content from the records, rows of selected
+# vertices, BFS level invariants) vs numpy / scipy
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name,n,s,d", zoo(), ids=[z[0] for z in zoo()])
+@pytest.mark.parametrize("both", [0, 1])
+def test_row_sums_and_rows_of_vs_numpy(name, n, s, d, both):
+    ss, dd = (np.concatenate([s, d]), np.concatenate([d, s])) if both else (s, d)
+    cnt, sm, xr = O.record_row_sums(n, s, d, both)
+    assert np.array_equal(cnt, np.bincount(ss, minlength=n))
+    ref_sum = np.zeros(n, np.uint64)
+    np.add.at(ref_sum, ss.astype(np.int64), dd.astype(np.uint64))
+    ref_xr = np.zeros(n, np.uint32)
+    np.bitwise_xor.at(ref_xr, ss.astype(np.int64), dd)
+    assert np.array_equal(sm, ref_sum) and np.array_equal(xr, ref_xr)
+    # the same content read from the rows of a CSR (numpy's stable sort)
+    perm = np.argsort(ss, kind="stable")
+    ptr = np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint64)
+    col = dd[perm]
+    csum, cxr = O.csr_row_sums(n, ptr, col)
+    assert np.array_equal(csum, ref_sum) and np.array_equal(cxr, ref_xr)
+    # one entry moved to the next row changes both checks
+    if col.size > 1 and cnt[ss[perm[0]]] > 0:
+        bad = ptr.copy()
+        r = int(ss[perm[0]])
+        bad[r + 1] -= 1
+        bsum, bxr = O.csr_row_sums(n, bad, col)
+        assert not (np.array_equal(bsum, ref_sum) and np.array_equal(bxr, ref_xr))
+    # rows of selected vertices in stored order, hubs included
+    rng = np.random.default_rng(3)
+    sel = np.unique(np.concatenate([rng.integers(0, n, 7), [int(np.argmax(cnt))]])).astype(np.uint32)
+    rp, rc = O.rows_of(n, s, d, both, sel)
+    for j, v in enumerate(sel):
+        assert np.array_equal(rc[rp[j]:rp[j + 1]], col[ptr[v]:ptr[v + 1]]), (name, v)
+
+
+@pytest.mark.parametrize("undirected", [False, True])
+def test_bfs_level_checker_vs_scipy_and_planted_errors(undirected):
+    n, (s, d) = 1024, O.gen_rmat(10, 8192, seed=6)
+    ss, dd = (O.symmetrize(s, d)[:2] if undirected else (s, d))
+    root = int(s[0])
+    dist = scipy_levels(n, ss, dd, root)
+    assert O.check_bfs_levels(n, s, d, int(undirected), root, dist) == 0
+    reached = np.nonzero(dist > 1)[0]
+    deep = reached[np.argmax(dist[reached])]
+    for plant in range(5):
+        bad = dist.copy()
+        if plant == 0:
+            bad[reached[0]] += 1      # a level one too deep
+        elif plant == 1:
+            bad[deep] -= 1            # one too shallow: no neighbour a level up
+        elif plant == 2:
+            bad[reached[0]] = 0       # a reachable vertex left unreached
+        elif plant == 3:
+            bad[dist > 0] += 1        # every level shifted: root is not 1
+        else:
+            bad = scipy_levels(n, ss, dd, int(reached[0]))  # another root's levels
+        assert O.check_bfs_levels(n, s, d, int(undirected), root, bad) > 0, plant
+
+
+# ---------------------------------------------------------------------------
 # WCC vs scipy connected components / brute force
 # ---------------------------------------------------------------------------
 def min_label(n, s, d):
