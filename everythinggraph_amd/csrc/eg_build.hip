@@ -13,10 +13,10 @@
 #include "eg_internal.h"
 
 static int sort_by(eg_graph *g, uint64_t m, uint64_t nkeys, const uint32_t *kin, uint32_t *kout,
-                   EgSortCols cols) {
+                   EgSortCols cols, const EgIdCheck *chk = nullptr) {
   void *tmp = nullptr;
   EG_TRY(eg_scratch(g, eg_radix_tmp_bytes(m, cols.np), &tmp));
-  return eg_radix_sort(g->stream, m, eg_bits_for(nkeys), kin, kout, cols, tmp);
+  return eg_radix_sort(g->stream, m, eg_bits_for(nkeys), kin, kout, cols, tmp, chk);
 }
 
 static EgSortCols cols2(const uint32_t *i0, uint32_t *o0, const float *i1, float *o1) {
@@ -63,7 +63,8 @@ static int degrees_from_sorted(eg_graph *g, const uint32_t *sorted, uint32_t *de
 // init_adj_sort (init_all.c:629-790) and init_adj_count (:444-496): stable sort
 // by src with the dst (and weight) as payload; row_ptr from the sorted src;
 // CSC = stable sort of the CSR entries by dst (the src is its payload).
-static int build_adjacency(eg_graph *g) {
+// The sort by src also checks every id against n (chk).
+static int build_adjacency(eg_graph *g, const EgIdCheck *chk) {
   const uint64_t m = g->m;
   const uint32_t n = g->n;
   const bool w = (g->flags & EG_WEIGHTED) != 0;
@@ -71,7 +72,8 @@ static int build_adjacency(eg_graph *g) {
   EG_TRY(eg_dalloc_t(&g->out_col, m, "col"));
   EG_TRY(eg_dalloc_t(&g->out_src, m, "row ids"));
   if (w) EG_TRY(eg_dalloc_t(&g->out_w, m, "weights"));
-  EG_TRY(sort_by(g, m, n, g->e_src, g->out_src, cols2(g->e_dst, g->out_col, g->e_w, g->out_w)));
+  EG_TRY(sort_by(g, m, n, g->e_src, g->out_src, cols2(g->e_dst, g->out_col, g->e_w, g->out_w),
+                 chk));
   EG_TRY(eg_ptr_from_sorted(g->stream, g->out_src, m, n, g->out_ptr, g->out_deg));
   if (g->flags & EG_SYMMETRIC) {
     // -s declares the graph symmetric: no in-edges are built (init_all.c:714)
@@ -381,8 +383,19 @@ int eg_build_layout(eg_graph *g) {
   EG_HIP(hipEventRecord(G.e[0], g->stream));
   int bad = 0;
   EG_TRY(eg_dalloc_t(&G.tmp, 1025, "id flag"));
-  int rc = eg_check_ids(g->stream, g->e_src, g->e_dst, m, n, G.tmp, &bad);
-  if (rc == EG_OK && bad) rc = eg_fail(EG_ERANGE, "edge file holds a vertex id >= n = %u", n);
+  const bool adjacency = g->load_mode == EG_MODE_ADJ_SORT ||
+                         g->load_mode == EG_MODE_ADJ_SORT_FULL ||
+                         g->load_mode == EG_MODE_ADJ_COUNT;
+  const bool wide = adjacency && (g->tune.wide_part > 0 || m >= (1ull << 32));
+  // the 32-bit adjacency build checks the ids inside its first sort
+  const bool fused_check = adjacency && !wide;
+  int rc = EG_OK;
+  if (fused_check) {
+    EG_HIP(hipMemsetAsync(G.tmp, 0, sizeof(uint32_t), g->stream));
+  } else {
+    rc = eg_check_ids(g->stream, g->e_src, g->e_dst, m, n, G.tmp, &bad);
+    if (rc == EG_OK && bad) rc = eg_fail(EG_ERANGE, "edge file holds a vertex id >= n = %u", n);
+  }
   if (rc == EG_OK) rc = eg_dalloc_t(&g->out_deg, n, "out degree");
   if (rc == EG_OK) rc = eg_dalloc_t(&g->in_deg, n, "in degree");
   if (rc == EG_OK) {
@@ -390,10 +403,21 @@ int eg_build_layout(eg_graph *g) {
       case EG_MODE_ADJ_SORT:
       case EG_MODE_ADJ_SORT_FULL:
       case EG_MODE_ADJ_COUNT:
-        if (g->tune.wide_part > 0 || m >= (1ull << 32))
+        if (wide) {
           rc = build_adjacency_wide(g, 1ull << (g->tune.wide_part > 0 ? g->tune.wide_part : 31));
-        else
-          rc = build_adjacency(g);
+        } else {
+          EgIdCheck chk;
+          chk.limit = n;
+          chk.vals = g->e_dst;
+          chk.flag = G.tmp;
+          rc = build_adjacency(g, &chk);
+          if (rc == EG_OK) {
+            uint32_t h = 0;
+            EG_HIP(hipMemcpyAsync(&h, G.tmp, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+            EG_HIP(hipStreamSynchronize(g->stream));
+            if (h) rc = eg_fail(EG_ERANGE, "edge file holds a vertex id >= n = %u", n);
+          }
+        }
         break;
       case EG_MODE_GRID_SORT:
       case EG_MODE_GRID_NOSORT:

@@ -358,10 +358,18 @@ struct EgSortCols {
   uint32_t *out[3] = {nullptr, nullptr, nullptr};
   int np = 0;
 };
+// Optional id check folded into the sort's first pass: flag (device, zeroed
+// by the caller) is set when a key or vals[i] is >= limit (vals = payload
+// column 0 is checked as the first scatter reads it).
+struct EgIdCheck {
+  uint32_t limit = 0;
+  const uint32_t *vals = nullptr;
+  uint32_t *flag = nullptr;
+};
 size_t eg_radix_tmp_bytes(uint64_t count, int np);
 int eg_radix_sort(hipStream_t s, uint64_t count, int key_bits,
                   const uint32_t *kin, uint32_t *kout, const EgSortCols &cols,
-                  void *tmp);
+                  void *tmp, const EgIdCheck *chk = nullptr);
 int eg_bits_for(uint64_t nkeys);
 
 // Device generator (eg_prims.hip).

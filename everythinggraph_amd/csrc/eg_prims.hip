@@ -6,8 +6,8 @@
 // (intSort::iSort, radixSort_ligra.h:355) and sequential/blocked scans
 // (utils_ligra.h:185-214).  Here the same contracts (stable by key; exclusive
 // prefix sums) are met with wave64 kernels: ballot-based stable ranking inside
-// a tile, LDS staging so that every tile writes contiguous runs, and a
-// decoupled count matrix scanned on the device.
+// a tile, LDS staging so that every tile writes contiguous runs, and a small
+// per-range count matrix scanned on the device.
 #include <stdarg.h>
 #include <stdio.h>
 
@@ -125,151 +125,209 @@ __global__ __launch_bounds__(kHistBlock) void k_histogram(
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort.  One pass = digit histogram per tile (count matrix
-// laid out digit-major), exclusive scan of the matrix, stable scatter.
-// The scatter ranks a tile stably: wave w owns the contiguous sub-tile
-// [w*1024, (w+1)*1024) and walks it in 16 steps of 64 lanes; lanes with equal
-// digits find each other with one ballot per digit bit, and a per-wave LDS
-// counter row carries the running count.  The tile is then reordered through
-// LDS so that each digit's run is written contiguously.
+// Stable LSD radix sort, range-blocked.  The items are cut into at most
+// kRbMaxRanges contiguous ranges of whole tiles (kRbTile items).  One pass =
+//  * k_rb_hist: the digit histogram of every range (one read of the keys),
+//    laid out digit-major, range-minor;
+//  * an exclusive scan of that small matrix: the start of every (digit,
+//    range) run in the output;
+//  * k_rb_scatter: one workgroup per range walks its tiles in order.  A tile
+//    is ranked stably (lanes with equal digits find each other with one
+//    ballot per digit bit, and a per-wave LDS row carries the running
+//    counts), reordered through LDS a column at a time so that each digit's
+//    run is written contiguously, and written at the range's running digit
+//    offsets, which then advance by the tile's counts.
+// No workgroup waits on another; stability follows from the range order, the
+// tile order inside a range and the rank order inside a tile.
 // ---------------------------------------------------------------------------
-constexpr int kRadixBlock = 256;
-constexpr int kRadixItems = 16;
-constexpr uint32_t kRadixTile = kRadixBlock * kRadixItems;  // 4096
-constexpr int kRadixWaves = kRadixBlock / 64;
+constexpr int kRbBlock = 512;
+constexpr int kRbItems = 16;
+constexpr uint32_t kRbTile = kRbBlock * kRbItems;  // 8192
+constexpr int kRbWaves = kRbBlock / 64;
+constexpr uint32_t kRbMaxRanges = 2048;
+constexpr int kRbHistUnroll = 8;
 
-__global__ __launch_bounds__(kRadixBlock) void k_radix_hist(
-    const uint32_t *__restrict__ keys, uint64_t count, int shift, uint32_t mask,
-    uint32_t ntiles, uint32_t *__restrict__ counts) {
-  __shared__ uint32_t s_h[256];
-  s_h[threadIdx.x] = 0;
+// Digit histogram of range r = [r * span, min((r + 1) * span, count)) into
+// counts[d * nranges + r]; with limit != 0 also flags keys >= limit.
+__global__ __launch_bounds__(kRbBlock) void k_rb_hist(
+    const uint32_t *__restrict__ keys, uint64_t count, uint64_t span, int shift, uint32_t mask,
+    uint32_t nranges, uint32_t *__restrict__ counts, uint32_t limit, uint32_t *__restrict__ flag) {
+  __shared__ uint32_t s_h[kRbWaves][256];
+  const int wave = threadIdx.x >> 6;
+  for (int j = threadIdx.x; j < kRbWaves * 256; j += kRbBlock) (&s_h[0][0])[j] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kRadixTile;
-#pragma unroll 4
-  for (int s = 0; s < kRadixItems; s++) {
-    const uint64_t i = base + (uint64_t)s * kRadixBlock + threadIdx.x;
-    if (i < count) atomicAdd(&s_h[(keys[i] >> shift) & mask], 1u);
+  const uint64_t lo = (uint64_t)blockIdx.x * span;
+  const uint64_t hi = lo + span < count ? lo + span : count;
+  bool bad = false;
+  for (uint64_t base = lo; base < hi; base += (uint64_t)kRbBlock * kRbHistUnroll) {
+    uint32_t k[kRbHistUnroll];
+#pragma unroll
+    for (int u = 0; u < kRbHistUnroll; u++) {
+      const uint64_t i = base + (uint64_t)u * kRbBlock + threadIdx.x;
+      k[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < kRbHistUnroll; u++) {
+      const uint64_t i = base + (uint64_t)u * kRbBlock + threadIdx.x;
+      if (i < hi) {
+        bad |= limit && k[u] >= limit;
+        atomicAdd(&s_h[wave][(k[u] >> shift) & mask], 1u);
+      }
+    }
   }
+  if (limit && __ballot(bad) && __lane_id() == 0) atomicOr(flag, 1u);
   __syncthreads();
-  if (threadIdx.x <= mask) counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+  for (int d = threadIdx.x; d <= (int)mask; d += kRbBlock) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kRbWaves; w++) c += s_h[w][d];
+    counts[(uint64_t)d * nranges + blockIdx.x] = c;
+  }
 }
 
+// One range per workgroup: tiles in order, each ranked, reordered through LDS
+// and written at the running offsets of its digits (offs: the scanned
+// counts).  With limit != 0 payload column 0 is checked against it.
 template <int NP, bool WRITE_KEYS>
-__global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(
+__global__ __launch_bounds__(kRbBlock) void k_rb_scatter(
     const uint32_t *__restrict__ kin, uint32_t *__restrict__ kout,
     const uint32_t *__restrict__ p0i, uint32_t *__restrict__ p0o,
     const uint32_t *__restrict__ p1i, uint32_t *__restrict__ p1o,
     const uint32_t *__restrict__ p2i, uint32_t *__restrict__ p2o, uint64_t count,
-    int shift, int nbits, uint32_t ntiles, const uint32_t *__restrict__ offs) {
-  __shared__ uint32_t s_wh[kRadixWaves][256];
+    uint64_t span, int shift, int nbits, uint32_t nranges, const uint32_t *__restrict__ offs,
+    uint32_t limit, uint32_t *__restrict__ flag) {
+  __shared__ uint32_t s_wh[kRbWaves][256];
   __shared__ uint32_t s_tstart[256];
   __shared__ uint32_t s_gofs[256];
-  __shared__ uint32_t s_wsum[kRadixWaves];
-  __shared__ uint32_t s_key[kRadixTile];
-  __shared__ uint32_t s_pay[NP > 0 ? NP : 1][kRadixTile];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_buf[kRbTile];
 
   const int tid = threadIdx.x;
   const int lane = __lane_id();
   const int wave = tid >> 6;
   const uint32_t mask = (1u << nbits) - 1u;
-  const uint64_t tbase = (uint64_t)blockIdx.x * kRadixTile;
-  const uint32_t* pin[3] = {p0i, p1i, p2i};
-  uint32_t* pout[3] = {p0o, p1o, p2o};
-
-#pragma unroll
-  for (int w = 0; w < kRadixWaves; w++) s_wh[w][tid] = 0;
-  s_gofs[tid] = tid <= (int)mask ? offs[(uint64_t)tid * ntiles + blockIdx.x] : 0u;
-  __syncthreads();
-
-  uint32_t key[kRadixItems];
-  uint32_t rank[kRadixItems];
-  uint32_t pay[NP > 0 ? NP : 1][kRadixItems];
-  const uint64_t wbase = tbase + (uint64_t)wave * 64 * kRadixItems;
+  const uint32_t *pin[3] = {p0i, p1i, p2i};
+  uint32_t *pout[3] = {p0o, p1o, p2o};
   const uint64_t lt = eg_lanemask_lt();
-#pragma unroll
-  for (int s = 0; s < kRadixItems; s++) {
-    const uint64_t i = wbase + (uint64_t)s * 64 + lane;
-    const bool valid = i < count;
-    const uint32_t k = valid ? kin[i] : 0u;
-    key[s] = k;
-#pragma unroll
-    for (int p = 0; p < NP; p++) pay[p][s] = valid ? pin[p][i] : 0u;
-    const uint32_t d = (k >> shift) & mask;
-    uint64_t peers = __ballot(valid);
-    for (int b = 0; b < nbits; b++) {
-      const int bit = (d >> b) & 1;
-      const uint64_t bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
-    }
-    const uint32_t r = __popcll(peers & lt);
-    rank[s] = 0;
-    if (valid) {
-      const uint32_t base = s_wh[wave][d];
-      rank[s] = base + r;
-      if (r == 0) s_wh[wave][d] = base + (uint32_t)__popcll(peers);
-    }
-  }
-  __syncthreads();
+  const uint64_t lo = (uint64_t)blockIdx.x * span;
+  const uint64_t hi = lo + span < count ? lo + span : count;
+  if (tid < 256) s_gofs[tid] = tid <= (int)mask ? offs[(uint64_t)tid * nranges + blockIdx.x] : 0u;
+  bool bad = false;
 
-  // Per digit (one thread each): exclusive prefix over waves and the tile's
-  // exclusive scan over digits.
-  {
-    const int d = tid;
-    uint32_t run = 0;
-#pragma unroll
-    for (int w = 0; w < kRadixWaves; w++) {
-      const uint32_t c = s_wh[w][d];
-      s_wh[w][d] = run;
-      run += c;
-    }
-    const uint32_t inc = wave_incl_scan(run);
-    if (lane == 63) s_wsum[wave] = inc;
+  for (uint64_t tbase = lo; tbase < hi; tbase += kRbTile) {
+    for (int j = tid; j < kRbWaves * 256; j += kRbBlock) (&s_wh[0][0])[j] = 0;
     __syncthreads();
-    uint32_t off = 0;
-    for (int w = 0; w < wave; w++) off += s_wsum[w];
-    s_tstart[d] = off + inc - run;
-  }
-  __syncthreads();
-
+    uint32_t key[kRbItems], pos[kRbItems];
+    uint32_t pay[NP > 0 ? NP : 1][kRbItems];
+    const uint64_t wbase = tbase + (uint64_t)wave * 64 * kRbItems;
 #pragma unroll
-  for (int s = 0; s < kRadixItems; s++) {
-    const uint64_t i = wbase + (uint64_t)s * 64 + lane;
-    if (i < count) {
-      const uint32_t d = (key[s] >> shift) & mask;
-      const uint32_t pos = s_tstart[d] + s_wh[wave][d] + rank[s];
-      s_key[pos] = key[s];
+    for (int s = 0; s < kRbItems; s++) {
+      const uint64_t i = wbase + (uint64_t)s * 64 + lane;
+      const bool valid = i < hi;
+      key[s] = valid ? kin[i] : 0u;
 #pragma unroll
-      for (int p = 0; p < NP; p++) s_pay[p][pos] = pay[p][s];
+      for (int p = 0; p < NP; p++) pay[p][s] = valid ? pin[p][i] : 0u;
+      if (NP > 0 && limit && valid) bad |= pay[0][s] >= limit;
     }
-  }
-  __syncthreads();
-
-  const uint64_t rem = count - tbase;
-  const uint32_t nvalid = rem < kRadixTile ? (uint32_t)rem : kRadixTile;
-  for (uint32_t j = tid; j < nvalid; j += kRadixBlock) {
-    const uint32_t k = s_key[j];
-    const uint32_t d = (k >> shift) & mask;
-    const uint32_t dst = s_gofs[d] + (j - s_tstart[d]);
-    if (WRITE_KEYS) kout[dst] = k;
+    // stable rank of each item among the wave's items of its digit
 #pragma unroll
-    for (int p = 0; p < NP; p++) pout[p][dst] = s_pay[p][j];
+    for (int s = 0; s < kRbItems; s++) {
+      const uint64_t i = wbase + (uint64_t)s * 64 + lane;
+      const bool valid = i < hi;
+      const uint32_t d = (key[s] >> shift) & mask;
+      uint64_t peers = __ballot(valid);
+      for (int b = 0; b < nbits; b++) {
+        const int bit = (d >> b) & 1;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      const uint32_t r = __popcll(peers & lt);
+      pos[s] = 0;
+      if (valid) {
+        const uint32_t base = s_wh[wave][d];
+        pos[s] = base + r;
+        if (r == 0) s_wh[wave][d] = base + (uint32_t)__popcll(peers);
+      }
+    }
+    __syncthreads();
+    // per digit (threads 0..255): prefix over waves and the tile's scan over digits
+    uint32_t run = 0, excl = 0;
+    if (tid < 256) {
+#pragma unroll
+      for (int w = 0; w < kRbWaves; w++) {
+        const uint32_t c = s_wh[w][tid];
+        s_wh[w][tid] = run;
+        run += c;
+      }
+      const uint32_t inc = wave_incl_scan(run);
+      excl = inc - run;
+      if (lane == 63) s_wsum[wave] = inc;
+    }
+    __syncthreads();
+    if (tid < 256) {
+      uint32_t off = 0;
+      for (int w = 0; w < wave; w++) off += s_wsum[w];
+      s_tstart[tid] = off + excl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kRbItems; s++) {
+      const uint64_t i = wbase + (uint64_t)s * 64 + lane;
+      if (i < hi) {
+        const uint32_t d = (key[s] >> shift) & mask;
+        pos[s] += s_tstart[d] + s_wh[wave][d];
+        s_buf[pos[s]] = key[s];
+      }
+    }
+    __syncthreads();
+    const uint64_t rem = hi - tbase;
+    const uint32_t nvalid = rem < kRbTile ? (uint32_t)rem : kRbTile;
+    uint32_t dst[kRbItems];
+#pragma unroll
+    for (int r = 0; r < kRbItems; r++) {
+      const uint32_t j = tid + (uint32_t)r * kRbBlock;
+      dst[r] = 0;
+      if (j < nvalid) {
+        const uint32_t k = s_buf[j];
+        const uint32_t d = (k >> shift) & mask;
+        dst[r] = s_gofs[d] + (j - s_tstart[d]);
+        if (WRITE_KEYS) kout[dst[r]] = k;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < kRbItems; s++) {
+        const uint64_t i = wbase + (uint64_t)s * 64 + lane;
+        if (i < hi) s_buf[pos[s]] = pay[p][s];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kRbItems; r++) {
+        const uint32_t j = tid + (uint32_t)r * kRbBlock;
+        if (j < nvalid) pout[p][dst[r]] = s_buf[j];
+      }
+    }
+    __syncthreads();  // every thread has read s_gofs / s_tstart / s_buf of this tile
+    if (tid < 256) s_gofs[tid] += run;
   }
+  if (limit && __ballot(bad) && lane == 0) atomicOr(flag, 1u);
 }
 
 template <int NP>
-int launch_scatter(hipStream_t s, bool wk, uint32_t ntiles, const uint32_t *kin,
-                   uint32_t *kout, const uint32_t *const *pi, uint32_t *const *po,
-                   uint64_t count, int shift, int nbits, const uint32_t *offs) {
+void launch_rb_scatter(hipStream_t s, bool wk, uint32_t nranges, const uint32_t *kin,
+                       uint32_t *kout, const uint32_t *const *pi, uint32_t *const *po,
+                       uint64_t count, uint64_t span, int shift, int nbits, const uint32_t *offs,
+                       uint32_t limit, uint32_t *flag) {
   if (wk)
-    hipLaunchKernelGGL((k_radix_scatter<NP, true>), dim3(ntiles), dim3(kRadixBlock), 0, s,
-                       kin, kout, pi[0], po[0], pi[1], po[1], pi[2], po[2], count, shift,
-                       nbits, ntiles, offs);
+    hipLaunchKernelGGL((k_rb_scatter<NP, true>), dim3(nranges), dim3(kRbBlock), 0, s, kin, kout,
+                       pi[0], po[0], pi[1], po[1], pi[2], po[2], count, span, shift, nbits,
+                       nranges, offs, limit, flag);
   else
-    hipLaunchKernelGGL((k_radix_scatter<NP, false>), dim3(ntiles), dim3(kRadixBlock), 0, s,
-                       kin, kout, pi[0], po[0], pi[1], po[1], pi[2], po[2], count, shift,
-                       nbits, ntiles, offs);
-  EG_LAUNCHED();
-  return EG_OK;
+    hipLaunchKernelGGL((k_rb_scatter<NP, false>), dim3(nranges), dim3(kRbBlock), 0, s, kin, kout,
+                       pi[0], po[0], pi[1], po[1], pi[2], po[2], count, span, shift, nbits,
+                       nranges, offs, limit, flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -520,23 +578,35 @@ int eg_bits_for(uint64_t nkeys) {
 
 static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
+// Range geometry of a sort of count items: nranges ranges of span items.
+static void rb_geometry(uint64_t count, uint32_t *nranges, uint64_t *span) {
+  const uint64_t ntiles = (count + kRbTile - 1) / kRbTile;
+  const uint64_t per = (ntiles + kRbMaxRanges - 1) / kRbMaxRanges;  // tiles per range
+  const uint64_t tp = per ? per : 1;
+  *span = tp * kRbTile;
+  *nranges = (uint32_t)((ntiles + tp - 1) / tp);
+  if (*nranges == 0) *nranges = 1;
+}
+
 size_t eg_radix_tmp_bytes(uint64_t count, int np) {
-  const uint64_t ntiles = (count + kRadixTile - 1) / kRadixTile;
-  const uint64_t cells = 256 * (ntiles ? ntiles : 1);
-  size_t b = 0;
-  b += 2 * align256(count * 4) * (size_t)(1 + np);  // ping-pong keys + payloads
-  b += align256((cells + 1) * 4);                   // count matrix (scanned in place)
+  uint32_t nr;
+  uint64_t span;
+  rb_geometry(count, &nr, &span);
+  const uint64_t cells = 256ull * nr;
+  size_t b = 2 * align256(count * 4) * (size_t)(1 + np);  // ping-pong keys + payloads
+  b += align256((cells + 1) * 4);                         // range counts (scanned in place)
   b += align256(eg_scan_tmp_bytes(cells));
   return b;
 }
 
 int eg_radix_sort(hipStream_t s, uint64_t count, int key_bits, const uint32_t *kin,
-                  uint32_t *kout, const EgSortCols &cols, void *tmp) {
+                  uint32_t *kout, const EgSortCols &cols, void *tmp, const EgIdCheck *chk) {
   if (count == 0) return EG_OK;
   if (count >= (1ull << 32))
     return eg_fail(EG_ENOTSUP, "radix sort: %llu items exceed 2^32",
                    (unsigned long long)count);
   const int np = cols.np;
+  if (np < 0 || np > 3) return eg_fail(EG_EINVAL, "radix sort: %d payload columns", np);
   uint8_t *t = reinterpret_cast<uint8_t *>(tmp);
   uint32_t *kb[2];
   uint32_t *pb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
@@ -548,10 +618,21 @@ int eg_radix_sort(hipStream_t s, uint64_t count, int key_bits, const uint32_t *k
       t += align256(count * 4);
     }
   }
-  const uint32_t ntiles = (uint32_t)((count + kRadixTile - 1) / kRadixTile);
+  uint32_t nranges;
+  uint64_t span;
+  rb_geometry(count, &nranges, &span);
   uint32_t *counts = reinterpret_cast<uint32_t *>(t);
-  t += align256(((uint64_t)256 * ntiles + 1) * 4);
+  t += align256(((uint64_t)256 * nranges + 1) * 4);
   void *scan_tmp = t;
+  // ids: keys in the first histogram, vals as payload column 0 of the first
+  // scatter (or by a separate pass when vals is not a payload column)
+  const uint32_t limit = chk ? chk->limit : 0u;
+  const bool vals_in_scatter = chk && chk->vals && np > 0 && chk->vals == cols.in[0];
+  if (limit && chk->vals && !vals_in_scatter) {
+    hipLaunchKernelGGL(k_check_ids, dim3(record_grid(count)), dim3(256), 0, s, kin, chk->vals,
+                       count, limit, chk->flag);
+    EG_LAUNCHED();
+  }
 
   int bits = key_bits < 1 ? 1 : key_bits;
   const int passes = (bits + 7) / 8;
@@ -566,20 +647,21 @@ int eg_radix_sort(hipStream_t s, uint64_t count, int key_bits, const uint32_t *k
     uint32_t *nk = last ? kout : kb[pass & 1];
     uint32_t *np_out[3] = {nullptr, nullptr, nullptr};
     for (int p = 0; p < np; p++) np_out[p] = last ? cols.out[p] : pb[pass & 1][p];
-
-    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kRadixBlock), 0, s, cur_k, count,
-                       shift, mask, ntiles, counts);
+    const uint32_t klimit = pass == 0 ? limit : 0u;
+    const uint32_t plimit = pass == 0 && vals_in_scatter ? limit : 0u;
+    hipLaunchKernelGGL(k_rb_hist, dim3(nranges), dim3(kRbBlock), 0, s, cur_k, count, span,
+                       shift, mask, nranges, counts, klimit, chk ? chk->flag : nullptr);
     EG_LAUNCHED();
-    const uint64_t cells = (uint64_t)(mask + 1) * ntiles;
-    EG_TRY(eg_scan_excl(s, counts, counts, cells, scan_tmp));
+    EG_TRY(eg_scan_excl(s, counts, counts, (uint64_t)(mask + 1) * nranges, scan_tmp));
     const bool wk = nk != nullptr;
+    uint32_t *fl = chk ? chk->flag : nullptr;
     switch (np) {
-      case 0: EG_TRY(launch_scatter<0>(s, wk, ntiles, cur_k, nk, cur_p, np_out, count, shift, nb, counts)); break;
-      case 1: EG_TRY(launch_scatter<1>(s, wk, ntiles, cur_k, nk, cur_p, np_out, count, shift, nb, counts)); break;
-      case 2: EG_TRY(launch_scatter<2>(s, wk, ntiles, cur_k, nk, cur_p, np_out, count, shift, nb, counts)); break;
-      case 3: EG_TRY(launch_scatter<3>(s, wk, ntiles, cur_k, nk, cur_p, np_out, count, shift, nb, counts)); break;
-      default: return eg_fail(EG_EINVAL, "radix sort: %d payload columns", np);
+      case 0: launch_rb_scatter<0>(s, wk, nranges, cur_k, nk, cur_p, np_out, count, span, shift, nb, counts, plimit, fl); break;
+      case 1: launch_rb_scatter<1>(s, wk, nranges, cur_k, nk, cur_p, np_out, count, span, shift, nb, counts, plimit, fl); break;
+      case 2: launch_rb_scatter<2>(s, wk, nranges, cur_k, nk, cur_p, np_out, count, span, shift, nb, counts, plimit, fl); break;
+      default: launch_rb_scatter<3>(s, wk, nranges, cur_k, nk, cur_p, np_out, count, span, shift, nb, counts, plimit, fl); break;
     }
+    EG_LAUNCHED();
     if (!last && nk == nullptr) return eg_fail(EG_EINVAL, "radix sort: key buffer missing");
     cur_k = nk;
     for (int p = 0; p < np; p++) cur_p[p] = np_out[p];
