@@ -225,3 +225,32 @@ def test_determinism_repeat_build():
             g.load_edges(s, d)
             outs.append(g.csc())
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("undirected", [False, True])
+def test_multi_tile_ranges_bit_exact(undirected):
+    """Past 2048 tiles of 8192 records a sort range holds several tiles (run
+    offsets cross tiles and ranges), and the last vertices have no entries."""
+    n = (1 << 20) + 777
+    s, d = O.gen_rmat(20, 20 << 20, seed=17)
+    with eg.Graph(n, 2, undirected=undirected) as g:
+        g.load_edges(s, d)
+        ss, sd, _ = stored(s, d, None, undirected)
+        assert ss.size > 2048 * 8192
+        ptr, col, _ = O.build_csr(n, ss, sd)
+        cp, row, _ = O.build_csc(n, ptr, col)
+        gptr, gcol, _ = g.csr()
+        assert np.array_equal(gptr, ptr) and np.array_equal(gcol, col)
+        hp, hr, _ = g.csc()
+        assert np.array_equal(hp, cp) and np.array_equal(hr, row)
+        od, idg = g.degrees()
+        assert np.array_equal(od, np.diff(ptr)) and np.array_equal(idg, np.diff(cp))
+
+
+def test_empty_and_single_record():
+    for s in (np.zeros(0, np.uint32), np.array([41], np.uint32)):
+        with eg.Graph(64, 2) as g:
+            g.load_edges(s, s)
+            ptr, col, _ = g.csr()
+            assert np.array_equal(ptr, np.concatenate([[0], np.cumsum(np.bincount(s, minlength=64))]))
+            assert np.array_equal(col, s)
