@@ -90,6 +90,7 @@ struct PprState {
   uint32_t max_grid = 0;      // workgroups of the SpMM launch (0: one per work item)
   bool capturing = false;     // a step is being recorded into a hipGraph
   uint32_t full_steps = 0;    // steps over every chunk since the last column change
+  bool skip_y = false;        // non-delta steps may leave Y unwritten (a later step writes it)
 };
 int eg_ppr_alloc(eg_graph *g, PprState *st);
 void eg_ppr_free(PprState *st);

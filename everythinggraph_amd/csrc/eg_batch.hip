@@ -932,17 +932,19 @@ __device__ __forceinline__ void spmm_slot(const uint32_t *__restrict__ idx,
   }
 }
 
-// Writes row v of Y and Cn; returns |y - x| (x = the row of X) when DELTA.
+// Writes row v of Cn, and of Y unless write_y is 0 (a step whose Y no later
+// step reads: the gathers read Cn only, and X only for the DELTA change);
+// returns |y - x| (x = the row of X) when DELTA, which needs write_y.
 template <bool DELTA>
 __device__ __forceinline__ double spmm_finish(uint32_t v, float sum, int lane, bool on,
                                               const uint64_t *__restrict__ mask, float seed_term,
                                               const uint32_t *__restrict__ deg, float one_minus_a,
-                                              const float *__restrict__ X, float *__restrict__ Y,
-                                              float *__restrict__ Cn) {
+                                              uint32_t write_y, const float *__restrict__ X,
+                                              float *__restrict__ Y, float *__restrict__ Cn) {
   if (!on) return 0.0;
   float y = one_minus_a * sum;
   if ((mask[v] >> lane) & 1ull) y += seed_term;
-  Y[(uint64_t)v * 64 + lane] = y;
+  if (DELTA || write_y) Y[(uint64_t)v * 64 + lane] = y;
   const uint32_t d = deg[v];
   Cn[(uint64_t)v * 64 + lane] = d ? y / (float)d : 0.f;
   if (!DELTA) return 0.0;
@@ -985,8 +987,8 @@ __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
     float2 *__restrict__ seg_part, const uint32_t *__restrict__ ptr,
     const float *__restrict__ C, const uint64_t *__restrict__ mask,
     const float *__restrict__ alpha_inv_seeds, const uint32_t *__restrict__ deg,
-    float one_minus_a, uint32_t width, const float *__restrict__ X, float *__restrict__ Y,
-    float *__restrict__ Cn, double *__restrict__ dpart) {
+    float one_minus_a, uint32_t width, uint32_t write_y, const float *__restrict__ X,
+    float *__restrict__ Y, float *__restrict__ Cn, double *__restrict__ dpart) {
   constexpr int R = 64 / W;                       // rows per wave
   constexpr int kUnits = kWaves * R;              // row slots per workgroup
   constexpr int kBinsPerUnit = kDeltaBins / kUnits;
@@ -1036,7 +1038,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_spmm(
         spmm_slot<D, W>(idx, C, row ? ptr[v] : 0u, row ? ptr[v + 1] : 0u, slot, q, on, acc);
         if (row) {
           const double d = spmm_finish<DELTA>(v, acc.value(), q, on, mask, seed_term, deg,
-                                              one_minus_a, X, Y, Cn);
+                                              one_minus_a, write_y, X, Y, Cn);
           if (DELTA) dbin[j % kBinsPerUnit] += d;
         }
       }
@@ -1063,8 +1065,8 @@ __global__ __launch_bounds__(kBlock) void k_ppr_finish(
     const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
     const float2 *__restrict__ seg_part, const uint64_t *__restrict__ mask,
     const float *__restrict__ alpha_inv_seeds, const uint32_t *__restrict__ deg,
-    float one_minus_a, uint32_t width, const float *__restrict__ X, float *__restrict__ Y,
-    float *__restrict__ Cn, double *__restrict__ dpart, uint32_t slot0) {
+    float one_minus_a, uint32_t width, uint32_t write_y, const float *__restrict__ X,
+    float *__restrict__ Y, float *__restrict__ Cn, double *__restrict__ dpart, uint32_t slot0) {
   const int lane = __lane_id();
   const bool on = (uint32_t)lane < width;
   const uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1076,7 +1078,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_finish(
       acc.merge(p.x, p.y);
     }
     d = spmm_finish<DELTA>(long_rows[j], acc.value(), lane, on, mask, alpha_inv_seeds[lane], deg,
-                           one_minus_a, X, Y, Cn);
+                           one_minus_a, write_y, X, Y, Cn);
   }
   store_delta<DELTA>(d, dpart, slot0 + blockIdx.x);
 }
@@ -1674,37 +1676,38 @@ int eg_ppr_col_init(eg_graph *g, const BatchView &bv, PprState *st, uint32_t q,
 // columns, with the tuned gather depth.
 template <bool DELTA, int D, int W>
 static void spmm_go(hipStream_t s, const BatchView &bv, const PprState *st, uint32_t grid,
-                    uint32_t nitems, float oma, uint32_t width) {
+                    uint32_t nitems, float oma, uint32_t width, uint32_t write_y) {
   const PullPlan &pl = *bv.pl;
   hipLaunchKernelGGL((k_ppr_spmm<DELTA, D, W>), dim3(grid), dim3(kBlock), 0, s, bv.idx,
                      pl.chunk_info, pl.seg_info, pl.nseg, nitems, st->seg_part, bv.ptr, st->C,
-                     st->mask, st->w + 64, bv.deg, oma, width, st->X, st->Y, st->Cn,
+                     st->mask, st->w + 64, bv.deg, oma, width, write_y, st->X, st->Y, st->Cn,
                      DELTA ? st->dpart : nullptr);
 }
 
 template <bool DELTA, int D>
 static void spmm_launch_w(hipStream_t s, const BatchView &bv, const PprState *st, uint32_t grid,
-                          uint32_t nitems, float oma, uint32_t width) {
+                          uint32_t nitems, float oma, uint32_t width, uint32_t write_y) {
   if (width <= 16)
-    spmm_go<DELTA, D, 16>(s, bv, st, grid, nitems, oma, width);
+    spmm_go<DELTA, D, 16>(s, bv, st, grid, nitems, oma, width, write_y);
   else if (width <= 32)
-    spmm_go<DELTA, D, 32>(s, bv, st, grid, nitems, oma, width);
+    spmm_go<DELTA, D, 32>(s, bv, st, grid, nitems, oma, width, write_y);
   else
-    spmm_go<DELTA, D, 64>(s, bv, st, grid, nitems, oma, width);
+    spmm_go<DELTA, D, 64>(s, bv, st, grid, nitems, oma, width, write_y);
 }
 
 static int launch_spmm(eg_graph *g, const BatchView &bv, const PprState *st, bool delta,
-                       uint32_t grid, uint32_t nitems, float oma, uint32_t width) {
+                       uint32_t grid, uint32_t nitems, float oma, uint32_t width,
+                       uint32_t write_y) {
   const bool d16 = g->tune.spmm_depth == 16;
   hipStream_t s = st->stream;
   if (delta && d16)
-    spmm_launch_w<true, 16>(s, bv, st, grid, nitems, oma, width);
+    spmm_launch_w<true, 16>(s, bv, st, grid, nitems, oma, width, 1u);
   else if (delta)
-    spmm_launch_w<true, 8>(s, bv, st, grid, nitems, oma, width);
+    spmm_launch_w<true, 8>(s, bv, st, grid, nitems, oma, width, 1u);
   else if (d16)
-    spmm_launch_w<false, 16>(s, bv, st, grid, nitems, oma, width);
+    spmm_launch_w<false, 16>(s, bv, st, grid, nitems, oma, width, write_y);
   else
-    spmm_launch_w<false, 8>(s, bv, st, grid, nitems, oma, width);
+    spmm_launch_w<false, 8>(s, bv, st, grid, nitems, oma, width, write_y);
   EG_LAUNCHED();
   return EG_OK;
 }
@@ -1750,20 +1753,24 @@ int eg_ppr_step(eg_graph *g, const BatchView &bv, PprState *st, double alpha, bo
   const uint32_t nchunks = skip ? bv.static_chunk0 : pl.nchunks;
   const uint32_t nitems = pl.nseg + nchunks;
   const uint32_t grid = st->max_grid && st->max_grid < nitems ? st->max_grid : nitems;
+  // Y rows: always with deltas, and in the first two full steps since the
+  // last column change (the static chunks' rows keep those values in both
+  // buffers); otherwise only unless the caller skips them (st->skip_y)
+  const uint32_t write_y = (want_delta || !st->skip_y || st->full_steps < 2) ? 1u : 0u;
   if (!skip) st->full_steps++;
   if (grid) {
     EgProf prof(st->capturing ? nullptr : g, "ppr_spmm", s);
-    EG_TRY(launch_spmm(g, bv, st, want_delta, grid, nitems, oma, width));
+    EG_TRY(launch_spmm(g, bv, st, want_delta, grid, nitems, oma, width, write_y));
   }
   if (pl.nlong) {
     if (want_delta)
       hipLaunchKernelGGL(k_ppr_finish<true>, dim3(nfin), dim3(kBlock), 0, s, pl.long_rows,
                          pl.seg_base, pl.nlong, st->seg_part, st->mask, st->w + 64, bv.deg, oma,
-                         width, st->X, st->Y, st->Cn, st->dpart, pl.nchunks);
+                         width, 1u, st->X, st->Y, st->Cn, st->dpart, pl.nchunks);
     else
       hipLaunchKernelGGL(k_ppr_finish<false>, dim3(nfin), dim3(kBlock), 0, s, pl.long_rows,
                          pl.seg_base, pl.nlong, st->seg_part, st->mask, st->w + 64, bv.deg, oma,
-                         width, st->X, st->Y, st->Cn, nullptr, 0u);
+                         width, write_y, st->X, st->Y, st->Cn, nullptr, 0u);
     EG_LAUNCHED();
   }
   if (want_delta) {
@@ -2024,14 +2031,29 @@ extern "C" int eg_ppr_batch(eg_graph *g, uint32_t k, const uint32_t *seed_ptr,
     for (uint32_t q = 0; q < kk && rc == EG_OK; q++)
       rc = eg_ppr_col_init(g, bv, &b->ppr, q, seeds + seed_ptr[q0 + q],
                            seed_ptr[q0 + q + 1] - seed_ptr[q0 + q], alpha);
+    // Only the last step's Y is read (the output); the steps before it write
+    // Cn alone (eg_ppr_step still writes Y in the first two full steps).
     int it = 0;
     if (rc == EG_OK && g->tune.graphs && iterations >= 6) {
       // two eager steps first, so that the recorded steps skip static chunks
-      for (; it < 2 && rc == EG_OK; it++) rc = eg_ppr_step(g, bv, &b->ppr, alpha);
-      if (rc == EG_OK) rc = ppr_graph_steps(g, bv, &b->ppr, alpha, (iterations - it) / 2);
-      it += 2 * ((iterations - it) / 2);
+      for (; it < 2 && rc == EG_OK; it++) {
+        b->ppr.skip_y = true;
+        rc = eg_ppr_step(g, bv, &b->ppr, alpha);
+      }
+      // replayed pairs, leaving at least one eager step to write Y
+      const int rest = iterations - it;
+      const int pairs = rest % 2 ? rest / 2 : rest / 2 - 1;
+      if (rc == EG_OK && pairs > 0) {
+        b->ppr.skip_y = true;
+        rc = ppr_graph_steps(g, bv, &b->ppr, alpha, pairs);
+        it += 2 * pairs;
+      }
     }
-    for (; it < iterations && rc == EG_OK; it++) rc = eg_ppr_step(g, bv, &b->ppr, alpha);
+    for (; it < iterations && rc == EG_OK; it++) {
+      b->ppr.skip_y = it + 1 < iterations;
+      rc = eg_ppr_step(g, bv, &b->ppr, alpha);
+    }
+    b->ppr.skip_y = false;
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;

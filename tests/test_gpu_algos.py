@@ -509,6 +509,31 @@ def test_ppr_graph_replay_matches_eager():
         assert np.abs(eager[30][q] - ref).sum() <= PR_L1_TOL
 
 
+def test_ppr_batch_without_static_rows():
+    """Batched PPR steps before the last write only the contribution rows;
+    on a graph where every vertex has edges (a directed cycle through all
+    vertices plus R-MAT records: no static rows to skip) the last step alone
+    must produce every row, eager and replayed alike."""
+    n = 1 << 12
+    s0, d0 = rmat(12, seed=57)
+    ring = np.arange(n, dtype=np.uint32)
+    s0 = np.concatenate([s0, ring])
+    d0 = np.concatenate([d0, (ring + 1) % n]).astype(np.uint32)
+    _, _, _, ptr, col, _, cp, row, _ = oracle_adj(n, s0, d0, undirected=False)
+    deg = np.diff(ptr).astype(np.uint32)
+    assert deg.min() >= 1
+    seeds = [[1, 2], [7], [100, 3, 3, 9], [4000]]
+    with eg.Graph(n, 2) as g:
+        g.load_edges(s0, d0)
+        for graphs in (0, 1):
+            g.tune_set("graphs", graphs)
+            for it in (3, 4, 6, 7, 30):
+                out = g.ppr_batch(seeds, 0.15, it)[0]
+                for q, sd in enumerate(seeds):
+                    ref = O.ppr(n, cp, row, deg, sd, 0.15, it)
+                    assert np.abs(out[q] - ref).sum() <= PR_L1_TOL, (graphs, it, q)
+
+
 @pytest.mark.parametrize("mode", [0, 6])
 def test_grid_bands_bit_identical(mode):
     """Grid PageRank / SpMV give the same bits for every source-band width."""
