@@ -142,7 +142,8 @@ def main():
         p, v = kv.split("=")
         g.tune_set(p, int(v))
     tuned = {p: g.tune_get(p) for p in ("pr_binned", "pr_xcd", "pr_scatter", "pull_batch", "pull_hot",
-                                        "pull_persist", "spmm_depth", "ms_batch")}
+                                        "pull_persist", "spmm_depth", "ms_batch",
+                                        "ms_push_div")}
 
     # ---- PageRank: warmup + timed steps ----
     iters = args.iterations

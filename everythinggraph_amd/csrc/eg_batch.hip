@@ -1317,8 +1317,8 @@ static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *pu
   EG_HIP(hipMemcpyAsync(st->h, st->ctr, 24, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
   const uint64_t cnt = st->h[1], deg = st->h[2];
-  // bfs_simple.c:191: pull once the frontier's edges exceed m/20
-  *push = bv.sym && (double)(cnt + deg) <= (double)g->m / 20.0;
+  // bfs_simple.c:191: pull once the frontier's edges exceed m/20 (m / ms_push_div)
+  *push = bv.sym && (double)(cnt + deg) <= (double)g->m / (double)g->tune.ms_push_div;
   st->qdeg = deg;
   st->qlen = 0;
   if (*push && cnt) {
@@ -1440,10 +1440,11 @@ static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<
   const unsigned long long *qn = st->ctr + 3;
   hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part,
                      st->ctl);
-  hipLaunchKernelGGL(k_msd_decide<NW>, dim3(1), dim3(kBlock), 0, s, part, nb,
-                     (double)g->m / 20.0, bv.sym ? 1 : 0, st->ctl, st->ctr, st->lvl_tab);
+  const double push_max = (double)g->m / (double)g->tune.ms_push_div;
+  hipLaunchKernelGGL(k_msd_decide<NW>, dim3(1), dim3(kBlock), 0, s, part, nb, push_max,
+                     bv.sym ? 1 : 0, st->ctl, st->ctr, st->lvl_tab);
   // push: the frontier's vertices with out-edges, their degree scan, the
-  // edge push (a push level's edges are at most m / 20) and the apply pass
+  // edge push (a push level's edges are at most push_max) and the apply pass
   hipLaunchKernelGGL(k_ms_compact<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr,
                      st->queue, st->ctr + 3, st->ctl);
   hipLaunchKernelGGL(k_msd_qdeg, dim3(256), dim3(256), 0, s, st->queue, qn, bv.ptr, st->qoff,
@@ -1453,7 +1454,8 @@ static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<
   hipLaunchKernelGGL(k_msd_scan_b, dim3(1), dim3(kScanParts), 0, s, st->qoff, qn, spart, st->ctl);
   hipLaunchKernelGGL(k_msd_scan_c, dim3(kScanParts), dim3(kBlock), 0, s, st->qoff, qn, spart,
                      st->ctl);
-  const uint32_t pgrid = std::min<uint32_t>(blocks_for(g->m / 20 + 1, kEdgesPerBlock), kDevGrid);
+  const uint32_t pgrid =
+      std::min<uint32_t>(blocks_for((uint64_t)push_max + 1, kEdgesPerBlock), kDevGrid);
   hipLaunchKernelGGL(k_ms_push<NW>, dim3(pgrid), dim3(kBlock), 0, s, st->queue, 0u, qn, st->qoff,
                      bv.ptr, bv.ms_idx, st->visit, st->seen, st->pend, active, st->ctl);
   const uint32_t nba = std::min(blocks_for(n, kBlock), kDevGrid);
@@ -1487,7 +1489,8 @@ static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<
 template <int NW>
 static int msd_graph(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<NW> &active) {
   uint64_t key[6] = {reinterpret_cast<uint64_t>(bv.pl), reinterpret_cast<uint64_t>(st->visit),
-                     reinterpret_cast<uint64_t>(bv.ms_idx), (uint64_t)g->tune.ms_batch, 0, 0};
+                     reinterpret_cast<uint64_t>(bv.ms_idx),
+                     (uint64_t)g->tune.ms_batch | ((uint64_t)g->tune.ms_push_div << 32), 0, 0};
   for (int j = 0; j < NW; j++) key[4] ^= active.w[j] * (0x9E3779B97F4A7C15ull + 2 * j);
   key[5] = (uint64_t)NW;
   if (st->exec && memcmp(key, st->exec_key, sizeof(key)) == 0) return EG_OK;

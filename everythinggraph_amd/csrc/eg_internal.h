@@ -128,6 +128,9 @@ struct EgTune {
                          // chunk-ordered tiles (0) or runs from every eighth by turns (1)
   int ms_lanes = 256;    // eg_bfs_batch: sources per multi-source traversal (64, 128, 256)
   int ms_device = 1;     // eg_bfs_batch: device-driven levels (1) or host-driven (0)
+  int ms_push_div = 20;  // multi-source BFS on symmetric graphs: a level pushes while the
+                         // union frontier's vertices plus out-degrees are at most
+                         // m / ms_push_div (20: the single-source rule of bfs_simple.c:191)
   int wide_part = 0;     // adjacency loads: build with 64-bit row offsets from sorted parts
                          // of 2^wide_part entries (10..31); 0: only past 2^32 entries,
                          // in parts of 2^31

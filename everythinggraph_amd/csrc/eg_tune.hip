@@ -7,6 +7,10 @@
 //                   lane per gather pass, 8 or 16;
 //  * spmm_depth  -- k_ppr_spmm: row gathers in flight per wave, 8 or 16;
 //  * ms_batch    -- k_ms_pull: entries per lane per index pass, 8 or 16;
+//  * ms_push_div -- multi-source BFS on symmetric graphs: a level pushes while
+//                   the union frontier's vertices plus out-degrees are at most
+//                   m / ms_push_div (20 = the single-source rule of
+//                   bfs_simple.c:191-197; timed: 5, 20, 40);
 //  * pull_persist -- k_pull on the degree-ordered copy: 1 or 2 persistent
 //                   workgroups per CU that keep the leading (highest-degree)
 //                   entries of the gathered vector in LDS; 0 = off;
@@ -47,6 +51,7 @@
 // keeps the fastest.
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "eg_internal.h"
@@ -70,6 +75,7 @@ int *param_slot(eg_graph *g, const char *name) {
   if (!strcmp(name, "pr_xcd")) return &g->tune.pr_xcd;
   if (!strcmp(name, "ms_lanes")) return &g->tune.ms_lanes;
   if (!strcmp(name, "ms_device")) return &g->tune.ms_device;
+  if (!strcmp(name, "ms_push_div")) return &g->tune.ms_push_div;
   if (!strcmp(name, "bfs_order")) return &g->tune.bfs_order;
   if (!strcmp(name, "wcc_order")) return &g->tune.wcc_order;
   if (!strcmp(name, "wide_part")) return &g->tune.wide_part;
@@ -137,6 +143,8 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
       return eg_fail(EG_EINVAL, "ms_lanes takes 64, 128 or 256");
   } else if (!strcmp(param, "ms_device")) {
     if (value != 0 && value != 1) return eg_fail(EG_EINVAL, "ms_device takes 0 or 1");
+  } else if (!strcmp(param, "ms_push_div")) {
+    if (value < 1 || value > 1000) return eg_fail(EG_EINVAL, "ms_push_div takes 1..1000");
   } else if (!strcmp(param, "wide_part")) {
     if (value != 0 && (value < 10 || value > 31))
       return eg_fail(EG_EINVAL, "wide_part takes 0 or a part size exponent 10..31");
@@ -278,17 +286,34 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
     EG_HIP(hipMemcpyAsync(deg.data(), g->out_deg, (size_t)g->n * 4, hipMemcpyDeviceToHost,
                           g->stream));
     EG_HIP(hipStreamSynchronize(g->stream));
-    const uint64_t stride = g->n / 64 ? g->n / 64 : 1;
-    for (uint64_t v = 0; v < g->n && roots.size() < 64; v += stride)
-      if (deg[v]) roots.push_back((uint32_t)v);
-    for (uint32_t v = 0; v < g->n && roots.size() < 64; v++)
-      if (deg[v]) roots.push_back(v);
+    // up to 256 roots drawn uniformly (fixed seed) from the vertices with
+    // edges, as queries arrive: the SpMM takes the first 64, the multi-source
+    // BFS all of them (one traversal of its widest lane count); ids in
+    // order when draws keep missing
+    std::vector<uint8_t> taken(g->n, 0);
+    uint64_t h = 0x243F6A8885A308D3ull;
+    for (uint32_t tries = 0; tries < 64 * 256 && roots.size() < 256; tries++) {
+      h += 0x9E3779B97F4A7C15ull;
+      uint64_t z = h;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      const uint32_t v = (uint32_t)((z ^ (z >> 31)) % g->n);
+      if (deg[v] && !taken[v]) {
+        taken[v] = 1;
+        roots.push_back(v);
+      }
+    }
+    for (uint32_t v = 0; v < g->n && roots.size() < 256; v++)
+      if (deg[v] && !taken[v]) {
+        taken[v] = 1;
+        roots.push_back(v);
+      }
     if (roots.empty()) roots.push_back(0);
   }
   if (kernels & EG_TUNE_SPMM) {
-    std::vector<uint32_t> sp(roots.size() + 1);
-    for (size_t i = 0; i <= roots.size(); i++) sp[i] = (uint32_t)i;
-    const uint32_t k = (uint32_t)roots.size();
+    const uint32_t k = (uint32_t)std::min<size_t>(roots.size(), 64);
+    std::vector<uint32_t> sp(k + 1);
+    for (uint32_t i = 0; i <= k; i++) sp[i] = i;
     for (int v : variants) {
       g->tune.spmm_depth = v;
       double ms;
@@ -301,6 +326,8 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
   }
   if (kernels & EG_TUNE_MSBFS) {
     const uint32_t k = (uint32_t)roots.size();
+    double best = 1e300;
+    int best_v = g->tune.ms_batch;
     for (int v : variants) {
       g->tune.ms_batch = v;
       double ms;
@@ -308,6 +335,23 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
       EG_TRY(time_best(g, 2, [&] { return eg_bfs_batch(g, k, roots.data(), nullptr, nullptr); },
                        &ms));
       res.push_back({"ms_batch", v, ms});
+      if (ms < best) {
+        best = ms;
+        best_v = v;
+      }
+    }
+    g->tune.ms_batch = best_v;
+    // the push / pull threshold of the union frontier (symmetric graphs only
+    // push); levels are the same whichever direction a level takes
+    if ((g->flags & (EG_UNDIRECTED | EG_SYMMETRIC)) || !g->in_ptr) {
+      for (int div : {5, 20, 40}) {
+        g->tune.ms_push_div = div;
+        double ms;
+        EG_TRY(eg_bfs_batch(g, k, roots.data(), nullptr, nullptr));
+        EG_TRY(time_best(g, 2, [&] { return eg_bfs_batch(g, k, roots.data(), nullptr, nullptr); },
+                         &ms));
+        res.push_back({"ms_push_div", div, ms});
+      }
     }
   }
   // keep the fastest variant of each parameter (of equal timings, the first)

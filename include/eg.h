@@ -276,6 +276,10 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * BFS index pass depth), each 8 or 16; "ms_lanes" (sources per multi-source
  * traversal: 64, 128 or 256, default 256); "ms_device" (1: device-driven
  * batched levels, the default; 0: the host decides and waits every level);
+ * "ms_push_div" (1..1000: a multi-source level on a symmetric graph pushes
+ * while its union frontier's vertices plus out-degrees are at most
+ * m / ms_push_div; 20, the default, is bfs_simple.c's rule; levels are the
+ * same for every value);
  * "pull_hot" (sources cached normally in the degree-ordered pull, a vertex
  * count, 0 = all); "pull_persist" (0, or 1-2
  * persistent pull workgroups per CU keeping the highest-degree sources' values
@@ -318,7 +322,7 @@ typedef struct {
   char param[24];
   int32_t value;
   int32_t chosen; /* 1 for the variant kept */
-  double ms;      /* device ms per iteration (pull, SpMM) or per 64-source sweep */
+  double ms;      /* device ms per iteration (pull, SpMM) or per multi-source call */
 } eg_tune_entry;
 int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, int32_t cap,
                 int32_t *count);

@@ -435,6 +435,7 @@ def test_tuning_variants_bit_identical():
                 assert g.tune_get(p) == v
             g.tune_set("pull_hot", 0 if v == 8 else n // 16)
             g.tune_set("pull_persist", 0 if v == 8 else 2)
+            g.tune_set("ms_push_div", 20 if v == 8 else 2)  # more levels pushed
             outs[v] = (g.pagerank(0.85, 20)[0], g.ppr_batch(seeds, 0.15, 20)[0],
                        g.bfs_batch(roots)[0], g.ppr([5, 6], 0.15, tol=0.0, max_iter=25)[0])
         with pytest.raises(eg.EgError) as e:
@@ -442,6 +443,9 @@ def test_tuning_variants_bit_identical():
         assert e.value.code == eg.EG_EINVAL
         with pytest.raises(eg.EgError) as e:
             g.tune_set("no_such_knob", 8)
+        assert e.value.code == eg.EG_EINVAL
+        with pytest.raises(eg.EgError) as e:
+            g.tune_set("ms_push_div", 0)
         assert e.value.code == eg.EG_EINVAL
     for a, b in zip(outs[8], outs[16]):
         assert np.array_equal(a, b)
@@ -464,12 +468,13 @@ def test_autotune_reports_and_keeps_fastest():
         params = {r[0] for r in rep}
         # the hot-set and persistent pull shapes exist only on the degree-ordered copy
         ordered = os.environ.get("EG_PULL_ORDER") != "none"
-        expect = {"pull_batch", "pr_binned", "pr_xcd", "spmm_depth", "ms_batch"}
+        expect = {"pull_batch", "pr_binned", "pr_xcd", "spmm_depth", "ms_batch", "ms_push_div"}
         if ordered:
             expect |= {"pull_hot", "pull_persist"}
         assert params == expect
         assert sorted(r[1] for r in rep if r[0] == "pr_binned") == [0, 1]
         assert sorted(r[1] for r in rep if r[0] == "pr_xcd") == [0, 1]
+        assert sorted(r[1] for r in rep if r[0] == "ms_push_div") == [5, 20, 40]
         if ordered:
             hot = sorted(r[1] for r in rep if r[0] == "pull_hot")
             assert hot == [0, n // 64, n // 16, n // 4]
