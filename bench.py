@@ -88,8 +88,12 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--serve-qps", type=float, action="append", default=None,
-                    help="offered Poisson rate(s) of the serving replay (default 50 and 100)")
+                    help="offered Poisson rate(s) of the serving replay (default 50, 100, "
+                         "200 and a burst)")
     ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-sustained-qps", type=float, default=300.0,
+                    help="offered rate of the sustained-overload replay (0: skip)")
+    ap.add_argument("--serve-sustained-requests", type=int, default=1500)
     ap.add_argument("--serve-ppr-grid", type=int, default=0,
                     help="SpMM workgroups while serving (0: library default)")
     ap.add_argument("--skip-serve", action="store_true")
@@ -299,6 +303,19 @@ def main():
             }
             log(f"serving saturation: engine {b['batched_qps']:.1f} q/s, plain "
                 f"{b['plain_qps']:.1f} q/s (burst of {args.serve_requests})")
+        # sustained overload: an offered rate past the plain path's saturation
+        # and below the engine's, for long enough that a backlog shows
+        if args.serve_sustained_qps > 0:
+            qps, nreq = args.serve_sustained_qps, args.serve_sustained_requests
+            row = {"offered_qps": qps, "requests": nreq}
+            for mode in ("batched", "plain"):
+                r = g.serve_replay(qps, nreq, bfs_fraction=0.8, ppr_max_seeds=8, seed=5,
+                                   plain=(mode == "plain"), ppr_grid=args.serve_ppr_grid)
+                row[mode] = {f: round(r[f], 3) for f in
+                             ("qps", "p50_ms", "p99_ms", "bfs_p99_ms", "ppr_p99_ms")}
+                log(f"serve {mode}: sustained {qps:.0f} q/s offered, {r['completed']} done at "
+                    f"{r['qps']:.1f} q/s; p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms")
+            extra["serving_sustained"] = row
 
     cpu = None
     if not args.skip_cpu:
