@@ -260,20 +260,33 @@ __global__ __launch_bounds__(kBlock) void k_stats_reduce(const double2 *__restri
 }
 
 // ---- edge array ----
+// Adds every record's FP32 term (x[src], or the FP32 product w * x[src]) to
+// its destination's FP64 accumulator: the terms are added in whatever order
+// the atomics land, but in FP64 the sum of up to 2^29 FP32 terms is within a
+// few FP64 ulps of exact, far below the FP32 rounding it is then given.
 template <bool WEIGHTED>
 __global__ void k_edge_push(const uint32_t *__restrict__ src, const uint32_t *__restrict__ dst,
                             const float *__restrict__ w, uint64_t m, const float *__restrict__ x,
-                            float *__restrict__ y) {
+                            double *__restrict__ y) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   float v = x[src[k]];
   if (WEIGHTED) v *= w[k];
-  unsafeAtomicAdd(&y[dst[k]], v);
+  unsafeAtomicAdd(&y[dst[k]], (double)v);
+}
+
+// y[v] = the FP32 rounding of the FP64 sum; the accumulator is cleared for
+// the next pass.
+__global__ void k_edge_round(uint32_t n, double *__restrict__ acc, float *__restrict__ y) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  y[v] = (float)acc[v];
+  acc[v] = 0.0;
 }
 
 __global__ void k_pr_apply(uint32_t n, float base, float damping, const double *dmass_in,
                            double damping_over_n, int uniform,
-                           const uint32_t *__restrict__ out_deg, float *__restrict__ acc,
+                           const uint32_t *__restrict__ out_deg, double *__restrict__ acc,
                            const float *__restrict__ rank_old, float *__restrict__ rank_new,
                            float *__restrict__ contrib_new, double2 *__restrict__ part) {
   __shared__ double s_d[kWaves], s_l[kWaves];
@@ -283,8 +296,8 @@ __global__ void k_pr_apply(uint32_t n, float base, float damping, const double *
   if (uniform) b += damping_over_n * (*dmass_in);
   const float basep = (float)b;
   if (v < n) {
-    const float out = basep + damping * acc[v];
-    acc[v] = 0.f;
+    const float out = basep + damping * (float)acc[v];
+    acc[v] = 0.0;
     rank_new[v] = out;
     const uint32_t d = out_deg[v];
     contrib_new[v] = d ? out / (float)d : 0.f;
@@ -319,7 +332,7 @@ struct GridCompute {
   double2 *part = nullptr;
   double2 *fin_part = nullptr;
   uint32_t *fin_counter = nullptr;
-  float *acc = nullptr;  // edge-array accumulator
+  double *acc = nullptr;  // edge-array accumulator (FP64, cleared after every pass)
   float2 *carry = nullptr;  // grid: accumulators carried between source bands
   double2 *apart = nullptr;
   uint32_t apart_n = 0;
@@ -498,16 +511,20 @@ static int pagerank_cells(eg_graph *g, const eg_pr_opts *o, float *rank_out, dou
   EG_TRY(pr_state(g, iters));
   const uint32_t n = g->n;
   hipStream_t s = g->stream;
+  // each allocated on its own: an edge-array SpMV may have made the
+  // accumulator alone
   if (!grid && !c->acc) {
     EG_TRY(eg_dalloc_t(&c->acc, n, "edge accumulator"));
-    EG_HIP(hipMemsetAsync(c->acc, 0, (size_t)n * 4, s));
+    EG_HIP(hipMemsetAsync(c->acc, 0, (size_t)n * 8, s));
+  }
+  if (!grid && !c->apart) {
     c->apart_n = blocks_for(n, kBlock);
     EG_TRY(eg_dalloc_t(&c->apart, c->apart_n, "apply partials"));
-    if (!c->fin_part) {
-      EG_TRY(eg_dalloc_t(&c->fin_part, 256, "fin partials"));
-      EG_TRY(eg_dalloc_t(&c->fin_counter, 1, "fin counter"));
-      EG_HIP(hipMemsetAsync(c->fin_counter, 0, 4, s));
-    }
+  }
+  if (!grid && !c->fin_part) {
+    EG_TRY(eg_dalloc_t(&c->fin_part, 256, "fin partials"));
+    EG_TRY(eg_dalloc_t(&c->fin_counter, 1, "fin counter"));
+    EG_HIP(hipMemsetAsync(c->fin_counter, 0, 4, s));
   }
   float *rank = g->rank_a, *rank_next = g->rank_b;
   float *contrib = g->contrib_a, *contrib_next = g->contrib_b;
@@ -614,15 +631,22 @@ static int spmv_host(eg_graph *g, const float *x, float *y, bool grid) {
   } else if (grid) {
     EG_TRY(eg_spmv_grid_device(g, g->contrib_a, g->rank_a));
   } else {
-    EG_HIP(hipMemsetAsync(g->rank_a, 0, (size_t)n * 4, s));
+    GridCompute *c = grid_state(g, true);
+    if (!c->acc) {
+      EG_TRY(eg_dalloc_t(&c->acc, n, "edge accumulator"));
+      EG_HIP(hipMemsetAsync(c->acc, 0, (size_t)n * 8, s));
+    }
     EgProf prof(g, "spmv_edges");
     if (g->el_w)
       hipLaunchKernelGGL((k_edge_push<true>), dim3(blocks_for(g->m, 256)), dim3(256), 0, s,
-                         g->el_src, g->el_dst, g->el_w, g->m, g->contrib_a, g->rank_a);
+                         g->el_src, g->el_dst, g->el_w, g->m, g->contrib_a, c->acc);
     else
       hipLaunchKernelGGL((k_edge_push<false>), dim3(blocks_for(g->m, 256)), dim3(256), 0, s,
                          g->el_src, g->el_dst, (const float *)nullptr, g->m, g->contrib_a,
-                         g->rank_a);
+                         c->acc);
+    EG_LAUNCHED();
+    hipLaunchKernelGGL(k_edge_round, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, c->acc,
+                       g->rank_a);
     EG_LAUNCHED();
   }
   if (y) EG_HIP(hipMemcpyAsync(y, g->rank_a, (size_t)n * 4, hipMemcpyDeviceToHost, s));

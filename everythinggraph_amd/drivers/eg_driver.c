@@ -50,6 +50,35 @@ void eg_dump(const void *data, size_t bytes) {
   }
 }
 
+/* EG_TUNE="param=value,param=value": kernel shapes for the graph handle
+ * (eg_tune_set), applied before the file is loaded; no reference
+ * counterpart.  A malformed entry or an unknown parameter is an error. */
+static void apply_tune_env(eg_graph *g) {
+  const char *env = getenv("EG_TUNE");
+  if (!env || !*env) return;
+  char buf[512];
+  if (strlen(env) >= sizeof(buf)) {
+    fprintf(stderr, "EG_TUNE is too long\n");
+    exit(1);
+  }
+  strcpy(buf, env);
+  for (char *item = strtok(buf, ","); item; item = strtok(NULL, ",")) {
+    char *eq = strchr(item, '=');
+    char *end = NULL;
+    if (!eq || eq == item) {
+      fprintf(stderr, "EG_TUNE: expected param=value, got '%s'\n", item);
+      exit(1);
+    }
+    *eq = '\0';
+    const long v = strtol(eq + 1, &end, 10);
+    if (end == eq + 1 || *end != '\0') {
+      fprintf(stderr, "EG_TUNE: bad value for %s\n", item);
+      exit(1);
+    }
+    eg_check(eg_tune_set(g, item, (int32_t)v), "EG_TUNE");
+  }
+}
+
 static void usage(void) {
   /* random.c:32-48, plus the options added by this implementation. */
   printf("Usage: ./random -f <graph file> -n <nb_nodes> [-m -u -r -L -w -b -a [mode] -p [bfs_root] -s -N ]\n");
@@ -127,6 +156,7 @@ int main(int argc, char **argv) {
   if (eg_device_query(&dev) == EG_OK)
     printf("#Running on %s (%d CUs, %.1f GiB HBM)\n", dev.name, dev.compute_units,
            (double)dev.hbm_bytes / (1u << 30));
+  apply_tune_env(G);
   eg_check(eg_graph_load_file(G, filename), "graph load");
   eg_check(eg_graph_get_info(G, &info), "graph info");
   nb_edges = info.m;

@@ -31,13 +31,13 @@ def graph_file(tmp_path_factory):
     return str(path), s, d
 
 
-def run_dump(tmp_path, dtype, drv, *args):
+def run_dump(tmp_path, dtype, drv, *args, env=None):
     """(stdout, the result array the binary wrote through EG_DUMP)."""
     path = tmp_path / f"{drv}.out"
     exe = os.path.join(eg.BIN_DIR, drv)
-    print("driver:", drv, *args, flush=True)
+    print("driver:", drv, *args, env or "", flush=True)
     r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120,
-                       env={**os.environ, "EG_DUMP": str(path)})
+                       env={**os.environ, "EG_DUMP": str(path), **(env or {})})
     assert r.returncode == 0, (drv, args, r.stdout[-2000:], r.stderr[-2000:])
     out = np.fromfile(path, dtype)
     assert out.size == N
@@ -93,6 +93,31 @@ def test_bfs_drivers(graph_file, tmp_path, drv, mode):
     found = int(re.search(r"Total nodes discovered:: (\d+)", out).group(1))
     assert found == np.count_nonzero(ref)
     assert np.array_equal(dist, ref)  # level + 1, 0 = unreached
+
+
+@pytest.mark.parametrize("mode", [2, 4])
+def test_bfs_driver_wide_offsets(graph_file, tmp_path, mode):
+    """bfs_simple on the layout with 64-bit row offsets (what a load past 2^32
+    stored entries builds), forced on a small graph through EG_TUNE: parts of
+    2^12 records, so the rows are placed across several sorted parts."""
+    path, s, d = graph_file
+    root = int(s[0])
+    out, dist = run_dump(tmp_path, np.uint32, "bfs_simple", "-f", path, "-n", N, "-m", mode,
+                         "-u", "-p", root, env={"EG_TUNE": "wide_part=12"})
+    su, du, _ = O.symmetrize(s, d)
+    ptr, col, _ = O.build_csr(N, su, du)
+    cp, row, _ = O.build_csc(N, ptr, col)
+    ref, _, _ = O.bfs(N, su.size, ptr, col, cp, row, root)
+    assert np.array_equal(dist, ref)
+
+
+@pytest.mark.parametrize("tune", ["wide_part", "no_such_knob=1", "pr_binned=x"])
+def test_driver_rejects_bad_tune_env(graph_file, tune):
+    path, _, _ = graph_file
+    exe = os.path.join(eg.BIN_DIR, "bfs_simple")
+    r = subprocess.run([exe, "-f", path, "-n", str(N), "-u"], capture_output=True, text=True,
+                       timeout=120, env={**os.environ, "EG_TUNE": tune})
+    assert r.returncode != 0 and "EG_TUNE" in r.stderr
 
 
 @pytest.mark.parametrize("mode", [2, 8, 0])
