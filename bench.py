@@ -145,9 +145,9 @@ def main():
     for kv in args.tune:
         p, v = kv.split("=")
         g.tune_set(p, int(v))
-    tuned = {p: g.tune_get(p) for p in ("pr_binned", "pr_xcd", "pr_scatter", "pull_batch", "pull_hot",
-                                        "pull_persist", "spmm_depth", "ms_batch",
-                                        "ms_push_div")}
+    tuned = {p: g.tune_get(p) for p in ("pr_binned", "pr_xcd", "pr_scatter", "pr_hot", "pr_persist",
+                                        "pull_batch", "pull_hot", "pull_persist", "spmm_depth",
+                                        "ms_batch", "ms_push_div")}
 
     # ---- PageRank: warmup + timed steps ----
     iters = args.iterations

@@ -64,8 +64,10 @@ constexpr int kTabPer = (kTileBins + 1 + kBlock - 1) / kBlock;  // table loads p
 constexpr int kRowPer = (kBinRows + kBlock - 1) / kBlock;       // rows finished per thread
 constexpr int kRowBatch = 4;                                    // of which loaded together
 constexpr int kFinBlocks = 128;
-constexpr uint32_t kHotMax = 16384;    // hot sources whose values phase 2 keeps in LDS (64 KB)
+constexpr uint32_t kHotMax = 16384;    // hot sources of one window, kept in LDS by phase 2 (64 KB)
+constexpr uint32_t kHotWins = 16;      // hot windows at most
 constexpr int kHotPer = (kHotMax + kBlock - 1) / kBlock;        // hot loads per thread
+static_assert(kHotMax == 1u << 14, "a hot window's offsets take 14 bits");
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
@@ -107,9 +109,10 @@ struct PbTile {
 
 // Phase-2 work item: bin rows [row0, row0 + nrows) with bin-major entries
 // [e0, e1), of which [e0, eh) are hot; nrows == 0 marks a part of a long row
-// whose sum goes to part_sum[row0].
+// whose sum goes to part_sum[row0].  The hot entries of window w are
+// [wseg[ws + w], wseg[ws + w + 1]) (wseg[ws] = e0, wseg[ws + nwin] = eh).
 struct PbItem {
-  uint32_t e0, eh, e1, row0, nrows, pad[3];
+  uint32_t e0, eh, e1, row0, nrows, ws, pad[2];
 };
 
 struct PbDev {
@@ -127,6 +130,9 @@ struct PbDev {
   uint32_t *fin_counter;
   uint32_t nbins, nitems, nlong;
   uint32_t nhot;           // sources below this id are hot
+  uint32_t win_shift;      // hot window w holds sources [w << win_shift, (w + 1) << win_shift)
+  uint32_t nwin;           // hot windows
+  const uint32_t *wseg;    // per item: the nwin + 1 starts of its windows' hot entries
   uint32_t persistent;     // phase 2: workgroups loop over items (else one each)
   uint32_t xcd_tiles;      // phase 1: interleaved walk of the tile list (else launch order)
   const float *w;          // bin-major weights (SpMV of a weighted graph), or null
@@ -148,6 +154,7 @@ __device__ __forceinline__ void pb_entry_keys(uint32_t v, uint32_t k, const PbSr
                                               const uint32_t *__restrict__ bin_row0,
                                               const uint32_t *__restrict__ chunk_src0,
                                               uint32_t nbins, uint32_t nchunks, uint32_t nhot,
+                                              uint32_t win_shift, uint32_t nwin,
                                               int bin_major, uint32_t *__restrict__ key,
                                               uint32_t *__restrict__ pay,
                                               uint32_t *__restrict__ pay2,
@@ -157,7 +164,7 @@ __device__ __forceinline__ void pb_entry_keys(uint32_t v, uint32_t k, const PbSr
   const bool hot = u < nhot;
   const uint32_t c = hot ? 0u : chunk_of[u];
   if (bin_major) {
-    key[k] = b * (nchunks + 1) + (hot ? 0u : 1u + c);
+    key[k] = b * (nchunks + nwin) + (hot ? u >> win_shift : nwin + c);
     pay[k] = v - bin_row0[b];
     pay2[k] = u;
     if (pay3) pay3[k] = __float_as_uint(src.w[k]);
@@ -168,9 +175,10 @@ __device__ __forceinline__ void pb_entry_keys(uint32_t v, uint32_t k, const PbSr
 }
 
 // key / payloads of every entry.  Sources below nhot are hot: they belong to
-// no chunk.
-// bin_major: key = bin * (nchunks + 1) + (hot ? 0 : 1 + chunk), payloads =
-//   row - first row of the bin, the source, and the weight bits (pay3);
+// no chunk but to the hot window source >> win_shift (nwin windows).
+// bin_major: key = bin * (nchunks + nwin) + (hot ? window : nwin + chunk),
+//   payloads = row - first row of the bin, the source, and the weight bits
+//   (pay3);
 // else key = hot ? nchunks * nbins : chunk * nbins + bin, payload = source -
 //   first source of the chunk (hot entries sort last and are not used).
 // Adjacency source: a wave per row, grid-stride; edge list: a thread per entry.
@@ -179,14 +187,14 @@ __global__ void k_pb_keys(uint32_t n, uint64_t m, PbSrcDev src,
                           const uint32_t *__restrict__ chunk_of,
                           const uint32_t *__restrict__ bin_row0,
                           const uint32_t *__restrict__ chunk_src0, uint32_t nbins,
-                          uint32_t nchunks, uint32_t nhot, int bin_major,
-                          uint32_t *__restrict__ key, uint32_t *__restrict__ pay,
+                          uint32_t nchunks, uint32_t nhot, uint32_t win_shift, uint32_t nwin,
+                          int bin_major, uint32_t *__restrict__ key, uint32_t *__restrict__ pay,
                           uint32_t *__restrict__ pay2, uint32_t *__restrict__ pay3) {
   if (src.row) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += stride)
       pb_entry_keys(src.row[k], (uint32_t)k, src, bin_of, chunk_of, bin_row0, chunk_src0, nbins,
-                    nchunks, nhot, bin_major, key, pay, pay2, pay3);
+                    nchunks, nhot, win_shift, nwin, bin_major, key, pay, pay2, pay3);
     return;
   }
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -196,7 +204,7 @@ __global__ void k_pb_keys(uint32_t n, uint64_t m, PbSrcDev src,
     const uint32_t k0 = src.ptr[v], k1 = src.ptr[v + 1];
     for (uint32_t k = k0 + lane; k < k1; k += 64)
       pb_entry_keys(v, k, src, bin_of, chunk_of, bin_row0, chunk_src0, nbins, nchunks, nhot,
-                    bin_major, key, pay, pay2, pay3);
+                    win_shift, nwin, bin_major, key, pay, pay2, pay3);
   }
 }
 
@@ -212,15 +220,16 @@ __global__ void k_pb_absmax(uint64_t m, const float *__restrict__ w, uint32_t *_
 }
 
 // u16 offsets of the sorted order; for the bin-major order also the packed
-// (source | row offset << 16) value slots of the hot entries.
+// (source within its window | row offset << 16) value slots of the hot
+// entries, whose keys modulo key_mod are below nwin.
 __global__ void k_pb_narrow(uint64_t m, const uint32_t *__restrict__ in,
                             uint16_t *__restrict__ out, const uint32_t *__restrict__ skey,
-                            const uint32_t *__restrict__ src, uint32_t key_mod,
-                            uint32_t *__restrict__ packed) {
+                            const uint32_t *__restrict__ src, uint32_t key_mod, uint32_t nwin,
+                            uint32_t win_mask, uint32_t *__restrict__ packed) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   out[k] = (uint16_t)in[k];
-  if (packed && skey[k] % key_mod == 0) packed[k] = src[k] | (in[k] << 16);
+  if (packed && skey[k] % key_mod < nwin) packed[k] = (src[k] & win_mask) | (in[k] << 16);
 }
 
 // ---------------------------------------------------------------------------
@@ -497,20 +506,25 @@ __device__ __forceinline__ void sum_batch(const PbBatch<G> &b, uint32_t base, ui
   }
 }
 
+// The first group of this wave's first block of the range [e0, e1).
+template <int G>
+__device__ __forceinline__ uint32_t first_block(uint32_t e0) {
+  return (e0 >> 3) + (threadIdx.x >> 6) * 64 * G;
+}
+
 // Sums the entries [e0, e1) of one item in 8-entry groups, G groups loaded
 // per lane before any is summed; the workgroup's waves take consecutive
 // 64 G-group blocks, so that together they stream one region of the range.
-// HOTRANGE: the entries are hot (packed value slots).
+// This wave starts at the block of group `base`.  HOTRANGE: the entries are
+// hot (packed value slots).
 template <bool HOTRANGE, bool WEIGHTED, bool SIGNED, int G>
-__device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, uint32_t e1, int S,
-                                                 bool part, unsigned long long *acc,
-                                                 const float *s_hot,
-                                                 unsigned long long &psum) {
-  if (e0 >= e1) return;
+__device__ __forceinline__ void accumulate_from(const PbDev &d, uint32_t base, uint32_t e0,
+                                                uint32_t e1, int S, bool part,
+                                                unsigned long long *acc, const float *s_hot,
+                                                unsigned long long &psum) {
   const int lane = __lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
   const uint32_t g1 = (e1 + 7) >> 3;
-  for (uint32_t base = (e0 >> 3) + wave * 64 * G; base < g1; base += kWaves * 64 * G) {
+  for (; base < g1; base += kWaves * 64 * G) {
     PbBatch<G> b;
     load_batch<HOTRANGE, WEIGHTED, G>(d, base, g1, lane, b);
     sum_batch<HOTRANGE, WEIGHTED, SIGNED, G>(b, base, g1, lane, e0, e1, S, part, acc, s_hot,
@@ -518,9 +532,44 @@ __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, ui
   }
 }
 
+template <bool HOTRANGE, bool WEIGHTED, bool SIGNED, int G>
+__device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, uint32_t e1, int S,
+                                                 bool part, unsigned long long *acc,
+                                                 const float *s_hot,
+                                                 unsigned long long &psum) {
+  if (e0 >= e1) return;
+  accumulate_from<HOTRANGE, WEIGHTED, SIGNED, G>(d, first_block<G>(e0), e0, e1, S, part, acc,
+                                                 s_hot, psum);
+}
+
+// Copies the contributions of hot window w into s_hot.  The loads are issued
+// before the first barrier, so that they overlap the wait for the waves still
+// reading the previous window.
+__device__ __forceinline__ void hot_window(const PbDev &d, const float *__restrict__ contrib,
+                                           uint32_t w, float *s_hot) {
+  const uint32_t first = w << d.win_shift;
+  const uint32_t cnt = min(1u << d.win_shift, d.nhot - first);
+  float hv[kHotPer];
+#pragma unroll
+  for (int j = 0; j < kHotPer; j++) {
+    const uint32_t i = threadIdx.x + j * kBlock;
+    hv[j] = i < cnt ? contrib[first + i] : 0.f;
+  }
+  __syncthreads();  // every wave is done with the previous window
+#pragma unroll
+  for (int j = 0; j < kHotPer; j++) {
+    const uint32_t i = threadIdx.x + j * kBlock;
+    if (i < cnt) s_hot[i] = hv[j];
+  }
+  __syncthreads();
+}
+
 // HOT: persistent workgroups (one per CU) that first copy the hot sources'
 // contributions into LDS, then take items blockIdx.x, blockIdx.x + gridDim.x,
-// ...; otherwise one item per workgroup and no hot range.
+// ...; otherwise one item per workgroup and no hot range.  With several hot
+// windows an item sums each window's entries after loading that window into
+// LDS; consecutive items walk the windows in alternating directions, so an
+// item starts on the window its predecessor ended with.
 // SPMV: rows receive the plain sum (y[r] = sum); otherwise the PageRank
 // update.  WEIGHTED: terms are w * x with the bin-major weights.
 template <bool HOT, bool SPMV, bool WEIGHTED>
@@ -534,18 +583,20 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
   const int lane = __lane_id();
   const uint32_t wave = tid >> 6;
   if (HOT) {
+    const uint32_t cnt = min(1u << d.win_shift, d.nhot);
     float hv[kHotPer];
 #pragma unroll
     for (int j = 0; j < kHotPer; j++) {
       const uint32_t i = tid + j * kBlock;
-      hv[j] = i < d.nhot ? contrib[i] : 0.f;
+      hv[j] = i < cnt ? contrib[i] : 0.f;
     }
 #pragma unroll
     for (int j = 0; j < kHotPer; j++) {
       const uint32_t i = tid + j * kBlock;
-      if (i < d.nhot) s_hot[i] = hv[j];
+      if (i < cnt) s_hot[i] = hv[j];
     }
   }
+  uint32_t loaded = 0u, down = 0u;  // window in s_hot; walk direction of the next item
   // one item per workgroup unless the launch is persistent
   const uint32_t item_end = d.persistent ? d.nitems : blockIdx.x + 1;
   for (uint32_t item = blockIdx.x; item < item_end; item += gridDim.x) {
@@ -557,8 +608,31 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
     unsigned long long psum = 0ull;
     // the hot form runs 16 waves per CU: more loads in flight per lane
     constexpr int G = HOT && !WEIGHTED ? 4 : 2;
-    if (HOT)
+    if (HOT && d.nwin == 1) {
       accumulate_range<true, WEIGHTED, SPMV, G>(d, it.e0, it.eh, S, part, s_acc, s_hot, psum);
+    } else if (HOT) {
+      for (uint32_t k = 0; k < d.nwin; k++) {
+        const uint32_t w = down ? d.nwin - 1 - k : k;
+        const uint32_t a0 = d.wseg[it.ws + w], a1 = d.wseg[it.ws + w + 1];
+        if (a0 == a1) continue;  // the same for the whole workgroup
+        if (w == loaded) {
+          accumulate_range<true, WEIGHTED, SPMV, G>(d, a0, a1, S, part, s_acc, s_hot, psum);
+          continue;
+        }
+        // this wave's first block of the window's entries is loaded before
+        // the window switch, so that its latency overlaps the barriers
+        const uint32_t base = first_block<G>(a0), g1 = (a1 + 7) >> 3;
+        PbBatch<G> b;
+        load_batch<true, WEIGHTED, G>(d, base, g1, lane, b);
+        hot_window(d, contrib, w, s_hot);
+        loaded = w;
+        sum_batch<true, WEIGHTED, SPMV, G>(b, base, g1, lane, a0, a1, S, part, s_acc, s_hot,
+                                           psum);
+        accumulate_from<true, WEIGHTED, SPMV, G>(d, base + kWaves * 64 * G, a0, a1, S, part,
+                                                 s_acc, s_hot, psum);
+      }
+      down ^= 1u;
+    }
     accumulate_range<false, WEIGHTED, SPMV, G>(d, HOT ? it.eh : it.e0, it.e1, S, part, s_acc,
                                                s_hot, psum);
     if (part) {
@@ -716,6 +790,8 @@ struct PbPlan {
   uint32_t tile_len = 0;   // scatter tile length the plan was cut with
   int hot_req = 0;         // hot setting the plan was built for
   uint32_t nhot = 0;       // sources below this id are hot
+  uint32_t win_shift = 0;  // log2 of the sources per hot window
+  uint32_t nwin = 0;       // hot windows (ceil(nhot / window))
   uint32_t cus = 0;        // compute units (persistent accumulation grid)
   uint32_t max_rowlen = 0; // most entries of one row
   float wmax = 1.f;        // largest |w| (1 without weights)
@@ -727,6 +803,7 @@ struct PbPlan {
   float *val = nullptr;
   float *w = nullptr;      // bin-major weights, or null
   PbItem *items = nullptr;
+  uint32_t *wseg = nullptr;  // per item: starts of its hot windows' entries
   uint4 *long_rows = nullptr;
   unsigned long long *part_sum = nullptr;
   double2 *item_part = nullptr, *fin_part = nullptr;
@@ -744,6 +821,7 @@ void eg_pb_free(eg_graph *g) {
   eg_dfree_t(&p->val);
   eg_dfree_t(&p->w);
   eg_dfree_t(&p->items);
+  eg_dfree_t(&p->wseg);
   eg_dfree_t(&p->long_rows);
   eg_dfree_t(&p->part_sum);
   eg_dfree_t(&p->item_part);
@@ -769,12 +847,13 @@ int upload(eg_graph *g, T **dst, const std::vector<T> &src, const char *what) {
 // receives the start of every key's segment (nkeys + 1 entries) on the host.
 int pb_order(eg_graph *g, const PbSrcDev &src, const uint32_t *bin_of, const uint32_t *chunk_of,
              const uint32_t *bin_row0, const uint32_t *chunk_src0, uint32_t nbins,
-             uint32_t nchunks, uint32_t nhot, int bin_major, uint16_t *out, uint32_t *packed,
-             float *wout, std::vector<uint32_t> &seg) {
+             uint32_t nchunks, uint32_t nhot, uint32_t win_shift, uint32_t nwin, int bin_major,
+             uint16_t *out, uint32_t *packed, float *wout, std::vector<uint32_t> &seg) {
   const uint32_t n = g->n;
   const uint64_t m = g->m;
   hipStream_t s = g->stream;
-  const uint64_t nkeys = bin_major ? (uint64_t)nbins * (nchunks + 1) : (uint64_t)nchunks * nbins + 1;
+  const uint64_t nkeys =
+      bin_major ? (uint64_t)nbins * (nchunks + nwin) : (uint64_t)nchunks * nbins + 1;
   const bool wt = bin_major && wout;
   uint32_t *key = nullptr, *pay = nullptr, *pay2 = nullptr, *pay3 = nullptr, *skey = nullptr,
            *spay = nullptr, *spay2 = nullptr, *segd = nullptr;
@@ -794,8 +873,8 @@ int pb_order(eg_graph *g, const PbSrcDev &src, const uint32_t *bin_of, const uin
         src.row ? std::min<uint64_t>(blocks_for(m, 256), 65536)
                 : std::min<uint64_t>(blocks_for((uint64_t)n * 64, 256), 65536);
     hipLaunchKernelGGL(k_pb_keys, dim3(grid), dim3(256), 0, s, n, m, src, bin_of, chunk_of,
-                       bin_row0, chunk_src0, nbins, nchunks, nhot, bin_major, key, pay, pay2,
-                       wt ? pay3 : nullptr);
+                       bin_row0, chunk_src0, nbins, nchunks, nhot, win_shift, nwin, bin_major,
+                       key, pay, pay2, wt ? pay3 : nullptr);
     if (hipGetLastError() != hipSuccess) {
       rc = eg_fail(EG_EDEVICE, "binned key launch");
       break;
@@ -812,7 +891,8 @@ int pb_order(eg_graph *g, const PbSrcDev &src, const uint32_t *bin_of, const uin
     if ((rc = eg_scratch(g, eg_radix_tmp_bytes(m, c.np), &tmp))) break;
     if ((rc = eg_radix_sort(s, m, eg_bits_for(nkeys), key, skey, c, tmp))) break;
     hipLaunchKernelGGL(k_pb_narrow, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, spay, out, skey,
-                       spay2, nchunks + 1, bin_major && nhot ? packed : nullptr);
+                       spay2, nchunks + nwin, nwin, (1u << win_shift) - 1u,
+                       bin_major && nhot ? packed : nullptr);
     if (hipGetLastError() != hipSuccess) {
       rc = eg_fail(EG_EDEVICE, "binned narrow launch");
       break;
@@ -893,15 +973,21 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   }
   EG_TRY(pb_counts(g, src.idx, scnt));
   // 2. hot sources, bins of rows and chunks of the other sources, each
-  // balanced by entries.  The hot copy pays only where the first sources take
-  // a fair share of the entries (degree-ordered or R-MAT ids), so it is
-  // dropped below 15%.  With EG_PB_GEOMETRY=small in the environment every
-  // cap is cut down so that small graphs exercise hot and cold entries in one
-  // bin, long-row parts and tile cuts (tests).
+  // balanced by entries.  Hot sources form windows of kHotMax, one in LDS at a
+  // time.  A hot entry saves about 8 bytes of streaming over a scattered one,
+  // and every window after the first costs each work item a switch (about a
+  // microsecond); items are about m / 2048 entries, so a window pays from
+  // about 2% of the entries on: trailing windows below that are dropped.
+  // The hot copy as a whole pays only where the first sources take a fair
+  // share of the entries (degree-ordered or R-MAT ids), so it is dropped
+  // below 15%.  With EG_PB_GEOMETRY=small in the environment every cap is
+  // cut down so that small graphs exercise hot and cold entries in one bin,
+  // several hot windows, long-row parts and tile cuts (tests).
   uint32_t rows_cap = kBinRows, src_cap = kChunkSrc, tile_len = want_tile, tile_bins = kTileBins;
   uint32_t hot_cap = want_hot > 0 ? (uint32_t)want_hot : 0u;
+  uint32_t win_shift = 14;  // kHotMax sources per window
   uint64_t target = std::max<uint64_t>(m / 2048, 4096);
-  double min_hot_share = 0.15;
+  double min_hot_share = 0.15, min_win_share = 0.02;
   {
     const char *e = getenv("EG_PB_GEOMETRY");
     if (e && !strcmp(e, "small")) {
@@ -911,15 +997,29 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
       tile_bins = 16;
       target = 256;
       hot_cap = std::min<uint32_t>(hot_cap, 40);
+      win_shift = 4;
       min_hot_share = 0.0;
+      min_win_share = 0.0;
     }
   }
   uint32_t nhot = std::min<uint32_t>(hot_cap, n);
   {
-    uint64_t hot_entries = 0;
-    for (uint32_t u = 0; u < nhot; u++) hot_entries += scnt[u];
+    uint64_t hot_entries = 0, win_entries = 0;
+    uint32_t kept = 0;  // sources of the windows kept
+    for (uint32_t u = 0; u < nhot; u++) {
+      win_entries += scnt[u];
+      if (u + 1 == nhot || ((u + 1) & ((1u << win_shift) - 1)) == 0) {  // a window ends
+        if (u >= (1u << win_shift) && (double)win_entries < min_win_share * (double)m) break;
+        hot_entries += win_entries;
+        win_entries = 0;
+        kept = u + 1;
+      }
+    }
+    nhot = kept;
     if ((double)hot_entries < min_hot_share * (double)m) nhot = 0;
   }
+  const uint32_t nwin = (nhot + (1u << win_shift) - 1) >> win_shift;
+  if (nwin > kHotWins) return eg_fail(EG_EINVAL, "binned plan: %u hot windows", nwin);
   uint32_t max_rowlen = 0;
   std::vector<uint32_t> bin_row0, bin_of(n), chunk_src0, chunk_of(n, 0u);
   {
@@ -954,7 +1054,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   }
   const uint32_t nbins = (uint32_t)bin_row0.size() - 1;
   const uint32_t nchunks = (uint32_t)chunk_src0.size() - 1;
-  if ((uint64_t)nbins * (nchunks + 1) >= (1ull << 31))
+  if ((uint64_t)nbins * (nchunks + nwin) >= (1ull << 31))
     return eg_fail(EG_ENOTSUP, "binned plan: %u bins x %u chunks", nbins, nchunks);
 
   PbPlan *p = new PbPlan();
@@ -962,6 +1062,8 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   p->nbins = nbins;
   p->nchunks = nchunks;
   p->nhot = nhot;
+  p->win_shift = win_shift;
+  p->nwin = nwin;
   p->cus = (uint32_t)cus;
   p->max_rowlen = max_rowlen;
   std::vector<uint32_t> seg_bm, seg_cm;
@@ -998,11 +1100,12 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
         }
         memcpy(&p->wmax, &wbits, 4);
       }
-      if ((rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, 1,
-                         p->dloc, reinterpret_cast<uint32_t *>(p->val), p->w, seg_bm)))
+      if ((rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot,
+                         win_shift, nwin, 1, p->dloc, reinterpret_cast<uint32_t *>(p->val), p->w,
+                         seg_bm)))
         break;
-      rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, 0,
-                    p->sloc, nullptr, nullptr, seg_cm);
+      rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, win_shift,
+                    nwin, 0, p->sloc, nullptr, nullptr, seg_cm);
     } while (0);
     (void)hipStreamSynchronize(s);
     eg_dfree_t(&d_bin_of);
@@ -1016,16 +1119,16 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
     }
   }
   // 3. segment table, scatter tiles, accumulation items.  Bin-major key of
-  // (bin b, chunk c) is b * (nchunks + 1) + 1 + c (the bin's hot entries at
-  // + 0 come first); chunk-major key is c * nbins + b.
-  const uint32_t kb = nchunks + 1;
+  // (bin b, chunk c) is b * (nchunks + nwin) + nwin + c (the bin's hot
+  // entries of window w at + w come first); chunk-major key is c * nbins + b.
+  const uint32_t kb = nchunks + nwin;
   std::vector<uint2> tab((size_t)nchunks * (nbins + 1));
   std::vector<uint2> csrc(nchunks);
   std::vector<PbTile> tiles;
   for (uint32_t c = 0; c < nchunks; c++) {
     uint2 *row = &tab[(size_t)c * (nbins + 1)];
     for (uint32_t b = 0; b < nbins; b++)
-      row[b] = make_uint2(seg_cm[(size_t)c * nbins + b], seg_bm[(size_t)b * kb + 1 + c]);
+      row[b] = make_uint2(seg_cm[(size_t)c * nbins + b], seg_bm[(size_t)b * kb + nwin + c]);
     const uint32_t cend = seg_cm[(size_t)(c + 1) * nbins];
     row[nbins] = make_uint2(cend, 0u);
     csrc[c] = make_uint2(chunk_src0[c], chunk_src0[c + 1] - chunk_src0[c]);
@@ -1056,10 +1159,18 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
     }
   }
   std::vector<PbItem> items;
+  std::vector<uint32_t> wseg;  // nwin + 1 window starts per item (at least one slot)
   std::vector<uint4> long_rows;
   uint32_t nparts = 0;
+  // the window starts of bin b clamped to [a0, a1), at the end of wseg
+  auto push_windows = [&](uint32_t b, uint32_t a0, uint32_t a1) {
+    const uint32_t ws = (uint32_t)wseg.size();
+    for (uint32_t w = 0; w <= nwin; w++)
+      wseg.push_back(std::min(std::max(seg_bm[(size_t)b * kb + w], a0), a1));
+    return ws;
+  };
   for (uint32_t b = 0; b < nbins; b++) {
-    const uint32_t e0 = seg_bm[(size_t)b * kb], eh = seg_bm[(size_t)b * kb + 1];
+    const uint32_t e0 = seg_bm[(size_t)b * kb], eh = seg_bm[(size_t)b * kb + nwin];
     const uint32_t e1 = seg_bm[(size_t)(b + 1) * kb];
     const uint32_t r0 = bin_row0[b], nr = bin_row0[b + 1] - r0;
     if (nr == 1 && e1 - e0 > target) {
@@ -1074,6 +1185,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
         it.e1 = a1;
         it.row0 = nparts + q;
         it.nrows = 0;
+        it.ws = push_windows(b, a0, a1);
         items.push_back(it);
       }
       nparts += np;
@@ -1084,6 +1196,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
       it.e1 = e1;
       it.row0 = r0;
       it.nrows = nr;
+      it.ws = push_windows(b, e0, e1);
       items.push_back(it);
     }
   }
@@ -1119,6 +1232,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
         (rc = upload(g, &p->chunk_src, csrc, "binned chunks")) ||
         (rc = upload(g, &p->tiles, tiles, "binned tiles")) ||
         (rc = upload(g, &p->items, items, "binned items")) ||
+        (rc = upload(g, &p->wseg, wseg, "binned hot windows")) ||
         (rc = upload(g, &p->long_rows, long_rows, "binned long rows")))
       break;
     if ((rc = eg_dalloc_t(&p->part_sum, nparts ? nparts : 1, "binned part sums")) ||
@@ -1140,7 +1254,8 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   // tiles, items, long rows and partials
   g->prep_plan_bytes = 8ull * (m + 16) + (src.w ? 4ull * (m + 16) : 0ull) +
                        8ull * p->nchunks * ((uint64_t)p->nbins + 1) + sizeof(PbTile) * p->ntiles +
-                       (sizeof(PbItem) + 16) * p->nitems + 16ull * p->nlong + 8ull * p->nparts;
+                       (sizeof(PbItem) + 16 + 4ull * (p->nwin + 1)) * p->nitems +
+                       16ull * p->nlong + 8ull * p->nparts;
   return EG_OK;
 }
 
@@ -1195,6 +1310,9 @@ static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int
   d.nitems = p->nitems;
   d.nlong = p->nlong;
   d.nhot = p->nhot;
+  d.win_shift = p->win_shift;
+  d.nwin = p->nwin;
+  d.wseg = p->wseg;
   d.xcd_tiles = g->tune.pr_xcd ? 1u : 0u;
   d.w = spmv ? p->w : nullptr;
   const bool wt = d.w != nullptr;

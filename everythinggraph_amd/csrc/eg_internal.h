@@ -117,12 +117,13 @@ struct EgTune {
                          // FP32 sums), -1 = binned when the graph has >= 2^22 entries
   int pr_tile = 0;       // binned PageRank: source offsets per scatter tile (0: 32768)
   int pr_scatter = 16;   // binned scatter: source offsets in flight per lane (8, 16 or 32)
-  int pr_hot = 16384;    // binned PageRank: sources below this id are read from an LDS copy
-                         // in phase 2 instead of being scattered (0: none, at most 16384)
+  int pr_hot = 262144;   // binned PageRank: sources below this id are read from an LDS copy
+                         // in phase 2 instead of being scattered (0: none, at most 262144,
+                         // in windows of 16384 sources; the plan keeps the windows that pay)
   int wcc_order = 1;     // undirected / symmetric graphs: WCC pulls on the ordered ids (0: edges)
   int bfs_order = 2;     // symmetric graphs: BFS on CSR rows ordered hubs first (1), on the
                          // degree-ordered ids with rows hubs first (2), or as built (0)
-  int pr_persist = 0;    // binned PageRank with hot sources: phase-2 workgroups loop over
+  int pr_persist = 1;    // binned PageRank with hot sources: phase-2 workgroups loop over
                          // items (one per CU) instead of one workgroup per item
   int pr_xcd = 0;        // binned scatter: every XCD walks a contiguous eighth of the
                          // chunk-ordered tiles (0) or runs from every eighth by turns (1)

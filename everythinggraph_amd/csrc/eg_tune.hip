@@ -31,7 +31,14 @@
 //  * pr_scatter  -- binned PageRank scatter: source offsets loaded per lane
 //                   before any value is written, 8, 16 or 32;
 //  * pr_hot      -- binned PageRank: sources below this id are read from an
-//                   LDS copy in phase 2 instead of being scattered (0..16384);
+//                   LDS copy in phase 2 instead of being scattered (0..262144,
+//                   in windows of 16384 sources loaded one at a time; the plan
+//                   keeps the leading windows that hold 2% of the entries;
+//                   timed: caps of 1, 4, 8 and 16 windows, each distinct plan
+//                   once);
+//  * pr_persist  -- binned phase 2 with hot sources: persistent workgroups,
+//                   one per CU (1), or one workgroup per work item (0); not
+//                   timed by eg_autotune;
 //  * pr_xcd      -- binned scatter tile walk: 0 = XCD x takes the x-th
 //                   eighth of the chunk-ordered tiles (the default); 1 = XCD x
 //                   takes a short run of tiles from every eighth by turns
@@ -131,7 +138,7 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
     if (value != 8 && value != 16 && value != 32)
       return eg_fail(EG_EINVAL, "pr_scatter takes 8, 16 or 32");
   } else if (!strcmp(param, "pr_hot")) {
-    if (value < 0 || value > 16384) return eg_fail(EG_EINVAL, "pr_hot takes 0..16384");
+    if (value < 0 || value > 262144) return eg_fail(EG_EINVAL, "pr_hot takes 0..262144");
   } else if (!strcmp(param, "bfs_order")) {
     if (value < 0 || value > 2) return eg_fail(EG_EINVAL, "bfs_order takes 0, 1 or 2");
   } else if (!strcmp(param, "pr_persist")) {
@@ -271,6 +278,31 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
       res.push_back({"pr_xcd", x, xms[x] / o.iterations});
     }
     g->tune.pr_xcd = xms[1] < xms[0] ? 1 : 0;
+    // hot window caps (each plan keeps only the windows that hold 2% of the
+    // entries, so caps past the graph's skew give the same plan)
+    {
+      const int caps[4] = {16384, 65536, 131072, 262144};
+      double best = 1e300;
+      int best_cap = g->tune.pr_hot;
+      uint32_t prev_hot = 0xFFFFFFFFu;
+      for (int c : caps) {
+        g->tune.pr_hot = c;
+        EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // warm (binned plan)
+        uint32_t geo[7];
+        EG_TRY(eg_pr_binned_geometry(g, geo));
+        if (geo[6] == prev_hot) continue;  // trimmed to the previous cap's plan
+        prev_hot = geo[6];
+        double ms;
+        EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); },
+                         &ms));
+        res.push_back({"pr_hot", c, ms / o.iterations});
+        if (ms < best) {
+          best = ms;
+          best_cap = c;
+        }
+      }
+      g->tune.pr_hot = best_cap;
+    }
     for (int pb = 0; pb <= 1; pb++) {
       g->tune.pr_binned = pb;
       double ms;

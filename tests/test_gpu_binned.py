@@ -64,9 +64,11 @@ def test_binned_pagerank_matches_oracle(scale, undirected, dangling):
     # from the FP64 oracle than the compensated ones
     assert np.abs(r.astype(np.float64) - rp).sum() <= PR_L1_TOL
     assert err.sum() <= np.abs(rp.astype(np.float64) - ref).sum() * 1.5 + 1e-9
-    # sources below 16384 are hot (read from LDS in phase 2), the rest in chunks
+    # hot sources (read from LDS in phase 2) come in windows of 16384; the
+    # rest are scattered in chunks
     assert geo["bins"] >= 1 and geo["items"] >= geo["bins"]
-    assert (geo["chunks"] >= 1) == (n > 16384)
+    assert geo["hot"] == n or geo["hot"] % 16384 == 0
+    assert (geo["chunks"] >= 1) == (geo["hot"] < n)
 
 
 def test_binned_reference_defaults():
@@ -152,16 +154,19 @@ for und in (False, True):
         out['half%d' % und] = g.pagerank(0.85, 40)[0]
         g.tune_set("pr_xcd", 1)      # scatter tiles dealt to XCDs in contiguous ranges
         out['xcd%d' % und] = g.pagerank(0.85, 40)[0]
+        g.tune_set("pr_persist", 0)  # a phase-2 workgroup per work item
+        out['item%d' % und] = g.pagerank(0.85, 40)[0]
 np.savez(sys.argv[2], **out)
 """
 
 
 def test_binned_independent_of_cut_and_order(tmp_path):
     """The same ranks, bit for bit, with the default bins and chunks, with
-    tiny ones (40 hot sources, long rows in parts, tiles cut by length and by
-    bins), with all, half or none of the sources hot, and with or without the
-    degree-ordered copy (built for runs of 32 iterations or more), and in
-    either scatter tile order."""
+    tiny ones (40 hot sources in three windows of 16, long rows in parts,
+    tiles cut by length and by bins), with all, half or none of the sources
+    hot, and with or without the degree-ordered copy (built for runs of 32
+    iterations or more), in either scatter tile order, and with persistent
+    phase-2 workgroups or one per work item."""
     script = tmp_path / "run.py"
     script.write_text(_GEOMETRY_SCRIPT)
     res = {}
@@ -181,6 +186,7 @@ def test_binned_independent_of_cut_and_order(tmp_path):
             assert np.array_equal(r[f"cold{und}"], first[f"pr{und}"]), (key, und)
             assert np.array_equal(r[f"half{und}"], first[f"pr{und}"]), (key, und)
             assert np.array_equal(r[f"xcd{und}"], first[f"pr{und}"]), (key, und)
+            assert np.array_equal(r[f"item{und}"], first[f"pr{und}"]), (key, und)
     for und in (0, 1):
         small = res[("small", "degree")][f"geo{und}"]
         bins, chunks, tiles, items, long_rows = (int(v) for v in small)
@@ -243,7 +249,43 @@ def test_binned_spmv_every_layout(weighted):
     for mode, y in out.items():
         spmv_check(y, ref, absref)
         assert np.array_equal(y, out[2]), mode
-    assert geo["hot"] == 16384  # the first sources of an R-MAT graph take many entries
+    assert geo["hot"] == n  # the first sources of an R-MAT graph take many entries
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_binned_hot_windows(weighted):
+    """Hot sources in several LDS windows of 16384 (R-MAT 16: 65536 sources):
+    one window, three (the last one partial) and four, each with one phase-2
+    workgroup per work item and with persistent workgroups (which walk the
+    windows in alternating directions), give the PageRank ranks and the SpMV
+    products of the single-window plan bit for bit, and the ranks stay within
+    the oracle's tolerance."""
+    scale = 16
+    n = 1 << scale
+    s0, d0 = rmat(scale, seed=71)
+    w = O.gen_weights(s0.size, seed=5) if weighted else None
+    ref, _, _ = oracle_pr(n, s0, d0, False, 20)
+    x = np.random.default_rng(9).uniform(-2, 2, n).astype(np.float32)
+    out = {}
+    with eg.Graph(n, 2, weighted=weighted) as g:
+        g.load_edges(s0, d0, w)
+        g.tune_set("pr_binned", 1)
+        for hot in (16384, 40000, 65536):
+            for persist in (0, 1):
+                print(f"hot windows: weighted={weighted} pr_hot={hot} persist={persist}",
+                      flush=True)
+                g.tune_set("pr_hot", hot)
+                g.tune_set("pr_persist", persist)
+                r, _, _ = g.pagerank(0.85, 20)
+                y = g.spmv(x)
+                assert g.binned_geometry()["hot"] == hot
+                out[(hot, persist)] = (r, y)
+    r0, y0 = out[(16384, 0)]
+    err = np.abs(r0.astype(np.float64) - ref)
+    assert err.sum() <= PR_L1_TOL and err.max() <= 1e-6 * ref.max()
+    for key, (r, y) in out.items():
+        assert np.array_equal(r, r0), key
+        assert np.array_equal(y, y0), key
 
 
 def test_binned_spmv_nonfinite_falls_back():
@@ -307,10 +349,12 @@ def test_binned_spmv_nonfinite_weights_fall_back(mode, case):
 
 
 def test_binned_uniform_graph_has_no_hot_set():
-    """Erdos-Renyi ids carry no degree skew: the first 16384 sources take
-    under 15% of the entries, so the plan scatters every source."""
-    n = 1 << 18
-    s0, d0 = O.gen_er(n, 1 << 21, seed=64)
+    """Erdos-Renyi ids carry no degree skew: every window of 16384 sources
+    takes 1/64 of the entries (under the 2% a window must hold to be kept
+    beyond the first), and the first alone is under 15%, so the plan scatters
+    every source."""
+    n = 1 << 20
+    s0, d0 = O.gen_er(n, 1 << 22, seed=64)
     ptr, col, _ = O.build_csr(n, s0, d0)
     cp, row, _ = O.build_csc(n, ptr, col)
     ref, _, _ = O.pagerank_pull(n, cp, row, np.diff(ptr).astype(np.uint32), 0.85, 10)
