@@ -608,9 +608,16 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
     unsigned long long psum = 0ull;
     // the hot form runs 16 waves per CU: more loads in flight per lane
     constexpr int G = HOT && !WEIGHTED ? 4 : 2;
+    // Cold entries not yet summed start at cn; with several hot windows,
+    // rounds of them fill the latency of the window switches.
+    const uint32_t c0 = HOT ? it.eh : it.e0;
+    uint32_t cn = c0;
     if (HOT && d.nwin == 1) {
       accumulate_range<true, WEIGHTED, SPMV, G>(d, it.e0, it.eh, S, part, s_acc, s_hot, psum);
     } else if (HOT) {
+      // groups per lane of the hot and cold rounds loaded ahead of a switch
+      constexpr int GS = 2, GC = 1;
+      constexpr uint32_t kRound = kWaves * 64 * GS, kColdRound = kWaves * 64 * GC;
       for (uint32_t k = 0; k < d.nwin; k++) {
         const uint32_t w = down ? d.nwin - 1 - k : k;
         const uint32_t a0 = d.wseg[it.ws + w], a1 = d.wseg[it.ws + w + 1];
@@ -619,22 +626,34 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
           accumulate_range<true, WEIGHTED, SPMV, G>(d, a0, a1, S, part, s_acc, s_hot, psum);
           continue;
         }
-        // this wave's first block of the window's entries is loaded before
-        // the window switch, so that its latency overlaps the barriers
-        const uint32_t base = first_block<G>(a0), g1 = (a1 + 7) >> 3;
-        PbBatch<G> b;
-        load_batch<true, WEIGHTED, G>(d, base, g1, lane, b);
+        // The first round of the window's entries and a round of cold
+        // entries are loaded before the switch, so that their latency
+        // overlaps the barriers.
+        const uint32_t hb = first_block<GS>(a0), hg1 = (a1 + 7) >> 3;
+        const uint32_t cb = first_block<GC>(cn), cg1 = (it.e1 + 7) >> 3;
+        const bool cold = cn < it.e1;  // the same for the whole workgroup
+        PbBatch<GS> b;
+        PbBatch<GC> c;
+        load_batch<true, WEIGHTED, GS>(d, hb, hg1, lane, b);
+        if (cold) load_batch<false, WEIGHTED, GC>(d, cb, cg1, lane, c);
         hot_window(d, contrib, w, s_hot);
         loaded = w;
-        sum_batch<true, WEIGHTED, SPMV, G>(b, base, g1, lane, a0, a1, S, part, s_acc, s_hot,
-                                           psum);
-        accumulate_from<true, WEIGHTED, SPMV, G>(d, base + kWaves * 64 * G, a0, a1, S, part,
-                                                 s_acc, s_hot, psum);
+        sum_batch<true, WEIGHTED, SPMV, GS>(b, hb, hg1, lane, a0, a1, S, part, s_acc, s_hot,
+                                            psum);
+        if (cold) {
+          sum_batch<false, WEIGHTED, SPMV, GC>(c, cb, cg1, lane, cn, it.e1, S, part, s_acc,
+                                               s_hot, psum);
+          cn = (uint32_t)std::min<uint64_t>(((uint64_t)(cn >> 3) + kColdRound) << 3, it.e1);
+        }
+        // the window's entries past its first round (group-aligned start)
+        const uint64_t rest = ((uint64_t)(a0 >> 3) + kRound) << 3;
+        if (rest < a1)
+          accumulate_range<true, WEIGHTED, SPMV, G>(d, (uint32_t)rest, a1, S, part, s_acc, s_hot,
+                                                    psum);
       }
       down ^= 1u;
     }
-    accumulate_range<false, WEIGHTED, SPMV, G>(d, HOT ? it.eh : it.e0, it.e1, S, part, s_acc,
-                                               s_hot, psum);
+    accumulate_range<false, WEIGHTED, SPMV, G>(d, cn, it.e1, S, part, s_acc, s_hot, psum);
     if (part) {
 #pragma unroll
       for (int o = 32; o >= 1; o >>= 1) psum += __shfl_xor(psum, o);
