@@ -66,8 +66,9 @@ constexpr int kRowBatch = 4;                                    // of which load
 constexpr int kFinBlocks = 128;
 constexpr uint32_t kHotMax = 16384;    // hot sources of one window, kept in LDS by phase 2 (64 KB)
 constexpr uint32_t kHotWins = 16;      // hot windows at most
-constexpr int kHotPer = (kHotMax + kBlock - 1) / kBlock;        // hot loads per thread
+constexpr int kHotPer4 = kHotMax / 4 / kBlock;                  // hot 16-byte loads per thread
 static_assert(kHotMax == 1u << 14, "a hot window's offsets take 14 bits");
+static_assert(kHotPer4 * 4 * kBlock == (int)kHotMax, "a hot window is copied in 16-byte pieces");
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
@@ -542,25 +543,42 @@ __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, ui
                                                  s_hot, psum);
 }
 
-// Copies the contributions of hot window w into s_hot.  The loads are issued
-// before the first barrier, so that they overlap the wait for the waves still
-// reading the previous window.
+// This thread's 16-byte pieces of hot window w's contributions (sources
+// w << win_shift ..., cnt of them): piece j holds sources 4 i .. 4 i + 3 of the
+// window, i = threadIdx.x + j * kBlock, zeros past cnt.  The plan keeps nhot
+// (so cnt) a multiple of 4 and the launch checks that contrib is 16-byte
+// aligned.
+__device__ __forceinline__ void hot_load(const PbDev &d, const float *__restrict__ contrib,
+                                         uint32_t w, float4 (&hv)[kHotPer4]) {
+  const uint32_t cnt = min(1u << d.win_shift, d.nhot - (w << d.win_shift));
+  const float4 *src = reinterpret_cast<const float4 *>(contrib + ((size_t)w << d.win_shift));
+#pragma unroll
+  for (int j = 0; j < kHotPer4; j++) {
+    const uint32_t i4 = threadIdx.x + j * kBlock;
+    hv[j] = 4 * i4 < cnt ? src[i4] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Stores the pieces of hot_load that hold sources of the window.
+__device__ __forceinline__ void hot_store(const PbDev &d, uint32_t w, const float4 (&hv)[kHotPer4],
+                                          float4 *s_hot4) {
+  const uint32_t cnt = min(1u << d.win_shift, d.nhot - (w << d.win_shift));
+#pragma unroll
+  for (int j = 0; j < kHotPer4; j++) {
+    const uint32_t i4 = threadIdx.x + j * kBlock;
+    if (4 * i4 < cnt) s_hot4[i4] = hv[j];
+  }
+}
+
+// Copies the contributions of hot window w into the LDS copy.  The loads are
+// issued before the first barrier, so that they overlap the wait for the
+// waves still reading the previous window.
 __device__ __forceinline__ void hot_window(const PbDev &d, const float *__restrict__ contrib,
-                                           uint32_t w, float *s_hot) {
-  const uint32_t first = w << d.win_shift;
-  const uint32_t cnt = min(1u << d.win_shift, d.nhot - first);
-  float hv[kHotPer];
-#pragma unroll
-  for (int j = 0; j < kHotPer; j++) {
-    const uint32_t i = threadIdx.x + j * kBlock;
-    hv[j] = i < cnt ? contrib[first + i] : 0.f;
-  }
+                                           uint32_t w, float4 *s_hot4) {
+  float4 hv[kHotPer4];
+  hot_load(d, contrib, w, hv);
   __syncthreads();  // every wave is done with the previous window
-#pragma unroll
-  for (int j = 0; j < kHotPer; j++) {
-    const uint32_t i = threadIdx.x + j * kBlock;
-    if (i < cnt) s_hot[i] = hv[j];
-  }
+  hot_store(d, w, hv, s_hot4);
   __syncthreads();
 }
 
@@ -576,25 +594,17 @@ template <bool HOT, bool SPMV, bool WEIGHTED>
 __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restrict__ y,
                                          const PrArgs &a, const float *__restrict__ contrib) {
   __shared__ unsigned long long s_acc[kBinRows + 64];  // + a dummy slot per lane
-  __shared__ float s_hot[HOT ? kHotMax : 1];
+  __shared__ float4 s_hot4[HOT ? kHotMax / 4 : 1];
+  float *s_hot = reinterpret_cast<float *>(s_hot4);
   __shared__ unsigned long long s_ps[kWaves];
   __shared__ double s_d[kWaves], s_l[kWaves];
   const uint32_t tid = threadIdx.x;
   const int lane = __lane_id();
   const uint32_t wave = tid >> 6;
-  if (HOT) {
-    const uint32_t cnt = min(1u << d.win_shift, d.nhot);
-    float hv[kHotPer];
-#pragma unroll
-    for (int j = 0; j < kHotPer; j++) {
-      const uint32_t i = tid + j * kBlock;
-      hv[j] = i < cnt ? contrib[i] : 0.f;
-    }
-#pragma unroll
-    for (int j = 0; j < kHotPer; j++) {
-      const uint32_t i = tid + j * kBlock;
-      if (i < cnt) s_hot[i] = hv[j];
-    }
+  if (HOT) {  // window 0; the first barrier of the item loop publishes it
+    float4 hv[kHotPer4];
+    hot_load(d, contrib, 0, hv);
+    hot_store(d, 0, hv, s_hot4);
   }
   uint32_t loaded = 0u, down = 0u;  // window in s_hot; walk direction of the next item
   // one item per workgroup unless the launch is persistent
@@ -636,7 +646,7 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
         PbBatch<GC> c;
         load_batch<true, WEIGHTED, GS>(d, hb, hg1, lane, b);
         if (cold) load_batch<false, WEIGHTED, GC>(d, cb, cg1, lane, c);
-        hot_window(d, contrib, w, s_hot);
+        hot_window(d, contrib, w, s_hot4);
         loaded = w;
         sum_batch<true, WEIGHTED, SPMV, GS>(b, hb, hg1, lane, a0, a1, S, part, s_acc, s_hot,
                                             psum);
@@ -1021,7 +1031,8 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
       min_win_share = 0.0;
     }
   }
-  uint32_t nhot = std::min<uint32_t>(hot_cap, n);
+  // a multiple of 4: phase 2 copies hot windows in 16-byte pieces
+  uint32_t nhot = std::min<uint32_t>(hot_cap, n) & ~3u;
   {
     uint64_t hot_entries = 0, win_entries = 0;
     uint32_t kept = 0;  // sources of the windows kept
@@ -1335,6 +1346,8 @@ static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int
   d.xcd_tiles = g->tune.pr_xcd ? 1u : 0u;
   d.w = spmv ? p->w : nullptr;
   const bool wt = d.w != nullptr;
+  if (p->nhot && (reinterpret_cast<uintptr_t>(x) & 15u))
+    return eg_fail(EG_EINVAL, "binned pass: contributions not 16-byte aligned");
   {
     EgProf prof(g, spmv ? "spmv_scatter" : "pr_scatter", s);
     const int sb = g->tune.pr_scatter;

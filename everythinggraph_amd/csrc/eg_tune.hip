@@ -37,8 +37,8 @@
 //                   timed: caps of 1, 4, 8 and 16 windows, each distinct plan
 //                   once);
 //  * pr_persist  -- binned phase 2 with hot sources: persistent workgroups,
-//                   one per CU (1), or one workgroup per work item (0); not
-//                   timed by eg_autotune;
+//                   one per CU (1), or one workgroup per work item (0); timed
+//                   on the plan of the kept pr_hot;
 //  * pr_xcd      -- binned scatter tile walk: 0 = XCD x takes the x-th
 //                   eighth of the chunk-ordered tiles (the default); 1 = XCD x
 //                   takes a short run of tiles from every eighth by turns
@@ -302,6 +302,16 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
         }
       }
       g->tune.pr_hot = best_cap;
+      // persistent phase-2 workgroups against one per work item, on that plan
+      EG_TRY(eg_pagerank(g, &o, nullptr, nullptr, nullptr));  // warm (binned plan)
+      double pms[2];
+      for (int pp = 0; pp <= 1; pp++) {
+        g->tune.pr_persist = pp;
+        EG_TRY(time_best(g, 3, [&] { return eg_pagerank(g, &o, nullptr, nullptr, nullptr); },
+                         &pms[pp]));
+        res.push_back({"pr_persist", pp, pms[pp] / o.iterations});
+      }
+      g->tune.pr_persist = pms[1] <= pms[0] ? 1 : 0;
     }
     for (int pb = 0; pb <= 1; pb++) {
       g->tune.pr_binned = pb;

@@ -471,8 +471,8 @@ def test_autotune_reports_and_keeps_fastest():
         params = {r[0] for r in rep}
         # the hot-set and persistent pull shapes exist only on the degree-ordered copy
         ordered = os.environ.get("EG_PULL_ORDER") != "none"
-        expect = {"pull_batch", "pr_binned", "pr_xcd", "pr_hot", "spmm_depth", "ms_batch",
-                  "ms_push_div"}
+        expect = {"pull_batch", "pr_binned", "pr_xcd", "pr_hot", "pr_persist", "spmm_depth",
+                  "ms_batch", "ms_push_div"}
         if ordered:
             expect |= {"pull_hot", "pull_persist"}
         assert params == expect
@@ -481,6 +481,7 @@ def test_autotune_reports_and_keeps_fastest():
         assert sorted(r[1] for r in rep if r[0] == "ms_push_div") == [5, 20, 40]
         # 4096 sources fit one hot window: every larger cap gives the same plan
         assert [r[1] for r in rep if r[0] == "pr_hot"] == [16384]
+        assert sorted(r[1] for r in rep if r[0] == "pr_persist") == [0, 1]
         if ordered:
             hot = sorted(r[1] for r in rep if r[0] == "pull_hot")
             assert hot == [0, n // 64, n // 16, n // 4]
