@@ -64,11 +64,16 @@ constexpr int kTabPer = (kTileBins + 1 + kBlock - 1) / kBlock;  // table loads p
 constexpr int kRowPer = (kBinRows + kBlock - 1) / kBlock;       // rows finished per thread
 constexpr int kRowBatch = 4;                                    // of which loaded together
 constexpr int kFinBlocks = 128;
-constexpr uint32_t kHotMax = 16384;    // hot sources of one window, kept in LDS by phase 2 (64 KB)
-constexpr uint32_t kHotWins = 16;      // hot windows at most
-constexpr int kHotPer4 = kHotMax / 4 / kBlock;                  // hot 16-byte loads per thread
-static_assert(kHotMax == 1u << 14, "a hot window's offsets take 14 bits");
-static_assert(kHotPer4 * 4 * kBlock == (int)kHotMax, "a hot window is copied in 16-byte pieces");
+// Hot sources of one window, kept in LDS by phase 2 (95 KB): what fits in
+// 160 KB beside the accumulators (64 KB + a dummy slot per lane) and the
+// workgroup's reduction slots.
+constexpr uint32_t kHotMax = kPbHotWindow;
+constexpr uint32_t kHotWins = kPbHotWindows;                       // hot windows at most
+constexpr int kHotPer4 = (kHotMax / 4 + kBlock - 1) / kBlock;      // hot 16-byte loads per thread
+static_assert(kHotMax % 4 == 0, "a hot window is copied in 16-byte pieces");
+static_assert(kHotMax < 65536, "a hot entry packs its source offset in 16 bits");
+static_assert((kBinRows + 64) * 8 + kHotMax * 4 + kWaves * 24 <= 160 * 1024,
+              "phase 2's LDS must fit one workgroup per CU");
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
   return (uint32_t)((count + per - 1) / per);
@@ -131,7 +136,7 @@ struct PbDev {
   uint32_t *fin_counter;
   uint32_t nbins, nitems, nlong;
   uint32_t nhot;           // sources below this id are hot
-  uint32_t win_shift;      // hot window w holds sources [w << win_shift, (w + 1) << win_shift)
+  uint32_t win;            // hot window w holds sources [w * win, (w + 1) * win)
   uint32_t nwin;           // hot windows
   const uint32_t *wseg;    // per item: the nwin + 1 starts of its windows' hot entries
   uint32_t persistent;     // phase 2: workgroups loop over items (else one each)
@@ -155,7 +160,7 @@ __device__ __forceinline__ void pb_entry_keys(uint32_t v, uint32_t k, const PbSr
                                               const uint32_t *__restrict__ bin_row0,
                                               const uint32_t *__restrict__ chunk_src0,
                                               uint32_t nbins, uint32_t nchunks, uint32_t nhot,
-                                              uint32_t win_shift, uint32_t nwin,
+                                              uint32_t win, uint32_t nwin,
                                               int bin_major, uint32_t *__restrict__ key,
                                               uint32_t *__restrict__ pay,
                                               uint32_t *__restrict__ pay2,
@@ -165,7 +170,7 @@ __device__ __forceinline__ void pb_entry_keys(uint32_t v, uint32_t k, const PbSr
   const bool hot = u < nhot;
   const uint32_t c = hot ? 0u : chunk_of[u];
   if (bin_major) {
-    key[k] = b * (nchunks + nwin) + (hot ? u >> win_shift : nwin + c);
+    key[k] = b * (nchunks + nwin) + (hot ? u / win : nwin + c);
     pay[k] = v - bin_row0[b];
     pay2[k] = u;
     if (pay3) pay3[k] = __float_as_uint(src.w[k]);
@@ -176,7 +181,7 @@ __device__ __forceinline__ void pb_entry_keys(uint32_t v, uint32_t k, const PbSr
 }
 
 // key / payloads of every entry.  Sources below nhot are hot: they belong to
-// no chunk but to the hot window source >> win_shift (nwin windows).
+// no chunk but to the hot window source / win (nwin windows).
 // bin_major: key = bin * (nchunks + nwin) + (hot ? window : nwin + chunk),
 //   payloads = row - first row of the bin, the source, and the weight bits
 //   (pay3);
@@ -188,14 +193,14 @@ __global__ void k_pb_keys(uint32_t n, uint64_t m, PbSrcDev src,
                           const uint32_t *__restrict__ chunk_of,
                           const uint32_t *__restrict__ bin_row0,
                           const uint32_t *__restrict__ chunk_src0, uint32_t nbins,
-                          uint32_t nchunks, uint32_t nhot, uint32_t win_shift, uint32_t nwin,
+                          uint32_t nchunks, uint32_t nhot, uint32_t win, uint32_t nwin,
                           int bin_major, uint32_t *__restrict__ key, uint32_t *__restrict__ pay,
                           uint32_t *__restrict__ pay2, uint32_t *__restrict__ pay3) {
   if (src.row) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += stride)
       pb_entry_keys(src.row[k], (uint32_t)k, src, bin_of, chunk_of, bin_row0, chunk_src0, nbins,
-                    nchunks, nhot, win_shift, nwin, bin_major, key, pay, pay2, pay3);
+                    nchunks, nhot, win, nwin, bin_major, key, pay, pay2, pay3);
     return;
   }
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -205,7 +210,7 @@ __global__ void k_pb_keys(uint32_t n, uint64_t m, PbSrcDev src,
     const uint32_t k0 = src.ptr[v], k1 = src.ptr[v + 1];
     for (uint32_t k = k0 + lane; k < k1; k += 64)
       pb_entry_keys(v, k, src, bin_of, chunk_of, bin_row0, chunk_src0, nbins, nchunks, nhot,
-                    win_shift, nwin, bin_major, key, pay, pay2, pay3);
+                    win, nwin, bin_major, key, pay, pay2, pay3);
   }
 }
 
@@ -226,11 +231,11 @@ __global__ void k_pb_absmax(uint64_t m, const float *__restrict__ w, uint32_t *_
 __global__ void k_pb_narrow(uint64_t m, const uint32_t *__restrict__ in,
                             uint16_t *__restrict__ out, const uint32_t *__restrict__ skey,
                             const uint32_t *__restrict__ src, uint32_t key_mod, uint32_t nwin,
-                            uint32_t win_mask, uint32_t *__restrict__ packed) {
+                            uint32_t win, uint32_t *__restrict__ packed) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   out[k] = (uint16_t)in[k];
-  if (packed && skey[k] % key_mod < nwin) packed[k] = (src[k] & win_mask) | (in[k] << 16);
+  if (packed && skey[k] % key_mod < nwin) packed[k] = (src[k] % win) | (in[k] << 16);
 }
 
 // ---------------------------------------------------------------------------
@@ -405,7 +410,7 @@ __device__ __forceinline__ void group_accumulate(unsigned long long *acc, const 
     for (int j = 0; j < 8; j++) {
       key[j] = w[j] >> 16;
       // masked slots may hold other words: keep the read inside the copy
-      v[j] = s_hot[w[j] & (kHotMax - 1)];
+      v[j] = s_hot[min(w[j] & 0xFFFFu, kHotMax - 1)];
     }
   } else {
     const uint32_t ks[8] = {w0.x & 0xFFFFu, w0.x >> 16, w0.y & 0xFFFFu, w0.y >> 16,
@@ -544,14 +549,14 @@ __device__ __forceinline__ void accumulate_range(const PbDev &d, uint32_t e0, ui
 }
 
 // This thread's 16-byte pieces of hot window w's contributions (sources
-// w << win_shift ..., cnt of them): piece j holds sources 4 i .. 4 i + 3 of the
+// w * win ..., cnt of them): piece j holds sources 4 i .. 4 i + 3 of the
 // window, i = threadIdx.x + j * kBlock, zeros past cnt.  The plan keeps nhot
 // (so cnt) a multiple of 4 and the launch checks that contrib is 16-byte
 // aligned.
 __device__ __forceinline__ void hot_load(const PbDev &d, const float *__restrict__ contrib,
                                          uint32_t w, float4 (&hv)[kHotPer4]) {
-  const uint32_t cnt = min(1u << d.win_shift, d.nhot - (w << d.win_shift));
-  const float4 *src = reinterpret_cast<const float4 *>(contrib + ((size_t)w << d.win_shift));
+  const uint32_t cnt = min(d.win, d.nhot - w * d.win);
+  const float4 *src = reinterpret_cast<const float4 *>(contrib + (size_t)w * d.win);
 #pragma unroll
   for (int j = 0; j < kHotPer4; j++) {
     const uint32_t i4 = threadIdx.x + j * kBlock;
@@ -562,7 +567,7 @@ __device__ __forceinline__ void hot_load(const PbDev &d, const float *__restrict
 // Stores the pieces of hot_load that hold sources of the window.
 __device__ __forceinline__ void hot_store(const PbDev &d, uint32_t w, const float4 (&hv)[kHotPer4],
                                           float4 *s_hot4) {
-  const uint32_t cnt = min(1u << d.win_shift, d.nhot - (w << d.win_shift));
+  const uint32_t cnt = min(d.win, d.nhot - w * d.win);
 #pragma unroll
   for (int j = 0; j < kHotPer4; j++) {
     const uint32_t i4 = threadIdx.x + j * kBlock;
@@ -819,7 +824,7 @@ struct PbPlan {
   uint32_t tile_len = 0;   // scatter tile length the plan was cut with
   int hot_req = 0;         // hot setting the plan was built for
   uint32_t nhot = 0;       // sources below this id are hot
-  uint32_t win_shift = 0;  // log2 of the sources per hot window
+  uint32_t win = 0;        // sources per hot window
   uint32_t nwin = 0;       // hot windows (ceil(nhot / window))
   uint32_t cus = 0;        // compute units (persistent accumulation grid)
   uint32_t max_rowlen = 0; // most entries of one row
@@ -876,7 +881,7 @@ int upload(eg_graph *g, T **dst, const std::vector<T> &src, const char *what) {
 // receives the start of every key's segment (nkeys + 1 entries) on the host.
 int pb_order(eg_graph *g, const PbSrcDev &src, const uint32_t *bin_of, const uint32_t *chunk_of,
              const uint32_t *bin_row0, const uint32_t *chunk_src0, uint32_t nbins,
-             uint32_t nchunks, uint32_t nhot, uint32_t win_shift, uint32_t nwin, int bin_major,
+             uint32_t nchunks, uint32_t nhot, uint32_t win, uint32_t nwin, int bin_major,
              uint16_t *out, uint32_t *packed, float *wout, std::vector<uint32_t> &seg) {
   const uint32_t n = g->n;
   const uint64_t m = g->m;
@@ -902,7 +907,7 @@ int pb_order(eg_graph *g, const PbSrcDev &src, const uint32_t *bin_of, const uin
         src.row ? std::min<uint64_t>(blocks_for(m, 256), 65536)
                 : std::min<uint64_t>(blocks_for((uint64_t)n * 64, 256), 65536);
     hipLaunchKernelGGL(k_pb_keys, dim3(grid), dim3(256), 0, s, n, m, src, bin_of, chunk_of,
-                       bin_row0, chunk_src0, nbins, nchunks, nhot, win_shift, nwin, bin_major,
+                       bin_row0, chunk_src0, nbins, nchunks, nhot, win, nwin, bin_major,
                        key, pay, pay2, wt ? pay3 : nullptr);
     if (hipGetLastError() != hipSuccess) {
       rc = eg_fail(EG_EDEVICE, "binned key launch");
@@ -920,7 +925,7 @@ int pb_order(eg_graph *g, const PbSrcDev &src, const uint32_t *bin_of, const uin
     if ((rc = eg_scratch(g, eg_radix_tmp_bytes(m, c.np), &tmp))) break;
     if ((rc = eg_radix_sort(s, m, eg_bits_for(nkeys), key, skey, c, tmp))) break;
     hipLaunchKernelGGL(k_pb_narrow, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, spay, out, skey,
-                       spay2, nchunks + nwin, nwin, (1u << win_shift) - 1u,
+                       spay2, nchunks + nwin, nwin, win,
                        bin_major && nhot ? packed : nullptr);
     if (hipGetLastError() != hipSuccess) {
       rc = eg_fail(EG_EDEVICE, "binned narrow launch");
@@ -1014,7 +1019,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   // several hot windows, long-row parts and tile cuts (tests).
   uint32_t rows_cap = kBinRows, src_cap = kChunkSrc, tile_len = want_tile, tile_bins = kTileBins;
   uint32_t hot_cap = want_hot > 0 ? (uint32_t)want_hot : 0u;
-  uint32_t win_shift = 14;  // kHotMax sources per window
+  uint32_t win = kHotMax;  // sources per hot window
   uint64_t target = std::max<uint64_t>(m / 2048, 4096);
   double min_hot_share = 0.15, min_win_share = 0.02;
   {
@@ -1026,7 +1031,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
       tile_bins = 16;
       target = 256;
       hot_cap = std::min<uint32_t>(hot_cap, 40);
-      win_shift = 4;
+      win = 16;
       min_hot_share = 0.0;
       min_win_share = 0.0;
     }
@@ -1038,8 +1043,8 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
     uint32_t kept = 0;  // sources of the windows kept
     for (uint32_t u = 0; u < nhot; u++) {
       win_entries += scnt[u];
-      if (u + 1 == nhot || ((u + 1) & ((1u << win_shift) - 1)) == 0) {  // a window ends
-        if (u >= (1u << win_shift) && (double)win_entries < min_win_share * (double)m) break;
+      if (u + 1 == nhot || (u + 1) % win == 0) {  // a window ends
+        if (u >= win && (double)win_entries < min_win_share * (double)m) break;
         hot_entries += win_entries;
         win_entries = 0;
         kept = u + 1;
@@ -1048,7 +1053,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
     nhot = kept;
     if ((double)hot_entries < min_hot_share * (double)m) nhot = 0;
   }
-  const uint32_t nwin = (nhot + (1u << win_shift) - 1) >> win_shift;
+  const uint32_t nwin = (nhot + win - 1) / win;
   if (nwin > kHotWins) return eg_fail(EG_EINVAL, "binned plan: %u hot windows", nwin);
   uint32_t max_rowlen = 0;
   std::vector<uint32_t> bin_row0, bin_of(n), chunk_src0, chunk_of(n, 0u);
@@ -1092,7 +1097,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   p->nbins = nbins;
   p->nchunks = nchunks;
   p->nhot = nhot;
-  p->win_shift = win_shift;
+  p->win = win;
   p->nwin = nwin;
   p->cus = (uint32_t)cus;
   p->max_rowlen = max_rowlen;
@@ -1131,10 +1136,10 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
         memcpy(&p->wmax, &wbits, 4);
       }
       if ((rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot,
-                         win_shift, nwin, 1, p->dloc, reinterpret_cast<uint32_t *>(p->val), p->w,
+                         win, nwin, 1, p->dloc, reinterpret_cast<uint32_t *>(p->val), p->w,
                          seg_bm)))
         break;
-      rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, win_shift,
+      rc = pb_order(g, sd, d_bin_of, d_chunk_of, d_row0, d_src0, nbins, nchunks, nhot, win,
                     nwin, 0, p->sloc, nullptr, nullptr, seg_cm);
     } while (0);
     (void)hipStreamSynchronize(s);
@@ -1340,7 +1345,7 @@ static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int
   d.nitems = p->nitems;
   d.nlong = p->nlong;
   d.nhot = p->nhot;
-  d.win_shift = p->win_shift;
+  d.win = p->win;
   d.nwin = p->nwin;
   d.wseg = p->wseg;
   d.xcd_tiles = g->tune.pr_xcd ? 1u : 0u;

@@ -98,6 +98,11 @@ struct EgOrdered {
   PullPlan plan;
 };
 
+// Binned PageRank: sources per hot window (phase 2 keeps one window's
+// contributions in LDS beside its accumulators) and windows at most.
+constexpr int kPbHotWindow = 24320;
+constexpr int kPbHotWindows = 16;
+
 // Kernel shapes of a graph, set by eg_autotune or eg_tune_set.  Every
 // variant computes bit-identical results; only the speed differs.
 struct EgTune {
@@ -117,9 +122,10 @@ struct EgTune {
                          // FP32 sums), -1 = binned when the graph has >= 2^22 entries
   int pr_tile = 0;       // binned PageRank: source offsets per scatter tile (0: 32768)
   int pr_scatter = 16;   // binned scatter: source offsets in flight per lane (8, 16 or 32)
-  int pr_hot = 262144;   // binned PageRank: sources below this id are read from an LDS copy
-                         // in phase 2 instead of being scattered (0: none, at most 262144,
-                         // in windows of 16384 sources; the plan keeps the windows that pay)
+  int pr_hot = kPbHotWindow * kPbHotWindows;  // binned PageRank: sources below this id are
+                         // read from an LDS copy in phase 2 instead of being scattered (0:
+                         // none; in windows of kPbHotWindow sources; the plan keeps the
+                         // windows that pay)
   int wcc_order = 1;     // undirected / symmetric graphs: WCC pulls on the ordered ids (0: edges)
   int bfs_order = 2;     // symmetric graphs: BFS on CSR rows ordered hubs first (1), on the
                          // degree-ordered ids with rows hubs first (2), or as built (0)

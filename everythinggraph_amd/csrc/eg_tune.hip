@@ -31,8 +31,8 @@
 //  * pr_scatter  -- binned PageRank scatter: source offsets loaded per lane
 //                   before any value is written, 8, 16 or 32;
 //  * pr_hot      -- binned PageRank: sources below this id are read from an
-//                   LDS copy in phase 2 instead of being scattered (0..262144,
-//                   in windows of 16384 sources loaded one at a time; the plan
+//                   LDS copy in phase 2 instead of being scattered (0..389120,
+//                   in windows of 24320 sources loaded one at a time; the plan
 //                   keeps the leading windows that hold 2% of the entries;
 //                   timed: caps of 1, 4, 8 and 16 windows, each distinct plan
 //                   once);
@@ -138,7 +138,8 @@ extern "C" int eg_tune_set(eg_graph *g, const char *param, int32_t value) {
     if (value != 8 && value != 16 && value != 32)
       return eg_fail(EG_EINVAL, "pr_scatter takes 8, 16 or 32");
   } else if (!strcmp(param, "pr_hot")) {
-    if (value < 0 || value > 262144) return eg_fail(EG_EINVAL, "pr_hot takes 0..262144");
+    if (value < 0 || value > kPbHotWindow * kPbHotWindows)
+      return eg_fail(EG_EINVAL, "pr_hot takes 0..%d", kPbHotWindow * kPbHotWindows);
   } else if (!strcmp(param, "bfs_order")) {
     if (value < 0 || value > 2) return eg_fail(EG_EINVAL, "bfs_order takes 0, 1 or 2");
   } else if (!strcmp(param, "pr_persist")) {
@@ -281,7 +282,8 @@ extern "C" int eg_autotune(eg_graph *g, uint32_t kernels, eg_tune_entry *out, in
     // hot window caps (each plan keeps only the windows that hold 2% of the
     // entries, so caps past the graph's skew give the same plan)
     {
-      const int caps[4] = {16384, 65536, 131072, 262144};
+      const int caps[4] = {kPbHotWindow, 4 * kPbHotWindow, 8 * kPbHotWindow,
+                           kPbHotWindows * kPbHotWindow};
       double best = 1e300;
       int best_cap = g->tune.pr_hot;
       uint32_t prev_hot = 0xFFFFFFFFu;

@@ -290,10 +290,10 @@ int eg_ppr(eg_graph *g, const uint32_t *seeds, uint32_t nseeds, double alpha, do
  * forms round differently, each is deterministic); "pr_tile" (source offsets
  * per binned scatter workgroup, 0 = 32768); "pr_scatter" (binned scatter
  * source offsets in flight per lane, 8, 16 or 32); "pr_hot" (binned PageRank:
- * sources below this id, at most 262144, are read from an LDS copy in phase 2
- * instead of being scattered, one window of 16384 sources at a time; the plan
+ * sources below this id, at most 389120, are read from an LDS copy in phase 2
+ * instead of being scattered, one window of 24320 sources at a time; the plan
  * keeps the leading windows that hold at least 2% of the entries each, and
- * none when they hold under 15% together; 0 = none, default 262144);
+ * none when they hold under 15% together; 0 = none, default 389120);
  * "pr_persist" (1: phase-2 workgroups loop over work items, one per CU, the
  * default; 0: one workgroup per work item); "pr_xcd" (binned
  * scatter: 0 = each of the eight XCDs walks a contiguous eighth of the

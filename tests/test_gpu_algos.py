@@ -480,7 +480,7 @@ def test_autotune_reports_and_keeps_fastest():
         assert sorted(r[1] for r in rep if r[0] == "pr_xcd") == [0, 1]
         assert sorted(r[1] for r in rep if r[0] == "ms_push_div") == [5, 20, 40]
         # 4096 sources fit one hot window: every larger cap gives the same plan
-        assert [r[1] for r in rep if r[0] == "pr_hot"] == [16384]
+        assert [r[1] for r in rep if r[0] == "pr_hot"] == [24320]
         assert sorted(r[1] for r in rep if r[0] == "pr_persist") == [0, 1]
         if ordered:
             hot = sorted(r[1] for r in rep if r[0] == "pull_hot")

@@ -64,10 +64,10 @@ def test_binned_pagerank_matches_oracle(scale, undirected, dangling):
     # from the FP64 oracle than the compensated ones
     assert np.abs(r.astype(np.float64) - rp).sum() <= PR_L1_TOL
     assert err.sum() <= np.abs(rp.astype(np.float64) - ref).sum() * 1.5 + 1e-9
-    # hot sources (read from LDS in phase 2) come in windows of 16384; the
+    # hot sources (read from LDS in phase 2) come in windows of 24320; the
     # rest are scattered in chunks
     assert geo["bins"] >= 1 and geo["items"] >= geo["bins"]
-    assert geo["hot"] == n or geo["hot"] % 16384 == 0
+    assert geo["hot"] == n or geo["hot"] % 24320 == 0
     assert (geo["chunks"] >= 1) == (geo["hot"] < n)
 
 
@@ -254,8 +254,8 @@ def test_binned_spmv_every_layout(weighted):
 
 @pytest.mark.parametrize("weighted", [False, True])
 def test_binned_hot_windows(weighted):
-    """Hot sources in several LDS windows of 16384 (R-MAT 16: 65536 sources):
-    one window, three (the last one partial) and four, each with one phase-2
+    """Hot sources in several LDS windows of 24320 (R-MAT 16: 65536 sources):
+    one window (partial), two (the last one partial) and three, each with one phase-2
     workgroup per work item and with persistent workgroups (which walk the
     windows in alternating directions), give the PageRank ranks and the SpMV
     products of the single-window plan bit for bit, and the ranks stay within
@@ -349,11 +349,11 @@ def test_binned_spmv_nonfinite_weights_fall_back(mode, case):
 
 
 def test_binned_uniform_graph_has_no_hot_set():
-    """Erdos-Renyi ids carry no degree skew: every window of 16384 sources
-    takes 1/64 of the entries (under the 2% a window must hold to be kept
+    """Erdos-Renyi ids carry no degree skew: every window of 24320 sources
+    takes 1.2% of the entries (under the 2% a window must hold to be kept
     beyond the first), and the first alone is under 15%, so the plan scatters
     every source."""
-    n = 1 << 20
+    n = 1 << 21
     s0, d0 = O.gen_er(n, 1 << 22, seed=64)
     ptr, col, _ = O.build_csr(n, s0, d0)
     cp, row, _ = O.build_csc(n, ptr, col)
