@@ -72,7 +72,7 @@ constexpr uint32_t kHotWins = kPbHotWindows;                       // hot window
 constexpr int kHotPer4 = (kHotMax / 4 + kBlock - 1) / kBlock;      // hot 16-byte loads per thread
 static_assert(kHotMax % 4 == 0, "a hot window is copied in 16-byte pieces");
 static_assert(kHotMax < 65536, "a hot entry packs its source offset in 16 bits");
-static_assert((kBinRows + 64) * 8 + kHotMax * 4 + kWaves * 24 <= 160 * 1024,
+static_assert((kBinRows + 64) * 8 + kHotMax * 4 + kWaves * 24 + 16 <= 160 * 1024,
               "phase 2's LDS must fit one workgroup per CU");
 
 inline uint32_t blocks_for(uint64_t count, uint32_t per) {
@@ -140,6 +140,8 @@ struct PbDev {
   uint32_t nwin;           // hot windows
   const uint32_t *wseg;    // per item: the nwin + 1 starts of its windows' hot entries
   uint32_t persistent;     // phase 2: workgroups loop over items (else one each)
+  uint32_t *work;          // persistent phase 2: items handed out, workgroups done (reset
+                           // to 0 by the last workgroup)
   uint32_t xcd_tiles;      // phase 1: interleaved walk of the tile list (else launch order)
   const float *w;          // bin-major weights (SpMV of a weighted graph), or null
 };
@@ -603,6 +605,7 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
   float *s_hot = reinterpret_cast<float *>(s_hot4);
   __shared__ unsigned long long s_ps[kWaves];
   __shared__ double s_d[kWaves], s_l[kWaves];
+  __shared__ uint32_t s_next;
   const uint32_t tid = threadIdx.x;
   const int lane = __lane_id();
   const uint32_t wave = tid >> 6;
@@ -612,9 +615,19 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
     hot_store(d, 0, hv, s_hot4);
   }
   uint32_t loaded = 0u, down = 0u;  // window in s_hot; walk direction of the next item
-  // one item per workgroup unless the launch is persistent
-  const uint32_t item_end = d.persistent ? d.nitems : blockIdx.x + 1;
-  for (uint32_t item = blockIdx.x; item < item_end; item += gridDim.x) {
+  // One item per workgroup unless the launch is persistent: then a workgroup
+  // takes item blockIdx.x first and the next unclaimed item after each one
+  // (items run largest first), so that the items balance over the CUs.
+  uint32_t item = blockIdx.x;
+  for (bool first = true;; first = false) {
+    if (!first) {
+      if (!d.persistent) break;
+      // every thread read s_next before the barriers of the item just done
+      if (tid == 0) s_next = gridDim.x + atomicAdd(d.work, 1u);
+      __syncthreads();
+      item = s_next;
+    }
+    if (item >= d.nitems) break;  // the same for the whole workgroup
     const PbItem it = d.items[item];
     const bool part = it.nrows == 0;
     if (!part)
@@ -727,6 +740,14 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
       d.item_part[item] = make_double2(td, tl);
     }
     __syncthreads();  // s_acc / s_d reuse by the next item
+  }
+  // the last workgroup out resets the item counters for the next launch
+  if (d.persistent && tid == 0) {
+    __threadfence();
+    if (atomicAdd(d.work + 1, 1u) == gridDim.x - 1) {
+      atomicExch(d.work, 0u);
+      atomicExch(d.work + 1, 0u);
+    }
   }
 }
 
@@ -842,6 +863,7 @@ struct PbPlan {
   unsigned long long *part_sum = nullptr;
   double2 *item_part = nullptr, *fin_part = nullptr;
   uint32_t *fin_counter = nullptr;
+  uint32_t *work = nullptr;  // persistent phase-2 item counters
 };
 
 void eg_pb_free(eg_graph *g) {
@@ -861,6 +883,7 @@ void eg_pb_free(eg_graph *g) {
   eg_dfree_t(&p->item_part);
   eg_dfree_t(&p->fin_part);
   eg_dfree_t(&p->fin_counter);
+  eg_dfree_t(&p->work);
   delete p;
   g->pb = nullptr;
 }
@@ -1273,9 +1296,11 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
     if ((rc = eg_dalloc_t(&p->part_sum, nparts ? nparts : 1, "binned part sums")) ||
         (rc = eg_dalloc_t(&p->item_part, p->nitems, "binned item partials")) ||
         (rc = eg_dalloc_t(&p->fin_part, kFinBlocks, "binned finish partials")) ||
-        (rc = eg_dalloc_t(&p->fin_counter, 1, "binned finish counter")))
+        (rc = eg_dalloc_t(&p->fin_counter, 1, "binned finish counter")) ||
+        (rc = eg_dalloc_t(&p->work, 2, "binned item counters")))
       break;
     if (hipMemsetAsync(p->fin_counter, 0, 4, s) != hipSuccess ||
+        hipMemsetAsync(p->work, 0, 8, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       rc = eg_fail(EG_EDEVICE, "binned plan upload");
   } while (0);
@@ -1341,6 +1366,7 @@ static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int
   d.item_part = p->item_part;
   d.fin_part = p->fin_part;
   d.fin_counter = p->fin_counter;
+  d.work = p->work;
   d.nbins = p->nbins;
   d.nitems = p->nitems;
   d.nlong = p->nlong;
