@@ -57,6 +57,7 @@ constexpr uint32_t kBinRows = 8192;    // rows per bin (u64 accumulators in LDS:
 constexpr uint32_t kChunkSrc = 12288;  // sources per chunk (FP32 window in LDS: 48 KB)
 constexpr uint32_t kTileBins = 2048;   // bins one scatter tile may span (table in LDS: 16 KB)
 constexpr uint32_t kTileLen = 32768;   // positions per scatter tile
+constexpr uint32_t kItemBins = 4;      // entry targets of bins per phase-2 work item
 constexpr int kBlock = 1024;           // two workgroups per CU (64 KB LDS each)
 constexpr int kWaves = kBlock / 64;
 constexpr int kWinPer = (kChunkSrc + kBlock - 1) / kBlock;     // window loads per thread
@@ -113,12 +114,14 @@ struct PbTile {
   uint32_t pad[3];
 };
 
-// Phase-2 work item: bin rows [row0, row0 + nrows) with bin-major entries
-// [e0, e1), of which [e0, eh) are hot; nrows == 0 marks a part of a long row
-// whose sum goes to part_sum[row0].  The hot entries of window w are
-// [wseg[ws + w], wseg[ws + w + 1]) (wseg[ws] = e0, wseg[ws + nwin] = eh).
+// Phase-2 work item: nb consecutive bins, rows [row0, row0 + nrows) with
+// bin-major entries [e0, e1); nrows == 0 marks a part of a long row whose sum
+// goes to part_sum[row0].  With s = nwin + 1, bin j of the item has its hot
+// entries of window w at [wseg[ws + j s + w], wseg[ws + j s + w + 1]) and its
+// scattered entries at [wseg[ws + j s + nwin], wseg[ws + (j + 1) s]) (the
+// last bin's end is wseg[ws + nb s] = e1).
 struct PbItem {
-  uint32_t e0, eh, e1, row0, nrows, ws, pad[2];
+  uint32_t e0, e1, row0, nrows, ws, nb, pad[2];
 };
 
 struct PbDev {
@@ -636,52 +639,53 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
     unsigned long long psum = 0ull;
     // the hot form runs 16 waves per CU: more loads in flight per lane
     constexpr int G = HOT && !WEIGHTED ? 4 : 2;
-    // Cold entries not yet summed start at cn; with several hot windows,
-    // rounds of them fill the latency of the window switches.
-    const uint32_t c0 = HOT ? it.eh : it.e0;
-    uint32_t cn = c0;
+    const uint32_t nb = it.nb, stride = d.nwin + 1;
+    // start of bin j's entries of hot window w (w = nwin: of its scattered
+    // entries), and the end of bin j
+    auto wstart = [&](uint32_t j, uint32_t w) { return d.wseg[it.ws + j * stride + w]; };
+    auto bin_end = [&](uint32_t j) { return d.wseg[it.ws + (j + 1) * stride]; };
     if (HOT && d.nwin == 1) {
-      accumulate_range<true, WEIGHTED, SPMV, G>(d, it.e0, it.eh, S, part, s_acc, s_hot, psum);
+      for (uint32_t j = 0; j < nb; j++)
+        accumulate_range<true, WEIGHTED, SPMV, G>(d, wstart(j, 0), wstart(j, 1), S, part, s_acc,
+                                                  s_hot, psum);
     } else if (HOT) {
-      // groups per lane of the hot and cold rounds loaded ahead of a switch
-      constexpr int GS = 2, GC = 1;
-      constexpr uint32_t kRound = kWaves * 64 * GS, kColdRound = kWaves * 64 * GC;
+      // groups per lane of a window's first round, loaded ahead of its switch
+      constexpr int GS = 2;
+      constexpr uint32_t kRound = kWaves * 64 * GS;
       for (uint32_t k = 0; k < d.nwin; k++) {
         const uint32_t w = down ? d.nwin - 1 - k : k;
-        const uint32_t a0 = d.wseg[it.ws + w], a1 = d.wseg[it.ws + w + 1];
-        if (a0 == a1) continue;  // the same for the whole workgroup
-        if (w == loaded) {
-          accumulate_range<true, WEIGHTED, SPMV, G>(d, a0, a1, S, part, s_acc, s_hot, psum);
-          continue;
+        // the first bin with entries in w (the same for the whole workgroup)
+        uint32_t jf = 0;
+        while (jf < nb && wstart(jf, w) == wstart(jf, w + 1)) jf++;
+        if (jf == nb) continue;
+        uint32_t j = jf;
+        if (w != loaded) {
+          // The first round of the window's first range is loaded before the
+          // switch, so that its latency overlaps the barriers.
+          const uint32_t a0 = wstart(jf, w), a1 = wstart(jf, w + 1);
+          const uint32_t hb = first_block<GS>(a0), hg1 = (a1 + 7) >> 3;
+          PbBatch<GS> bt;
+          load_batch<true, WEIGHTED, GS>(d, hb, hg1, lane, bt);
+          hot_window(d, contrib, w, s_hot4);
+          loaded = w;
+          sum_batch<true, WEIGHTED, SPMV, GS>(bt, hb, hg1, lane, a0, a1, S, part, s_acc, s_hot,
+                                              psum);
+          // the range's entries past its first round (group-aligned start)
+          const uint64_t rest = ((uint64_t)(a0 >> 3) + kRound) << 3;
+          if (rest < a1)
+            accumulate_range<true, WEIGHTED, SPMV, G>(d, (uint32_t)rest, a1, S, part, s_acc, s_hot,
+                                                      psum);
+          j++;
         }
-        // The first round of the window's entries and a round of cold
-        // entries are loaded before the switch, so that their latency
-        // overlaps the barriers.
-        const uint32_t hb = first_block<GS>(a0), hg1 = (a1 + 7) >> 3;
-        const uint32_t cb = first_block<GC>(cn), cg1 = (it.e1 + 7) >> 3;
-        const bool cold = cn < it.e1;  // the same for the whole workgroup
-        PbBatch<GS> b;
-        PbBatch<GC> c;
-        load_batch<true, WEIGHTED, GS>(d, hb, hg1, lane, b);
-        if (cold) load_batch<false, WEIGHTED, GC>(d, cb, cg1, lane, c);
-        hot_window(d, contrib, w, s_hot4);
-        loaded = w;
-        sum_batch<true, WEIGHTED, SPMV, GS>(b, hb, hg1, lane, a0, a1, S, part, s_acc, s_hot,
-                                            psum);
-        if (cold) {
-          sum_batch<false, WEIGHTED, SPMV, GC>(c, cb, cg1, lane, cn, it.e1, S, part, s_acc,
-                                               s_hot, psum);
-          cn = (uint32_t)std::min<uint64_t>(((uint64_t)(cn >> 3) + kColdRound) << 3, it.e1);
-        }
-        // the window's entries past its first round (group-aligned start)
-        const uint64_t rest = ((uint64_t)(a0 >> 3) + kRound) << 3;
-        if (rest < a1)
-          accumulate_range<true, WEIGHTED, SPMV, G>(d, (uint32_t)rest, a1, S, part, s_acc, s_hot,
-                                                    psum);
+        for (; j < nb; j++)
+          accumulate_range<true, WEIGHTED, SPMV, G>(d, wstart(j, w), wstart(j, w + 1), S, part,
+                                                    s_acc, s_hot, psum);
       }
       down ^= 1u;
     }
-    accumulate_range<false, WEIGHTED, SPMV, G>(d, cn, it.e1, S, part, s_acc, s_hot, psum);
+    for (uint32_t j = 0; j < nb; j++)
+      accumulate_range<false, WEIGHTED, SPMV, G>(d, wstart(j, HOT ? d.nwin : 0u), bin_end(j), S,
+                                                 part, s_acc, s_hot, psum);
     if (part) {
 #pragma unroll
       for (int o = 32; o >= 1; o >>= 1) psum += __shfl_xor(psum, o);
@@ -1112,6 +1116,33 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   }
   const uint32_t nbins = (uint32_t)bin_row0.size() - 1;
   const uint32_t nchunks = (uint32_t)chunk_src0.size() - 1;
+  // Phase-2 work items: runs of consecutive bins with at most rows_cap rows
+  // and kItemBins targets of entries together (each switch of hot window then
+  // serves them all, while phase 1 keeps the bins' shorter write segments); a
+  // bin holding one long row stays alone and is cut into parts.  Row offsets
+  // are stored relative to the first row of the bin's item.
+  std::vector<uint32_t> grp_bin0, row_base(nbins);
+  {
+    const uint64_t item_ents = kItemBins * target;
+    uint32_t rows = 0;
+    uint64_t ents = 0;
+    bool alone = false;  // the open item is a long row
+    for (uint32_t b = 0; b < nbins; b++) {
+      const uint32_t nr = bin_row0[b + 1] - bin_row0[b];
+      const uint64_t eb = (uint64_t)hptr[bin_row0[b + 1]] - hptr[bin_row0[b]];
+      const bool long_row = nr == 1 && eb > target;
+      if (b == 0 || alone || long_row || rows + nr > rows_cap || ents + eb > item_ents) {
+        grp_bin0.push_back(b);
+        rows = 0;
+        ents = 0;
+      }
+      alone = long_row;
+      row_base[b] = bin_row0[grp_bin0.back()];
+      rows += nr;
+      ents += eb;
+    }
+    grp_bin0.push_back(nbins);
+  }
   if ((uint64_t)nbins * (nchunks + nwin) >= (1ull << 31))
     return eg_fail(EG_ENOTSUP, "binned plan: %u bins x %u chunks", nbins, nchunks);
 
@@ -1132,7 +1163,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
     do {
       if ((rc = upload(g, &d_bin_of, bin_of, "binned bin_of")) ||
           (rc = upload(g, &d_chunk_of, chunk_of, "binned chunk_of")) ||
-          (rc = upload(g, &d_row0, bin_row0, "binned bin rows")) ||
+          (rc = upload(g, &d_row0, row_base, "binned row bases")) ||
           (rc = upload(g, &d_src0, chunk_src0, "binned chunk sources")))
         break;
       if ((rc = eg_dalloc_t(&p->dloc, m + 16, "binned row offsets")) ||
@@ -1217,21 +1248,19 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
     }
   }
   std::vector<PbItem> items;
-  std::vector<uint32_t> wseg;  // nwin + 1 window starts per item (at least one slot)
+  std::vector<uint32_t> wseg;  // per item: nb (nwin + 1) window starts and the end
   std::vector<uint4> long_rows;
   uint32_t nparts = 0;
-  // the window starts of bin b clamped to [a0, a1), at the end of wseg
+  // bin b's window starts clamped to [a0, a1), at the end of wseg
   auto push_windows = [&](uint32_t b, uint32_t a0, uint32_t a1) {
-    const uint32_t ws = (uint32_t)wseg.size();
     for (uint32_t w = 0; w <= nwin; w++)
       wseg.push_back(std::min(std::max(seg_bm[(size_t)b * kb + w], a0), a1));
-    return ws;
   };
-  for (uint32_t b = 0; b < nbins; b++) {
-    const uint32_t e0 = seg_bm[(size_t)b * kb], eh = seg_bm[(size_t)b * kb + nwin];
-    const uint32_t e1 = seg_bm[(size_t)(b + 1) * kb];
-    const uint32_t r0 = bin_row0[b], nr = bin_row0[b + 1] - r0;
-    if (nr == 1 && e1 - e0 > target) {
+  for (size_t gi = 0; gi + 1 < grp_bin0.size(); gi++) {
+    const uint32_t b0 = grp_bin0[gi], b1 = grp_bin0[gi + 1];
+    const uint32_t e0 = seg_bm[(size_t)b0 * kb], e1 = seg_bm[(size_t)b1 * kb];
+    const uint32_t r0 = bin_row0[b0], nr = bin_row0[b1] - r0;
+    if (b1 == b0 + 1 && nr == 1 && e1 - e0 > target) {
       const uint32_t np = (uint32_t)((e1 - e0 + target - 1) / target);
       long_rows.push_back(make_uint4(r0, nparts, np, 0u));
       for (uint32_t q = 0; q < np; q++) {
@@ -1239,22 +1268,26 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
         const uint32_t a1 = std::min<uint64_t>(e1, (uint64_t)a0 + target);
         PbItem it = {};
         it.e0 = a0;
-        it.eh = std::min(std::max(eh, a0), a1);
         it.e1 = a1;
         it.row0 = nparts + q;
         it.nrows = 0;
-        it.ws = push_windows(b, a0, a1);
+        it.nb = 1;
+        it.ws = (uint32_t)wseg.size();
+        push_windows(b0, a0, a1);
+        wseg.push_back(a1);
         items.push_back(it);
       }
       nparts += np;
     } else {
       PbItem it = {};
       it.e0 = e0;
-      it.eh = eh;
       it.e1 = e1;
       it.row0 = r0;
       it.nrows = nr;
-      it.ws = push_windows(b, e0, e1);
+      it.nb = b1 - b0;
+      it.ws = (uint32_t)wseg.size();
+      for (uint32_t b = b0; b < b1; b++) push_windows(b, e0, e1);
+      wseg.push_back(e1);
       items.push_back(it);
     }
   }
@@ -1314,7 +1347,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   // tiles, items, long rows and partials
   g->prep_plan_bytes = 8ull * (m + 16) + (src.w ? 4ull * (m + 16) : 0ull) +
                        8ull * p->nchunks * ((uint64_t)p->nbins + 1) + sizeof(PbTile) * p->ntiles +
-                       (sizeof(PbItem) + 16 + 4ull * (p->nwin + 1)) * p->nitems +
+                       (sizeof(PbItem) + 16) * p->nitems + 4ull * wseg.size() +
                        16ull * p->nlong + 8ull * p->nparts;
   return EG_OK;
 }

@@ -66,7 +66,7 @@ def test_binned_pagerank_matches_oracle(scale, undirected, dangling):
     assert err.sum() <= np.abs(rp.astype(np.float64) - ref).sum() * 1.5 + 1e-9
     # hot sources (read from LDS in phase 2) come in windows of 24320; the
     # rest are scattered in chunks
-    assert geo["bins"] >= 1 and geo["items"] >= geo["bins"]
+    assert geo["bins"] >= 1 and geo["items"] >= 1  # an item groups bins or is a long row's part
     assert geo["hot"] == n or geo["hot"] % 24320 == 0
     assert (geo["chunks"] >= 1) == (geo["hot"] < n)
 
@@ -190,7 +190,7 @@ def test_binned_independent_of_cut_and_order(tmp_path):
     for und in (0, 1):
         small = res[("small", "degree")][f"geo{und}"]
         bins, chunks, tiles, items, long_rows = (int(v) for v in small)
-        assert long_rows > 0 and items > bins  # hub rows summed in parts
+        assert long_rows > 0 and items != bins  # hub rows in parts, short bins grouped
         assert tiles > chunks                  # chunk regions cut into several tiles
         assert bins > 16                       # tiles also cut by the bins they span
 
