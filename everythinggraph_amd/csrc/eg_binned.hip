@@ -132,6 +132,7 @@ struct PbDev {
   const uint16_t *dloc;    // bin-major row offsets
   float *val;              // bin-major values
   const PbItem *items;
+  uint32_t nrun;           // phase 2 sums items [0, nrun) (the others keep their statistics)
   const uint4 *long_rows;  // (row, first part, parts, 0)
   unsigned long long *part_sum;
   double2 *item_part;
@@ -630,7 +631,7 @@ __device__ __forceinline__ void pb_accum(const PbDev &d, int S, float *__restric
       __syncthreads();
       item = s_next;
     }
-    if (item >= d.nitems) break;  // the same for the whole workgroup
+    if (item >= d.nrun) break;  // the same for the whole workgroup
     const PbItem it = d.items[item];
     const bool part = it.nrows == 0;
     if (!part)
@@ -846,6 +847,7 @@ __global__ __launch_bounds__(256) void k_pb_finish(PbDev d, int S, float *__rest
 struct PbPlan {
   const void *for_key = nullptr;  // entry source the plan was built for
   uint32_t nbins = 0, nchunks = 0, ntiles = 0, nitems = 0, nlong = 0, nparts = 0;
+  uint32_t nlive = 0;      // items with entries or parts (the rest are last in the list)
   uint32_t tile_len = 0;   // scatter tile length the plan was cut with
   int hot_req = 0;         // hot setting the plan was built for
   uint32_t nhot = 0;       // sources below this id are hot
@@ -1293,9 +1295,15 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   }
   // Accumulation items, largest first: workgroups are dispatched in launch
   // order, so the longest items start first and the short ones fill the tail.
+  // Items of rows without entries come last, so that phase 2 can leave them
+  // out (eg_pb_iterate's skip_static).
   std::stable_sort(items.begin(), items.end(), [](const PbItem &x, const PbItem &y) {
+    const bool xs = x.e0 == x.e1, ys = y.e0 == y.e1;
+    if (xs != ys) return ys;
     return x.e1 - x.e0 + 4 * x.nrows > y.e1 - y.e0 + 4 * y.nrows;
   });
+  uint32_t nlive = 0;
+  while (nlive < items.size() && items[nlive].e0 != items[nlive].e1) nlive++;
   // Workgroups are dealt to the 8 XCDs round-robin by launch index.  Launch
   // index j runs tile (j % 8) * ceil(T / 8) + j / 8, so every XCD walks a
   // contiguous eighth of the chunk-ordered tile list and the tiles of one
@@ -1315,6 +1323,7 @@ int eg_pb_build(eg_graph *g, const PbSource &src) {
   p->hot_req = want_hot;
   p->ntiles = (uint32_t)tiles.size();
   p->nitems = (uint32_t)items.size();
+  p->nlive = nlive;
   p->nlong = (uint32_t)long_rows.size();
   p->nparts = nparts;
   int rc = EG_OK;
@@ -1382,7 +1391,7 @@ bool eg_pb_spmv_exact(const eg_graph *g, double xmax) {
 // and the statistics of eg_pull.hip.  SpMV: y[v] = sum of w * x[u] (w = 1
 // without weights); a is not used.
 static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int S,
-                     hipStream_t stream, bool spmv) {
+                     hipStream_t stream, bool spmv, bool skip_static) {
   const PbPlan *p = g->pb;
   if (!p) return eg_fail(EG_ESTATE, "binned plan not built");
   hipStream_t s = stream ? stream : g->stream;
@@ -1402,6 +1411,7 @@ static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int
   d.work = p->work;
   d.nbins = p->nbins;
   d.nitems = p->nitems;
+  d.nrun = skip_static ? p->nlive : p->nitems;
   d.nlong = p->nlong;
   d.nhot = p->nhot;
   d.win = p->win;
@@ -1428,8 +1438,10 @@ static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int
   {
     EgProf prof(g, spmv ? "spmv_accum" : "pr_accum", s);
     d.persistent = p->nhot && g->tune.pr_persist ? 1u : 0u;
-    const dim3 grid(d.persistent ? std::min(p->nitems, p->cus) : p->nitems);
-    if (p->nhot) {
+    const dim3 grid(d.persistent ? std::min(d.nrun, p->cus) : d.nrun);
+    if (d.nrun == 0) {
+      // every item keeps its statistics
+    } else if (p->nhot) {
       if (!spmv)
         hipLaunchKernelGGL((k_pb_accum_hot<false, false>), grid, dim3(kBlock), 0, s, d, S, y, a, x);
       else if (wt)
@@ -1458,8 +1470,8 @@ static int pb_launch(eg_graph *g, const float *x, float *y, const PrArgs &a, int
 }
 
 int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, int S,
-                  hipStream_t stream) {
-  return pb_launch(g, contrib, y, a, S, stream, false);
+                  hipStream_t stream, bool skip_static) {
+  return pb_launch(g, contrib, y, a, S, stream, false, skip_static && !a.base_vec);
 }
 
 // SpMV pass y = A^T x over the built plan, x and y in device memory; xmax =
@@ -1471,7 +1483,7 @@ int eg_pb_spmv(eg_graph *g, const float *x, float *y, double xmax, hipStream_t s
     return eg_fail(EG_EINVAL, "binned SpMV needs finite weights and products");
   const double bound = (double)p->wmax * (double)p->max_rowlen * xmax * 1.001;
   PrArgs a = {};
-  return pb_launch(g, x, y, a, eg_pb_scale(bound), stream, true);
+  return pb_launch(g, x, y, a, eg_pb_scale(bound), stream, true, false);
 }
 
 extern "C" int eg_pr_binned_geometry(eg_graph *g, uint32_t *out) {

@@ -486,7 +486,7 @@ static int pagerank_binned_cells(eg_graph *g, int iters, double damping, double 
     a.contrib_new = contrib_next;
     a.dmass_in = g->pr_stats + 2 * it;
     a.stats_out = g->pr_stats + 2 * (it + 1);
-    EG_TRY(eg_pb_iterate(g, contrib, rank_next, a, S, nullptr));
+    EG_TRY(eg_pb_iterate(g, contrib, rank_next, a, S, nullptr, !uniform && it >= 2));
     std::swap(rank, rank_next);
     std::swap(contrib, contrib_next);
   }

@@ -50,5 +50,10 @@ __device__ __forceinline__ void finish_row(uint32_t r, float sum, float *__restr
 // One binned PageRank iteration (eg_binned.hip) from contrib into y and
 // a.contrib_new, with the statistics of the pull kernel; S is the fixed-point
 // scale of eg_pb_scale.  The plan must have been built for the orientation.
+// skip_static: leave out the work items of rows without entries.  Valid from
+// the third iteration of a run with a constant base term (dangling mass
+// dropped, no base vector): such a row's value is the base term from the
+// first iteration on, so both iterate buffers hold it after two, and its
+// item's statistics (dangling sum, zero change) repeat exactly.
 int eg_pb_iterate(eg_graph *g, const float *contrib, float *y, const PrArgs &a, int S,
-                  hipStream_t stream);
+                  hipStream_t stream, bool skip_static = false);

@@ -685,7 +685,7 @@ extern "C" int eg_pagerank(eg_graph *g, const eg_pr_opts *o, float *rank_out, do
     a.dmass_in = g->pr_stats + 2 * it;
     a.stats_out = g->pr_stats + 2 * (it + 1);
     if (binned)
-      EG_TRY(eg_pb_iterate(g, contrib, rank_next, a, S, nullptr));
+      EG_TRY(eg_pb_iterate(g, contrib, rank_next, a, S, nullptr, !uniform && it >= 2));
     else
       EG_TRY(launch_pull(g, true, ptr, idx, nullptr, contrib, rank_next, *pl, a));
     std::swap(rank, rank_next);
