@@ -825,16 +825,32 @@ __global__ __launch_bounds__(256) void k_pb_finish(PbDev d, int S, float *__rest
     s_last = prev == gridDim.x - 1;
   }
   __syncthreads();
-  if (s_last && tid == 0) {
-    __threadfence();
-    double td = 0.0, tl = 0.0;
-    for (uint32_t b = 0; b < gridDim.x; b++) {
-      const double *pp = reinterpret_cast<const double *>(&d.fin_part[b]);
-      td += __builtin_nontemporal_load(pp);
-      tl += __builtin_nontemporal_load(pp + 1);
+  if (!s_last) return;  // the same for the whole block
+  // the last block: every thread sums a strided share of the block partials,
+  // then the waves and the block reduce them (a fixed order)
+  __threadfence();
+  double td = 0.0, tl = 0.0;
+  for (uint32_t b = tid; b < gridDim.x; b += 256) {
+    const double *pp = reinterpret_cast<const double *>(&d.fin_part[b]);
+    td += __builtin_nontemporal_load(pp);
+    tl += __builtin_nontemporal_load(pp + 1);
+  }
+  td = eg_wave_sum_f64(td);
+  tl = eg_wave_sum_f64(tl);
+  __syncthreads();  // s_d / s_l reuse
+  if (lane == 0) {
+    s_d[wave] = td;
+    s_l[wave] = tl;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sd = 0.0, sl = 0.0;
+    for (int q = 0; q < 4; q++) {
+      sd += s_d[q];
+      sl += s_l[q];
     }
-    a.stats_out[0] = td;
-    a.stats_out[1] = tl;
+    a.stats_out[0] = sd;
+    a.stats_out[1] = sl;
     *d.fin_counter = 0u;
   }
 }
