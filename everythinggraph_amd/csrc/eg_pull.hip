@@ -317,16 +317,22 @@ __global__ __launch_bounds__(kBlock) void k_pull_finish(float *__restrict__ y, P
   const uint32_t gthreads = gridDim.x * kBlock;
   const float basep = PR ? pr_base(a) : 0.f;
   double dang = 0.0, l1 = 0.0;
-  for (uint32_t j = gtid; j < pl.nlong; j += gthreads) {
+  // a wave per long row: lanes merge strided segment partials, then the wave
+  // merges the lanes' pairs in a fixed order
+  const uint32_t gwave = gtid >> 6, nwaves = gthreads >> 6;
+  for (uint32_t j = gwave; j < pl.nlong; j += nwaves) {
     const uint32_t r = pl.long_rows[j];
     EgKahan acc;
-    for (uint32_t s = pl.seg_base[j]; s < pl.seg_base[j + 1]; s++) {
+    for (uint32_t s = pl.seg_base[j] + lane; s < pl.seg_base[j + 1]; s += 64) {
       const float2 p = pl.seg_part[s];
       acc.merge(p.x, p.y);
     }
-    const uint32_t deg = PR ? a.out_deg[r] : 0u;
-    const float rold = PR ? a.rank_old[r] : 0.f;
-    finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
+    eg_wave_kahan_allreduce(acc);
+    if (lane == 0) {
+      const uint32_t deg = PR ? a.out_deg[r] : 0u;
+      const float rold = PR ? a.rank_old[r] : 0.f;
+      finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
+    }
   }
   if (!PR) return;
   for (uint32_t c = gtid; c < pl.nchunks; c += gthreads) {
@@ -353,16 +359,32 @@ __global__ __launch_bounds__(kBlock) void k_pull_finish(float *__restrict__ y, P
     s_last = prev == gridDim.x - 1;
   }
   __syncthreads();
-  if (s_last && tid == 0) {
-    __threadfence();
-    double td = 0.0, tl = 0.0;
-    for (uint32_t b = 0; b < gridDim.x; b++) {
-      const double *pp = reinterpret_cast<const double *>(&pl.fin_part[b]);
-      td += __builtin_nontemporal_load(pp);
-      tl += __builtin_nontemporal_load(pp + 1);
+  if (!s_last) return;  // the same for the whole block
+  // the last block: every thread sums a strided share of the block partials,
+  // then the waves and the block reduce them (a fixed order)
+  __threadfence();
+  double td = 0.0, tl = 0.0;
+  for (uint32_t b = tid; b < gridDim.x; b += kBlock) {
+    const double *pp = reinterpret_cast<const double *>(&pl.fin_part[b]);
+    td += __builtin_nontemporal_load(pp);
+    tl += __builtin_nontemporal_load(pp + 1);
+  }
+  td = eg_wave_sum_f64(td);
+  tl = eg_wave_sum_f64(tl);
+  __syncthreads();  // s_d / s_l reuse
+  if (lane == 0) {
+    s_d[wave] = td;
+    s_l[wave] = tl;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sd = 0.0, sl = 0.0;
+    for (int q = 0; q < kWaves; q++) {
+      sd += s_d[q];
+      sl += s_l[q];
     }
-    a.stats_out[0] = td;
-    a.stats_out[1] = tl;
+    a.stats_out[0] = sd;
+    a.stats_out[1] = sl;
     *pl.fin_counter = 0u;
   }
 }
