@@ -317,13 +317,30 @@ __global__ __launch_bounds__(kBlock) void k_pull_finish(float *__restrict__ y, P
   const uint32_t gthreads = gridDim.x * kBlock;
   const float basep = PR ? pr_base(a) : 0.f;
   double dang = 0.0, l1 = 0.0;
-  // a wave per long row: lanes merge strided segment partials, then the wave
-  // merges the lanes' pairs in a fixed order
-  const uint32_t gwave = gtid >> 6, nwaves = gthreads >> 6;
-  for (uint32_t j = gwave; j < pl.nlong; j += nwaves) {
+  // Long rows of up to kWaveSegs segments: a thread each, merging the
+  // segment partials in order.  Longer ones (the hubs): a wave each, lanes
+  // merging strided partials, then the lanes' pairs in a fixed order.
+  constexpr uint32_t kWaveSegs = 32;
+  for (uint32_t j = gtid; j < pl.nlong; j += gthreads) {
+    const uint32_t s0 = pl.seg_base[j], s1 = pl.seg_base[j + 1];
+    if (s1 - s0 > kWaveSegs) continue;
     const uint32_t r = pl.long_rows[j];
     EgKahan acc;
-    for (uint32_t s = pl.seg_base[j] + lane; s < pl.seg_base[j + 1]; s += 64) {
+    for (uint32_t s = s0; s < s1; s++) {
+      const float2 p = pl.seg_part[s];
+      acc.merge(p.x, p.y);
+    }
+    const uint32_t deg = PR ? a.out_deg[r] : 0u;
+    const float rold = PR ? a.rank_old[r] : 0.f;
+    finish_row<PR>(r, acc.value(), y, a, basep, deg, rold, dang, l1);
+  }
+  const uint32_t gwave = gtid >> 6, nwaves = gthreads >> 6;
+  for (uint32_t j = gwave; j < pl.nlong; j += nwaves) {
+    const uint32_t s0 = pl.seg_base[j], s1 = pl.seg_base[j + 1];
+    if (s1 - s0 <= kWaveSegs) continue;  // the same for the whole wave
+    const uint32_t r = pl.long_rows[j];
+    EgKahan acc;
+    for (uint32_t s = s0 + lane; s < s1; s += 64) {
       const float2 p = pl.seg_part[s];
       acc.merge(p.x, p.y);
     }
