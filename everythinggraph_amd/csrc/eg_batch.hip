@@ -145,6 +145,10 @@ __device__ __forceinline__ Lanes<NW> wave_or(Lanes<NW> v) {
 
 // ctr word holding the progress bits of lane word j
 __host__ __device__ constexpr int progress_slot(int j) { return j == 0 ? 0 : 3 + j; }
+// ctr words 8 and 9: count and degree sum of the frontier the running level
+// builds (device-driven levels); the partials of the passes over n start at 10.
+constexpr int kCtrFront = 8;
+constexpr int kCtrPart = 10;
 
 // Device-driven levels (eg_bfs_batch): control words written by the level
 // kernels themselves, so that the host enqueues levels without reading
@@ -184,6 +188,32 @@ __device__ __forceinline__ void publish_progress(Lanes<NW> nw, uint64_t *s_or,
       unsigned long long *slot = &ctr[progress_slot(j)];
       const uint64_t known = __atomic_load_n(slot, __ATOMIC_RELAXED);
       if (b & ~known) atomicOr(slot, (unsigned long long)b);
+    }
+  }
+}
+
+// Out-degree statistics of the vertices a level reaches, added to fs for the
+// next level's direction decision: fs[0] those with out-edges, fs[1] their
+// degree sum.  One pair of atomics per workgroup that reached any.
+__device__ __forceinline__ void publish_front_stats(unsigned long long c, unsigned long long d,
+                                                    unsigned long long *s_cd,
+                                                    unsigned long long *__restrict__ fs) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    c += __shfl_xor(c, o);
+    d += __shfl_xor(d, o);
+  }
+  if (__lane_id() == 0) {
+    s_cd[2 * (threadIdx.x >> 6)] = c;
+    s_cd[2 * (threadIdx.x >> 6) + 1] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    c = s_cd[0] + s_cd[2] + s_cd[4] + s_cd[6];
+    d = s_cd[1] + s_cd[3] + s_cd[5] + s_cd[7];
+    if (c) {
+      atomicAdd(&fs[0], c);
+      atomicAdd(&fs[1], d);
     }
   }
 }
@@ -287,7 +317,8 @@ __device__ __forceinline__ void ms_pull_item(
     const uint32_t *__restrict__ long_rows, uint32_t nseg, uint64_t *__restrict__ seg_or,
     uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
-    const Lanes<NW> &active, unsigned long long *__restrict__ ctr) {
+    const Lanes<NW> &active, unsigned long long *__restrict__ ctr, unsigned long long &fc,
+    unsigned long long &fd) {
   static_assert(kLds <= 65535 && kChunk <= 65535, "u16 LDS offsets");
   __shared__ uint32_t s_idx[kLds];
   __shared__ uint16_t s_ptr[kChunk + 1];
@@ -371,7 +402,12 @@ __device__ __forceinline__ void ms_pull_item(
           if (acc.covers(need)) break;
         }
       }
-      nwall = nwall | finish_ms_new<NW>(n, r, acc & need, seen, nvisit, lvl, s_tab);
+      const Lanes<NW> nw = finish_ms_new<NW>(n, r, acc & need, seen, nvisit, lvl, s_tab);
+      if (nw.any()) {
+        fc += rb > ra;
+        fd += rb - ra;
+      }
+      nwall = nwall | nw;
     } else {
       s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
     }
@@ -393,7 +429,14 @@ __device__ __forceinline__ void ms_pull_item(
         if (acc.covers(need)) break;
       }
     }
-    if (lane == 0) nwall = nwall | finish_ms_new<NW>(n, r, acc & need, seen, nvisit, lvl, s_tab);
+    if (lane == 0) {
+      const Lanes<NW> nw = finish_ms_new<NW>(n, r, acc & need, seen, nvisit, lvl, s_tab);
+      if (nw.any()) {
+        fc += 1;  // a listed row has more than kSerialRow entries
+        fd += rb - ra;
+      }
+      nwall = nwall | nw;
+    }
   }
   publish_progress<NW>(nwall, s_or, ctr);
 }
@@ -409,35 +452,49 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
     const uint32_t *__restrict__ long_rows, uint32_t nseg, uint32_t nitems,
     uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
-    Lanes<NW> active, unsigned long long *__restrict__ ctr, const uint32_t *__restrict__ ctl) {
+    Lanes<NW> active, unsigned long long *__restrict__ ctr, const uint32_t *__restrict__ ctl,
+    unsigned long long *__restrict__ fs) {
+  __shared__ unsigned long long s_cd[2 * kWaves];
   if (ms_skip(ctl, 0)) return;
+  unsigned long long fc = 0, fd = 0;
   for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
     ms_pull_item<B, NW>(item, n, ptr, idx, chunk_info, seg_info, long_rows, nseg, seg_or, seen,
-                        visit, nvisit, lvl, tab, active, ctr);
+                        visit, nvisit, lvl, tab, active, ctr, fc, fd);
     __syncthreads();  // the item's LDS is reused by the next one
   }
+  if (fs) publish_front_stats(fc, fd, s_cd, fs);
 }
 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
-    uint32_t n, const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ seg_base, uint32_t nlong,
+    uint32_t n, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ long_rows,
+    const uint32_t *__restrict__ seg_base, uint32_t nlong,
     const uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
-    Lanes<NW> active, unsigned long long *__restrict__ ctr, const uint32_t *__restrict__ ctl) {
+    Lanes<NW> active, unsigned long long *__restrict__ ctr, const uint32_t *__restrict__ ctl,
+    unsigned long long *__restrict__ fs) {
   __shared__ uint64_t s_or[kWaves * NW];
+  __shared__ unsigned long long s_cd[2 * kWaves];
   if (ms_skip(ctl, 0)) return;
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   Lanes<NW> nw = Lanes<NW>::zero();
+  unsigned long long fc = 0, fd = 0;
   if (j < nlong) {
     Lanes<NW> acc = Lanes<NW>::zero();
     for (uint32_t s = seg_base[j]; s < seg_base[j + 1]; s++) acc = acc | ld_lanes<NW>(seg_or, s);
-    nw = finish_ms_row<NW>(n, long_rows[j], acc, active, seen, nvisit, lvl, tab);
+    const uint32_t r = long_rows[j];
+    nw = finish_ms_row<NW>(n, r, acc, active, seen, nvisit, lvl, tab);
+    if (fs && nw.any()) {
+      fc = 1;  // a long row has entries
+      fd = ptr[r + 1] - ptr[r];
+    }
   }
   publish_progress<NW>(nw, s_or, ctr);
+  if (fs) publish_front_stats(fc, fd, s_cd, fs);
 }
 
 // Frontier statistics of visit: [1] vertices with out-edges, [2] their degree
-// sum (per-workgroup partials, summed by k_ms_stats_sum).
+// sum (per-workgroup partials, summed by k_ms_stats_sum into out[0..1]).
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_stats(uint32_t n, const uint64_t *__restrict__ visit,
                                                      const uint32_t *__restrict__ ptr,
@@ -471,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_stats(uint32_t n, const uint64_t 
 
 __global__ __launch_bounds__(kBlock) void k_ms_stats_sum(const unsigned long long *__restrict__ part,
                                                          uint32_t np,
-                                                         unsigned long long *__restrict__ ctr) {
+                                                         unsigned long long *__restrict__ out) {
   __shared__ unsigned long long s_c[kWaves], s_d[kWaves];
   unsigned long long c = 0, d = 0;
   for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
@@ -489,8 +546,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_stats_sum(const unsigned long lon
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    ctr[1] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
-    ctr[2] = s_d[0] + s_d[1] + s_d[2] + s_d[3];
+    out[0] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    out[1] = s_d[0] + s_d[1] + s_d[2] + s_d[3];
   }
 }
 
@@ -595,35 +652,47 @@ __global__ __launch_bounds__(kBlock) void k_ms_push(
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, uint64_t *__restrict__ pend,
+__global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, const uint32_t *__restrict__ ptr,
+                                                     uint64_t *__restrict__ pend,
                                                      uint64_t *__restrict__ seen,
                                                      uint64_t *__restrict__ nvisit,
                                                      uint16_t *__restrict__ lvl,
                                                      const uint16_t *__restrict__ tab,
                                                      Lanes<NW> active,
                                                      unsigned long long *__restrict__ ctr,
-                                                     const uint32_t *__restrict__ ctl) {
+                                                     const uint32_t *__restrict__ ctl,
+                                                     unsigned long long *__restrict__ fs) {
   __shared__ uint64_t s_or[kWaves * NW];
   __shared__ uint16_t s_tab[64 * NW];
+  __shared__ unsigned long long s_cd[2 * kWaves];
   if (ms_skip(ctl, 1)) return;
   for (uint32_t i = threadIdx.x; i < 64 * NW; i += blockDim.x) s_tab[i] = tab[i];
   __syncthreads();
   Lanes<NW> all = Lanes<NW>::zero();
+  unsigned long long fc = 0, fd = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     const Lanes<NW> p = ld_lanes<NW>(pend, v);
     if (p.any()) {
       st_lanes<NW>(pend, v, Lanes<NW>::zero());
-      all = all | finish_ms_row<NW>(n, v, p, active, seen, nvisit, lvl, s_tab);
+      const Lanes<NW> nw = finish_ms_row<NW>(n, v, p, active, seen, nvisit, lvl, s_tab);
+      if (fs && nw.any()) {
+        const uint32_t dv = ptr[v + 1] - ptr[v];
+        fc += dv > 0;
+        fd += dv;
+      }
+      all = all | nw;
     } else {
       st_lanes<NW>(nvisit, v, Lanes<NW>::zero());
     }
   }
   publish_progress<NW>(all, s_or, ctr);
+  if (fs) publish_front_stats(fc, fd, s_cd, fs);
 }
 
 // ---- device-driven levels (eg_bfs_batch) ----
-// Every level enqueues the same kernels with the same grids: statistics of
-// the frontier, the direction decision, the push kernels (queue, degrees,
+// Every level enqueues the same kernels with the same grids: the direction
+// decision (from the frontier statistics that the last level's kernels added
+// up as they reached its vertices; k_ms_stats counts the seeds), the push kernels (queue, degrees,
 // their scan, the edge push, the apply pass) and the pull kernels; the ones
 // of the other direction return at once.  The decision applies the
 // reference's rule (bfs_simple.c:191-197) to the frontier of all lanes, as the
@@ -632,36 +701,24 @@ __global__ void k_msd_init(uint32_t *__restrict__ ctl) {
   if (threadIdx.x < kCtlWords) ctl[threadIdx.x] = threadIdx.x == kCtlLevel ? 2u : 0u;
 }
 
-// Sums the statistics partials of k_ms_stats, picks the direction, arms the
-// level: level table, zeroed progress words and queue length.
+// Takes the statistics of the frontier the last level built (fs, zeroed for
+// this level's kernels), picks the direction, arms the level: level table,
+// zeroed progress words and queue length.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_msd_decide(const unsigned long long *__restrict__ part,
-                                                       uint32_t np, double thr, int sym,
+__global__ __launch_bounds__(kBlock) void k_msd_decide(unsigned long long *__restrict__ fs,
+                                                       double thr, int sym,
                                                        uint32_t *__restrict__ ctl,
                                                        unsigned long long *__restrict__ ctr,
                                                        uint16_t *__restrict__ tab) {
-  __shared__ unsigned long long s_c[kWaves], s_d[kWaves];
   if (ctl[kCtlDone]) return;
   const uint32_t level = ctl[kCtlLevel];
-  unsigned long long c = 0, d = 0;
-  for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
-    c += part[2 * i];
-    d += part[2 * i + 1];
-  }
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    c += __shfl_xor(c, o);
-    d += __shfl_xor(d, o);
-  }
-  if (__lane_id() == 0) {
-    s_c[threadIdx.x >> 6] = c;
-    s_d[threadIdx.x >> 6] = d;
-  }
-  __syncthreads();  // also orders every thread's read of the level before its update
+  __syncthreads();  // orders every thread's read of the level before its update
   for (uint32_t i = threadIdx.x; i < 64 * NW; i += blockDim.x) tab[i] = (uint16_t)level;
   if (threadIdx.x == 0) {
-    const unsigned long long cnt = s_c[0] + s_c[1] + s_c[2] + s_c[3];
-    const unsigned long long deg = s_d[0] + s_d[1] + s_d[2] + s_d[3];
+    const unsigned long long cnt = fs[0];
+    const unsigned long long deg = fs[1];
+    fs[0] = 0;
+    fs[1] = 0;
     ctr[1] = cnt;
     ctr[2] = deg;
     ctr[3] = 0;
@@ -1225,10 +1282,11 @@ int eg_ms_alloc(eg_graph *g, MsState *st, uint32_t nw) {
   EG_TRY(eg_dalloc_t(&st->lvl, (uint64_t)n * 64 * nw, "ms levels"));
   EG_TRY(eg_dalloc_t(&st->queue, (uint64_t)n + 64, "ms queue"));
   EG_TRY(eg_dalloc_t(&st->qoff, (uint64_t)n + 65, "ms queue offsets"));
-  // [0] and [4] progress words, [1] count, [2] degree sum, [3] queue length;
-  // [8..) holds the frontier statistics partials (2 per workgroup) or the
-  // per-lane reached partials (64 nw x 64) and sums (64 nw), never both at once.
-  EG_TRY(eg_dalloc_t(&st->ctr, 8 + 2 * (uint64_t)kStatBlocks + 64 * 64 * nw + 64 * nw,
+  // [0] and [4..6] progress words, [1] count, [2] degree sum, [3] queue
+  // length, [8..9] the statistics of the frontier a device-driven level
+  // builds; [10..) holds the frontier statistics partials (2 per workgroup) or
+  // the per-lane reached partials (64 nw x 64) and sums (64 nw), never both at once.
+  EG_TRY(eg_dalloc_t(&st->ctr, kCtrPart + 2 * (uint64_t)kStatBlocks + 64 * 64 * nw + 64 * nw,
                      "ms counters"));
   EG_TRY(eg_dalloc_t(&st->lvl_tab, 64 * nw, "ms level table"));
   EG_TRY(eg_dalloc_t(&st->roots, 2 * kMaxLanes, "ms roots"));
@@ -1307,12 +1365,12 @@ template <int NW>
 static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *push) {
   const uint32_t n = g->n;
   hipStream_t s = st->stream;
-  unsigned long long *part = st->ctr + 8;
+  unsigned long long *part = st->ctr + kCtrPart;
   const uint32_t nb = blocks_for(n, kBlock) < kStatBlocks ? blocks_for(n, kBlock) : kStatBlocks;
   hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part,
                      (const uint32_t *)nullptr);
   EG_LAUNCHED();
-  hipLaunchKernelGGL(k_ms_stats_sum, dim3(1), dim3(kBlock), 0, s, part, nb, st->ctr);
+  hipLaunchKernelGGL(k_ms_stats_sum, dim3(1), dim3(kBlock), 0, s, part, nb, st->ctr + 1);
   EG_LAUNCHED();
   EG_HIP(hipMemcpyAsync(st->h, st->ctr, 24, hipMemcpyDeviceToHost, s));
   EG_HIP(hipStreamSynchronize(s));
@@ -1382,9 +1440,9 @@ static int ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_
       EG_LAUNCHED();
     }
     const uint32_t nb = blocks_for(n, kBlock) < 4096 ? blocks_for(n, kBlock) : 4096;
-    hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->pend, st->seen,
-                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr,
-                       (const uint32_t *)nullptr);
+    hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nb), dim3(kBlock), 0, s, n, bv.ptr, st->pend,
+                       st->seen, st->nvisit, st->lvl, st->lvl_tab, active, st->ctr,
+                       (const uint32_t *)nullptr, (unsigned long long *)nullptr);
     EG_LAUNCHED();
   } else {
     EG_TRY(ms_seg_or(st, pl));
@@ -1394,19 +1452,21 @@ static int ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_
         hipLaunchKernelGGL((k_ms_pull<16, NW>), dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s,
                            n, bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows,
                            pl.nseg, pl.nseg + pl.nchunks, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl,
-                           st->lvl_tab, active, st->ctr, (const uint32_t *)nullptr);
+                           st->lvl_tab, active, st->ctr, (const uint32_t *)nullptr,
+                           (unsigned long long *)nullptr);
       else
         hipLaunchKernelGGL((k_ms_pull<8, NW>), dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
                            bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                            pl.nseg + pl.nchunks, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
-                           active, st->ctr, (const uint32_t *)nullptr);
+                           active, st->ctr, (const uint32_t *)nullptr,
+                           (unsigned long long *)nullptr);
       EG_LAUNCHED();
     }
     if (pl.nlong) {
       hipLaunchKernelGGL(k_ms_pull_finish<NW>, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock),
-                         0, s, n, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen,
-                         st->nvisit, st->lvl, st->lvl_tab, active, st->ctr,
-                         (const uint32_t *)nullptr);
+                         0, s, n, bv.ptr, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or,
+                         st->seen, st->nvisit, st->lvl, st->lvl_tab, active, st->ctr,
+                         (const uint32_t *)nullptr, (unsigned long long *)nullptr);
       EG_LAUNCHED();
     }
   }
@@ -1433,15 +1493,13 @@ template <int NW>
 static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<NW> &active) {
   const uint32_t n = g->n;
   hipStream_t s = st->stream;
-  unsigned long long *part = st->ctr + 8;
+  unsigned long long *fs = st->ctr + kCtrFront;
   const uint32_t nb = std::min(blocks_for(n, kBlock), kStatBlocks);
   const PullPlan &pl = *bv.pl;
   uint32_t *spart = reinterpret_cast<uint32_t *>(st->scan_tmp);
   const unsigned long long *qn = st->ctr + 3;
-  hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part,
-                     st->ctl);
   const double push_max = (double)g->m / (double)g->tune.ms_push_div;
-  hipLaunchKernelGGL(k_msd_decide<NW>, dim3(1), dim3(kBlock), 0, s, part, nb, push_max,
+  hipLaunchKernelGGL(k_msd_decide<NW>, dim3(1), dim3(kBlock), 0, s, fs, push_max,
                      bv.sym ? 1 : 0, st->ctl, st->ctr, st->lvl_tab);
   // push: the frontier's vertices with out-edges, their degree scan, the
   // edge push (a push level's edges are at most push_max) and the apply pass
@@ -1459,8 +1517,8 @@ static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<
   hipLaunchKernelGGL(k_ms_push<NW>, dim3(pgrid), dim3(kBlock), 0, s, st->queue, 0u, qn, st->qoff,
                      bv.ptr, bv.ms_idx, st->visit, st->seen, st->pend, active, st->ctl);
   const uint32_t nba = std::min(blocks_for(n, kBlock), kDevGrid);
-  hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nba), dim3(kBlock), 0, s, n, st->pend, st->seen,
-                     st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl);
+  hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nba), dim3(kBlock), 0, s, n, bv.ptr, st->pend,
+                     st->seen, st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl, fs);
   // pull over the plan
   const uint32_t nitems = pl.nseg + pl.nchunks;
   const uint32_t pull_grid = std::max(1u, std::min(nitems, kDevPullGrid));
@@ -1468,16 +1526,16 @@ static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<
     hipLaunchKernelGGL((k_ms_pull<16, NW>), dim3(pull_grid), dim3(kBlock), 0, s, n,
                        bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                        nitems, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab, active,
-                       st->ctr, st->ctl);
+                       st->ctr, st->ctl, fs);
   else
     hipLaunchKernelGGL((k_ms_pull<8, NW>), dim3(pull_grid), dim3(kBlock), 0, s, n,
                        bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                        nitems, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
-                       active, st->ctr, st->ctl);
+                       active, st->ctr, st->ctl, fs);
   if (pl.nlong)
     hipLaunchKernelGGL(k_ms_pull_finish<NW>, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock), 0,
-                       s, n, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen,
-                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl);
+                       s, n, bv.ptr, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen,
+                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl, fs);
   hipLaunchKernelGGL(k_msd_end<NW>, dim3(1), dim3(1), 0, s, st->ctl, st->ctr);
   EG_LAUNCHED();
   std::swap(st->visit, st->nvisit);
@@ -1525,6 +1583,17 @@ static int msd_run(eg_graph *g, const BatchView &bv, MsState *st, const uint64_t
   if (graphs) EG_TRY(msd_graph<NW>(g, bv, st, active));
   hipLaunchKernelGGL(k_msd_init, dim3(1), dim3(64), 0, s, st->ctl);
   EG_LAUNCHED();
+  // statistics of the seeded frontier; every later level's kernels count the
+  // frontier they build
+  {
+    const uint32_t nb = std::min(blocks_for(g->n, kBlock), kStatBlocks);
+    unsigned long long *part = st->ctr + kCtrPart;
+    hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, g->n, st->visit, bv.ptr, part,
+                       st->ctl);
+    hipLaunchKernelGGL(k_ms_stats_sum, dim3(1), dim3(kBlock), 0, s, part, nb,
+                       st->ctr + kCtrFront);
+    EG_LAUNCHED();
+  }
   // levels per batch: the last traversal's depth first (rounded up to a swap
   // cycle), then 4, 8, 16, 32 at a time (a few idle levels at the end of a
   // deep traversal against one host wait per batch)
@@ -1594,7 +1663,7 @@ int eg_ms_reached(eg_graph *g, MsState *st, uint32_t k, const uint32_t *lanes,
   uint32_t h[kMaxLanes];
   memcpy(h, lanes, k * 4);
   EG_HIP(hipMemcpyAsync(st->roots, h, k * 4, hipMemcpyHostToDevice, s));
-  unsigned long long *part = st->ctr + 8, *sums = st->ctr + 8 + 64 * 64 * st->nw;
+  unsigned long long *part = st->ctr + kCtrPart, *sums = st->ctr + kCtrPart + 64 * 64 * st->nw;
   hipLaunchKernelGGL(k_ms_reached, dim3(nb, k), dim3(kBlock), 0, s, g->n, st->lvl, st->roots, part);
   EG_LAUNCHED();
   hipLaunchKernelGGL(k_ms_reached_sum, dim3(k), dim3(64), 0, s, part, nb, sums);
