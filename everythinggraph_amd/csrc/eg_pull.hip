@@ -407,8 +407,10 @@ __global__ __launch_bounds__(kBlock) void k_pull_finish(float *__restrict__ y, P
 }
 
 __global__ void k_pr_init(uint32_t n, float init, const uint32_t *__restrict__ out_deg,
-                          float *__restrict__ rank, float *__restrict__ contrib) {
+                          float *__restrict__ rank, float *__restrict__ contrib, double dmass0,
+                          double *__restrict__ stats) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v == 0) stats[0] = dmass0;
   if (v >= n) return;
   const uint32_t d = out_deg[v];
   rank[v] = init;
@@ -623,12 +625,10 @@ void eg_pr_resolve(const eg_graph *g, const eg_pr_opts *o, double *damping, int 
 // mass of that iterate (init times the number of sinks) in pr_stats[0].
 int eg_pr_start(eg_graph *g, double init, const uint32_t *deg, float *rank, float *contrib) {
   const uint32_t n = g->n;
-  hipLaunchKernelGGL(k_pr_init, dim3(blocks_for(n, 256)), dim3(256), 0, g->stream, n, (float)init,
-                     deg, rank, contrib);
-  EG_LAUNCHED();
   const double d0 = (double)g->ndangling * (double)(float)init;
-  EG_HIP(hipMemcpyAsync(g->pr_stats, &d0, sizeof(double), hipMemcpyHostToDevice, g->stream));
-  EG_HIP(hipStreamSynchronize(g->stream));
+  hipLaunchKernelGGL(k_pr_init, dim3(std::max(1u, blocks_for(n, 256))), dim3(256), 0, g->stream, n,
+                     (float)init, deg, rank, contrib, d0, g->pr_stats);
+  EG_LAUNCHED();
   return EG_OK;
 }
 
