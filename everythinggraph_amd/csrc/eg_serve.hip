@@ -922,7 +922,9 @@ extern "C" int eg_serve_wait(eg_server *S, uint64_t ticket, double timeout_ms,
   }
   const eg_serve_result r = q->res;
   const std::string err = q->err;
-  S->all.erase(it);
+  // by key: the wait released the lock, and a submit that rehashed the table
+  // in the meantime invalidated `it`
+  S->all.erase(ticket);
   lk.unlock();
   delete q;
   if (res) *res = r;

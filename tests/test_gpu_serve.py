@@ -282,3 +282,26 @@ def test_solo_pipeline_equals_single_vector(binned, undirected):
                 assert all(info["levels"] == 1 for info, _ in got)
             if cap == 5:
                 assert all(info["levels"] == 5 for info, _ in got)
+
+
+def test_wait_while_submitting():
+    """A waiter blocked on one ticket while another thread submits hundreds of
+    queries (the ticket table grows and rehashes under it) gets its result,
+    and every query matches the oracle."""
+    import threading
+    n, s0, d0, m, ptr, col, cp, row = build(11, 47, True)
+    qs = queries(n, np.diff(ptr), 300, seed=9)
+    k = next(i for i, q in enumerate(qs) if q[0] == "ppr")
+    qs.insert(0, qs.pop(k))  # the waiter's query runs longest
+    with eg.Graph(n, 2, undirected=True) as g:
+        g.load_edges(s0, d0)
+        with g.serve() as S:
+            first = submit(S, qs[0])
+            box = {}
+            waiter = threading.Thread(target=lambda: box.setdefault("r", S.wait(first)))
+            waiter.start()
+            tickets = [submit(S, q) for q in qs[1:]]
+            waiter.join(60)
+            assert not waiter.is_alive()
+            res = [box["r"]] + [S.wait(t) for t in tickets]
+    check_against_oracle(n, m, ptr, col, cp, row, qs, res)
