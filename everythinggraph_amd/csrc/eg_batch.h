@@ -38,6 +38,7 @@ struct MsState {
   uint64_t *seg_or = nullptr;  // [nseg] long-row partial ORs
   uint32_t seg_cap = 0;
   uint32_t *queue = nullptr, *qoff = nullptr;  // frontier vertices with edges, degree scan
+  uint64_t *fbits = nullptr;  // device-driven levels: bit v = v is in the frontier and has edges
   unsigned long long *ctr = nullptr;  // [0] progress, [1] count, [2] degsum, [3] qlen, [8..) partials
   uint16_t *lvl_tab = nullptr;                 // [64 nw] level written per lane this step
   uint32_t *roots = nullptr;                   // [512]: lanes, roots (staging)

@@ -218,6 +218,33 @@ __device__ __forceinline__ void publish_front_stats(unsigned long long c, unsign
   }
 }
 
+// Frontier bitmap of the device-driven levels (bit v of word v / 64: v is in
+// the frontier and has out-edges), kept by the kernels that build the frontier
+// so that the push queue is compacted from n / 8 bytes instead of the masks.
+// Sets the bits of rows r + i (bit i of mask) to bit i of bits and leaves the
+// others.  Every bit has one writer per level; rows of one word may belong to
+// two workgroups, hence the atomics.
+__device__ __forceinline__ void fbits_put(uint64_t *fbits, uint32_t r, uint64_t mask,
+                                          uint64_t bits) {
+  unsigned long long *f = reinterpret_cast<unsigned long long *>(fbits);
+  bits &= mask;
+  const uint32_t w = r >> 6, o = r & 63u;
+  const uint64_t m0 = mask << o, b0 = bits << o;
+  if (m0 == ~0ull) {
+    f[w] = b0;
+  } else {
+    atomicAnd(&f[w], ~m0);
+    if (b0) atomicOr(&f[w], b0);
+  }
+  if (o) {
+    const uint64_t m1 = mask >> (64u - o), b1 = bits >> (64u - o);
+    if (m1) {
+      atomicAnd(&f[w + 1], ~m1);
+      if (b1) atomicOr(&f[w + 1], b1);
+    }
+  }
+}
+
 // Level store: lane-major u16, lvl[lane * n + v].
 template <int NW>
 __device__ __forceinline__ void store_levels(uint32_t n, uint32_t r, const Lanes<NW> &nw,
@@ -318,7 +345,7 @@ __device__ __forceinline__ void ms_pull_item(
     uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
     const Lanes<NW> &active, unsigned long long *__restrict__ ctr, unsigned long long &fc,
-    unsigned long long &fd) {
+    unsigned long long &fd, uint64_t *__restrict__ fbits) {
   static_assert(kLds <= 65535 && kChunk <= 65535, "u16 LDS offsets");
   __shared__ uint32_t s_idx[kLds];
   __shared__ uint16_t s_ptr[kChunk + 1];
@@ -368,7 +395,14 @@ __device__ __forceinline__ void ms_pull_item(
   for (uint32_t i = tid; i <= nrows; i += kBlock) s_ptr[i] = (uint16_t)(ptr[r0 + i] - k0);
   any = __syncthreads_or(any);
   if (!any) {
-    for (uint32_t i = tid; i < nrows; i += kBlock) st_lanes<NW>(nvisit, r0 + i, Lanes<NW>::zero());
+    for (uint32_t i = tid; i < nrows; i += kBlock) {
+      st_lanes<NW>(nvisit, r0 + i, Lanes<NW>::zero());
+      // lane 0 holds the wave's first row
+      if (fbits) {
+        const uint64_t m = __ballot(1);  // the wave's rows, from lane 0's
+        if (lane == 0) fbits_put(fbits, r0 + i, m, 0ull);
+      }
+    }
     return;
   }
   for (uint32_t base = k0; base < k1; base += kBlock * B) {
@@ -388,7 +422,9 @@ __device__ __forceinline__ void ms_pull_item(
   Lanes<NW> nwall = Lanes<NW>::zero();
   for (uint32_t i = tid; i < nrows; i += kBlock) {
     const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
-    if (rb - ra <= kSerialRow) {
+    bool hit = false;  // a new frontier row with entries (listed rows: their own bit below)
+    const bool serial = rb - ra <= kSerialRow;
+    if (serial) {
       const uint32_t r = r0 + i;
       const Lanes<NW> need = active & ~ld_lanes<NW>(seen, r);
       Lanes<NW> acc = Lanes<NW>::zero();
@@ -406,10 +442,15 @@ __device__ __forceinline__ void ms_pull_item(
       if (nw.any()) {
         fc += rb > ra;
         fd += rb - ra;
+        hit = rb > ra;
       }
       nwall = nwall | nw;
     } else {
       s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
+    }
+    if (fbits) {
+      const uint64_t m = __ballot(serial), b = __ballot(hit);
+      if (lane == 0) fbits_put(fbits, r0 + i, m, b);
     }
   }
   __syncthreads();
@@ -435,6 +476,13 @@ __device__ __forceinline__ void ms_pull_item(
         fc += 1;  // a listed row has more than kSerialRow entries
         fd += rb - ra;
       }
+      if (fbits) {
+        unsigned long long *f = reinterpret_cast<unsigned long long *>(&fbits[r >> 6]);
+        if (nw.any())
+          atomicOr(f, 1ull << (r & 63u));
+        else
+          atomicAnd(f, ~(1ull << (r & 63u)));
+      }
       nwall = nwall | nw;
     }
   }
@@ -453,13 +501,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull(
     uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen, const uint64_t *__restrict__ visit,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
     Lanes<NW> active, unsigned long long *__restrict__ ctr, const uint32_t *__restrict__ ctl,
-    unsigned long long *__restrict__ fs) {
+    unsigned long long *__restrict__ fs, uint64_t *__restrict__ fbits) {
   __shared__ unsigned long long s_cd[2 * kWaves];
   if (ms_skip(ctl, 0)) return;
   unsigned long long fc = 0, fd = 0;
   for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
     ms_pull_item<B, NW>(item, n, ptr, idx, chunk_info, seg_info, long_rows, nseg, seg_or, seen,
-                        visit, nvisit, lvl, tab, active, ctr, fc, fd);
+                        visit, nvisit, lvl, tab, active, ctr, fc, fd, fbits);
     __syncthreads();  // the item's LDS is reused by the next one
   }
   if (fs) publish_front_stats(fc, fd, s_cd, fs);
@@ -472,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
     const uint64_t *__restrict__ seg_or, uint64_t *__restrict__ seen,
     uint64_t *__restrict__ nvisit, uint16_t *__restrict__ lvl, const uint16_t *__restrict__ tab,
     Lanes<NW> active, unsigned long long *__restrict__ ctr, const uint32_t *__restrict__ ctl,
-    unsigned long long *__restrict__ fs) {
+    unsigned long long *__restrict__ fs, uint64_t *__restrict__ fbits) {
   __shared__ uint64_t s_or[kWaves * NW];
   __shared__ unsigned long long s_cd[2 * kWaves];
   if (ms_skip(ctl, 0)) return;
@@ -488,6 +536,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_pull_finish(
       fc = 1;  // a long row has entries
       fd = ptr[r + 1] - ptr[r];
     }
+    if (fbits) {  // no chunk holds a long row: its bit is set here only
+      unsigned long long *f = reinterpret_cast<unsigned long long *>(&fbits[r >> 6]);
+      if (nw.any())
+        atomicOr(f, 1ull << (r & 63u));
+      else
+        atomicAnd(f, ~(1ull << (r & 63u)));
+    }
   }
   publish_progress<NW>(nw, s_or, ctr);
   if (fs) publish_front_stats(fc, fd, s_cd, fs);
@@ -499,15 +554,25 @@ template <int NW>
 __global__ __launch_bounds__(kBlock) void k_ms_stats(uint32_t n, const uint64_t *__restrict__ visit,
                                                      const uint32_t *__restrict__ ptr,
                                                      unsigned long long *__restrict__ part,
-                                                     const uint32_t *__restrict__ ctl) {
+                                                     const uint32_t *__restrict__ ctl,
+                                                     uint64_t *__restrict__ fbits) {
   __shared__ unsigned long long s_c[kWaves], s_d[kWaves];
   if (ms_skip(ctl, -1)) return;
   unsigned long long c = 0, d = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    if (ld_lanes<NW>(visit, v).any()) {
+  // a wave takes 64 consecutive vertices (one bitmap word) per step
+  for (uint32_t vb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); vb < n;
+       vb += gridDim.x * blockDim.x) {
+    const uint32_t v = vb + __lane_id();
+    bool hit = false;
+    if (v < n && ld_lanes<NW>(visit, v).any()) {
       const uint32_t dv = ptr[v + 1] - ptr[v];
       c += dv > 0;
       d += dv;
+      hit = dv > 0;
+    }
+    if (fbits) {
+      const uint64_t b = __ballot(hit);
+      if (__lane_id() == 0) fbits[vb >> 6] = b;
     }
   }
 #pragma unroll
@@ -661,7 +726,8 @@ __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, const uint32_t 
                                                      Lanes<NW> active,
                                                      unsigned long long *__restrict__ ctr,
                                                      const uint32_t *__restrict__ ctl,
-                                                     unsigned long long *__restrict__ fs) {
+                                                     unsigned long long *__restrict__ fs,
+                                                     uint64_t *__restrict__ fbits) {
   __shared__ uint64_t s_or[kWaves * NW];
   __shared__ uint16_t s_tab[64 * NW];
   __shared__ unsigned long long s_cd[2 * kWaves];
@@ -670,19 +736,30 @@ __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, const uint32_t 
   __syncthreads();
   Lanes<NW> all = Lanes<NW>::zero();
   unsigned long long fc = 0, fd = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const Lanes<NW> p = ld_lanes<NW>(pend, v);
-    if (p.any()) {
-      st_lanes<NW>(pend, v, Lanes<NW>::zero());
-      const Lanes<NW> nw = finish_ms_row<NW>(n, v, p, active, seen, nvisit, lvl, s_tab);
-      if (fs && nw.any()) {
-        const uint32_t dv = ptr[v + 1] - ptr[v];
-        fc += dv > 0;
-        fd += dv;
+  // a wave takes 64 consecutive vertices (one bitmap word) per step
+  for (uint32_t vb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); vb < n;
+       vb += gridDim.x * blockDim.x) {
+    const uint32_t v = vb + __lane_id();
+    bool hit = false;
+    if (v < n) {
+      const Lanes<NW> p = ld_lanes<NW>(pend, v);
+      if (p.any()) {
+        st_lanes<NW>(pend, v, Lanes<NW>::zero());
+        const Lanes<NW> nw = finish_ms_row<NW>(n, v, p, active, seen, nvisit, lvl, s_tab);
+        if (fs && nw.any()) {
+          const uint32_t dv = ptr[v + 1] - ptr[v];
+          fc += dv > 0;
+          fd += dv;
+          hit = dv > 0;
+        }
+        all = all | nw;
+      } else {
+        st_lanes<NW>(nvisit, v, Lanes<NW>::zero());
       }
-      all = all | nw;
-    } else {
-      st_lanes<NW>(nvisit, v, Lanes<NW>::zero());
+    }
+    if (fbits) {
+      const uint64_t b = __ballot(hit);
+      if (__lane_id() == 0) fbits[vb >> 6] = b;
     }
   }
   publish_progress<NW>(all, s_or, ctr);
@@ -745,6 +822,46 @@ __global__ void k_msd_end(uint32_t *__restrict__ ctl, const unsigned long long *
 #pragma unroll
   for (int j = 0; j < NW; j++) p |= ctr[progress_slot(j)];
   if (!p) ctl[kCtlDone] = 1u;
+}
+
+// Push queue of a device-driven level from the frontier bitmap: a thread
+// expands one word, a workgroup reserves its slots with one atomic.
+__global__ __launch_bounds__(kBlock) void k_msd_compact_bits(uint32_t nwords,
+                                                            const uint64_t *__restrict__ fbits,
+                                                            uint32_t *__restrict__ q,
+                                                            unsigned long long *__restrict__ qlen,
+                                                            const uint32_t *__restrict__ ctl) {
+  __shared__ uint32_t s_w[kWaves];
+  __shared__ uint32_t s_base;
+  if (ms_skip(ctl, 1)) return;
+  const int lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < nwords; b0 += gridDim.x * blockDim.x) {
+    const uint32_t wi = b0 + threadIdx.x;
+    uint64_t word = wi < nwords ? fbits[wi] : 0ull;
+    const uint32_t cnt = (uint32_t)__popcll(word);
+    uint32_t x = cnt;  // inclusive scan over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < kWaves; w++) t += s_w[w];
+      s_base = t ? (uint32_t)atomicAdd(qlen, (unsigned long long)t) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = s_base + x - cnt;
+    for (uint32_t w = 0; w < wave; w++) pos += s_w[w];
+    while (word) {
+      q[pos++] = 64u * wi + (uint32_t)(__ffsll((unsigned long long)word) - 1);
+      word &= word - 1ull;
+    }
+    __syncthreads();  // s_w, s_base reuse
+  }
 }
 
 // Out-degrees of the queue (length *qn), grid-stride.
@@ -1282,6 +1399,7 @@ int eg_ms_alloc(eg_graph *g, MsState *st, uint32_t nw) {
   EG_TRY(eg_dalloc_t(&st->lvl, (uint64_t)n * 64 * nw, "ms levels"));
   EG_TRY(eg_dalloc_t(&st->queue, (uint64_t)n + 64, "ms queue"));
   EG_TRY(eg_dalloc_t(&st->qoff, (uint64_t)n + 65, "ms queue offsets"));
+  EG_TRY(eg_dalloc_t(&st->fbits, (uint64_t)n / 64 + 2, "ms frontier bitmap"));
   // [0] and [4..6] progress words, [1] count, [2] degree sum, [3] queue
   // length, [8..9] the statistics of the frontier a device-driven level
   // builds; [10..) holds the frontier statistics partials (2 per workgroup) or
@@ -1307,6 +1425,7 @@ void eg_ms_free(MsState *st) {
   eg_dfree_t(&st->visit);
   eg_dfree_t(&st->nvisit);
   eg_dfree_t(&st->pend);
+  eg_dfree_t(&st->fbits);
   eg_dfree_t(&st->lvl);
   eg_dfree_t(&st->seg_or);
   eg_dfree_t(&st->queue);
@@ -1368,7 +1487,7 @@ static int frontier_prep(eg_graph *g, const BatchView &bv, MsState *st, bool *pu
   unsigned long long *part = st->ctr + kCtrPart;
   const uint32_t nb = blocks_for(n, kBlock) < kStatBlocks ? blocks_for(n, kBlock) : kStatBlocks;
   hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr, part,
-                     (const uint32_t *)nullptr);
+                     (const uint32_t *)nullptr, (uint64_t *)nullptr);
   EG_LAUNCHED();
   hipLaunchKernelGGL(k_ms_stats_sum, dim3(1), dim3(kBlock), 0, s, part, nb, st->ctr + 1);
   EG_LAUNCHED();
@@ -1442,7 +1561,7 @@ static int ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_
     const uint32_t nb = blocks_for(n, kBlock) < 4096 ? blocks_for(n, kBlock) : 4096;
     hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nb), dim3(kBlock), 0, s, n, bv.ptr, st->pend,
                        st->seen, st->nvisit, st->lvl, st->lvl_tab, active, st->ctr,
-                       (const uint32_t *)nullptr, (unsigned long long *)nullptr);
+                       (const uint32_t *)nullptr, (unsigned long long *)nullptr, (uint64_t *)nullptr);
     EG_LAUNCHED();
   } else {
     EG_TRY(ms_seg_or(st, pl));
@@ -1453,20 +1572,21 @@ static int ms_level(eg_graph *g, const BatchView &bv, MsState *st, const uint64_
                            n, bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows,
                            pl.nseg, pl.nseg + pl.nchunks, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl,
                            st->lvl_tab, active, st->ctr, (const uint32_t *)nullptr,
-                           (unsigned long long *)nullptr);
+                           (unsigned long long *)nullptr, (uint64_t *)nullptr);
       else
         hipLaunchKernelGGL((k_ms_pull<8, NW>), dim3(pl.nseg + pl.nchunks), dim3(kBlock), 0, s, n,
                            bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                            pl.nseg + pl.nchunks, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
                            active, st->ctr, (const uint32_t *)nullptr,
-                           (unsigned long long *)nullptr);
+                           (unsigned long long *)nullptr, (uint64_t *)nullptr);
       EG_LAUNCHED();
     }
     if (pl.nlong) {
       hipLaunchKernelGGL(k_ms_pull_finish<NW>, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock),
                          0, s, n, bv.ptr, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or,
                          st->seen, st->nvisit, st->lvl, st->lvl_tab, active, st->ctr,
-                         (const uint32_t *)nullptr, (unsigned long long *)nullptr);
+                         (const uint32_t *)nullptr, (unsigned long long *)nullptr,
+                         (uint64_t *)nullptr);
       EG_LAUNCHED();
     }
   }
@@ -1503,8 +1623,9 @@ static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<
                      bv.sym ? 1 : 0, st->ctl, st->ctr, st->lvl_tab);
   // push: the frontier's vertices with out-edges, their degree scan, the
   // edge push (a push level's edges are at most push_max) and the apply pass
-  hipLaunchKernelGGL(k_ms_compact<NW>, dim3(nb), dim3(kBlock), 0, s, n, st->visit, bv.ptr,
-                     st->queue, st->ctr + 3, st->ctl);
+  const uint32_t nwords = (n + 63) / 64;
+  hipLaunchKernelGGL(k_msd_compact_bits, dim3(std::min(blocks_for(nwords, kBlock), kStatBlocks)),
+                     dim3(kBlock), 0, s, nwords, st->fbits, st->queue, st->ctr + 3, st->ctl);
   hipLaunchKernelGGL(k_msd_qdeg, dim3(256), dim3(256), 0, s, st->queue, qn, bv.ptr, st->qoff,
                      st->ctl);
   hipLaunchKernelGGL(k_msd_scan_a, dim3(kScanParts), dim3(kBlock), 0, s, st->qoff, qn, spart,
@@ -1518,7 +1639,8 @@ static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<
                      bv.ptr, bv.ms_idx, st->visit, st->seen, st->pend, active, st->ctl);
   const uint32_t nba = std::min(blocks_for(n, kBlock), kDevGrid);
   hipLaunchKernelGGL(k_ms_apply<NW>, dim3(nba), dim3(kBlock), 0, s, n, bv.ptr, st->pend,
-                     st->seen, st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl, fs);
+                     st->seen, st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl, fs,
+                     st->fbits);
   // pull over the plan
   const uint32_t nitems = pl.nseg + pl.nchunks;
   const uint32_t pull_grid = std::max(1u, std::min(nitems, kDevPullGrid));
@@ -1526,16 +1648,16 @@ static int msd_level(eg_graph *g, const BatchView &bv, MsState *st, const Lanes<
     hipLaunchKernelGGL((k_ms_pull<16, NW>), dim3(pull_grid), dim3(kBlock), 0, s, n,
                        bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                        nitems, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab, active,
-                       st->ctr, st->ctl, fs);
+                       st->ctr, st->ctl, fs, st->fbits);
   else
     hipLaunchKernelGGL((k_ms_pull<8, NW>), dim3(pull_grid), dim3(kBlock), 0, s, n,
                        bv.ptr, bv.ms_idx, pl.chunk_info, pl.seg_info, pl.long_rows, pl.nseg,
                        nitems, st->seg_or, st->seen, st->visit, st->nvisit, st->lvl, st->lvl_tab,
-                       active, st->ctr, st->ctl, fs);
+                       active, st->ctr, st->ctl, fs, st->fbits);
   if (pl.nlong)
     hipLaunchKernelGGL(k_ms_pull_finish<NW>, dim3(blocks_for(pl.nlong, kBlock)), dim3(kBlock), 0,
                        s, n, bv.ptr, pl.long_rows, pl.seg_base, pl.nlong, st->seg_or, st->seen,
-                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl, fs);
+                       st->nvisit, st->lvl, st->lvl_tab, active, st->ctr, st->ctl, fs, st->fbits);
   hipLaunchKernelGGL(k_msd_end<NW>, dim3(1), dim3(1), 0, s, st->ctl, st->ctr);
   EG_LAUNCHED();
   std::swap(st->visit, st->nvisit);
@@ -1589,7 +1711,7 @@ static int msd_run(eg_graph *g, const BatchView &bv, MsState *st, const uint64_t
     const uint32_t nb = std::min(blocks_for(g->n, kBlock), kStatBlocks);
     unsigned long long *part = st->ctr + kCtrPart;
     hipLaunchKernelGGL(k_ms_stats<NW>, dim3(nb), dim3(kBlock), 0, s, g->n, st->visit, bv.ptr, part,
-                       st->ctl);
+                       st->ctl, st->fbits);
     hipLaunchKernelGGL(k_ms_stats_sum, dim3(1), dim3(kBlock), 0, s, part, nb,
                        st->ctr + kCtrFront);
     EG_LAUNCHED();
