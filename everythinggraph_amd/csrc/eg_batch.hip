@@ -260,7 +260,31 @@ __device__ __forceinline__ void store_levels(uint32_t n, uint32_t r, const Lanes
   }
 }
 
+// Level store of a whole wave (every lane active; a lane without new lanes
+// passes zero): the wave walks the union of its new lanes, so that the stores
+// of one lane index go out together, 64 rows in one instruction, instead of
+// one scattered 2-byte store per (row, lane) pair.  The same values as
+// store_levels.
 template <int NW>
+__device__ __forceinline__ void store_levels_wave(uint32_t n, uint32_t r, const Lanes<NW> &nw,
+                                                  uint16_t *__restrict__ lvl,
+                                                  const uint16_t *tab) {
+  const Lanes<NW> u = wave_or(nw);
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    uint64_t b = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u.w[j] >> 32))
+                  << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u.w[j]);
+    while (b) {
+      const int k = __ffsll((unsigned long long)b) - 1;
+      if ((nw.w[j] >> k) & 1ull) lvl[(uint64_t)(64 * j + k) * n + r] = tab[64 * j + k];
+      b &= b - 1ull;
+    }
+  }
+}
+
+// LV: store the levels here (else the caller stores them for its wave)
+template <int NW, bool LV = true>
 __device__ __forceinline__ Lanes<NW> finish_ms_row(uint32_t n, uint32_t r, const Lanes<NW> &acc,
                                                    const Lanes<NW> &active,
                                                    uint64_t *__restrict__ seen,
@@ -272,7 +296,7 @@ __device__ __forceinline__ Lanes<NW> finish_ms_row(uint32_t n, uint32_t r, const
   st_lanes<NW>(nvisit, r, nw);
   if (nw.any()) {
     st_lanes<NW>(seen, r, s | nw);
-    store_levels<NW>(n, r, nw, lvl, tab);
+    if (LV) store_levels<NW>(n, r, nw, lvl, tab);
   }
   return nw;
 }
@@ -313,7 +337,7 @@ __global__ void k_ms_seed(uint32_t n, uint32_t k, const uint32_t *__restrict__ l
 
 // Rows whose new lanes nw are final: record them (only this thread writes
 // row r in this level).
-template <int NW>
+template <int NW, bool LV = true>
 __device__ __forceinline__ Lanes<NW> finish_ms_new(uint32_t n, uint32_t r, const Lanes<NW> &nw,
                                                    uint64_t *__restrict__ seen,
                                                    uint64_t *__restrict__ nvisit,
@@ -322,7 +346,7 @@ __device__ __forceinline__ Lanes<NW> finish_ms_new(uint32_t n, uint32_t r, const
   st_lanes<NW>(nvisit, r, nw);
   if (nw.any()) {
     st_lanes<NW>(seen, r, ld_lanes<NW>(seen, r) | nw);
-    store_levels<NW>(n, r, nw, lvl, tab);
+    if (LV) store_levels<NW>(n, r, nw, lvl, tab);
   }
   return nw;
 }
@@ -420,10 +444,14 @@ __device__ __forceinline__ void ms_pull_item(
   }
   __syncthreads();
   Lanes<NW> nwall = Lanes<NW>::zero();
-  for (uint32_t i = tid; i < nrows; i += kBlock) {
-    const uint32_t ra = s_ptr[i], rb = s_ptr[i + 1];
+  // every lane of a wave runs every step (the level store is the wave's)
+  for (uint32_t ib = tid - lane; ib < nrows; ib += kBlock) {
+    const uint32_t i = ib + lane;
+    const bool live = i < nrows;
+    const uint32_t ra = live ? s_ptr[i] : 0u, rb = live ? s_ptr[i + 1] : 0u;
     bool hit = false;  // a new frontier row with entries (listed rows: their own bit below)
-    const bool serial = rb - ra <= kSerialRow;
+    const bool serial = live && rb - ra <= kSerialRow;
+    Lanes<NW> nwr = Lanes<NW>::zero();
     if (serial) {
       const uint32_t r = r0 + i;
       const Lanes<NW> need = active & ~ld_lanes<NW>(seen, r);
@@ -438,20 +466,22 @@ __device__ __forceinline__ void ms_pull_item(
           if (acc.covers(need)) break;
         }
       }
-      const Lanes<NW> nw = finish_ms_new<NW>(n, r, acc & need, seen, nvisit, lvl, s_tab);
+      const Lanes<NW> nw = finish_ms_new<NW, false>(n, r, acc & need, seen, nvisit, lvl, s_tab);
       if (nw.any()) {
         fc += rb > ra;
         fd += rb - ra;
         hit = rb > ra;
       }
       nwall = nwall | nw;
-    } else {
+      nwr = nw;
+    } else if (live) {
       s_list[atomicAdd(&s_nlist, 1u)] = (uint16_t)i;
     }
     if (fbits) {
       const uint64_t m = __ballot(serial), b = __ballot(hit);
-      if (lane == 0) fbits_put(fbits, r0 + i, m, b);
+      if (lane == 0) fbits_put(fbits, r0 + ib, m, b);
     }
+    store_levels_wave<NW>(n, r0 + i, nwr, lvl, s_tab);
   }
   __syncthreads();
   const uint32_t nlist = s_nlist;
@@ -741,11 +771,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, const uint32_t 
        vb += gridDim.x * blockDim.x) {
     const uint32_t v = vb + __lane_id();
     bool hit = false;
+    Lanes<NW> nwr = Lanes<NW>::zero();
     if (v < n) {
       const Lanes<NW> p = ld_lanes<NW>(pend, v);
       if (p.any()) {
         st_lanes<NW>(pend, v, Lanes<NW>::zero());
-        const Lanes<NW> nw = finish_ms_row<NW>(n, v, p, active, seen, nvisit, lvl, s_tab);
+        const Lanes<NW> nw = finish_ms_row<NW, false>(n, v, p, active, seen, nvisit, lvl, s_tab);
+        nwr = nw;
         if (fs && nw.any()) {
           const uint32_t dv = ptr[v + 1] - ptr[v];
           fc += dv > 0;
@@ -761,6 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_apply(uint32_t n, const uint32_t 
       const uint64_t b = __ballot(hit);
       if (__lane_id() == 0) fbits[vb >> 6] = b;
     }
+    store_levels_wave<NW>(n, v, nwr, lvl, s_tab);
   }
   publish_progress<NW>(all, s_or, ctr);
   if (fs) publish_front_stats(fc, fd, s_cd, fs);
