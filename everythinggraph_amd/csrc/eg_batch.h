@@ -27,8 +27,9 @@ int eg_batch_view(eg_graph *g, BatchView *v);
 
 // Up to 64 nw BFS lanes over one traversal (nw = 1, 2 or 4 lane words per
 // vertex): lane i of vertex v is bit i % 64 of word v * nw + i / 64 of the
-// masks.  Levels are u16, lane-major (lvl[lane * n + v]), root = 1, 0 = not
-// reached.
+// masks.  Levels are u16, lane-major (lvl[lane * n + v]), root = 1; a level
+// holds only where the lane's seen bit is set (elsewhere it may be left from
+// an earlier traversal: nothing clears the store).
 struct MsState {
   uint32_t nw = 1;
   hipStream_t stream = nullptr;  // every step is enqueued here

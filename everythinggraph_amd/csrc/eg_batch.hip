@@ -305,18 +305,16 @@ __device__ __forceinline__ Lanes<NW> finish_ms_row(uint32_t n, uint32_t r, const
 // thread per element would exceed the 2^32 threads of a launch.
 template <int NW>
 __global__ void k_ms_reset(uint32_t n, uint64_t *__restrict__ seen, uint64_t *__restrict__ visit,
-                           uint64_t *__restrict__ nvisit, uint64_t *__restrict__ pend,
-                           uint16_t *__restrict__ lvl) {
+                           uint64_t *__restrict__ nvisit, uint64_t *__restrict__ pend) {
+  // the level store is not cleared: its readers take a level only where the
+  // lane's seen bit is set
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * 16 * NW;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * NW;
        i += stride) {
-    if (i < (uint64_t)n * NW) {
-      seen[i] = 0;
-      visit[i] = 0;
-      nvisit[i] = 0;
-      pend[i] = 0;
-    }
-    reinterpret_cast<uint64_t *>(lvl)[i] = 0;  // 64 NW u16 = 16 NW u64 per vertex
+    seen[i] = 0;
+    visit[i] = 0;
+    nvisit[i] = 0;
+    pend[i] = 0;
   }
 }
 
@@ -997,48 +995,46 @@ __global__ __launch_bounds__(kBlock) void k_msd_scan_c(uint32_t *__restrict__ a,
 }
 
 __global__ void k_ms_extract(uint32_t n, uint32_t k, uint32_t lane0,
-                             const uint16_t *__restrict__ lvl, const uint32_t *__restrict__ perm,
+                             const uint16_t *__restrict__ lvl, const uint64_t *__restrict__ seen,
+                             uint32_t nw, const uint32_t *__restrict__ perm,
                              uint32_t *__restrict__ out) {
-  // out[i * n + v] = level of vertex v (layout id) in lane lane0 + i
+  // out[i * n + v] = level of vertex v (layout id) in lane lane0 + i, 0 where
+  // the lane did not reach it
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * k;
        t += stride) {
     const uint32_t i = (uint32_t)(t / n), v = (uint32_t)(t % n);
     const uint32_t vv = perm ? perm[v] : v;
-    out[t] = lvl[(uint64_t)(lane0 + i) * n + vv];
+    const uint32_t L = lane0 + i;
+    const bool reached = (seen[(uint64_t)vv * nw + (L >> 6)] >> (L & 63u)) & 1ull;
+    out[t] = reached ? lvl[(uint64_t)L * n + vv] : 0u;
   }
 }
 
-// Retired lanes: drop their bits from every mask and zero their levels.  A
-// lane retires after a level in which it reached nothing, so its frontier
-// bits are already clear; nvisit and pend are rewritten by every level.
+// Retired lanes: drop their bits from every mask (their levels then no
+// longer count: a level holds only where the seen bit is set).  A lane
+// retires after a level in which it reached nothing, so its frontier bits are
+// already clear; nvisit and pend are rewritten by every level.
 template <int NW>
 __global__ void k_ms_clear_lanes(uint32_t n, Lanes<NW> lanes, uint64_t *__restrict__ seen,
-                                 uint64_t *__restrict__ visit, uint16_t *__restrict__ lvl) {
+                                 uint64_t *__restrict__ visit) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   st_lanes<NW>(seen, v, ld_lanes<NW>(seen, v) & ~lanes);
   st_lanes<NW>(visit, v, ld_lanes<NW>(visit, v) & ~lanes);
-#pragma unroll
-  for (int j = 0; j < NW; j++) {
-    uint64_t b = lanes.w[j];
-    while (b) {
-      const int i = 64 * j + __ffsll((unsigned long long)b) - 1;
-      lvl[(uint64_t)i * n + v] = 0;
-      b &= b - 1;
-    }
-  }
 }
 
 // Reached vertices of lane lanes[blockIdx.y]: per-workgroup partial counts.
-__global__ __launch_bounds__(kBlock) void k_ms_reached(uint32_t n, const uint16_t *__restrict__ lvl,
+__global__ __launch_bounds__(kBlock) void k_ms_reached(uint32_t n, const uint64_t *__restrict__ seen,
+                                                       uint32_t nw,
                                                        const uint32_t *__restrict__ lanes,
                                                        unsigned long long *__restrict__ part) {
   __shared__ unsigned long long s_c[kWaves];
-  const uint16_t *row = lvl + (uint64_t)lanes[blockIdx.y] * n;
+  const uint32_t L = lanes[blockIdx.y];
+  const uint64_t *col = seen + (L >> 6);
   unsigned long long c = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    c += row[v] != 0;
+    c += (col[(uint64_t)v * nw] >> (L & 63u)) & 1ull;
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
   if (__lane_id() == 0) s_c[threadIdx.x >> 6] = c;
@@ -1485,16 +1481,16 @@ static Lanes<NW> to_lanes(const uint64_t *w) {
 
 int eg_ms_reset(eg_graph *g, MsState *st) {
   const uint32_t n = g->n;
-  const uint64_t count = (uint64_t)n * 16 * st->nw;
+  const uint64_t count = (uint64_t)n * st->nw;
   if (st->nw == 4)
     hipLaunchKernelGGL(k_ms_reset<4>, dim3(stride_grid(count)), dim3(256), 0, st->stream, n,
-                       st->seen, st->visit, st->nvisit, st->pend, st->lvl);
+                       st->seen, st->visit, st->nvisit, st->pend);
   else if (st->nw == 2)
     hipLaunchKernelGGL(k_ms_reset<2>, dim3(stride_grid(count)), dim3(256), 0, st->stream, n,
-                       st->seen, st->visit, st->nvisit, st->pend, st->lvl);
+                       st->seen, st->visit, st->nvisit, st->pend);
   else
     hipLaunchKernelGGL(k_ms_reset<1>, dim3(stride_grid(count)), dim3(256), 0, st->stream, n,
-                       st->seen, st->visit, st->nvisit, st->pend, st->lvl);
+                       st->seen, st->visit, st->nvisit, st->pend);
   EG_LAUNCHED();
   st->qlen = 0;
   st->qdeg = 0;
@@ -1787,7 +1783,7 @@ int eg_ms_extract_lane(eg_graph *g, const BatchView &bv, MsState *st, uint32_t l
                        uint32_t *out) {
   if (lane >= 64 * st->nw) return eg_fail(EG_EINVAL, "lane %u out of range", lane);
   hipLaunchKernelGGL(k_ms_extract, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream, g->n, 1u,
-                     lane, st->lvl, bv.perm, out);
+                     lane, st->lvl, st->seen, st->nw, bv.perm, out);
   EG_LAUNCHED();
   return EG_OK;
 }
@@ -1798,13 +1794,13 @@ int eg_ms_clear_lanes(eg_graph *g, MsState *st, const uint64_t *lanes) {
   if (!any) return EG_OK;
   if (st->nw == 4)
     hipLaunchKernelGGL(k_ms_clear_lanes<4>, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
-                       g->n, to_lanes<4>(lanes), st->seen, st->visit, st->lvl);
+                       g->n, to_lanes<4>(lanes), st->seen, st->visit);
   else if (st->nw == 2)
     hipLaunchKernelGGL(k_ms_clear_lanes<2>, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
-                       g->n, to_lanes<2>(lanes), st->seen, st->visit, st->lvl);
+                       g->n, to_lanes<2>(lanes), st->seen, st->visit);
   else
     hipLaunchKernelGGL(k_ms_clear_lanes<1>, dim3(blocks_for(g->n, 256)), dim3(256), 0, st->stream,
-                       g->n, to_lanes<1>(lanes), st->seen, st->visit, st->lvl);
+                       g->n, to_lanes<1>(lanes), st->seen, st->visit);
   EG_LAUNCHED();
   return EG_OK;
 }
@@ -1819,7 +1815,8 @@ int eg_ms_reached(eg_graph *g, MsState *st, uint32_t k, const uint32_t *lanes,
   memcpy(h, lanes, k * 4);
   EG_HIP(hipMemcpyAsync(st->roots, h, k * 4, hipMemcpyHostToDevice, s));
   unsigned long long *part = st->ctr + kCtrPart, *sums = st->ctr + kCtrPart + 64 * 64 * st->nw;
-  hipLaunchKernelGGL(k_ms_reached, dim3(nb, k), dim3(kBlock), 0, s, g->n, st->lvl, st->roots, part);
+  hipLaunchKernelGGL(k_ms_reached, dim3(nb, k), dim3(kBlock), 0, s, g->n, st->seen, st->nw,
+                     st->roots, part);
   EG_LAUNCHED();
   hipLaunchKernelGGL(k_ms_reached_sum, dim3(k), dim3(64), 0, s, part, nb, sums);
   EG_LAUNCHED();
@@ -2174,7 +2171,7 @@ extern "C" int eg_bfs_batch(eg_graph *g, uint32_t k, const uint32_t *roots, uint
     for (uint32_t l0 = 0; dist && l0 < kk; l0 += 64) {
       const uint32_t kl = kk - l0 < 64 ? kk - l0 : 64;
       hipLaunchKernelGGL(k_ms_extract, dim3(stride_grid((uint64_t)n * kl)), dim3(256), 0, s, n,
-                         kl, l0, mst->lvl, bv.perm, b->ext);
+                         kl, l0, mst->lvl, mst->seen, mst->nw, bv.perm, b->ext);
       EG_LAUNCHED();
       EG_HIP(hipMemcpyAsync(dist + (uint64_t)(g0 + l0) * n, b->ext, (uint64_t)n * kl * 4,
                             hipMemcpyDeviceToHost, s));
